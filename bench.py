@@ -1,0 +1,237 @@
+"""Benchmark: particle-pushes/s of TEAPOT tracking with the 3D space-charge kick on one
+MI355X.
+
+Workload (default, `ring3d`): 4,000,000 macroparticles (4D waterbag, radii 6 mm / 1 mrad,
+z Gaussian sigma 0.3 m, dE sigma 1 MeV, protons at 1 GeV) tracked through a 24-cell ring
+of FODO cells with two sextupoles each (k1 = +-0.5 m^-2, k2 = +-2 m^-3), a
+SpaceChargeCalc3D(256, 256, 128) node every ~0.5 m.  One particle-push is one
+macroparticle advanced through one space-charge interval (the maps between two SC nodes
+plus the SC kick with its extrema, deposit, Poisson solve and kick).  One bench step is
+one ring cell (its maps and its SC nodes); `value` = N x SC intervals / wall time, with
+the bunch already resident in HBM.
+
+`fodo2p5d` runs the 2.5D workload instead (100k particles, 128x128x64 SpaceChargeCalc2p5Drb,
+FODO cells, 12 SC nodes per cell).
+
+The JSON line carries `roofline` (dominant kernel: algorithmic bytes per launch / its
+mean HIP-event duration over a profiled pass) and `cpu_baseline` (the CPU oracle on a
+bounded sample of the same workload, timed on this host).
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path[:0] = [os.path.join(ROOT, "py-orbit_amd")]
+
+HBM_PEAK_GBS = 8000.0
+
+
+def parse():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=24)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--workload", default="ring3d", choices=["ring3d", "fodo2p5d"])
+    ap.add_argument("--n", type=int, default=0, help="override particle count")
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
+    ap.add_argument("--cpu-threads", type=int, default=0)
+    ap.add_argument("--profile-steps", type=int, default=2)
+    return ap.parse_args()
+
+
+def build_workload(args):
+    import spacecharge
+    from orbit.space_charge.sc2p5d import setSC2p5DrbAccNodes
+    from orbit.space_charge.sc3d import setSC3DAccNodes
+    from workloads import (PIPE_RADIUS, fodo_lattice, gaussian_coords, make_bunch,
+                           ring_lattice, waterbag_coords)
+    if args.workload == "ring3d":
+        n = args.n or 4_000_000
+        grid = (256, 256, 128)
+        coords = waterbag_coords(n)
+        lat = ring_lattice(24)
+        calc = spacecharge.SpaceChargeCalc3D(*grid)
+        scn = setSC3DAccNodes(lat, 0.49, calc)
+        cell_len = 6  # QF S1 D1 QD S2 D2 (the ring's turn counter node follows the cells)
+    else:
+        n = args.n or 100_000
+        grid = (128, 128, 64)
+        coords = gaussian_coords(n)
+        lat = fodo_lattice(100)
+        calc = spacecharge.SpaceChargeCalc2p5Drb(*grid)
+        scn = setSC2p5DrbAccNodes(lat, 0.49, calc, PIPE_RADIUS)
+        cell_len = 4
+    b = make_bunch(coords)
+    # SC nodes per cell (children of the cell's first-level nodes)
+    nodes = lat.getNodes()
+    cells = []
+    for i in range(0, len(nodes) - (len(nodes) % cell_len), cell_len):
+        ids = {id(x) for x in scn}
+        nsc = sum(1 for nd in nodes[i:i + cell_len] for ch in nd.getBodyChildren() if id(ch) in ids)
+        cells.append((i, i + cell_len - 1, nsc))
+    return b, lat, calc, cells, n, grid, coords
+
+
+def run_cells(lat, b, cells, k0, count):
+    pushes_sc = 0
+    for j in range(count):
+        i0, i1, nsc = cells[(k0 + j) % len(cells)]
+        lat.trackBunch(b, index_start=i0, index_stop=i1)
+        pushes_sc += nsc
+    return pushes_sc
+
+
+def kernel_bytes(workload, n, grid):
+    """Algorithmic (compulsory) HBM bytes per launch of each kernel."""
+    nx, ny, nz = grid
+
+    def pad(v):
+        m = 1
+        while m < 2 * v - 1:
+            m <<= 1
+        return max(m, 2)
+    M0, M1, M2 = pad(nx), pad(ny), pad(nz)
+    b = {"push_extrema": 96 * n, "push": 96 * n}
+    if workload == "ring3d":
+        ng = nx * ny * nz
+        H2 = M2 // 2 + 1
+        qx, qy = M0 // 2 + 1, M1 // 2 + 1
+        b.update({
+            "deposit3d": 24 * n + 8 * ng,
+            "fill_zero": 8 * ng,
+            "fft3_z_r2c": 8 * ng + 16 * nx * ny * H2,
+            "fft3_strided_fwd": 16 * nx * ny * H2 + 16 * nx * M1 * H2,
+            "fft3_strided_conv": 32 * nx * M1 * H2 + 8 * qx * qy * H2,
+            "fft3_strided_inv": 16 * nx * M1 * H2 + 16 * nx * ny * H2,
+            "fft3_z_c2r": 16 * nx * ny * H2 + 8 * ng,
+        })
+    else:
+        H = M1 // 2 + 1
+        b.update({
+            "sc2d_deposit": 24 * n + 8 * nx * ny + 8 * nz,
+            "fft_rows_r2c": 8 * nx * ny + 16 * nx * H,
+            "fft_cols": 32 * nx * H + 16 * M0 * H,
+            "fft_rows_c2r": 16 * nx * H + 8 * nx * ny,
+        })
+    return b
+
+
+def profile(N, lat, b, cells, steps):
+    import ctypes as C
+    N.call("ob_profile_enable", 1)
+    run_cells(lat, b, cells, 0, steps)
+    N.synchronize()
+    buf = C.create_string_buffer(1 << 16)
+    L = N.load()
+    L.ob_profile_report.argtypes = [C.c_char_p, C.c_int]
+    L.ob_profile_report.restype = C.c_int
+    if L.ob_profile_report(buf, len(buf)) != 0:
+        raise RuntimeError(L.ob_last_error().decode())
+    N.call("ob_profile_enable", 0)
+    rows = {}
+    for line in buf.value.decode().strip().splitlines():
+        name, cnt, tot, mn, mx = line.split()
+        rows[name] = (int(cnt), float(tot), float(mn), float(mx))
+    return rows
+
+
+def cpu_baseline(args, n, grid, coords):
+    """The CPU oracle on a bounded sample: one SC interval of the same workload (the maps
+    of one interval plus one SC step), after one untimed step that builds the Green
+    function.  Threads: all host cores (OpenMP, particle loops and FFT lines)."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import numpy as np
+    import oracle as O
+    threads = args.cpu_threads or len(os.sched_getaffinity(0))
+    O.set_threads(threads)
+    if args.workload == "ring3d":
+        ns = n
+        ob = O.Bunch(coords[:, :ns].copy(), macrosize=1e13 / ns, kin_energy=1.0)
+        calc = O.SC3D(*grid)
+        ob.apply("drift", 0.5)
+        calc.trackBunch(ob, 0.5)
+        t0 = time.perf_counter()
+        ob.apply("drift", 0.5)
+        calc.trackBunch(ob, 0.5)
+        dt = time.perf_counter() - t0
+        sample = (f"{ns} particles, 1 SC interval (drift 0.5 m + SpaceChargeCalc3D"
+                  f"{grid} step incl. 2x{2*grid[0]}x{2*grid[1]}x{2*grid[2]} FFTs), "
+                  f"after one untimed step")
+    else:
+        ns = n
+        ob = O.Bunch(coords[:, :ns].copy(), macrosize=1e13 / ns, kin_energy=1.0)
+        calc = O.SC2p5Drb(*grid)
+        calc.trackBunch(ob, 0.5, 0.1)
+        t0 = time.perf_counter()
+        reps = 20
+        for _ in range(reps):
+            ob.apply("drift", 0.5)
+            calc.trackBunch(ob, 0.5, 0.1)
+        dt = (time.perf_counter() - t0) / reps
+        sample = f"{ns} particles, 20 SC intervals (drift 0.5 m + SpaceChargeCalc2p5Drb{grid} step)"
+    return {"value": ns / dt, "unit": "particle-pushes/s", "cores": threads, "kind": "port",
+            "sample": sample, "seconds": dt}
+
+
+def main():
+    args = parse()
+    if args.gpus != 1:
+        print("bench.py: this build measures one GPU; multi-GPU runs are not part of it "
+              "(the path shards by particles with one rho all-reduce per SC step, "
+              "see DESIGN.md)", file=sys.stderr)
+        sys.exit(2)
+    import _obnative as N
+    b, lat, calc, cells, n, grid, coords = build_workload(args)
+    N.synchronize()
+    run_cells(lat, b, cells, 0, args.warmup)
+    N.synchronize()
+    t0 = time.perf_counter()
+    pushes_sc = run_cells(lat, b, cells, args.warmup, args.steps)
+    N.synchronize()
+    dt = time.perf_counter() - t0
+    value = n * pushes_sc / dt
+
+    prof = profile(N, lat, b, cells, args.profile_steps)
+    kb = kernel_bytes(args.workload, n, grid)
+    dom = max((k for k in prof if k in kb), key=lambda k: prof[k][1])
+    cnt, tot, _, _ = prof[dom]
+    avg_s = tot / cnt / 1e3
+    achieved = kb[dom] / avg_s / 1e9
+    roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
+                "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                "kernel": dom, "avg_us": round(avg_s * 1e6, 2), "bytes_per_launch": kb[dom]}
+    kernels = {k: {"count": v[0], "avg_us": round(v[1] / v[0] * 1e3, 2),
+                   "share": round(v[1] / sum(x[1] for x in prof.values()), 4)}
+               for k, v in sorted(prof.items(), key=lambda kv: -kv[1][1])}
+    cpu = None
+    if not args.no_cpu:
+        cpu = cpu_baseline(args, n, grid, coords)
+    out = {
+        "metric": "particle-pushes/s incl. SC kick, 1 MI355X",
+        "value": round(value, 1),
+        "unit": "particle-pushes/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(dt / args.steps * 1e3, 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic",
+        "config": {"workload": args.workload, "particles": n, "grid": list(grid),
+                   "sc_intervals_timed": pushes_sc,
+                   "step": "one ring cell" if args.workload == "ring3d" else "one FODO cell",
+                   "parallelism": "1 GPU"},
+        "roofline": roofline,
+        "cpu_baseline": cpu,
+        "kernels": kernels,
+    }
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,137 @@
+/*
+ * orbit_mi355x.h -- C ABI of liborbit_mi355x.so, the MI355X (gfx950) implementation
+ * of py-orbit's TEAPOT macroparticle tracking and space-charge kick.
+ *
+ * Every entry point returns 0 on success and a non-zero status on failure; the
+ * message of the last failure on the calling thread is ob_last_error().  (The
+ * reference aborts the process instead: src/mpi/orbit_mpi.cc:37-75.)
+ *
+ * Data stays on the device.  Calls are asynchronous on the library's stream:
+ * host reads (ob_bunch_get_*) and ob_synchronize() wait for the work queued
+ * before them and report any error a device kernel flagged meanwhile.
+ * Handles are not thread-safe.
+ *
+ * Reference interfaces replaced (paths relative to the reference tree root):
+ *   bunch        src/orbit/wrap_bunch.cc:1178-1232, wrap_syncpart.cc:549-566
+ *   teapot_base  src/teapot/wrap_teapotbase.cc:488-512 (arg formats :29-473)
+ *   spacecharge  src/spacecharge/wrap_spacechargecalc2p5d_rb.cc:38-51,189-198,
+ *                wrap_spacechargecalc3d.cc:38-45,171-177
+ */
+#ifndef ORBIT_MI355X_H
+#define ORBIT_MI355X_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct ob_bunch ob_bunch;
+typedef struct ob_sc2p5drb ob_sc2p5drb;
+typedef struct ob_sc3d ob_sc3d;
+
+/* ---- library ---------------------------------------------------------- */
+/* Select the device and create the library stream.  Called implicitly by the
+ * first create call with device 0.  (src/main/main.cc:27-82 starts the runtime.) */
+int ob_init(int device);
+/* Message of the last failure on this thread ("" if none). */
+const char* ob_last_error(void);
+/* Wait for all queued work; returns non-zero if a kernel flagged an error
+ * (e.g. a zero-extent bunch in a space-charge calculator). */
+int ob_synchronize(void);
+/* Library version string. */
+const char* ob_version(void);
+
+/* ---- Bunch (src/orbit/Bunch.hh:54-293, wrap_bunch.cc:1178-1232) --------- */
+/* Bunch(): protons, macroSize 0, synchronous kinetic energy 0 (Bunch.cc:404-425). */
+int ob_bunch_create(ob_bunch** out);
+int ob_bunch_destroy(ob_bunch* b);
+/* mass(v) GeV, charge(v) e, classicalRadius, macroSize(v): wrap_bunch.cc "mass",
+ * "charge", "classicalRadius", "macroSize". */
+int ob_bunch_set_mass(ob_bunch* b, double mass);
+int ob_bunch_set_charge(ob_bunch* b, double charge);
+int ob_bunch_set_classical_radius(ob_bunch* b, double r0);
+int ob_bunch_set_macrosize(ob_bunch* b, double macrosize);
+/* out[4] = mass, charge, classicalRadius, macroSize */
+int ob_bunch_get_attrs(ob_bunch* b, double* out);
+/* SyncPart.kinEnergy(T) setter (wrap_syncpart.cc:70-92). */
+int ob_bunch_set_kin_energy(ob_bunch* b, double T_GeV);
+/* out[6] = kinEnergy, momentum, beta, gamma, time, mass (wrap_syncpart.cc:549-566) */
+int ob_bunch_get_sync(ob_bunch* b, double* out);
+int ob_bunch_set_sync_time(ob_bunch* b, double t);
+/* addParticle(x,xp,y,yp,z,dE) (wrap_bunch.cc "addParticle"). */
+int ob_bunch_add_particle(ob_bunch* b, double x, double xp, double y, double yp, double z,
+                          double dE);
+/* Bulk replacement of all particles from host SoA arrays (n each). */
+int ob_bunch_set_coords(ob_bunch* b, int64_t n, const double* x, const double* xp,
+                        const double* y, const double* yp, const double* z, const double* dE);
+/* Copy all coordinates to host SoA arrays (each of getSize() doubles). */
+int ob_bunch_get_coords(ob_bunch* b, double* x, double* xp, double* y, double* yp, double* z,
+                        double* dE);
+/* One coordinate: k = 0..5 for x, xp, y, yp, z, dE (wrap_bunch.cc "x".."dE"). */
+int ob_bunch_get_coord(ob_bunch* b, int64_t i, int k, double* v);
+int ob_bunch_set_coord(ob_bunch* b, int64_t i, int k, double v);
+/* getSize / getSizeGlobal (one process: identical, Bunch.cc:850-861). */
+int ob_bunch_get_size(ob_bunch* b, int64_t* n);
+/* deleteParticle(i) marks the particle dead; compress() removes dead particles
+ * keeping order (Bunch.cc:704-763). */
+int ob_bunch_delete_particle(ob_bunch* b, int64_t i);
+int ob_bunch_compress(ob_bunch* b);
+/* Per-particle "macrosize" attribute (ParticleMacroSize; read at Grid2D.cc:122-131).
+ * n must equal getSize(); NULL removes the attribute. */
+int ob_bunch_set_macrosizes(ob_bunch* b, int64_t n, const double* m);
+int ob_bunch_get_macrosizes(ob_bunch* b, int64_t n, double* m);
+/* Device pointers of the SoA coordinate arrays (x, xp, y, yp, z, dE), valid until the
+ * next size change; pending maps are applied first. */
+int ob_bunch_device_coords(ob_bunch* b, double** ptrs6);
+
+/* ---- teapot_base maps (src/teapot/teapotbase.cc; wrap_teapotbase.cc:488-512) ----
+ * Maps are queued on the bunch and applied by one fused kernel when the bunch is
+ * next needed (a space-charge kick, a host read, ob_bunch_flush); results are
+ * identical to applying them one by one.  useCharge is accepted and ignored as in
+ * the reference (teapotbase.cc:620-700, 336-372). */
+int ob_tpb_rotatexy(ob_bunch* b, double angle);                        /* :80   */
+int ob_tpb_drift(ob_bunch* b, double length);                          /* :158  */
+int ob_tpb_wrapbunch(ob_bunch* b, double length);                      /* :202  */
+int ob_tpb_kick(ob_bunch* b, double kx, double ky, double kE, int useCharge); /* :232 */
+int ob_tpb_multp(ob_bunch* b, int pole, double kl, int skew, int useCharge);  /* :336 */
+int ob_tpb_multpfringeIN(ob_bunch* b, int pole, double kl, int skew, int useCharge);  /* :395 */
+int ob_tpb_multpfringeOUT(ob_bunch* b, int pole, double kl, int skew, int useCharge); /* :508 */
+int ob_tpb_quad1(ob_bunch* b, double length, double kq, int useCharge); /* :620 */
+int ob_tpb_quad2(ob_bunch* b, double length);                           /* :719 */
+int ob_tpb_quadfringeIN(ob_bunch* b, double kq, int useCharge);         /* :784 */
+int ob_tpb_quadfringeOUT(ob_bunch* b, double kq, int useCharge);        /* :854 */
+/* Apply the queued maps now. */
+int ob_bunch_flush(ob_bunch* b);
+
+/* ---- SpaceChargeCalc2p5Drb (src/spacecharge/SpaceChargeCalc2p5Drb.cc) ---- */
+/* SpaceChargeCalc2p5Drb(nx, ny, nz[, xy_ratio]) (:31-63). */
+int ob_sc2p5drb_create(int nx, int ny, int nz, double xy_ratio, ob_sc2p5drb** out);
+int ob_sc2p5drb_destroy(ob_sc2p5drb* c);
+/* trackBunch(bunch, length, pipe_radius) (:110-168). */
+int ob_sc2p5drb_track(ob_sc2p5drb* c, ob_bunch* b, double length, double pipe_radius);
+/* getRhoGrid / getPhiGrid / getLongGrid / getLongDerivativeGrid (wrap :54-72 and
+ * following): which = 0 rho, 1 phi ([nx][ny] row-major), 2 z line density, 3 its
+ * derivative ([nz]).  lims receives xMin,xMax,yMin,yMax (which 0,1) or zMin,zMax. */
+int ob_sc2p5drb_get_grid(ob_sc2p5drb* c, int which, double* out, double* lims);
+/* out[4] = x_center, y_center, a_bunch, totalMacrosize of the last trackBunch. */
+int ob_sc2p5drb_get_moments(ob_sc2p5drb* c, double* out);
+/* set/getLongAveragingPointsN (:313-320). */
+int ob_sc2p5drb_set_long_avg_n(ob_sc2p5drb* c, int n);
+int ob_sc2p5drb_get_long_avg_n(ob_sc2p5drb* c, int* n);
+
+/* ---- SpaceChargeCalc3D (src/spacecharge/SpaceChargeCalc3D.cc) ---- */
+int ob_sc3d_create(int nx, int ny, int nz, ob_sc3d** out);             /* :32-58 */
+int ob_sc3d_destroy(ob_sc3d* c);
+int ob_sc3d_track(ob_sc3d* c, ob_bunch* b, double length);             /* :100-148 */
+/* which = 0 rho, 1 phi ([nz][nx][ny]); lims = xMin,xMax,yMin,yMax,zMin,zMax. */
+int ob_sc3d_get_grid(ob_sc3d* c, int which, double* out, double* lims);
+int ob_sc3d_set_ratio_limit(ob_sc3d* c, double r);                     /* :334-337 */
+int ob_sc3d_get_ratio_limit(ob_sc3d* c, double* r);
+/* Number of Green-function rebuilds so far (ratio changes, :186-203). */
+int ob_sc3d_green_rebuilds(ob_sc3d* c, int* n);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,352 @@
+"""ctypes front end for the CPU oracle (liborbit_oracle.so) and the reference
+Grid build (_ref/libref_grid.so).
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, __graft_entry__.smoke() and the
+cpu_baseline leg of bench.py, always as the checker, never as the thing run or
+measured.  The product (py-orbit_amd/) never imports this module.
+"""
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+_D = C.POINTER(C.c_double)
+_I = C.POINTER(C.c_int)
+
+
+def build(ref=True):
+    """Compile the oracle (and the reference Grid build when the reference
+    tree is present).  Called by __graft_entry__.build()."""
+    subprocess.run(["make", "-s", "-C", HERE], check=True)
+    if ref:
+        subprocess.run(["make", "-s", "-C", HERE, "ref"], check=True)
+
+
+def _dp(a):
+    return a.ctypes.data_as(_D)
+
+
+def _ip(a):
+    return a.ctypes.data_as(_I)
+
+
+class _Sync(C.Structure):
+    _fields_ = [(n, C.c_double) for n in
+                ("mass", "charge", "classical_radius", "macrosize", "T", "p", "beta", "gamma", "time")]
+
+
+class _Bunch(C.Structure):
+    _fields_ = [("n", C.c_int64)] + [(n, _D) for n in ("x", "xp", "y", "yp", "z", "dE")] + \
+               [("msize", _D), ("s", _Sync)]
+
+
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        path = os.path.join(HERE, "liborbit_oracle.so")
+        if not os.path.exists(path):
+            raise RuntimeError("oracle library not built: run `make -C oracle`")
+        L = C.CDLL(path)
+        vp, d, i = C.c_void_p, C.c_double, C.c_int
+        bp = C.POINTER(_Bunch)
+        sig = {
+            "orc_set_threads": (None, [i]),
+            "orc_sync_set_kin_energy": (None, [C.POINTER(_Sync), d]),
+            "orc_rotatexy": (None, [bp, d]), "orc_drift": (None, [bp, d]),
+            "orc_wrapbunch": (None, [bp, d]), "orc_kick": (None, [bp, d, d, d]),
+            "orc_multp": (None, [bp, i, d, i]), "orc_multpfringeIN": (None, [bp, i, d, i]),
+            "orc_multpfringeOUT": (None, [bp, i, d, i]), "orc_quad1": (None, [bp, d, d]),
+            "orc_quad2": (None, [bp, d]), "orc_quadfringeIN": (None, [bp, d]),
+            "orc_quadfringeOUT": (None, [bp, d]), "orc_extrema_xyz": (None, [bp, _D]),
+            "orc_fft_c2c": (None, [_D, _D, i, i]),
+            "orc_fft_r2c_2d": (None, [_D, i, i, _D]), "orc_fft_c2r_2d": (None, [_D, i, i, _D]),
+            "orc_fft_r2c_3d": (None, [_D, i, i, i, _D]), "orc_fft_c2r_3d": (None, [_D, i, i, i, _D]),
+            "orc_sc2p5drb_create": (vp, [i, i, i, d]), "orc_sc2p5drb_destroy": (None, [vp]),
+            "orc_sc2p5drb_set_long_avg_n": (None, [vp, i]),
+            "orc_sc2p5drb_track": (i, [vp, bp, d, d]),
+            "orc_sc2p5drb_get_grid": (i, [vp, i, _D, _D]),
+            "orc_sc2p5drb_get_moments": (None, [vp, _D]),
+            "orc_sc3d_create": (vp, [i, i, i]), "orc_sc3d_destroy": (None, [vp]),
+            "orc_sc3d_set_ratio_limit": (None, [vp, d]), "orc_sc3d_track": (i, [vp, bp, d]),
+            "orc_sc3d_get_grid": (C.c_int64, [vp, i, _D, _D]),
+            "orc_sc3d_green_rebuilds": (i, [vp]),
+            "orc_twiss_emittances": (None, [bp, _D]),
+        }
+        for name, (res, args) in sig.items():
+            f = getattr(L, name)
+            f.restype = res
+            f.argtypes = args
+        _lib = L
+    return _lib
+
+
+def set_threads(n):
+    lib().orc_set_threads(int(n))
+
+
+class Bunch:
+    """SoA FP64 bunch for the oracle.  coords is a (6, n) float64 array:
+    rows x, xp, y, yp, z, dE."""
+
+    def __init__(self, coords, mass=0.938272013, charge=1.0, classical_radius=1.534698e-18,
+                 macrosize=0.0, kin_energy=1.0, msize=None):
+        self.coords = np.ascontiguousarray(np.array(coords, dtype=np.float64))
+        assert self.coords.ndim == 2 and self.coords.shape[0] == 6
+        self.msize = None if msize is None else np.ascontiguousarray(msize, dtype=np.float64)
+        self._c = _Bunch()
+        self._c.n = self.coords.shape[1]
+        for k, name in enumerate(("x", "xp", "y", "yp", "z", "dE")):
+            setattr(self._c, name, self.coords[k].ctypes.data_as(_D))
+        self._c.msize = _dp(self.msize) if self.msize is not None else _D()
+        s = self._c.s
+        s.mass, s.charge, s.classical_radius, s.macrosize = mass, charge, classical_radius, macrosize
+        lib().orc_sync_set_kin_energy(C.byref(s), kin_energy)
+
+    @property
+    def ref(self):
+        return C.byref(self._c)
+
+    @property
+    def sync(self):
+        s = self._c.s
+        return dict(T=s.T, p=s.p, beta=s.beta, gamma=s.gamma, time=s.time, mass=s.mass)
+
+    def apply(self, op, *args):
+        """Apply a teapot_base map by its reference name (drift, quad1, ...)."""
+        L = lib()
+        table = {
+            "drift": L.orc_drift, "quad1": L.orc_quad1, "quad2": L.orc_quad2,
+            "quadfringeIN": L.orc_quadfringeIN, "quadfringeOUT": L.orc_quadfringeOUT,
+            "multp": L.orc_multp, "multpfringeIN": L.orc_multpfringeIN,
+            "multpfringeOUT": L.orc_multpfringeOUT, "wrapbunch": L.orc_wrapbunch,
+            "rotatexy": L.orc_rotatexy, "kick": L.orc_kick,
+        }
+        table[op](self.ref, *args)
+
+    def extrema(self):
+        out = np.zeros(6)
+        lib().orc_extrema_xyz(self.ref, _dp(out))
+        return out
+
+    def emittances(self):
+        out = np.zeros(3)
+        lib().orc_twiss_emittances(self.ref, _dp(out))
+        return out
+
+
+class SC2p5Drb:
+    def __init__(self, nx, ny, nz, xy_ratio=1.0):
+        self.nx, self.ny, self.nz = nx, ny, nz
+        self.h = lib().orc_sc2p5drb_create(nx, ny, nz, xy_ratio)
+
+    def __del__(self):
+        if getattr(self, "h", None) and _lib is not None:
+            _lib.orc_sc2p5drb_destroy(self.h)
+            self.h = None
+
+    def setLongAveragingPointsN(self, n):
+        lib().orc_sc2p5drb_set_long_avg_n(self.h, n)
+
+    def trackBunch(self, bunch, length, pipe_radius):
+        rc = lib().orc_sc2p5drb_track(self.h, bunch.ref, length, pipe_radius)
+        if rc != 0:
+            raise RuntimeError("oracle: bunch has zero extent")
+
+    def grid(self, which):
+        n = self.nx * self.ny if which in (0, 1) else self.nz
+        out = np.zeros(n)
+        lims = np.zeros(4)
+        lib().orc_sc2p5drb_get_grid(self.h, which, _dp(out), _dp(lims))
+        if which in (0, 1):
+            out = out.reshape(self.nx, self.ny)
+            return out, lims
+        return out, lims[:2]
+
+    def moments(self):
+        out = np.zeros(4)
+        lib().orc_sc2p5drb_get_moments(self.h, _dp(out))
+        return out
+
+
+class SC3D:
+    def __init__(self, nx, ny, nz):
+        self.nx, self.ny, self.nz = nx, ny, nz
+        self.h = lib().orc_sc3d_create(nx, ny, nz)
+
+    def __del__(self):
+        if getattr(self, "h", None) and _lib is not None:
+            _lib.orc_sc3d_destroy(self.h)
+            self.h = None
+
+    def setRatioLimit(self, r):
+        lib().orc_sc3d_set_ratio_limit(self.h, r)
+
+    def trackBunch(self, bunch, length):
+        rc = lib().orc_sc3d_track(self.h, bunch.ref, length)
+        if rc != 0:
+            raise RuntimeError("oracle: bunch has zero extent")
+
+    def grid(self, which):
+        out = np.zeros(self.nx * self.ny * self.nz)
+        lims = np.zeros(6)
+        lib().orc_sc3d_get_grid(self.h, which, _dp(out), _dp(lims))
+        return out.reshape(self.nz, self.nx, self.ny), lims
+
+    def green_rebuilds(self):
+        return lib().orc_sc3d_green_rebuilds(self.h)
+
+
+def long_derivative(zg, zmin, zmax, n_avg=3):
+    """calculateLongDerivative of SpaceChargeCalc2p5Drb on a given line density."""
+    L = lib()
+    f = L.orc_flat_long_derivative
+    f.restype = None
+    f.argtypes = [C.c_int, C.c_double, C.c_double, C.c_int, _D, _D]
+    zg = np.ascontiguousarray(zg, dtype=np.float64)
+    out = np.zeros(len(zg))
+    f(len(zg), zmin, zmax, n_avg, _dp(zg), _dp(out))
+    return out
+
+
+def fft_c2c(z, inverse=False):
+    re = np.ascontiguousarray(z.real, dtype=np.float64).copy()
+    im = np.ascontiguousarray(z.imag, dtype=np.float64).copy()
+    lib().orc_fft_c2c(_dp(re), _dp(im), len(re), int(inverse))
+    return re + 1j * im
+
+
+def fft_r2c_2d(a):
+    a = np.ascontiguousarray(a, dtype=np.float64)
+    n0, n1 = a.shape
+    out = np.zeros((n0, n1 // 2 + 1, 2))
+    lib().orc_fft_r2c_2d(_dp(a), n0, n1, _dp(out))
+    return out[..., 0] + 1j * out[..., 1]
+
+
+def fft_c2r_2d(s, n1):
+    n0 = s.shape[0]
+    buf = np.ascontiguousarray(np.stack([s.real, s.imag], axis=-1))
+    out = np.zeros((n0, n1))
+    lib().orc_fft_c2r_2d(_dp(buf), n0, n1, _dp(out))
+    return out
+
+
+def fft_r2c_3d(a):
+    a = np.ascontiguousarray(a, dtype=np.float64)
+    n0, n1, n2 = a.shape
+    out = np.zeros((n0, n1, n2 // 2 + 1, 2))
+    lib().orc_fft_r2c_3d(_dp(a), n0, n1, n2, _dp(out))
+    return out[..., 0] + 1j * out[..., 1]
+
+
+# --------------------------------------------------------------------------
+# Grid-level functions with identical signatures in the oracle (orc_flat_*)
+# and in the reference Grid build (ref_*, oracle/_ref, present only where the
+# reference tree was available at build time; tests that need it skip
+# otherwise).
+
+
+def ref_available():
+    return os.path.exists(os.path.join(HERE, "_ref", "libref_grid.so"))
+
+
+def _f(a):
+    return np.ascontiguousarray(a, dtype=np.float64)
+
+
+class GridAPI:
+    def __init__(self, L, prefix):
+        lg, i, d = C.c_long, C.c_int, C.c_double
+        sig = {
+            "grid2d_bin": [i, i, _D, lg, _D, _D, _D, _D],
+            "grid2d_indfrac": [i, i, _D, lg, _D, _D, _I, _D, _I, _D],
+            "grid2d_grad": [i, i, _D, _D, lg, _D, _D, _D, _D, _D],
+            "grid1d_bin": [i, d, d, lg, _D, _D, _D],
+            "grid1d_value": [i, d, d, _D, lg, _D, _D, _D],
+            "grid3d_bin": [i, i, i, _D, lg, _D, _D, _D, _D, _D],
+            "grid3d_indfrac": [i, i, i, _D, lg, _D, _D, _D, _I, _D],
+            "grid3d_grad": [i, i, i, _D, _D, lg, _D, _D, _D, _D, _D, _D],
+        }
+        self.f = {}
+        for name, args in sig.items():
+            fn = getattr(L, prefix + name)
+            fn.restype = None
+            fn.argtypes = args
+            self.f[name] = fn
+
+    def grid2d_bin(self, nx, ny, lims, x, y, m):
+        x, y, m, lims = _f(x), _f(y), _f(m), _f(lims)
+        out = np.zeros(nx * ny)
+        self.f["grid2d_bin"](nx, ny, _dp(lims), len(x), _dp(x), _dp(y), _dp(m), _dp(out))
+        return out.reshape(nx, ny)
+
+    def grid2d_indfrac(self, nx, ny, lims, x, y):
+        x, y, lims = _f(x), _f(y), _f(lims)
+        n = len(x)
+        ix, iy = np.zeros(n, np.int32), np.zeros(n, np.int32)
+        fx, fy = np.zeros(n), np.zeros(n)
+        self.f["grid2d_indfrac"](nx, ny, _dp(lims), n, _dp(x), _dp(y), _ip(ix), _dp(fx), _ip(iy), _dp(fy))
+        return ix, fx, iy, fy
+
+    def grid2d_grad(self, nx, ny, lims, arr, x, y):
+        x, y, lims, arr = _f(x), _f(y), _f(lims), _f(arr).ravel()
+        n = len(x)
+        ex, ey, val = np.zeros(n), np.zeros(n), np.zeros(n)
+        self.f["grid2d_grad"](nx, ny, _dp(lims), _dp(arr), n, _dp(x), _dp(y), _dp(ex), _dp(ey), _dp(val))
+        return ex, ey, val
+
+    def grid1d_bin(self, nz, zmin, zmax, z, m):
+        z, m = _f(z), _f(m)
+        out = np.zeros(nz)
+        self.f["grid1d_bin"](nz, zmin, zmax, len(z), _dp(z), _dp(m), _dp(out))
+        return out
+
+    def grid1d_value(self, nz, zmin, zmax, arr, z):
+        z, arr = _f(z), _f(arr)
+        val, grad = np.zeros(len(z)), np.zeros(len(z))
+        self.f["grid1d_value"](nz, zmin, zmax, _dp(arr), len(z), _dp(z), _dp(val), _dp(grad))
+        return val, grad
+
+    def grid3d_bin(self, nx, ny, nz, lims, x, y, z, m):
+        x, y, z, m, lims = _f(x), _f(y), _f(z), _f(m), _f(lims)
+        out = np.zeros(nx * ny * nz)
+        self.f["grid3d_bin"](nx, ny, nz, _dp(lims), len(x), _dp(x), _dp(y), _dp(z), _dp(m), _dp(out))
+        return out.reshape(nz, nx, ny)
+
+    def grid3d_indfrac(self, nx, ny, nz, lims, x, y, z):
+        x, y, z, lims = _f(x), _f(y), _f(z), _f(lims)
+        n = len(x)
+        ind = np.zeros(3 * n, np.int32)
+        frac = np.zeros(3 * n)
+        self.f["grid3d_indfrac"](nx, ny, nz, _dp(lims), n, _dp(x), _dp(y), _dp(z), _ip(ind), _dp(frac))
+        return ind.reshape(n, 3), frac.reshape(n, 3)
+
+    def grid3d_grad(self, nx, ny, nz, lims, arr, x, y, z):
+        x, y, z, lims, arr = _f(x), _f(y), _f(z), _f(lims), _f(arr).ravel()
+        n = len(x)
+        gx, gy, gz = np.zeros(n), np.zeros(n), np.zeros(n)
+        self.f["grid3d_grad"](nx, ny, nz, _dp(lims), _dp(arr), n, _dp(x), _dp(y), _dp(z),
+                              _dp(gx), _dp(gy), _dp(gz))
+        return gx, gy, gz
+
+
+_apis = {}
+
+
+def orc_grid():
+    """Grid functions of the oracle restatement."""
+    if "orc" not in _apis:
+        _apis["orc"] = GridAPI(lib(), "orc_flat_")
+    return _apis["orc"]
+
+
+def ref_grid():
+    """Grid functions of the reference's own Grid1D/2D/3D (oracle/_ref)."""
+    if "ref" not in _apis:
+        _apis["ref"] = GridAPI(C.CDLL(os.path.join(HERE, "_ref", "libref_grid.so")), "ref_")
+    return _apis["ref"]

@@ -1,0 +1,112 @@
+"""Loader for liborbit_mi355x.so, the C ABI declared in include/orbit_mi355x.h.
+
+The library is the only compute path: if it is missing or no device is visible, every
+call raises.  There is no CPU fallback.
+"""
+import ctypes as C
+import os
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "liborbit_mi355x.so")
+
+_D = C.POINTER(C.c_double)
+_vp = C.c_void_p
+_i, _d, _i64 = C.c_int, C.c_double, C.c_int64
+
+# name -> argtypes (all functions return int status unless listed in _RET)
+SIGNATURES = {
+    "ob_init": [_i],
+    "ob_synchronize": [],
+    "ob_bunch_create": [C.POINTER(_vp)],
+    "ob_bunch_destroy": [_vp],
+    "ob_bunch_set_mass": [_vp, _d],
+    "ob_bunch_set_charge": [_vp, _d],
+    "ob_bunch_set_classical_radius": [_vp, _d],
+    "ob_bunch_set_macrosize": [_vp, _d],
+    "ob_bunch_get_attrs": [_vp, _D],
+    "ob_bunch_set_kin_energy": [_vp, _d],
+    "ob_bunch_get_sync": [_vp, _D],
+    "ob_bunch_set_sync_time": [_vp, _d],
+    "ob_bunch_add_particle": [_vp, _d, _d, _d, _d, _d, _d],
+    "ob_bunch_set_coords": [_vp, _i64, _D, _D, _D, _D, _D, _D],
+    "ob_bunch_get_coords": [_vp, _D, _D, _D, _D, _D, _D],
+    "ob_bunch_get_coord": [_vp, _i64, _i, _D],
+    "ob_bunch_set_coord": [_vp, _i64, _i, _d],
+    "ob_bunch_get_size": [_vp, C.POINTER(_i64)],
+    "ob_bunch_delete_particle": [_vp, _i64],
+    "ob_bunch_compress": [_vp],
+    "ob_bunch_set_macrosizes": [_vp, _i64, _D],
+    "ob_bunch_get_macrosizes": [_vp, _i64, _D],
+    "ob_bunch_device_coords": [_vp, C.POINTER(_vp)],
+    "ob_bunch_flush": [_vp],
+    "ob_tpb_rotatexy": [_vp, _d],
+    "ob_tpb_drift": [_vp, _d],
+    "ob_tpb_wrapbunch": [_vp, _d],
+    "ob_tpb_kick": [_vp, _d, _d, _d, _i],
+    "ob_tpb_multp": [_vp, _i, _d, _i, _i],
+    "ob_tpb_multpfringeIN": [_vp, _i, _d, _i, _i],
+    "ob_tpb_multpfringeOUT": [_vp, _i, _d, _i, _i],
+    "ob_tpb_quad1": [_vp, _d, _d, _i],
+    "ob_tpb_quad2": [_vp, _d],
+    "ob_tpb_quadfringeIN": [_vp, _d, _i],
+    "ob_tpb_quadfringeOUT": [_vp, _d, _i],
+    "ob_sc2p5drb_create": [_i, _i, _i, _d, C.POINTER(_vp)],
+    "ob_sc2p5drb_destroy": [_vp],
+    "ob_sc2p5drb_track": [_vp, _vp, _d, _d],
+    "ob_sc2p5drb_get_grid": [_vp, _i, _D, _D],
+    "ob_sc2p5drb_get_moments": [_vp, _D],
+    "ob_sc2p5drb_set_long_avg_n": [_vp, _i],
+    "ob_sc2p5drb_get_long_avg_n": [_vp, C.POINTER(_i)],
+    "ob_sc3d_create": [_i, _i, _i, C.POINTER(_vp)],
+    "ob_sc3d_destroy": [_vp],
+    "ob_sc3d_track": [_vp, _vp, _d],
+    "ob_sc3d_get_grid": [_vp, _i, _D, _D],
+    "ob_sc3d_set_ratio_limit": [_vp, _d],
+    "ob_sc3d_get_ratio_limit": [_vp, _D],
+    "ob_sc3d_green_rebuilds": [_vp, C.POINTER(_i)],
+}
+_RET = {"ob_last_error": C.c_char_p, "ob_version": C.c_char_p}
+
+_lib = None
+
+
+class OrbitError(RuntimeError):
+    """A failure reported by liborbit_mi355x (the reference would abort the process)."""
+
+
+def load():
+    """Load the native library (once)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise OrbitError(f"{LIB_PATH} is not built: run python py-orbit_amd/build_lib.py")
+    L = C.CDLL(LIB_PATH)
+    for name, args in SIGNATURES.items():
+        f = getattr(L, name)
+        f.argtypes = args
+        f.restype = C.c_int
+    for name, res in _RET.items():
+        f = getattr(L, name)
+        f.argtypes = []
+        f.restype = res
+    _lib = L
+    return L
+
+
+def call(name, *args):
+    """Call an entry point; raise OrbitError with the library's message on failure."""
+    L = load()
+    rc = getattr(L, name)(*args)
+    if rc != 0:
+        msg = L.ob_last_error().decode(errors="replace")
+        raise OrbitError(f"{name}: {msg}")
+    return rc
+
+
+def dptr(a):
+    return a.ctypes.data_as(_D)
+
+
+def synchronize():
+    call("ob_synchronize")

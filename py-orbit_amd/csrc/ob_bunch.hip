@@ -1,0 +1,739 @@
+// ob_bunch.hip -- library runtime, device-resident SoA bunch and the teapot_base entry
+// points.
+//
+// The bunch keeps six FP64 coordinate arrays (x, xp, y, yp, z, dE) in HBM and a host
+// mirror used only by per-particle accessors and bulk copies.  teapot_base calls do not
+// launch anything: they append an Op (with host-precomputed constants) to the bunch's
+// queue, and the queue is applied by one fused kernel when the bunch is next needed.
+#include <cmath>
+#include <cstring>
+#include <mutex>
+
+#include "ob_internal.hpp"
+
+namespace ob {
+
+// ---------------------------------------------------------------------------
+// errors and runtime
+static thread_local std::string t_last_error;
+void set_error(const std::string& msg) { t_last_error = msg; }
+
+namespace {
+int g_device = -1;
+hipStream_t g_stream = nullptr;
+int g_cus = 0;
+
+struct Watched {
+  int* dev;
+  int* host;  // pinned
+  hipEvent_t ev;
+  std::string what;
+};
+std::vector<Watched> g_watch;
+}  // namespace
+
+hipStream_t stream() { return g_stream; }
+int device_cu_count() { return g_cus; }
+
+static void init_device(int device) {
+  if (g_stream != nullptr) {
+    OB_REQUIRE(device == g_device, "ob_init: library already bound to another device");
+    return;
+  }
+  int n = 0;
+  OB_HIP(hipGetDeviceCount(&n));
+  OB_REQUIRE(n > 0, "ob_init: no HIP device visible");
+  OB_REQUIRE(device >= 0 && device < n, "ob_init: device index out of range");
+  OB_HIP(hipSetDevice(device));
+  OB_HIP(hipStreamCreateWithFlags(&g_stream, hipStreamNonBlocking));
+  hipDeviceProp_t prop;
+  OB_HIP(hipGetDeviceProperties(&prop, device));
+  g_cus = prop.multiProcessorCount;
+  g_device = device;
+}
+
+void ensure_init() {
+  if (g_stream == nullptr) init_device(0);
+}
+
+// ---------------------------------------------------------------------------
+// kernel profiler
+namespace {
+struct ProfRec {
+  const char* name;
+  hipEvent_t a, b;
+};
+bool g_prof = false;
+std::vector<ProfRec> g_prof_recs;
+std::vector<hipEvent_t> g_prof_pool;
+hipEvent_t prof_event() {
+  if (!g_prof_pool.empty()) {
+    hipEvent_t e = g_prof_pool.back();
+    g_prof_pool.pop_back();
+    return e;
+  }
+  hipEvent_t e;
+  OB_HIP(hipEventCreate(&e));
+  return e;
+}
+}  // namespace
+
+bool profiling() { return g_prof; }
+
+void prof_mark(const char* name, bool begin) {
+  if (begin) {
+    ProfRec r{name, prof_event(), nullptr};
+    OB_HIP(hipEventRecord(r.a, g_stream));
+    g_prof_recs.push_back(r);
+  } else {
+    for (size_t i = g_prof_recs.size(); i-- > 0;)
+      if (g_prof_recs[i].b == nullptr && g_prof_recs[i].name == name) {
+        g_prof_recs[i].b = prof_event();
+        OB_HIP(hipEventRecord(g_prof_recs[i].b, g_stream));
+        return;
+      }
+  }
+}
+
+void watch_status(int* dev_status, const char* what) {
+  Watched w;
+  w.dev = dev_status;
+  OB_HIP(hipHostMalloc((void**)&w.host, sizeof(int), hipHostMallocDefault));
+  *w.host = 0;
+  OB_HIP(hipEventCreateWithFlags(&w.ev, hipEventDisableTiming));
+  OB_HIP(hipMemcpyAsync(w.host, dev_status, sizeof(int), hipMemcpyDeviceToHost, g_stream));
+  OB_HIP(hipEventRecord(w.ev, g_stream));
+  w.what = what;
+  g_watch.push_back(w);
+}
+
+void check_statuses(bool wait) {
+  std::string err;
+  size_t keep = 0;
+  for (size_t i = 0; i < g_watch.size(); i++) {
+    Watched& w = g_watch[i];
+    bool done;
+    if (wait) {
+      OB_HIP(hipEventSynchronize(w.ev));
+      done = true;
+    } else {
+      hipError_t q = hipEventQuery(w.ev);
+      OB_REQUIRE(q == hipSuccess || q == hipErrorNotReady,
+                 std::string("status query failed: ") + hipGetErrorString(q));
+      done = (q == hipSuccess);
+    }
+    if (done) {
+      if (*w.host != 0 && err.empty()) err = w.what;
+      (void)hipEventDestroy(w.ev);
+      (void)hipHostFree(w.host);
+    } else {
+      g_watch[keep++] = w;
+    }
+  }
+  g_watch.resize(keep);
+  OB_REQUIRE(err.empty(), err);
+}
+
+}  // namespace ob
+
+// ---------------------------------------------------------------------------
+// Bunch
+struct ob_bunch {
+  // attributes (Bunch.cc:404-425) and synchronous particle (SyncPart.cc)
+  double mass = 0.938272013, charge = 1.0, r0 = 1.534698e-18, macrosize = 0.0;
+  double T = 0.0, p = 0.0, beta = 0.0, gamma = 1.0, time = 0.0;
+
+  int64_t n = 0, cap = 0;
+  double* d = nullptr;        // 6 * cap doubles (SoA)
+  double* d_msize = nullptr;  // per-particle macrosize (cap) or null
+  std::vector<double> h[6];   // host mirror
+  std::vector<double> h_msize;
+  std::vector<uint8_t> alive; // host-side flags for deleteParticle/compress
+  bool has_msize = false;
+  bool host_valid = true, dev_valid = true;
+
+  std::vector<ob::Op> queue;
+};
+
+namespace ob {
+namespace {
+
+void sync_set_momentum(ob_bunch* b, double pz) {  // SyncPart.cc:133-138, 236-317
+  const double m = b->mass;
+  const double p2 = 0.0 * 0.0 + 0.0 * 0.0 + pz * pz;
+  const double p_abs = sqrt(p2);
+  const double m2 = m * m;
+  const double w2 = m2 + p2;
+  const double w = sqrt(w2);
+  b->T = p2 / (sqrt(m2 + p2) + m);
+  b->p = p_abs;
+  b->beta = p_abs / w;
+  b->gamma = w / m;
+}
+
+void dev_reserve(ob_bunch* b, int64_t want) {
+  if (want <= b->cap) return;
+  int64_t cap = b->cap ? b->cap : 1024;
+  while (cap < want) cap *= 2;
+  double* nd = nullptr;
+  OB_HIP(hipMalloc((void**)&nd, sizeof(double) * 6 * cap));
+  if (b->d) OB_HIP(hipFree(b->d));
+  b->d = nd;
+  if (b->d_msize) {
+    OB_HIP(hipFree(b->d_msize));
+    b->d_msize = nullptr;
+  }
+  b->cap = cap;
+  b->dev_valid = false;
+}
+
+void upload(ob_bunch* b) {
+  if (b->dev_valid) return;
+  dev_reserve(b, b->n);
+  hipStream_t s = stream();
+  for (int k = 0; k < 6; k++)
+    if (b->n)
+      OB_HIP(hipMemcpyAsync(b->d + (size_t)k * b->cap, b->h[k].data(), sizeof(double) * b->n,
+                            hipMemcpyHostToDevice, s));
+  if (b->has_msize) {
+    if (!b->d_msize) OB_HIP(hipMalloc((void**)&b->d_msize, sizeof(double) * b->cap));
+    if (b->n)
+      OB_HIP(hipMemcpyAsync(b->d_msize, b->h_msize.data(), sizeof(double) * b->n,
+                            hipMemcpyHostToDevice, s));
+  }
+  OB_HIP(hipStreamSynchronize(s));  // host vectors may change right after
+  b->dev_valid = true;
+}
+
+void download(ob_bunch* b) {
+  if (b->host_valid) return;
+  bunch_flush(b, nullptr);
+  hipStream_t s = stream();
+  for (int k = 0; k < 6; k++) {
+    b->h[k].resize(b->n);
+    if (b->n)
+      OB_HIP(hipMemcpyAsync(b->h[k].data(), b->d + (size_t)k * b->cap, sizeof(double) * b->n,
+                            hipMemcpyDeviceToHost, s));
+  }
+  OB_HIP(hipStreamSynchronize(s));
+  check_statuses(true);
+  b->host_valid = true;
+}
+
+// Host is about to modify the mirror: make it current and mark the device stale.
+void host_write(ob_bunch* b) {
+  download(b);
+  b->dev_valid = false;
+}
+
+}  // namespace
+
+DeviceBunchView bunch_device_view(ob_bunch* b) {
+  upload(b);
+  DeviceBunchView v;
+  for (int k = 0; k < 6; k++) v.c[k] = b->d + (size_t)k * b->cap;
+  v.msize = b->has_msize ? b->d_msize : nullptr;
+  v.n = b->n;
+  return v;
+}
+
+int64_t bunch_size(ob_bunch* b) { return b->n; }
+
+SyncInfo bunch_sync(ob_bunch* b) {
+  return SyncInfo{b->mass, b->charge, b->r0, b->macrosize, b->T, b->p, b->beta, b->gamma, b->time};
+}
+
+int bunch_flush(ob_bunch* b, double* ext_partials) {
+  if (b->queue.empty() && ext_partials == nullptr) return 0;
+  DeviceBunchView v = bunch_device_view(b);
+  size_t pos = 0;
+  int blocks = 0;
+  do {
+    OpProgram prog;
+    std::memset(&prog, 0, sizeof(prog));
+    size_t take = std::min(b->queue.size() - pos, (size_t)kMaxOpsPerLaunch);
+    prog.nops = (int32_t)take;
+    for (size_t k = 0; k < take; k++) prog.ops[k] = b->queue[pos + k];
+    pos += take;
+    const bool last = pos >= b->queue.size();
+    prog.want_extrema = (last && ext_partials) ? 1 : 0;
+    blocks = launch_push(v, prog, prog.want_extrema ? ext_partials : nullptr, stream());
+  } while (pos < b->queue.size());
+  b->queue.clear();
+  b->host_valid = false;
+  return blocks;
+}
+
+void bunch_queue(ob_bunch* b, const Op& op) {
+  if (b->n == 0) return;  // maps over an empty bunch are no-ops
+  b->queue.push_back(op);
+  b->host_valid = false;
+  if (b->queue.size() >= 256) bunch_flush(b, nullptr);
+}
+
+}  // namespace ob
+
+// ---------------------------------------------------------------------------
+// C ABI
+#define OB_API extern "C" __attribute__((visibility("default")))
+#define OB_TRY try {
+#define OB_CATCH                                   \
+  }                                                \
+  catch (const ob::Error& e) {                     \
+    ob::set_error(e.msg);                          \
+    return 1;                                      \
+  }                                                \
+  catch (const std::exception& e) {                \
+    ob::set_error(std::string("std: ") + e.what()); \
+    return 1;                                      \
+  }                                                \
+  return 0;
+#define OB_CHECK_B(b) OB_REQUIRE((b) != nullptr, "null bunch handle")
+
+OB_API const char* ob_last_error(void) { return ob::t_last_error.c_str(); }
+OB_API const char* ob_version(void) { return "orbit_mi355x 0.1 (gfx950, FP64)"; }
+
+OB_API int ob_init(int device) {
+  OB_TRY ob::init_device(device);
+  OB_CATCH
+}
+
+OB_API int ob_synchronize(void) {
+  OB_TRY ob::ensure_init();
+  OB_HIP(hipStreamSynchronize(ob::stream()));
+  ob::check_statuses(true);
+  OB_CATCH
+}
+
+OB_API int ob_profile_enable(int on) {
+  OB_TRY ob::ensure_init();
+  OB_HIP(hipStreamSynchronize(ob::stream()));
+  for (auto& r : ob::g_prof_recs) {
+    ob::g_prof_pool.push_back(r.a);
+    if (r.b) ob::g_prof_pool.push_back(r.b);
+  }
+  ob::g_prof_recs.clear();
+  ob::g_prof = on != 0;
+  OB_CATCH
+}
+
+OB_API int ob_profile_report(char* buf, int buflen) {
+  OB_TRY ob::ensure_init();
+  OB_HIP(hipStreamSynchronize(ob::stream()));
+  std::vector<std::string> names;
+  std::vector<double> tot, mn, mx;
+  std::vector<long> cnt;
+  for (auto& r : ob::g_prof_recs) {
+    if (!r.b) continue;
+    float ms = 0.f;
+    OB_HIP(hipEventElapsedTime(&ms, r.a, r.b));
+    size_t k = 0;
+    while (k < names.size() && names[k] != r.name) k++;
+    if (k == names.size()) {
+      names.push_back(r.name);
+      tot.push_back(0.0);
+      cnt.push_back(0);
+      mn.push_back(1e300);
+      mx.push_back(0.0);
+    }
+    tot[k] += ms;
+    cnt[k]++;
+    mn[k] = std::min(mn[k], (double)ms);
+    mx[k] = std::max(mx[k], (double)ms);
+  }
+  std::string s;
+  char line[256];
+  for (size_t k = 0; k < names.size(); k++) {
+    snprintf(line, sizeof(line), "%s %ld %.6f %.6f %.6f\n", names[k].c_str(), cnt[k], tot[k],
+             mn[k], mx[k]);
+    s += line;
+  }
+  OB_REQUIRE((int)s.size() < buflen, "profile report buffer too small");
+  std::memcpy(buf, s.c_str(), s.size() + 1);
+  OB_CATCH
+}
+
+OB_API int ob_bunch_create(ob_bunch** out) {
+  OB_TRY OB_REQUIRE(out != nullptr, "null output pointer");
+  ob::ensure_init();
+  *out = new ob_bunch();
+  OB_CATCH
+}
+
+OB_API int ob_bunch_destroy(ob_bunch* b) {
+  OB_TRY if (!b) return 0;
+  if (b->d) (void)hipFree(b->d);
+  if (b->d_msize) (void)hipFree(b->d_msize);
+  delete b;
+  OB_CATCH
+}
+
+OB_API int ob_bunch_set_mass(ob_bunch* b, double v) {
+  OB_TRY OB_CHECK_B(b);
+  b->mass = v;
+  OB_CATCH
+}
+OB_API int ob_bunch_set_charge(ob_bunch* b, double v) {
+  OB_TRY OB_CHECK_B(b);
+  b->charge = v;
+  OB_CATCH
+}
+OB_API int ob_bunch_set_classical_radius(ob_bunch* b, double v) {
+  OB_TRY OB_CHECK_B(b);
+  b->r0 = v;
+  OB_CATCH
+}
+OB_API int ob_bunch_set_macrosize(ob_bunch* b, double v) {
+  OB_TRY OB_CHECK_B(b);
+  b->macrosize = v;
+  OB_CATCH
+}
+OB_API int ob_bunch_get_attrs(ob_bunch* b, double* out) {
+  OB_TRY OB_CHECK_B(b);
+  out[0] = b->mass;
+  out[1] = b->charge;
+  out[2] = b->r0;
+  out[3] = b->macrosize;
+  OB_CATCH
+}
+OB_API int ob_bunch_set_kin_energy(ob_bunch* b, double T) {  // wrap_syncpart.cc:70-92
+  OB_TRY OB_CHECK_B(b);
+  const double m = b->mass;
+  ob::sync_set_momentum(b, sqrt(T * (T + 2.0 * m)));
+  OB_CATCH
+}
+OB_API int ob_bunch_get_sync(ob_bunch* b, double* out) {
+  OB_TRY OB_CHECK_B(b);
+  out[0] = b->T;
+  out[1] = b->p;
+  out[2] = b->beta;
+  out[3] = b->gamma;
+  out[4] = b->time;
+  out[5] = b->mass;
+  OB_CATCH
+}
+OB_API int ob_bunch_set_sync_time(ob_bunch* b, double t) {
+  OB_TRY OB_CHECK_B(b);
+  b->time = t;
+  OB_CATCH
+}
+
+OB_API int ob_bunch_add_particle(ob_bunch* b, double x, double xp, double y, double yp, double z,
+                                 double dE) {
+  OB_TRY OB_CHECK_B(b);
+  ob::host_write(b);
+  const double v[6] = {x, xp, y, yp, z, dE};
+  for (int k = 0; k < 6; k++) b->h[k].push_back(v[k]);
+  b->alive.push_back(1);
+  if (b->has_msize) b->h_msize.push_back(0.0);
+  b->n++;
+  OB_CATCH
+}
+
+OB_API int ob_bunch_set_coords(ob_bunch* b, int64_t n, const double* x, const double* xp,
+                               const double* y, const double* yp, const double* z,
+                               const double* dE) {
+  OB_TRY OB_CHECK_B(b);
+  OB_REQUIRE(n >= 0, "negative particle count");
+  b->queue.clear();
+  const double* src[6] = {x, xp, y, yp, z, dE};
+  for (int k = 0; k < 6; k++) OB_REQUIRE(n == 0 || src[k] != nullptr, "null coordinate array");
+  ob::dev_reserve(b, n);
+  b->n = n;
+  hipStream_t s = ob::stream();
+  for (int k = 0; k < 6; k++)
+    if (n)
+      OB_HIP(hipMemcpyAsync(b->d + (size_t)k * b->cap, src[k], sizeof(double) * n,
+                            hipMemcpyHostToDevice, s));
+  OB_HIP(hipStreamSynchronize(s));
+  for (int k = 0; k < 6; k++) b->h[k].clear();
+  b->alive.assign(n, 1);
+  b->has_msize = false;
+  b->h_msize.clear();
+  b->dev_valid = true;
+  b->host_valid = false;
+  OB_CATCH
+}
+
+OB_API int ob_bunch_get_coords(ob_bunch* b, double* x, double* xp, double* y, double* yp,
+                               double* z, double* dE) {
+  OB_TRY OB_CHECK_B(b);
+  double* dst[6] = {x, xp, y, yp, z, dE};
+  if (b->host_valid || !b->dev_valid) {
+    ob::download(b);
+    for (int k = 0; k < 6; k++)
+      if (dst[k] && b->n) std::memcpy(dst[k], b->h[k].data(), sizeof(double) * b->n);
+    return 0;
+  }
+  ob::bunch_flush(b, nullptr);
+  hipStream_t s = ob::stream();
+  for (int k = 0; k < 6; k++)
+    if (dst[k] && b->n)
+      OB_HIP(hipMemcpyAsync(dst[k], b->d + (size_t)k * b->cap, sizeof(double) * b->n,
+                            hipMemcpyDeviceToHost, s));
+  OB_HIP(hipStreamSynchronize(s));
+  ob::check_statuses(true);
+  OB_CATCH
+}
+
+OB_API int ob_bunch_get_coord(ob_bunch* b, int64_t i, int k, double* v) {
+  OB_TRY OB_CHECK_B(b);
+  OB_REQUIRE(i >= 0 && i < b->n, "particle index out of range");
+  OB_REQUIRE(k >= 0 && k < 6, "coordinate index out of range");
+  ob::download(b);
+  *v = b->h[k][i];
+  OB_CATCH
+}
+
+OB_API int ob_bunch_set_coord(ob_bunch* b, int64_t i, int k, double v) {
+  OB_TRY OB_CHECK_B(b);
+  OB_REQUIRE(i >= 0 && i < b->n, "particle index out of range");
+  OB_REQUIRE(k >= 0 && k < 6, "coordinate index out of range");
+  ob::host_write(b);
+  b->h[k][i] = v;
+  OB_CATCH
+}
+
+OB_API int ob_bunch_get_size(ob_bunch* b, int64_t* n) {
+  OB_TRY OB_CHECK_B(b);
+  *n = b->n;
+  OB_CATCH
+}
+
+OB_API int ob_bunch_delete_particle(ob_bunch* b, int64_t i) {
+  OB_TRY OB_CHECK_B(b);
+  OB_REQUIRE(i >= 0 && i < b->n, "particle index out of range");
+  if ((int64_t)b->alive.size() != b->n) b->alive.assign(b->n, 1);
+  b->alive[i] = 0;
+  OB_CATCH
+}
+
+OB_API int ob_bunch_compress(ob_bunch* b) {  // Bunch.cc:704-763 (order kept)
+  OB_TRY OB_CHECK_B(b);
+  if ((int64_t)b->alive.size() != b->n) {
+    b->alive.assign(b->n, 1);
+    return 0;
+  }
+  bool any_dead = false;
+  for (int64_t i = 0; i < b->n; i++) any_dead |= (b->alive[i] == 0);
+  if (!any_dead) return 0;
+  ob::host_write(b);
+  int64_t w = 0;
+  for (int64_t i = 0; i < b->n; i++) {
+    if (!b->alive[i]) continue;
+    for (int k = 0; k < 6; k++) b->h[k][w] = b->h[k][i];
+    if (b->has_msize) b->h_msize[w] = b->h_msize[i];
+    w++;
+  }
+  for (int k = 0; k < 6; k++) b->h[k].resize(w);
+  if (b->has_msize) b->h_msize.resize(w);
+  b->n = w;
+  b->alive.assign(w, 1);
+  OB_CATCH
+}
+
+OB_API int ob_bunch_set_macrosizes(ob_bunch* b, int64_t n, const double* m) {
+  OB_TRY OB_CHECK_B(b);
+  if (m == nullptr) {
+    b->has_msize = false;
+    b->h_msize.clear();
+    return 0;
+  }
+  OB_REQUIRE(n == b->n, "macrosize array length differs from the bunch size");
+  ob::download(b);
+  b->h_msize.assign(m, m + n);
+  b->has_msize = true;
+  if (b->dev_valid) {
+    if (!b->d_msize) OB_HIP(hipMalloc((void**)&b->d_msize, sizeof(double) * b->cap));
+    if (n)
+      OB_HIP(hipMemcpyAsync(b->d_msize, m, sizeof(double) * n, hipMemcpyHostToDevice,
+                            ob::stream()));
+    OB_HIP(hipStreamSynchronize(ob::stream()));
+  }
+  OB_CATCH
+}
+
+OB_API int ob_bunch_get_macrosizes(ob_bunch* b, int64_t n, double* m) {
+  OB_TRY OB_CHECK_B(b);
+  OB_REQUIRE(b->has_msize, "bunch has no macrosize attribute");
+  OB_REQUIRE(n == b->n, "macrosize array length differs from the bunch size");
+  if (n) std::memcpy(m, b->h_msize.data(), sizeof(double) * n);
+  OB_CATCH
+}
+
+OB_API int ob_bunch_device_coords(ob_bunch* b, double** ptrs6) {
+  OB_TRY OB_CHECK_B(b);
+  ob::DeviceBunchView v = ob::bunch_device_view(b);
+  ob::bunch_flush(b, nullptr);
+  for (int k = 0; k < 6; k++) ptrs6[k] = v.c[k];
+  b->host_valid = false;
+  OB_CATCH
+}
+
+OB_API int ob_bunch_flush(ob_bunch* b) {
+  OB_TRY OB_CHECK_B(b);
+  ob::bunch_flush(b, nullptr);
+  OB_CATCH
+}
+
+// ---------------------------------------------------------------------------
+// teapot_base (teapotbase.cc).  Host work per call = the reference's pre-loop code.
+namespace {
+ob::Op mkop(int kind) {
+  ob::Op op;
+  std::memset(&op, 0, sizeof(op));
+  op.kind = kind;
+  return op;
+}
+double factorial(int n) {  // teapotbase.cc:45-57
+  double f = 1.0;
+  for (int i = 1; i <= n; i++) f = i * f;
+  return f;
+}
+void drift_op(ob_bunch* b, double length) {  // teapotbase.cc:158-183
+  const double v = 2.99792458e+8 * b->beta;
+  b->time = b->time + length / v;
+  ob::Op op = mkop(ob::OP_DRIFT);
+  op.p[0] = length;
+  op.p[1] = 1.0 / (b->gamma * b->gamma);
+  op.p[2] = 1.0 / (b->p * b->beta);
+  ob::bunch_queue(b, op);
+}
+}  // namespace
+
+OB_API int ob_tpb_rotatexy(ob_bunch* b, double angle) {
+  OB_TRY OB_CHECK_B(b);
+  ob::Op op = mkop(ob::OP_ROTATEXY);
+  op.p[0] = cos(angle);
+  op.p[1] = sin(angle);
+  ob::bunch_queue(b, op);
+  OB_CATCH
+}
+
+OB_API int ob_tpb_drift(ob_bunch* b, double length) {
+  OB_TRY OB_CHECK_B(b);
+  drift_op(b, length);
+  OB_CATCH
+}
+
+OB_API int ob_tpb_wrapbunch(ob_bunch* b, double length) {
+  OB_TRY OB_CHECK_B(b);
+  ob::Op op = mkop(ob::OP_WRAPBUNCH);
+  op.p[0] = length;
+  ob::bunch_queue(b, op);
+  OB_CATCH
+}
+
+OB_API int ob_tpb_kick(ob_bunch* b, double kx, double ky, double kE, int) {
+  OB_TRY OB_CHECK_B(b);
+  ob::Op op = mkop(ob::OP_KICK);
+  op.p[0] = kx;
+  op.p[1] = ky;
+  op.p[2] = kE;
+  ob::bunch_queue(b, op);
+  OB_CATCH
+}
+
+OB_API int ob_tpb_multp(ob_bunch* b, int pole, double kl, int skew, int) {
+  OB_TRY OB_CHECK_B(b);
+  OB_REQUIRE(pole >= 0 && pole < 47, "multp: pole out of range");
+  if (b->charge == 0.) return 0;  // teapotbase.cc:338-340
+  ob::Op op = mkop(ob::OP_MULTP);
+  op.i0 = pole;
+  op.i1 = skew;
+  op.p[0] = kl / factorial(pole);
+  ob::bunch_queue(b, op);
+  OB_CATCH
+}
+
+static int multpfringe(ob_bunch* b, int pole, double kl, int skew, bool out) {
+  OB_REQUIRE(pole >= 0 && pole < 47, "multpfringe: pole out of range");
+  if (b->charge == 0.) return 0;  // teapotbase.cc:397-399
+  ob::Op op = mkop(out ? ob::OP_MULTPFRINGE_OUT : ob::OP_MULTPFRINGE_IN);
+  op.i0 = pole;
+  op.i1 = skew;
+  const double klfactlp1 = kl / (4.0 * factorial(pole + 2));
+  double kre = skew ? 0.0 : klfactlp1, kim = skew ? klfactlp1 : 0.0;
+  if (!out) {  // -kterm (teapotbase.cc:459-464)
+    kre = -kre;
+    kim = -kim;
+  }
+  op.p[0] = 1.0 / (b->p * b->beta);
+  op.p[1] = kre;
+  op.p[2] = kim;
+  ob::bunch_queue(b, op);
+  return 0;
+}
+
+OB_API int ob_tpb_multpfringeIN(ob_bunch* b, int pole, double kl, int skew, int) {
+  OB_TRY OB_CHECK_B(b);
+  multpfringe(b, pole, kl, skew, false);
+  OB_CATCH
+}
+OB_API int ob_tpb_multpfringeOUT(ob_bunch* b, int pole, double kl, int skew, int) {
+  OB_TRY OB_CHECK_B(b);
+  multpfringe(b, pole, kl, skew, true);
+  OB_CATCH
+}
+
+OB_API int ob_tpb_quad1(ob_bunch* b, double length, double kq, int) {  // teapotbase.cc:620-700
+  OB_TRY OB_CHECK_B(b);
+  if (kq == 0. || b->charge == 0.) {
+    drift_op(b, length);
+    return 0;
+  }
+  const double v = 2.99792458e+8 * b->beta;
+  if (length > 0.) b->time = b->time + length / v;
+  const double gamma2i = 1.0 / (b->gamma * b->gamma);
+  const double dp_p_coeff = 1.0 / (b->p * b->beta);
+  double sqrt_kq, kqlength, cx, sx, cy, sy;
+  double m[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+  if (kq > 0.) {
+    sqrt_kq = pow(kq, 0.5);
+    kqlength = sqrt_kq * length;
+    cx = cos(kqlength); sx = sin(kqlength); cy = cosh(kqlength); sy = sinh(kqlength);
+    m[0] = cx; m[1] = sx / sqrt_kq; m[2] = -sx * sqrt_kq; m[3] = cx;
+    m[4] = cy; m[5] = sy / sqrt_kq; m[6] = sy * sqrt_kq; m[7] = cy;
+  } else {
+    sqrt_kq = pow(-kq, 0.5);
+    kqlength = sqrt_kq * length;
+    cx = cosh(kqlength); sx = sinh(kqlength); cy = cos(kqlength); sy = sin(kqlength);
+    m[0] = cx; m[1] = sx / sqrt_kq; m[2] = sx * sqrt_kq; m[3] = cx;
+    m[4] = cy; m[5] = sy / sqrt_kq; m[6] = -sy * sqrt_kq; m[7] = cy;
+  }
+  ob::Op op = mkop(ob::OP_QUAD1);
+  for (int k = 0; k < 8; k++) op.p[k] = m[k];
+  op.p[8] = gamma2i;
+  op.p[9] = length;
+  op.p[10] = dp_p_coeff;
+  ob::bunch_queue(b, op);
+  OB_CATCH
+}
+
+OB_API int ob_tpb_quad2(ob_bunch* b, double length) {
+  OB_TRY OB_CHECK_B(b);
+  ob::Op op = mkop(ob::OP_QUAD2);
+  op.p[0] = length;
+  op.p[1] = 1.0 / (b->gamma * b->gamma);
+  op.p[2] = 1.0 / (b->p * b->beta);
+  ob::bunch_queue(b, op);
+  OB_CATCH
+}
+
+static void quadfringe(ob_bunch* b, double kq, bool out) {
+  if (b->charge == 0.) return;  // teapotbase.cc:786-788
+  ob::Op op = mkop(out ? ob::OP_QUADFRINGE_OUT : ob::OP_QUADFRINGE_IN);
+  op.p[0] = 1.0 / (b->p * b->beta);
+  op.p[1] = kq;
+  ob::bunch_queue(b, op);
+}
+OB_API int ob_tpb_quadfringeIN(ob_bunch* b, double kq, int) {
+  OB_TRY OB_CHECK_B(b);
+  quadfringe(b, kq, false);
+  OB_CATCH
+}
+OB_API int ob_tpb_quadfringeOUT(ob_bunch* b, double kq, int) {
+  OB_TRY OB_CHECK_B(b);
+  quadfringe(b, kq, true);
+  OB_CATCH
+}

@@ -1,0 +1,144 @@
+// ob_internal.hpp -- shared types of liborbit_mi355x (host + device).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cfloat>
+#include <cstdint>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/orbit_mi355x.h"
+
+namespace ob {
+
+// ---------------------------------------------------------------------------
+// Errors: every C entry point returns a status; the message is kept per thread.
+void set_error(const std::string& msg);
+struct Error {
+  std::string msg;
+};
+#define OB_HIP(call)                                                                 \
+  do {                                                                               \
+    hipError_t e_ = (call);                                                          \
+    if (e_ != hipSuccess)                                                            \
+      throw ::ob::Error{std::string(#call) + ": " + hipGetErrorString(e_)};          \
+  } while (0)
+#define OB_REQUIRE(cond, msg)              \
+  do {                                     \
+    if (!(cond)) throw ::ob::Error{(msg)}; \
+  } while (0)
+
+hipStream_t stream();
+int device_cu_count();
+void ensure_init();
+
+// Kernel timing with HIP events on the library stream (ob_profile_enable/_report).  A
+// ProfScope brackets one launch when profiling is on and costs nothing otherwise.
+bool profiling();
+void prof_mark(const char* name, bool begin);
+struct ProfScope {
+  const char* name;
+  explicit ProfScope(const char* n) : name(n) {
+    if (profiling()) prof_mark(name, true);
+  }
+  ~ProfScope() {
+    if (profiling()) prof_mark(name, false);
+  }
+};
+
+// Pending device-side status words: a kernel writes a non-zero code when it finds a
+// condition the reference treats as fatal.  The host reads them at sync points.
+void watch_status(int* dev_status, const char* what);
+void check_statuses(bool wait);
+
+// ---------------------------------------------------------------------------
+// Fused map program.  Each op carries host-precomputed constants so the device does
+// exactly the reference's per-particle arithmetic (teapotbase.cc).
+enum OpKind : int32_t {
+  OP_DRIFT = 1,
+  OP_QUAD1,
+  OP_QUAD2,
+  OP_QUADFRINGE_IN,
+  OP_QUADFRINGE_OUT,
+  OP_MULTP,
+  OP_MULTPFRINGE_IN,
+  OP_MULTPFRINGE_OUT,
+  OP_WRAPBUNCH,
+  OP_ROTATEXY,
+  OP_KICK,
+  OP_SC2P5DRB_KICK,
+  OP_SC3D_KICK,
+};
+
+struct Op {
+  int32_t kind;
+  int32_t i0, i1, i2;  // integer parameters (pole, skew, ...)
+  double p[12];        // double parameters / device pointers
+};
+static_assert(sizeof(Op) == 112, "Op layout");
+
+constexpr int kMaxOpsPerLaunch = 24;
+struct OpProgram {
+  int32_t nops;
+  int32_t want_extrema;  // write per-block min/max of x, y, z after the last op
+  int32_t pad[2];
+  Op ops[kMaxOpsPerLaunch];
+};
+
+// Space-charge state shared between the 2.5D kernels and the kick op.
+struct SC2DState {
+  double ext[6];                         // xMin xMax yMin yMax zMin zMax of the bunch
+  double xMin, xMax, yMin, yMax, dx, dy; // rho/phi grid (Grid2D::setGridX/Y)
+  double zMin, zMax, dz;                 // z line-density grid (Grid1D::setGridZ)
+  double total, x_c, y_c, a_bunch;       // bunchAnalysis moments (:229-254)
+  double scale_coeff;                    // log(dx_solver/dx_rho)
+  double factor;                         // transverse kick factor (:129-133)
+  double lsf_in, a2;                     // longitudinal RB coefficients (:138-141)
+  int32_t status;                        // 1: zero-size bunch (reference aborts, :176-189)
+  int32_t skip;                          // 1: do not kick
+};
+
+struct SC3DState {
+  double ext[6];
+  double xMin, xMax, yMin, yMax, zMin, zMax, dx, dy, dz;  // rest-frame z limits
+  double zc;                                              // (zMax+zMin)/2 rest frame
+  int32_t status, skip;
+};
+
+// Device pointers travel inside Op::p as raw 64-bit patterns.
+inline double ptr_bits(const void* p) {
+  double d;
+  std::memcpy(&d, &p, sizeof(d));
+  return d;
+}
+
+struct DeviceBunchView {
+  double* c[6];
+  const double* msize;  // nullptr if no per-particle macrosize
+  int64_t n;
+};
+
+// Launch the fused map program over a bunch; partial extrema (6 per block) go to
+// ext_partials when prog.want_extrema.  Returns the number of blocks used.
+int launch_push(const DeviceBunchView& v, const OpProgram& prog, double* ext_partials,
+                hipStream_t s);
+int push_grid_blocks(int64_t n);
+
+}  // namespace ob
+
+struct ob_bunch;
+namespace ob {
+// Bunch internals used by the space-charge calculators.
+DeviceBunchView bunch_device_view(ob_bunch* b);
+// Queue an op; flushes first when the queue is full.
+void bunch_queue(ob_bunch* b, const Op& op);
+// Apply queued ops; if ext_partials != nullptr compute partial extrema too.
+int bunch_flush(ob_bunch* b, double* ext_partials);
+int64_t bunch_size(ob_bunch* b);
+struct SyncInfo {
+  double mass, charge, r0, macrosize, T, p, beta, gamma, time;
+};
+SyncInfo bunch_sync(ob_bunch* b);
+}  // namespace ob

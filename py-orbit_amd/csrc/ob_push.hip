@@ -1,0 +1,286 @@
+// ob_push.hip -- the fused TEAPOT map kernel.
+//
+// One launch applies a program of queued maps (drift, quad1, quad2, quad/multipole
+// fringes, multipole kicks, wraps, rotations, plain kicks) and the deferred
+// space-charge kicks to every particle, holding the six FP64 coordinates in
+// registers: each particle is read once and written once per launch (96 B) however
+// many maps the program holds.  When the program feeds a space-charge node it also
+// emits per-block min/max of x, y, z for the next grid (BunchExtremaCalculator.cc:
+// 37-83), so the extrema pass costs no extra HBM read.
+//
+// Arithmetic follows the reference routine of each map operation for operation
+// (cited per case), compiled with -ffp-contract=off so nothing is fused: results
+// are bit-identical to src/teapot/teapotbase.cc built with g++ for x86-64.
+#include "ob_internal.hpp"
+#include "ob_sc_device.hpp"
+
+namespace ob {
+
+namespace {
+
+struct cplx {
+  double re, im;
+};
+__device__ __forceinline__ cplx cmul(cplx a, cplx b) {  // std::complex<double> product
+  return {a.re * b.re - a.im * b.im, a.re * b.im + a.im * b.re};
+}
+__device__ __forceinline__ cplx cadd(cplx a, cplx b) { return {a.re + b.re, a.im + b.im}; }
+__device__ __forceinline__ cplx csub(cplx a, cplx b) { return {a.re - b.re, a.im - b.im}; }
+
+// teapotbase.cc:395-490 / 508-603.  p[0]=dp_p_coeff, p[1..2]=kk (+/-kterm), i0=pole.
+__device__ __forceinline__ void multpfringe(const Op& op, double& x, double& xp, double& y,
+                                            double& yp, double& z, double dE) {
+  const int lm1 = op.i0, l = op.i0 + 1, lp2 = op.i0 + 3;
+  const cplx rootm1 = {0.0, 1.0};
+  const cplx cxl = {(double)l, 0.0}, cxlp2 = {(double)lp2, 0.0};
+  const cplx kk = {op.p[1], op.p[2]};
+  const double px = xp, py = yp;
+  const double dp_p = dE * op.p[0];
+  const double KNL = 1.0 / (1.0 + dp_p);
+  const cplx zz = {x, y};
+  cplx zlm1 = {1.0, 0.0};
+  for (int k = 0; k < lm1; k++) zlm1 = cmul(zlm1, zz);
+  const cplx zl = cmul(zlm1, zz);
+  const cplx fxterm = {l * x, -lp2 * y};
+  const cplx dxfxterm = cmul(cxl, cadd(fxterm, zz));
+  const cplx dyfxterm = cmul(rootm1, csub(cmul(cxl, fxterm), cmul(cxlp2, zz)));
+  const cplx fyterm = {l * y, lp2 * x};
+  const cplx dxfyterm = cadd(cmul(cxl, fyterm), cmul(cmul(rootm1, cxlp2), zz));
+  const cplx dyfyterm = cmul(cxl, cadd(cmul(rootm1, fyterm), zz));
+  const cplx fxcx = cmul(cmul(kk, zl), fxterm);
+  const cplx dxfxcx = cmul(cmul(kk, zlm1), dxfxterm);
+  const cplx dyfxcx = cmul(cmul(kk, zlm1), dyfxterm);
+  const cplx fycx = cmul(cmul(kk, zl), fyterm);
+  const cplx dxfycx = cmul(cmul(kk, zlm1), dxfyterm);
+  const cplx dyfycx = cmul(cmul(kk, zlm1), dyfyterm);
+  x -= fxcx.re * KNL;
+  y -= fycx.re * KNL;
+  const double M11 = 1.0 - dxfxcx.re * KNL;
+  const double M12 = -dxfycx.re * KNL;
+  const double M21 = -dyfxcx.re * KNL;
+  const double M22 = 1.0 - dyfycx.re * KNL;
+  const double detM = M11 * M22 - M12 * M21;
+  const double pxnew = (M22 * px - M12 * py) / detM;
+  const double pynew = (-M21 * px + M11 * py) / detM;
+  xp = pxnew;
+  yp = pynew;
+  z -= (pxnew * fxcx.re + pynew * fycx.re) * KNL * KNL;
+}
+
+// teapotbase.cc:784-835 (IN) / 854-905 (OUT).  p[0]=dp_p_coeff, p[1]=kq.
+template <bool OUT>
+__device__ __forceinline__ void quadfringe(const Op& op, double& x, double& xp, double& y,
+                                           double& yp, double& z, double dE) {
+  const double kq = op.p[1];
+  const double dp_p = dE * op.p[0];
+  const double KNL = 1.0 / (1.0 + dp_p);
+  const double x0 = x, xp0 = xp, y0 = y, yp0 = yp;
+  const double t = (kq * KNL / 4.) * (x0 * x0 - y0 * y0);
+  const double detM = 1.0 - t * t;
+  const double dx = (kq * KNL / 12.) * x0 * (x0 * x0 + 3. * y0 * y0);
+  const double dxp = (kq * KNL / 4.) * (xp0 * (x0 * x0 + y0 * y0) - 2. * yp0 * x0 * y0);
+  const double dy = (kq * KNL / 12.) * y0 * (y0 * y0 + 3. * x0 * x0);
+  const double dyp = (kq * KNL / 4.) * (-yp0 * (x0 * x0 + y0 * y0) + 2. * xp0 * x0 * y0);
+  const double dz = (kq * KNL * KNL / 12.) *
+                    (xp0 * x0 * (x0 * x0 + 3. * y0 * y0) - yp0 * y0 * (y0 * y0 + 3. * x0 * x0));
+  if (!OUT) {
+    x += dx;
+    xp -= dxp;
+    xp /= detM;
+    y -= dy;
+    yp -= dyp;
+    yp /= detM;
+    z += dz;
+  } else {
+    x -= dx;
+    xp += dxp;
+    xp /= detM;
+    y += dy;
+    yp += dyp;
+    yp /= detM;
+    z -= dz;
+  }
+}
+
+__device__ __forceinline__ void apply_op(const Op& op, double& x, double& xp, double& y,
+                                         double& yp, double& z, double& dE) {
+  switch (op.kind) {
+    case OP_DRIFT: {  // teapotbase.cc:158-183. p: length, gamma2i, dp_p_coeff
+      const double length = op.p[0], gamma2i = op.p[1];
+      const double dp_p = dE * op.p[2];
+      const double KNL = 1.0 / (1.0 + dp_p);
+      x += KNL * length * xp;
+      y += KNL * length * yp;
+      double phifac = (xp * xp + yp * yp + dp_p * dp_p * gamma2i) / 2.0;
+      phifac = (phifac * KNL - dp_p * gamma2i) * KNL;
+      z -= length * phifac;
+      break;
+    }
+    case OP_QUAD1: {  // teapotbase.cc:686-699. p: m11 m12 m21 m22 m33 m34 m43 m44, gamma2i,
+                      // length, dp_p_coeff
+      const double dp_p = dE * op.p[10];
+      const double x0 = x, xp0 = xp, y0 = y, yp0 = yp;
+      x = x0 * op.p[0] + xp0 * op.p[1];
+      xp = x0 * op.p[2] + xp0 * op.p[3];
+      y = y0 * op.p[4] + yp0 * op.p[5];
+      yp = y0 * op.p[6] + yp0 * op.p[7];
+      z += dp_p * op.p[8] * op.p[9];
+      break;
+    }
+    case OP_QUAD2: {  // teapotbase.cc:719-745. p: length, gamma2i, dp_p_coeff
+      const double length = op.p[0], gamma2i = op.p[1];
+      const double dp_p = dE * op.p[2];
+      const double KNL = 1.0 / (1.0 + dp_p);
+      x -= KNL * length * dp_p * xp;
+      y -= KNL * length * dp_p * yp;
+      double phifac = (xp * xp + yp * yp + dp_p * dp_p * gamma2i) / 2.0;
+      phifac = (phifac * KNL + dp_p * dp_p * gamma2i) * KNL;
+      z -= length * phifac;
+      break;
+    }
+    case OP_QUADFRINGE_IN:
+      quadfringe<false>(op, x, xp, y, yp, z, dE);
+      break;
+    case OP_QUADFRINGE_OUT:
+      quadfringe<true>(op, x, xp, y, yp, z, dE);
+      break;
+    case OP_MULTP: {  // teapotbase.cc:336-372. p[0]=kl/pole!, i0=pole, i1=skew
+      const cplx zz = {x, y};
+      cplx zn = {1.0, 0.0};
+      for (int k = 0; k < op.i0; k++) zn = cmul(zn, zz);
+      const double kl1 = op.p[0];
+      if (op.i1) {
+        xp += kl1 * zn.im;
+        yp += kl1 * zn.re;
+      } else {
+        xp -= kl1 * zn.re;
+        yp += kl1 * zn.im;
+      }
+      break;
+    }
+    case OP_MULTPFRINGE_IN:
+    case OP_MULTPFRINGE_OUT:
+      multpfringe(op, x, xp, y, yp, z, dE);
+      break;
+    case OP_WRAPBUNCH: {  // teapotbase.cc:202-211. p[0]=length
+      const double length = op.p[0];
+      if (z < -length / 2.0) z += length;
+      if (z > length / 2.0) z -= length;
+      break;
+    }
+    case OP_ROTATEXY: {  // teapotbase.cc:80-101. p: cos, sin
+      const double cs = op.p[0], sn = op.p[1];
+      const double xt = x, pxt = xp, yt = y, pyt = yp;
+      x = cs * xt - sn * yt;
+      xp = cs * pxt - sn * pyt;
+      y = sn * xt + cs * yt;
+      yp = sn * pxt + cs * pyt;
+      break;
+    }
+    case OP_KICK:  // teapotbase.cc:232-257. p: kx, ky, kE
+      if (op.p[0] != 0.) xp += op.p[0];
+      if (op.p[1] != 0.) yp += op.p[1];
+      if (op.p[2] != 0.) dE += op.p[2];
+      break;
+    case OP_SC2P5DRB_KICK:
+      sc2p5drb_kick(op, x, xp, y, yp, z, dE);
+      break;
+    case OP_SC3D_KICK:
+      sc3d_kick(op, x, xp, y, yp, z, dE);
+      break;
+    default:
+      break;
+  }
+}
+
+constexpr int kPushThreads = 256;
+constexpr int kMaxPushBlocks = 4096;
+
+__device__ __forceinline__ double wave_min(double v) {
+  for (int o = 32; o > 0; o >>= 1) v = fmin(v, __shfl_xor(v, o, 64));
+  return v;
+}
+__device__ __forceinline__ double wave_max(double v) {
+  for (int o = 32; o > 0; o >>= 1) v = fmax(v, __shfl_xor(v, o, 64));
+  return v;
+}
+
+template <bool EXTREMA>
+__global__ __launch_bounds__(kPushThreads) void push_kernel(DeviceBunchView v, OpProgram prog,
+                                                            double* __restrict__ partials) {
+  double lo[3] = {DBL_MAX, DBL_MAX, DBL_MAX};
+  double hi[3] = {-DBL_MAX, -DBL_MAX, -DBL_MAX};
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  double* __restrict__ cx = v.c[0];
+  double* __restrict__ cxp = v.c[1];
+  double* __restrict__ cy = v.c[2];
+  double* __restrict__ cyp = v.c[3];
+  double* __restrict__ cz = v.c[4];
+  double* __restrict__ cdE = v.c[5];
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < v.n; i += stride) {
+    double x = cx[i], xp = cxp[i], y = cy[i], yp = cyp[i], z = cz[i], dE = cdE[i];
+    for (int k = 0; k < prog.nops; k++) apply_op(prog.ops[k], x, xp, y, yp, z, dE);
+    cx[i] = x;
+    cxp[i] = xp;
+    cy[i] = y;
+    cyp[i] = yp;
+    cz[i] = z;
+    cdE[i] = dE;
+    if (EXTREMA) {
+      lo[0] = fmin(lo[0], x);
+      hi[0] = fmax(hi[0], x);
+      lo[1] = fmin(lo[1], y);
+      hi[1] = fmax(hi[1], y);
+      lo[2] = fmin(lo[2], z);
+      hi[2] = fmax(hi[2], z);
+    }
+  }
+  if (EXTREMA) {
+    __shared__ double red[kPushThreads / 64][6];
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    double r[6];
+    for (int k = 0; k < 3; k++) {
+      r[2 * k] = wave_min(lo[k]);
+      r[2 * k + 1] = wave_max(hi[k]);
+    }
+    if (lane == 0)
+      for (int k = 0; k < 6; k++) red[w][k] = r[k];
+    __syncthreads();
+    if (threadIdx.x < 6) {
+      const int k = threadIdx.x;
+      double acc = red[0][k];
+      for (int j = 1; j < kPushThreads / 64; j++)
+        acc = (k & 1) ? fmax(acc, red[j][k]) : fmin(acc, red[j][k]);
+      partials[(int64_t)blockIdx.x * 6 + k] = acc;
+    }
+  }
+}
+
+}  // namespace
+
+int push_grid_blocks(int64_t n) {
+  int64_t b = (n + kPushThreads - 1) / kPushThreads;
+  if (b < 1) b = 1;
+  if (b > kMaxPushBlocks) b = kMaxPushBlocks;
+  return (int)b;
+}
+
+int launch_push(const DeviceBunchView& v, const OpProgram& prog, double* ext_partials,
+                hipStream_t s) {
+  const int blocks = push_grid_blocks(v.n);
+  if (prog.want_extrema) {
+    OB_REQUIRE(ext_partials != nullptr, "push: extrema requested without a buffer");
+    ProfScope ps("push_extrema");
+    hipLaunchKernelGGL(push_kernel<true>, dim3(blocks), dim3(kPushThreads), 0, s, v, prog,
+                       ext_partials);
+  } else {
+    if (prog.nops == 0 || v.n == 0) return 0;
+    ProfScope ps("push");
+    hipLaunchKernelGGL(push_kernel<false>, dim3(blocks), dim3(kPushThreads), 0, s, v, prog,
+                       nullptr);
+  }
+  OB_HIP(hipGetLastError());
+  return blocks;
+}
+
+}  // namespace ob

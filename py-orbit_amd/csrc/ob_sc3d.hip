@@ -1,0 +1,617 @@
+// ob_sc3d.hip -- SpaceChargeCalc3D on the device.
+//
+// Per trackBunch (SpaceChargeCalc3D.cc:100-148):
+//   1. fused map kernel flushes queued maps and emits per-block extrema; one block reduces
+//      them and the six numbers come to the host (the Green-function rebuild decision of
+//      bunchAnalysis, :186-203, launches kernels, so it is taken on the host);
+//   2. host: bunchAnalysis grid sizing (:150-249), identical scalar arithmetic;
+//   3. (on a shape change) Green function 1/r and its FFT, built on the device (:160-247);
+//   4. 27-point TSC deposit (Grid3D.cc:281-432) with FP64 atomics;
+//   5. Hockney solve (PoissonSolverFFT3D.cc:249-347) in five passes over a z-fastest
+//      layout: z r2c (two real lines per complex FFT), y FFT, x FFT x Green x inverse x FFT,
+//      inverse y FFT, z c2r.  Zero padding is pruned: lines that are entirely zero are
+//      never read, and only the nx*ny*nz outputs that are kept are written;
+//   6. the kick (:120-147) is queued on the bunch and runs fused into the next map kernel.
+//
+// The Green spectrum of the even function 1/r is real and even in kx and ky, so only the
+// quadrant kx <= M0/2, ky <= M1/2 of its real part is stored (8x less than the complex
+// spectrum the reference keeps; the discarded imaginary part is rounding noise).
+#include <cmath>
+#include <cstring>
+
+#include "ob_fft.hpp"
+#include "ob_internal.hpp"
+#include "ob_sc_device.hpp"
+
+namespace ob {
+namespace {
+
+constexpr int kThreads = 256;
+constexpr int kZPairs = 4;      // complex z-lines (= 8 real lines) per block in the z passes
+constexpr int kTileLines = 8;   // contiguous kz lines per block in the y and x passes
+
+enum StridedMode { FWD = 0, INV = 1, CONV = 2, GREEN_OUT = 3 };
+
+struct GreenGeom {
+  int M0, M1, M2;
+  double dx, dy, dz;
+};
+
+__device__ __forceinline__ double green3d(int i, int j, int k, const GreenGeom& g) {
+  // PoissonSolverFFT3D::_defineGreenF (:160-247), nBunches_ = 0
+  const int ii = (i <= g.M0 / 2) ? i : g.M0 - i;
+  const int jj = (j <= g.M1 / 2) ? j : g.M1 - j;
+  const int kk = (k <= g.M2 / 2) ? k : g.M2 - k;
+  const double rZ = kk * g.dz, rZ2 = rZ * rZ;
+  const double rY = jj * g.dy, rY2 = rY * rY;
+  const double rX = ii * g.dx, rX2 = rX * rX;
+  const double rTot = sqrt(rX2 + rY2 + rZ2);
+  return (ii != 0 || jj != 0 || kk != 0) ? 1. / rTot : 0.0;
+}
+
+// ---- extrema reduction (one block) -------------------------------------------
+__global__ __launch_bounds__(1024) void reduce_extrema(const double* __restrict__ partials,
+                                                       int nblk, double* __restrict__ out) {
+  __shared__ double red[16][6];
+  double r[6] = {DBL_MAX, -DBL_MAX, DBL_MAX, -DBL_MAX, DBL_MAX, -DBL_MAX};
+  for (int b = threadIdx.x; b < nblk; b += blockDim.x)
+    for (int k = 0; k < 6; k++) {
+      const double v = partials[b * 6 + k];
+      r[k] = (k & 1) ? fmax(r[k], v) : fmin(r[k], v);
+    }
+  for (int k = 0; k < 6; k++)
+    for (int o = 32; o > 0; o >>= 1) {
+      const double v = __shfl_xor(r[k], o, 64);
+      r[k] = (k & 1) ? fmax(r[k], v) : fmin(r[k], v);
+    }
+  if ((threadIdx.x & 63) == 0)
+    for (int k = 0; k < 6; k++) red[threadIdx.x >> 6][k] = r[k];
+  __syncthreads();
+  if (threadIdx.x < 6) {
+    const int k = threadIdx.x;
+    double acc = red[0][k];
+    for (int w = 1; w < (int)(blockDim.x >> 6); w++)
+      acc = (k & 1) ? fmax(acc, red[w][k]) : fmin(acc, red[w][k]);
+    out[k] = acc;
+  }
+}
+
+__global__ void fill_zero(double* __restrict__ a, int64_t n) {
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) a[i] = 0.0;
+}
+
+// ---- deposit: Grid3D::binValue (Grid3D.cc:307-432), z-fastest layout ----------
+struct Grid3 {
+  int nx, ny, nz;
+  double xMin, xMax, yMin, yMax, zMin, zMax, dx, dy, dz;
+};
+
+__global__ __launch_bounds__(kThreads) void deposit3d(DeviceBunchView v, Grid3 g,
+                                                      double macrosize, double* __restrict__ rho) {
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  const int64_t sx = (int64_t)g.ny * g.nz, sy = g.nz;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < v.n; i += stride) {
+    const double x = v.c[0][i], y = v.c[2][i], z = v.c[4][i];
+    if (x < g.xMin || x > g.xMax || y < g.yMin || y > g.yMax || z < g.zMin || z > g.zMax) continue;
+    const double m = v.msize ? v.msize[i] : macrosize;
+    int iX, iY, iZ;
+    double xF, yF, zF;
+    node_ind_frac(x, g.xMin, g.dx, g.nx, iX, xF);
+    node_ind_frac(y, g.yMin, g.dy, g.ny, iY, yF);
+    grid3d_z_ind_frac(z, g.zMin, g.dz, g.nz, iZ, zF);
+    const double wx[3] = {0.5 * (0.5 - xF) * (0.5 - xF), 0.75 - xF * xF, 0.5 * (0.5 + xF) * (0.5 + xF)};
+    const double wy[3] = {0.5 * (0.5 - yF) * (0.5 - yF), 0.75 - yF * yF, 0.5 * (0.5 + yF) * (0.5 + yF)};
+    int iZm = iZ - 1, iZp = iZ + 1;
+    if (iZm < 0) iZm = g.nz - 1;
+    if (iZp >= g.nz) iZp = 0;
+    double* base = rho + (int64_t)(iX - 1) * sx + (int64_t)(iY - 1) * sy;
+    if (g.nz >= 3) {
+      const double wz[3] = {0.5 * (0.5 - zF) * (0.5 - zF), 0.75 - zF * zF,
+                            0.5 * (0.5 + zF) * (0.5 + zF)};
+      const int zz[3] = {iZm, iZ, iZp};
+#pragma unroll
+      for (int a = 0; a < 3; a++)
+#pragma unroll
+        for (int c = 0; c < 3; c++) {
+          const double tmp = wy[c] * wz[a] * m;
+#pragma unroll
+          for (int d = 0; d < 3; d++) atomicAdd(base + d * sx + c * sy + zz[a], wx[d] * tmp);
+        }
+    } else if (g.nz == 2) {
+      const double wz[2] = {1.0 - zF, zF};
+      for (int a = 0; a < 2; a++)
+        for (int c = 0; c < 3; c++) {
+          const double tmp = wy[c] * wz[a] * m;
+          for (int d = 0; d < 3; d++) atomicAdd(base + d * sx + c * sy + a, wx[d] * tmp);
+        }
+    } else {
+      for (int c = 0; c < 3; c++) {
+        const double tmp = wy[c] * m;
+        for (int d = 0; d < 3; d++) atomicAdd(base + d * sx + c * sy, wx[d] * tmp);
+      }
+    }
+  }
+}
+
+// ---- z pass r2c: two real z-lines per complex FFT -------------------------------
+// Real line q = i*NY + j (i < rows, j < NY) holds nz_in values (rest zero-padded to M2).
+template <bool GREEN>
+__global__ __launch_bounds__(kThreads) void fft3_z_r2c(const double* __restrict__ rho,
+                                                       int64_t nlines, int NY, int nz_in, int M2,
+                                                       int M1, double2* __restrict__ spec,
+                                                       const double2* __restrict__ tw, int nmax,
+                                                       GreenGeom gg) {
+  extern __shared__ double sm[];
+  const int ls = M2 + 1;
+  double* re = sm;
+  double* im = sm + kZPairs * ls;
+  const int log2n = ilog2(M2);
+  const int H2 = M2 / 2 + 1;
+  const int64_t pair0 = (int64_t)blockIdx.x * kZPairs;
+  for (int idx = threadIdx.x; idx < kZPairs * M2; idx += blockDim.x) {
+    const int lp = idx / M2, k = idx - lp * M2;
+    const int64_t q0 = 2 * (pair0 + lp), q1 = q0 + 1;
+    double v0 = 0.0, v1 = 0.0;
+    if (k < nz_in) {
+      if (GREEN) {
+        if (q0 < nlines) v0 = green3d((int)(q0 / NY), (int)(q0 % NY), k, gg);
+        if (q1 < nlines) v1 = green3d((int)(q1 / NY), (int)(q1 % NY), k, gg);
+      } else {
+        if (q0 < nlines) v0 = rho[q0 * nz_in + k];
+        if (q1 < nlines) v1 = rho[q1 * nz_in + k];
+      }
+    }
+    const int p = bitrev(k, log2n);
+    re[lp * ls + p] = v0;
+    im[lp * ls + p] = v1;
+  }
+  __syncthreads();
+  lds_fft_dit(re, im, M2, log2n, kZPairs, tw, nmax, -1.0, ls);
+  for (int idx = threadIdx.x; idx < kZPairs * H2; idx += blockDim.x) {
+    const int lp = idx / H2, k = idx - lp * H2;
+    const int64_t q0 = 2 * (pair0 + lp), q1 = q0 + 1;
+    if (q0 >= nlines) continue;
+    const int nk = (M2 - k) & (M2 - 1);
+    const double zr = re[lp * ls + k], zi = im[lp * ls + k];
+    const double nr = re[lp * ls + nk], ni = im[lp * ls + nk];
+    const int64_t i0 = q0 / NY, j0 = q0 % NY;
+    spec[(i0 * M1 + j0) * H2 + k] = make_double2((zr + nr) * 0.5, (zi - ni) * 0.5);
+    if (q1 < nlines) {
+      const int64_t i1 = q1 / NY, j1 = q1 % NY;
+      spec[(i1 * M1 + j1) * H2 + k] = make_double2((zi + ni) * 0.5, -(zr - nr) * 0.5);
+    }
+  }
+}
+
+// ---- z pass c2r: two complex half-lines per complex inverse FFT ------------------
+__global__ __launch_bounds__(kThreads) void fft3_z_c2r(const double2* __restrict__ spec,
+                                                       int64_t nlines, int NY, int M2, int M1,
+                                                       int nz_out, double denom,
+                                                       double* __restrict__ phi,
+                                                       const double2* __restrict__ tw, int nmax) {
+  extern __shared__ double sm[];
+  const int ls = M2 + 1;
+  double* re = sm;
+  double* im = sm + kZPairs * ls;
+  const int log2n = ilog2(M2);
+  const int H2 = M2 / 2 + 1;
+  const int64_t pair0 = (int64_t)blockIdx.x * kZPairs;
+  for (int idx = threadIdx.x; idx < kZPairs * M2; idx += blockDim.x) {
+    const int lp = idx / M2, k = idx - lp * M2;
+    const int64_t q0 = 2 * (pair0 + lp), q1 = q0 + 1;
+    const int kk = (k <= M2 / 2) ? k : M2 - k;
+    double2 a = make_double2(0.0, 0.0), b = make_double2(0.0, 0.0);
+    if (q0 < nlines) a = spec[((q0 / NY) * M1 + (q0 % NY)) * H2 + kk];
+    if (q1 < nlines) b = spec[((q1 / NY) * M1 + (q1 % NY)) * H2 + kk];
+    if (kk == 0 || kk == M2 / 2) {  // c2r ignores the imaginary DC/Nyquist parts
+      a.y = 0.0;
+      b.y = 0.0;
+    }
+    double zr, zi;
+    if (k <= M2 / 2) {
+      zr = a.x - b.y;
+      zi = a.y + b.x;
+    } else {
+      zr = a.x + b.y;
+      zi = b.x - a.y;
+    }
+    re[lp * ls + k] = zr;
+    im[lp * ls + k] = zi;
+  }
+  __syncthreads();
+  lds_fft_dif(re, im, M2, log2n, kZPairs, tw, nmax, +1.0, ls);
+  for (int idx = threadIdx.x; idx < kZPairs * nz_out; idx += blockDim.x) {
+    const int lp = idx / nz_out, k = idx - lp * nz_out;
+    const int64_t q0 = 2 * (pair0 + lp), q1 = q0 + 1;
+    const int p = bitrev(k, log2n);
+    if (q0 < nlines) phi[q0 * nz_out + k] = denom * re[lp * ls + p];
+    if (q1 < nlines) phi[q1 * nz_out + k] = denom * im[lp * ls + p];
+  }
+}
+
+// ---- strided passes (y and x axes) -----------------------------------------------
+// Block = (outer index o, tile of up to kTileLines consecutive kz).  Element p of line
+// (o, kz) is spec[o*outer_stride + p*line_stride + kz].
+template <int MODE>
+__global__ __launch_bounds__(kThreads) void fft3_strided(double2* __restrict__ spec, int n,
+                                                         int n_in, int n_out, int64_t line_stride,
+                                                         int64_t outer_stride, int H2,
+                                                         double* __restrict__ ghat, int qy,
+                                                         int M0, int M1,
+                                                         const double2* __restrict__ tw,
+                                                         int nmax, int tile) {
+  extern __shared__ double sm[];
+  const int ls = n + 1;
+  double* re = sm;
+  double* im = sm + tile * ls;
+  const int log2n = ilog2(n);
+  const int tiles = (H2 + tile - 1) / tile;
+  const int o = blockIdx.x / tiles;
+  const int k0 = (blockIdx.x - o * tiles) * tile;
+  const int nl = min(tile, H2 - k0);
+  double2* base = spec + (int64_t)o * outer_stride + k0;
+  for (int idx = threadIdx.x; idx < nl * n; idx += blockDim.x) {
+    const int p = idx / nl, l = idx - p * nl;
+    const double2 v = (p < n_in) ? base[(int64_t)p * line_stride + l] : make_double2(0.0, 0.0);
+    const int pos = (MODE == INV) ? p : bitrev(p, log2n);
+    re[l * ls + pos] = v.x;
+    im[l * ls + pos] = v.y;
+  }
+  __syncthreads();
+  if (MODE == INV) {
+    lds_fft_dif(re, im, n, log2n, nl, tw, nmax, +1.0, ls);
+  } else {
+    lds_fft_dit(re, im, n, log2n, nl, tw, nmax, -1.0, ls);
+  }
+  if (MODE == GREEN_OUT) {  // x pass of the Green transform: keep the real quadrant
+    if (o > M1 / 2) return;
+    for (int idx = threadIdx.x; idx < nl * (M0 / 2 + 1); idx += blockDim.x) {
+      const int p = idx / nl, l = idx - p * nl;
+      ghat[((int64_t)p * qy + o) * H2 + k0 + l] = re[l * ls + p];
+    }
+    return;
+  }
+  if (MODE == CONV) {  // x pass: multiply by the Green spectrum, inverse x FFT
+    const int gy = (o <= M1 / 2) ? o : M1 - o;
+    for (int idx = threadIdx.x; idx < nl * n; idx += blockDim.x) {
+      const int p = idx / nl, l = idx - p * nl;
+      const int gx = (p <= M0 / 2) ? p : M0 - p;
+      const double g = ghat[((int64_t)gx * qy + gy) * H2 + k0 + l];
+      re[l * ls + p] *= g;
+      im[l * ls + p] *= g;
+    }
+    __syncthreads();
+    lds_fft_dif(re, im, n, log2n, nl, tw, nmax, +1.0, ls);
+  }
+  for (int idx = threadIdx.x; idx < nl * n_out; idx += blockDim.x) {
+    const int p = idx / nl, l = idx - p * nl;
+    const int pos = (MODE == FWD) ? p : bitrev(p, log2n);
+    base[(int64_t)p * line_stride + l] = make_double2(re[l * ls + pos], im[l * ls + pos]);
+  }
+}
+
+size_t zpass_lds(int M2) { return sizeof(double) * 2 * kZPairs * (M2 + 1); }
+// Lines per block for the strided passes: up to kTileLines, within 144 KiB of LDS.
+int tile_lines(int n) {
+  int t = (144 * 1024) / (int)(sizeof(double) * 2 * (n + 1));
+  return t < 1 ? 1 : (t > kTileLines ? kTileLines : t);
+}
+size_t strided_lds(int n) { return sizeof(double) * 2 * tile_lines(n) * (n + 1); }
+constexpr int kMaxDynLDS = 160 * 1024;
+
+}  // namespace
+}  // namespace ob
+
+struct ob_sc3d {
+  int nx, ny, nz, M0, M1, M2, H2, qx, qy, nmax;
+  double ratio_limit = 1.2, xy_ratio = 1.0, xz_ratio = 1.0;
+  double sMin[3], sMax[3], sd[3];  // solver grid limits and steps (PoissonSolver3D.cc)
+  double g[9];                     // rho/phi grid: xMin,xMax,yMin,yMax,zMin,zMax (rest), dx,dy,dz
+  int rebuilds = 0;
+  double *rho = nullptr, *phi = nullptr, *ghat = nullptr, *partials = nullptr, *ext_dev = nullptr;
+  double* ext_host = nullptr;
+  double2 *spec = nullptr, *tw = nullptr;
+};
+
+namespace ob {
+namespace {
+
+void launch_strided(ob_sc3d* c, int mode, int n, int n_in, int n_out, int64_t line_stride,
+                    int64_t outer_stride, int outers) {
+  const int tile = tile_lines(n);
+  const int tiles = (c->H2 + tile - 1) / tile;
+  const dim3 grid(outers * tiles);
+  const size_t lds = strided_lds(n);
+  hipStream_t s = stream();
+  switch (mode) {
+    case FWD:
+      {
+        ob::ProfScope ps_("fft3_strided_fwd");
+        hipLaunchKernelGGL(fft3_strided<FWD>, grid, dim3(kThreads), lds, s, c->spec, n, n_in, n_out,
+                           line_stride, outer_stride, c->H2, c->ghat, c->qy, c->M0, c->M1, c->tw,
+                           c->nmax, tile);
+      }
+      break;
+    case INV:
+      {
+        ob::ProfScope ps_("fft3_strided_inv");
+        hipLaunchKernelGGL(fft3_strided<INV>, grid, dim3(kThreads), lds, s, c->spec, n, n_in, n_out,
+                           line_stride, outer_stride, c->H2, c->ghat, c->qy, c->M0, c->M1, c->tw,
+                           c->nmax, tile);
+      }
+      break;
+    case CONV:
+      {
+        ob::ProfScope ps_("fft3_strided_conv");
+        hipLaunchKernelGGL(fft3_strided<CONV>, grid, dim3(kThreads), lds, s, c->spec, n, n_in, n_out,
+                           line_stride, outer_stride, c->H2, c->ghat, c->qy, c->M0, c->M1, c->tw,
+                           c->nmax, tile);
+      }
+      break;
+    default:
+      {
+        ob::ProfScope ps_("fft3_strided_greenout");
+        hipLaunchKernelGGL(fft3_strided<GREEN_OUT>, grid, dim3(kThreads), lds, s, c->spec, n, n_in,
+                           n_out, line_stride, outer_stride, c->H2, c->ghat, c->qy, c->M0, c->M1,
+                           c->tw, c->nmax, tile);
+      }
+  }
+  OB_HIP(hipGetLastError());
+}
+
+// PoissonSolverFFT3D::setGridXYZ + _defineGreenF (:136-247)
+void set_solver_grid(ob_sc3d* c, double xMin, double xMax, double yMin, double yMax, double zMin,
+                     double zMax) {
+  c->sMin[0] = xMin; c->sMax[0] = xMax; c->sd[0] = (xMax - xMin) / (c->nx - 1);
+  c->sMin[1] = yMin; c->sMax[1] = yMax; c->sd[1] = (yMax - yMin) / (c->ny - 1);
+  c->sMin[2] = zMin; c->sMax[2] = zMax; c->sd[2] = (zMax - zMin) / c->nz;
+  GreenGeom gg{c->M0, c->M1, c->M2, c->sd[0], c->sd[1], c->sd[2]};
+  hipStream_t s = stream();
+  const int64_t nlines = (int64_t)c->M0 * c->M1;
+  const int64_t blocks = (nlines / 2 + kZPairs - 1) / kZPairs;
+  {
+    ob::ProfScope ps_("fft3_z_r2c_green");
+    hipLaunchKernelGGL(fft3_z_r2c<true>, dim3((unsigned)blocks), dim3(kThreads), zpass_lds(c->M2),
+                       s, nullptr, nlines, c->M1, c->M2, c->M2, c->M1, c->spec, c->tw, c->nmax, gg);
+  }
+  OB_HIP(hipGetLastError());
+  const int64_t H2 = c->H2;
+  launch_strided(c, FWD, c->M1, c->M1, c->M1, H2, (int64_t)c->M1 * H2, c->M0);
+  launch_strided(c, GREEN_OUT, c->M0, c->M0, 0, (int64_t)c->M1 * H2, H2, c->M1 / 2 + 1);
+  c->rebuilds++;
+}
+
+void solve(ob_sc3d* c, double denom) {
+  hipStream_t s = stream();
+  const int64_t nl = (int64_t)c->nx * c->ny;
+  const int64_t blocks = ((nl + 1) / 2 + kZPairs - 1) / kZPairs;
+  GreenGeom gg{};
+  {
+    ob::ProfScope ps_("fft3_z_r2c");
+    hipLaunchKernelGGL(fft3_z_r2c<false>, dim3((unsigned)blocks), dim3(kThreads), zpass_lds(c->M2),
+                       s, c->rho, nl, c->ny, c->nz, c->M2, c->M1, c->spec, c->tw, c->nmax, gg);
+  }
+  OB_HIP(hipGetLastError());
+  const int64_t H2 = c->H2;
+  launch_strided(c, FWD, c->M1, c->ny, c->M1, H2, (int64_t)c->M1 * H2, c->nx);
+  launch_strided(c, CONV, c->M0, c->nx, c->nx, (int64_t)c->M1 * H2, H2, c->M1);
+  launch_strided(c, INV, c->M1, c->M1, c->ny, H2, (int64_t)c->M1 * H2, c->nx);
+  {
+    ob::ProfScope ps_("fft3_z_c2r");
+    hipLaunchKernelGGL(fft3_z_c2r, dim3((unsigned)blocks), dim3(kThreads), zpass_lds(c->M2), s,
+                       c->spec, nl, c->ny, c->M2, c->M1, c->nz, denom, c->phi, c->tw, c->nmax);
+  }
+  OB_HIP(hipGetLastError());
+}
+
+}  // namespace
+}  // namespace ob
+
+#define OB_API extern "C" __attribute__((visibility("default")))
+#define OB_TRY try {
+#define OB_CATCH               \
+  }                            \
+  catch (const ob::Error& e) { \
+    ob::set_error(e.msg);      \
+    return 1;                  \
+  }                            \
+  return 0;
+
+OB_API int ob_sc3d_create(int nx, int ny, int nz, ob_sc3d** out) {
+  OB_TRY OB_REQUIRE(out != nullptr, "null output pointer");
+  OB_REQUIRE(nx >= 3 && ny >= 3, "SpaceChargeCalc3D: grid size too small (should be >= 3)");
+  OB_REQUIRE(nz >= 1, "SpaceChargeCalc3D: number of z bins should be >= 1");
+  ob::ensure_init();
+  ob_sc3d* c = new ob_sc3d();
+  c->nx = nx; c->ny = ny; c->nz = nz;
+  c->M0 = ob::pad_len(nx);
+  c->M1 = ob::pad_len(ny);
+  c->M2 = ob::pad_len(nz);
+  if (c->M0 > 2048 || c->M1 > 2048 || c->M2 > 2048) {
+    delete c;
+    throw ob::Error{"SpaceChargeCalc3D: grid larger than 1024 points per axis"};
+  }
+  c->H2 = c->M2 / 2 + 1;
+  c->qx = c->M0 / 2 + 1;
+  c->qy = c->M1 / 2 + 1;
+  c->nmax = std::max(c->M0, std::max(c->M1, c->M2));
+  const size_t ng = (size_t)nx * ny * nz;
+  OB_HIP(hipMalloc((void**)&c->rho, sizeof(double) * ng));
+  OB_HIP(hipMalloc((void**)&c->phi, sizeof(double) * ng));
+  OB_HIP(hipMalloc((void**)&c->spec, sizeof(double2) * (size_t)c->M0 * c->M1 * c->H2));
+  OB_HIP(hipMalloc((void**)&c->ghat, sizeof(double) * (size_t)c->qx * c->qy * c->H2));
+  OB_HIP(hipMalloc((void**)&c->partials, sizeof(double) * 6 * 4096));
+  OB_HIP(hipMalloc((void**)&c->ext_dev, sizeof(double) * 6));
+  OB_HIP(hipHostMalloc((void**)&c->ext_host, sizeof(double) * 6, hipHostMallocDefault));
+  OB_HIP(hipMemset(c->rho, 0, sizeof(double) * ng));
+  OB_HIP(hipMemset(c->phi, 0, sizeof(double) * ng));
+  c->tw = ob::make_twiddles(c->nmax);
+  OB_HIP(hipFuncSetAttribute((const void*)ob::fft3_strided<ob::FWD>,
+                             hipFuncAttributeMaxDynamicSharedMemorySize, ob::kMaxDynLDS));
+  OB_HIP(hipFuncSetAttribute((const void*)ob::fft3_strided<ob::INV>,
+                             hipFuncAttributeMaxDynamicSharedMemorySize, ob::kMaxDynLDS));
+  OB_HIP(hipFuncSetAttribute((const void*)ob::fft3_strided<ob::CONV>,
+                             hipFuncAttributeMaxDynamicSharedMemorySize, ob::kMaxDynLDS));
+  OB_HIP(hipFuncSetAttribute((const void*)ob::fft3_strided<ob::GREEN_OUT>,
+                             hipFuncAttributeMaxDynamicSharedMemorySize, ob::kMaxDynLDS));
+  OB_HIP(hipFuncSetAttribute((const void*)ob::fft3_z_r2c<true>,
+                             hipFuncAttributeMaxDynamicSharedMemorySize, ob::kMaxDynLDS));
+  OB_HIP(hipFuncSetAttribute((const void*)ob::fft3_z_r2c<false>,
+                             hipFuncAttributeMaxDynamicSharedMemorySize, ob::kMaxDynLDS));
+  OB_HIP(hipFuncSetAttribute((const void*)ob::fft3_z_c2r,
+                             hipFuncAttributeMaxDynamicSharedMemorySize, ob::kMaxDynLDS));
+  // SpaceChargeCalc3D(:32-44): solver on [-1,1]^3 (xy_ratio = xz_ratio = 1); rho/phi grids
+  // take the solver limits.
+  ob::set_solver_grid(c, -1.0, 1.0, -1.0, 1.0, -1.0, 1.0);
+  c->rebuilds = 0;
+  for (int k = 0; k < 3; k++) {
+    c->g[2 * k] = c->sMin[k];
+    c->g[2 * k + 1] = c->sMax[k];
+    c->g[6 + k] = c->sd[k];
+  }
+  OB_HIP(hipStreamSynchronize(ob::stream()));
+  *out = c;
+  OB_CATCH
+}
+
+OB_API int ob_sc3d_destroy(ob_sc3d* c) {
+  OB_TRY if (!c) return 0;
+  (void)hipStreamSynchronize(ob::stream());
+  (void)hipFree(c->rho); (void)hipFree(c->phi); (void)hipFree(c->spec); (void)hipFree(c->ghat);
+  (void)hipFree(c->partials); (void)hipFree(c->ext_dev); (void)hipFree(c->tw);
+  (void)hipHostFree(c->ext_host);
+  delete c;
+  OB_CATCH
+}
+
+OB_API int ob_sc3d_track(ob_sc3d* c, ob_bunch* b, double length) {
+  OB_TRY OB_REQUIRE(c != nullptr && b != nullptr, "null handle");
+  ob::check_statuses(false);
+  const int64_t n = ob::bunch_size(b);
+  if (n < 2) return 0;  // :102-103
+  hipStream_t s = ob::stream();
+  const int nblk = ob::bunch_flush(b, c->partials);
+  {
+    ob::ProfScope ps_("reduce_extrema");
+    hipLaunchKernelGGL(ob::reduce_extrema, dim3(1), dim3(1024), 0, s, c->partials, nblk, c->ext_dev);
+  }
+  OB_HIP(hipGetLastError());
+  OB_HIP(hipMemcpyAsync(c->ext_host, c->ext_dev, sizeof(double) * 6, hipMemcpyDeviceToHost, s));
+  OB_HIP(hipStreamSynchronize(s));
+  const ob::SyncInfo sy = ob::bunch_sync(b);
+  const double gamma = sy.gamma, beta = sy.beta;
+  // bunchAnalysis (:150-249), scalar arithmetic as in the reference
+  double xMin = c->ext_host[0], xMax = c->ext_host[1], yMin = c->ext_host[2];
+  double yMax = c->ext_host[3], zMin = c->ext_host[4], zMax = c->ext_host[5];
+  double center = (zMax + zMin) / 2.0;
+  double width = (zMax - zMin) * gamma / 2.0;
+  zMin = center - width;
+  zMax = center + width;
+  OB_REQUIRE(!(xMin >= xMax || yMin >= yMax || zMin >= zMax),
+             "SpaceChargeCalc3D::bunchAnalysis: the bunch min and max sizes are wrong (zero "
+             "extent); cannot calculate space charge");
+  const double xy_ratio_beam = (xMax - xMin) / (yMax - yMin);
+  const double xz_ratio_beam = (xMax - xMin) / (zMax - zMin);
+  if (c->ratio_limit < xy_ratio_beam / c->xy_ratio || 1.0 / c->ratio_limit > xy_ratio_beam / c->xy_ratio ||
+      c->ratio_limit < xz_ratio_beam / c->xz_ratio || 1.0 / c->ratio_limit > xz_ratio_beam / c->xz_ratio) {
+    c->xy_ratio = xy_ratio_beam;
+    c->xz_ratio = xz_ratio_beam;
+    ob::set_solver_grid(c, -(xMax - xMin) / 2.0, (xMax - xMin) / 2.0, -(yMax - yMin) / 2.0,
+                        (yMax - yMin) / 2.0, -(zMax - zMin) / 2.0, (zMax - zMin) / 2.0);
+  }
+  const double scx = (xMax - xMin) / (c->sMax[0] - c->sMin[0]);
+  const double scy = (yMax - yMin) / (c->sMax[1] - c->sMin[1]);
+  const double scz = (zMax - zMin) / (c->sMax[2] - c->sMin[2]);
+  double sc = scx;
+  if (sc < scy) sc = scy;
+  if (sc < scz) sc = scz;
+  ob::Grid3 g;
+  g.nx = c->nx; g.ny = c->ny; g.nz = c->nz;
+  center = (xMax + xMin) / 2.0;
+  width = (c->sMax[0] - c->sMin[0]) * sc / 2.0;
+  g.xMin = center - width; g.xMax = center + width; g.dx = (g.xMax - g.xMin) / (c->nx - 1);
+  center = (yMax + yMin) / 2.0;
+  width = (c->sMax[1] - c->sMin[1]) * sc / 2.0;
+  g.yMin = center - width; g.yMax = center + width; g.dy = (g.yMax - g.yMin) / (c->ny - 1);
+  center = (zMax + zMin) / 2.0;
+  width = (c->sMax[2] - c->sMin[2]) * sc / (2.0 * gamma);
+  g.zMin = center - width; g.zMax = center + width; g.dz = (g.zMax - g.zMin) / c->nz;
+  // deposit in lab z
+  const int64_t ng = (int64_t)c->nx * c->ny * c->nz;
+  {
+    ob::ProfScope ps_("fill_zero");
+    hipLaunchKernelGGL(ob::fill_zero, dim3(1024), dim3(256), 0, s, c->rho, ng);
+  }
+  ob::DeviceBunchView v = ob::bunch_device_view(b);
+  int dblk = (int)((n + ob::kThreads - 1) / ob::kThreads);
+  const int cap = 16 * ob::device_cu_count();
+  if (dblk > cap) dblk = cap;
+  {
+    ob::ProfScope ps_("deposit3d");
+    hipLaunchKernelGGL(ob::deposit3d, dim3(dblk), dim3(ob::kThreads), 0, s, v, g, sy.macrosize,
+                       c->rho);
+  }
+  OB_HIP(hipGetLastError());
+  // move the z grid to the rest frame (:242-248)
+  center = (g.zMax + g.zMin) / 2.0;
+  width = (g.zMax - g.zMin) * gamma / 2.0;
+  const double rzMin = center - width, rzMax = center + width;
+  const double rdz = (rzMax - rzMin) / c->nz;
+  c->g[0] = g.xMin; c->g[1] = g.xMax; c->g[2] = g.yMin; c->g[3] = g.yMax;
+  c->g[4] = rzMin; c->g[5] = rzMax; c->g[6] = g.dx; c->g[7] = g.dy; c->g[8] = rdz;
+  // findPotential (PoissonSolverFFT3D.cc:249-347)
+  const double scale_coeff = c->sd[0] / g.dx;
+  const double denom = scale_coeff / ((double)c->M0 * c->M1 * c->M2);
+  ob::solve(c, denom);
+  // kick (:120-147), fused into the next map kernel
+  ob::Op op;
+  std::memset(&op, 0, sizeof(op));
+  op.kind = ob::OP_SC3D_KICK;
+  op.i0 = c->nx; op.i1 = c->ny; op.i2 = c->nz;
+  op.p[0] = ob::ptr_bits(c->phi);
+  op.p[1] = g.xMin; op.p[2] = g.dx; op.p[3] = g.yMin; op.p[4] = g.dy;
+  op.p[5] = rzMin; op.p[6] = rdz;
+  op.p[7] = (rzMax + rzMin) / 2.0;                                   // :125
+  op.p[8] = gamma;
+  op.p[9] = length * sy.r0 / ((beta * beta) * (gamma * gamma));     // :121
+  op.p[10] = length * sy.r0 * sy.mass;                               // :122
+  ob::bunch_queue(b, op);
+  OB_CATCH
+}
+
+OB_API int ob_sc3d_get_grid(ob_sc3d* c, int which, double* out, double* lims) {
+  OB_TRY OB_REQUIRE(c != nullptr, "null handle");
+  OB_REQUIRE(which == 0 || which == 1, "grid selector must be 0 (rho) or 1 (phi)");
+  const size_t ng = (size_t)c->nx * c->ny * c->nz;
+  if (out) {
+    std::vector<double> tmp(ng);
+    OB_HIP(hipMemcpyAsync(tmp.data(), which == 0 ? c->rho : c->phi, sizeof(double) * ng,
+                          hipMemcpyDeviceToHost, ob::stream()));
+    OB_HIP(hipStreamSynchronize(ob::stream()));
+    // device [ix][iy][iz] -> reference Arr3D[iz][ix][iy]
+    for (int ix = 0; ix < c->nx; ix++)
+      for (int iy = 0; iy < c->ny; iy++)
+        for (int iz = 0; iz < c->nz; iz++)
+          out[((size_t)iz * c->nx + ix) * c->ny + iy] = tmp[((size_t)ix * c->ny + iy) * c->nz + iz];
+  }
+  if (lims)
+    for (int k = 0; k < 6; k++) lims[k] = c->g[k];
+  OB_CATCH
+}
+
+OB_API int ob_sc3d_set_ratio_limit(ob_sc3d* c, double r) {
+  OB_TRY OB_REQUIRE(c != nullptr, "null handle");
+  c->ratio_limit = r;
+  OB_CATCH
+}
+OB_API int ob_sc3d_get_ratio_limit(ob_sc3d* c, double* r) {
+  OB_TRY OB_REQUIRE(c != nullptr, "null handle");
+  *r = c->ratio_limit;
+  OB_CATCH
+}
+OB_API int ob_sc3d_green_rebuilds(ob_sc3d* c, int* n) {
+  OB_TRY OB_REQUIRE(c != nullptr, "null handle");
+  *n = c->rebuilds;
+  OB_CATCH
+}

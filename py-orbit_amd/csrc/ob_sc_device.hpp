@@ -1,0 +1,241 @@
+// ob_sc_device.hpp -- device-side grid index/weight rules and the space-charge kicks.
+//
+// The index and fraction rules reproduce Grid2D.cc:337-350 (node-centred x,y with the
+// [1, n-2] clamp), Grid1D.cc:568-626 (cell-centred, periodic at both ends) and
+// Grid3D.cc:226-264 (cell-centred z clamped to [1, nz-2]).  With -ffp-contract=off the
+// device computes the same bits as the reference on x86-64: bin indices are exact.
+#pragma once
+
+#include <cfloat>
+
+#include "ob_internal.hpp"
+
+namespace ob {
+
+template <typename T>
+__device__ __forceinline__ T op_ptr(double d) {
+  return reinterpret_cast<T>(__double_as_longlong(d));
+}
+
+// Grid2D::getIndAndFracX/Y (Grid2D.cc:337-350); Grid3D.cc:212-224 is identical.
+__device__ __forceinline__ void node_ind_frac(double v, double vmin, double d, int n, int& ind,
+                                              double& frac) {
+  int i = (int)((v - vmin) / d + 0.5);
+  if (i < 1) i = 1;
+  if (i > (n - 2)) i = n - 2;
+  ind = i;
+  frac = (v - (vmin + i * d)) / d;
+}
+
+// Grid1D::getIndAndWZ (Grid1D.cc:568-626), quirk: periodic wrap at both ends.
+__device__ __forceinline__ void grid1d_ind_wz(double z, double zMin, double dz, int nz, int& i0,
+                                              int& ip, double& w0, double& wp) {
+  if (nz > 1) {
+    const double ind_dbl = ((z - zMin) / dz) - 0.5;
+    int iZ0 = (int)ind_dbl;
+    // In range for every z in [zMin, zMax]; the clamp only keeps a NaN coordinate (which
+    // the reference aborts on, Grid1D.cc:578-580) from addressing outside the grid.
+    iZ0 = iZ0 < 0 ? 0 : (iZ0 > nz - 1 ? nz - 1 : iZ0);
+    const double zFrac = (z - (zMin + (iZ0 + 0.5) * dz)) / dz;
+    if (iZ0 == 0 && zFrac < 0.) {
+      i0 = nz - 1; ip = 0; w0 = -zFrac; wp = 1.0 + zFrac;
+      return;
+    }
+    if (iZ0 == nz - 1) {
+      if (zFrac > 0.) {
+        i0 = iZ0; ip = 0; w0 = 1.0 - zFrac; wp = zFrac;
+      } else {
+        ip = iZ0; i0 = nz - 2; w0 = -zFrac; wp = 1.0 + zFrac;
+      }
+      return;
+    }
+    i0 = iZ0; ip = iZ0 + 1; w0 = 1.0 - zFrac; wp = zFrac;
+  } else {
+    i0 = 0; ip = 0; w0 = 1.0; wp = 0.0;
+  }
+}
+
+// Grid1D::getValue (Grid1D.cc:191-210): zero outside [zMin, zMax].
+__device__ __forceinline__ double grid1d_value(const double* __restrict__ arr, double z,
+                                               double zMin, double zMax, double dz, int nz) {
+  if (z < zMin || z > zMax) return 0.;
+  int i0, ip;
+  double w0, wp;
+  grid1d_ind_wz(z, zMin, dz, nz, i0, ip, w0, wp);
+  if (nz > 1) return w0 * arr[i0] + wp * arr[ip];
+  return arr[0];
+}
+
+// Grid2D::calcGradient(x, y) (Grid2D.cc:212-256) on a row-major [nx][ny] array.
+__device__ __forceinline__ void grid2d_gradient(const double* __restrict__ a, int nx, int ny,
+                                                double xMin, double dx, double yMin, double dy,
+                                                double x, double y, double& ex, double& ey) {
+  int iX, iY;
+  double xF, yF;
+  node_ind_frac(x, xMin, dx, nx, iX, xF);
+  node_ind_frac(y, yMin, dy, ny, iY, yF);
+  const double xF2 = xF * xF, yF2 = yF * yF;
+  const double Wxm = 0.5 * (0.25 - xF + xF2), Wx0 = 0.75 - xF2, Wxp = 0.5 * (0.25 + xF + xF2);
+  const double Wym = 0.5 * (0.25 - yF + yF2), Wy0 = 0.75 - yF2, Wyp = 0.5 * (0.25 + yF + yF2);
+  const double dWxm = (-1.0) * (0.5 - xF), dWx0 = (-1.0) * (+2.) * xF,
+               dWxp = (-1.0) * (-(0.5 + xF));
+  const double dWym = (-1.0) * (0.5 - yF), dWy0 = (-1.0) * (+2.) * yF,
+               dWyp = (-1.0) * (-(0.5 + yF));
+  const double* r0 = a + (int64_t)(iX - 1) * ny + iY;
+  const double* r1 = a + (int64_t)iX * ny + iY;
+  const double* r2 = a + (int64_t)(iX + 1) * ny + iY;
+  const double a00 = r0[-1], a01 = r0[0], a02 = r0[1];
+  const double a10 = r1[-1], a11 = r1[0], a12 = r1[1];
+  const double a20 = r2[-1], a21 = r2[0], a22 = r2[1];
+  double e = dWxm * Wym * a00 + dWxm * Wy0 * a01 + dWxm * Wyp * a02 + dWx0 * Wym * a10 +
+             dWx0 * Wy0 * a11 + dWx0 * Wyp * a12 + dWxp * Wym * a20 + dWxp * Wy0 * a21 +
+             dWxp * Wyp * a22;
+  ex = e / dx;
+  e = Wxm * dWym * a00 + Wxm * dWy0 * a01 + Wxm * dWyp * a02 + Wx0 * dWym * a10 +
+      Wx0 * dWy0 * a11 + Wx0 * dWyp * a12 + Wxp * dWym * a20 + Wxp * dWy0 * a21 +
+      Wxp * dWyp * a22;
+  ey = e / dy;
+}
+
+// Grid3D::getGridIndAndFrac z part (Grid3D.cc:236-263), longWrapping = 0.
+__device__ __forceinline__ void grid3d_z_ind_frac(double z, double zMin, double dz, int nz,
+                                                  int& zi, double& zf) {
+  if (nz > 1) {
+    zi = (int)((z - zMin) / dz);
+    zf = 0.0;
+    if (nz > 2) {
+      if (zi <= 0) zi = 1;
+      if (zi >= nz - 1) zi = nz - 2;
+      zf = (z - (zMin + (zi + 0.5) * dz)) / dz;
+    }
+    if (nz == 2) {
+      zf = (z - (zMin + 0.5 * dz)) / dz;
+      if (zi < 0) { zi = 0; zf = 0.0; }
+      else if (zi > 1) { zi = 1; zf = 1.0; }
+    }
+  } else {
+    zi = 0;
+    zf = 0.0;
+  }
+}
+
+// Grid3D::calcSheetGradient (Grid3D.cc:718-733).  The device keeps 3D grids z-fastest,
+// a[(ix*ny + iy)*nz + iz] (the layout of the reference's FFT buffer, PoissonSolverFFT3D.cc:
+// 305-309); the reference's Arr3D[iz][ix][iy] order is restored only when copied out.
+__device__ __forceinline__ double sheet_grad(const double* __restrict__ a, int nx, int ny, int nz,
+                                             int iZ, int iX, int iY, double xm, double x0,
+                                             double xp, double ym, double y0, double yp) {
+  const int64_t sx = (int64_t)ny * nz, sy = nz;
+  const double* s = a + (int64_t)iX * sx + (int64_t)iY * sy + iZ;
+  return xm * ym * s[-sx - sy] + x0 * ym * s[-sy] + xp * ym * s[sx - sy] + xm * y0 * s[-sx] +
+         x0 * y0 * s[0] + xp * y0 * s[sx] + xm * yp * s[-sx + sy] + x0 * yp * s[sy] +
+         xp * yp * s[sx + sy];
+}
+
+// Grid3D::calcGradient (Grid3D.cc:438-532).
+__device__ __forceinline__ void grid3d_gradient(const double* __restrict__ a, int nx, int ny,
+                                                int nz, double xMin, double dx, double yMin,
+                                                double dy, double zMin, double dz, double x,
+                                                double y, double z, double& gX, double& gY,
+                                                double& gZ) {
+  int iX, iY, iZ;
+  double xF, yF, zF;
+  node_ind_frac(x, xMin, dx, nx, iX, xF);
+  node_ind_frac(y, yMin, dy, ny, iY, yF);
+  grid3d_z_ind_frac(z, zMin, dz, nz, iZ, zF);
+  const double Wxm = 0.5 * (0.5 - xF) * (0.5 - xF), Wx0 = 0.75 - xF * xF,
+               Wxp = 0.5 * (0.5 + xF) * (0.5 + xF);
+  const double Wym = 0.5 * (0.5 - yF) * (0.5 - yF), Wy0 = 0.75 - yF * yF,
+               Wyp = 0.5 * (0.5 + yF) * (0.5 + yF);
+  double Wzm = 0.0, Wz0 = 0.0, Wzp = 0.0;
+  if (nz >= 3) {
+    Wzm = 0.5 * (0.5 - zF) * (0.5 - zF);
+    Wz0 = 0.75 - zF * zF;
+    Wzp = 0.5 * (0.5 + zF) * (0.5 + zF);
+  } else if (nz == 2) {
+    Wzm = 1.0 - zF; Wz0 = 0.0; Wzp = zF;
+  }
+  const double dWxm = (-1.0) * (0.5 - xF) / dx, dWx0 = (-1.0) * 2. * xF / dx,
+               dWxp = (-1.0) * -(0.5 + xF) / dx;
+  const double dWym = (-1.0) * (0.5 - yF) / dy, dWy0 = (-1.0) * 2. * yF / dy,
+               dWyp = (-1.0) * -(0.5 + yF) / dy;
+  double dWzm = 0.0, dWz0 = 0.0, dWzp = 0.0;
+  if (nz >= 3) {
+    dWzm = (-1.0) * (0.5 - zF) / dz;
+    dWz0 = (-1.0) * 2. * zF / dz;
+    dWzp = (-1.0) * -(0.5 + zF) / dz;
+  } else if (nz == 2) {
+    dWzm = (-1.0) * 1.0 / dz;
+    dWz0 = 0.0;
+    dWzp = (-1.0) * -1.0 / dz;
+  }
+  int iZm = iZ - 1, iZp = iZ + 1;
+  if (iZm < 0) iZm = nz - 1;
+  if (iZp >= nz) iZp = 0;
+  if (nz >= 3) {
+    gX = sheet_grad(a, nx, ny, nz, iZm, iX, iY, dWxm, dWx0, dWxp, Wym, Wy0, Wyp) * Wzm +
+         sheet_grad(a, nx, ny, nz, iZ, iX, iY, dWxm, dWx0, dWxp, Wym, Wy0, Wyp) * Wz0 +
+         sheet_grad(a, nx, ny, nz, iZp, iX, iY, dWxm, dWx0, dWxp, Wym, Wy0, Wyp) * Wzp;
+    gY = sheet_grad(a, nx, ny, nz, iZm, iX, iY, Wxm, Wx0, Wxp, dWym, dWy0, dWyp) * Wzm +
+         sheet_grad(a, nx, ny, nz, iZ, iX, iY, Wxm, Wx0, Wxp, dWym, dWy0, dWyp) * Wz0 +
+         sheet_grad(a, nx, ny, nz, iZp, iX, iY, Wxm, Wx0, Wxp, dWym, dWy0, dWyp) * Wzp;
+    gZ = sheet_grad(a, nx, ny, nz, iZm, iX, iY, Wxm, Wx0, Wxp, Wym, Wy0, Wyp) * dWzm +
+         sheet_grad(a, nx, ny, nz, iZ, iX, iY, Wxm, Wx0, Wxp, Wym, Wy0, Wyp) * dWz0 +
+         sheet_grad(a, nx, ny, nz, iZp, iX, iY, Wxm, Wx0, Wxp, Wym, Wy0, Wyp) * dWzp;
+  } else if (nz == 2) {
+    gX = sheet_grad(a, nx, ny, nz, 0, iX, iY, dWxm, dWx0, dWxp, Wym, Wy0, Wyp) * Wzm +
+         sheet_grad(a, nx, ny, nz, 1, iX, iY, dWxm, dWx0, dWxp, Wym, Wy0, Wyp) * Wzp;
+    gY = sheet_grad(a, nx, ny, nz, 0, iX, iY, Wxm, Wx0, Wxp, dWym, dWy0, dWyp) * Wzm +
+         sheet_grad(a, nx, ny, nz, 1, iX, iY, Wxm, Wx0, Wxp, dWym, dWy0, dWyp) * Wzp;
+    gZ = sheet_grad(a, nx, ny, nz, 0, iX, iY, Wxm, Wx0, Wxp, Wym, Wy0, Wyp) * dWzm +
+         sheet_grad(a, nx, ny, nz, 1, iX, iY, Wxm, Wx0, Wxp, Wym, Wy0, Wyp) * dWzp;
+  } else {
+    gX = sheet_grad(a, nx, ny, nz, 0, iX, iY, dWxm, dWx0, dWxp, Wym, Wy0, Wyp);
+    gY = sheet_grad(a, nx, ny, nz, 0, iX, iY, Wxm, Wx0, Wxp, dWym, dWy0, dWyp);
+    gZ = 0.0;
+  }
+}
+
+// SpaceChargeCalc2p5Drb kick loop body (SpaceChargeCalc2p5Drb.cc:146-167).
+// op: p[0]=SC2DState*, p[1]=phi, p[2]=z line density, p[3]=its derivative,
+//     p[4]=2*log(pipe_radius), p[5]=-L*r0*m/gamma^2; i0=nx, i1=ny, i2=nz.
+__device__ __forceinline__ void sc2p5drb_kick(const Op& op, double x, double& xp, double y,
+                                              double& yp, double z, double& dE) {
+  const SC2DState* st = op_ptr<const SC2DState*>(op.p[0]);
+  if (st->skip) return;
+  const double xc = st->x_c, yc = st->y_c;
+  const double r2 = (x - xc) * (x - xc) + (y - yc) * (y - yc);
+  double ex, ey;
+  grid2d_gradient(op_ptr<const double*>(op.p[1]), op.i0, op.i1, st->xMin, st->dx, st->yMin,
+                  st->dy, x, y, ex, ey);
+  const double ez =
+      grid1d_value(op_ptr<const double*>(op.p[3]), z, st->zMin, st->zMax, st->dz, op.i2);
+  const double Lfactor =
+      -grid1d_value(op_ptr<const double*>(op.p[2]), z, st->zMin, st->zMax, st->dz, op.i2) *
+      st->factor;
+  xp += ex * Lfactor;
+  yp += ey * Lfactor;
+  double coeff;
+  if (r2 <= st->a2)
+    coeff = st->lsf_in - r2 / st->a2;
+  else
+    coeff = op.p[4] - log(r2);
+  dE += ez * op.p[5] * coeff;
+}
+
+// SpaceChargeCalc3D kick loop body (SpaceChargeCalc3D.cc:132-147).
+// op: p[0]=phi (z-fastest), p[1..6]=xMin,dx,yMin,dy,zMin,dz of the rest-frame phi grid,
+//     p[7]=z_center, p[8]=gamma, p[9]=trans_factor, p[10]=long_factor; i0=nx, i1=ny, i2=nz.
+__device__ __forceinline__ void sc3d_kick(const Op& op, double x, double& xp, double y,
+                                          double& yp, double z, double& dE) {
+  const double zc = op.p[7], gamma = op.p[8];
+  const double zr = (z - zc) * gamma + zc;
+  double ex, ey, ez;
+  grid3d_gradient(op_ptr<const double*>(op.p[0]), op.i0, op.i1, op.i2, op.p[1], op.p[2], op.p[3],
+                  op.p[4], op.p[5], op.p[6], x, y, zr, ex, ey, ez);
+  xp += -ex * op.p[9];
+  yp += -ey * op.p[9];
+  dE += -ez * op.p[10];
+}
+
+}  // namespace ob

@@ -1,0 +1,13 @@
+"""setSC2p5DrbAccNodes (reference: py/orbit/space_charge/sc2p5d/scLatticeModifications.py:32-47)."""
+from orbit.space_charge.sc2p5d.scAccNodes import SC2p5Drb_AccNode
+from orbit.space_charge.scLatticeModifications import setSC_General_AccNodes
+
+
+def setSC2p5DrbAccNodes(lattice, sc_path_length_min, space_charge_calculator, pipe_radius):
+    scNodes = setSC_General_AccNodes(lattice, sc_path_length_min, space_charge_calculator,
+                                     SC2p5Drb_AccNode)
+    for scNode in scNodes:
+        scNode.setName(scNode.getName() + "SC2p5Drb")
+        scNode.setPipeRadius(pipe_radius)
+    lattice.initialize()
+    return scNodes

@@ -1,0 +1,264 @@
+"""spacecharge -- space-charge calculators on the device.
+
+Mirrors the reference module's SpaceChargeCalc2p5Drb
+(src/spacecharge/wrap_spacechargecalc2p5d_rb.cc:38-51,189-198) and SpaceChargeCalc3D
+(wrap_spacechargecalc3d.cc:38-45,171-177).  The grids returned by getRhoGrid() etc. are
+host snapshots of the device grids taken at the call (the reference returns live views
+of its host arrays); their accessors follow Grid1D/Grid2D/Grid3D (wrap_grid*.cc).
+"""
+import ctypes as C
+
+import numpy as np
+
+import _obnative as N
+
+
+class Grid1D:
+    """Snapshot of a 1D cell-centred grid (src/spacecharge/Grid1D.cc)."""
+
+    def __init__(self, arr, zMin, zMax):
+        self._a = np.array(arr, dtype=np.float64)
+        self._zmin, self._zmax = float(zMin), float(zMax)
+
+    def getSizeZ(self):
+        return len(self._a)
+
+    def getMinZ(self):
+        return self._zmin
+
+    def getMaxZ(self):
+        return self._zmax
+
+    def getStepZ(self):
+        return (self._zmax - self._zmin) / len(self._a)
+
+    def getGridZ(self, i):
+        return self._zmin + (i + 0.5) * self.getStepZ()
+
+    def getValueOnGrid(self, i):
+        return float(self._a[i])
+
+    def getSum(self):
+        return float(self._a.sum())
+
+    def array(self):
+        return self._a.copy()
+
+
+class Grid2D:
+    """Snapshot of a 2D node-centred grid, arr[ix][iy] (src/spacecharge/Grid2D.cc)."""
+
+    def __init__(self, arr, xMin, xMax, yMin, yMax):
+        self._a = np.array(arr, dtype=np.float64)
+        self._l = (float(xMin), float(xMax), float(yMin), float(yMax))
+
+    def getSizeX(self):
+        return self._a.shape[0]
+
+    def getSizeY(self):
+        return self._a.shape[1]
+
+    def getMinX(self):
+        return self._l[0]
+
+    def getMaxX(self):
+        return self._l[1]
+
+    def getMinY(self):
+        return self._l[2]
+
+    def getMaxY(self):
+        return self._l[3]
+
+    def getStepX(self):
+        return (self._l[1] - self._l[0]) / (self.getSizeX() - 1)
+
+    def getStepY(self):
+        return (self._l[3] - self._l[2]) / (self.getSizeY() - 1)
+
+    def getGridX(self, i):
+        return self._l[0] + i * self.getStepX()
+
+    def getGridY(self, i):
+        return self._l[2] + i * self.getStepY()
+
+    def getValueOnGrid(self, ix, iy):
+        return float(self._a[ix, iy])
+
+    def getSum(self):
+        return float(self._a.sum())
+
+    def array(self):
+        return self._a.copy()
+
+
+class Grid3D:
+    """Snapshot of a 3D grid, arr[iz][ix][iy] (src/spacecharge/Grid3D.cc); z cell-centred."""
+
+    def __init__(self, arr, lims):
+        self._a = np.array(arr, dtype=np.float64)
+        self._l = tuple(float(v) for v in lims)
+
+    def getSizeX(self):
+        return self._a.shape[1]
+
+    def getSizeY(self):
+        return self._a.shape[2]
+
+    def getSizeZ(self):
+        return self._a.shape[0]
+
+    def getMinX(self):
+        return self._l[0]
+
+    def getMaxX(self):
+        return self._l[1]
+
+    def getMinY(self):
+        return self._l[2]
+
+    def getMaxY(self):
+        return self._l[3]
+
+    def getMinZ(self):
+        return self._l[4]
+
+    def getMaxZ(self):
+        return self._l[5]
+
+    def getStepX(self):
+        return (self._l[1] - self._l[0]) / (self.getSizeX() - 1)
+
+    def getStepY(self):
+        return (self._l[3] - self._l[2]) / (self.getSizeY() - 1)
+
+    def getStepZ(self):
+        return (self._l[5] - self._l[4]) / self.getSizeZ()
+
+    def getValueOnGrid(self, ix, iy, iz):
+        return float(self._a[iz, ix, iy])
+
+    def getSum(self):
+        return float(self._a.sum())
+
+    def array(self):
+        return self._a.copy()
+
+
+class SpaceChargeCalc2p5Drb:
+    """SpaceChargeCalc2p5Drb(nx, ny, nz[, xy_ratio]) (SpaceChargeCalc2p5Drb.cc:31-63)."""
+
+    def __init__(self, xSize, ySize, zSize, xy_ratio=1.0):
+        self._n = (int(xSize), int(ySize), int(zSize))
+        h = C.c_void_p()
+        N.call("ob_sc2p5drb_create", self._n[0], self._n[1], self._n[2], float(xy_ratio),
+               C.byref(h))
+        self._h = h
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h and N._lib is not None:
+            N._lib.ob_sc2p5drb_destroy(h)
+            self._h = None
+
+    def trackBunch(self, bunch, length, pipe_radius):
+        """trackBunch(bunch, length, pipe_radius) (:110-168)."""
+        bunch._push_attrs()
+        N.call("ob_sc2p5drb_track", self._h, bunch._h, float(length), float(pipe_radius))
+
+    def _grid(self, which):
+        nx, ny, nz = self._n
+        n = nx * ny if which <= 1 else nz
+        out = np.zeros(n)
+        lims = np.zeros(4)
+        N.call("ob_sc2p5drb_get_grid", self._h, which, N.dptr(out), N.dptr(lims))
+        return out, lims
+
+    def getRhoGrid(self):
+        a, l = self._grid(0)
+        return Grid2D(a.reshape(self._n[0], self._n[1]), *l)
+
+    def getPhiGrid(self):
+        a, l = self._grid(1)
+        return Grid2D(a.reshape(self._n[0], self._n[1]), *l)
+
+    def getLongGrid(self):
+        a, l = self._grid(2)
+        return Grid1D(a, l[0], l[1])
+
+    def getLongDerivativeGrid(self):
+        a, l = self._grid(3)
+        return Grid1D(a, l[0], l[1])
+
+    def setLongAveragingPointsN(self, n):
+        N.call("ob_sc2p5drb_set_long_avg_n", self._h, int(n))
+
+    def getLongAveragingPointsN(self):
+        v = C.c_int()
+        N.call("ob_sc2p5drb_get_long_avg_n", self._h, C.byref(v))
+        return int(v.value)
+
+    def getMoments(self):
+        """(x_center, y_center, a_bunch, totalMacrosize) of the last trackBunch."""
+        out = np.zeros(4)
+        N.call("ob_sc2p5drb_get_moments", self._h, N.dptr(out))
+        return tuple(float(v) for v in out)
+
+
+class SpaceChargeCalc3D:
+    """SpaceChargeCalc3D(nx, ny, nz) (SpaceChargeCalc3D.cc:32-58)."""
+
+    def __init__(self, xSize, ySize, zSize):
+        self._n = (int(xSize), int(ySize), int(zSize))
+        h = C.c_void_p()
+        N.call("ob_sc3d_create", self._n[0], self._n[1], self._n[2], C.byref(h))
+        self._h = h
+        self._freq = 402.5e6
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h and N._lib is not None:
+            N._lib.ob_sc3d_destroy(h)
+            self._h = None
+
+    def trackBunch(self, bunch, length):
+        """trackBunch(bunch, length) (:100-148)."""
+        bunch._push_attrs()
+        N.call("ob_sc3d_track", self._h, bunch._h, float(length))
+
+    def _grid(self, which):
+        nx, ny, nz = self._n
+        out = np.zeros(nx * ny * nz)
+        lims = np.zeros(6)
+        N.call("ob_sc3d_get_grid", self._h, which, N.dptr(out), N.dptr(lims))
+        return Grid3D(out.reshape(nz, nx, ny), lims)
+
+    def getRhoGrid(self):
+        return self._grid(0)
+
+    def getPhiGrid(self):
+        return self._grid(1)
+
+    def setRatioLimit(self, r):
+        N.call("ob_sc3d_set_ratio_limit", self._h, float(r))
+
+    def getRatioLimit(self):
+        v = C.c_double()
+        N.call("ob_sc3d_get_ratio_limit", self._h, C.byref(v))
+        return float(v.value)
+
+    def numExtBunches(self, n=None):
+        """Neighbour-bunch images (wrappedBunchAnalysis, :251-331) are not built: only 0."""
+        if n is not None and int(n) != 0:
+            raise NotImplementedError("SpaceChargeCalc3D: external bunches are not supported")
+        return 0
+
+    def freqOfBunches(self, f=None):
+        if f is not None:
+            self._freq = float(f)
+        return self._freq
+
+    def getGreenRebuilds(self):
+        v = C.c_int()
+        N.call("ob_sc3d_green_rebuilds", self._h, C.byref(v))
+        return int(v.value)
