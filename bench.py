@@ -125,11 +125,7 @@ def profile(N, lat, b, cells, steps):
     run_cells(lat, b, cells, 0, steps)
     N.synchronize()
     buf = C.create_string_buffer(1 << 16)
-    L = N.load()
-    L.ob_profile_report.argtypes = [C.c_char_p, C.c_int]
-    L.ob_profile_report.restype = C.c_int
-    if L.ob_profile_report(buf, len(buf)) != 0:
-        raise RuntimeError(L.ob_last_error().decode())
+    N.call("ob_profile_report", buf, len(buf))
     N.call("ob_profile_enable", 0)
     rows = {}
     for line in buf.value.decode().strip().splitlines():

@@ -130,6 +130,17 @@ int ob_sc3d_set_ratio_limit(ob_sc3d* c, double r);                     /* :334-3
 int ob_sc3d_get_ratio_limit(ob_sc3d* c, double* r);
 /* Number of Green-function rebuilds so far (ratio changes, :186-203). */
 int ob_sc3d_green_rebuilds(ob_sc3d* c, int* n);
+/* Deposit strategy (no reference counterpart; results agree to FP64 summation order):
+ * tiled != 0 sorts the device bunch by xy tile and accumulates each tile in LDS (default),
+ * tiled == 0 scatters in particle order with global atomics.  lds_budget_bytes bounds the
+ * per-tile LDS block (0 keeps the current value). */
+int ob_sc3d_set_deposit_mode(ob_sc3d* c, int tiled, int lds_budget_bytes);
+
+/* ---- kernel timing (HIP events on the library stream) ---- */
+/* Start (on != 0) or stop collecting per-kernel timings; clears earlier records. */
+int ob_profile_enable(int on);
+/* Lines "name count total_ms min_ms max_ms" for the collected launches. */
+int ob_profile_report(char* buf, int buflen);
 
 #ifdef __cplusplus
 }

@@ -64,6 +64,9 @@ SIGNATURES = {
     "ob_sc3d_set_ratio_limit": [_vp, _d],
     "ob_sc3d_get_ratio_limit": [_vp, _D],
     "ob_sc3d_green_rebuilds": [_vp, C.POINTER(_i)],
+    "ob_sc3d_set_deposit_mode": [_vp, _i, _i],
+    "ob_profile_enable": [_i],
+    "ob_profile_report": [C.c_char_p, _i],
 }
 _RET = {"ob_last_error": C.c_char_p, "ob_version": C.c_char_p}
 

@@ -146,6 +146,14 @@ struct ob_bunch {
   int64_t n = 0, cap = 0;
   double* d = nullptr;        // 6 * cap doubles (SoA)
   double* d_msize = nullptr;  // per-particle macrosize (cap) or null
+  // Space-charge calculators keep the device arrays sorted by grid tile; perm[k] is the
+  // user index of device slot k (valid when `permuted`).  Host-visible order never
+  // changes: reads and raw device access restore it first.
+  double* d_alt = nullptr;
+  double* d_msize_alt = nullptr;
+  int32_t* d_perm = nullptr;
+  int32_t* d_perm_alt = nullptr;
+  bool permuted = false;
   std::vector<double> h[6];   // host mirror
   std::vector<double> h_msize;
   std::vector<uint8_t> alive; // host-side flags for deleteParticle/compress
@@ -183,8 +191,60 @@ void dev_reserve(ob_bunch* b, int64_t want) {
     OB_HIP(hipFree(b->d_msize));
     b->d_msize = nullptr;
   }
+  for (void** p : {(void**)&b->d_alt, (void**)&b->d_msize_alt, (void**)&b->d_perm,
+                   (void**)&b->d_perm_alt})
+    if (*p) {
+      OB_HIP(hipFree(*p));
+      *p = nullptr;
+    }
+  b->permuted = false;
   b->cap = cap;
   b->dev_valid = false;
+}
+
+__global__ void unpermute_kernel(const double* __restrict__ src, double* __restrict__ dst,
+                                 int64_t cap, const int32_t* __restrict__ perm,
+                                 const double* __restrict__ msrc, double* __restrict__ mdst,
+                                 int64_t n) {
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
+    const int64_t p = perm[i];
+#pragma unroll
+    for (int k = 0; k < 6; k++) dst[k * cap + p] = src[k * cap + i];
+    if (msrc) mdst[p] = msrc[i];
+  }
+}
+
+__global__ void iota_kernel(int32_t* __restrict__ perm, int64_t n) {
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride)
+    perm[i] = (int32_t)i;
+}
+
+void ensure_alt(ob_bunch* b) {
+  if (!b->d_alt) OB_HIP(hipMalloc((void**)&b->d_alt, sizeof(double) * 6 * b->cap));
+  if (!b->d_perm) OB_HIP(hipMalloc((void**)&b->d_perm, sizeof(int32_t) * b->cap));
+  if (!b->d_perm_alt) OB_HIP(hipMalloc((void**)&b->d_perm_alt, sizeof(int32_t) * b->cap));
+  if (b->has_msize && !b->d_msize_alt)
+    OB_HIP(hipMalloc((void**)&b->d_msize_alt, sizeof(double) * b->cap));
+}
+
+// Put the device arrays back in user order (pending maps are unaffected: they are
+// elementwise).
+void restore_order(ob_bunch* b) {
+  if (!b->permuted) return;
+  ensure_alt(b);
+  const int blocks = (int)std::min<int64_t>((b->n + 255) / 256, 4096);
+  if (b->n) {
+    ProfScope ps("unpermute");
+    hipLaunchKernelGGL(unpermute_kernel, dim3(blocks), dim3(256), 0, stream(), b->d, b->d_alt,
+                       b->cap, b->d_perm, b->has_msize ? b->d_msize : nullptr,
+                       b->has_msize ? b->d_msize_alt : nullptr, b->n);
+    OB_HIP(hipGetLastError());
+  }
+  std::swap(b->d, b->d_alt);
+  if (b->has_msize) std::swap(b->d_msize, b->d_msize_alt);
+  b->permuted = false;
 }
 
 void upload(ob_bunch* b) {
@@ -203,11 +263,13 @@ void upload(ob_bunch* b) {
   }
   OB_HIP(hipStreamSynchronize(s));  // host vectors may change right after
   b->dev_valid = true;
+  b->permuted = false;
 }
 
 void download(ob_bunch* b) {
   if (b->host_valid) return;
   bunch_flush(b, nullptr);
+  restore_order(b);
   hipStream_t s = stream();
   for (int k = 0; k < 6; k++) {
     b->h[k].resize(b->n);
@@ -238,6 +300,34 @@ DeviceBunchView bunch_device_view(ob_bunch* b) {
 }
 
 int64_t bunch_size(ob_bunch* b) { return b->n; }
+
+SortBuffers bunch_sort_begin(ob_bunch* b) {
+  upload(b);
+  ensure_alt(b);
+  if (!b->permuted && b->n) {
+    const int blocks = (int)std::min<int64_t>((b->n + 255) / 256, 4096);
+    hipLaunchKernelGGL(iota_kernel, dim3(blocks), dim3(256), 0, stream(), b->d_perm, b->n);
+    OB_HIP(hipGetLastError());
+  }
+  SortBuffers s;
+  s.cur = b->d;
+  s.alt = b->d_alt;
+  s.cap = b->cap;
+  s.perm = b->d_perm;
+  s.perm_alt = b->d_perm_alt;
+  s.msize = b->has_msize ? b->d_msize : nullptr;
+  s.msize_alt = b->has_msize ? b->d_msize_alt : nullptr;
+  s.n = b->n;
+  return s;
+}
+
+void bunch_sort_commit(ob_bunch* b) {
+  std::swap(b->d, b->d_alt);
+  std::swap(b->d_perm, b->d_perm_alt);
+  if (b->has_msize) std::swap(b->d_msize, b->d_msize_alt);
+  b->permuted = true;
+  b->host_valid = false;
+}
 
 SyncInfo bunch_sync(ob_bunch* b) {
   return SyncInfo{b->mass, b->charge, b->r0, b->macrosize, b->T, b->p, b->beta, b->gamma, b->time};
@@ -362,8 +452,10 @@ OB_API int ob_bunch_create(ob_bunch** out) {
 
 OB_API int ob_bunch_destroy(ob_bunch* b) {
   OB_TRY if (!b) return 0;
-  if (b->d) (void)hipFree(b->d);
-  if (b->d_msize) (void)hipFree(b->d_msize);
+  (void)hipStreamSynchronize(ob::stream());
+  for (void* p : {(void*)b->d, (void*)b->d_msize, (void*)b->d_alt, (void*)b->d_msize_alt,
+                  (void*)b->d_perm, (void*)b->d_perm_alt})
+    if (p) (void)hipFree(p);
   delete b;
   OB_CATCH
 }
@@ -440,6 +532,7 @@ OB_API int ob_bunch_set_coords(ob_bunch* b, int64_t n, const double* x, const do
   for (int k = 0; k < 6; k++) OB_REQUIRE(n == 0 || src[k] != nullptr, "null coordinate array");
   ob::dev_reserve(b, n);
   b->n = n;
+  b->permuted = false;
   hipStream_t s = ob::stream();
   for (int k = 0; k < 6; k++)
     if (n)
@@ -466,6 +559,7 @@ OB_API int ob_bunch_get_coords(ob_bunch* b, double* x, double* xp, double* y, do
     return 0;
   }
   ob::bunch_flush(b, nullptr);
+  ob::restore_order(b);
   hipStream_t s = ob::stream();
   for (int k = 0; k < 6; k++)
     if (dst[k] && b->n)
@@ -541,6 +635,8 @@ OB_API int ob_bunch_set_macrosizes(ob_bunch* b, int64_t n, const double* m) {
   }
   OB_REQUIRE(n == b->n, "macrosize array length differs from the bunch size");
   ob::download(b);
+  ob::bunch_flush(b, nullptr);
+  ob::restore_order(b);  // m is in user order
   b->h_msize.assign(m, m + n);
   b->has_msize = true;
   if (b->dev_valid) {
@@ -563,8 +659,9 @@ OB_API int ob_bunch_get_macrosizes(ob_bunch* b, int64_t n, double* m) {
 
 OB_API int ob_bunch_device_coords(ob_bunch* b, double** ptrs6) {
   OB_TRY OB_CHECK_B(b);
-  ob::DeviceBunchView v = ob::bunch_device_view(b);
   ob::bunch_flush(b, nullptr);
+  ob::restore_order(b);
+  ob::DeviceBunchView v = ob::bunch_device_view(b);
   for (int k = 0; k < 6; k++) ptrs6[k] = v.c[k];
   b->host_valid = false;
   OB_CATCH

@@ -137,6 +137,18 @@ void bunch_queue(ob_bunch* b, const Op& op);
 // Apply queued ops; if ext_partials != nullptr compute partial extrema too.
 int bunch_flush(ob_bunch* b, double* ext_partials);
 int64_t bunch_size(ob_bunch* b);
+// Tile sorting of the device arrays: begin returns the current and spare buffers (the
+// permutation is initialised to the identity when the bunch is in user order); the caller
+// writes the reordered particles into the spare buffers and commits, which swaps them in.
+struct SortBuffers {
+  double *cur, *alt;   // 6 * cap each
+  int64_t cap;
+  int32_t *perm, *perm_alt;
+  double *msize, *msize_alt;  // null when the bunch has no macrosize attribute
+  int64_t n;
+};
+SortBuffers bunch_sort_begin(ob_bunch* b);
+void bunch_sort_commit(ob_bunch* b);
 struct SyncInfo {
   double mass, charge, r0, macrosize, T, p, beta, gamma, time;
 };

@@ -22,6 +22,7 @@
 #include "ob_fft.hpp"
 #include "ob_internal.hpp"
 #include "ob_sc_device.hpp"
+#include "ob_tiled3d.hpp"
 
 namespace ob {
 namespace {
@@ -312,6 +313,9 @@ struct ob_sc3d {
   double *rho = nullptr, *phi = nullptr, *ghat = nullptr, *partials = nullptr, *ext_dev = nullptr;
   double* ext_host = nullptr;
   double2 *spec = nullptr, *tw = nullptr;
+  bool tiled = true;              // tile-sorted LDS deposit (else particle-order atomics)
+  int lds_budget = 110 * 1024;    // LDS bytes for one deposit tile
+  ob::TileWork work;
 };
 
 namespace ob {
@@ -481,6 +485,7 @@ OB_API int ob_sc3d_destroy(ob_sc3d* c) {
   (void)hipFree(c->rho); (void)hipFree(c->phi); (void)hipFree(c->spec); (void)hipFree(c->ghat);
   (void)hipFree(c->partials); (void)hipFree(c->ext_dev); (void)hipFree(c->tw);
   (void)hipHostFree(c->ext_host);
+  c->work.release();
   delete c;
   OB_CATCH
 }
@@ -543,11 +548,15 @@ OB_API int ob_sc3d_track(ob_sc3d* c, ob_bunch* b, double length) {
     ob::ProfScope ps_("fill_zero");
     hipLaunchKernelGGL(ob::fill_zero, dim3(1024), dim3(256), 0, s, c->rho, ng);
   }
-  ob::DeviceBunchView v = ob::bunch_device_view(b);
-  int dblk = (int)((n + ob::kThreads - 1) / ob::kThreads);
-  const int cap = 16 * ob::device_cu_count();
-  if (dblk > cap) dblk = cap;
-  {
+  const ob::TileGeom tg = ob::make_tile_geom(g.nx, g.ny, g.nz, g.xMin, g.xMax, g.yMin, g.yMax,
+                                             g.zMin, g.zMax, g.dx, g.dy, g.dz, c->lds_budget);
+  if (c->tiled && tg.T > 0) {
+    ob::sort_and_deposit_3d(b, tg, sy.macrosize, c->rho, c->work);
+  } else {  // particle-order deposit with global atomics
+    ob::DeviceBunchView v = ob::bunch_device_view(b);
+    int dblk = (int)((n + ob::kThreads - 1) / ob::kThreads);
+    const int cap = 16 * ob::device_cu_count();
+    if (dblk > cap) dblk = cap;
     ob::ProfScope ps_("deposit3d");
     hipLaunchKernelGGL(ob::deposit3d, dim3(dblk), dim3(ob::kThreads), 0, s, v, g, sy.macrosize,
                        c->rho);
@@ -613,5 +622,14 @@ OB_API int ob_sc3d_get_ratio_limit(ob_sc3d* c, double* r) {
 OB_API int ob_sc3d_green_rebuilds(ob_sc3d* c, int* n) {
   OB_TRY OB_REQUIRE(c != nullptr, "null handle");
   *n = c->rebuilds;
+  OB_CATCH
+}
+
+OB_API int ob_sc3d_set_deposit_mode(ob_sc3d* c, int tiled, int lds_budget_bytes) {
+  OB_TRY OB_REQUIRE(c != nullptr, "null handle");
+  OB_REQUIRE(lds_budget_bytes == 0 || (lds_budget_bytes >= 1024 && lds_budget_bytes <= 150 * 1024),
+             "LDS budget must be 0 (keep) or 1 KiB..150 KiB");
+  c->tiled = tiled != 0;
+  if (lds_budget_bytes) c->lds_budget = lds_budget_bytes;
   OB_CATCH
 }

@@ -258,6 +258,10 @@ class SpaceChargeCalc3D:
             self._freq = float(f)
         return self._freq
 
+    def setDepositMode(self, tiled=True, lds_budget_bytes=0):
+        """Tile-sorted LDS deposit (default) or particle-order global atomics."""
+        N.call("ob_sc3d_set_deposit_mode", self._h, 1 if tiled else 0, int(lds_budget_bytes))
+
     def getGreenRebuilds(self):
         v = C.c_int()
         N.call("ob_sc3d_green_rebuilds", self._h, C.byref(v))
