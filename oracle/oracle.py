@@ -213,6 +213,33 @@ def long_derivative(zg, zmin, zmax, n_avg=3):
     return out
 
 
+def poisson2d(rho, solver_lims, rho_lims):
+    """PoissonSolverFFT2D::findPotential with the solver on solver_lims."""
+    L = lib()
+    f = L.orc_flat_poisson2d
+    f.restype = None
+    f.argtypes = [C.c_int, C.c_int, _D, _D, _D, _D]
+    rho = np.ascontiguousarray(rho, dtype=np.float64)
+    sl, rl = _f(solver_lims), _f(rho_lims)
+    out = np.zeros_like(rho)
+    f(rho.shape[0], rho.shape[1], _dp(sl), _dp(rl), _dp(rho), _dp(out))
+    return out
+
+
+def poisson3d(rho, solver_lims, rho_lims):
+    """PoissonSolverFFT3D::findPotential; rho is [nz][nx][ny]."""
+    L = lib()
+    f = L.orc_flat_poisson3d
+    f.restype = None
+    f.argtypes = [C.c_int, C.c_int, C.c_int, _D, _D, _D, _D]
+    rho = np.ascontiguousarray(rho, dtype=np.float64)
+    nz, nx, ny = rho.shape
+    sl, rl = _f(solver_lims), _f(rho_lims)
+    out = np.zeros_like(rho)
+    f(nx, ny, nz, _dp(sl), _dp(rl), _dp(rho), _dp(out))
+    return out
+
+
 def fft_c2c(z, inverse=False):
     re = np.ascontiguousarray(z.real, dtype=np.float64).copy()
     im = np.ascontiguousarray(z.imag, dtype=np.float64).copy()

@@ -1395,3 +1395,20 @@ void orc_flat_long_derivative(int nz, double zmin, double zmax, int n_avg, const
   orc_grid1d_set_z(&c.zd, zmin, zmax);
   calc_long_derivative(&c);
 }
+
+/* Direct Poisson solves on a given rho grid (solver built on solver_lims, rho/phi on
+ * rho_lims), for brute-force checks of the Hockney convolution and its rescaling. */
+void orc_flat_poisson2d(int nx, int ny, const double* sl, const double* rl, const double* rho,
+                        double* phi) {
+  orc_poisson2d* p = orc_poisson2d_create(nx, ny, sl[0], sl[1], sl[2], sl[3]);
+  orc_grid2d r = mk2(nx, ny, rl, (double*)rho), f = mk2(nx, ny, rl, phi);
+  orc_poisson2d_find_potential(p, &r, &f);
+  orc_poisson2d_destroy(p);
+}
+void orc_flat_poisson3d(int nx, int ny, int nz, const double* sl, const double* rl,
+                        const double* rho, double* phi) {
+  orc_poisson3d* p = orc_poisson3d_create(nx, ny, nz, sl[0], sl[1], sl[2], sl[3], sl[4], sl[5]);
+  orc_grid3d r = mk3(nx, ny, nz, rl, (double*)rho), f = mk3(nx, ny, nz, rl, phi);
+  orc_poisson3d_find_potential(p, &r, &f);
+  orc_poisson3d_destroy(p);
+}

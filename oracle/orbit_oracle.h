@@ -171,6 +171,10 @@ void orc_flat_grid3d_indfrac(int nx, int ny, int nz, const double* l, long n, co
 void orc_flat_grid3d_grad(int nx, int ny, int nz, const double* l, const double* arr, long n,
                           const double* x, const double* y, const double* z, double* gx,
                           double* gy, double* gz);
+void orc_flat_poisson2d(int nx, int ny, const double* solver_lims, const double* rho_lims,
+                        const double* rho, double* phi);
+void orc_flat_poisson3d(int nx, int ny, int nz, const double* solver_lims,
+                        const double* rho_lims, const double* rho, double* phi);
 void orc_flat_long_derivative(int nz, double zmin, double zmax, int n_avg, const double* zg,
                               double* zd);
 

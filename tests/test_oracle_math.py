@@ -1,0 +1,292 @@
+"""Pins of the oracle against what the mathematics fixes, for the parts no reference build
+or golden vector covers here (TEAPOT maps, FFT, Poisson solves, the space-charge kicks).
+
+Each check is independent of the oracle's own formula: numpy's FFT, brute-force
+convolution, textbook transfer matrices, analytic fields of uniform beams, exact
+polynomial fits, symplecticity of the maps' Jacobians."""
+import math
+
+import numpy as np
+import pytest
+
+import oracle as O
+from helpers import rel_l2
+
+# ---------------------------------------------------------------- kinematics
+
+
+def test_sync_particle_kinematics():
+    # protons at 1 GeV kinetic energy: gamma = 1 + T/m, beta = sqrt(1-1/gamma^2), p = m*beta*gamma
+    s = O.Bunch(np.zeros((6, 1)), kin_energy=1.0).sync
+    m = 0.938272013
+    g = 1.0 + 1.0 / m
+    assert s["gamma"] == pytest.approx(g, rel=1e-15)
+    assert s["beta"] == pytest.approx(math.sqrt(1 - 1 / g ** 2), rel=1e-15)
+    assert s["p"] == pytest.approx(m * g * math.sqrt(1 - 1 / g ** 2), rel=1e-15)
+
+
+# ---------------------------------------------------------------- FFT
+
+@pytest.mark.parametrize("n", [2, 8, 64, 512])
+def test_fft_c2c_vs_numpy(n):
+    r = np.random.default_rng(n)
+    z = r.normal(size=n) + 1j * r.normal(size=n)
+    assert rel_l2(O.fft_c2c(z), np.fft.fft(z)) < 1e-14
+    assert rel_l2(O.fft_c2c(z, inverse=True), np.fft.ifft(z) * n) < 1e-14
+
+
+def test_fft_r2c_2d_3d_vs_numpy():
+    r = np.random.default_rng(1)
+    a = r.normal(size=(16, 32))
+    assert rel_l2(O.fft_r2c_2d(a), np.fft.rfft2(a)) < 1e-14
+    b = r.normal(size=(8, 16, 32))
+    assert rel_l2(O.fft_r2c_3d(b), np.fft.rfftn(b)) < 1e-14
+    s = np.fft.rfft2(a)
+    assert rel_l2(O.fft_c2r_2d(s, 32), a * a.size) < 1e-14
+
+
+# ---------------------------------------------------------------- Poisson solves
+
+def _brute2d(rho, dx, dy):
+    nx, ny = rho.shape
+    i = np.arange(nx)[:, None, None, None] - np.arange(nx)[None, None, :, None]
+    j = np.arange(ny)[None, :, None, None] - np.arange(ny)[None, None, None, :]
+    r2 = (i * dx) ** 2 + (j * dy) ** 2
+    with np.errstate(divide="ignore"):
+        G = np.where(r2 > 0, -0.5 * np.log(r2), 0.0)
+    return np.einsum("abcd,cd->ab", G, rho)
+
+
+@pytest.mark.parametrize("nx,ny,solver", [(8, 8, "same"), (12, 10, "same"), (16, 16, "scaled")])
+def test_poisson2d_vs_brute_force(nx, ny, solver):
+    """The zero-padded FFT convolution equals the direct sum of G(r) = -ln|r| (G(0) = 0);
+    with the solver on another grid the log rescaling must make it exact too."""
+    r = np.random.default_rng(nx + ny)
+    rho = r.uniform(0, 1, size=(nx, ny))
+    rl = np.array([-2e-3, 3e-3, -1e-3, 1e-3 * (5e-3 / (nx - 1)) / (2e-3 / (ny - 1)) * (2e-3 / 2e-3)])
+    dx = (rl[1] - rl[0]) / (nx - 1)
+    dy_target = dx * 0.7
+    rl[3] = rl[2] + dy_target * (ny - 1)
+    dy = (rl[3] - rl[2]) / (ny - 1)
+    if solver == "same":
+        sl = rl.copy()
+    else:  # same aspect ratio, 37x larger: exercises the log(dx_solver/dx_rho) rescale
+        sl = rl * 37.0
+    phi = O.poisson2d(rho, sl, rl)
+    assert rel_l2(phi, _brute2d(rho, dx, dy)) < 1e-13
+
+
+def test_poisson3d_vs_brute_force():
+    nx, ny, nz = 6, 5, 4
+    r = np.random.default_rng(3)
+    rho = r.uniform(0, 1, size=(nz, nx, ny))
+    rl = np.array([-1.0, 1.5, -1.0, 1.0, -0.6, 1.0])
+    dx, dy, dz = (rl[1] - rl[0]) / (nx - 1), (rl[3] - rl[2]) / (ny - 1), (rl[5] - rl[4]) / nz
+    sl = rl * 3.0  # solver box of the same shape, 3x larger
+    phi = O.poisson3d(rho, sl, rl)
+    iz, ix, iy = np.meshgrid(np.arange(nz), np.arange(nx), np.arange(ny), indexing="ij")
+    P = np.stack([ix.ravel() * dx, iy.ravel() * dy, iz.ravel() * dz], axis=1)
+    d = np.sqrt(((P[:, None, :] - P[None, :, :]) ** 2).sum(-1))
+    with np.errstate(divide="ignore"):
+        G = np.where(d > 0, 1.0 / d, 0.0)
+    brute = (G @ rho.ravel()).reshape(nz, nx, ny)
+    assert rel_l2(phi, brute) < 1e-13
+
+
+# ---------------------------------------------------------------- dlambda/dz
+
+def test_long_derivative_exact_on_quadratic_density():
+    """3-point least squares reproduces a quadratic exactly: dlambda/dz = B'(z)/dz^2 for a
+    per-bin content B that is quadratic in z."""
+    nz, zmin, zmax = 16, -2.0, 3.0
+    dz = (zmax - zmin) / nz
+    zc = zmin + (np.arange(nz) + 0.5) * dz
+    B = 3.0 + 0.7 * zc - 0.2 * zc ** 2
+    zd = O.long_derivative(B, zmin, zmax, 3)
+    assert np.allclose(zd, (0.7 - 0.4 * zc) / dz, rtol=1e-10, atol=1e-10)
+
+
+def test_long_derivative_matches_polyfit_windows():
+    nz, zmin, zmax, navg = 12, 0.0, 1.0, 5
+    dz = (zmax - zmin) / nz
+    B = np.random.default_rng(4).uniform(1, 2, nz)
+    zd = O.long_derivative(B, zmin, zmax, navg)
+    for iz in range(nz):
+        s = min(max(iz - navg // 2, 0), nz - navg)
+        c = np.polyfit(np.arange(1, navg + 1, dtype=float), B[s:s + navg], 2)
+        x = iz - s + 1
+        assert zd[iz] == pytest.approx((2 * c[0] * x + c[1]) / dz / dz, rel=1e-9)
+
+
+# ---------------------------------------------------------------- TEAPOT maps
+
+def _one(c6, op, *args):
+    ob = O.Bunch(np.array(c6, dtype=float).reshape(6, -1), kin_energy=1.0)
+    ob.apply(op, *args)
+    return ob
+
+
+def test_quad1_is_thick_lens_matrix():
+    x0 = np.array([2e-3, -1e-4, -1e-3, 3e-4, 0.0, 0.0])  # dE = 0: on momentum
+    for kq in (0.5, -0.5):
+        L = 0.37
+        out = _one(x0, "quad1", L, kq).coords[:, 0]
+        w = math.sqrt(abs(kq))
+        foc = np.array([[math.cos(w * L), math.sin(w * L) / w], [-w * math.sin(w * L), math.cos(w * L)]])
+        defo = np.array([[math.cosh(w * L), math.sinh(w * L) / w], [w * math.sinh(w * L), math.cosh(w * L)]])
+        mx, my = (foc, defo) if kq > 0 else (defo, foc)
+        assert np.allclose(out[:2], mx @ x0[:2], rtol=1e-14, atol=1e-18)
+        assert np.allclose(out[2:4], my @ x0[2:4], rtol=1e-14, atol=1e-18)
+        assert abs(np.linalg.det(mx) - 1) < 1e-14 and abs(np.linalg.det(my) - 1) < 1e-14
+
+
+def test_multp_is_thin_multipole_kick():
+    r = np.random.default_rng(5)
+    c = r.normal(0, 1e-2, (6, 100))
+    for pole, kl, skew in ((1, 0.3, 0), (2, -1.7, 0), (3, 0.9, 1)):
+        ob = O.Bunch(c, kin_energy=1.0)
+        ob.apply("multp", pole, kl, skew)
+        zn = (c[0] + 1j * c[2]) ** pole * kl / math.factorial(pole)
+        if skew:
+            ex, ey = zn.imag, zn.real
+        else:
+            ex, ey = -zn.real, zn.imag
+        # kicks ~1e-7 read back as differences of ~1e-2 coordinates: ~1e-11 resolution
+        assert rel_l2(ob.coords[1] - c[1], ex) < 1e-9
+        assert rel_l2(ob.coords[3] - c[3], ey) < 1e-9
+
+
+def test_drift_composes():
+    c = np.random.default_rng(6).normal(0, 1e-3, (6, 50))
+    a = O.Bunch(c, kin_energy=1.0)
+    a.apply("drift", 0.3)
+    a.apply("drift", 0.45)
+    b = O.Bunch(c, kin_energy=1.0)
+    b.apply("drift", 0.75)
+    assert np.allclose(a.coords, b.coords, rtol=1e-13, atol=1e-18)
+    # on-axis, on-momentum particle does not move
+    z0 = O.Bunch(np.zeros((6, 1)), kin_energy=1.0)
+    z0.apply("drift", 5.0)
+    assert not z0.coords.any()
+
+
+MAPS = [("drift", (0.7,)), ("quad1", (0.25, 0.5)), ("quad1", (0.25, -0.5)), ("quad2", (0.25,)),
+        ("quadfringeIN", (0.5,)), ("quadfringeOUT", (-0.5,)), ("multp", (2, 0.4, 0)),
+        ("multpfringeIN", (2, 2.0, 0)), ("multpfringeOUT", (1, -0.7, 1)),
+        ("rotatexy", (0.3,)), ("kick", (1e-4, 2e-4, 1e-5))]
+
+
+@pytest.mark.parametrize("op,args", MAPS, ids=[m[0] + str(i) for i, m in enumerate(MAPS)])
+def test_maps_are_symplectic(op, args):
+    """J^T Omega J = Omega with canonical pairs (x, x'), (y, y'), (z, dE/(p beta)).  A
+    dropped term or a wrong sign in any map breaks this at the 1e-4 level; the central
+    differences resolve ~1e-8."""
+    s = O.Bunch(np.zeros((6, 1)), kin_energy=1.0).sync
+    c = 1.0 / (s["p"] * s["beta"])
+    Om = np.zeros((6, 6))
+    Om[0, 1], Om[1, 0], Om[2, 3], Om[3, 2], Om[4, 5], Om[5, 4] = 1, -1, 1, -1, c, -c
+    for x0 in (np.array([3e-3, 1e-3, -2e-3, 5e-4, 0.1, 2e-3]),
+               np.array([-1e-2, -2e-3, 4e-3, 1e-3, -0.3, -3e-3])):
+        J = np.zeros((6, 6))
+        for j in range(6):
+            h = 1e-7 * max(abs(x0[j]), 1e-3)
+            pts = np.stack([x0, x0], axis=1)
+            pts[j, 0] += h
+            pts[j, 1] -= h
+            ob = O.Bunch(pts, kin_energy=1.0)
+            ob.apply(op, *args)
+            J[:, j] = (ob.coords[:, 0] - ob.coords[:, 1]) / (2 * h)
+        assert np.abs(J.T @ Om @ J - Om).max() < 1e-6
+
+
+# ---------------------------------------------------------------- space-charge kicks
+
+def _uniform_disk(n, R, seed):
+    r = np.random.default_rng(seed)
+    rad = R * np.sqrt(r.random(n))
+    th = 2 * np.pi * r.random(n)
+    return rad * np.cos(th), rad * np.sin(th)
+
+
+def test_sc2p5drb_transverse_kick_uniform_disk():
+    """Uniform round beam of radius R, uniform in z: inside, x'' kick = 2 r0 lambda L x /
+    (beta^2 gamma^3 R^2) (defocusing), lambda = particles per metre."""
+    n, R, Lz, L = 400000, 5e-3, 20.0, 0.5
+    x, y = _uniform_disk(n, R, 1)
+    c = np.zeros((6, n))
+    c[0], c[2] = x, y
+    c[4] = np.random.default_rng(2).uniform(-Lz / 2, Lz / 2, n)
+    ms = 1e13 / n
+    ob = O.Bunch(c, macrosize=ms, kin_energy=1.0)
+    calc = O.SC2p5Drb(128, 128, 16)
+    calc.trackBunch(ob, L, 0.1)
+    s = ob.sync
+    lam = 1e13 / Lz
+    K = 2 * 1.534698e-18 * lam * L / (s["beta"] ** 2 * s["gamma"] ** 3 * R ** 2)
+    inside = np.hypot(x, y) < 0.7 * R
+    dxp = ob.coords[1] - c[1]
+    assert np.median(dxp[inside] / (K * x[inside])) == pytest.approx(1.0, abs=0.03)
+    # a_bunch of a uniform disk is its radius: a^2 = 2 (sx^2 + sy^2) = R^2
+    assert calc.moments()[2] == pytest.approx(R, rel=0.02)
+
+
+def test_sc2p5drb_longitudinal_kick_linear_ramp():
+    """Line density with a linear slope: dE kick = dlambda/dz * (-L r0 m / gamma^2) *
+    (1 + 2 ln(b/a) - r^2/a^2) inside the beam (Baartman's g-factor)."""
+    n, R, L, b = 400000, 5e-3, 0.5, 0.05
+    rng = np.random.default_rng(3)
+    x, y = _uniform_disk(n, R, 4)
+    # density proportional to (1 + s z) on [-1, 1], s = 0.5, by inverse CDF
+    s_ = 0.5
+    u = rng.random(n)
+    z = (-1 + np.sqrt(1 - 2 * s_ * (1 - s_ / 2) + 4 * s_ * u * 1.0 + 0 * u)) / s_
+    z = np.clip(z, -1, 1)
+    c = np.zeros((6, n))
+    c[0], c[2], c[4] = x, y, z
+    ob = O.Bunch(c, macrosize=1e13 / n, kin_energy=1.0)
+    calc = O.SC2p5Drb(64, 64, 32)
+    calc.trackBunch(ob, L, b)
+    g = ob.sync["gamma"]
+    dlam = 1e13 / 2.0 * s_  # d/dz of 1e13 (1 + s z)/2
+    r2 = (x - calc.moments()[0]) ** 2 + (y - calc.moments()[1]) ** 2
+    a = calc.moments()[2]
+    expect = dlam * (-L * 1.534698e-18 * 0.938272013 / g ** 2) * (1 + 2 * np.log(b / a) - r2 / a ** 2)
+    core = (np.hypot(x, y) < 0.7 * R) & (np.abs(z) < 0.6)
+    ratio = (ob.coords[5] - c[5])[core] / expect[core]
+    assert np.median(ratio) == pytest.approx(1.0, abs=0.05)
+
+
+def test_sc3d_kick_uniform_sphere():
+    """Uniform sphere of radius R in the rest frame (z semi-axis R/gamma in the lab): inside,
+    x' kick = Q r0 L x / (R^3 beta^2 gamma^2), dE kick = Q r0 m L z_rest / R^3."""
+    n, R, L = 300000, 4e-3, 0.5
+    rng = np.random.default_rng(5)
+    v = rng.normal(size=(3, n))
+    v *= R * rng.random(n) ** (1 / 3) / np.sqrt((v ** 2).sum(0))
+    s = O.Bunch(np.zeros((6, 1)), kin_energy=1.0).sync
+    g, b = s["gamma"], s["beta"]
+    c = np.zeros((6, n))
+    c[0], c[2], c[4] = v[0], v[1], v[2] / g
+    ob = O.Bunch(c, macrosize=1e11 / n, kin_energy=1.0)
+    calc = O.SC3D(48, 48, 48)
+    calc.trackBunch(ob, L)
+    Q = 1e11
+    inside = np.sqrt((v ** 2).sum(0)) < 0.6 * R
+    kx = Q * 1.534698e-18 * L / (R ** 3 * b ** 2 * g ** 2)
+    assert np.median((ob.coords[1] - c[1])[inside] / (kx * v[0][inside])) == pytest.approx(1.0, abs=0.05)
+    kz = Q * 1.534698e-18 * 0.938272013 * L / R ** 3
+    assert np.median((ob.coords[5] - c[5])[inside] / (kz * v[2][inside])) == pytest.approx(1.0, abs=0.05)
+
+
+# ---------------------------------------------------------------- diagnostics
+
+def test_extrema_and_emittance_vs_numpy():
+    c = np.random.default_rng(9).normal(0, 1e-3, (6, 1000))
+    c[5] *= 0.0  # no dispersion term
+    ob = O.Bunch(c, kin_energy=1.0)
+    e = ob.extrema()
+    assert list(e) == [c[0].min(), c[0].max(), c[2].min(), c[2].max(), c[4].min(), c[4].max()]
+    em = ob.emittances()
+    for k in range(2):
+        cov = np.cov(c[2 * k], c[2 * k + 1], bias=True)
+        assert em[k] == pytest.approx(math.sqrt(np.linalg.det(cov)), rel=1e-9)
