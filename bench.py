@@ -27,6 +27,16 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path[:0] = [os.path.join(ROOT, "py-orbit_amd")]
 
 HBM_PEAK_GBS = 8000.0
+METRIC = "particle-pushes/s incl. SC kick, 1 MI355X; field/\u03b5-growth err vs high-N FP64"
+# rocprofv3 FETCH_SIZE/WRITE_SIZE summary of the same workload (bytes per dispatch, already
+# scaled by the copy-kernel yardstick); see the commands recorded in the file.
+TRAFFIC_FILES = {"ring3d": "profiles/ring3d_hbm_traffic.md",
+                 "fodo2p5d": "profiles/fodo2p5d_hbm_traffic.md"}
+# profiler label -> kernel name in the rocprofv3 summaries
+ROCPROF_NAME = {"fft3_strided_conv": "fft3_strided<2>", "fft3_strided_fwd": "fft3_strided<0>",
+                "fft3_strided_inv": "fft3_strided<1>", "push_extrema": "push_kernel<true>",
+                "push": "push_kernel<false>", "fft_cols": "fft_cols<true>",
+                "fft3_z_r2c": "fft3_z_r2c<false>"}
 
 
 def parse():
@@ -119,6 +129,19 @@ def kernel_bytes(workload, n, grid):
     return b
 
 
+def measured_traffic(workload, label):
+    """HBM bytes per launch of `label` from the committed rocprofv3 summary, or None."""
+    path = os.path.join(ROOT, TRAFFIC_FILES[workload])
+    name = ROCPROF_NAME.get(label, label)
+    if not os.path.exists(path):
+        return None
+    for line in open(path):
+        f = line.split()
+        if len(f) == 4 and f[0] == "traffic_bytes" and f[1] == name:
+            return int(f[2]) + int(f[3])
+    return None
+
+
 def profile(N, lat, b, cells, steps):
     import ctypes as C
     N.call("ob_profile_enable", 1)
@@ -175,9 +198,9 @@ def cpu_baseline(args, n, grid, coords):
 def main():
     args = parse()
     if args.gpus != 1:
-        print("bench.py: this build measures one GPU; multi-GPU runs are not part of it "
-              "(the path shards by particles with one rho all-reduce per SC step, "
-              "see DESIGN.md)", file=sys.stderr)
+        print("bench.py: only one GPU is measured; runs on several GPUs are not supported "
+              "(the path would shard by particles with one rho all-reduce per SC step)",
+              file=sys.stderr)
         sys.exit(2)
     import _obnative as N
     b, lat, calc, cells, n, grid, coords = build_workload(args)
@@ -197,7 +220,8 @@ def main():
     avg_s = tot / cnt / 1e3
     achieved = kb[dom] / avg_s / 1e9
     roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
-                "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
+                "traffic": measured_traffic(args.workload, dom),
                 "kernel": dom, "avg_us": round(avg_s * 1e6, 2), "bytes_per_launch": kb[dom]}
     kernels = {k: {"count": v[0], "avg_us": round(v[1] / v[0] * 1e3, 2),
                    "share": round(v[1] / sum(x[1] for x in prof.values()), 4)}
@@ -206,7 +230,7 @@ def main():
     if not args.no_cpu:
         cpu = cpu_baseline(args, n, grid, coords)
     out = {
-        "metric": "particle-pushes/s incl. SC kick, 1 MI355X",
+        "metric": METRIC,
         "value": round(value, 1),
         "unit": "particle-pushes/s",
         "n_gpus": 1,

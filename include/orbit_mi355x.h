@@ -141,6 +141,9 @@ int ob_sc3d_set_deposit_mode(ob_sc3d* c, int tiled, int lds_budget_bytes);
 int ob_profile_enable(int on);
 /* Lines "name count total_ms min_ms max_ms" for the collected launches. */
 int ob_profile_report(char* buf, int buflen);
+/* A plain streaming copy of n doubles (8n bytes read, 8n written) timed over reps launches
+ * (no reference counterpart): the yardstick for the hardware byte counters. */
+int ob_calibrate_copy(int64_t n, int reps, double* ms_per_copy);
 
 #ifdef __cplusplus
 }

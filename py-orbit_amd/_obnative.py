@@ -67,6 +67,7 @@ SIGNATURES = {
     "ob_sc3d_set_deposit_mode": [_vp, _i, _i],
     "ob_profile_enable": [_i],
     "ob_profile_report": [C.c_char_p, _i],
+    "ob_calibrate_copy": [_i64, _i, _D],
 }
 _RET = {"ob_last_error": C.c_char_p, "ob_version": C.c_char_p}
 

@@ -834,3 +834,44 @@ OB_API int ob_tpb_quadfringeOUT(ob_bunch* b, double kq, int) {
   quadfringe(b, kq, true);
   OB_CATCH
 }
+
+// ---------------------------------------------------------------------------
+// Calibration: a plain streaming copy of n doubles (8n bytes read, 8n written), used to
+// scale hardware byte counters.  No reference counterpart.
+namespace {
+__global__ __launch_bounds__(256) void copy_kernel(const double* __restrict__ src,
+                                                   double* __restrict__ dst, int64_t n) {
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride)
+    dst[i] = src[i];
+}
+}  // namespace
+
+OB_API int ob_calibrate_copy(int64_t n, int reps, double* ms_per_copy) {
+  OB_TRY OB_REQUIRE(n > 0 && reps > 0, "n and reps must be positive");
+  ob::ensure_init();
+  double *a = nullptr, *b2 = nullptr;
+  OB_HIP(hipMalloc((void**)&a, sizeof(double) * n));
+  OB_HIP(hipMalloc((void**)&b2, sizeof(double) * n));
+  hipStream_t s = ob::stream();
+  hipLaunchKernelGGL(copy_kernel, dim3(1024), dim3(256), 0, s, b2, a, n);  // touch
+  hipEvent_t e0, e1;
+  OB_HIP(hipEventCreate(&e0));
+  OB_HIP(hipEventCreate(&e1));
+  const int blocks = (int)std::min<int64_t>((n + 255) / 256, 8 * 1024);
+  OB_HIP(hipEventRecord(e0, s));
+  for (int r = 0; r < reps; r++) {
+    ob::ProfScope ps("copy_kernel");
+    hipLaunchKernelGGL(copy_kernel, dim3(blocks), dim3(256), 0, s, a, b2, n);
+  }
+  OB_HIP(hipEventRecord(e1, s));
+  OB_HIP(hipEventSynchronize(e1));
+  float ms = 0.f;
+  OB_HIP(hipEventElapsedTime(&ms, e0, e1));
+  *ms_per_copy = ms / reps;
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  (void)hipFree(a);
+  (void)hipFree(b2);
+  OB_CATCH
+}
