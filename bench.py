@@ -32,11 +32,11 @@ METRIC = "particle-pushes/s incl. SC kick, 1 MI355X; field/\u03b5-growth err vs 
 # scaled by the copy-kernel yardstick); see the commands recorded in the file.
 TRAFFIC_FILES = {"ring3d": "profiles/ring3d_hbm_traffic.md",
                  "fodo2p5d": "profiles/fodo2p5d_hbm_traffic.md"}
-# profiler label -> kernel name in the rocprofv3 summaries
-ROCPROF_NAME = {"fft3_strided_conv": "fft3_strided<2>", "fft3_strided_fwd": "fft3_strided<0>",
-                "fft3_strided_inv": "fft3_strided<1>", "push_extrema": "push_kernel<true>",
-                "push": "push_kernel<false>", "fft_cols": "fft_cols<true>",
-                "fft3_z_r2c": "fft3_z_r2c<false>"}
+# profiler label -> kernel name (prefix) in the rocprofv3 summaries
+ROCPROF_NAME = {"fft3_strided_conv": "fft3_strided<2,", "fft3_strided_fwd": "fft3_strided<0,",
+                "fft3_strided_inv": "fft3_strided<1,", "push_extrema": "push_kernel<true>",
+                "push": "push_kernel<false>", "fft_cols": "fft_cols<false,",
+                "fft3_z_r2c": "fft3_z_r2c<", "fft3_z_c2r": "fft3_z_c2r<"}
 
 
 def parse():
@@ -137,7 +137,7 @@ def measured_traffic(workload, label):
         return None
     for line in open(path):
         f = line.split()
-        if len(f) == 4 and f[0] == "traffic_bytes" and f[1] == name:
+        if len(f) == 4 and f[0] == "traffic_bytes" and f[1].startswith(name):
             return int(f[2]) + int(f[3])
     return None
 

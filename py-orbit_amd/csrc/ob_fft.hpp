@@ -1,92 +1,191 @@
 // ob_fft.hpp -- FP64 complex FFTs staged in LDS, used by the Hockney Poisson solvers.
 //
-// A workgroup transforms `lines` lines of power-of-two length n held in LDS as split
-// re/im arrays (line stride n).  Forward transforms use decimation in time (input in
-// bit-reversed order, output natural); inverse transforms use decimation in frequency
-// (input natural, output bit-reversed), so a forward -> multiply -> inverse sequence
-// needs no permutation pass in between.  Conventions follow FFTW: forward exponent -1,
-// unnormalised.
+// A workgroup transforms `lines` lines of power-of-two length N held in LDS as split
+// re/im arrays (line l starts at l*ls; ls = N+1 spreads lines over the banks).  The
+// transform is a Stockham autosort FFT: stages of radix 8 (4 or 2 for the remaining
+// factor of N), natural order in and out, so forward -> multiply -> inverse needs no
+// permutation.  A stage reads each radix-R butterfly into registers, applies the twiddles
+// and an R-point DFT, and after a barrier writes it back in place.  Conventions follow
+// FFTW: forward exponent -1, unnormalised.
 //
-// Twiddles come from a table tw[k] = exp(-2 pi i k / nmax), k < nmax/2, computed on the
+// Twiddles come from a table tw[q] = exp(-2 pi i q / nmax), q < nmax, computed on the
 // host in long double from the exact rational angle and rounded once.
 #pragma once
+
+#include <type_traits>
 
 #include "ob_internal.hpp"
 
 namespace ob {
 
-__device__ __forceinline__ int bitrev(int j, int log2n) {
-  return (int)(__brev((unsigned)j) >> (32 - log2n));
+__host__ __device__ constexpr int clog2(int n) { return n <= 1 ? 0 : 1 + clog2(n / 2); }
+
+// Stage radices for N = 2^k: 8s, with the remaining factor 2^(k mod 3) taken first as a 4
+// (k mod 3 = 2), two 4s in place of an 8 and a 2 (k mod 3 = 1, k >= 4), or a 2 (N = 2).
+__host__ __device__ constexpr int fft_stages(int n) {
+  return clog2(n) % 3 == 0 ? clog2(n) / 3
+                           : (clog2(n) % 3 == 2 ? 1 + clog2(n) / 3
+                                                : (clog2(n) == 1 ? 1 : 2 + (clog2(n) - 4) / 3));
+}
+__host__ __device__ constexpr int fft_radix(int n, int s) {
+  return clog2(n) % 3 == 0 ? 8
+                           : (clog2(n) % 3 == 2 ? (s == 0 ? 4 : 8)
+                                                : (clog2(n) == 1 ? 2 : (s < 2 ? 4 : 8)));
+}
+__host__ __device__ constexpr int fft_span(int n, int s) {  // product of the radices before s
+  return s == 0 ? 1 : fft_span(n, s - 1) * fft_radix(n, s - 1);
 }
 
-// Radix-2 DIT, in place; data enters in bit-reversed order.  sign: -1 forward, +1 inverse.
-// Line l starts at l*ls (ls >= n; ls = n+1 spreads lines over the LDS banks).
-__device__ __forceinline__ void lds_fft_dit(double* __restrict__ re, double* __restrict__ im,
-                                            int n, int log2n, int lines,
-                                            const double2* __restrict__ tw, int nmax,
-                                            double sign, int ls = 0) {
-  if (ls == 0) ls = n;
-  const int nb = lines * (n >> 1);
-  for (int s = 1; s <= log2n; s++) {
-    const int half = 1 << (s - 1);
-    const int tstep = nmax >> s;
-    for (int t = threadIdx.x; t < nb; t += blockDim.x) {
-      const int line = t >> (log2n - 1);
-      const int j = t & ((n >> 1) - 1);
-      const int k = j & (half - 1);
-      const int a = line * ls + ((j >> (s - 1)) << s) + k;
-      const int c = a + half;
-      const double2 w = tw[k * tstep];
-      const double wr = w.x, wi = sign < 0 ? w.y : -w.y;
-      const double cr = re[c], ci = im[c];
-      const double tr = cr * wr - ci * wi;
-      const double ti = cr * wi + ci * wr;
-      const double ar = re[a], ai = im[a];
-      re[c] = ar - tr;
-      im[c] = ai - ti;
-      re[a] = ar + tr;
-      im[a] = ai + ti;
+// In-register R-point DFTs, natural order; SIGN -1 forward, +1 inverse.
+template <int SIGN>
+__device__ __forceinline__ void dft2(double* r, double* i) {
+  const double ar = r[0], ai = i[0];
+  r[0] = ar + r[1];
+  i[0] = ai + i[1];
+  r[1] = ar - r[1];
+  i[1] = ai - i[1];
+}
+
+template <int SIGN>
+__device__ __forceinline__ void dft4(double* r, double* i) {
+  const double t0r = r[0] + r[2], t0i = i[0] + i[2];
+  const double t1r = r[0] - r[2], t1i = i[0] - i[2];
+  const double t2r = r[1] + r[3], t2i = i[1] + i[3];
+  const double dr = r[1] - r[3], di = i[1] - i[3];
+  // (x1 - x3) * exp(SIGN i pi / 2) = (x1 - x3) * SIGN i
+  const double t3r = SIGN > 0 ? -di : di, t3i = SIGN > 0 ? dr : -dr;
+  r[0] = t0r + t2r;
+  i[0] = t0i + t2i;
+  r[2] = t0r - t2r;
+  i[2] = t0i - t2i;
+  r[1] = t1r + t3r;
+  i[1] = t1i + t3i;
+  r[3] = t1r - t3r;
+  i[3] = t1i - t3i;
+}
+
+template <int SIGN>
+__device__ __forceinline__ void dft8(double* r, double* i) {
+  double er[4] = {r[0], r[2], r[4], r[6]}, ei[4] = {i[0], i[2], i[4], i[6]};
+  double orr[4] = {r[1], r[3], r[5], r[7]}, oi[4] = {i[1], i[3], i[5], i[7]};
+  dft4<SIGN>(er, ei);
+  dft4<SIGN>(orr, oi);
+  constexpr double h = 0.70710678118654752440;  // sqrt(1/2)
+  constexpr double s = SIGN > 0 ? 1.0 : -1.0;
+  // X[k] = E[k] + W^k O[k], X[k+4] = E[k] - W^k O[k], W = exp(SIGN 2 pi i / 8)
+  double xr, xi;
+  r[0] = er[0] + orr[0];
+  i[0] = ei[0] + oi[0];
+  r[4] = er[0] - orr[0];
+  i[4] = ei[0] - oi[0];
+  xr = h * orr[1] - s * h * oi[1];  // W^1 = h + s h i
+  xi = s * h * orr[1] + h * oi[1];
+  r[1] = er[1] + xr;
+  i[1] = ei[1] + xi;
+  r[5] = er[1] - xr;
+  i[5] = ei[1] - xi;
+  xr = -s * oi[2];                  // W^2 = s i
+  xi = s * orr[2];
+  r[2] = er[2] + xr;
+  i[2] = ei[2] + xi;
+  r[6] = er[2] - xr;
+  i[6] = ei[2] - xi;
+  xr = -h * orr[3] - s * h * oi[3];  // W^3 = -h + s h i
+  xi = s * h * orr[3] - h * oi[3];
+  r[3] = er[3] + xr;
+  i[3] = ei[3] + xi;
+  r[7] = er[3] - xr;
+  i[7] = ei[3] - xi;
+}
+
+template <int R, int SIGN>
+__device__ __forceinline__ void dft(double* r, double* i) {
+  if constexpr (R == 8)
+    dft8<SIGN>(r, i);
+  else if constexpr (R == 4)
+    dft4<SIGN>(r, i);
+  else
+    dft2<SIGN>(r, i);
+}
+
+// One Stockham stage S of an N-point transform over `lines` (<= LMAX) lines, NT threads.
+template <int N, int S, int LMAX, int NT, int SIGN>
+__device__ __forceinline__ void fft_stage(double* __restrict__ re, double* __restrict__ im,
+                                          int lines, int ls, const double2* __restrict__ tw,
+                                          int nmax) {
+  constexpr int R = fft_radix(N, S);
+  constexpr int Ns = fft_span(N, S);
+  constexpr int NB = N / R;  // butterflies per line
+  constexpr int BPT = (LMAX * NB + NT - 1) / NT;
+  const int total = lines * NB;
+  const int tstep = nmax / (Ns * R);
+  double vr[BPT][R], vi[BPT][R];
+#pragma unroll
+  for (int q = 0; q < BPT; q++) {
+    const int b = (int)threadIdx.x + q * NT;
+    if (b < total) {
+      const int line = b / NB, j = b - line * NB;
+      const double* lr = re + line * ls + j;
+      const double* li = im + line * ls + j;
+#pragma unroll
+      for (int m = 0; m < R; m++) {
+        vr[q][m] = lr[m * NB];
+        vi[q][m] = li[m * NB];
+      }
+      if constexpr (Ns > 1) {
+        const int k = j & (Ns - 1);
+#pragma unroll
+        for (int m = 1; m < R; m++) {  // v[m] *= exp(SIGN 2 pi i m k / (Ns R))
+          const double2 w = tw[m * k * tstep];
+          const double wr = w.x, wi = SIGN < 0 ? w.y : -w.y;
+          const double a = vr[q][m], c = vi[q][m];
+          vr[q][m] = a * wr - c * wi;
+          vi[q][m] = a * wi + c * wr;
+        }
+      }
+      dft<R, SIGN>(vr[q], vi[q]);
     }
-    __syncthreads();
+  }
+  __syncthreads();
+#pragma unroll
+  for (int q = 0; q < BPT; q++) {
+    const int b = (int)threadIdx.x + q * NT;
+    if (b < total) {
+      const int line = b / NB, j = b - line * NB;
+      const int k = j & (Ns - 1);
+      double* dr = re + line * ls + (j - k) * R + k;
+      double* di = im + line * ls + (j - k) * R + k;
+#pragma unroll
+      for (int m = 0; m < R; m++) {
+        dr[m * Ns] = vr[q][m];
+        di[m * Ns] = vi[q][m];
+      }
+    }
+  }
+  __syncthreads();
+}
+
+template <int N, int S, int LMAX, int NT, int SIGN>
+__device__ __forceinline__ void fft_stages_from(double* re, double* im, int lines, int ls,
+                                                const double2* tw, int nmax) {
+  if constexpr (S < fft_stages(N)) {
+    fft_stage<N, S, LMAX, NT, SIGN>(re, im, lines, ls, tw, nmax);
+    fft_stages_from<N, S + 1, LMAX, NT, SIGN>(re, im, lines, ls, tw, nmax);
   }
 }
 
-// Radix-2 DIF, in place; output leaves in bit-reversed order.
-__device__ __forceinline__ void lds_fft_dif(double* __restrict__ re, double* __restrict__ im,
-                                            int n, int log2n, int lines,
-                                            const double2* __restrict__ tw, int nmax,
-                                            double sign, int ls = 0) {
-  if (ls == 0) ls = n;
-  const int nb = lines * (n >> 1);
-  for (int s = log2n; s >= 1; s--) {
-    const int half = 1 << (s - 1);
-    const int tstep = nmax >> s;
-    for (int t = threadIdx.x; t < nb; t += blockDim.x) {
-      const int line = t >> (log2n - 1);
-      const int j = t & ((n >> 1) - 1);
-      const int k = j & (half - 1);
-      const int a = line * ls + ((j >> (s - 1)) << s) + k;
-      const int c = a + half;
-      const double2 w = tw[k * tstep];
-      const double wr = w.x, wi = sign < 0 ? w.y : -w.y;
-      const double ar = re[a], ai = im[a], cr = re[c], ci = im[c];
-      re[a] = ar + cr;
-      im[a] = ai + ci;
-      const double dr = ar - cr, di = ai - ci;
-      re[c] = dr * wr - di * wi;
-      im[c] = dr * wi + di * wr;
-    }
-    __syncthreads();
-  }
+// N-point FFT of `lines` (<= LMAX) lines in LDS with NT threads; every thread of the block
+// must call it, and the data must be visible to all threads on entry (barrier before).
+// Returns after a barrier.
+template <int N, int LMAX, int NT, int SIGN>
+__device__ __forceinline__ void lds_fft(double* __restrict__ re, double* __restrict__ im,
+                                        int lines, int ls, const double2* __restrict__ tw,
+                                        int nmax) {
+  fft_stages_from<N, 0, LMAX, NT, SIGN>(re, im, lines, ls, tw, nmax);
 }
 
-// Host: twiddle table for transforms up to length nmax (power of two).
+// Host: twiddle table of nmax entries (nmax a power of two).
 double2* make_twiddles(int nmax);
-__host__ __device__ inline int ilog2(int n) {
-  int l = 0;
-  while ((1 << l) < n) l++;
-  return l;
-}
 // Padded transform length for an n-point free-space convolution: the reference pads to
 // 2n (PoissonSolverFFT2D.cc:27-28); any length >= 2n-1 gives the same linear convolution,
 // and the next power of two of 2n-1 equals 2n whenever n is a power of two.
@@ -94,6 +193,26 @@ inline int pad_len(int n) {
   int m = 1;
   while (m < 2 * n - 1) m <<= 1;
   return m < 2 ? 2 : m;
+}
+
+// Calls f(std::integral_constant<int, N>{}) for the runtime power of two n in [2, 4096].
+template <typename F>
+inline void dispatch_pow2(int n, F&& f) {
+  switch (n) {
+    case 2: f(std::integral_constant<int, 2>{}); break;
+    case 4: f(std::integral_constant<int, 4>{}); break;
+    case 8: f(std::integral_constant<int, 8>{}); break;
+    case 16: f(std::integral_constant<int, 16>{}); break;
+    case 32: f(std::integral_constant<int, 32>{}); break;
+    case 64: f(std::integral_constant<int, 64>{}); break;
+    case 128: f(std::integral_constant<int, 128>{}); break;
+    case 256: f(std::integral_constant<int, 256>{}); break;
+    case 512: f(std::integral_constant<int, 512>{}); break;
+    case 1024: f(std::integral_constant<int, 1024>{}); break;
+    case 2048: f(std::integral_constant<int, 2048>{}); break;
+    case 4096: f(std::integral_constant<int, 4096>{}); break;
+    default: throw Error{"FFT length is not a power of two in [2, 4096]"};
+  }
 }
 
 }  // namespace ob

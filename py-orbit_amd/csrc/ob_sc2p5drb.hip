@@ -23,9 +23,9 @@
 namespace ob {
 
 double2* make_twiddles(int nmax) {
-  std::vector<double2> h(nmax / 2 > 0 ? nmax / 2 : 1);
+  std::vector<double2> h(nmax);
   const long double pi = 3.141592653589793238462643383279502884L;
-  for (int k = 0; k < nmax / 2; k++) {
+  for (int k = 0; k < nmax; k++) {
     long double ang = 2.0L * pi * (long double)k / (long double)nmax;
     h[k].x = (double)cosl(ang);
     h[k].y = (double)(-sinl(ang));
@@ -287,120 +287,110 @@ __global__ __launch_bounds__(kSetupThreads) void sc2d_post(SC2DParams P, SC2DSta
 
 // ---- 5: Poisson solve --------------------------------------------------------
 // (a) two real rows per complex FFT along y, r2c split.  rows >= nrows_in are zero;
-// columns >= ncols_in are zero (padding).  out: spectrum [*][H] complex (H = M1/2+1).
+// columns >= ncols_in are zero (padding).  out: spectrum [*][H] complex (H = N/2+1).
+template <int N>
 __global__ __launch_bounds__(kFFTThreads) void fft_rows_r2c(const double* __restrict__ in,
                                                             int nrows_in, int ncols_in,
-                                                            int ld_in, int M1,
-                                                            double2* __restrict__ out,
+                                                            int ld_in, double2* __restrict__ out,
                                                             const double2* __restrict__ tw,
                                                             int nmax, const SC2DState* st) {
   if (st && st->skip) return;
   extern __shared__ double sm[];
   double* re = sm;
-  double* im = sm + M1;
-  const int log2n = ilog2(M1);
+  double* im = sm + N;
   const int r0 = 2 * blockIdx.x, r1 = r0 + 1;
-  for (int j = threadIdx.x; j < M1; j += blockDim.x) {
-    const double v0 = (j < ncols_in) ? in[(int64_t)r0 * ld_in + j] : 0.0;
-    const double v1 = (j < ncols_in && r1 < nrows_in) ? in[(int64_t)r1 * ld_in + j] : 0.0;
-    const int q = bitrev(j, log2n);
-    re[q] = v0;
-    im[q] = v1;
+  for (int j = threadIdx.x; j < N; j += kFFTThreads) {
+    re[j] = (j < ncols_in) ? in[(int64_t)r0 * ld_in + j] : 0.0;
+    im[j] = (j < ncols_in && r1 < nrows_in) ? in[(int64_t)r1 * ld_in + j] : 0.0;
   }
   __syncthreads();
-  lds_fft_dit(re, im, M1, log2n, 1, tw, nmax, -1.0);
-  const int H = M1 / 2 + 1;
-  for (int k = threadIdx.x; k < H; k += blockDim.x) {
-    const int nk = (M1 - k) & (M1 - 1);
+  lds_fft<N, 1, kFFTThreads, -1>(re, im, 1, N, tw, nmax);
+  constexpr int H = N / 2 + 1;
+  for (int k = threadIdx.x; k < H; k += kFFTThreads) {
+    const int nk = (N - k) & (N - 1);
     const double zr = re[k], zi = im[k], nr = re[nk], ni = im[nk];
     out[(int64_t)r0 * H + k] = make_double2((zr + nr) * 0.5, (zi - ni) * 0.5);
     if (r1 < nrows_in) out[(int64_t)r1 * H + k] = make_double2((zi + ni) * 0.5, -(zr - nr) * 0.5);
   }
 }
 
-// (b) column transforms along x.  GREEN: full forward transform of all M0 rows, written
-// transposed as ghat[k][i].  SOLVE: rows >= nrows_in are zero; forward, multiply by the
-// rescaled Green spectrum, inverse (DIF), write only rows < nrows_in back.
-template <bool GREEN>
+// (b) column transforms along x (N = M0).  GREEN: full forward transform of all N rows,
+// written transposed as ghat[k][i].  SOLVE: rows >= nrows_in are zero; forward, multiply by
+// the rescaled Green spectrum, inverse, write only rows < nrows_in back.
+template <bool GREEN, int N>
 __global__ __launch_bounds__(kFFTThreads) void fft_cols(double2* __restrict__ spec, int nrows_in,
-                                                        int M0, int H,
-                                                        double2* __restrict__ ghat_t,
+                                                        int H, double2* __restrict__ ghat_t,
                                                         const double2* __restrict__ tw, int nmax,
                                                         const SC2DState* st, double npad) {
   if (st && st->skip) return;
   extern __shared__ double sm[];
   double* re = sm;
-  double* im = sm + M0;
-  const int log2n = ilog2(M0);
+  double* im = sm + N;
   const int k = blockIdx.x;
-  for (int i = threadIdx.x; i < M0; i += blockDim.x) {
+  for (int i = threadIdx.x; i < N; i += kFFTThreads) {
     const double2 v = (i < nrows_in) ? spec[(int64_t)i * H + k] : make_double2(0.0, 0.0);
-    const int q = bitrev(i, log2n);
-    re[q] = v.x;
-    im[q] = v.y;
+    re[i] = v.x;
+    im[i] = v.y;
   }
   __syncthreads();
-  lds_fft_dit(re, im, M0, log2n, 1, tw, nmax, -1.0);
-  if (GREEN) {
-    for (int i = threadIdx.x; i < M0; i += blockDim.x)
-      ghat_t[(int64_t)k * M0 + i] = make_double2(re[i], im[i]);
+  lds_fft<N, 1, kFFTThreads, -1>(re, im, 1, N, tw, nmax);
+  if constexpr (GREEN) {
+    for (int i = threadIdx.x; i < N; i += kFFTThreads)
+      ghat_t[(int64_t)k * N + i] = make_double2(re[i], im[i]);
     return;
-  }
-  // PoissonSolverFFT2D.cc:215-236: Green spectrum rescaled to the rho grid step
-  const double s = st->scale_coeff;
-  for (int i = threadIdx.x; i < M0; i += blockDim.x) {
-    const double2 g = ghat_t[(int64_t)k * M0 + i];
-    const double gr_re = ((i == 0 && k == 0) ? (g.x + s * npad) : g.x) - s;
-    const double gr_im = g.y;
-    const double a = re[i], b = im[i];
-    re[i] = a * gr_re - b * gr_im;
-    im[i] = a * gr_im + b * gr_re;
-  }
-  __syncthreads();
-  lds_fft_dif(re, im, M0, log2n, 1, tw, nmax, +1.0);
-  for (int i = threadIdx.x; i < nrows_in; i += blockDim.x) {
-    const int q = bitrev(i, log2n);
-    spec[(int64_t)i * H + k] = make_double2(re[q], im[q]);
+  } else {
+    // PoissonSolverFFT2D.cc:215-236: Green spectrum rescaled to the rho grid step
+    const double s = st->scale_coeff;
+    for (int i = threadIdx.x; i < N; i += kFFTThreads) {
+      const double2 g = ghat_t[(int64_t)k * N + i];
+      const double gr_re = ((i == 0 && k == 0) ? (g.x + s * npad) : g.x) - s;
+      const double gr_im = g.y;
+      const double a = re[i], b = im[i];
+      re[i] = a * gr_re - b * gr_im;
+      im[i] = a * gr_im + b * gr_re;
+    }
+    __syncthreads();
+    lds_fft<N, 1, kFFTThreads, +1>(re, im, 1, N, tw, nmax);
+    for (int i = threadIdx.x; i < nrows_in; i += kFFTThreads)
+      spec[(int64_t)i * H + k] = make_double2(re[i], im[i]);
   }
 }
 
 // (c) c2r of two rows per complex inverse FFT; writes the first ncols_out columns scaled.
+template <int N>
 __global__ __launch_bounds__(kFFTThreads) void fft_rows_c2r(const double2* __restrict__ spec,
-                                                            int nrows, int M1,
-                                                            double* __restrict__ out, int ld_out,
-                                                            int ncols_out, double denom,
+                                                            int nrows, double* __restrict__ out,
+                                                            int ld_out, int ncols_out,
+                                                            double denom,
                                                             const double2* __restrict__ tw,
                                                             int nmax, const SC2DState* st) {
   if (st && st->skip) return;
   extern __shared__ double sm[];
   double* re = sm;
-  double* im = sm + M1;
-  const int log2n = ilog2(M1);
-  const int H = M1 / 2 + 1;
+  double* im = sm + N;
+  constexpr int H = N / 2 + 1;
   const int r0 = 2 * blockIdx.x, r1 = r0 + 1;
   const bool has1 = r1 < nrows;
-  for (int k = threadIdx.x; k < M1; k += blockDim.x) {
-    const int kk = (k <= M1 / 2) ? k : M1 - k;
-    double2 a = spec[(int64_t)r0 * H + kk];
-    double2 b = has1 ? spec[(int64_t)r1 * H + kk] : make_double2(0.0, 0.0);
-    if (kk == 0 || kk == M1 / 2) {  // c2r ignores the imaginary DC/Nyquist parts
+  for (int k = threadIdx.x; k < H; k += kFFTThreads) {
+    double2 a = spec[(int64_t)r0 * H + k];
+    double2 b = has1 ? spec[(int64_t)r1 * H + k] : make_double2(0.0, 0.0);
+    if (k == 0 || k == N / 2) {  // c2r ignores the imaginary DC/Nyquist parts
       a.y = 0.0;
       b.y = 0.0;
     }
-    if (k <= M1 / 2) {  // Z = X0 + i X1
-      re[k] = a.x - b.y;
-      im[k] = a.y + b.x;
-    } else {            // X(k) = conj(X(M1-k))
-      re[k] = a.x + b.y;
-      im[k] = b.x - a.y;
+    // Z = X0 + i X1 with X(N-k) = conj(X(k))
+    re[k] = a.x - b.y;
+    im[k] = a.y + b.x;
+    if (k > 0 && k < N / 2) {
+      re[N - k] = a.x + b.y;
+      im[N - k] = b.x - a.y;
     }
   }
   __syncthreads();
-  lds_fft_dif(re, im, M1, log2n, 1, tw, nmax, +1.0);
-  for (int j = threadIdx.x; j < ncols_out; j += blockDim.x) {
-    const int q = bitrev(j, log2n);
-    out[(int64_t)r0 * ld_out + j] = denom * re[q];
-    if (has1) out[(int64_t)r1 * ld_out + j] = denom * im[q];
+  lds_fft<N, 1, kFFTThreads, +1>(re, im, 1, N, tw, nmax);
+  for (int j = threadIdx.x; j < ncols_out; j += kFFTThreads) {
+    out[(int64_t)r0 * ld_out + j] = denom * re[j];
+    if (has1) out[(int64_t)r1 * ld_out + j] = denom * im[j];
   }
 }
 
@@ -432,27 +422,71 @@ struct ob_sc2p5drb {
 
 namespace ob {
 namespace {
-size_t fft_lds(int n) { return sizeof(double) * 2 * n; }
+template <int N>
+size_t fft_lds() { return sizeof(double) * 2 * N; }
+
+// Dynamic LDS above 64 KiB has to be allowed per kernel; done once per instantiation.
+template <typename K>
+void allow_lds(K* kernel, size_t bytes) {
+  if (bytes > 64 * 1024)
+    OB_HIP(hipFuncSetAttribute((const void*)kernel, hipFuncAttributeMaxDynamicSharedMemorySize,
+                               (int)bytes));
+}
+
+void launch_rows_r2c(ob_sc2p5drb* c, const double* in, int nrows, int ncols, int ld,
+                     const SC2DState* st) {
+  dispatch_pow2(c->M1, [&](auto NC) {
+    constexpr int N = decltype(NC)::value;
+    if constexpr (N >= 8) {
+      static bool ready = false;
+      if (!ready) {
+        allow_lds(fft_rows_r2c<N>, fft_lds<N>());
+        ready = true;
+      }
+      ob::ProfScope ps_("fft_rows_r2c");
+      hipLaunchKernelGGL(fft_rows_r2c<N>, dim3((nrows + 1) / 2), dim3(kFFTThreads), fft_lds<N>(),
+                         stream(), in, nrows, ncols, ld, c->spec, c->tw, c->nmax, st);
+    }
+  });
+  OB_HIP(hipGetLastError());
+}
+
+template <bool GREEN>
+void launch_cols(ob_sc2p5drb* c, int nrows, const SC2DState* st, double npad) {
+  dispatch_pow2(c->M0, [&](auto NC) {
+    constexpr int N = decltype(NC)::value;
+    if constexpr (N >= 8) {
+      static bool ready = false;
+      if (!ready) {
+        allow_lds(fft_cols<GREEN, N>, fft_lds<N>());
+        ready = true;
+      }
+      ob::ProfScope ps_(GREEN ? "fft_cols_green" : "fft_cols");
+      hipLaunchKernelGGL((fft_cols<GREEN, N>), dim3(c->H), dim3(kFFTThreads), fft_lds<N>(),
+                         stream(), c->spec, nrows, c->H, c->ghat_t, c->tw, c->nmax, st, npad);
+    }
+  });
+  OB_HIP(hipGetLastError());
+}
 
 void launch_fft_pipeline(ob_sc2p5drb* c, const SC2DState* st) {
-  hipStream_t s = stream();
-  {
-    ob::ProfScope ps_("fft_rows_r2c");
-    hipLaunchKernelGGL(fft_rows_r2c, dim3((c->nx + 1) / 2), dim3(kFFTThreads), fft_lds(c->M1), s,
-                       c->rho, c->nx, c->ny, c->ny, c->M1, c->spec, c->tw, c->nmax, st);
-  }
-  {
-    ob::ProfScope ps_("fft_cols");
-    hipLaunchKernelGGL(fft_cols<false>, dim3(c->H), dim3(kFFTThreads), fft_lds(c->M0), s, c->spec,
-                       c->nx, c->M0, c->H, c->ghat_t, c->tw, c->nmax, st,
-                       (double)c->M0 * (double)c->M1);
-  }
+  launch_rows_r2c(c, c->rho, c->nx, c->ny, c->ny, st);
+  launch_cols<false>(c, c->nx, st, (double)c->M0 * (double)c->M1);
   const double denom = 1.0 / (c->M0 * c->M1);
-  {
-    ob::ProfScope ps_("fft_rows_c2r");
-    hipLaunchKernelGGL(fft_rows_c2r, dim3((c->nx + 1) / 2), dim3(kFFTThreads), fft_lds(c->M1), s,
-                       c->spec, c->nx, c->M1, c->phi, c->ny, c->ny, denom, c->tw, c->nmax, st);
-  }
+  dispatch_pow2(c->M1, [&](auto NC) {
+    constexpr int N = decltype(NC)::value;
+    if constexpr (N >= 8) {
+      static bool ready = false;
+      if (!ready) {
+        allow_lds(fft_rows_c2r<N>, fft_lds<N>());
+        ready = true;
+      }
+      ob::ProfScope ps_("fft_rows_c2r");
+      hipLaunchKernelGGL(fft_rows_c2r<N>, dim3((c->nx + 1) / 2), dim3(kFFTThreads), fft_lds<N>(),
+                         stream(), c->spec, c->nx, c->phi, c->ny, c->ny, denom, c->tw, c->nmax,
+                         st);
+    }
+  });
   OB_HIP(hipGetLastError());
 }
 
@@ -466,18 +500,10 @@ void define_green(ob_sc2p5drb* c) {
     hipLaunchKernelGGL(green2d_fill, dim3((tot + 255) / 256), dim3(256), 0, s, G, c->M0, c->M1,
                        c->dx_solver, c->dy_solver);
   }
-  // full transform: all M0 rows are non-zero
-  {
-    ob::ProfScope ps_("fft_rows_r2c");
-    hipLaunchKernelGGL(fft_rows_r2c, dim3((c->M0 + 1) / 2), dim3(kFFTThreads), fft_lds(c->M1), s,
-                       G, c->M0, c->M1, c->M1, c->M1, c->spec, c->tw, c->nmax, nullptr);
-  }
-  {
-    ob::ProfScope ps_("fft_cols_green");
-    hipLaunchKernelGGL(fft_cols<true>, dim3(c->H), dim3(kFFTThreads), fft_lds(c->M0), s, c->spec,
-                       c->M0, c->M0, c->H, c->ghat_t, c->tw, c->nmax, nullptr, 0.0);
-  }
   OB_HIP(hipGetLastError());
+  // full transform: all M0 rows are non-zero
+  launch_rows_r2c(c, G, c->M0, c->M1, c->M1, nullptr);
+  launch_cols<true>(c, c->M0, nullptr, 0.0);
   OB_HIP(hipStreamSynchronize(s));
   OB_HIP(hipFree(G));
 }

@@ -28,10 +28,12 @@ namespace ob {
 namespace {
 
 constexpr int kThreads = 256;
-constexpr int kZPairs = 4;      // complex z-lines (= 8 real lines) per block in the z passes
-constexpr int kTileLines = 8;   // contiguous kz lines per block in the y and x passes
+// Complex lines (= pairs of real z-lines) per block in the z passes.
+__host__ __device__ constexpr int zpass_pairs(int n) { return n >= 256 ? (n >= 2048 ? 1 : 2048 / n) : 8; }
+// Consecutive kz lines per block in the y and x passes (LDS <= 131 KiB).
+__host__ __device__ constexpr int strided_lines(int n) { return n >= 2048 ? 4 : 8; }
 
-enum StridedMode { FWD = 0, INV = 1, CONV = 2, GREEN_OUT = 3 };
+enum StridedMode { FWD = 0, INV = 1, CONV = 2 };
 
 struct GreenGeom {
   int M0, M1, M2;
@@ -136,176 +138,215 @@ __global__ __launch_bounds__(kThreads) void deposit3d(DeviceBunchView v, Grid3 g
 }
 
 // ---- z pass r2c: two real z-lines per complex FFT -------------------------------
-// Real line q = i*NY + j (i < rows, j < NY) holds nz_in values (rest zero-padded to M2).
-template <bool GREEN>
+// Real line q = i*NY + j (q < nlines) holds nz_in values, zero-padded to N = M2.  The
+// half spectrum kz <= N/2 of line q goes to spec[(i*M1 + j)*H2p + kz].
+template <int N>
 __global__ __launch_bounds__(kThreads) void fft3_z_r2c(const double* __restrict__ rho,
-                                                       int64_t nlines, int NY, int nz_in, int M2,
-                                                       int M1, double2* __restrict__ spec,
-                                                       const double2* __restrict__ tw, int nmax,
-                                                       GreenGeom gg) {
+                                                       int64_t nlines, int NY, int nz_in, int M1,
+                                                       int H2p, double2* __restrict__ spec,
+                                                       const double2* __restrict__ tw, int nmax) {
+  constexpr int P = zpass_pairs(N), ls = N + 1, H2 = N / 2 + 1;
   extern __shared__ double sm[];
-  const int ls = M2 + 1;
   double* re = sm;
-  double* im = sm + kZPairs * ls;
-  const int log2n = ilog2(M2);
-  const int H2 = M2 / 2 + 1;
-  const int64_t pair0 = (int64_t)blockIdx.x * kZPairs;
-  for (int idx = threadIdx.x; idx < kZPairs * M2; idx += blockDim.x) {
-    const int lp = idx / M2, k = idx - lp * M2;
+  double* im = sm + P * ls;
+  const int64_t pair0 = (int64_t)blockIdx.x * P;
+  for (int idx = threadIdx.x; idx < P * N; idx += kThreads) {
+    const int lp = idx / N, k = idx - lp * N;
     const int64_t q0 = 2 * (pair0 + lp), q1 = q0 + 1;
     double v0 = 0.0, v1 = 0.0;
     if (k < nz_in) {
-      if (GREEN) {
-        if (q0 < nlines) v0 = green3d((int)(q0 / NY), (int)(q0 % NY), k, gg);
-        if (q1 < nlines) v1 = green3d((int)(q1 / NY), (int)(q1 % NY), k, gg);
-      } else {
-        if (q0 < nlines) v0 = rho[q0 * nz_in + k];
-        if (q1 < nlines) v1 = rho[q1 * nz_in + k];
-      }
+      if (q0 < nlines) v0 = rho[q0 * nz_in + k];
+      if (q1 < nlines) v1 = rho[q1 * nz_in + k];
     }
-    const int p = bitrev(k, log2n);
-    re[lp * ls + p] = v0;
-    im[lp * ls + p] = v1;
+    re[lp * ls + k] = v0;
+    im[lp * ls + k] = v1;
   }
   __syncthreads();
-  lds_fft_dit(re, im, M2, log2n, kZPairs, tw, nmax, -1.0, ls);
-  for (int idx = threadIdx.x; idx < kZPairs * H2; idx += blockDim.x) {
+  lds_fft<N, P, kThreads, -1>(re, im, P, ls, tw, nmax);
+  for (int idx = threadIdx.x; idx < P * H2; idx += kThreads) {
     const int lp = idx / H2, k = idx - lp * H2;
     const int64_t q0 = 2 * (pair0 + lp), q1 = q0 + 1;
     if (q0 >= nlines) continue;
-    const int nk = (M2 - k) & (M2 - 1);
+    const int nk = (N - k) & (N - 1);
     const double zr = re[lp * ls + k], zi = im[lp * ls + k];
     const double nr = re[lp * ls + nk], ni = im[lp * ls + nk];
-    const int64_t i0 = q0 / NY, j0 = q0 % NY;
-    spec[(i0 * M1 + j0) * H2 + k] = make_double2((zr + nr) * 0.5, (zi - ni) * 0.5);
+    const int64_t i0 = q0 / NY, j0 = q0 - i0 * NY;
+    spec[(i0 * M1 + j0) * H2p + k] = make_double2((zr + nr) * 0.5, (zi - ni) * 0.5);
     if (q1 < nlines) {
-      const int64_t i1 = q1 / NY, j1 = q1 % NY;
-      spec[(i1 * M1 + j1) * H2 + k] = make_double2((zi + ni) * 0.5, -(zr - nr) * 0.5);
+      const int64_t i1 = q1 / NY, j1 = q1 - i1 * NY;
+      spec[(i1 * M1 + j1) * H2p + k] = make_double2((zi + ni) * 0.5, -(zr - nr) * 0.5);
     }
   }
 }
 
-// ---- z pass c2r: two complex half-lines per complex inverse FFT ------------------
+// ---- z pass c2r: two half spectra per complex inverse FFT ------------------------
+template <int N>
 __global__ __launch_bounds__(kThreads) void fft3_z_c2r(const double2* __restrict__ spec,
-                                                       int64_t nlines, int NY, int M2, int M1,
+                                                       int64_t nlines, int NY, int M1, int H2p,
                                                        int nz_out, double denom,
                                                        double* __restrict__ phi,
                                                        const double2* __restrict__ tw, int nmax) {
+  constexpr int P = zpass_pairs(N), ls = N + 1, H2 = N / 2 + 1;
   extern __shared__ double sm[];
-  const int ls = M2 + 1;
   double* re = sm;
-  double* im = sm + kZPairs * ls;
-  const int log2n = ilog2(M2);
-  const int H2 = M2 / 2 + 1;
-  const int64_t pair0 = (int64_t)blockIdx.x * kZPairs;
-  for (int idx = threadIdx.x; idx < kZPairs * M2; idx += blockDim.x) {
-    const int lp = idx / M2, k = idx - lp * M2;
+  double* im = sm + P * ls;
+  const int64_t pair0 = (int64_t)blockIdx.x * P;
+  for (int idx = threadIdx.x; idx < P * H2; idx += kThreads) {
+    const int lp = idx / H2, k = idx - lp * H2;
     const int64_t q0 = 2 * (pair0 + lp), q1 = q0 + 1;
-    const int kk = (k <= M2 / 2) ? k : M2 - k;
     double2 a = make_double2(0.0, 0.0), b = make_double2(0.0, 0.0);
-    if (q0 < nlines) a = spec[((q0 / NY) * M1 + (q0 % NY)) * H2 + kk];
-    if (q1 < nlines) b = spec[((q1 / NY) * M1 + (q1 % NY)) * H2 + kk];
-    if (kk == 0 || kk == M2 / 2) {  // c2r ignores the imaginary DC/Nyquist parts
+    if (q0 < nlines) {
+      const int64_t i0 = q0 / NY;
+      a = spec[(i0 * M1 + (q0 - i0 * NY)) * H2p + k];
+    }
+    if (q1 < nlines) {
+      const int64_t i1 = q1 / NY;
+      b = spec[(i1 * M1 + (q1 - i1 * NY)) * H2p + k];
+    }
+    if (k == 0 || k == N / 2) {  // c2r ignores the imaginary DC/Nyquist parts
       a.y = 0.0;
       b.y = 0.0;
     }
-    double zr, zi;
-    if (k <= M2 / 2) {
-      zr = a.x - b.y;
-      zi = a.y + b.x;
-    } else {
-      zr = a.x + b.y;
-      zi = b.x - a.y;
+    // Z = X0 + i X1 with X(N-k) = conj(X(k))
+    re[lp * ls + k] = a.x - b.y;
+    im[lp * ls + k] = a.y + b.x;
+    if (k > 0 && k < N / 2) {
+      re[lp * ls + N - k] = a.x + b.y;
+      im[lp * ls + N - k] = b.x - a.y;
     }
-    re[lp * ls + k] = zr;
-    im[lp * ls + k] = zi;
   }
   __syncthreads();
-  lds_fft_dif(re, im, M2, log2n, kZPairs, tw, nmax, +1.0, ls);
-  for (int idx = threadIdx.x; idx < kZPairs * nz_out; idx += blockDim.x) {
+  lds_fft<N, P, kThreads, +1>(re, im, P, ls, tw, nmax);
+  for (int idx = threadIdx.x; idx < P * nz_out; idx += kThreads) {
     const int lp = idx / nz_out, k = idx - lp * nz_out;
     const int64_t q0 = 2 * (pair0 + lp), q1 = q0 + 1;
-    const int p = bitrev(k, log2n);
-    if (q0 < nlines) phi[q0 * nz_out + k] = denom * re[lp * ls + p];
-    if (q1 < nlines) phi[q1 * nz_out + k] = denom * im[lp * ls + p];
+    if (q0 < nlines) phi[q0 * nz_out + k] = denom * re[lp * ls + k];
+    if (q1 < nlines) phi[q1 * nz_out + k] = denom * im[lp * ls + k];
   }
 }
 
 // ---- strided passes (y and x axes) -----------------------------------------------
-// Block = (outer index o, tile of up to kTileLines consecutive kz).  Element p of line
-// (o, kz) is spec[o*outer_stride + p*line_stride + kz].
-template <int MODE>
-__global__ __launch_bounds__(kThreads) void fft3_strided(double2* __restrict__ spec, int n,
-                                                         int n_in, int n_out, int64_t line_stride,
+// Block = (outer index o, tile of up to L consecutive kz).  Element p of line (o, kz) is
+// spec[o*outer_stride + p*line_stride + kz]; a tile row is L*16 contiguous bytes, aligned
+// because H2p is a multiple of 8.  Inputs p >= n_in are zero; outputs p < n_out are written.
+template <int MODE, int N>
+__global__ __launch_bounds__(kThreads) void fft3_strided(double2* __restrict__ spec, int n_in,
+                                                         int n_out, int64_t line_stride,
                                                          int64_t outer_stride, int H2,
-                                                         double* __restrict__ ghat, int qy,
-                                                         int M0, int M1,
+                                                         const double* __restrict__ ghat, int qy,
+                                                         int H2p, int M1,
                                                          const double2* __restrict__ tw,
-                                                         int nmax, int tile) {
+                                                         int nmax) {
+  constexpr int L = strided_lines(N), ls = N + 1;
   extern __shared__ double sm[];
-  const int ls = n + 1;
   double* re = sm;
-  double* im = sm + tile * ls;
-  const int log2n = ilog2(n);
-  const int tiles = (H2 + tile - 1) / tile;
+  double* im = sm + L * ls;
+  const int tiles = (H2 + L - 1) / L;
   const int o = blockIdx.x / tiles;
-  const int k0 = (blockIdx.x - o * tiles) * tile;
-  const int nl = min(tile, H2 - k0);
+  const int k0 = (blockIdx.x - o * tiles) * L;
+  const int nl = min(L, H2 - k0);
   double2* base = spec + (int64_t)o * outer_stride + k0;
-  for (int idx = threadIdx.x; idx < nl * n; idx += blockDim.x) {
-    const int p = idx / nl, l = idx - p * nl;
-    const double2 v = (p < n_in) ? base[(int64_t)p * line_stride + l] : make_double2(0.0, 0.0);
-    const int pos = (MODE == INV) ? p : bitrev(p, log2n);
-    re[l * ls + pos] = v.x;
-    im[l * ls + pos] = v.y;
+  for (int idx = threadIdx.x; idx < L * N; idx += kThreads) {
+    const int p = idx / L, l = idx - p * L;
+    double2 v = make_double2(0.0, 0.0);
+    if (p < n_in && l < nl) v = base[(int64_t)p * line_stride + l];
+    re[l * ls + p] = v.x;
+    im[l * ls + p] = v.y;
   }
   __syncthreads();
-  if (MODE == INV) {
-    lds_fft_dif(re, im, n, log2n, nl, tw, nmax, +1.0, ls);
-  } else {
-    lds_fft_dit(re, im, n, log2n, nl, tw, nmax, -1.0, ls);
-  }
-  if (MODE == GREEN_OUT) {  // x pass of the Green transform: keep the real quadrant
-    if (o > M1 / 2) return;
-    for (int idx = threadIdx.x; idx < nl * (M0 / 2 + 1); idx += blockDim.x) {
-      const int p = idx / nl, l = idx - p * nl;
-      ghat[((int64_t)p * qy + o) * H2 + k0 + l] = re[l * ls + p];
-    }
-    return;
-  }
-  if (MODE == CONV) {  // x pass: multiply by the Green spectrum, inverse x FFT
+  if constexpr (MODE == INV)
+    lds_fft<N, L, kThreads, +1>(re, im, nl, ls, tw, nmax);
+  else
+    lds_fft<N, L, kThreads, -1>(re, im, nl, ls, tw, nmax);
+  if constexpr (MODE == CONV) {  // x pass: multiply by the Green spectrum, inverse x FFT
     const int gy = (o <= M1 / 2) ? o : M1 - o;
-    for (int idx = threadIdx.x; idx < nl * n; idx += blockDim.x) {
-      const int p = idx / nl, l = idx - p * nl;
-      const int gx = (p <= M0 / 2) ? p : M0 - p;
-      const double g = ghat[((int64_t)gx * qy + gy) * H2 + k0 + l];
-      re[l * ls + p] *= g;
-      im[l * ls + p] *= g;
+    for (int idx = threadIdx.x; idx < L * N; idx += kThreads) {
+      const int p = idx / L, l = idx - p * L;
+      if (l < nl) {
+        const int gx = (p <= N / 2) ? p : N - p;
+        const double g = ghat[((int64_t)gx * qy + gy) * H2p + k0 + l];
+        re[l * ls + p] *= g;
+        im[l * ls + p] *= g;
+      }
     }
     __syncthreads();
-    lds_fft_dif(re, im, n, log2n, nl, tw, nmax, +1.0, ls);
+    lds_fft<N, L, kThreads, +1>(re, im, nl, ls, tw, nmax);
   }
-  for (int idx = threadIdx.x; idx < nl * n_out; idx += blockDim.x) {
-    const int p = idx / nl, l = idx - p * nl;
-    const int pos = (MODE == FWD) ? p : bitrev(p, log2n);
-    base[(int64_t)p * line_stride + l] = make_double2(re[l * ls + pos], im[l * ls + pos]);
+  for (int idx = threadIdx.x; idx < L * n_out; idx += kThreads) {
+    const int p = idx / L, l = idx - p * L;
+    if (l < nl) base[(int64_t)p * line_stride + l] = make_double2(re[l * ls + p], im[l * ls + p]);
   }
 }
 
-size_t zpass_lds(int M2) { return sizeof(double) * 2 * kZPairs * (M2 + 1); }
-// Lines per block for the strided passes: up to kTileLines, within 144 KiB of LDS.
-int tile_lines(int n) {
-  int t = (144 * 1024) / (int)(sizeof(double) * 2 * (n + 1));
-  return t < 1 ? 1 : (t > kTileLines ? kTileLines : t);
+// ---- Green spectrum (PoissonSolverFFT3D.cc:160-247) ----------------------------------
+// 1/r on the padded grid is real and even along every axis, so its spectrum is real and
+// even: it is the product of three even-symmetric transforms over the quadrant
+// i <= M0/2, j <= M1/2, k <= M2/2.  Two real even lines share one complex FFT: the
+// transform of a + i b is A + i B with A, B real.  ghat[(i*qy + j)*H2p + k].
+template <int N>  // N = M2
+__global__ __launch_bounds__(kThreads) void green3d_z(int qx, int qy, int H2p,
+                                                      double* __restrict__ ghat, GreenGeom gg,
+                                                      const double2* __restrict__ tw, int nmax) {
+  constexpr int P = zpass_pairs(N), ls = N + 1, H2 = N / 2 + 1;
+  extern __shared__ double sm[];
+  double* re = sm;
+  double* im = sm + P * ls;
+  const int64_t nlines = (int64_t)qx * qy;
+  const int64_t pair0 = (int64_t)blockIdx.x * P;
+  for (int idx = threadIdx.x; idx < P * N; idx += kThreads) {
+    const int lp = idx / N, k = idx - lp * N;
+    const int64_t q0 = 2 * (pair0 + lp), q1 = q0 + 1;
+    re[lp * ls + k] = q0 < nlines ? green3d((int)(q0 / qy), (int)(q0 % qy), k, gg) : 0.0;
+    im[lp * ls + k] = q1 < nlines ? green3d((int)(q1 / qy), (int)(q1 % qy), k, gg) : 0.0;
+  }
+  __syncthreads();
+  lds_fft<N, P, kThreads, -1>(re, im, P, ls, tw, nmax);
+  for (int idx = threadIdx.x; idx < P * H2; idx += kThreads) {
+    const int lp = idx / H2, k = idx - lp * H2;
+    const int64_t q0 = 2 * (pair0 + lp), q1 = q0 + 1;
+    if (q0 < nlines) ghat[q0 * H2p + k] = re[lp * ls + k];
+    if (q1 < nlines) ghat[q1 * H2p + k] = im[lp * ls + k];
+  }
 }
-size_t strided_lds(int n) { return sizeof(double) * 2 * tile_lines(n) * (n + 1); }
-constexpr int kMaxDynLDS = 160 * 1024;
+
+// In-place even-symmetric transform along one axis of the real quadrant array: line
+// (o, kz) holds N/2+1 samples at ghat[o*outer_stride + p*line_stride + kz], extended
+// evenly to N points; its first N/2+1 spectral values replace them.
+template <int N>
+__global__ __launch_bounds__(kThreads) void green3d_axis(double* __restrict__ ghat,
+                                                         int64_t line_stride,
+                                                         int64_t outer_stride, int H2,
+                                                         const double2* __restrict__ tw,
+                                                         int nmax) {
+  constexpr int L = strided_lines(N), C = L / 2, ls = N + 1, Q = N / 2 + 1;
+  extern __shared__ double sm[];
+  double* re = sm;
+  double* im = sm + C * ls;
+  const int tiles = (H2 + L - 1) / L;
+  const int o = blockIdx.x / tiles;
+  const int k0 = (blockIdx.x - o * tiles) * L;
+  const int nl = min(L, H2 - k0);
+  double* base = ghat + (int64_t)o * outer_stride + k0;
+  for (int idx = threadIdx.x; idx < L * Q; idx += kThreads) {
+    const int p = idx / L, l = idx - p * L;
+    const double v = (l < nl) ? base[(int64_t)p * line_stride + l] : 0.0;
+    double* d = ((l & 1) ? im : re) + (l >> 1) * ls;
+    d[p] = v;
+    if (p > 0 && p < N / 2) d[N - p] = v;
+  }
+  __syncthreads();
+  lds_fft<N, C, kThreads, -1>(re, im, (nl + 1) >> 1, ls, tw, nmax);
+  for (int idx = threadIdx.x; idx < L * Q; idx += kThreads) {
+    const int p = idx / L, l = idx - p * L;
+    if (l < nl) base[(int64_t)p * line_stride + l] = ((l & 1) ? im : re)[(l >> 1) * ls + p];
+  }
+}
 
 }  // namespace
 }  // namespace ob
 
 struct ob_sc3d {
-  int nx, ny, nz, M0, M1, M2, H2, qx, qy, nmax;
+  int nx, ny, nz, M0, M1, M2, H2, H2p, qx, qy, nmax;
   double ratio_limit = 1.2, xy_ratio = 1.0, xz_ratio = 1.0;
   double sMin[3], sMax[3], sd[3];  // solver grid limits and steps (PoissonSolver3D.cc)
   double g[9];                     // rho/phi grid: xMin,xMax,yMin,yMax,zMin,zMax (rest), dx,dy,dz
@@ -321,47 +362,46 @@ struct ob_sc3d {
 namespace ob {
 namespace {
 
-void launch_strided(ob_sc3d* c, int mode, int n, int n_in, int n_out, int64_t line_stride,
-                    int64_t outer_stride, int outers) {
-  const int tile = tile_lines(n);
-  const int tiles = (c->H2 + tile - 1) / tile;
-  const dim3 grid(outers * tiles);
-  const size_t lds = strided_lds(n);
-  hipStream_t s = stream();
-  switch (mode) {
-    case FWD:
-      {
-        ob::ProfScope ps_("fft3_strided_fwd");
-        hipLaunchKernelGGL(fft3_strided<FWD>, grid, dim3(kThreads), lds, s, c->spec, n, n_in, n_out,
-                           line_stride, outer_stride, c->H2, c->ghat, c->qy, c->M0, c->M1, c->tw,
-                           c->nmax, tile);
-      }
-      break;
-    case INV:
-      {
-        ob::ProfScope ps_("fft3_strided_inv");
-        hipLaunchKernelGGL(fft3_strided<INV>, grid, dim3(kThreads), lds, s, c->spec, n, n_in, n_out,
-                           line_stride, outer_stride, c->H2, c->ghat, c->qy, c->M0, c->M1, c->tw,
-                           c->nmax, tile);
-      }
-      break;
-    case CONV:
-      {
-        ob::ProfScope ps_("fft3_strided_conv");
-        hipLaunchKernelGGL(fft3_strided<CONV>, grid, dim3(kThreads), lds, s, c->spec, n, n_in, n_out,
-                           line_stride, outer_stride, c->H2, c->ghat, c->qy, c->M0, c->M1, c->tw,
-                           c->nmax, tile);
-      }
-      break;
-    default:
-      {
-        ob::ProfScope ps_("fft3_strided_greenout");
-        hipLaunchKernelGGL(fft3_strided<GREEN_OUT>, grid, dim3(kThreads), lds, s, c->spec, n, n_in,
-                           n_out, line_stride, outer_stride, c->H2, c->ghat, c->qy, c->M0, c->M1,
-                           c->tw, c->nmax, tile);
-      }
+// Dynamic LDS above 64 KiB has to be allowed per kernel; done once per instantiation.
+template <typename K>
+void allow_lds(K* kernel, size_t bytes) {
+  if (bytes > 64 * 1024)
+    OB_HIP(hipFuncSetAttribute((const void*)kernel, hipFuncAttributeMaxDynamicSharedMemorySize,
+                               (int)bytes));
+}
+
+template <int N>
+size_t zpass_lds() { return sizeof(double) * 2 * zpass_pairs(N) * (N + 1); }
+template <int N>
+size_t strided_lds() { return sizeof(double) * 2 * strided_lines(N) * (N + 1); }
+
+template <int MODE, int N>
+void launch_strided_n(ob_sc3d* c, int n_in, int n_out, int64_t line_stride,
+                      int64_t outer_stride, int outers, const char* label) {
+  static bool ready = false;
+  const size_t lds = strided_lds<N>();
+  if (!ready) {
+    allow_lds(fft3_strided<MODE, N>, lds);
+    ready = true;
   }
+  const int tiles = (c->H2 + strided_lines(N) - 1) / strided_lines(N);
+  ob::ProfScope ps_(label);
+  hipLaunchKernelGGL((fft3_strided<MODE, N>), dim3(outers * tiles), dim3(kThreads), lds,
+                     stream(), c->spec, n_in, n_out, line_stride, outer_stride, c->H2, c->ghat,
+                     c->qy, c->H2p, c->M1, c->tw, c->nmax);
   OB_HIP(hipGetLastError());
+}
+
+template <int MODE>
+void launch_strided(ob_sc3d* c, int n, int n_in, int n_out, int64_t line_stride,
+                    int64_t outer_stride, int outers, const char* label) {
+  dispatch_pow2(n, [&](auto NC) {
+    constexpr int N = decltype(NC)::value;
+    if constexpr (N >= 8 && N <= 2048)
+      launch_strided_n<MODE, N>(c, n_in, n_out, line_stride, outer_stride, outers, label);
+    else
+      throw Error{"SpaceChargeCalc3D: transverse FFT length out of range"};
+  });
 }
 
 // PoissonSolverFFT3D::setGridXYZ + _defineGreenF (:136-247)
@@ -372,40 +412,88 @@ void set_solver_grid(ob_sc3d* c, double xMin, double xMax, double yMin, double y
   c->sMin[2] = zMin; c->sMax[2] = zMax; c->sd[2] = (zMax - zMin) / c->nz;
   GreenGeom gg{c->M0, c->M1, c->M2, c->sd[0], c->sd[1], c->sd[2]};
   hipStream_t s = stream();
-  const int64_t nlines = (int64_t)c->M0 * c->M1;
-  const int64_t blocks = (nlines / 2 + kZPairs - 1) / kZPairs;
-  {
-    ob::ProfScope ps_("fft3_z_r2c_green");
-    hipLaunchKernelGGL(fft3_z_r2c<true>, dim3((unsigned)blocks), dim3(kThreads), zpass_lds(c->M2),
-                       s, nullptr, nlines, c->M1, c->M2, c->M2, c->M1, c->spec, c->tw, c->nmax, gg);
-  }
+  dispatch_pow2(c->M2, [&](auto NC) {
+    constexpr int N = decltype(NC)::value;
+    if constexpr (N <= 2048) {
+      static bool ready = false;
+      const size_t lds = zpass_lds<N>();
+      if (!ready) {
+        allow_lds(green3d_z<N>, lds);
+        ready = true;
+      }
+      const int64_t pairs = ((int64_t)c->qx * c->qy + 1) / 2;
+      const int64_t blocks = (pairs + zpass_pairs(N) - 1) / zpass_pairs(N);
+      ob::ProfScope ps_("green3d_z");
+      hipLaunchKernelGGL(green3d_z<N>, dim3((unsigned)blocks), dim3(kThreads), lds, s, c->qx,
+                         c->qy, c->H2p, c->ghat, gg, c->tw, c->nmax);
+    }
+  });
   OB_HIP(hipGetLastError());
-  const int64_t H2 = c->H2;
-  launch_strided(c, FWD, c->M1, c->M1, c->M1, H2, (int64_t)c->M1 * H2, c->M0);
-  launch_strided(c, GREEN_OUT, c->M0, c->M0, 0, (int64_t)c->M1 * H2, H2, c->M1 / 2 + 1);
+  auto axis = [&](int n, int64_t line_stride, int64_t outer_stride, int outers) {
+    dispatch_pow2(n, [&](auto NC) {
+      constexpr int N = decltype(NC)::value;
+      if constexpr (N >= 8 && N <= 2048) {
+        static bool ready = false;
+        const size_t lds = sizeof(double) * 2 * (strided_lines(N) / 2) * (N + 1);
+        if (!ready) {
+          allow_lds(green3d_axis<N>, lds);
+          ready = true;
+        }
+        const int tiles = (c->H2 + strided_lines(N) - 1) / strided_lines(N);
+        ob::ProfScope ps_("green3d_axis");
+        hipLaunchKernelGGL(green3d_axis<N>, dim3(outers * tiles), dim3(kThreads), lds, s,
+                           c->ghat, line_stride, outer_stride, c->H2, c->tw, c->nmax);
+      }
+    });
+    OB_HIP(hipGetLastError());
+  };
+  const int64_t H2p = c->H2p;
+  axis(c->M1, H2p, (int64_t)c->qy * H2p, c->qx);  // y: lines (i, kz)
+  axis(c->M0, (int64_t)c->qy * H2p, H2p, c->qy);  // x: lines (j, kz)
   c->rebuilds++;
 }
 
 void solve(ob_sc3d* c, double denom) {
   hipStream_t s = stream();
   const int64_t nl = (int64_t)c->nx * c->ny;
-  const int64_t blocks = ((nl + 1) / 2 + kZPairs - 1) / kZPairs;
-  GreenGeom gg{};
-  {
-    ob::ProfScope ps_("fft3_z_r2c");
-    hipLaunchKernelGGL(fft3_z_r2c<false>, dim3((unsigned)blocks), dim3(kThreads), zpass_lds(c->M2),
-                       s, c->rho, nl, c->ny, c->nz, c->M2, c->M1, c->spec, c->tw, c->nmax, gg);
-  }
+  const int64_t H2p = c->H2p;
+  dispatch_pow2(c->M2, [&](auto NC) {
+    constexpr int N = decltype(NC)::value;
+    if constexpr (N <= 2048) {
+      static bool ready = false;
+      const size_t lds = zpass_lds<N>();
+      if (!ready) {
+        allow_lds(fft3_z_r2c<N>, lds);
+        ready = true;
+      }
+      const int64_t blocks = ((nl + 1) / 2 + zpass_pairs(N) - 1) / zpass_pairs(N);
+      ob::ProfScope ps_("fft3_z_r2c");
+      hipLaunchKernelGGL(fft3_z_r2c<N>, dim3((unsigned)blocks), dim3(kThreads), lds, s, c->rho,
+                         nl, c->ny, c->nz, c->M1, c->H2p, c->spec, c->tw, c->nmax);
+    }
+  });
   OB_HIP(hipGetLastError());
-  const int64_t H2 = c->H2;
-  launch_strided(c, FWD, c->M1, c->ny, c->M1, H2, (int64_t)c->M1 * H2, c->nx);
-  launch_strided(c, CONV, c->M0, c->nx, c->nx, (int64_t)c->M1 * H2, H2, c->M1);
-  launch_strided(c, INV, c->M1, c->M1, c->ny, H2, (int64_t)c->M1 * H2, c->nx);
-  {
-    ob::ProfScope ps_("fft3_z_c2r");
-    hipLaunchKernelGGL(fft3_z_c2r, dim3((unsigned)blocks), dim3(kThreads), zpass_lds(c->M2), s,
-                       c->spec, nl, c->ny, c->M2, c->M1, c->nz, denom, c->phi, c->tw, c->nmax);
-  }
+  launch_strided<FWD>(c, c->M1, c->ny, c->M1, H2p, (int64_t)c->M1 * H2p, c->nx,
+                      "fft3_strided_fwd");
+  launch_strided<CONV>(c, c->M0, c->nx, c->nx, (int64_t)c->M1 * H2p, H2p, c->M1,
+                       "fft3_strided_conv");
+  launch_strided<INV>(c, c->M1, c->M1, c->ny, H2p, (int64_t)c->M1 * H2p, c->nx,
+                      "fft3_strided_inv");
+  dispatch_pow2(c->M2, [&](auto NC) {
+    constexpr int N = decltype(NC)::value;
+    if constexpr (N <= 2048) {
+      static bool ready = false;
+      const size_t lds = zpass_lds<N>();
+      if (!ready) {
+        allow_lds(fft3_z_c2r<N>, lds);
+        ready = true;
+      }
+      const int64_t blocks = ((nl + 1) / 2 + zpass_pairs(N) - 1) / zpass_pairs(N);
+      ob::ProfScope ps_("fft3_z_c2r");
+      hipLaunchKernelGGL(fft3_z_c2r<N>, dim3((unsigned)blocks), dim3(kThreads), lds, s, c->spec,
+                         nl, c->ny, c->M1, c->H2p, c->nz, denom, c->phi, c->tw, c->nmax);
+    }
+  });
   OB_HIP(hipGetLastError());
 }
 
@@ -437,34 +525,24 @@ OB_API int ob_sc3d_create(int nx, int ny, int nz, ob_sc3d** out) {
     throw ob::Error{"SpaceChargeCalc3D: grid larger than 1024 points per axis"};
   }
   c->H2 = c->M2 / 2 + 1;
+  c->H2p = (c->H2 + 7) & ~7;  // kz rows padded to 8 complex: 128-byte aligned tile rows
   c->qx = c->M0 / 2 + 1;
   c->qy = c->M1 / 2 + 1;
   c->nmax = std::max(c->M0, std::max(c->M1, c->M2));
   const size_t ng = (size_t)nx * ny * nz;
   OB_HIP(hipMalloc((void**)&c->rho, sizeof(double) * ng));
   OB_HIP(hipMalloc((void**)&c->phi, sizeof(double) * ng));
-  OB_HIP(hipMalloc((void**)&c->spec, sizeof(double2) * (size_t)c->M0 * c->M1 * c->H2));
-  OB_HIP(hipMalloc((void**)&c->ghat, sizeof(double) * (size_t)c->qx * c->qy * c->H2));
+  // spectrum after the y pass: x < nx only (the x pass reads and writes just those planes)
+  OB_HIP(hipMalloc((void**)&c->spec, sizeof(double2) * (size_t)nx * c->M1 * c->H2p));
+  OB_HIP(hipMalloc((void**)&c->ghat, sizeof(double) * (size_t)c->qx * c->qy * c->H2p));
   OB_HIP(hipMalloc((void**)&c->partials, sizeof(double) * 6 * 4096));
   OB_HIP(hipMalloc((void**)&c->ext_dev, sizeof(double) * 6));
   OB_HIP(hipHostMalloc((void**)&c->ext_host, sizeof(double) * 6, hipHostMallocDefault));
   OB_HIP(hipMemset(c->rho, 0, sizeof(double) * ng));
   OB_HIP(hipMemset(c->phi, 0, sizeof(double) * ng));
+  OB_HIP(hipMemset(c->spec, 0, sizeof(double2) * (size_t)nx * c->M1 * c->H2p));
+  OB_HIP(hipMemset(c->ghat, 0, sizeof(double) * (size_t)c->qx * c->qy * c->H2p));
   c->tw = ob::make_twiddles(c->nmax);
-  OB_HIP(hipFuncSetAttribute((const void*)ob::fft3_strided<ob::FWD>,
-                             hipFuncAttributeMaxDynamicSharedMemorySize, ob::kMaxDynLDS));
-  OB_HIP(hipFuncSetAttribute((const void*)ob::fft3_strided<ob::INV>,
-                             hipFuncAttributeMaxDynamicSharedMemorySize, ob::kMaxDynLDS));
-  OB_HIP(hipFuncSetAttribute((const void*)ob::fft3_strided<ob::CONV>,
-                             hipFuncAttributeMaxDynamicSharedMemorySize, ob::kMaxDynLDS));
-  OB_HIP(hipFuncSetAttribute((const void*)ob::fft3_strided<ob::GREEN_OUT>,
-                             hipFuncAttributeMaxDynamicSharedMemorySize, ob::kMaxDynLDS));
-  OB_HIP(hipFuncSetAttribute((const void*)ob::fft3_z_r2c<true>,
-                             hipFuncAttributeMaxDynamicSharedMemorySize, ob::kMaxDynLDS));
-  OB_HIP(hipFuncSetAttribute((const void*)ob::fft3_z_r2c<false>,
-                             hipFuncAttributeMaxDynamicSharedMemorySize, ob::kMaxDynLDS));
-  OB_HIP(hipFuncSetAttribute((const void*)ob::fft3_z_c2r,
-                             hipFuncAttributeMaxDynamicSharedMemorySize, ob::kMaxDynLDS));
   // SpaceChargeCalc3D(:32-44): solver on [-1,1]^3 (xy_ratio = xz_ratio = 1); rho/phi grids
   // take the solver limits.
   ob::set_solver_grid(c, -1.0, 1.0, -1.0, 1.0, -1.0, 1.0);

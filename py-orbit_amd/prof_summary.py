@@ -22,7 +22,7 @@ def short(name):
     n = n.split("(")[0]
     for pre in ("void ", "ob::"):
         n = n.replace(pre, "")
-    return n.strip()
+    return n.strip().replace(", ", ",")
 
 
 def kernel_rows(db):
