@@ -184,6 +184,182 @@ __device__ __forceinline__ void lds_fft(double* __restrict__ re, double* __restr
   fft_stages_from<N, 0, LMAX, NT, SIGN>(re, im, lines, ls, tw, nmax);
 }
 
+// ---- transforms with register-resident ends ------------------------------------------
+// A plan runs the radices of fft_radix in order (REV = false) or reversed (REV = true); any
+// order is a valid Stockham factorisation.  The first stage of a plan reads its butterflies
+// straight from a loader (global memory) and the last stage leaves its results in
+// registers, so only the middle stages go through LDS.  Thread t, butterfly q handles
+// b = t + q*NT as line = b % LMAX, j = b / LMAX (lines fastest: a wave reads LMAX lines x a
+// few consecutive points, which is contiguous when the lines are adjacent in memory).  In
+// the first stage it holds points p = j + m*N/R0; after the last stage it holds outputs
+// p = j + m*N/RL.  A forward plan's output registers are the input registers of the reversed
+// plan, so forward -> pointwise multiply -> inverse needs no LDS exchange in between.
+template <int N, bool REV>
+__host__ __device__ constexpr int plan_radix(int s) {
+  return REV ? fft_radix(N, fft_stages(N) - 1 - s) : fft_radix(N, s);
+}
+template <int N, bool REV>
+__host__ __device__ constexpr int plan_span(int s) {
+  return s == 0 ? 1 : plan_span<N, REV>(s - 1) * plan_radix<N, REV>(s - 1);
+}
+
+template <int N, bool REV, int LMAX, int NT>
+struct RegPlan {
+  static constexpr int S = fft_stages(N);
+  static constexpr int R0 = plan_radix<N, REV>(0), NB0 = N / R0;
+  static constexpr int B0 = (LMAX * NB0 + NT - 1) / NT;
+  static constexpr int RL = plan_radix<N, REV>(S - 1), NBL = N / RL;
+  static constexpr int BL = (LMAX * NBL + NT - 1) / NT;
+};
+
+// LDS -> LDS stage S of plan REV (threads j-fastest within a line).
+template <int N, int S, bool REV, int LMAX, int NT, int SIGN>
+__device__ __forceinline__ void plan_stage_ll(double* __restrict__ re, double* __restrict__ im,
+                                              int lines, int ls, const double2* __restrict__ tw,
+                                              int nmax) {
+  constexpr int R = plan_radix<N, REV>(S);
+  constexpr int Ns = plan_span<N, REV>(S);
+  constexpr int NB = N / R;
+  constexpr int BPT = (LMAX * NB + NT - 1) / NT;
+  const int total = lines * NB;
+  const int tstep = nmax / (Ns * R);
+  double vr[BPT][R], vi[BPT][R];
+#pragma unroll
+  for (int q = 0; q < BPT; q++) {
+    const int b = (int)threadIdx.x + q * NT;
+    if (b < total) {
+      const int line = b / NB, j = b - line * NB;
+      const int k = j & (Ns - 1);
+#pragma unroll
+      for (int m = 0; m < R; m++) {
+        vr[q][m] = re[line * ls + j + m * NB];
+        vi[q][m] = im[line * ls + j + m * NB];
+      }
+#pragma unroll
+      for (int m = 1; m < R; m++) {
+        const double2 w = tw[m * k * tstep];
+        const double wr = w.x, wi = SIGN < 0 ? w.y : -w.y;
+        const double a = vr[q][m], c = vi[q][m];
+        vr[q][m] = a * wr - c * wi;
+        vi[q][m] = a * wi + c * wr;
+      }
+      dft<R, SIGN>(vr[q], vi[q]);
+    }
+  }
+  __syncthreads();
+#pragma unroll
+  for (int q = 0; q < BPT; q++) {
+    const int b = (int)threadIdx.x + q * NT;
+    if (b < total) {
+      const int line = b / NB, j = b - line * NB;
+      const int k = j & (Ns - 1);
+#pragma unroll
+      for (int m = 0; m < R; m++) {
+        re[line * ls + (j - k) * R + k + m * Ns] = vr[q][m];
+        im[line * ls + (j - k) * R + k + m * Ns] = vi[q][m];
+      }
+    }
+  }
+  __syncthreads();
+}
+
+template <int N, int S, int SEND, bool REV, int LMAX, int NT, int SIGN>
+__device__ __forceinline__ void plan_stages_ll(double* re, double* im, int lines, int ls,
+                                               const double2* tw, int nmax) {
+  if constexpr (S < SEND) {
+    plan_stage_ll<N, S, REV, LMAX, NT, SIGN>(re, im, lines, ls, tw, nmax);
+    plan_stages_ll<N, S + 1, SEND, REV, LMAX, NT, SIGN>(re, im, lines, ls, tw, nmax);
+  }
+}
+
+// First stage (no twiddles) on registers already holding points p = j + m*N/R0; for a plan
+// of more than one stage the results go to LDS (followed by a barrier).  The caller must
+// make sure no thread still reads the LDS lines (barrier) before calling.
+template <int N, bool REV, int LMAX, int NT, int SIGN>
+__device__ __forceinline__ void plan_first_regs(
+    double* __restrict__ re, double* __restrict__ im, int lines, int ls,
+    double (&vr)[RegPlan<N, REV, LMAX, NT>::B0][RegPlan<N, REV, LMAX, NT>::R0],
+    double (&vi)[RegPlan<N, REV, LMAX, NT>::B0][RegPlan<N, REV, LMAX, NT>::R0]) {
+  using P = RegPlan<N, REV, LMAX, NT>;
+#pragma unroll
+  for (int q = 0; q < P::B0; q++) dft<P::R0, SIGN>(vr[q], vi[q]);
+  if constexpr (P::S > 1) {
+#pragma unroll
+    for (int q = 0; q < P::B0; q++) {
+      const int b = (int)threadIdx.x + q * NT;
+      const int line = b % LMAX, j = b / LMAX;
+      if (b < LMAX * P::NB0 && line < lines) {
+#pragma unroll
+        for (int m = 0; m < P::R0; m++) {
+          re[line * ls + j * P::R0 + m] = vr[q][m];
+          im[line * ls + j * P::R0 + m] = vi[q][m];
+        }
+      }
+    }
+    __syncthreads();
+  }
+}
+
+// Last stage: from LDS into registers holding outputs p = j + m*N/RL.  No barrier at the
+// end; LDS may be overwritten only after one.
+template <int N, bool REV, int LMAX, int NT, int SIGN>
+__device__ __forceinline__ void plan_last_regs(
+    const double* __restrict__ re, const double* __restrict__ im, int lines, int ls,
+    const double2* __restrict__ tw, int nmax,
+    double (&vr)[RegPlan<N, REV, LMAX, NT>::BL][RegPlan<N, REV, LMAX, NT>::RL],
+    double (&vi)[RegPlan<N, REV, LMAX, NT>::BL][RegPlan<N, REV, LMAX, NT>::RL]) {
+  using P = RegPlan<N, REV, LMAX, NT>;
+  constexpr int R = P::RL, NB = P::NBL;
+  const int tstep = nmax / N;
+#pragma unroll
+  for (int q = 0; q < P::BL; q++) {
+    const int b = (int)threadIdx.x + q * NT;
+    const int line = b % LMAX, j = b / LMAX;
+    if (b < LMAX * NB && line < lines) {
+#pragma unroll
+      for (int m = 0; m < R; m++) {
+        vr[q][m] = re[line * ls + j + m * NB];
+        vi[q][m] = im[line * ls + j + m * NB];
+      }
+#pragma unroll
+      for (int m = 1; m < R; m++) {  // k = j: the span of the last stage is N/R
+        const double2 w = tw[m * j * tstep];
+        const double wr = w.x, wi = SIGN < 0 ? w.y : -w.y;
+        const double a = vr[q][m], c = vi[q][m];
+        vr[q][m] = a * wr - c * wi;
+        vi[q][m] = a * wi + c * wr;
+      }
+      dft<R, SIGN>(vr[q], vi[q]);
+    }
+  }
+}
+
+// Transform of registers in the first-stage layout of plan REV into registers in its
+// last-stage layout.  in and out may differ in shape; for one-stage plans they coincide.
+template <int N, bool REV, int LMAX, int NT, int SIGN>
+__device__ __forceinline__ void plan_regs(
+    double* __restrict__ re, double* __restrict__ im, int lines, int ls,
+    const double2* __restrict__ tw, int nmax,
+    double (&vr)[RegPlan<N, REV, LMAX, NT>::B0][RegPlan<N, REV, LMAX, NT>::R0],
+    double (&vi)[RegPlan<N, REV, LMAX, NT>::B0][RegPlan<N, REV, LMAX, NT>::R0],
+    double (&ur)[RegPlan<N, REV, LMAX, NT>::BL][RegPlan<N, REV, LMAX, NT>::RL],
+    double (&ui)[RegPlan<N, REV, LMAX, NT>::BL][RegPlan<N, REV, LMAX, NT>::RL]) {
+  using P = RegPlan<N, REV, LMAX, NT>;
+  plan_first_regs<N, REV, LMAX, NT, SIGN>(re, im, lines, ls, vr, vi);
+  if constexpr (P::S == 1) {
+#pragma unroll
+    for (int q = 0; q < P::BL; q++)
+#pragma unroll
+      for (int m = 0; m < P::RL; m++) {
+        ur[q][m] = vr[q][m];
+        ui[q][m] = vi[q][m];
+      }
+  } else {
+    plan_stages_ll<N, 1, P::S - 1, REV, LMAX, NT, SIGN>(re, im, lines, ls, tw, nmax);
+    plan_last_regs<N, REV, LMAX, NT, SIGN>(re, im, lines, ls, tw, nmax, ur, ui);
+  }
+}
+
 // Host: twiddle table of nmax entries (nmax a power of two).
 double2* make_twiddles(int nmax);
 // Padded transform length for an n-point free-space convolution: the reference pads to

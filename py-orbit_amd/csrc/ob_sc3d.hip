@@ -30,8 +30,9 @@ namespace {
 constexpr int kThreads = 256;
 // Complex lines (= pairs of real z-lines) per block in the z passes.
 __host__ __device__ constexpr int zpass_pairs(int n) { return n >= 256 ? (n >= 2048 ? 1 : 2048 / n) : 8; }
-// Consecutive kz lines per block in the y and x passes (LDS <= 131 KiB).
+// Consecutive kz lines per block in the y and x passes (LDS <= 131 KiB), and threads.
 __host__ __device__ constexpr int strided_lines(int n) { return n >= 2048 ? 4 : 8; }
+__host__ __device__ constexpr int strided_threads(int n) { return n >= 256 ? 512 : 256; }
 
 enum StridedMode { FWD = 0, INV = 1, CONV = 2 };
 
@@ -229,15 +230,18 @@ __global__ __launch_bounds__(kThreads) void fft3_z_c2r(const double2* __restrict
 // Block = (outer index o, tile of up to L consecutive kz).  Element p of line (o, kz) is
 // spec[o*outer_stride + p*line_stride + kz]; a tile row is L*16 contiguous bytes, aligned
 // because H2p is a multiple of 8.  Inputs p >= n_in are zero; outputs p < n_out are written.
+// The first FFT stage reads global memory straight into registers and the last one writes
+// from registers; in the x pass (CONV) the Green multiply sits between the forward and the
+// inverse transform on the registers, so only the middle stages use LDS.
 template <int MODE, int N>
-__global__ __launch_bounds__(kThreads) void fft3_strided(double2* __restrict__ spec, int n_in,
-                                                         int n_out, int64_t line_stride,
-                                                         int64_t outer_stride, int H2,
-                                                         const double* __restrict__ ghat, int qy,
-                                                         int H2p, int M1,
-                                                         const double2* __restrict__ tw,
-                                                         int nmax) {
-  constexpr int L = strided_lines(N), ls = N + 1;
+__global__ __launch_bounds__(strided_threads(N)) void fft3_strided(
+    double2* __restrict__ spec, int n_in, int n_out, int64_t line_stride, int64_t outer_stride,
+    int H2, const double* __restrict__ ghat, int qy, int H2p, int M1,
+    const double2* __restrict__ tw, int nmax) {
+  constexpr int L = strided_lines(N), NT = strided_threads(N), ls = N + 1;
+  constexpr bool REV0 = (MODE == INV);
+  constexpr int SIGN0 = (MODE == INV) ? +1 : -1;
+  using P0 = RegPlan<N, REV0, L, NT>;
   extern __shared__ double sm[];
   double* re = sm;
   double* im = sm + L * ls;
@@ -246,35 +250,69 @@ __global__ __launch_bounds__(kThreads) void fft3_strided(double2* __restrict__ s
   const int k0 = (blockIdx.x - o * tiles) * L;
   const int nl = min(L, H2 - k0);
   double2* base = spec + (int64_t)o * outer_stride + k0;
-  for (int idx = threadIdx.x; idx < L * N; idx += kThreads) {
-    const int p = idx / L, l = idx - p * L;
-    double2 v = make_double2(0.0, 0.0);
-    if (p < n_in && l < nl) v = base[(int64_t)p * line_stride + l];
-    re[l * ls + p] = v.x;
-    im[l * ls + p] = v.y;
+  double vr[P0::B0][P0::R0], vi[P0::B0][P0::R0];
+#pragma unroll
+  for (int q = 0; q < P0::B0; q++) {
+    const int b = (int)threadIdx.x + q * NT;
+    const int line = b % L, j = b / L;
+    const bool ok = b < L * P0::NB0 && line < nl;
+#pragma unroll
+    for (int m = 0; m < P0::R0; m++) {
+      const int p = j + m * P0::NB0;
+      double2 v = make_double2(0.0, 0.0);
+      if (ok && p < n_in) v = base[(int64_t)p * line_stride + line];
+      vr[q][m] = v.x;
+      vi[q][m] = v.y;
+    }
   }
-  __syncthreads();
-  if constexpr (MODE == INV)
-    lds_fft<N, L, kThreads, +1>(re, im, nl, ls, tw, nmax);
-  else
-    lds_fft<N, L, kThreads, -1>(re, im, nl, ls, tw, nmax);
-  if constexpr (MODE == CONV) {  // x pass: multiply by the Green spectrum, inverse x FFT
+  double ur[P0::BL][P0::RL], ui[P0::BL][P0::RL];
+  plan_regs<N, REV0, L, NT, SIGN0>(re, im, nl, ls, tw, nmax, vr, vi, ur, ui);
+  if constexpr (MODE == CONV) {
+    using PI = RegPlan<N, true, L, NT>;
     const int gy = (o <= M1 / 2) ? o : M1 - o;
-    for (int idx = threadIdx.x; idx < L * N; idx += kThreads) {
-      const int p = idx / L, l = idx - p * L;
-      if (l < nl) {
-        const int gx = (p <= N / 2) ? p : N - p;
-        const double g = ghat[((int64_t)gx * qy + gy) * H2p + k0 + l];
-        re[l * ls + p] *= g;
-        im[l * ls + p] *= g;
+#pragma unroll
+    for (int q = 0; q < P0::BL; q++) {
+      const int b = (int)threadIdx.x + q * NT;
+      const int line = b % L, j = b / L;
+      if (b < L * P0::NBL && line < nl) {
+#pragma unroll
+        for (int m = 0; m < P0::RL; m++) {
+          const int p = j + m * P0::NBL;
+          const int gx = (p <= N / 2) ? p : N - p;
+          const double g = ghat[((int64_t)gx * qy + gy) * H2p + k0 + line];
+          ur[q][m] *= g;
+          ui[q][m] *= g;
+        }
       }
     }
-    __syncthreads();
-    lds_fft<N, L, kThreads, +1>(re, im, nl, ls, tw, nmax);
-  }
-  for (int idx = threadIdx.x; idx < L * n_out; idx += kThreads) {
-    const int p = idx / L, l = idx - p * L;
-    if (l < nl) base[(int64_t)p * line_stride + l] = make_double2(re[l * ls + p], im[l * ls + p]);
+    __syncthreads();  // every thread has read its last-stage inputs from LDS
+    double wr[PI::BL][PI::RL], wi[PI::BL][PI::RL];
+    plan_regs<N, true, L, NT, +1>(re, im, nl, ls, tw, nmax, ur, ui, wr, wi);
+#pragma unroll
+    for (int q = 0; q < PI::BL; q++) {
+      const int b = (int)threadIdx.x + q * NT;
+      const int line = b % L, j = b / L;
+      if (b < L * PI::NBL && line < nl) {
+#pragma unroll
+        for (int m = 0; m < PI::RL; m++) {
+          const int p = j + m * PI::NBL;
+          if (p < n_out) base[(int64_t)p * line_stride + line] = make_double2(wr[q][m], wi[q][m]);
+        }
+      }
+    }
+  } else {
+#pragma unroll
+    for (int q = 0; q < P0::BL; q++) {
+      const int b = (int)threadIdx.x + q * NT;
+      const int line = b % L, j = b / L;
+      if (b < L * P0::NBL && line < nl) {
+#pragma unroll
+        for (int m = 0; m < P0::RL; m++) {
+          const int p = j + m * P0::NBL;
+          if (p < n_out) base[(int64_t)p * line_stride + line] = make_double2(ur[q][m], ui[q][m]);
+        }
+      }
+    }
   }
 }
 
@@ -386,7 +424,7 @@ void launch_strided_n(ob_sc3d* c, int n_in, int n_out, int64_t line_stride,
   }
   const int tiles = (c->H2 + strided_lines(N) - 1) / strided_lines(N);
   ob::ProfScope ps_(label);
-  hipLaunchKernelGGL((fft3_strided<MODE, N>), dim3(outers * tiles), dim3(kThreads), lds,
+  hipLaunchKernelGGL((fft3_strided<MODE, N>), dim3(outers * tiles), dim3(strided_threads(N)), lds,
                      stream(), c->spec, n_in, n_out, line_stride, outer_stride, c->H2, c->ghat,
                      c->qy, c->H2p, c->M1, c->tw, c->nmax);
   OB_HIP(hipGetLastError());
