@@ -36,9 +36,12 @@ __host__ __device__ constexpr int fft_span(int n, int s) {  // product of the ra
   return s == 0 ? 1 : fft_span(n, s - 1) * fft_radix(n, s - 1);
 }
 
-// In-register R-point DFTs, natural order; SIGN -1 forward, +1 inverse.
+// In-register R-point DFTs, natural order; SIGN -1 forward, +1 inverse.  The FFT arithmetic
+// may fuse multiply-adds (the solver is checked to a relative tolerance, not bit for bit;
+// the rest of the library is built without contraction).
 template <int SIGN>
 __device__ __forceinline__ void dft2(double* r, double* i) {
+#pragma clang fp contract(fast)
   const double ar = r[0], ai = i[0];
   r[0] = ar + r[1];
   i[0] = ai + i[1];
@@ -48,6 +51,7 @@ __device__ __forceinline__ void dft2(double* r, double* i) {
 
 template <int SIGN>
 __device__ __forceinline__ void dft4(double* r, double* i) {
+#pragma clang fp contract(fast)
   const double t0r = r[0] + r[2], t0i = i[0] + i[2];
   const double t1r = r[0] - r[2], t1i = i[0] - i[2];
   const double t2r = r[1] + r[3], t2i = i[1] + i[3];
@@ -66,6 +70,7 @@ __device__ __forceinline__ void dft4(double* r, double* i) {
 
 template <int SIGN>
 __device__ __forceinline__ void dft8(double* r, double* i) {
+#pragma clang fp contract(fast)
   double er[4] = {r[0], r[2], r[4], r[6]}, ei[4] = {i[0], i[2], i[4], i[6]};
   double orr[4] = {r[1], r[3], r[5], r[7]}, oi[4] = {i[1], i[3], i[5], i[7]};
   dft4<SIGN>(er, ei);
@@ -113,6 +118,7 @@ template <int N, int S, int LMAX, int NT, int SIGN>
 __device__ __forceinline__ void fft_stage(double* __restrict__ re, double* __restrict__ im,
                                           int lines, int ls, const double2* __restrict__ tw,
                                           int nmax) {
+#pragma clang fp contract(fast)
   constexpr int R = fft_radix(N, S);
   constexpr int Ns = fft_span(N, S);
   constexpr int NB = N / R;  // butterflies per line
@@ -186,14 +192,24 @@ __device__ __forceinline__ void lds_fft(double* __restrict__ re, double* __restr
 
 // ---- transforms with register-resident ends ------------------------------------------
 // A plan runs the radices of fft_radix in order (REV = false) or reversed (REV = true); any
-// order is a valid Stockham factorisation.  The first stage of a plan reads its butterflies
-// straight from a loader (global memory) and the last stage leaves its results in
-// registers, so only the middle stages go through LDS.  Thread t, butterfly q handles
-// b = t + q*NT as line = b % LMAX, j = b / LMAX (lines fastest: a wave reads LMAX lines x a
-// few consecutive points, which is contiguous when the lines are adjacent in memory).  In
-// the first stage it holds points p = j + m*N/R0; after the last stage it holds outputs
-// p = j + m*N/RL.  A forward plan's output registers are the input registers of the reversed
-// plan, so forward -> pointwise multiply -> inverse needs no LDS exchange in between.
+// order is a valid Stockham factorisation.  The first stage of a plan takes its butterflies
+// from registers filled by the caller (e.g. straight from global memory) and the last stage
+// leaves its results in registers, so only the exchanges between stages go through LDS.
+// At the ends thread t, butterfly q handles b = t + q*NT as line = b % LMAX, j = b / LMAX
+// (lines fastest: a wave touches LMAX lines x a few consecutive points, contiguous when the
+// lines are adjacent in memory); it holds points p = j + m*N/R0 before the first stage and
+// outputs p = j + m*N/RL after the last.  A forward plan's output registers are the input
+// registers of the reversed plan, so forward -> pointwise multiply -> inverse needs no LDS
+// exchange in between.  Middle stages run j-fastest within a line.
+//
+// LDS layout per exchange e (data written by stage e-1, read by stage e): element i of line
+// l is at l*LS + i + P*(i/Q), chosen for the 64 banks of 4 bytes (32 doubles per row):
+//  - stage e-1 writes groups of Ns = span(e-1) consecutive points every Q = Ns*R points;
+//    when Ns < 32 the groups of 32 consecutive butterflies would share banks, so Ns points
+//    of padding follow every Q points;
+//  - the first exchange is written by line-fastest threads at stride R0: LS = 1 (mod 32);
+//  - the last exchange is read by line-fastest threads, 8 lines x 4 points per half wave:
+//    LS = 4 (mod 32).
 template <int N, bool REV>
 __host__ __device__ constexpr int plan_radix(int s) {
   return REV ? fft_radix(N, fft_stages(N) - 1 - s) : fft_radix(N, s);
@@ -201,6 +217,40 @@ __host__ __device__ constexpr int plan_radix(int s) {
 template <int N, bool REV>
 __host__ __device__ constexpr int plan_span(int s) {
   return s == 0 ? 1 : plan_span<N, REV>(s - 1) * plan_radix<N, REV>(s - 1);
+}
+__host__ __device__ constexpr int round_to(int v, int mod, int rem) {
+  return v + ((rem - v % mod) % mod + mod) % mod;
+}
+template <int N, bool REV>
+__host__ __device__ constexpr int xch_q(int e) { return plan_span<N, REV>(e); }
+template <int N, bool REV>
+__host__ __device__ constexpr int xch_p(int e) {
+  return e <= 1 ? 0
+                : ((plan_span<N, REV>(e - 1) < 32 && plan_span<N, REV>(e) % 32 == 0)
+                       ? plan_span<N, REV>(e - 1)
+                       : 0);
+}
+template <int N, bool REV>
+__host__ __device__ constexpr int xch_ls(int e) {
+  return round_to(N + xch_p<N, REV>(e) * (N / xch_q<N, REV>(e)), 32,
+                  (e == fft_stages(N) - 1 && e != 1) ? 4 : 1);
+}
+template <int N, bool REV, int E>
+__device__ __forceinline__ int xch_idx(int line, int i) {
+  constexpr int Q = xch_q<N, REV>(E), P = xch_p<N, REV>(E), LS = xch_ls<N, REV>(E);
+  if constexpr (P == 0)
+    return line * LS + i;
+  else
+    return line * LS + i + P * (i / Q);
+}
+// Doubles per re (or im) array of LDS for plan REV over LMAX lines.
+template <int N, bool REV, int E = 1>
+__host__ __device__ constexpr int plan_lds_line() {
+  if constexpr (E >= fft_stages(N))
+    return 0;
+  else
+    return xch_ls<N, REV>(E) > plan_lds_line<N, REV, E + 1>() ? xch_ls<N, REV>(E)
+                                                              : plan_lds_line<N, REV, E + 1>();
 }
 
 template <int N, bool REV, int LMAX, int NT>
@@ -210,13 +260,15 @@ struct RegPlan {
   static constexpr int B0 = (LMAX * NB0 + NT - 1) / NT;
   static constexpr int RL = plan_radix<N, REV>(S - 1), NBL = N / RL;
   static constexpr int BL = (LMAX * NBL + NT - 1) / NT;
+  static constexpr int LDS_LINE = plan_lds_line<N, REV>();  // doubles per line, re or im
 };
 
-// LDS -> LDS stage S of plan REV (threads j-fastest within a line).
+// Middle stage S of plan REV: LDS (exchange S) -> LDS (exchange S+1).
 template <int N, int S, bool REV, int LMAX, int NT, int SIGN>
 __device__ __forceinline__ void plan_stage_ll(double* __restrict__ re, double* __restrict__ im,
-                                              int lines, int ls, const double2* __restrict__ tw,
+                                              int lines, const double2* __restrict__ tw,
                                               int nmax) {
+#pragma clang fp contract(fast)
   constexpr int R = plan_radix<N, REV>(S);
   constexpr int Ns = plan_span<N, REV>(S);
   constexpr int NB = N / R;
@@ -232,8 +284,9 @@ __device__ __forceinline__ void plan_stage_ll(double* __restrict__ re, double* _
       const int k = j & (Ns - 1);
 #pragma unroll
       for (int m = 0; m < R; m++) {
-        vr[q][m] = re[line * ls + j + m * NB];
-        vi[q][m] = im[line * ls + j + m * NB];
+        const int a = xch_idx<N, REV, S>(line, j + m * NB);
+        vr[q][m] = re[a];
+        vi[q][m] = im[a];
       }
 #pragma unroll
       for (int m = 1; m < R; m++) {
@@ -255,8 +308,9 @@ __device__ __forceinline__ void plan_stage_ll(double* __restrict__ re, double* _
       const int k = j & (Ns - 1);
 #pragma unroll
       for (int m = 0; m < R; m++) {
-        re[line * ls + (j - k) * R + k + m * Ns] = vr[q][m];
-        im[line * ls + (j - k) * R + k + m * Ns] = vi[q][m];
+        const int a = xch_idx<N, REV, S + 1>(line, (j - k) * R + k + m * Ns);
+        re[a] = vr[q][m];
+        im[a] = vi[q][m];
       }
     }
   }
@@ -264,20 +318,20 @@ __device__ __forceinline__ void plan_stage_ll(double* __restrict__ re, double* _
 }
 
 template <int N, int S, int SEND, bool REV, int LMAX, int NT, int SIGN>
-__device__ __forceinline__ void plan_stages_ll(double* re, double* im, int lines, int ls,
+__device__ __forceinline__ void plan_stages_ll(double* re, double* im, int lines,
                                                const double2* tw, int nmax) {
   if constexpr (S < SEND) {
-    plan_stage_ll<N, S, REV, LMAX, NT, SIGN>(re, im, lines, ls, tw, nmax);
-    plan_stages_ll<N, S + 1, SEND, REV, LMAX, NT, SIGN>(re, im, lines, ls, tw, nmax);
+    plan_stage_ll<N, S, REV, LMAX, NT, SIGN>(re, im, lines, tw, nmax);
+    plan_stages_ll<N, S + 1, SEND, REV, LMAX, NT, SIGN>(re, im, lines, tw, nmax);
   }
 }
 
-// First stage (no twiddles) on registers already holding points p = j + m*N/R0; for a plan
-// of more than one stage the results go to LDS (followed by a barrier).  The caller must
-// make sure no thread still reads the LDS lines (barrier) before calling.
+// First stage (no twiddles) on registers holding points p = j + m*N/R0; for a plan of more
+// than one stage the results go to LDS (exchange 1), followed by a barrier.  No thread may
+// still be reading the LDS lines when it is called.
 template <int N, bool REV, int LMAX, int NT, int SIGN>
 __device__ __forceinline__ void plan_first_regs(
-    double* __restrict__ re, double* __restrict__ im, int lines, int ls,
+    double* __restrict__ re, double* __restrict__ im, int lines,
     double (&vr)[RegPlan<N, REV, LMAX, NT>::B0][RegPlan<N, REV, LMAX, NT>::R0],
     double (&vi)[RegPlan<N, REV, LMAX, NT>::B0][RegPlan<N, REV, LMAX, NT>::R0]) {
   using P = RegPlan<N, REV, LMAX, NT>;
@@ -291,8 +345,9 @@ __device__ __forceinline__ void plan_first_regs(
       if (b < LMAX * P::NB0 && line < lines) {
 #pragma unroll
         for (int m = 0; m < P::R0; m++) {
-          re[line * ls + j * P::R0 + m] = vr[q][m];
-          im[line * ls + j * P::R0 + m] = vi[q][m];
+          const int a = xch_idx<N, REV, 1>(line, j * P::R0 + m);
+          re[a] = vr[q][m];
+          im[a] = vi[q][m];
         }
       }
     }
@@ -300,14 +355,15 @@ __device__ __forceinline__ void plan_first_regs(
   }
 }
 
-// Last stage: from LDS into registers holding outputs p = j + m*N/RL.  No barrier at the
-// end; LDS may be overwritten only after one.
+// Last stage: from LDS (exchange S-1) into registers holding outputs p = j + m*N/RL.  No
+// barrier at the end; LDS may be overwritten only after one.
 template <int N, bool REV, int LMAX, int NT, int SIGN>
 __device__ __forceinline__ void plan_last_regs(
-    const double* __restrict__ re, const double* __restrict__ im, int lines, int ls,
+    const double* __restrict__ re, const double* __restrict__ im, int lines,
     const double2* __restrict__ tw, int nmax,
     double (&vr)[RegPlan<N, REV, LMAX, NT>::BL][RegPlan<N, REV, LMAX, NT>::RL],
     double (&vi)[RegPlan<N, REV, LMAX, NT>::BL][RegPlan<N, REV, LMAX, NT>::RL]) {
+#pragma clang fp contract(fast)
   using P = RegPlan<N, REV, LMAX, NT>;
   constexpr int R = P::RL, NB = P::NBL;
   const int tstep = nmax / N;
@@ -318,8 +374,9 @@ __device__ __forceinline__ void plan_last_regs(
     if (b < LMAX * NB && line < lines) {
 #pragma unroll
       for (int m = 0; m < R; m++) {
-        vr[q][m] = re[line * ls + j + m * NB];
-        vi[q][m] = im[line * ls + j + m * NB];
+        const int a = xch_idx<N, REV, P::S - 1>(line, j + m * NB);
+        vr[q][m] = re[a];
+        vi[q][m] = im[a];
       }
 #pragma unroll
       for (int m = 1; m < R; m++) {  // k = j: the span of the last stage is N/R
@@ -335,17 +392,18 @@ __device__ __forceinline__ void plan_last_regs(
 }
 
 // Transform of registers in the first-stage layout of plan REV into registers in its
-// last-stage layout.  in and out may differ in shape; for one-stage plans they coincide.
+// last-stage layout (for a one-stage plan the two coincide).  re/im: LDS arrays of
+// LMAX * RegPlan::LDS_LINE doubles each.
 template <int N, bool REV, int LMAX, int NT, int SIGN>
 __device__ __forceinline__ void plan_regs(
-    double* __restrict__ re, double* __restrict__ im, int lines, int ls,
+    double* __restrict__ re, double* __restrict__ im, int lines,
     const double2* __restrict__ tw, int nmax,
     double (&vr)[RegPlan<N, REV, LMAX, NT>::B0][RegPlan<N, REV, LMAX, NT>::R0],
     double (&vi)[RegPlan<N, REV, LMAX, NT>::B0][RegPlan<N, REV, LMAX, NT>::R0],
     double (&ur)[RegPlan<N, REV, LMAX, NT>::BL][RegPlan<N, REV, LMAX, NT>::RL],
     double (&ui)[RegPlan<N, REV, LMAX, NT>::BL][RegPlan<N, REV, LMAX, NT>::RL]) {
   using P = RegPlan<N, REV, LMAX, NT>;
-  plan_first_regs<N, REV, LMAX, NT, SIGN>(re, im, lines, ls, vr, vi);
+  plan_first_regs<N, REV, LMAX, NT, SIGN>(re, im, lines, vr, vi);
   if constexpr (P::S == 1) {
 #pragma unroll
     for (int q = 0; q < P::BL; q++)
@@ -355,8 +413,8 @@ __device__ __forceinline__ void plan_regs(
         ui[q][m] = vi[q][m];
       }
   } else {
-    plan_stages_ll<N, 1, P::S - 1, REV, LMAX, NT, SIGN>(re, im, lines, ls, tw, nmax);
-    plan_last_regs<N, REV, LMAX, NT, SIGN>(re, im, lines, ls, tw, nmax, ur, ui);
+    plan_stages_ll<N, 1, P::S - 1, REV, LMAX, NT, SIGN>(re, im, lines, tw, nmax);
+    plan_last_regs<N, REV, LMAX, NT, SIGN>(re, im, lines, tw, nmax, ur, ui);
   }
 }
 

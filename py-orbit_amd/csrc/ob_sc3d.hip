@@ -36,6 +36,15 @@ __host__ __device__ constexpr int strided_threads(int n) { return n >= 256 ? 512
 
 enum StridedMode { FWD = 0, INV = 1, CONV = 2 };
 
+// LDS doubles per line (re or im) of a strided block: the larger of the forward and the
+// reversed plan's padded layouts.
+template <int N>
+__host__ __device__ constexpr int strided_line_doubles() {
+  return RegPlan<N, false, 1, 1>::LDS_LINE > RegPlan<N, true, 1, 1>::LDS_LINE
+             ? RegPlan<N, false, 1, 1>::LDS_LINE
+             : RegPlan<N, true, 1, 1>::LDS_LINE;
+}
+
 struct GreenGeom {
   int M0, M1, M2;
   double dx, dy, dz;
@@ -233,24 +242,26 @@ __global__ __launch_bounds__(kThreads) void fft3_z_c2r(const double2* __restrict
 // The first FFT stage reads global memory straight into registers and the last one writes
 // from registers; in the x pass (CONV) the Green multiply sits between the forward and the
 // inverse transform on the registers, so only the middle stages use LDS.
+//
+// Zero padding is pruned at compile time: the forward y pass and the x pass read only
+// p < N/2 (n_in <= N/2 because N >= 2 n_in - 1), and the x pass and the inverse y pass keep
+// only p < N/2, so half of the first-stage inputs are known zeros and half of the
+// last-stage outputs are never formed.  The x pass handles ky = o and its mirror M1 - o in
+// one block: both rows multiply by the same Green values, which stay in registers.
 template <int MODE, int N>
-__global__ __launch_bounds__(strided_threads(N)) void fft3_strided(
-    double2* __restrict__ spec, int n_in, int n_out, int64_t line_stride, int64_t outer_stride,
-    int H2, const double* __restrict__ ghat, int qy, int H2p, int M1,
-    const double2* __restrict__ tw, int nmax) {
-  constexpr int L = strided_lines(N), NT = strided_threads(N), ls = N + 1;
-  constexpr bool REV0 = (MODE == INV);
-  constexpr int SIGN0 = (MODE == INV) ? +1 : -1;
-  using P0 = RegPlan<N, REV0, L, NT>;
-  extern __shared__ double sm[];
-  double* re = sm;
-  double* im = sm + L * ls;
-  const int tiles = (H2 + L - 1) / L;
-  const int o = blockIdx.x / tiles;
-  const int k0 = (blockIdx.x - o * tiles) * L;
-  const int nl = min(L, H2 - k0);
-  double2* base = spec + (int64_t)o * outer_stride + k0;
-  double vr[P0::B0][P0::R0], vi[P0::B0][P0::R0];
+using StridedIn = RegPlan<N, MODE == INV, strided_lines(N), strided_threads(N)>;
+template <int N>
+using StridedFwd = RegPlan<N, false, strided_lines(N), strided_threads(N)>;
+
+// Global -> registers in the first-stage layout of the row's plan.
+template <int MODE, int N>
+__device__ __forceinline__ void strided_load(const double2* __restrict__ base, int nl, int n_in,
+                                             int64_t line_stride,
+                                             double (&vr)[StridedIn<MODE, N>::B0][StridedIn<MODE, N>::R0],
+                                             double (&vi)[StridedIn<MODE, N>::B0][StridedIn<MODE, N>::R0]) {
+  constexpr int L = strided_lines(N), NT = strided_threads(N);
+  constexpr bool HALF_IN = (MODE != INV);
+  using P0 = StridedIn<MODE, N>;
 #pragma unroll
   for (int q = 0; q < P0::B0; q++) {
     const int b = (int)threadIdx.x + q * NT;
@@ -260,41 +271,47 @@ __global__ __launch_bounds__(strided_threads(N)) void fft3_strided(
     for (int m = 0; m < P0::R0; m++) {
       const int p = j + m * P0::NB0;
       double2 v = make_double2(0.0, 0.0);
-      if (ok && p < n_in) v = base[(int64_t)p * line_stride + line];
+      if (!(HALF_IN && 2 * m >= P0::R0) && ok && p < n_in) v = base[(int64_t)p * line_stride + line];
       vr[q][m] = v.x;
       vi[q][m] = v.y;
     }
   }
+}
+
+// Transform(s) of one row already in registers, then registers -> global.
+template <int MODE, int N>
+__device__ __forceinline__ void strided_compute(
+    double* re, double* im, double2* __restrict__ base, int nl, int n_out, int64_t line_stride,
+    const double2* __restrict__ tw, int nmax,
+    const double (&g)[StridedFwd<N>::BL][StridedFwd<N>::RL],
+    double (&vr)[StridedIn<MODE, N>::B0][StridedIn<MODE, N>::R0],
+    double (&vi)[StridedIn<MODE, N>::B0][StridedIn<MODE, N>::R0]) {
+  constexpr int L = strided_lines(N), NT = strided_threads(N);
+  constexpr bool REV0 = (MODE == INV);
+  constexpr int SIGN0 = (MODE == INV) ? +1 : -1;
+  constexpr bool HALF_OUT = (MODE != FWD);
+  using P0 = StridedIn<MODE, N>;
   double ur[P0::BL][P0::RL], ui[P0::BL][P0::RL];
-  plan_regs<N, REV0, L, NT, SIGN0>(re, im, nl, ls, tw, nmax, vr, vi, ur, ui);
+  plan_regs<N, REV0, L, NT, SIGN0>(re, im, nl, tw, nmax, vr, vi, ur, ui);
   if constexpr (MODE == CONV) {
     using PI = RegPlan<N, true, L, NT>;
-    const int gy = (o <= M1 / 2) ? o : M1 - o;
 #pragma unroll
-    for (int q = 0; q < P0::BL; q++) {
-      const int b = (int)threadIdx.x + q * NT;
-      const int line = b % L, j = b / L;
-      if (b < L * P0::NBL && line < nl) {
+    for (int q = 0; q < P0::BL; q++)
 #pragma unroll
-        for (int m = 0; m < P0::RL; m++) {
-          const int p = j + m * P0::NBL;
-          const int gx = (p <= N / 2) ? p : N - p;
-          const double g = ghat[((int64_t)gx * qy + gy) * H2p + k0 + line];
-          ur[q][m] *= g;
-          ui[q][m] *= g;
-        }
+      for (int m = 0; m < P0::RL; m++) {
+        ur[q][m] *= g[q][m];
+        ui[q][m] *= g[q][m];
       }
-    }
     __syncthreads();  // every thread has read its last-stage inputs from LDS
     double wr[PI::BL][PI::RL], wi[PI::BL][PI::RL];
-    plan_regs<N, true, L, NT, +1>(re, im, nl, ls, tw, nmax, ur, ui, wr, wi);
+    plan_regs<N, true, L, NT, +1>(re, im, nl, tw, nmax, ur, ui, wr, wi);
 #pragma unroll
     for (int q = 0; q < PI::BL; q++) {
       const int b = (int)threadIdx.x + q * NT;
       const int line = b % L, j = b / L;
       if (b < L * PI::NBL && line < nl) {
 #pragma unroll
-        for (int m = 0; m < PI::RL; m++) {
+        for (int m = 0; m < PI::RL / 2; m++) {
           const int p = j + m * PI::NBL;
           if (p < n_out) base[(int64_t)p * line_stride + line] = make_double2(wr[q][m], wi[q][m]);
         }
@@ -307,12 +324,68 @@ __global__ __launch_bounds__(strided_threads(N)) void fft3_strided(
       const int line = b % L, j = b / L;
       if (b < L * P0::NBL && line < nl) {
 #pragma unroll
-        for (int m = 0; m < P0::RL; m++) {
+        for (int m = 0; m < (HALF_OUT ? P0::RL / 2 : P0::RL); m++) {
           const int p = j + m * P0::NBL;
           if (p < n_out) base[(int64_t)p * line_stride + line] = make_double2(ur[q][m], ui[q][m]);
         }
       }
     }
+  }
+}
+
+// Each block transforms two rows of its kz tile: the x pass takes ky = o and M1 - o, the
+// y passes take outer rows 2o and 2o+1.  The second row's loads are issued before the
+// first row is transformed, so they overlap its FFT work.
+template <int MODE, int N>
+__global__ __launch_bounds__(strided_threads(N)) void fft3_strided(
+    double2* __restrict__ spec, int n_in, int n_out, int64_t line_stride, int64_t outer_stride,
+    int outers, int H2, const double* __restrict__ ghat, int qy, int H2p, int M1,
+    const double2* __restrict__ tw, int nmax) {
+  constexpr int L = strided_lines(N), NT = strided_threads(N);
+  using PF = StridedFwd<N>;
+  using P0 = StridedIn<MODE, N>;
+  extern __shared__ double sm[];
+  double* re = sm;
+  double* im = sm + L * strided_line_doubles<N>();
+  const int tiles = (H2 + L - 1) / L;
+  const int o = blockIdx.x / tiles;
+  const int k0 = (blockIdx.x - o * tiles) * L;
+  const int nl = min(L, H2 - k0);
+  int r0, r1;
+  bool two;
+  if constexpr (MODE == CONV) {
+    r0 = o;
+    r1 = M1 - o;
+    two = (o != 0 && 2 * o != M1);
+  } else {
+    r0 = 2 * o;
+    r1 = 2 * o + 1;
+    two = r1 < outers;
+  }
+  double2* base0 = spec + (int64_t)r0 * outer_stride + k0;
+  double2* base1 = spec + (int64_t)r1 * outer_stride + k0;
+  double ar[P0::B0][P0::R0], ai[P0::B0][P0::R0], br[P0::B0][P0::R0], bi[P0::B0][P0::R0];
+  strided_load<MODE, N>(base0, nl, n_in, line_stride, ar, ai);
+  if (two) strided_load<MODE, N>(base1, nl, n_in, line_stride, br, bi);
+  double g[PF::BL][PF::RL];
+  if constexpr (MODE == CONV) {  // Green values of (ky = o, this kz tile), shared by both rows
+#pragma unroll
+    for (int q = 0; q < PF::BL; q++) {
+      const int b = (int)threadIdx.x + q * NT;
+      const int line = b % L, j = b / L;
+      const bool ok = b < L * PF::NBL && line < nl;
+#pragma unroll
+      for (int m = 0; m < PF::RL; m++) {
+        const int p = j + m * PF::NBL;
+        const int gx = (p <= N / 2) ? p : N - p;
+        g[q][m] = ok ? ghat[((int64_t)gx * qy + o) * H2p + k0 + line] : 0.0;
+      }
+    }
+  }
+  strided_compute<MODE, N>(re, im, base0, nl, n_out, line_stride, tw, nmax, g, ar, ai);
+  if (two) {
+    __syncthreads();  // LDS reads of the first row are done
+    strided_compute<MODE, N>(re, im, base1, nl, n_out, line_stride, tw, nmax, g, br, bi);
   }
 }
 
@@ -411,7 +484,7 @@ void allow_lds(K* kernel, size_t bytes) {
 template <int N>
 size_t zpass_lds() { return sizeof(double) * 2 * zpass_pairs(N) * (N + 1); }
 template <int N>
-size_t strided_lds() { return sizeof(double) * 2 * strided_lines(N) * (N + 1); }
+size_t strided_lds() { return sizeof(double) * 2 * strided_lines(N) * strided_line_doubles<N>(); }
 
 template <int MODE, int N>
 void launch_strided_n(ob_sc3d* c, int n_in, int n_out, int64_t line_stride,
@@ -423,10 +496,14 @@ void launch_strided_n(ob_sc3d* c, int n_in, int n_out, int64_t line_stride,
     ready = true;
   }
   const int tiles = (c->H2 + strided_lines(N) - 1) / strided_lines(N);
+  OB_REQUIRE(MODE == INV || 2 * n_in <= N, "strided FFT: input longer than half the line");
+  OB_REQUIRE(MODE == FWD || 2 * n_out <= N, "strided FFT: output longer than half the line");
+  // x pass: ky pairs (o, M1-o), o <= M1/2; y passes: row pairs (2o, 2o+1)
+  const int blocks_outer = (MODE == CONV) ? c->M1 / 2 + 1 : (outers + 1) / 2;
   ob::ProfScope ps_(label);
-  hipLaunchKernelGGL((fft3_strided<MODE, N>), dim3(outers * tiles), dim3(strided_threads(N)), lds,
-                     stream(), c->spec, n_in, n_out, line_stride, outer_stride, c->H2, c->ghat,
-                     c->qy, c->H2p, c->M1, c->tw, c->nmax);
+  hipLaunchKernelGGL((fft3_strided<MODE, N>), dim3(blocks_outer * tiles), dim3(strided_threads(N)),
+                     lds, stream(), c->spec, n_in, n_out, line_stride, outer_stride, outers, c->H2,
+                     c->ghat, c->qy, c->H2p, c->M1, c->tw, c->nmax);
   OB_HIP(hipGetLastError());
 }
 
