@@ -329,6 +329,8 @@ void bunch_sort_commit(ob_bunch* b) {
   b->host_valid = false;
 }
 
+void bunch_device_written(ob_bunch* b) { b->host_valid = false; }
+
 SyncInfo bunch_sync(ob_bunch* b) {
   return SyncInfo{b->mass, b->charge, b->r0, b->macrosize, b->T, b->p, b->beta, b->gamma, b->time};
 }

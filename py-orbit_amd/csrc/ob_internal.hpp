@@ -153,4 +153,6 @@ struct SyncInfo {
   double mass, charge, r0, macrosize, T, p, beta, gamma, time;
 };
 SyncInfo bunch_sync(ob_bunch* b);
+// A kernel outside the map queue has written the device coordinates.
+void bunch_device_written(ob_bunch* b);
 }  // namespace ob

@@ -466,7 +466,7 @@ struct ob_sc3d {
   double* ext_host = nullptr;
   double2 *spec = nullptr, *tw = nullptr;
   bool tiled = true;              // tile-sorted LDS deposit (else particle-order atomics)
-  int lds_budget = 110 * 1024;    // LDS bytes for one deposit tile
+  int lds_budget = 16 * 1024;     // LDS bytes for one deposit / kick tile
   ob::TileWork work;
 };
 
@@ -766,7 +766,15 @@ OB_API int ob_sc3d_track(ob_sc3d* c, ob_bunch* b, double length) {
   const double scale_coeff = c->sd[0] / g.dx;
   const double denom = scale_coeff / ((double)c->M0 * c->M1 * c->M2);
   ob::solve(c, denom);
-  // kick (:120-147), fused into the next map kernel
+  // kick (:120-147): over the deposit's tiles when the bunch was tile-sorted, else fused
+  // into the next map kernel
+  if (c->tiled && tg.T > 0) {
+    const ob::Kick3D k{c->phi, rzMin, rdz, (rzMax + rzMin) / 2.0, gamma,
+                       length * sy.r0 / ((beta * beta) * (gamma * gamma)),  // :121
+                       length * sy.r0 * sy.mass};                           // :122
+    ob::tile_kick_3d(b, tg, k, c->work);
+    return 0;
+  }
   ob::Op op;
   std::memset(&op, 0, sizeof(op));
   op.kind = ob::OP_SC3D_KICK;

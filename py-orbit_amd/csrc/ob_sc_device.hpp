@@ -119,30 +119,37 @@ __device__ __forceinline__ void grid3d_z_ind_frac(double z, double zMin, double 
   }
 }
 
+// A window of a z-fastest 3D array: node (iX, iY, iZ) is at
+// a[(iX - x0)*sx + (iY - y0)*sy + (iZ - z0)].  The whole grid is the window with origin 0,
+// sx = ny*nz, sy = nz; an LDS copy of a tile is a window with the tile's origin.
+struct Grid3View {
+  const double* a;
+  int64_t sx, sy;
+  int x0, y0, z0;
+  __device__ __forceinline__ const double* at(int iX, int iY, int iZ) const {
+    return a + (int64_t)(iX - x0) * sx + (int64_t)(iY - y0) * sy + (iZ - z0);
+  }
+};
+
 // Grid3D::calcSheetGradient (Grid3D.cc:718-733).  The device keeps 3D grids z-fastest,
 // a[(ix*ny + iy)*nz + iz] (the layout of the reference's FFT buffer, PoissonSolverFFT3D.cc:
 // 305-309); the reference's Arr3D[iz][ix][iy] order is restored only when copied out.
-__device__ __forceinline__ double sheet_grad(const double* __restrict__ a, int nx, int ny, int nz,
-                                             int iZ, int iX, int iY, double xm, double x0,
-                                             double xp, double ym, double y0, double yp) {
-  const int64_t sx = (int64_t)ny * nz, sy = nz;
-  const double* s = a + (int64_t)iX * sx + (int64_t)iY * sy + iZ;
+__device__ __forceinline__ double sheet_grad(const Grid3View& v, int iZ, int iX, int iY,
+                                             double xm, double x0, double xp, double ym,
+                                             double y0, double yp) {
+  const int64_t sx = v.sx, sy = v.sy;
+  const double* s = v.at(iX, iY, iZ);
   return xm * ym * s[-sx - sy] + x0 * ym * s[-sy] + xp * ym * s[sx - sy] + xm * y0 * s[-sx] +
          x0 * y0 * s[0] + xp * y0 * s[sx] + xm * yp * s[-sx + sy] + x0 * yp * s[sy] +
          xp * yp * s[sx + sy];
 }
 
-// Grid3D::calcGradient (Grid3D.cc:438-532).
-__device__ __forceinline__ void grid3d_gradient(const double* __restrict__ a, int nx, int ny,
-                                                int nz, double xMin, double dx, double yMin,
-                                                double dy, double zMin, double dz, double x,
-                                                double y, double z, double& gX, double& gY,
-                                                double& gZ) {
-  int iX, iY, iZ;
-  double xF, yF, zF;
-  node_ind_frac(x, xMin, dx, nx, iX, xF);
-  node_ind_frac(y, yMin, dy, ny, iY, yF);
-  grid3d_z_ind_frac(z, zMin, dz, nz, iZ, zF);
+// Grid3D::calcGradient (Grid3D.cc:438-532): TSC weights and their derivatives at the
+// particle's node (iX, iY, iZ), combined over the 27 surrounding nodes.
+__device__ __forceinline__ void grid3d_gradient_at(const Grid3View& a, int nz, int iX, int iY,
+                                                   int iZ, double xF, double yF, double zF,
+                                                   double dx, double dy, double dz, double& gX,
+                                                   double& gY, double& gZ) {
   const double Wxm = 0.5 * (0.5 - xF) * (0.5 - xF), Wx0 = 0.75 - xF * xF,
                Wxp = 0.5 * (0.5 + xF) * (0.5 + xF);
   const double Wym = 0.5 * (0.5 - yF) * (0.5 - yF), Wy0 = 0.75 - yF * yF,
@@ -173,27 +180,42 @@ __device__ __forceinline__ void grid3d_gradient(const double* __restrict__ a, in
   if (iZm < 0) iZm = nz - 1;
   if (iZp >= nz) iZp = 0;
   if (nz >= 3) {
-    gX = sheet_grad(a, nx, ny, nz, iZm, iX, iY, dWxm, dWx0, dWxp, Wym, Wy0, Wyp) * Wzm +
-         sheet_grad(a, nx, ny, nz, iZ, iX, iY, dWxm, dWx0, dWxp, Wym, Wy0, Wyp) * Wz0 +
-         sheet_grad(a, nx, ny, nz, iZp, iX, iY, dWxm, dWx0, dWxp, Wym, Wy0, Wyp) * Wzp;
-    gY = sheet_grad(a, nx, ny, nz, iZm, iX, iY, Wxm, Wx0, Wxp, dWym, dWy0, dWyp) * Wzm +
-         sheet_grad(a, nx, ny, nz, iZ, iX, iY, Wxm, Wx0, Wxp, dWym, dWy0, dWyp) * Wz0 +
-         sheet_grad(a, nx, ny, nz, iZp, iX, iY, Wxm, Wx0, Wxp, dWym, dWy0, dWyp) * Wzp;
-    gZ = sheet_grad(a, nx, ny, nz, iZm, iX, iY, Wxm, Wx0, Wxp, Wym, Wy0, Wyp) * dWzm +
-         sheet_grad(a, nx, ny, nz, iZ, iX, iY, Wxm, Wx0, Wxp, Wym, Wy0, Wyp) * dWz0 +
-         sheet_grad(a, nx, ny, nz, iZp, iX, iY, Wxm, Wx0, Wxp, Wym, Wy0, Wyp) * dWzp;
+    gX = sheet_grad(a, iZm, iX, iY, dWxm, dWx0, dWxp, Wym, Wy0, Wyp) * Wzm +
+         sheet_grad(a, iZ, iX, iY, dWxm, dWx0, dWxp, Wym, Wy0, Wyp) * Wz0 +
+         sheet_grad(a, iZp, iX, iY, dWxm, dWx0, dWxp, Wym, Wy0, Wyp) * Wzp;
+    gY = sheet_grad(a, iZm, iX, iY, Wxm, Wx0, Wxp, dWym, dWy0, dWyp) * Wzm +
+         sheet_grad(a, iZ, iX, iY, Wxm, Wx0, Wxp, dWym, dWy0, dWyp) * Wz0 +
+         sheet_grad(a, iZp, iX, iY, Wxm, Wx0, Wxp, dWym, dWy0, dWyp) * Wzp;
+    gZ = sheet_grad(a, iZm, iX, iY, Wxm, Wx0, Wxp, Wym, Wy0, Wyp) * dWzm +
+         sheet_grad(a, iZ, iX, iY, Wxm, Wx0, Wxp, Wym, Wy0, Wyp) * dWz0 +
+         sheet_grad(a, iZp, iX, iY, Wxm, Wx0, Wxp, Wym, Wy0, Wyp) * dWzp;
   } else if (nz == 2) {
-    gX = sheet_grad(a, nx, ny, nz, 0, iX, iY, dWxm, dWx0, dWxp, Wym, Wy0, Wyp) * Wzm +
-         sheet_grad(a, nx, ny, nz, 1, iX, iY, dWxm, dWx0, dWxp, Wym, Wy0, Wyp) * Wzp;
-    gY = sheet_grad(a, nx, ny, nz, 0, iX, iY, Wxm, Wx0, Wxp, dWym, dWy0, dWyp) * Wzm +
-         sheet_grad(a, nx, ny, nz, 1, iX, iY, Wxm, Wx0, Wxp, dWym, dWy0, dWyp) * Wzp;
-    gZ = sheet_grad(a, nx, ny, nz, 0, iX, iY, Wxm, Wx0, Wxp, Wym, Wy0, Wyp) * dWzm +
-         sheet_grad(a, nx, ny, nz, 1, iX, iY, Wxm, Wx0, Wxp, Wym, Wy0, Wyp) * dWzp;
+    gX = sheet_grad(a, 0, iX, iY, dWxm, dWx0, dWxp, Wym, Wy0, Wyp) * Wzm +
+         sheet_grad(a, 1, iX, iY, dWxm, dWx0, dWxp, Wym, Wy0, Wyp) * Wzp;
+    gY = sheet_grad(a, 0, iX, iY, Wxm, Wx0, Wxp, dWym, dWy0, dWyp) * Wzm +
+         sheet_grad(a, 1, iX, iY, Wxm, Wx0, Wxp, dWym, dWy0, dWyp) * Wzp;
+    gZ = sheet_grad(a, 0, iX, iY, Wxm, Wx0, Wxp, Wym, Wy0, Wyp) * dWzm +
+         sheet_grad(a, 1, iX, iY, Wxm, Wx0, Wxp, Wym, Wy0, Wyp) * dWzp;
   } else {
-    gX = sheet_grad(a, nx, ny, nz, 0, iX, iY, dWxm, dWx0, dWxp, Wym, Wy0, Wyp);
-    gY = sheet_grad(a, nx, ny, nz, 0, iX, iY, Wxm, Wx0, Wxp, dWym, dWy0, dWyp);
+    gX = sheet_grad(a, 0, iX, iY, dWxm, dWx0, dWxp, Wym, Wy0, Wyp);
+    gY = sheet_grad(a, 0, iX, iY, Wxm, Wx0, Wxp, dWym, dWy0, dWyp);
     gZ = 0.0;
   }
+}
+
+// Grid3D::calcGradient (Grid3D.cc:438-532) at (x, y, z) on the whole grid a.
+__device__ __forceinline__ void grid3d_gradient(const double* __restrict__ a, int nx, int ny,
+                                                int nz, double xMin, double dx, double yMin,
+                                                double dy, double zMin, double dz, double x,
+                                                double y, double z, double& gX, double& gY,
+                                                double& gZ) {
+  int iX, iY, iZ;
+  double xF, yF, zF;
+  node_ind_frac(x, xMin, dx, nx, iX, xF);
+  node_ind_frac(y, yMin, dy, ny, iY, yF);
+  grid3d_z_ind_frac(z, zMin, dz, nz, iZ, zF);
+  const Grid3View v{a, (int64_t)ny * nz, nz, 0, 0, 0};
+  grid3d_gradient_at(v, nz, iX, iY, iZ, xF, yF, zF, dx, dy, dz, gX, gY, gZ);
 }
 
 // SpaceChargeCalc2p5Drb kick loop body (SpaceChargeCalc2p5Drb.cc:146-167).

@@ -1,19 +1,23 @@
-// ob_tiled3d.hip -- tile-sorted 3D charge deposition (Grid3D::binValue, Grid3D.cc:307-432).
+// ob_tiled3d.hip -- tile-sorted 3D charge deposition (Grid3D::binValue, Grid3D.cc:307-432)
+// and potential gather for the kick (SpaceChargeCalc3D.cc:120-147).
 //
 // Scattering 27 FP64 atomics per particle in random particle order over a 64 MiB grid
-// misses L2 on almost every update.  Instead, each SC step:
-//   1. tile_hist    key = xy-tile of the particle's TSC centre node; per-block LDS
+// misses L2 on almost every update, and gathering 27 potential values per particle in
+// particle order touches a different cache line per lane.  Instead, each SC step:
+//   1. tile_hist    key = T x T x Tz box of the particle's TSC centre node; per-block LDS
 //                   histogram, merged into global tile counts;
 //   2. tile_scan    exclusive scan of the counts; work items of at most kChunk particles
 //                   per tile;
 //   3. tile_scatter counting-sort scatter of the bunch (6 coordinates, macrosize and the
 //                   user index) into the spare buffers, ranges reserved per block;
 //   4. tile_deposit one workgroup per work item accumulates its particles into an LDS
-//                   block of (T+2) x (T+2) x nz cells, then adds the block to rho with
-//                   coalesced atomics (z is the fastest axis).
-// The bunch stays sorted for the kick, whose 27-point potential gathers then hit cache.
-// Weights and indices are the reference's (same device helpers as the unsorted path);
-// only the FP64 summation order differs.
+//                   block of the box plus a one-node halo, then adds the block to rho with
+//                   atomics;
+//   5. tile_kick    after the Poisson solve, one workgroup per work item copies the same
+//                   block of phi into LDS and computes its particles' kicks from it.
+// The blocks are small (~14 KiB for 8 x 8 x 16 boxes), so many workgroups share a CU.
+// Weights, indices and the kick arithmetic are the reference's (same device helpers as
+// the unsorted path); only the FP64 summation order of rho differs.
 #include "ob_internal.hpp"
 #include "ob_sc_device.hpp"
 #include "ob_tiled3d.hpp"
@@ -23,6 +27,7 @@ namespace {
 
 constexpr int kThreads = 256;
 constexpr int kChunk = 4096;
+constexpr int kMaxTiles = 16383;  // (ntiles + 1) 32-bit counters in 64 KiB of LDS
 
 __device__ __forceinline__ bool inside(const TileGeom& g, double x, double y, double z) {
   return !(x < g.xMin || x > g.xMax || y < g.yMin || y > g.yMax || z < g.zMin || z > g.zMax);
@@ -30,12 +35,36 @@ __device__ __forceinline__ bool inside(const TileGeom& g, double x, double y, do
 
 __device__ __forceinline__ uint32_t tile_key(const TileGeom& g, double x, double y, double z) {
   if (!inside(g, x, y, z)) return (uint32_t)g.ntiles;
-  int iX, iY;
-  double xF, yF;
+  int iX, iY, iZ;
+  double xF, yF, zF;
   node_ind_frac(x, g.xMin, g.dx, g.nx, iX, xF);
   node_ind_frac(y, g.yMin, g.dy, g.ny, iY, yF);
-  return (uint32_t)((iX / g.T) * g.ty + (iY / g.T));
+  grid3d_z_ind_frac(z, g.zMin, g.dz, g.nz, iZ, zF);
+  return (uint32_t)(((iX / g.T) * g.ty + (iY / g.T)) * g.tz + iZ / g.Tz);
 }
+
+// Tile t's LDS block: nodes x0..x0+W-1, y0..y0+W-1, z0..z0+Wz-1, z fastest.
+struct TileBox {
+  int x0, y0, z0, W, Wz;
+  __device__ __forceinline__ TileBox(const TileGeom& g, int t) {
+    const int tzi = t % g.tz, r = t / g.tz;
+    const int tyi = r % g.ty, txi = r / g.ty;
+    W = g.T + 2;
+    Wz = g.Tz + 2 * g.zh;
+    x0 = txi * g.T - 1;
+    y0 = tyi * g.T - 1;
+    z0 = tzi * g.Tz - g.zh;
+  }
+  __device__ __forceinline__ int cells() const { return W * W * Wz; }
+  __device__ __forceinline__ int idx(int gx, int gy, int gz) const {
+    return ((gx - x0) * W + (gy - y0)) * Wz + (gz - z0);
+  }
+  // nodes gx-1..gx+1, gy-1..gy+1, zlo..zhi inside the block
+  __device__ __forceinline__ bool holds(int gx, int gy, int zlo, int zhi) const {
+    return gx - 1 >= x0 && gx + 1 < x0 + W && gy - 1 >= y0 && gy + 1 < y0 + W && zlo >= z0 &&
+           zhi < z0 + Wz;
+  }
+};
 
 __global__ __launch_bounds__(kThreads) void tile_hist(const double* __restrict__ cur, int64_t cap,
                                                       int64_t n, TileGeom g,
@@ -130,6 +159,43 @@ __global__ __launch_bounds__(kThreads) void tile_scatter(
   }
 }
 
+// Grid3D::binValue for one particle into a z-fastest window (global rho or an LDS block).
+__device__ __forceinline__ void deposit_particle(double* __restrict__ base, int64_t sx,
+                                                 int64_t sy, int nz, int zoff, double m, double xF,
+                                                 double yF, int iZ, double zF) {
+  const double wx[3] = {0.5 * (0.5 - xF) * (0.5 - xF), 0.75 - xF * xF, 0.5 * (0.5 + xF) * (0.5 + xF)};
+  const double wy[3] = {0.5 * (0.5 - yF) * (0.5 - yF), 0.75 - yF * yF, 0.5 * (0.5 + yF) * (0.5 + yF)};
+  // base: node (iX-1, iY-1, 0) of the window; zoff: window z origin
+  if (nz >= 3) {
+    int iZm = iZ - 1, iZp = iZ + 1;
+    if (iZm < 0) iZm = nz - 1;
+    if (iZp >= nz) iZp = 0;
+    const double wz[3] = {0.5 * (0.5 - zF) * (0.5 - zF), 0.75 - zF * zF,
+                          0.5 * (0.5 + zF) * (0.5 + zF)};
+    const int zz[3] = {iZm - zoff, iZ - zoff, iZp - zoff};
+#pragma unroll
+    for (int a = 0; a < 3; a++)
+#pragma unroll
+      for (int c = 0; c < 3; c++) {
+        const double tmp = wy[c] * wz[a] * m;
+#pragma unroll
+        for (int d = 0; d < 3; d++) atomicAdd(base + d * sx + c * sy + zz[a], wx[d] * tmp);
+      }
+  } else if (nz == 2) {
+    const double wz[2] = {1.0 - zF, zF};
+    for (int a = 0; a < 2; a++)
+      for (int c = 0; c < 3; c++) {
+        const double tmp = wy[c] * wz[a] * m;
+        for (int d = 0; d < 3; d++) atomicAdd(base + d * sx + c * sy + (a - zoff), wx[d] * tmp);
+      }
+  } else {
+    for (int c = 0; c < 3; c++) {
+      const double tmp = wy[c] * m;
+      for (int d = 0; d < 3; d++) atomicAdd(base + d * sx + c * sy - zoff, wx[d] * tmp);
+    }
+  }
+}
+
 __global__ __launch_bounds__(kThreads) void tile_deposit(const double* __restrict__ cur,
                                                          int64_t cap,
                                                          const double* __restrict__ msize,
@@ -140,12 +206,11 @@ __global__ __launch_bounds__(kThreads) void tile_deposit(const double* __restric
   if (blockIdx.x >= *nitems) return;
   extern __shared__ double acc[];
   const int4 it = items[blockIdx.x];
-  const int t = it.x;
-  const int W = g.T + 2, nz = g.nz;
-  const int cells = W * W * nz;
-  const int x0 = (t / g.ty) * g.T - 1, y0 = (t % g.ty) * g.T - 1;
+  const TileBox box(g, it.x);
+  const int nz = g.nz, cells = box.cells();
   for (int c = threadIdx.x; c < cells; c += blockDim.x) acc[c] = 0.0;
   __syncthreads();
+  const int64_t gsx = (int64_t)g.ny * nz;
   for (int i = it.y + threadIdx.x; i < it.z; i += blockDim.x) {
     const double x = cur[i], y = cur[2 * cap + i], z = cur[4 * cap + i];
     const double m = msize ? msize[i] : macrosize;
@@ -154,48 +219,79 @@ __global__ __launch_bounds__(kThreads) void tile_deposit(const double* __restric
     node_ind_frac(x, g.xMin, g.dx, g.nx, iX, xF);
     node_ind_frac(y, g.yMin, g.dy, g.ny, iY, yF);
     grid3d_z_ind_frac(z, g.zMin, g.dz, nz, iZ, zF);
-    const double wx[3] = {0.5 * (0.5 - xF) * (0.5 - xF), 0.75 - xF * xF, 0.5 * (0.5 + xF) * (0.5 + xF)};
-    const double wy[3] = {0.5 * (0.5 - yF) * (0.5 - yF), 0.75 - yF * yF, 0.5 * (0.5 + yF) * (0.5 + yF)};
-    // local origin of the 3x3 stencil: node (iX-1, iY-1) in tile coordinates
-    double* base = acc + ((iX - 1 - x0) * W + (iY - 1 - y0)) * nz;
-    if (nz >= 3) {
-      int iZm = iZ - 1, iZp = iZ + 1;
-      if (iZm < 0) iZm = nz - 1;
-      if (iZp >= nz) iZp = 0;
-      const double wz[3] = {0.5 * (0.5 - zF) * (0.5 - zF), 0.75 - zF * zF,
-                            0.5 * (0.5 + zF) * (0.5 + zF)};
-      const int zz[3] = {iZm, iZ, iZp};
-#pragma unroll
-      for (int a = 0; a < 3; a++)
-#pragma unroll
-        for (int c = 0; c < 3; c++) {
-          const double tmp = wy[c] * wz[a] * m;
-#pragma unroll
-          for (int d = 0; d < 3; d++) atomicAdd(base + (d * W + c) * nz + zz[a], wx[d] * tmp);
-        }
-    } else if (nz == 2) {
-      const double wz[2] = {1.0 - zF, zF};
-      for (int a = 0; a < 2; a++)
-        for (int c = 0; c < 3; c++) {
-          const double tmp = wy[c] * wz[a] * m;
-          for (int d = 0; d < 3; d++) atomicAdd(base + (d * W + c) * nz + a, wx[d] * tmp);
-        }
-    } else {
-      for (int c = 0; c < 3; c++) {
-        const double tmp = wy[c] * m;
-        for (int d = 0; d < 3; d++) atomicAdd(base + (d * W + c) * nz, wx[d] * tmp);
-      }
+    const int zlo = nz >= 3 ? iZ - 1 : 0, zhi = nz >= 3 ? iZ + 1 : nz - 1;
+    if (box.holds(iX, iY, zlo, zhi)) {
+      deposit_particle(acc + box.idx(iX - 1, iY - 1, box.z0), box.W * box.Wz, box.Wz, nz,
+                             box.z0, m, xF, yF, iZ, zF);
+    } else {  // not in this tile's block (cannot happen for a consistent sort): global
+      deposit_particle(rho + (iX - 1) * gsx + (int64_t)(iY - 1) * nz, gsx, nz, nz, 0, m,
+                              xF, yF, iZ, zF);
     }
   }
   __syncthreads();
-  const int64_t sx = (int64_t)g.ny * nz;
   for (int c = threadIdx.x; c < cells; c += blockDim.x) {
     const double v = acc[c];
     if (v == 0.0) continue;
-    const int lz = c % nz, r = c / nz, ly = r % W, lx = r / W;
-    const int gx = x0 + lx, gy = y0 + ly;
-    atomicAdd(rho + gx * sx + (int64_t)gy * nz + lz, v);
+    const int lz = c % box.Wz, r = c / box.Wz, ly = r % box.W, lx = r / box.W;
+    const int gx = box.x0 + lx, gy = box.y0 + ly, gz = box.z0 + lz;
+    if (gx < 0 || gx >= g.nx || gy < 0 || gy >= g.ny || gz < 0 || gz >= nz) continue;
+    atomicAdd(rho + gx * gsx + (int64_t)gy * nz + gz, v);
   }
+}
+
+// SpaceChargeCalc3D kick of particle i, gathering phi through view v (which must hold the
+// 27 nodes) or the whole grid.
+__device__ __forceinline__ void kick_particle(double* __restrict__ cur, int64_t cap, int64_t i,
+                                              const TileGeom& g, const Kick3D& k,
+                                              const double* lds, const TileBox* box) {
+  const double x = cur[i], y = cur[2 * cap + i], z = cur[4 * cap + i];
+  const double zr = (z - k.zc) * k.gamma + k.zc;
+  int iX, iY, iZ;
+  double xF, yF, zF;
+  node_ind_frac(x, g.xMin, g.dx, g.nx, iX, xF);
+  node_ind_frac(y, g.yMin, g.dy, g.ny, iY, yF);
+  grid3d_z_ind_frac(zr, k.rzMin, k.rdz, g.nz, iZ, zF);
+  const int zlo = g.nz >= 3 ? iZ - 1 : 0, zhi = g.nz >= 3 ? iZ + 1 : g.nz - 1;
+  double ex, ey, ez;
+  if (box && box->holds(iX, iY, zlo, zhi)) {
+    const Grid3View v{lds, (int64_t)box->W * box->Wz, box->Wz, box->x0, box->y0, box->z0};
+    grid3d_gradient_at(v, g.nz, iX, iY, iZ, xF, yF, zF, g.dx, g.dy, k.rdz, ex, ey, ez);
+  } else {
+    const Grid3View v{k.phi, (int64_t)g.ny * g.nz, g.nz, 0, 0, 0};
+    grid3d_gradient_at(v, g.nz, iX, iY, iZ, xF, yF, zF, g.dx, g.dy, k.rdz, ex, ey, ez);
+  }
+  cur[cap + i] += -ex * k.trans_factor;
+  cur[3 * cap + i] += -ey * k.trans_factor;
+  cur[5 * cap + i] += -ez * k.long_factor;
+}
+
+// Blocks 0..nitems-1: work items; block nitems: the particles outside the grid (last in the
+// sorted order, from offsets[ntiles]), kicked from the whole grid like the reference.
+__global__ __launch_bounds__(kThreads) void tile_kick(double* __restrict__ cur, int64_t cap,
+                                                      int64_t n, TileGeom g, Kick3D k,
+                                                      const int4* __restrict__ items,
+                                                      const uint32_t* __restrict__ nitems,
+                                                      const uint32_t* __restrict__ offsets) {
+  const uint32_t ni = *nitems;
+  if (blockIdx.x > ni) return;
+  if (blockIdx.x == ni) {
+    for (int64_t i = (int64_t)offsets[g.ntiles] + threadIdx.x; i < n; i += blockDim.x)
+      kick_particle(cur, cap, i, g, k, nullptr, nullptr);
+    return;
+  }
+  extern __shared__ double sphi[];
+  const int4 it = items[blockIdx.x];
+  const TileBox box(g, it.x);
+  const int cells = box.cells();
+  const int64_t gsx = (int64_t)g.ny * g.nz;
+  for (int c = threadIdx.x; c < cells; c += blockDim.x) {
+    const int lz = c % box.Wz, r = c / box.Wz, ly = r % box.W, lx = r / box.W;
+    const int gx = box.x0 + lx, gy = box.y0 + ly, gz = box.z0 + lz;
+    const bool in = gx >= 0 && gx < g.nx && gy >= 0 && gy < g.ny && gz >= 0 && gz < g.nz;
+    sphi[c] = in ? k.phi[gx * gsx + (int64_t)gy * g.nz + gz] : 0.0;
+  }
+  __syncthreads();
+  for (int i = it.y + threadIdx.x; i < it.z; i += blockDim.x) kick_particle(cur, cap, i, g, k, sphi, &box);
 }
 
 }  // namespace
@@ -207,23 +303,29 @@ TileGeom make_tile_geom(int nx, int ny, int nz, double xMin, double xMax, double
   g.nx = nx; g.ny = ny; g.nz = nz;
   g.xMin = xMin; g.xMax = xMax; g.yMin = yMin; g.yMax = yMax; g.zMin = zMin; g.zMax = zMax;
   g.dx = dx; g.dy = dy; g.dz = dz;
-  g.T = 0;
-  for (int T : {8, 6, 4, 3, 2, 1})
-    if ((int64_t)(T + 2) * (T + 2) * nz * 8 <= lds_budget) {
-      g.T = T;
+  g.T = g.Tz = g.zh = g.tx = g.ty = g.tz = g.ntiles = 0;
+  // Small boxes first (many workgroups per CU); larger ones only to bound the tile count.
+  static const int shapes[][2] = {{8, 16}, {8, 8},  {16, 16}, {8, 32}, {16, 32},
+                                  {32, 32}, {6, 8}, {4, 8},   {4, 4},  {2, 2},  {1, 1}};
+  for (const auto& sh : shapes) {
+    const int T = sh[0];
+    const int Tz = nz <= 2 ? nz : std::min(sh[1], nz);
+    const int zh = nz >= 3 ? 1 : 0;
+    const int64_t tx = (nx + T - 1) / T, ty = (ny + T - 1) / T, tz = (nz + Tz - 1) / Tz;
+    const int64_t ntiles = tx * ty * tz;
+    const int64_t lds = (int64_t)(T + 2) * (T + 2) * (Tz + 2 * zh) * 8;
+    if (lds <= lds_budget && ntiles <= kMaxTiles) {
+      g.T = T; g.Tz = Tz; g.zh = zh;
+      g.tx = (int)tx; g.ty = (int)ty; g.tz = (int)tz; g.ntiles = (int)ntiles;
       break;
     }
-  if (g.T > 0) {
-    g.tx = (nx + g.T - 1) / g.T;
-    g.ty = (ny + g.T - 1) / g.T;
-    g.ntiles = g.tx * g.ty;
-  } else {
-    g.tx = g.ty = g.ntiles = 0;
   }
   return g;
 }
 
-size_t tile_lds_bytes(const TileGeom& g) { return sizeof(double) * (g.T + 2) * (g.T + 2) * g.nz; }
+size_t tile_lds_bytes(const TileGeom& g) {
+  return sizeof(double) * (g.T + 2) * (g.T + 2) * (g.Tz + 2 * g.zh);
+}
 
 void TileWork::reserve(int64_t n, int ntiles) {
   if (n > n_cap) {
@@ -269,6 +371,8 @@ void sort_and_deposit_3d(ob_bunch* b, const TileGeom& g, double macrosize, doubl
                                hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024));
     OB_HIP(hipFuncSetAttribute((const void*)tile_scatter,
                                hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024));
+    OB_HIP(hipFuncSetAttribute((const void*)tile_kick,
+                               hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024));
     attr_done = true;
   }
   const int64_t n = bunch_size(b);
@@ -309,6 +413,21 @@ void sort_and_deposit_3d(ob_bunch* b, const TileGeom& g, double macrosize, doubl
                        w.nitems, rho);
   }
   OB_HIP(hipGetLastError());
+}
+
+void tile_kick_3d(ob_bunch* b, const TileGeom& g, const Kick3D& k, TileWork& w) {
+  const int64_t n = bunch_size(b);
+  if (n == 0) return;
+  DeviceBunchView v = bunch_device_view(b);
+  const int64_t max_items = n / kChunk + g.ntiles + 1;
+  {
+    ProfScope ps("tile_kick");
+    hipLaunchKernelGGL(tile_kick, dim3((unsigned)(max_items + 1)), dim3(kThreads),
+                       tile_lds_bytes(g), stream(), v.c[0], (int64_t)(v.c[1] - v.c[0]), n, g, k,
+                       w.items, w.nitems, w.offsets);
+  }
+  OB_HIP(hipGetLastError());
+  bunch_device_written(b);
 }
 
 }  // namespace ob

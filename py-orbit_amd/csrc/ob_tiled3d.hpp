@@ -1,18 +1,21 @@
-// ob_tiled3d.hpp -- tile-sorted 3D deposition (see ob_tiled3d.hip).
+// ob_tiled3d.hpp -- tile-sorted 3D deposition and kick (see ob_tiled3d.hip).
 #pragma once
 
 #include "ob_internal.hpp"
 
 namespace ob {
 
-// Grid of the deposit plus its xy tiling: tiles of T x T centre nodes, all nz z cells.
+// Grid of the deposit plus its tiling into boxes of T x T x Tz centre nodes.  Each tile owns
+// an LDS block of (T+2) x (T+2) x (Tz + 2*zh) nodes: the box plus the one-node halo the
+// 3-point stencils reach (zh = 1 for nz >= 3; for nz <= 2 one z tile spans the grid).
 struct TileGeom {
   int nx, ny, nz;
-  int T, tx, ty, ntiles;
+  int T, Tz, zh, tx, ty, tz, ntiles;
   double xMin, xMax, yMin, yMax, zMin, zMax, dx, dy, dz;
 };
 
-// Picks the largest tile whose (T+2)^2 * nz LDS block fits lds_budget bytes (T = 0 if none).
+// Picks a tile shape whose LDS block fits lds_budget bytes and whose tile count fits the
+// LDS histograms of the sort (T = 0 if none does).
 TileGeom make_tile_geom(int nx, int ny, int nz, double xMin, double xMax, double yMin,
                         double yMax, double zMin, double zMax, double dx, double dy, double dz,
                         int lds_budget);
@@ -30,8 +33,18 @@ struct TileWork {
 };
 
 // Sort the bunch by tile (device order only) and deposit its charge into rho, a
-// z-fastest [nx][ny][nz] array that must already be zero.
+// z-fastest [nx][ny][nz] array that must already be zero.  The work items stay valid for
+// tile_kick_3d until the particles move.
 void sort_and_deposit_3d(ob_bunch* b, const TileGeom& g, double macrosize, double* rho,
                          TileWork& w);
+
+// The SpaceChargeCalc3D kick (SpaceChargeCalc3D.cc:120-147) of a bunch sorted by
+// sort_and_deposit_3d with the same geometry: phi is the rest-frame potential on the same
+// [nx][ny][nz] nodes, z grid (rzMin, rdz), bunch centre zc.
+struct Kick3D {
+  const double* phi;
+  double rzMin, rdz, zc, gamma, trans_factor, long_factor;
+};
+void tile_kick_3d(ob_bunch* b, const TileGeom& g, const Kick3D& k, TileWork& w);
 
 }  // namespace ob
