@@ -128,6 +128,14 @@ int ob_sc3d_track(ob_sc3d* c, ob_bunch* b, double length);             /* :100-1
 int ob_sc3d_get_grid(ob_sc3d* c, int which, double* out, double* lims);
 int ob_sc3d_set_ratio_limit(ob_sc3d* c, double r);                     /* :334-337 */
 int ob_sc3d_get_ratio_limit(ob_sc3d* c, double* r);
+/* numExtBunches(n) / freqOfBunches(f) (:84-97; wrap_spacechargecalc3d.cc).  n > 0 selects
+ * wrappedBunchAnalysis (:251-328): the bunch is binned with z wrapped to one bunch spacing
+ * c*beta/f, the Green function holds n image bunches (odd n rounds up, PoissonSolverFFT3D.cc:
+ * 101-106) and is rebuilt on every call, and the kick wraps z (:138-140).  Default f = 402.5 MHz. */
+int ob_sc3d_set_num_ext_bunches(ob_sc3d* c, int n);
+int ob_sc3d_get_num_ext_bunches(ob_sc3d* c, int* n);
+int ob_sc3d_set_freq(ob_sc3d* c, double f);
+int ob_sc3d_get_freq(ob_sc3d* c, double* f);
 /* Number of Green-function rebuilds so far (ratio changes, :186-203). */
 int ob_sc3d_green_rebuilds(ob_sc3d* c, int* n);
 /* Deposit strategy (no reference counterpart; results agree to FP64 summation order):

@@ -75,6 +75,9 @@ def lib():
             "orc_sc3d_set_ratio_limit": (None, [vp, d]), "orc_sc3d_track": (i, [vp, bp, d]),
             "orc_sc3d_get_grid": (C.c_int64, [vp, i, _D, _D]),
             "orc_sc3d_green_rebuilds": (i, [vp]),
+            "orc_sc3d_set_num_ext_bunches": (None, [vp, i]),
+            "orc_sc3d_get_num_ext_bunches": (i, [vp]),
+            "orc_sc3d_set_freq": (None, [vp, d]), "orc_sc3d_get_freq": (d, [vp]),
             "orc_twiss_emittances": (None, [bp, _D]),
         }
         for name, (res, args) in sig.items():
@@ -186,6 +189,16 @@ class SC3D:
     def setRatioLimit(self, r):
         lib().orc_sc3d_set_ratio_limit(self.h, r)
 
+    def numExtBunches(self, n=None):
+        if n is not None:
+            lib().orc_sc3d_set_num_ext_bunches(self.h, int(n))
+        return lib().orc_sc3d_get_num_ext_bunches(self.h)
+
+    def freqOfBunches(self, f=None):
+        if f is not None:
+            lib().orc_sc3d_set_freq(self.h, float(f))
+        return lib().orc_sc3d_get_freq(self.h)
+
     def trackBunch(self, bunch, length):
         rc = lib().orc_sc3d_track(self.h, bunch.ref, length)
         if rc != 0:
@@ -237,6 +250,21 @@ def poisson3d(rho, solver_lims, rho_lims):
     sl, rl = _f(solver_lims), _f(rho_lims)
     out = np.zeros_like(rho)
     f(nx, ny, nz, _dp(sl), _dp(rl), _dp(rho), _dp(out))
+    return out
+
+
+def poisson3d_images(rho, solver_lims, rho_lims, nbunches, lam):
+    """PoissonSolverFFT3D::findPotential with nbunches external-bunch images at spacing lam
+    in the Green function (PoissonSolverFFT3D.cc:184-196)."""
+    L = lib()
+    f = L.orc_flat_poisson3d_images
+    f.restype = None
+    f.argtypes = [C.c_int, C.c_int, C.c_int, _D, _D, C.c_int, C.c_double, _D, _D]
+    rho = np.ascontiguousarray(rho, dtype=np.float64)
+    nz, nx, ny = rho.shape
+    sl, rl = _f(solver_lims), _f(rho_lims)
+    out = np.zeros_like(rho)
+    f(nx, ny, nz, _dp(sl), _dp(rl), int(nbunches), float(lam), _dp(rho), _dp(out))
     return out
 
 
@@ -298,6 +326,8 @@ class GridAPI:
             "grid3d_bin": [i, i, i, _D, lg, _D, _D, _D, _D, _D],
             "grid3d_indfrac": [i, i, i, _D, lg, _D, _D, _D, _I, _D],
             "grid3d_grad": [i, i, i, _D, _D, lg, _D, _D, _D, _D, _D, _D],
+            "grid3d_bin_wrapped": [i, i, i, _D, lg, _D, _D, _D, _D, d, _D],
+            "grid3d_grad_wrapped": [i, i, i, _D, _D, lg, _D, _D, _D, _D, _D, _D],
         }
         self.f = {}
         for name, args in sig.items():
@@ -359,6 +389,21 @@ class GridAPI:
         gx, gy, gz = np.zeros(n), np.zeros(n), np.zeros(n)
         self.f["grid3d_grad"](nx, ny, nz, _dp(lims), _dp(arr), n, _dp(x), _dp(y), _dp(z),
                               _dp(gx), _dp(gy), _dp(gz))
+        return gx, gy, gz
+
+    def grid3d_bin_wrapped(self, nx, ny, nz, lims, x, y, z, m, lam):
+        x, y, z, m, lims = _f(x), _f(y), _f(z), _f(m), _f(lims)
+        out = np.zeros(nx * ny * nz)
+        self.f["grid3d_bin_wrapped"](nx, ny, nz, _dp(lims), len(x), _dp(x), _dp(y), _dp(z),
+                                     _dp(m), float(lam), _dp(out))
+        return out.reshape(nz, nx, ny)
+
+    def grid3d_grad_wrapped(self, nx, ny, nz, lims, arr, x, y, z):
+        x, y, z, lims, arr = _f(x), _f(y), _f(z), _f(lims), _f(arr).ravel()
+        n = len(x)
+        gx, gy, gz = np.zeros(n), np.zeros(n), np.zeros(n)
+        self.f["grid3d_grad_wrapped"](nx, ny, nz, _dp(lims), _dp(arr), n, _dp(x), _dp(y),
+                                      _dp(z), _dp(gx), _dp(gy), _dp(gz))
         return gx, gy, gz
 
 

@@ -514,7 +514,7 @@ void orc_grid1d_gradient(const orc_grid1d* g, double z, double* ez) {
 void orc_grid3d_set_x(orc_grid3d* g, double a, double b) { g->xMin = a; g->xMax = b; g->dx = (g->xMax - g->xMin) / (g->nx - 1); }
 void orc_grid3d_set_y(orc_grid3d* g, double a, double b) { g->yMin = a; g->yMax = b; g->dy = (g->yMax - g->yMin) / (g->ny - 1); }
 void orc_grid3d_set_z(orc_grid3d* g, double a, double b) { g->zMin = a; g->zMax = b; g->dz = (g->zMax - g->zMin) / g->nz; }
-/* :212-264, longWrapping = 0 */
+/* :212-264 */
 void orc_grid3d_ind_frac(const orc_grid3d* g, double x, double y, double z, int ind[3], double frac[3]) {
   int i = (int)((x - g->xMin) / g->dx + 0.5);
   if (i < 1) i = 1;
@@ -530,8 +530,13 @@ void orc_grid3d_ind_frac(const orc_grid3d* g, double x, double y, double z, int 
     int zi = (int)((z - g->zMin) / g->dz);
     double zf = 0.0;
     if (g->nz > 2) {
-      if (zi <= 0) zi = 1;
-      if (zi >= g->nz - 1) zi = g->nz - 2;
+      if (!g->wrap) {
+        if (zi <= 0) zi = 1;
+        if (zi >= g->nz - 1) zi = g->nz - 2;
+      } else {
+        if (zi <= 0) zi = 0;
+        if (zi >= g->nz - 1) zi = g->nz - 1;
+      }
       zf = (z - (g->zMin + (zi + 0.5) * g->dz)) / g->dz;
     }
     if (g->nz == 2) {
@@ -593,8 +598,9 @@ void orc_grid3d_bin_value(orc_grid3d* g, double m, double x, double y, double z)
     }
   }
 }
-/* :281-305 (longWrapping = 0) */
+/* :281-305: binBunch(bunch) sets longWrapping = 0 (:307-310) */
 void orc_grid3d_bin_bunch(orc_grid3d* g, const orc_bunch* b) {
+  g->wrap = 0;
   int nt = g_threads;
   if (nt <= 1 || b->n < 4096) {
     for (int64_t i = 0; i < b->n; i++)
@@ -628,6 +634,14 @@ void orc_grid3d_bin_bunch(orc_grid3d* g, const orc_bunch* b) {
   for (int64_t k = 0; k < (int64_t)ng; k++)
     for (int t = 0; t < nt; t++) g->arr[k] += priv[ng * t + k];
   free(priv);
+}
+/* binBunch(bunch, lambda) with longWrapping = 1 (:281-305, serial) */
+void orc_grid3d_bin_bunch_wrapped(orc_grid3d* g, const orc_bunch* b, double lambda) {
+  g->wrap = 1;
+  for (int64_t i = 0; i < b->n; i++) {
+    double z = remainder(b->z[i], lambda);
+    orc_grid3d_bin_value(g, b->msize ? b->msize[i] : b->s.macrosize, b->x[i], b->y[i], z);
+  }
 }
 /* :718-733 */
 static double sheet_grad(const orc_grid3d* g, int iZ, int iX, int iY, double xm, double x0, double xp,
@@ -894,9 +908,11 @@ void orc_poisson2d_find_potential(orc_poisson2d* p, const orc_grid2d* rho, orc_g
 }
 
 /* ------------------------------------------------------------------------ */
-/* PoissonSolverFFT3D: src/spacecharge/PoissonSolverFFT3D.cc (nBunches_ = 0) */
+/* PoissonSolverFFT3D: src/spacecharge/PoissonSolverFFT3D.cc */
 struct orc_poisson3d {
   int nx, ny, nz, M0, M1, M2;
+  int nbunches;   /* nBunches_ (:34) */
+  double lambda;  /* spacing of the external bunches (:108-110) */
   double xMin, xMax, yMin, yMax, zMin, zMax, dx, dy, dz;
   double* ghat; /* [M0][M1][M2/2+1][2] */
   double *in, *spec, *res;
@@ -912,7 +928,17 @@ static void poisson3d_define_green(orc_poisson3d* p) { /* :160-247 */
       for (int iX = 0; iX <= M0 / 2; iX++) {
         double rX = iX * p->dx, rX2 = rX * rX;
         double rTot = sqrt(rX2 + rY2 + rZ2);
-        G[iZ + (size_t)M2 * (iY + (size_t)M1 * iX)] = (iX != 0 || iY != 0 || iZ != 0) ? 1. / rTot : 0.0;
+        double externalPhi = 0.;
+        if (p->nbunches != 0) { /* :184-196 */
+          for (int iB = -p->nbunches / 2; iB <= p->nbunches / 2; iB++) {
+            if (iB == 0) continue;
+            double rZt = rZ + iB * p->lambda;
+            double rTotExt = sqrt(rX2 + rY2 + rZt * rZt);
+            if (rTotExt > 1.0e-10) externalPhi += 1.0 / rTotExt;
+          }
+        }
+        G[iZ + (size_t)M2 * (iY + (size_t)M1 * iX)] =
+            (iX != 0 || iY != 0 || iZ != 0) ? 1. / rTot + externalPhi : externalPhi;
       }
       for (int iX = M0 / 2 + 1; iX < M0; iX++)
         G[iZ + (size_t)M2 * (iY + (size_t)M1 * iX)] = G[iZ + (size_t)M2 * (iY + (size_t)M1 * (M0 - iX))];
@@ -954,6 +980,11 @@ void orc_poisson3d_destroy(orc_poisson3d* p) {
   if (!p) return;
   free(p->in); free(p->res); free(p->ghat); free(p->spec); free(p);
 }
+void orc_poisson3d_set_ext_bunches(orc_poisson3d* p, int n) { /* :101-106 */
+  if (n % 2 != 0) n = n + 1;
+  p->nbunches = n;
+}
+void orc_poisson3d_set_ext_spacing(orc_poisson3d* p, double lambda) { p->lambda = lambda; }
 void orc_poisson3d_limits(const orc_poisson3d* p, double o[6]) {
   o[0] = p->xMin; o[1] = p->xMax; o[2] = p->yMin; o[3] = p->yMax; o[4] = p->zMin; o[5] = p->zMax;
 }
@@ -1161,9 +1192,10 @@ int orc_sc2p5drb_get_grid(const orc_sc2p5drb* c, int which, double* out, double*
 void orc_sc2p5drb_get_moments(const orc_sc2p5drb* c, double o[4]) { memcpy(o, c->mom, sizeof(c->mom)); }
 
 /* ------------------------------------------------------------------------ */
-/* SpaceChargeCalc3D: src/spacecharge/SpaceChargeCalc3D.cc (nBunches_ = 0) */
+/* SpaceChargeCalc3D: src/spacecharge/SpaceChargeCalc3D.cc */
 struct orc_sc3d {
   double xy_ratio, xz_ratio, ratio_limit;
+  double frequency; /* frequency_ (:58) */
   orc_poisson3d* solver;
   orc_grid3d rho, phi;
   int rebuilds;
@@ -1171,6 +1203,7 @@ struct orc_sc3d {
 orc_sc3d* orc_sc3d_create(int nx, int ny, int nz) { /* :32-58 */
   orc_sc3d* c = (orc_sc3d*)calloc(1, sizeof(orc_sc3d));
   c->xy_ratio = 1.0; c->xz_ratio = 1.0; c->ratio_limit = 1.2;
+  c->frequency = 402.5e+6;
   c->solver = orc_poisson3d_create(nx, ny, nz, -c->xy_ratio, c->xy_ratio, -1. / c->xy_ratio,
                                    1. / c->xy_ratio, -1. / c->xz_ratio, 1. / c->xz_ratio);
   orc_grid3d* gs[2] = {&c->rho, &c->phi};
@@ -1189,6 +1222,11 @@ void orc_sc3d_destroy(orc_sc3d* c) {
   free(c->rho.arr); free(c->phi.arr); free(c);
 }
 void orc_sc3d_set_ratio_limit(orc_sc3d* c, double r) { c->ratio_limit = r; }
+void orc_sc3d_set_num_ext_bunches(orc_sc3d* c, int n) { orc_poisson3d_set_ext_bunches(c->solver, n); }
+int orc_sc3d_get_num_ext_bunches(const orc_sc3d* c) { return c->solver->nbunches; }
+void orc_sc3d_set_freq(orc_sc3d* c, double f) { c->frequency = f; }
+double orc_sc3d_get_freq(const orc_sc3d* c) { return c->frequency; }
+#define ORC_C_LIGHT 2.99792458e+8 /* OrbitConst.cc:27 */
 int orc_sc3d_green_rebuilds(const orc_sc3d* c) { return c->rebuilds; }
 
 /* :150-249 */
@@ -1242,19 +1280,63 @@ static int sc3d_analysis(orc_sc3d* c, orc_bunch* b) {
   orc_grid3d_set_z(&c->rho, zMin, zMax); orc_grid3d_set_z(&c->phi, zMin, zMax);
   return 0;
 }
+/* wrappedBunchAnalysis :251-328 */
+static int sc3d_wrapped_analysis(orc_sc3d* c, orc_bunch* b) {
+  double e[6];
+  orc_extrema_xyz(b, e);
+  double xMin = e[0], xMax = e[1], yMin = e[2], yMax = e[3], zMin = e[4], zMax = e[5];
+  c->rho.wrap = 1;
+  c->phi.wrap = 1;
+  double gamma = b->s.gamma, beta = b->s.beta;
+  double center = (zMax + zMin) / 2.0;
+  double width = (zMax - zMin) / 2.0;
+  zMin = center - width;
+  zMax = center + width;
+  double lambda = ORC_C_LIGHT * beta / c->frequency;
+  double lambda_cm = ORC_C_LIGHT * beta * gamma / c->frequency;
+  if (fabs(zMax) > lambda / 2 || fabs(zMin) > lambda / 2) {
+    double extension_coeff = 1.0001;
+    center = 0.;
+    width = extension_coeff * lambda / 2;
+    zMin = center - width;
+    zMax = center + width;
+  }
+  if (xMin >= xMax || yMin >= yMax || zMin >= zMax) return -1;
+  orc_grid3d_set_x(&c->rho, xMin, xMax); orc_grid3d_set_x(&c->phi, xMin, xMax);
+  orc_grid3d_set_y(&c->rho, yMin, yMax); orc_grid3d_set_y(&c->phi, yMin, yMax);
+  orc_grid3d_set_z(&c->rho, zMin, zMax); orc_grid3d_set_z(&c->phi, zMin, zMax);
+  memset(c->rho.arr, 0, sizeof(double) * (size_t)c->rho.nx * c->rho.ny * c->rho.nz);
+  orc_grid3d_bin_bunch_wrapped(&c->rho, b, lambda);
+  center = (zMax + zMin) / 2.0;
+  width = (zMax - zMin) * gamma / 2.0;
+  zMin = center - width; zMax = center + width;
+  orc_grid3d_set_z(&c->rho, zMin, zMax); orc_grid3d_set_z(&c->phi, zMin, zMax);
+  orc_poisson3d_set_ext_spacing(c->solver, lambda_cm);
+  orc_poisson3d_set_grid(c->solver, c->rho.xMin, c->rho.xMax, c->rho.yMin, c->rho.yMax,
+                         c->rho.zMin, c->rho.zMax);
+  c->rebuilds++;
+  return 0;
+}
 /* :100-148 */
 int orc_sc3d_track(orc_sc3d* c, orc_bunch* b, double length) {
   if (b->n < 2) return 0;
-  if (sc3d_analysis(c, b) != 0) return -1;
+  int next = c->solver->nbunches;
+  if (next > 0) {
+    if (sc3d_wrapped_analysis(c, b) != 0) return -1;
+  } else if (sc3d_analysis(c, b) != 0) {
+    return -1;
+  }
   orc_poisson3d_find_potential(c->solver, &c->rho, &c->phi);
   double gamma = b->s.gamma, beta = b->s.beta;
   double trans_factor = length * b->s.classical_radius / ((beta * beta) * (gamma * gamma));
   double long_factor = length * b->s.classical_radius * b->s.mass;
   double z_center = (c->phi.zMax + c->phi.zMin) / 2.0;
+  double lambda_cm = ORC_C_LIGHT * beta * gamma / c->frequency;
 #pragma omp parallel for num_threads(g_threads) if (g_threads > 1)
   for (int64_t i = 0; i < b->n; i++) {
     double x = b->x[i], y = b->y[i];
     double z = (b->z[i] - z_center) * gamma + z_center;
+    if (next > 0) z = remainder(z, lambda_cm);
     double ex, ey, ez;
     orc_grid3d_gradient(&c->phi, x, y, z, &ex, &ey, &ez);
     b->xp[i] += -ex * trans_factor;
@@ -1320,6 +1402,7 @@ static orc_grid2d mk2(int nx, int ny, const double* l, double* arr) {
 static orc_grid3d mk3(int nx, int ny, int nz, const double* l, double* arr) {
   orc_grid3d g;
   g.nx = nx; g.ny = ny; g.nz = nz; g.arr = arr;
+  g.wrap = 0;
   orc_grid3d_set_x(&g, l[0], l[1]);
   orc_grid3d_set_y(&g, l[2], l[3]);
   orc_grid3d_set_z(&g, l[4], l[5]);
@@ -1383,6 +1466,22 @@ void orc_flat_grid3d_grad(int nx, int ny, int nz, const double* l, const double*
   for (long i = 0; i < n; i++) orc_grid3d_gradient(&g, x[i], y[i], z[i], gx + i, gy + i, gz + i);
 }
 
+void orc_flat_grid3d_bin_wrapped(int nx, int ny, int nz, const double* l, long n, const double* x,
+                                 const double* y, const double* z, const double* m, double lambda,
+                                 double* rho) {
+  memset(rho, 0, sizeof(double) * (size_t)nx * ny * nz);
+  orc_grid3d g = mk3(nx, ny, nz, l, rho);
+  g.wrap = 1;
+  for (long i = 0; i < n; i++) orc_grid3d_bin_value(&g, m[i], x[i], y[i], remainder(z[i], lambda));
+}
+void orc_flat_grid3d_grad_wrapped(int nx, int ny, int nz, const double* l, const double* arr,
+                                  long n, const double* x, const double* y, const double* z,
+                                  double* gx, double* gy, double* gz) {
+  orc_grid3d g = mk3(nx, ny, nz, l, (double*)arr);
+  g.wrap = 1;
+  for (long i = 0; i < n; i++) orc_grid3d_gradient(&g, x[i], y[i], z[i], gx + i, gy + i, gz + i);
+}
+
 /* calculateLongDerivative on a given line density (SpaceChargeCalc2p5Drb.cc:257-310). */
 void orc_flat_long_derivative(int nz, double zmin, double zmax, int n_avg, const double* zg,
                               double* zd) {
@@ -1408,6 +1507,16 @@ void orc_flat_poisson2d(int nx, int ny, const double* sl, const double* rl, cons
 void orc_flat_poisson3d(int nx, int ny, int nz, const double* sl, const double* rl,
                         const double* rho, double* phi) {
   orc_poisson3d* p = orc_poisson3d_create(nx, ny, nz, sl[0], sl[1], sl[2], sl[3], sl[4], sl[5]);
+  orc_grid3d r = mk3(nx, ny, nz, rl, (double*)rho), f = mk3(nx, ny, nz, rl, phi);
+  orc_poisson3d_find_potential(p, &r, &f);
+  orc_poisson3d_destroy(p);
+}
+void orc_flat_poisson3d_images(int nx, int ny, int nz, const double* sl, const double* rl,
+                               int nbunches, double lambda, const double* rho, double* phi) {
+  orc_poisson3d* p = orc_poisson3d_create(nx, ny, nz, sl[0], sl[1], sl[2], sl[3], sl[4], sl[5]);
+  orc_poisson3d_set_ext_bunches(p, nbunches);
+  orc_poisson3d_set_ext_spacing(p, lambda);
+  orc_poisson3d_set_grid(p, sl[0], sl[1], sl[2], sl[3], sl[4], sl[5]);
   orc_grid3d r = mk3(nx, ny, nz, rl, (double*)rho), f = mk3(nx, ny, nz, rl, phi);
   orc_poisson3d_find_potential(p, &r, &f);
   orc_poisson3d_destroy(p);

@@ -92,6 +92,7 @@ typedef struct {
   int nx, ny, nz;
   double xMin, xMax, yMin, yMax, zMin, zMax, dx, dy, dz;
   double* arr;
+  int wrap; /* longWrapping (Grid3D.cc:187-197): periodic z clamp and stencil */
 } orc_grid3d;
 void orc_grid3d_set_x(orc_grid3d* g, double a, double b);
 void orc_grid3d_set_y(orc_grid3d* g, double a, double b);
@@ -99,6 +100,8 @@ void orc_grid3d_set_z(orc_grid3d* g, double a, double b);
 void orc_grid3d_ind_frac(const orc_grid3d* g, double x, double y, double z, int ind[3], double frac[3]);
 void orc_grid3d_bin_value(orc_grid3d* g, double m, double x, double y, double z);
 void orc_grid3d_bin_bunch(orc_grid3d* g, const orc_bunch* b);
+/* binBunch(bunch, lambda) with longWrapping = 1 (Grid3D.cc:281-305): z -> remainder(z, lambda) */
+void orc_grid3d_bin_bunch_wrapped(orc_grid3d* g, const orc_bunch* b, double lambda);
 void orc_grid3d_gradient(const orc_grid3d* g, double x, double y, double z,
                          double* gx, double* gy, double* gz);
 
@@ -125,6 +128,10 @@ void orc_poisson3d_set_grid(orc_poisson3d* p, double xMin, double xMax, double y
                             double zMin, double zMax);
 void orc_poisson3d_find_potential(orc_poisson3d* p, const orc_grid3d* rho, orc_grid3d* phi);
 void orc_poisson3d_limits(const orc_poisson3d* p, double out6[6]);
+/* setNumberOfExternalBunches (odd counts round up, PoissonSolverFFT3D.cc:101-106) and
+ * setSpacingOfExternalBunches (:108-110); both take effect at the next Green rebuild. */
+void orc_poisson3d_set_ext_bunches(orc_poisson3d* p, int n);
+void orc_poisson3d_set_ext_spacing(orc_poisson3d* p, double lambda);
 
 /* ---- SpaceChargeCalc2p5Drb (src/spacecharge/SpaceChargeCalc2p5Drb.cc) ---- */
 typedef struct orc_sc2p5drb orc_sc2p5drb;
@@ -148,6 +155,12 @@ int orc_sc3d_track(orc_sc3d* c, orc_bunch* b, double length);
 /* which: 0 rho, 1 phi; lims: xMin,xMax,yMin,yMax,zMin,zMax (rest frame z) */
 int64_t orc_sc3d_get_grid(const orc_sc3d* c, int which, double* out, double* lims);
 int orc_sc3d_green_rebuilds(const orc_sc3d* c);
+/* numExtBunches / freqOfBunches (SpaceChargeCalc3D.cc:84-97); n > 0 selects
+ * wrappedBunchAnalysis (:251-328) and the periodic z kick (:138-140). */
+void orc_sc3d_set_num_ext_bunches(orc_sc3d* c, int n);
+int orc_sc3d_get_num_ext_bunches(const orc_sc3d* c);
+void orc_sc3d_set_freq(orc_sc3d* c, double f);
+double orc_sc3d_get_freq(const orc_sc3d* c);
 
 /* ---- BunchTwissAnalysis emittance (src/orbit/BunchDiagnostics/BunchTwissAnalysis.cc) ---- */
 void orc_twiss_emittances(const orc_bunch* b, double out3[3]);
@@ -164,6 +177,16 @@ void orc_flat_grid1d_bin(int nz, double zmin, double zmax, long n, const double*
                          double* out);
 void orc_flat_grid1d_value(int nz, double zmin, double zmax, const double* arr, long n,
                            const double* z, double* val, double* grad);
+/* longWrapping = 1 variants: bin with z -> remainder(z, lambda); gradient with periodic z */
+void orc_flat_grid3d_bin_wrapped(int nx, int ny, int nz, const double* l, long n, const double* x,
+                                 const double* y, const double* z, const double* m, double lambda,
+                                 double* rho);
+void orc_flat_grid3d_grad_wrapped(int nx, int ny, int nz, const double* l, const double* arr,
+                                  long n, const double* x, const double* y, const double* z,
+                                  double* gx, double* gy, double* gz);
+/* 3D solve with external-bunch images in the Green function */
+void orc_flat_poisson3d_images(int nx, int ny, int nz, const double* sl, const double* rl,
+                               int nbunches, double lambda, const double* rho, double* phi);
 void orc_flat_grid3d_bin(int nx, int ny, int nz, const double* l, long n, const double* x,
                          const double* y, const double* z, const double* m, double* rho);
 void orc_flat_grid3d_indfrac(int nx, int ny, int nz, const double* l, long n, const double* x,

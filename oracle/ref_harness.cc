@@ -6,6 +6,7 @@
 // Only the Grid classes build here: Bunch (and so teapotbase and
 // BunchExtremaCalculator) needs the Python-2 C-API MPI communicator wrapper, and
 // PoissonSolverFFT2D/3D need fftw3.h; neither exists in this image.
+#include <cmath>
 #include "Grid1D.hh"
 #include "Grid2D.hh"
 #include "Grid3D.hh"
@@ -106,6 +107,38 @@ REF_API void ref_grid3d_grad(int nx, int ny, int nz, const double* lims, const d
   g.setGridX(lims[0], lims[1]);
   g.setGridY(lims[2], lims[3]);
   g.setGridZ(lims[4], lims[5]);
+  for (int iz = 0; iz < nz; iz++)
+    for (int ix = 0; ix < nx; ix++)
+      for (int iy = 0; iy < ny; iy++) g.setValue(arr[((long)iz * nx + ix) * ny + iy], ix, iy, iz);
+  for (long i = 0; i < n; i++) g.calcGradient(x[i], gx[i], y[i], gy[i], z[i], gz[i]);
+}
+
+// Grid3D::binBunch(bunch, lambda) with longWrapping = 1 (Grid3D.cc:281-305): z is wrapped
+// with remainder(z, lambda) and the z stencil is periodic.
+REF_API void ref_grid3d_bin_wrapped(int nx, int ny, int nz, const double* lims, long n,
+                                    const double* x, const double* y, const double* z,
+                                    const double* m, double lambda, double* rho) {
+  Grid3D g(nx, ny, nz);
+  g.setGridX(lims[0], lims[1]);
+  g.setGridY(lims[2], lims[3]);
+  g.setGridZ(lims[4], lims[5]);
+  g.setZero();
+  g.setLongWrapping(1);
+  for (long i = 0; i < n; i++) g.binValue(m[i], x[i], y[i], remainder(z[i], lambda));
+  for (int iz = 0; iz < nz; iz++)
+    for (int ix = 0; ix < nx; ix++)
+      for (int iy = 0; iy < ny; iy++) rho[((long)iz * nx + ix) * ny + iy] = g.getValueOnGrid(ix, iy, iz);
+}
+
+// Grid3D::calcGradient with longWrapping = 1.
+REF_API void ref_grid3d_grad_wrapped(int nx, int ny, int nz, const double* lims, const double* arr,
+                                     long n, const double* x, const double* y, const double* z,
+                                     double* gx, double* gy, double* gz) {
+  Grid3D g(nx, ny, nz);
+  g.setGridX(lims[0], lims[1]);
+  g.setGridY(lims[2], lims[3]);
+  g.setGridZ(lims[4], lims[5]);
+  g.setLongWrapping(1);
   for (int iz = 0; iz < nz; iz++)
     for (int ix = 0; ix < nx; ix++)
       for (int iy = 0; iy < ny; iy++) g.setValue(arr[((long)iz * nx + ix) * ny + iy], ix, iy, iz);

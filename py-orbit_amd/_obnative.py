@@ -65,6 +65,10 @@ SIGNATURES = {
     "ob_sc3d_get_ratio_limit": [_vp, _D],
     "ob_sc3d_green_rebuilds": [_vp, C.POINTER(_i)],
     "ob_sc3d_set_deposit_mode": [_vp, _i, _i],
+    "ob_sc3d_set_num_ext_bunches": [_vp, _i],
+    "ob_sc3d_get_num_ext_bunches": [_vp, C.POINTER(_i)],
+    "ob_sc3d_set_freq": [_vp, _d],
+    "ob_sc3d_get_freq": [_vp, _D],
     "ob_profile_enable": [_i],
     "ob_profile_report": [C.c_char_p, _i],
     "ob_calibrate_copy": [_i64, _i, _D],

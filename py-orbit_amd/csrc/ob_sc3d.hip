@@ -28,6 +28,7 @@ namespace ob {
 namespace {
 
 constexpr int kThreads = 256;
+constexpr double kSpeedOfLight = 2.99792458e+8;  // OrbitConst.cc:27
 // Complex lines (= pairs of real z-lines) per block in the z passes.
 __host__ __device__ constexpr int zpass_pairs(int n) { return n >= 256 ? (n >= 2048 ? 1 : 2048 / n) : 8; }
 // Consecutive kz lines per block in the y and x passes (LDS <= 131 KiB), and threads.
@@ -48,10 +49,12 @@ __host__ __device__ constexpr int strided_line_doubles() {
 struct GreenGeom {
   int M0, M1, M2;
   double dx, dy, dz;
+  int nbunches;   // external bunches (images), even
+  double lambda;  // their spacing
 };
 
 __device__ __forceinline__ double green3d(int i, int j, int k, const GreenGeom& g) {
-  // PoissonSolverFFT3D::_defineGreenF (:160-247), nBunches_ = 0
+  // PoissonSolverFFT3D::_defineGreenF (:160-247)
   const int ii = (i <= g.M0 / 2) ? i : g.M0 - i;
   const int jj = (j <= g.M1 / 2) ? j : g.M1 - j;
   const int kk = (k <= g.M2 / 2) ? k : g.M2 - k;
@@ -59,7 +62,14 @@ __device__ __forceinline__ double green3d(int i, int j, int k, const GreenGeom& 
   const double rY = jj * g.dy, rY2 = rY * rY;
   const double rX = ii * g.dx, rX2 = rX * rX;
   const double rTot = sqrt(rX2 + rY2 + rZ2);
-  return (ii != 0 || jj != 0 || kk != 0) ? 1. / rTot : 0.0;
+  double externalPhi = 0.;
+  for (int b = -g.nbunches / 2; b <= g.nbunches / 2; b++) {  // :184-196
+    if (b == 0) continue;
+    const double rZt = rZ + b * g.lambda;
+    const double rTotExt = sqrt(rX2 + rY2 + rZt * rZt);
+    if (rTotExt > 1.0e-10) externalPhi += 1.0 / rTotExt;
+  }
+  return (ii != 0 || jj != 0 || kk != 0) ? 1. / rTot + externalPhi : externalPhi;
 }
 
 // ---- extrema reduction (one block) -------------------------------------------
@@ -100,19 +110,23 @@ struct Grid3 {
   double xMin, xMax, yMin, yMax, zMin, zMax, dx, dy, dz;
 };
 
+// lambda != 0: binBunch(bunch, lambda) with longWrapping = 1 (Grid3D.cc:281-305).
 __global__ __launch_bounds__(kThreads) void deposit3d(DeviceBunchView v, Grid3 g,
-                                                      double macrosize, double* __restrict__ rho) {
+                                                      double macrosize, double lambda,
+                                                      double* __restrict__ rho) {
   const int64_t stride = (int64_t)gridDim.x * blockDim.x;
   const int64_t sx = (int64_t)g.ny * g.nz, sy = g.nz;
+  const bool wrap = lambda != 0.0;
   for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < v.n; i += stride) {
-    const double x = v.c[0][i], y = v.c[2][i], z = v.c[4][i];
+    const double x = v.c[0][i], y = v.c[2][i];
+    const double z = wrap ? remainder(v.c[4][i], lambda) : v.c[4][i];
     if (x < g.xMin || x > g.xMax || y < g.yMin || y > g.yMax || z < g.zMin || z > g.zMax) continue;
     const double m = v.msize ? v.msize[i] : macrosize;
     int iX, iY, iZ;
     double xF, yF, zF;
     node_ind_frac(x, g.xMin, g.dx, g.nx, iX, xF);
     node_ind_frac(y, g.yMin, g.dy, g.ny, iY, yF);
-    grid3d_z_ind_frac(z, g.zMin, g.dz, g.nz, iZ, zF);
+    grid3d_z_ind_frac(z, g.zMin, g.dz, g.nz, iZ, zF, wrap);
     const double wx[3] = {0.5 * (0.5 - xF) * (0.5 - xF), 0.75 - xF * xF, 0.5 * (0.5 + xF) * (0.5 + xF)};
     const double wy[3] = {0.5 * (0.5 - yF) * (0.5 - yF), 0.75 - yF * yF, 0.5 * (0.5 + yF) * (0.5 + yF)};
     int iZm = iZ - 1, iZp = iZ + 1;
@@ -465,6 +479,10 @@ struct ob_sc3d {
   double *rho = nullptr, *phi = nullptr, *ghat = nullptr, *partials = nullptr, *ext_dev = nullptr;
   double* ext_host = nullptr;
   double2 *spec = nullptr, *tw = nullptr;
+  int nbunches = 0;                // external bunches (PoissonSolverFFT3D nBunches_, even)
+  double frequency = 402.5e+6;     // bunch frequency (SpaceChargeCalc3D.cc:58)
+  double lambda_cm = 0.0;          // image spacing used by the current Green function
+  bool phi_wrap = false;           // phiGrid longWrapping (set by wrapped calls, sticky)
   bool tiled = true;              // tile-sorted LDS deposit (else particle-order atomics)
   int lds_budget = 16 * 1024;     // LDS bytes for one deposit / kick tile
   ob::TileWork work;
@@ -525,7 +543,7 @@ void set_solver_grid(ob_sc3d* c, double xMin, double xMax, double yMin, double y
   c->sMin[0] = xMin; c->sMax[0] = xMax; c->sd[0] = (xMax - xMin) / (c->nx - 1);
   c->sMin[1] = yMin; c->sMax[1] = yMax; c->sd[1] = (yMax - yMin) / (c->ny - 1);
   c->sMin[2] = zMin; c->sMax[2] = zMax; c->sd[2] = (zMax - zMin) / c->nz;
-  GreenGeom gg{c->M0, c->M1, c->M2, c->sd[0], c->sd[1], c->sd[2]};
+  GreenGeom gg{c->M0, c->M1, c->M2, c->sd[0], c->sd[1], c->sd[2], c->nbunches, c->lambda_cm};
   hipStream_t s = stream();
   dispatch_pow2(c->M2, [&](auto NC) {
     constexpr int N = decltype(NC)::value;
@@ -699,42 +717,69 @@ OB_API int ob_sc3d_track(ob_sc3d* c, ob_bunch* b, double length) {
   OB_HIP(hipStreamSynchronize(s));
   const ob::SyncInfo sy = ob::bunch_sync(b);
   const double gamma = sy.gamma, beta = sy.beta;
-  // bunchAnalysis (:150-249), scalar arithmetic as in the reference
   double xMin = c->ext_host[0], xMax = c->ext_host[1], yMin = c->ext_host[2];
   double yMax = c->ext_host[3], zMin = c->ext_host[4], zMax = c->ext_host[5];
-  double center = (zMax + zMin) / 2.0;
-  double width = (zMax - zMin) * gamma / 2.0;
-  zMin = center - width;
-  zMax = center + width;
-  OB_REQUIRE(!(xMin >= xMax || yMin >= yMax || zMin >= zMax),
-             "SpaceChargeCalc3D::bunchAnalysis: the bunch min and max sizes are wrong (zero "
-             "extent); cannot calculate space charge");
-  const double xy_ratio_beam = (xMax - xMin) / (yMax - yMin);
-  const double xz_ratio_beam = (xMax - xMin) / (zMax - zMin);
-  if (c->ratio_limit < xy_ratio_beam / c->xy_ratio || 1.0 / c->ratio_limit > xy_ratio_beam / c->xy_ratio ||
-      c->ratio_limit < xz_ratio_beam / c->xz_ratio || 1.0 / c->ratio_limit > xz_ratio_beam / c->xz_ratio) {
-    c->xy_ratio = xy_ratio_beam;
-    c->xz_ratio = xz_ratio_beam;
-    ob::set_solver_grid(c, -(xMax - xMin) / 2.0, (xMax - xMin) / 2.0, -(yMax - yMin) / 2.0,
-                        (yMax - yMin) / 2.0, -(zMax - zMin) / 2.0, (zMax - zMin) / 2.0);
-  }
-  const double scx = (xMax - xMin) / (c->sMax[0] - c->sMin[0]);
-  const double scy = (yMax - yMin) / (c->sMax[1] - c->sMin[1]);
-  const double scz = (zMax - zMin) / (c->sMax[2] - c->sMin[2]);
-  double sc = scx;
-  if (sc < scy) sc = scy;
-  if (sc < scz) sc = scz;
+  double center, width;
+  const bool wrapped = c->nbunches > 0;
+  double lambda = 0.0, lambda_cm = 0.0;
   ob::Grid3 g;
   g.nx = c->nx; g.ny = c->ny; g.nz = c->nz;
-  center = (xMax + xMin) / 2.0;
-  width = (c->sMax[0] - c->sMin[0]) * sc / 2.0;
-  g.xMin = center - width; g.xMax = center + width; g.dx = (g.xMax - g.xMin) / (c->nx - 1);
-  center = (yMax + yMin) / 2.0;
-  width = (c->sMax[1] - c->sMin[1]) * sc / 2.0;
-  g.yMin = center - width; g.yMax = center + width; g.dy = (g.yMax - g.yMin) / (c->ny - 1);
-  center = (zMax + zMin) / 2.0;
-  width = (c->sMax[2] - c->sMin[2]) * sc / (2.0 * gamma);
-  g.zMin = center - width; g.zMax = center + width; g.dz = (g.zMax - g.zMin) / c->nz;
+  if (wrapped) {
+    // wrappedBunchAnalysis (:251-328), scalar arithmetic as in the reference
+    c->phi_wrap = true;
+    center = (zMax + zMin) / 2.0;
+    width = (zMax - zMin) / 2.0;
+    zMin = center - width;
+    zMax = center + width;
+    lambda = ob::kSpeedOfLight * beta / c->frequency;
+    lambda_cm = ob::kSpeedOfLight * beta * gamma / c->frequency;
+    if (fabs(zMax) > lambda / 2 || fabs(zMin) > lambda / 2) {
+      const double extension_coeff = 1.0001;
+      center = 0.;
+      width = extension_coeff * lambda / 2;
+      zMin = center - width;
+      zMax = center + width;
+    }
+    OB_REQUIRE(!(xMin >= xMax || yMin >= yMax || zMin >= zMax),
+               "SpaceChargeCalc3D::wrappedBunchAnalysis: the bunch min and max sizes are wrong "
+               "(zero extent); cannot calculate space charge");
+    g.xMin = xMin; g.xMax = xMax; g.dx = (g.xMax - g.xMin) / (c->nx - 1);
+    g.yMin = yMin; g.yMax = yMax; g.dy = (g.yMax - g.yMin) / (c->ny - 1);
+    g.zMin = zMin; g.zMax = zMax; g.dz = (g.zMax - g.zMin) / c->nz;
+  } else {
+    // bunchAnalysis (:150-249), scalar arithmetic as in the reference
+    center = (zMax + zMin) / 2.0;
+    width = (zMax - zMin) * gamma / 2.0;
+    zMin = center - width;
+    zMax = center + width;
+    OB_REQUIRE(!(xMin >= xMax || yMin >= yMax || zMin >= zMax),
+               "SpaceChargeCalc3D::bunchAnalysis: the bunch min and max sizes are wrong (zero "
+               "extent); cannot calculate space charge");
+    const double xy_ratio_beam = (xMax - xMin) / (yMax - yMin);
+    const double xz_ratio_beam = (xMax - xMin) / (zMax - zMin);
+    if (c->ratio_limit < xy_ratio_beam / c->xy_ratio || 1.0 / c->ratio_limit > xy_ratio_beam / c->xy_ratio ||
+        c->ratio_limit < xz_ratio_beam / c->xz_ratio || 1.0 / c->ratio_limit > xz_ratio_beam / c->xz_ratio) {
+      c->xy_ratio = xy_ratio_beam;
+      c->xz_ratio = xz_ratio_beam;
+      ob::set_solver_grid(c, -(xMax - xMin) / 2.0, (xMax - xMin) / 2.0, -(yMax - yMin) / 2.0,
+                          (yMax - yMin) / 2.0, -(zMax - zMin) / 2.0, (zMax - zMin) / 2.0);
+    }
+    const double scx = (xMax - xMin) / (c->sMax[0] - c->sMin[0]);
+    const double scy = (yMax - yMin) / (c->sMax[1] - c->sMin[1]);
+    const double scz = (zMax - zMin) / (c->sMax[2] - c->sMin[2]);
+    double sc = scx;
+    if (sc < scy) sc = scy;
+    if (sc < scz) sc = scz;
+    center = (xMax + xMin) / 2.0;
+    width = (c->sMax[0] - c->sMin[0]) * sc / 2.0;
+    g.xMin = center - width; g.xMax = center + width; g.dx = (g.xMax - g.xMin) / (c->nx - 1);
+    center = (yMax + yMin) / 2.0;
+    width = (c->sMax[1] - c->sMin[1]) * sc / 2.0;
+    g.yMin = center - width; g.yMax = center + width; g.dy = (g.yMax - g.yMin) / (c->ny - 1);
+    center = (zMax + zMin) / 2.0;
+    width = (c->sMax[2] - c->sMin[2]) * sc / (2.0 * gamma);
+    g.zMin = center - width; g.zMax = center + width; g.dz = (g.zMax - g.zMin) / c->nz;
+  }
   // deposit in lab z
   const int64_t ng = (int64_t)c->nx * c->ny * c->nz;
   {
@@ -743,7 +788,8 @@ OB_API int ob_sc3d_track(ob_sc3d* c, ob_bunch* b, double length) {
   }
   const ob::TileGeom tg = ob::make_tile_geom(g.nx, g.ny, g.nz, g.xMin, g.xMax, g.yMin, g.yMax,
                                              g.zMin, g.zMax, g.dx, g.dy, g.dz, c->lds_budget);
-  if (c->tiled && tg.T > 0) {
+  const bool tiled = c->tiled && tg.T > 0 && !wrapped;
+  if (tiled) {
     ob::sort_and_deposit_3d(b, tg, sy.macrosize, c->rho, c->work);
   } else {  // particle-order deposit with global atomics
     ob::DeviceBunchView v = ob::bunch_device_view(b);
@@ -752,7 +798,7 @@ OB_API int ob_sc3d_track(ob_sc3d* c, ob_bunch* b, double length) {
     if (dblk > cap) dblk = cap;
     ob::ProfScope ps_("deposit3d");
     hipLaunchKernelGGL(ob::deposit3d, dim3(dblk), dim3(ob::kThreads), 0, s, v, g, sy.macrosize,
-                       c->rho);
+                       lambda, c->rho);
   }
   OB_HIP(hipGetLastError());
   // move the z grid to the rest frame (:242-248)
@@ -762,13 +808,17 @@ OB_API int ob_sc3d_track(ob_sc3d* c, ob_bunch* b, double length) {
   const double rdz = (rzMax - rzMin) / c->nz;
   c->g[0] = g.xMin; c->g[1] = g.xMax; c->g[2] = g.yMin; c->g[3] = g.yMax;
   c->g[4] = rzMin; c->g[5] = rzMax; c->g[6] = g.dx; c->g[7] = g.dy; c->g[8] = rdz;
+  if (wrapped) {  // the solver takes the rho grid and the image spacing (:326-330)
+    c->lambda_cm = lambda_cm;
+    ob::set_solver_grid(c, g.xMin, g.xMax, g.yMin, g.yMax, rzMin, rzMax);
+  }
   // findPotential (PoissonSolverFFT3D.cc:249-347)
   const double scale_coeff = c->sd[0] / g.dx;
   const double denom = scale_coeff / ((double)c->M0 * c->M1 * c->M2);
   ob::solve(c, denom);
   // kick (:120-147): over the deposit's tiles when the bunch was tile-sorted, else fused
   // into the next map kernel
-  if (c->tiled && tg.T > 0) {
+  if (tiled && !c->phi_wrap) {
     const ob::Kick3D k{c->phi, rzMin, rdz, (rzMax + rzMin) / 2.0, gamma,
                        length * sy.r0 / ((beta * beta) * (gamma * gamma)),  // :121
                        length * sy.r0 * sy.mass};                           // :122
@@ -778,7 +828,7 @@ OB_API int ob_sc3d_track(ob_sc3d* c, ob_bunch* b, double length) {
   ob::Op op;
   std::memset(&op, 0, sizeof(op));
   op.kind = ob::OP_SC3D_KICK;
-  op.i0 = c->nx; op.i1 = c->ny; op.i2 = c->nz;
+  op.i0 = c->nx; op.i1 = c->ny; op.i2 = c->nz | (c->phi_wrap ? ob::kSC3DWrapBit : 0);
   op.p[0] = ob::ptr_bits(c->phi);
   op.p[1] = g.xMin; op.p[2] = g.dx; op.p[3] = g.yMin; op.p[4] = g.dy;
   op.p[5] = rzMin; op.p[6] = rdz;
@@ -786,6 +836,7 @@ OB_API int ob_sc3d_track(ob_sc3d* c, ob_bunch* b, double length) {
   op.p[8] = gamma;
   op.p[9] = length * sy.r0 / ((beta * beta) * (gamma * gamma));     // :121
   op.p[10] = length * sy.r0 * sy.mass;                               // :122
+  op.p[11] = wrapped ? ob::kSpeedOfLight * beta * gamma / c->frequency : 0.0;  // :123, 138-140
   ob::bunch_queue(b, op);
   OB_CATCH
 }
@@ -823,6 +874,29 @@ OB_API int ob_sc3d_get_ratio_limit(ob_sc3d* c, double* r) {
 OB_API int ob_sc3d_green_rebuilds(ob_sc3d* c, int* n) {
   OB_TRY OB_REQUIRE(c != nullptr, "null handle");
   *n = c->rebuilds;
+  OB_CATCH
+}
+
+OB_API int ob_sc3d_set_num_ext_bunches(ob_sc3d* c, int n) {
+  OB_TRY OB_REQUIRE(c != nullptr, "null handle");
+  OB_REQUIRE(n >= 0, "number of external bunches must be >= 0");
+  if (n % 2 != 0) n = n + 1;  // PoissonSolverFFT3D.cc:101-106
+  c->nbunches = n;
+  OB_CATCH
+}
+OB_API int ob_sc3d_get_num_ext_bunches(ob_sc3d* c, int* n) {
+  OB_TRY OB_REQUIRE(c != nullptr && n != nullptr, "null handle");
+  *n = c->nbunches;
+  OB_CATCH
+}
+OB_API int ob_sc3d_set_freq(ob_sc3d* c, double f) {
+  OB_TRY OB_REQUIRE(c != nullptr, "null handle");
+  c->frequency = f;
+  OB_CATCH
+}
+OB_API int ob_sc3d_get_freq(ob_sc3d* c, double* f) {
+  OB_TRY OB_REQUIRE(c != nullptr && f != nullptr, "null handle");
+  *f = c->frequency;
   OB_CATCH
 }
 

@@ -97,15 +97,20 @@ __device__ __forceinline__ void grid2d_gradient(const double* __restrict__ a, in
   ey = e / dy;
 }
 
-// Grid3D::getGridIndAndFrac z part (Grid3D.cc:236-263), longWrapping = 0.
+// Grid3D::getGridIndAndFrac z part (Grid3D.cc:236-263); wrap = longWrapping.
 __device__ __forceinline__ void grid3d_z_ind_frac(double z, double zMin, double dz, int nz,
-                                                  int& zi, double& zf) {
+                                                  int& zi, double& zf, bool wrap = false) {
   if (nz > 1) {
     zi = (int)((z - zMin) / dz);
     zf = 0.0;
     if (nz > 2) {
-      if (zi <= 0) zi = 1;
-      if (zi >= nz - 1) zi = nz - 2;
+      if (!wrap) {
+        if (zi <= 0) zi = 1;
+        if (zi >= nz - 1) zi = nz - 2;
+      } else {
+        if (zi <= 0) zi = 0;
+        if (zi >= nz - 1) zi = nz - 1;
+      }
       zf = (z - (zMin + (zi + 0.5) * dz)) / dz;
     }
     if (nz == 2) {
@@ -208,12 +213,12 @@ __device__ __forceinline__ void grid3d_gradient(const double* __restrict__ a, in
                                                 int nz, double xMin, double dx, double yMin,
                                                 double dy, double zMin, double dz, double x,
                                                 double y, double z, double& gX, double& gY,
-                                                double& gZ) {
+                                                double& gZ, bool wrap = false) {
   int iX, iY, iZ;
   double xF, yF, zF;
   node_ind_frac(x, xMin, dx, nx, iX, xF);
   node_ind_frac(y, yMin, dy, ny, iY, yF);
-  grid3d_z_ind_frac(z, zMin, dz, nz, iZ, zF);
+  grid3d_z_ind_frac(z, zMin, dz, nz, iZ, zF, wrap);
   const Grid3View v{a, (int64_t)ny * nz, nz, 0, 0, 0};
   grid3d_gradient_at(v, nz, iX, iY, iZ, xF, yF, zF, dx, dy, dz, gX, gY, gZ);
 }
@@ -247,14 +252,21 @@ __device__ __forceinline__ void sc2p5drb_kick(const Op& op, double x, double& xp
 
 // SpaceChargeCalc3D kick loop body (SpaceChargeCalc3D.cc:132-147).
 // op: p[0]=phi (z-fastest), p[1..6]=xMin,dx,yMin,dy,zMin,dz of the rest-frame phi grid,
-//     p[7]=z_center, p[8]=gamma, p[9]=trans_factor, p[10]=long_factor; i0=nx, i1=ny, i2=nz.
+//     p[7]=z_center, p[8]=gamma, p[9]=trans_factor, p[10]=long_factor, p[11]=spacing of
+//     external bunches in the rest frame (0: none); i0=nx, i1=ny, i2=nz | kSC3DWrapBit when
+//     the potential grid wraps periodically in z (Grid3D longWrapping).
+constexpr int kSC3DWrapBit = 1 << 30;
+constexpr int kSC3DNzMask = kSC3DWrapBit - 1;
 __device__ __forceinline__ void sc3d_kick(const Op& op, double x, double& xp, double y,
                                           double& yp, double z, double& dE) {
-  const double zc = op.p[7], gamma = op.p[8];
-  const double zr = (z - zc) * gamma + zc;
+  const double zc = op.p[7], gamma = op.p[8], lambda_cm = op.p[11];
+  double zr = (z - zc) * gamma + zc;
+  if (lambda_cm != 0.0) zr = remainder(zr, lambda_cm);  // external bunches (:138-140)
   double ex, ey, ez;
-  grid3d_gradient(op_ptr<const double*>(op.p[0]), op.i0, op.i1, op.i2, op.p[1], op.p[2], op.p[3],
-                  op.p[4], op.p[5], op.p[6], x, y, zr, ex, ey, ez);
+  const int nz = op.i2 & kSC3DNzMask;
+  const bool wrap = (op.i2 & kSC3DWrapBit) != 0;
+  grid3d_gradient(op_ptr<const double*>(op.p[0]), op.i0, op.i1, nz, op.p[1], op.p[2], op.p[3],
+                  op.p[4], op.p[5], op.p[6], x, y, zr, ex, ey, ez, wrap);
   xp += -ex * op.p[9];
   yp += -ey * op.p[9];
   dE += -ez * op.p[10];

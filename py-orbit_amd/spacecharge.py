@@ -213,7 +213,6 @@ class SpaceChargeCalc3D:
         h = C.c_void_p()
         N.call("ob_sc3d_create", self._n[0], self._n[1], self._n[2], C.byref(h))
         self._h = h
-        self._freq = 402.5e6
 
     def __del__(self):
         h = getattr(self, "_h", None)
@@ -248,15 +247,21 @@ class SpaceChargeCalc3D:
         return float(v.value)
 
     def numExtBunches(self, n=None):
-        """Neighbour-bunch images (wrappedBunchAnalysis, :251-331) are not built: only 0."""
-        if n is not None and int(n) != 0:
-            raise NotImplementedError("SpaceChargeCalc3D: external bunches are not supported")
-        return 0
+        """numExtBunches([n]) (wrap_spacechargecalc3d.cc): n > 0 accounts for n neighbour
+        bunches (wrappedBunchAnalysis, SpaceChargeCalc3D.cc:251-328); odd n rounds up."""
+        if n is not None:
+            N.call("ob_sc3d_set_num_ext_bunches", self._h, int(n))
+        v = C.c_int()
+        N.call("ob_sc3d_get_num_ext_bunches", self._h, C.byref(v))
+        return int(v.value)
 
     def freqOfBunches(self, f=None):
+        """freqOfBunches([f]): bunch frequency in Hz (default 402.5e6, :58)."""
         if f is not None:
-            self._freq = float(f)
-        return self._freq
+            N.call("ob_sc3d_set_freq", self._h, float(f))
+        v = C.c_double()
+        N.call("ob_sc3d_get_freq", self._h, C.byref(v))
+        return float(v.value)
 
     def setDepositMode(self, tiled=True, lds_budget_bytes=0):
         """Tile-sorted LDS deposit (default) or particle-order global atomics."""

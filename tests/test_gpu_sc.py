@@ -148,6 +148,36 @@ def test_sc3d_step(grid):
         ob.coords[:] = out
 
 
+@pytest.mark.parametrize("zscale,nb", [(0.05, 4), (1.0, 3)])
+def test_sc3d_external_bunches(zscale, nb):
+    """numExtBunches > 0 (wrappedBunchAnalysis): z wrapped to one bunch spacing, image bunches
+    in the Green function (rebuilt every call), periodic kick; then back to 0 external
+    bunches, where the potential grid keeps its periodic z stencil as in the reference."""
+    import spacecharge
+    from workloads import waterbag_coords
+    c = waterbag_coords(20000, seed=17)
+    c[4] *= zscale  # 0.05: inside half a spacing (0.33 m); 1.0: the extension branch
+    b, ob = _bunches(c)
+    calc = spacecharge.SpaceChargeCalc3D(16, 16, 32)
+    ocalc = O.SC3D(16, 16, 32)
+    assert calc.numExtBunches(nb) == ocalc.numExtBunches(nb) == nb + nb % 2
+    assert calc.freqOfBunches() == ocalc.freqOfBunches() == 402.5e6
+    for step in range(3):
+        if step == 2:
+            calc.numExtBunches(0)
+            ocalc.numExtBunches(0)
+        calc.trackBunch(b, 0.5)
+        ocalc.trackBunch(ob, 0.5)
+        out = b.getCoordinates()
+        assert rel_l2(calc.getRhoGrid().array(), ocalc.grid(0)[0]) < 1e-12
+        assert rel_l2(calc.getPhiGrid().array(), ocalc.grid(1)[0]) < 1e-12
+        assert calc.getGreenRebuilds() == ocalc.green_rebuilds()
+        for k in (1, 3, 5):
+            assert rel_l2(out[k] - c[k], ob.coords[k] - c[k]) < 1e-10
+        c = out.copy()
+        ob.coords[:] = out
+
+
 def test_sc3d_bin_indices_exact():
     import spacecharge
     c = sparse_coords(nx=32)

@@ -93,6 +93,37 @@ def test_poisson3d_vs_brute_force():
     assert rel_l2(phi, brute) < 1e-13
 
 
+def test_poisson3d_external_bunch_images_vs_brute_force():
+    """Green function with nb image bunches at spacing lam along z: the Hockney solve equals
+    the direct sum of the charge of every cell and of its images at z + b*lam (b = +-1 ..
+    +-nb/2), each contributing 1/distance; a cell's own images contribute
+    2*(1 + 1/2 + ... + 1/(nb/2))/lam (harmonic sum) to its own potential."""
+    nx, ny, nz, nb, lam = 5, 4, 6, 4, 0.37
+    r = np.random.default_rng(9)
+    rho = r.uniform(0, 1, size=(nz, nx, ny))
+    rl = np.array([-1.0, 1.2, -0.5, 0.7, -0.4, 0.5])
+    dx, dy, dz = (rl[1] - rl[0]) / (nx - 1), (rl[3] - rl[2]) / (ny - 1), (rl[5] - rl[4]) / nz
+    phi = O.poisson3d_images(rho, rl, rl, nb, lam)
+    iz, ix, iy = np.meshgrid(np.arange(nz), np.arange(nx), np.arange(ny), indexing="ij")
+    P = np.stack([ix.ravel() * dx, iy.ravel() * dy, iz.ravel() * dz], axis=1)
+    brute = np.zeros(P.shape[0])
+    q = rho.ravel()
+    for b in [b for b in range(-nb // 2, nb // 2 + 1)]:
+        Q = P + np.array([0.0, 0.0, b * lam])
+        d = np.sqrt(((P[:, None, :] - Q[None, :, :]) ** 2).sum(-1))
+        with np.errstate(divide="ignore"):
+            G = np.where(d > 0, 1.0 / d, 0.0)
+        brute += G @ q
+    assert rel_l2(phi.ravel(), brute) < 1e-13
+    # a single unit charge: its own-image term is the harmonic sum
+    one = np.zeros((nz, nx, ny))
+    one[2, 2, 1] = 1.0
+    p1 = O.poisson3d_images(one, rl, rl, nb, lam)
+    p0 = O.poisson3d_images(one, rl, rl, 0, lam)
+    harmonic = 2.0 * sum(1.0 / k for k in range(1, nb // 2 + 1)) / lam
+    assert abs((p1 - p0)[2, 2, 1] - harmonic) < 1e-12 * harmonic
+
+
 # ---------------------------------------------------------------- dlambda/dz
 
 def test_long_derivative_exact_on_quadratic_density():
