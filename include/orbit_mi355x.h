@@ -56,6 +56,8 @@ int ob_bunch_set_macrosize(ob_bunch* b, double macrosize);
 int ob_bunch_get_attrs(ob_bunch* b, double* out);
 /* SyncPart.kinEnergy(T) setter (wrap_syncpart.cc:70-92). */
 int ob_bunch_set_kin_energy(ob_bunch* b, double T_GeV);
+/* SyncPart.momentum(p) setter (wrap_syncpart.cc:136-156, SyncPart.cc:133-138, 236-317). */
+int ob_bunch_set_sync_momentum(ob_bunch* b, double p_GeV);
 /* out[6] = kinEnergy, momentum, beta, gamma, time, mass (wrap_syncpart.cc:549-566) */
 int ob_bunch_get_sync(ob_bunch* b, double* out);
 int ob_bunch_set_sync_time(ob_bunch* b, double t);
@@ -103,6 +105,25 @@ int ob_tpb_quadfringeIN(ob_bunch* b, double kq, int useCharge);         /* :784 
 int ob_tpb_quadfringeOUT(ob_bunch* b, double kq, int useCharge);        /* :854 */
 /* Apply the queued maps now. */
 int ob_bunch_flush(ob_bunch* b);
+
+/* A map sequence in one call (the per-SC-interval sequence the lattice layer builds from
+ * the TEAPOT nodes, teapot.py:548-976): identical results to calling the entry points
+ * above one by one, applied by one fused launch before returning (asynchronously on the
+ * library stream).  Arguments per kind (a, b, c real; ia, ib, ic integer):
+ *   ROTATEXY a=angle | DRIFT a=length | WRAPBUNCH a=ring length | KICK a,b,c=kx,ky,kE ic=useCharge
+ *   MULTP / MULTPFRINGE_IN / _OUT  ia=pole a=kl ib=skew ic=useCharge
+ *   QUAD1 a=length b=kq ic=useCharge | QUAD2 a=length | QUADFRINGE_IN / _OUT b=kq ic=useCharge */
+enum {
+  OB_MAP_ROTATEXY = 1, OB_MAP_DRIFT, OB_MAP_WRAPBUNCH, OB_MAP_KICK, OB_MAP_MULTP,
+  OB_MAP_MULTPFRINGE_IN, OB_MAP_MULTPFRINGE_OUT, OB_MAP_QUAD1, OB_MAP_QUAD2,
+  OB_MAP_QUADFRINGE_IN, OB_MAP_QUADFRINGE_OUT
+};
+typedef struct ob_map_op {
+  int32_t kind;
+  int32_t ia, ib, ic;
+  double a, b, c;
+} ob_map_op;
+int ob_track_sequence(ob_bunch* b, const ob_map_op* ops, int nops);
 
 /* ---- SpaceChargeCalc2p5Drb (src/spacecharge/SpaceChargeCalc2p5Drb.cc) ---- */
 /* SpaceChargeCalc2p5Drb(nx, ny, nz[, xy_ratio]) (:31-63). */

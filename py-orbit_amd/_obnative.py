@@ -25,6 +25,7 @@ SIGNATURES = {
     "ob_bunch_set_macrosize": [_vp, _d],
     "ob_bunch_get_attrs": [_vp, _D],
     "ob_bunch_set_kin_energy": [_vp, _d],
+    "ob_bunch_set_sync_momentum": [_vp, _d],
     "ob_bunch_get_sync": [_vp, _D],
     "ob_bunch_set_sync_time": [_vp, _d],
     "ob_bunch_add_particle": [_vp, _d, _d, _d, _d, _d, _d],
@@ -39,6 +40,7 @@ SIGNATURES = {
     "ob_bunch_get_macrosizes": [_vp, _i64, _D],
     "ob_bunch_device_coords": [_vp, C.POINTER(_vp)],
     "ob_bunch_flush": [_vp],
+    "ob_track_sequence": [_vp, _vp, _i],
     "ob_tpb_rotatexy": [_vp, _d],
     "ob_tpb_drift": [_vp, _d],
     "ob_tpb_wrapbunch": [_vp, _d],

@@ -31,7 +31,10 @@ class SyncPart:
             N.call("ob_bunch_set_kin_energy", self._b._h, float(value))
         return float(self._get()[0])
 
-    def momentum(self):
+    def momentum(self, value=None):
+        """momentum() -> GeV/c; momentum(p) sets p along z (wrap_syncpart.cc:136-156)."""
+        if value is not None:
+            self.momentum_set(value)
         return float(self._get()[1])
 
     def beta(self):
@@ -51,6 +54,47 @@ class SyncPart:
     def energyToMomentum(self, ek):  # SyncPart.cc energyToMomentum
         m = self.mass()
         return float(np.sqrt(ek * (ek + 2.0 * m)))
+
+    def momentum_set(self, p):
+        N.call("ob_bunch_set_sync_momentum", self._b._h, float(p))
+
+    def rVector(self, v=None):
+        """Position of the synchronous particle (host-side bookkeeping only)."""
+        if v is not None:
+            self._r = tuple(float(a) for a in v)
+        return getattr(self, "_r", (0.0, 0.0, 0.0))
+
+    def nxVector(self, v=None):
+        """x-axis ort of the synchronous particle (host-side bookkeeping only)."""
+        if v is not None:
+            self._nx = tuple(float(a) for a in v)
+        return getattr(self, "_nx", (1.0, 0.0, 0.0))
+
+    def pVector(self, v=None):
+        """Momentum vector; the tracked path uses its length (SyncPart.cc:152-164, 236-317):
+        setting it sets momentum = |p|, and the vector is returned along z."""
+        if v is not None:
+            px, py, pz = (float(a) for a in v)
+            self.momentum_set(float(np.sqrt(px * px + py * py + pz * pz)))
+        return (0.0, 0.0, self.momentum())
+
+    def x(self):
+        return self.rVector()[0]
+
+    def y(self):
+        return self.rVector()[1]
+
+    def z(self):
+        return self.rVector()[2]
+
+    def px(self):
+        return self.pVector()[0]
+
+    def py(self):
+        return self.pVector()[1]
+
+    def pz(self):
+        return self.pVector()[2]
 
     def momentumToEnergy(self, p):  # SyncPart.cc momentumToEnergy
         m = self.mass()
@@ -280,6 +324,160 @@ class Bunch:
     def flush(self):
         """Apply queued maps now (they are otherwise applied lazily)."""
         N.call("ob_bunch_flush", self._h)
+
+    # ---- text files: dumpBunch / readBunch (Bunch.cc:894-1080, 177-284, 1116-1270;
+    #      SyncPart.cc:323-540).  Same layout and number formatting as the reference:
+    #      bunch attributes with 6 significant digits, the synchronous particle with 14,
+    #      coordinates with 8 (C++ ostream default float format, i.e. printf %g). ------
+    def dumpBunch(self, fileName=None):
+        """dumpBunch() prints to stdout; dumpBunch(fileName) writes the file."""
+        import sys
+        text = self._dump_text()
+        if fileName is None:
+            sys.stdout.write(text)
+            sys.stdout.flush()
+        else:
+            with open(fileName, "w") as f:
+                f.write(text)
+
+    def _dump_text(self):
+        g6 = lambda v: "%.6g" % v
+        g14 = lambda v: "%.14g" % v
+        g8 = lambda v: "%.8g" % v
+        has_m = self._msize is not None
+        out = ["% PARTICLE_ATTRIBUTES_CONTROLLERS_NAMES " + ("macrosize " if has_m else "") + "\n"]
+        for name in sorted(self._int_attrs):
+            out.append("% BUNCH_ATTRIBUTE_INT " + name + "   " + "%d" % self._int_attrs[name] + " \n")
+        dbl = dict(self._dbl_attrs)
+        dbl.update({"mass": self.mass(), "charge": self.charge(),
+                    "classical_radius": self.classicalRadius(), "macro_size": self.macroSize()})
+        for name in sorted(dbl):
+            out.append("% BUNCH_ATTRIBUTE_DOUBLE " + name + "   " + g6(dbl[name]) + " \n")
+        sp = self._sync
+        px, py, pz = sp.pVector()
+        out.append("%  SYNC_PART_COORDS " + " ".join(g14(v) for v in sp.rVector()) +
+                   "  x, y, z positions in [m]\n")
+        out.append("%  SYNC_PART_MOMENTUM " + " ".join(g14(v) for v in (px, py, pz)) +
+                   "  px, py, pz momentum component in GeV/c\n")
+        out.append("%  SYNC_PART_X_AXIS " + " ".join(g14(v) for v in sp.nxVector()) +
+                   "  nxx, nxy, pxz - x-axis ort coordinates\n")
+        out.append("%  info only: energy of the synchronous particle [GeV] = " +
+                   g14(sp.kinEnergy()) + " \n")
+        out.append("%  info only: momentum of the synchronous particle [GeV/c] = " +
+                   g14(sp.momentum()) + " \n")
+        out.append("%  info only: beta=v/c of the synchronous particle = " + g14(sp.beta()) + " \n")
+        out.append("%  info only: gamma=1/sqrt(1-(v/c)**2) of the synchronous particle = " +
+                   g14(sp.gamma()) + " \n")
+        out.append("%  SYNC_PART_TIME " + g14(sp.time()) + "  time in [sec]\n")
+        out.append("% x[m] px[rad] y[m] py[rad] z[m]  (pz or dE [GeV]) " +
+                   ("macro_size " if has_m else "") + "\n")
+        c = self.getCoordinates()
+        cols = [c[k] for k in range(6)] + ([np.asarray(self._msize)] if has_m else [])
+        for row in zip(*cols):
+            out.append("".join(g8(v) + " " for v in row) + "\n")
+        return "".join(out)
+
+    def readBunch(self, fileName, nParts=None):
+        """readBunch(fileName[, nParts]): bunch attributes, synchronous particle, then the
+        particles (appended; at most nParts).  A data line holds 6 values, or 6 plus one
+        per particle attribute the bunch already has; anything else is an error, as in the
+        reference (which aborts)."""
+        with open(fileName) as f:
+            lines = f.read().split("\n")
+        # initBunchAttributes: the leading '%' lines (Bunch.cc:177-284)
+        self._int_attrs = {}
+        self._dbl_attrs = {}
+        for ln in lines:
+            if not ln or ln[0] != "%":
+                break
+            t = ln.split()
+            if len(t) > 3 and t[1] == "BUNCH_ATTRIBUTE_INT":
+                self._int_attrs[t[2]] = int(float(t[3]))
+            elif len(t) > 3 and t[1] == "BUNCH_ATTRIBUTE_DOUBLE":
+                v = float(t[3])
+                setter = {"mass": self.mass, "charge": self.charge,
+                          "classical_radius": self.classicalRadius,
+                          "macro_size": self.macroSize}.get(t[2])
+                if setter is not None:
+                    setter(v)
+                else:
+                    self._dbl_attrs[t[2]] = v
+        # readSyncPart (SyncPart.cc:323-460)
+        sp = self._sync
+        for ln in lines:
+            if not ln or ln[0] != "%":
+                break
+            t = ln.split()
+            if len(t) > 5 and t[1] == "SYNC_PART_COORDS":
+                sp.rVector((float(t[2]), float(t[3]), float(t[4])))
+            elif len(t) > 5 and t[1] == "SYNC_PART_MOMENTUM":
+                sp.pVector((float(t[2]), float(t[3]), float(t[4])))
+            elif len(t) > 5 and t[1] == "SYNC_PART_X_AXIS":
+                sp.nxVector((float(t[2]), float(t[3]), float(t[4])))
+            elif len(t) > 3 and t[1] == "SYNC_PART_TIME":
+                sp.time(float(t[2]))
+        # readBunchCoords (Bunch.cc:1116-1270)
+        n_attr = 1 if self._msize is not None else 0
+        rows = []
+        for k, ln in enumerate(lines):
+            if not ln or ln[0] == "%":
+                continue
+            t = ln.split()
+            if len(t) != 6 + n_attr and len(t) != 6:
+                raise N.OrbitError(f"Bunch.readBunch: {fileName}: error in data structure in "
+                                   f"line {k + 1}: {ln!r}")
+            rows.append([float(v) for v in t] + ([0.0] * n_attr if len(t) == 6 else []))
+            if nParts is not None and len(rows) >= int(nParts):
+                break
+        if not rows:
+            return self.getSize()
+        data = np.array(rows, dtype=np.float64).T
+        old = self.getCoordinates()
+        m_old = None if self._msize is None else np.asarray(self._msize)
+        self.setCoordinates(np.concatenate([old, data[:6]], axis=1))
+        if n_attr:
+            self.setMacroSizes(np.concatenate([m_old, data[6]]))
+        return self.getSize()
+
+    # ---- full-precision binary checkpoint (no reference counterpart) --------
+    def saveBunchBinary(self, fileName):
+        """Exact checkpoint: coordinates, macrosizes, bunch attributes and the synchronous
+        particle, bit for bit (numpy .npz, nothing executable)."""
+        sp = self._sync
+        meta = np.array([self.mass(), self.charge(), self.classicalRadius(), self.macroSize(),
+                         sp.momentum(), sp.time()], dtype=np.float64)
+        arrays = {"coords": self.getCoordinates(), "meta": meta,
+                  "sync_r": np.array(sp.rVector()), "sync_nx": np.array(sp.nxVector()),
+                  "dbl_names": np.array(sorted(self._dbl_attrs), dtype=np.str_),
+                  "dbl_vals": np.array([self._dbl_attrs[k] for k in sorted(self._dbl_attrs)]),
+                  "int_names": np.array(sorted(self._int_attrs), dtype=np.str_),
+                  "int_vals": np.array([self._int_attrs[k] for k in sorted(self._int_attrs)],
+                                       dtype=np.int64)}
+        if self._msize is not None:
+            arrays["macrosizes"] = np.asarray(self._msize)
+        with open(fileName, "wb") as f:
+            np.savez(f, **arrays)
+
+    def readBunchBinary(self, fileName):
+        """Restore a checkpoint written by saveBunchBinary (replaces the particles)."""
+        with np.load(fileName, allow_pickle=False) as z:
+            m = z["meta"]
+            self.mass(m[0])
+            self.charge(m[1])
+            self.classicalRadius(m[2])
+            self.macroSize(m[3])
+            self._sync.momentum(m[4])
+            self._sync.time(m[5])
+            self._sync.rVector(tuple(z["sync_r"]))
+            self._sync.nxVector(tuple(z["sync_nx"]))
+            self._dbl_attrs = {str(k): float(v) for k, v in zip(z["dbl_names"], z["dbl_vals"])}
+            self._int_attrs = {str(k): int(v) for k, v in zip(z["int_names"], z["int_vals"])}
+            self.setCoordinates(z["coords"])
+            if "macrosizes" in z.files:
+                self.setMacroSizes(z["macrosizes"])
+            else:
+                self.removePartAttr("macrosize")
+        return self.getSize()
 
 
 class BunchTwissAnalysis:

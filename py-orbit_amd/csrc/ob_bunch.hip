@@ -496,6 +496,12 @@ OB_API int ob_bunch_set_kin_energy(ob_bunch* b, double T) {  // wrap_syncpart.cc
   ob::sync_set_momentum(b, sqrt(T * (T + 2.0 * m)));
   OB_CATCH
 }
+OB_API int ob_bunch_set_sync_momentum(ob_bunch* b, double p) {  // SyncPart::setMomentum
+  OB_TRY OB_CHECK_B(b);
+  OB_REQUIRE(p >= 0.0, "synchronous momentum must be >= 0");
+  ob::sync_set_momentum(b, p);
+  OB_CATCH
+}
 OB_API int ob_bunch_get_sync(ob_bunch* b, double* out) {
   OB_TRY OB_CHECK_B(b);
   out[0] = b->T;
@@ -834,6 +840,34 @@ OB_API int ob_tpb_quadfringeIN(ob_bunch* b, double kq, int) {
 OB_API int ob_tpb_quadfringeOUT(ob_bunch* b, double kq, int) {
   OB_TRY OB_CHECK_B(b);
   quadfringe(b, kq, true);
+  OB_CATCH
+}
+
+// A whole map sequence in one call: each op goes through the same host-side constants as
+// its single entry point above, then the sequence runs as one fused launch.
+OB_API int ob_track_sequence(ob_bunch* b, const ob_map_op* ops, int nops) {
+  OB_TRY OB_CHECK_B(b);
+  OB_REQUIRE(nops >= 0 && (nops == 0 || ops != nullptr), "invalid op list");
+  for (int k = 0; k < nops; k++) {
+    const ob_map_op& o = ops[k];
+    int rc = 0;
+    switch (o.kind) {
+      case OB_MAP_ROTATEXY: rc = ob_tpb_rotatexy(b, o.a); break;
+      case OB_MAP_DRIFT: rc = ob_tpb_drift(b, o.a); break;
+      case OB_MAP_WRAPBUNCH: rc = ob_tpb_wrapbunch(b, o.a); break;
+      case OB_MAP_KICK: rc = ob_tpb_kick(b, o.a, o.b, o.c, o.ic); break;
+      case OB_MAP_MULTP: rc = ob_tpb_multp(b, o.ia, o.a, o.ib, o.ic); break;
+      case OB_MAP_MULTPFRINGE_IN: rc = ob_tpb_multpfringeIN(b, o.ia, o.a, o.ib, o.ic); break;
+      case OB_MAP_MULTPFRINGE_OUT: rc = ob_tpb_multpfringeOUT(b, o.ia, o.a, o.ib, o.ic); break;
+      case OB_MAP_QUAD1: rc = ob_tpb_quad1(b, o.a, o.b, o.ic); break;
+      case OB_MAP_QUAD2: rc = ob_tpb_quad2(b, o.a); break;
+      case OB_MAP_QUADFRINGE_IN: rc = ob_tpb_quadfringeIN(b, o.b, o.ic); break;
+      case OB_MAP_QUADFRINGE_OUT: rc = ob_tpb_quadfringeOUT(b, o.b, o.ic); break;
+      default: throw ob::Error{"ob_track_sequence: unknown map kind " + std::to_string(o.kind)};
+    }
+    if (rc != 0) return rc;  // message already set by the map entry point
+  }
+  ob::bunch_flush(b, nullptr);
   OB_CATCH
 }
 

@@ -75,6 +75,21 @@ def test_fused_program_bit_exact():
     assert np.array_equal(out, ob.coords), np.abs(out - ob.coords).max()
 
 
+def test_track_sequence_entry_point_bit_exact():
+    """ob_track_sequence (one call for a whole map list) gives the bits of the oracle applying
+    the same maps one at a time."""
+    import teapot_base as TPB
+    c = coords(20000, 4)
+    ob = O.Bunch(c, kin_energy=1.0)
+    b = product_bunch(c)
+    for op in OPS * 2:
+        ob.apply(op[0], *op[1:])
+    TPB.trackSequence(b, OPS * 2)
+    out = b.getCoordinates()
+    assert np.array_equal(out, ob.coords), np.abs(out - ob.coords).max()
+    assert b.getSyncParticle().time() == ob.sync["time"]
+
+
 def test_per_particle_accessors_and_empty_bunch():
     from bunch import Bunch
     import teapot_base as TPB
