@@ -165,6 +165,36 @@ int ob_sc3d_green_rebuilds(ob_sc3d* c, int* n);
  * per-tile LDS block (0 keeps the current value). */
 int ob_sc3d_set_deposit_mode(ob_sc3d* c, int tiled, int lds_budget_bytes);
 
+/* ---- batches of bunches (no reference counterpart) ---------------------------------
+ * B independent bunches of uneven sizes, each with its own SpaceChargeCalc2p5Drb (grids
+ * may differ), sharing one lattice and one synchronous particle (same mass, charge,
+ * classical radius and momentum: checked).  Every phase of a step runs as one launch over
+ * all bunches; results equal tracking each bunch alone.  The batch owns the bunches' map
+ * queues while it is in use: queue maps with ob_batch_track_sequence, not ob_tpb_*. */
+typedef struct ob_batch ob_batch;
+int ob_batch_create(ob_bunch** bunches, int nb, ob_batch** out);
+int ob_batch_destroy(ob_batch* batch);
+/* One calculator per bunch, in bunch order. */
+int ob_batch_set_sc2p5drb(ob_batch* batch, ob_sc2p5drb** calcs);
+/* Queue maps for every bunch (applied lazily, like ob_tpb_*). */
+int ob_batch_track_sequence(ob_batch* batch, const ob_map_op* ops, int nops);
+/* SpaceChargeCalc2p5Drb.trackBunch(bunch, length, pipe_radius) of every bunch. */
+int ob_batch_sc2p5drb_track(ob_batch* batch, double length, double pipe_radius);
+int ob_batch_flush(ob_batch* batch);
+/* Wait; fails if a bunch had zero extent in its last SC step (the reference aborts). */
+int ob_batch_synchronize(ob_batch* batch);
+/* Autotuning: particles per work item and the largest rho grid (bytes) deposited in LDS.
+ * Tuned by timing on the first SC step unless set here. */
+int ob_batch_set_tuning(ob_batch* batch, int chunk, int lds_max_bytes);
+int ob_batch_get_tuning(ob_batch* batch, int* chunk, int* lds_max_bytes, int* tuned);
+/* hipGraph: the batch calls between begin and end are recorded (not run) and replayed by
+ * ob_batch_replay(count).  Pending work is applied before and at the end of the capture,
+ * so a replay starts and ends with nothing pending.  Bunch sizes must not change between
+ * the capture and its replays. */
+int ob_batch_capture_begin(ob_batch* batch);
+int ob_batch_capture_end(ob_batch* batch);
+int ob_batch_replay(ob_batch* batch, int count);
+
 /* ---- kernel timing (HIP events on the library stream) ---- */
 /* Start (on != 0) or stop collecting per-kernel timings; clears earlier records. */
 int ob_profile_enable(int on);

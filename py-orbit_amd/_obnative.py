@@ -71,6 +71,18 @@ SIGNATURES = {
     "ob_sc3d_get_num_ext_bunches": [_vp, C.POINTER(_i)],
     "ob_sc3d_set_freq": [_vp, _d],
     "ob_sc3d_get_freq": [_vp, _D],
+    "ob_batch_create": [C.POINTER(_vp), _i, C.POINTER(_vp)],
+    "ob_batch_destroy": [_vp],
+    "ob_batch_set_sc2p5drb": [_vp, C.POINTER(_vp)],
+    "ob_batch_track_sequence": [_vp, _vp, _i],
+    "ob_batch_sc2p5drb_track": [_vp, _d, _d],
+    "ob_batch_flush": [_vp],
+    "ob_batch_synchronize": [_vp],
+    "ob_batch_set_tuning": [_vp, _i, _i],
+    "ob_batch_get_tuning": [_vp, C.POINTER(_i), C.POINTER(_i), C.POINTER(_i)],
+    "ob_batch_capture_begin": [_vp],
+    "ob_batch_capture_end": [_vp],
+    "ob_batch_replay": [_vp, _i],
     "ob_profile_enable": [_i],
     "ob_profile_report": [C.c_char_p, _i],
     "ob_calibrate_copy": [_i64, _i, _D],

@@ -161,6 +161,7 @@ struct ob_bunch {
   bool host_valid = true, dev_valid = true;
 
   std::vector<ob::Op> queue;
+  ob_batch* owner = nullptr;  // batch applying this bunch's maps, if any
 };
 
 namespace ob {
@@ -331,11 +332,27 @@ void bunch_sort_commit(ob_bunch* b) {
 
 void bunch_device_written(ob_bunch* b) { b->host_valid = false; }
 
+std::vector<Op> bunch_take_queue(ob_bunch* b) {
+  std::vector<Op> q;
+  q.swap(b->queue);
+  return q;
+}
+
+void bunch_set_sync_time(ob_bunch* b, double t) { b->time = t; }
+
+void bunch_set_owner(ob_bunch* b, ob_batch* owner) { b->owner = owner; }
+
 SyncInfo bunch_sync(ob_bunch* b) {
   return SyncInfo{b->mass, b->charge, b->r0, b->macrosize, b->T, b->p, b->beta, b->gamma, b->time};
 }
 
 int bunch_flush(ob_bunch* b, double* ext_partials) {
+  if (b->owner) {  // maps and kicks of a batched bunch are the batch's
+    OB_REQUIRE(ext_partials == nullptr,
+               "this bunch belongs to a batch: track it through the batch");
+    batch_flush_owner(b->owner);
+    return 0;
+  }
   if (b->queue.empty() && ext_partials == nullptr) return 0;
   DeviceBunchView v = bunch_device_view(b);
   size_t pos = 0;
@@ -360,7 +377,7 @@ void bunch_queue(ob_bunch* b, const Op& op) {
   if (b->n == 0) return;  // maps over an empty bunch are no-ops
   b->queue.push_back(op);
   b->host_valid = false;
-  if (b->queue.size() >= 256) bunch_flush(b, nullptr);
+  if (b->queue.size() >= 256 && !b->owner) bunch_flush(b, nullptr);
 }
 
 }  // namespace ob
@@ -454,6 +471,7 @@ OB_API int ob_bunch_create(ob_bunch** out) {
 
 OB_API int ob_bunch_destroy(ob_bunch* b) {
   OB_TRY if (!b) return 0;
+  OB_REQUIRE(b->owner == nullptr, "bunch belongs to a batch: destroy the batch first");
   (void)hipStreamSynchronize(ob::stream());
   for (void* p : {(void*)b->d, (void*)b->d_msize, (void*)b->d_alt, (void*)b->d_msize_alt,
                   (void*)b->d_perm, (void*)b->d_perm_alt})
@@ -843,28 +861,33 @@ OB_API int ob_tpb_quadfringeOUT(ob_bunch* b, double kq, int) {
   OB_CATCH
 }
 
+namespace ob {
+// Queue one map of the public op list through its single entry point (host constants).
+int queue_map_op(ob_bunch* b, const ob_map_op& o) {
+  switch (o.kind) {
+    case OB_MAP_ROTATEXY: return ob_tpb_rotatexy(b, o.a);
+    case OB_MAP_DRIFT: return ob_tpb_drift(b, o.a);
+    case OB_MAP_WRAPBUNCH: return ob_tpb_wrapbunch(b, o.a);
+    case OB_MAP_KICK: return ob_tpb_kick(b, o.a, o.b, o.c, o.ic);
+    case OB_MAP_MULTP: return ob_tpb_multp(b, o.ia, o.a, o.ib, o.ic);
+    case OB_MAP_MULTPFRINGE_IN: return ob_tpb_multpfringeIN(b, o.ia, o.a, o.ib, o.ic);
+    case OB_MAP_MULTPFRINGE_OUT: return ob_tpb_multpfringeOUT(b, o.ia, o.a, o.ib, o.ic);
+    case OB_MAP_QUAD1: return ob_tpb_quad1(b, o.a, o.b, o.ic);
+    case OB_MAP_QUAD2: return ob_tpb_quad2(b, o.a);
+    case OB_MAP_QUADFRINGE_IN: return ob_tpb_quadfringeIN(b, o.b, o.ic);
+    case OB_MAP_QUADFRINGE_OUT: return ob_tpb_quadfringeOUT(b, o.b, o.ic);
+    default: throw Error{"unknown map kind " + std::to_string(o.kind)};
+  }
+}
+}  // namespace ob
+
 // A whole map sequence in one call: each op goes through the same host-side constants as
 // its single entry point above, then the sequence runs as one fused launch.
 OB_API int ob_track_sequence(ob_bunch* b, const ob_map_op* ops, int nops) {
   OB_TRY OB_CHECK_B(b);
   OB_REQUIRE(nops >= 0 && (nops == 0 || ops != nullptr), "invalid op list");
   for (int k = 0; k < nops; k++) {
-    const ob_map_op& o = ops[k];
-    int rc = 0;
-    switch (o.kind) {
-      case OB_MAP_ROTATEXY: rc = ob_tpb_rotatexy(b, o.a); break;
-      case OB_MAP_DRIFT: rc = ob_tpb_drift(b, o.a); break;
-      case OB_MAP_WRAPBUNCH: rc = ob_tpb_wrapbunch(b, o.a); break;
-      case OB_MAP_KICK: rc = ob_tpb_kick(b, o.a, o.b, o.c, o.ic); break;
-      case OB_MAP_MULTP: rc = ob_tpb_multp(b, o.ia, o.a, o.ib, o.ic); break;
-      case OB_MAP_MULTPFRINGE_IN: rc = ob_tpb_multpfringeIN(b, o.ia, o.a, o.ib, o.ic); break;
-      case OB_MAP_MULTPFRINGE_OUT: rc = ob_tpb_multpfringeOUT(b, o.ia, o.a, o.ib, o.ic); break;
-      case OB_MAP_QUAD1: rc = ob_tpb_quad1(b, o.a, o.b, o.ic); break;
-      case OB_MAP_QUAD2: rc = ob_tpb_quad2(b, o.a); break;
-      case OB_MAP_QUADFRINGE_IN: rc = ob_tpb_quadfringeIN(b, o.b, o.ic); break;
-      case OB_MAP_QUADFRINGE_OUT: rc = ob_tpb_quadfringeOUT(b, o.b, o.ic); break;
-      default: throw ob::Error{"ob_track_sequence: unknown map kind " + std::to_string(o.kind)};
-    }
+    const int rc = ob::queue_map_op(b, ops[k]);
     if (rc != 0) return rc;  // message already set by the map entry point
   }
   ob::bunch_flush(b, nullptr);

@@ -124,6 +124,49 @@ struct DeviceBunchView {
 // ext_partials when prog.want_extrema.  Returns the number of blocks used.
 int launch_push(const DeviceBunchView& v, const OpProgram& prog, double* ext_partials,
                 hipStream_t s);
+
+// ---- batches of bunches, each with its own SpaceChargeCalc2p5Drb ----------------------
+// Device-resident record of one bunch of a batch.
+struct BatchBunch {
+  DeviceBunchView v;
+  int32_t item_begin, item_end;  // its work items (particle ranges); partials are per item
+  int32_t rows_begin, cols_begin;  // its first block in the batched FFT row / column passes
+  SC2DState* st;
+  double *rho, *phi, *zg, *zd;
+  double2 *spec, *ghat_t, *tw;
+  int32_t nx, ny, nz, M0, M1, H, nmax, n_long_avg;
+  double xy_ratio, dx_solver, macrosize, r0, mass;
+  double gamma_D;  // (beta*beta)*pow(gamma,3) of the synchronous particle
+  int32_t deposit_lds;  // 1: accumulate rho of a work item in LDS (grid fits), 0: atomics
+  int32_t pad;
+};
+struct BatchItem {  // particles [begin, end) of one bunch
+  int32_t bunch, pad;
+  int64_t begin, end;
+};
+struct BatchKick {  // deferred 2.5D kicks of the whole batch (common factors, op p[4], p[5])
+  int32_t active, pad;
+  double p4, p5;
+};
+// Push of all bunches: the deferred kick (if active) then prog; per-item extrema partials.
+void launch_push_batch(const BatchBunch* bb, const BatchItem* items, int nitems,
+                       const OpProgram& prog, const BatchKick& kick, double* partials,
+                       int threads, hipStream_t s);
+// SpaceChargeCalc2p5Drb steps of all bunches after launch_push_batch with extrema.
+struct BatchShape {
+  int nb, nitems, row_blocks, col_blocks, max_M, deposit_threads;
+  size_t deposit_lds_bytes;
+};
+void launch_sc2d_batch(const BatchBunch* bb, const BatchItem* items, const BatchShape& sh,
+                       const double* partials, double length, double pipe_radius, hipStream_t s);
+}  // namespace ob
+struct ob_sc2p5drb;
+namespace ob {
+// Calculator fields of a batch record; mark a calculator as tracked; its status word
+// (1: the last step found a zero-extent bunch).
+void sc2d_batch_record(ob_sc2p5drb* c, BatchBunch& r);
+void sc2d_mark_tracked(ob_sc2p5drb* c);
+int sc2d_read_status(ob_sc2p5drb* c);
 int push_grid_blocks(int64_t n);
 
 }  // namespace ob
@@ -153,6 +196,18 @@ struct SyncInfo {
   double mass, charge, r0, macrosize, T, p, beta, gamma, time;
 };
 SyncInfo bunch_sync(ob_bunch* b);
+// Take the queued (not yet applied) map ops of a bunch.
+std::vector<Op> bunch_take_queue(ob_bunch* b);
+// Host-side synchronous time (advanced by the drift maps when queued).
+void bunch_set_sync_time(ob_bunch* b, double t);
+}  // namespace ob
+struct ob_batch;
+namespace ob {
+// A batched bunch's flush applies the whole batch's pending work (ob_batch.hip).
+void bunch_set_owner(ob_bunch* b, ob_batch* owner);
+void batch_flush_owner(ob_batch* B);
+// Queue one map of the public op list (ob_map_op) exactly as its entry point does.
+int queue_map_op(ob_bunch* b, const ob_map_op& o);
 // A kernel outside the map queue has written the device coordinates.
 void bunch_device_written(ob_bunch* b);
 }  // namespace ob

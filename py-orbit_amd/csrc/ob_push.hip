@@ -256,7 +256,95 @@ __global__ __launch_bounds__(kPushThreads) void push_kernel(DeviceBunchView v, O
   }
 }
 
+// All bunches of a batch in one launch: block = one work item (a particle range of one
+// bunch).  The bunch's deferred 2.5D kick comes first (its op is assembled from the
+// bunch's record and the batch's common factors), then the map program; extrema are
+// reduced per work item.
+template <bool EXTREMA>
+__global__ __launch_bounds__(kPushThreads) void push_batch_kernel(
+    const BatchBunch* __restrict__ bb, const BatchItem* __restrict__ items, OpProgram prog,
+    BatchKick kick, double* __restrict__ partials) {
+  const BatchItem it = items[blockIdx.x];
+  const BatchBunch& B = bb[it.bunch];
+  Op kop;
+  kop.kind = OP_SC2P5DRB_KICK;
+  kop.i0 = B.nx;
+  kop.i1 = B.ny;
+  kop.i2 = B.nz;
+  kop.p[0] = __longlong_as_double((long long)(uintptr_t)B.st);
+  kop.p[1] = __longlong_as_double((long long)(uintptr_t)B.phi);
+  kop.p[2] = __longlong_as_double((long long)(uintptr_t)B.zg);
+  kop.p[3] = __longlong_as_double((long long)(uintptr_t)B.zd);
+  kop.p[4] = kick.p4;
+  kop.p[5] = kick.p5;
+  double lo[3] = {DBL_MAX, DBL_MAX, DBL_MAX};
+  double hi[3] = {-DBL_MAX, -DBL_MAX, -DBL_MAX};
+  double* __restrict__ cx = B.v.c[0];
+  double* __restrict__ cxp = B.v.c[1];
+  double* __restrict__ cy = B.v.c[2];
+  double* __restrict__ cyp = B.v.c[3];
+  double* __restrict__ cz = B.v.c[4];
+  double* __restrict__ cdE = B.v.c[5];
+  for (int64_t i = it.begin + threadIdx.x; i < it.end; i += kPushThreads) {
+    double x = cx[i], xp = cxp[i], y = cy[i], yp = cyp[i], z = cz[i], dE = cdE[i];
+    if (kick.active) sc2p5drb_kick(kop, x, xp, y, yp, z, dE);
+    for (int k = 0; k < prog.nops; k++) apply_op(prog.ops[k], x, xp, y, yp, z, dE);
+    cx[i] = x;
+    cxp[i] = xp;
+    cy[i] = y;
+    cyp[i] = yp;
+    cz[i] = z;
+    cdE[i] = dE;
+    if (EXTREMA) {
+      lo[0] = fmin(lo[0], x);
+      hi[0] = fmax(hi[0], x);
+      lo[1] = fmin(lo[1], y);
+      hi[1] = fmax(hi[1], y);
+      lo[2] = fmin(lo[2], z);
+      hi[2] = fmax(hi[2], z);
+    }
+  }
+  if (EXTREMA) {
+    __shared__ double red[kPushThreads / 64][6];
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    double r[6];
+    for (int k = 0; k < 3; k++) {
+      r[2 * k] = wave_min(lo[k]);
+      r[2 * k + 1] = wave_max(hi[k]);
+    }
+    if (lane == 0)
+      for (int k = 0; k < 6; k++) red[w][k] = r[k];
+    __syncthreads();
+    if (threadIdx.x < 6) {
+      const int k = threadIdx.x;
+      double acc = red[0][k];
+      for (int j = 1; j < kPushThreads / 64; j++)
+        acc = (k & 1) ? fmax(acc, red[j][k]) : fmin(acc, red[j][k]);
+      partials[(int64_t)blockIdx.x * 6 + k] = acc;
+    }
+  }
+}
+
 }  // namespace
+
+void launch_push_batch(const BatchBunch* bb, const BatchItem* items, int nitems,
+                       const OpProgram& prog, const BatchKick& kick, double* partials,
+                       int threads, hipStream_t s) {
+  OB_REQUIRE(threads == kPushThreads, "batch push: unsupported block size");
+  if (nitems == 0) return;
+  if (prog.want_extrema) {
+    OB_REQUIRE(partials != nullptr, "batch push: extrema requested without a buffer");
+    ProfScope ps("push_batch_extrema");
+    hipLaunchKernelGGL(push_batch_kernel<true>, dim3(nitems), dim3(kPushThreads), 0, s, bb, items,
+                       prog, kick, partials);
+  } else {
+    if (prog.nops == 0 && !kick.active) return;
+    ProfScope ps("push_batch");
+    hipLaunchKernelGGL(push_batch_kernel<false>, dim3(nitems), dim3(kPushThreads), 0, s, bb,
+                       items, prog, kick, nullptr);
+  }
+  OB_HIP(hipGetLastError());
+}
 
 int push_grid_blocks(int64_t n) {
   int64_t b = (n + kPushThreads - 1) / kPushThreads;

@@ -54,13 +54,14 @@ struct SC2DParams {
 };
 
 // ---- 2: setup -------------------------------------------------------------
-__global__ __launch_bounds__(kSetupThreads) void sc2d_setup(SC2DParams P,
-                                                            const double* __restrict__ partials,
-                                                            SC2DState* st, double* rho,
-                                                            double* zg, double* zd) {
+// partials[pb..pe) hold per-block extrema of this bunch.  One block of kSetupThreads.
+__device__ __forceinline__ void sc2d_setup_body(const SC2DParams& P,
+                                                const double* __restrict__ partials, int pb,
+                                                int pe, SC2DState* st, double* rho, double* zg,
+                                                double* zd) {
   __shared__ double red[kSetupThreads / 64][6];
   double r[6] = {DBL_MAX, -DBL_MAX, DBL_MAX, -DBL_MAX, DBL_MAX, -DBL_MAX};
-  for (int b = threadIdx.x; b < P.nblk; b += blockDim.x)
+  for (int b = pb + threadIdx.x; b < pe; b += blockDim.x)
     for (int k = 0; k < 6; k++) {
       const double v = partials[b * 6 + k];
       r[k] = (k & 1) ? fmax(r[k], v) : fmin(r[k], v);
@@ -117,11 +118,20 @@ __global__ __launch_bounds__(kSetupThreads) void sc2d_setup(SC2DParams P,
   }
 }
 
+__global__ __launch_bounds__(kSetupThreads) void sc2d_setup(SC2DParams P,
+                                                            const double* __restrict__ partials,
+                                                            SC2DState* st, double* rho,
+                                                            double* zg, double* zd) {
+  sc2d_setup_body(P, partials, 0, P.nblk, st, rho, zg, zd);
+}
+
 // ---- 3: deposit -----------------------------------------------------------
-__global__ __launch_bounds__(kDepositThreads) void sc2d_deposit(SC2DParams P, DeviceBunchView v,
-                                                                const SC2DState* st,
-                                                                double* __restrict__ rho,
-                                                                double* __restrict__ zg) {
+// Particles first, first+stride, ... < end of one bunch; the block's z line density is
+// summed in LDS and added to zg at the end.
+__device__ __forceinline__ void sc2d_deposit_body(const SC2DParams& P, const DeviceBunchView& v,
+                                                  const SC2DState* st, double* __restrict__ rho,
+                                                  double* __restrict__ zg, int64_t first,
+                                                  int64_t end, int64_t stride) {
   __shared__ double zl[kMaxZBinsLDS];
   const int nz = P.nz;
   for (int i = threadIdx.x; i < nz; i += blockDim.x) zl[i] = 0.0;
@@ -130,8 +140,7 @@ __global__ __launch_bounds__(kDepositThreads) void sc2d_deposit(SC2DParams P, De
     const double xMin = st->xMin, xMax = st->xMax, yMin = st->yMin, yMax = st->yMax;
     const double dx = st->dx, dy = st->dy, zMin = st->zMin, zMax = st->zMax, dz = st->dz;
     const int nx = P.nx, ny = P.ny;
-    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
-    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < v.n; i += stride) {
+    for (int64_t i = first; i < end; i += stride) {
       const double x = v.c[0][i], y = v.c[2][i], z = v.c[4][i];
       const double m = v.msize ? v.msize[i] : P.macrosize;
       // Grid2D::binValue (Grid2D.cc:142-166)
@@ -168,6 +177,14 @@ __global__ __launch_bounds__(kDepositThreads) void sc2d_deposit(SC2DParams P, De
     if (zl[i] != 0.0) atomicAdd(&zg[i], zl[i]);
 }
 
+__global__ __launch_bounds__(kDepositThreads) void sc2d_deposit(SC2DParams P, DeviceBunchView v,
+                                                                const SC2DState* st,
+                                                                double* __restrict__ rho,
+                                                                double* __restrict__ zg) {
+  sc2d_deposit_body(P, v, st, rho, zg, (int64_t)blockIdx.x * blockDim.x + threadIdx.x, v.n,
+                    (int64_t)gridDim.x * blockDim.x);
+}
+
 // ---- 4: post (one block) ---------------------------------------------------
 __device__ double block_sum(double v, double* sh) {
   for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
@@ -186,10 +203,10 @@ __device__ __forceinline__ double ipow(double x, int j) {  // exact for the smal
   return r;
 }
 
-__global__ __launch_bounds__(kSetupThreads) void sc2d_post(SC2DParams P, SC2DState* st,
-                                                           const double* __restrict__ rho,
-                                                           const double* __restrict__ zg,
-                                                           double* __restrict__ zd) {
+__device__ __forceinline__ void sc2d_post_body(const SC2DParams& P, SC2DState* st,
+                                               const double* __restrict__ rho,
+                                               const double* __restrict__ zg,
+                                               double* __restrict__ zd) {
   __shared__ double sh[kSetupThreads / 64];
   if (st->skip) return;
   const int nz = P.nz;
@@ -285,20 +302,28 @@ __global__ __launch_bounds__(kSetupThreads) void sc2d_post(SC2DParams P, SC2DSta
   }
 }
 
+__global__ __launch_bounds__(kSetupThreads) void sc2d_post(SC2DParams P, SC2DState* st,
+                                                           const double* __restrict__ rho,
+                                                           const double* __restrict__ zg,
+                                                           double* __restrict__ zd) {
+  sc2d_post_body(P, st, rho, zg, zd);
+}
+
 // ---- 5: Poisson solve --------------------------------------------------------
 // (a) two real rows per complex FFT along y, r2c split.  rows >= nrows_in are zero;
 // columns >= ncols_in are zero (padding).  out: spectrum [*][H] complex (H = N/2+1).
+// blk: row pair (rows 2 blk, 2 blk + 1).
 template <int N>
-__global__ __launch_bounds__(kFFTThreads) void fft_rows_r2c(const double* __restrict__ in,
-                                                            int nrows_in, int ncols_in,
-                                                            int ld_in, double2* __restrict__ out,
-                                                            const double2* __restrict__ tw,
-                                                            int nmax, const SC2DState* st) {
+__device__ __forceinline__ void rows_r2c_body(const double* __restrict__ in, int nrows_in,
+                                              int ncols_in, int ld_in,
+                                              double2* __restrict__ out,
+                                              const double2* __restrict__ tw, int nmax,
+                                              const SC2DState* st, int blk) {
   if (st && st->skip) return;
   extern __shared__ double sm[];
   double* re = sm;
   double* im = sm + N;
-  const int r0 = 2 * blockIdx.x, r1 = r0 + 1;
+  const int r0 = 2 * blk, r1 = r0 + 1;
   for (int j = threadIdx.x; j < N; j += kFFTThreads) {
     re[j] = (j < ncols_in) ? in[(int64_t)r0 * ld_in + j] : 0.0;
     im[j] = (j < ncols_in && r1 < nrows_in) ? in[(int64_t)r1 * ld_in + j] : 0.0;
@@ -314,19 +339,27 @@ __global__ __launch_bounds__(kFFTThreads) void fft_rows_r2c(const double* __rest
   }
 }
 
+template <int N>
+__global__ __launch_bounds__(kFFTThreads) void fft_rows_r2c(const double* __restrict__ in,
+                                                            int nrows_in, int ncols_in,
+                                                            int ld_in, double2* __restrict__ out,
+                                                            const double2* __restrict__ tw,
+                                                            int nmax, const SC2DState* st) {
+  rows_r2c_body<N>(in, nrows_in, ncols_in, ld_in, out, tw, nmax, st, blockIdx.x);
+}
+
 // (b) column transforms along x (N = M0).  GREEN: full forward transform of all N rows,
 // written transposed as ghat[k][i].  SOLVE: rows >= nrows_in are zero; forward, multiply by
 // the rescaled Green spectrum, inverse, write only rows < nrows_in back.
 template <bool GREEN, int N>
-__global__ __launch_bounds__(kFFTThreads) void fft_cols(double2* __restrict__ spec, int nrows_in,
-                                                        int H, double2* __restrict__ ghat_t,
-                                                        const double2* __restrict__ tw, int nmax,
-                                                        const SC2DState* st, double npad) {
+__device__ __forceinline__ void cols_body(double2* __restrict__ spec, int nrows_in, int H,
+                                          double2* __restrict__ ghat_t,
+                                          const double2* __restrict__ tw, int nmax,
+                                          const SC2DState* st, double npad, int k) {
   if (st && st->skip) return;
   extern __shared__ double sm[];
   double* re = sm;
   double* im = sm + N;
-  const int k = blockIdx.x;
   for (int i = threadIdx.x; i < N; i += kFFTThreads) {
     const double2 v = (i < nrows_in) ? spec[(int64_t)i * H + k] : make_double2(0.0, 0.0);
     re[i] = v.x;
@@ -356,20 +389,26 @@ __global__ __launch_bounds__(kFFTThreads) void fft_cols(double2* __restrict__ sp
   }
 }
 
+template <bool GREEN, int N>
+__global__ __launch_bounds__(kFFTThreads) void fft_cols(double2* __restrict__ spec, int nrows_in,
+                                                        int H, double2* __restrict__ ghat_t,
+                                                        const double2* __restrict__ tw, int nmax,
+                                                        const SC2DState* st, double npad) {
+  cols_body<GREEN, N>(spec, nrows_in, H, ghat_t, tw, nmax, st, npad, blockIdx.x);
+}
+
 // (c) c2r of two rows per complex inverse FFT; writes the first ncols_out columns scaled.
 template <int N>
-__global__ __launch_bounds__(kFFTThreads) void fft_rows_c2r(const double2* __restrict__ spec,
-                                                            int nrows, double* __restrict__ out,
-                                                            int ld_out, int ncols_out,
-                                                            double denom,
-                                                            const double2* __restrict__ tw,
-                                                            int nmax, const SC2DState* st) {
+__device__ __forceinline__ void rows_c2r_body(const double2* __restrict__ spec, int nrows,
+                                              double* __restrict__ out, int ld_out, int ncols_out,
+                                              double denom, const double2* __restrict__ tw,
+                                              int nmax, const SC2DState* st, int blk) {
   if (st && st->skip) return;
   extern __shared__ double sm[];
   double* re = sm;
   double* im = sm + N;
   constexpr int H = N / 2 + 1;
-  const int r0 = 2 * blockIdx.x, r1 = r0 + 1;
+  const int r0 = 2 * blk, r1 = r0 + 1;
   const bool has1 = r1 < nrows;
   for (int k = threadIdx.x; k < H; k += kFFTThreads) {
     double2 a = spec[(int64_t)r0 * H + k];
@@ -391,6 +430,127 @@ __global__ __launch_bounds__(kFFTThreads) void fft_rows_c2r(const double2* __res
   for (int j = threadIdx.x; j < ncols_out; j += kFFTThreads) {
     out[(int64_t)r0 * ld_out + j] = denom * re[j];
     if (has1) out[(int64_t)r1 * ld_out + j] = denom * im[j];
+  }
+}
+
+template <int N>
+__global__ __launch_bounds__(kFFTThreads) void fft_rows_c2r(const double2* __restrict__ spec,
+                                                            int nrows, double* __restrict__ out,
+                                                            int ld_out, int ncols_out,
+                                                            double denom,
+                                                            const double2* __restrict__ tw,
+                                                            int nmax, const SC2DState* st) {
+  rows_c2r_body<N>(spec, nrows, out, ld_out, ncols_out, denom, tw, nmax, st, blockIdx.x);
+}
+
+// ---- batches: every bunch of a batch in one launch per phase ----------------------
+__device__ __forceinline__ SC2DParams batch_params(const BatchBunch& B, double length,
+                                                   double pipe_radius) {
+  SC2DParams P;
+  P.nx = B.nx;
+  P.ny = B.ny;
+  P.nz = B.nz;
+  P.nblk = B.item_end - B.item_begin;
+  P.xy_ratio = B.xy_ratio;
+  P.dx_solver = B.dx_solver;
+  P.macrosize = B.macrosize;
+  P.factor0 = 2 * length * B.r0 / B.gamma_D;  // same operations as the single-bunch host path
+  P.pipe_radius = pipe_radius;
+  P.n_long_avg = B.n_long_avg;
+  return P;
+}
+
+// Bunch whose block range [first(b), first(b+1)) holds block blk (first ascending).
+__device__ __forceinline__ int batch_find(const BatchBunch* bb, int nb, int blk, bool rows) {
+  int lo = 0, hi = nb - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    const int f = rows ? bb[mid].rows_begin : bb[mid].cols_begin;
+    if (f <= blk) lo = mid; else hi = mid - 1;
+  }
+  return lo;
+}
+
+__global__ __launch_bounds__(kSetupThreads) void sc2d_setup_batch(const BatchBunch* bb,
+                                                                  const double* partials,
+                                                                  double length,
+                                                                  double pipe_radius) {
+  const BatchBunch& B = bb[blockIdx.x];
+  const SC2DParams P = batch_params(B, length, pipe_radius);
+  sc2d_setup_body(P, partials, B.item_begin, B.item_end, B.st, B.rho, B.zg, B.zd);
+}
+
+// One block per work item.  Bunches whose grid fits (deposit_lds) accumulate their item's
+// rho in LDS and add it to the grid with one atomic per touched node.
+__global__ __launch_bounds__(kDepositThreads) void sc2d_deposit_batch(const BatchBunch* bb,
+                                                                      const BatchItem* items) {
+  extern __shared__ double lrho[];
+  const BatchItem it = items[blockIdx.x];
+  const BatchBunch& B = bb[it.bunch];
+  const SC2DParams P = batch_params(B, 0.0, 1.0);
+  if (B.deposit_lds) {
+    const int ng = B.nx * B.ny;
+    for (int i = threadIdx.x; i < ng; i += blockDim.x) lrho[i] = 0.0;
+    __syncthreads();
+    sc2d_deposit_body(P, B.v, B.st, lrho, B.zg, it.begin + threadIdx.x, it.end, blockDim.x);
+    __syncthreads();
+    for (int i = threadIdx.x; i < ng; i += blockDim.x)
+      if (lrho[i] != 0.0) atomicAdd(&B.rho[i], lrho[i]);
+  } else {
+    sc2d_deposit_body(P, B.v, B.st, B.rho, B.zg, it.begin + threadIdx.x, it.end, blockDim.x);
+  }
+}
+
+__global__ __launch_bounds__(kSetupThreads) void sc2d_post_batch(const BatchBunch* bb,
+                                                                 double length,
+                                                                 double pipe_radius) {
+  const BatchBunch& B = bb[blockIdx.x];
+  const SC2DParams P = batch_params(B, length, pipe_radius);
+  sc2d_post_body(P, B.st, B.rho, B.zg, B.zd);
+}
+
+#define OB_BATCH_FFT_SIZES(X) X(8) X(16) X(32) X(64) X(128) X(256) X(512) X(1024)
+
+__global__ __launch_bounds__(kFFTThreads) void rows_r2c_batch(const BatchBunch* bb, int nb) {
+  const int b = batch_find(bb, nb, blockIdx.x, true);
+  const BatchBunch& B = bb[b];
+  const int blk = blockIdx.x - B.rows_begin;
+  switch (B.M1) {
+#define OB_CASE(n) \
+  case n: rows_r2c_body<n>(B.rho, B.nx, B.ny, B.ny, B.spec, B.tw, B.nmax, B.st, blk); break;
+    OB_BATCH_FFT_SIZES(OB_CASE)
+#undef OB_CASE
+    default: break;
+  }
+}
+
+__global__ __launch_bounds__(kFFTThreads) void cols_batch(const BatchBunch* bb, int nb) {
+  const int b = batch_find(bb, nb, blockIdx.x, false);
+  const BatchBunch& B = bb[b];
+  const int k = blockIdx.x - B.cols_begin;
+  const double npad = (double)B.M0 * (double)B.M1;
+  switch (B.M0) {
+#define OB_CASE(n) \
+  case n: cols_body<false, n>(B.spec, B.nx, B.H, B.ghat_t, B.tw, B.nmax, B.st, npad, k); break;
+    OB_BATCH_FFT_SIZES(OB_CASE)
+#undef OB_CASE
+    default: break;
+  }
+}
+
+__global__ __launch_bounds__(kFFTThreads) void rows_c2r_batch(const BatchBunch* bb, int nb) {
+  const int b = batch_find(bb, nb, blockIdx.x, true);
+  const BatchBunch& B = bb[b];
+  const int blk = blockIdx.x - B.rows_begin;
+  const double denom = 1.0 / (B.M0 * B.M1);
+  switch (B.M1) {
+#define OB_CASE(n) \
+  case n:                                                                                    \
+    rows_c2r_body<n>(B.spec, B.nx, B.phi, B.ny, B.ny, denom, B.tw, B.nmax, B.st, blk); \
+    break;
+    OB_BATCH_FFT_SIZES(OB_CASE)
+#undef OB_CASE
+    default: break;
   }
 }
 
@@ -508,6 +668,77 @@ void define_green(ob_sc2p5drb* c) {
   OB_HIP(hipFree(G));
 }
 }  // namespace
+
+void sc2d_batch_record(ob_sc2p5drb* c, BatchBunch& r) {
+  r.st = c->st;
+  r.rho = c->rho;
+  r.phi = c->phi;
+  r.zg = c->zg;
+  r.zd = c->zd;
+  r.spec = c->spec;
+  r.ghat_t = c->ghat_t;
+  r.tw = c->tw;
+  r.nx = c->nx;
+  r.ny = c->ny;
+  r.nz = c->nz;
+  r.M0 = c->M0;
+  r.M1 = c->M1;
+  r.H = c->H;
+  r.nmax = c->nmax;
+  r.n_long_avg = c->n_long_avg;
+  r.xy_ratio = c->xy_ratio;
+  r.dx_solver = c->dx_solver;
+}
+
+void sc2d_mark_tracked(ob_sc2p5drb* c) { c->tracked = true; }
+
+int sc2d_read_status(ob_sc2p5drb* c) {
+  int v = 0;
+  OB_HIP(hipMemcpy(&v, &c->st->status, sizeof(int), hipMemcpyDeviceToHost));
+  return v;
+}
+
+void launch_sc2d_batch(const BatchBunch* bb, const BatchItem* items, const BatchShape& sh,
+                       const double* partials, double length, double pipe_radius,
+                       hipStream_t s) {
+  static bool ready = false;
+  if (!ready) {
+    OB_HIP(hipFuncSetAttribute((const void*)sc2d_deposit_batch,
+                               hipFuncAttributeMaxDynamicSharedMemorySize, 128 * 1024));
+    ready = true;
+  }
+  OB_REQUIRE(sh.deposit_lds_bytes <= 128 * 1024, "batch deposit: LDS block too large");
+  OB_REQUIRE(sh.max_M <= 1024, "batch: grids up to 512 points per axis");
+  {
+    ProfScope ps("sc2d_setup_batch");
+    hipLaunchKernelGGL(sc2d_setup_batch, dim3(sh.nb), dim3(kSetupThreads), 0, s, bb, partials,
+                       length, pipe_radius);
+  }
+  {
+    ProfScope ps("sc2d_deposit_batch");
+    hipLaunchKernelGGL(sc2d_deposit_batch, dim3(sh.nitems), dim3(kDepositThreads),
+                       sh.deposit_lds_bytes, s, bb, items);
+  }
+  {
+    ProfScope ps("sc2d_post_batch");
+    hipLaunchKernelGGL(sc2d_post_batch, dim3(sh.nb), dim3(kSetupThreads), 0, s, bb, length,
+                       pipe_radius);
+  }
+  const size_t fl = sizeof(double) * 2 * sh.max_M;
+  {
+    ProfScope ps("fft_rows_r2c_batch");
+    hipLaunchKernelGGL(rows_r2c_batch, dim3(sh.row_blocks), dim3(kFFTThreads), fl, s, bb, sh.nb);
+  }
+  {
+    ProfScope ps("fft_cols_batch");
+    hipLaunchKernelGGL(cols_batch, dim3(sh.col_blocks), dim3(kFFTThreads), fl, s, bb, sh.nb);
+  }
+  {
+    ProfScope ps("fft_rows_c2r_batch");
+    hipLaunchKernelGGL(rows_c2r_batch, dim3(sh.row_blocks), dim3(kFFTThreads), fl, s, bb, sh.nb);
+  }
+  OB_HIP(hipGetLastError());
+}
 }  // namespace ob
 
 #define OB_API extern "C" __attribute__((visibility("default")))

@@ -15,47 +15,73 @@ def _h(bunch):
     return bunch._h
 
 
+def _batched(bunch):
+    return getattr(bunch, "_batch", None)
+
+
 def rotatexy(bunch, angle):
+    if _batched(bunch):
+        return bunch._batch.trackSequence([("rotatexy", angle)])
     N.call("ob_tpb_rotatexy", _h(bunch), float(angle))
 
 
 def drift(bunch, length):
+    if _batched(bunch):
+        return bunch._batch.trackSequence([("drift", length)])
     N.call("ob_tpb_drift", _h(bunch), float(length))
 
 
 def wrapbunch(bunch, length):
+    if _batched(bunch):
+        return bunch._batch.trackSequence([("wrapbunch", length)])
     N.call("ob_tpb_wrapbunch", _h(bunch), float(length))
 
 
 def kick(bunch, kx, ky, kE, useCharge=1):
+    if _batched(bunch):
+        return bunch._batch.trackSequence([("kick", kx, ky, kE, useCharge)])
     N.call("ob_tpb_kick", _h(bunch), float(kx), float(ky), float(kE), int(useCharge))
 
 
 def multp(bunch, pole, kl, skew, useCharge=1):
+    if _batched(bunch):
+        return bunch._batch.trackSequence([("multp", pole, kl, skew, useCharge)])
     N.call("ob_tpb_multp", _h(bunch), int(pole), float(kl), int(skew), int(useCharge))
 
 
 def multpfringeIN(bunch, pole, kl, skew, useCharge=1):
+    if _batched(bunch):
+        return bunch._batch.trackSequence([("multpfringeIN", pole, kl, skew, useCharge)])
     N.call("ob_tpb_multpfringeIN", _h(bunch), int(pole), float(kl), int(skew), int(useCharge))
 
 
 def multpfringeOUT(bunch, pole, kl, skew, useCharge=1):
+    if _batched(bunch):
+        return bunch._batch.trackSequence([("multpfringeOUT", pole, kl, skew, useCharge)])
     N.call("ob_tpb_multpfringeOUT", _h(bunch), int(pole), float(kl), int(skew), int(useCharge))
 
 
 def quad1(bunch, length, kq, useCharge=1):
+    if _batched(bunch):
+        return bunch._batch.trackSequence([("quad1", length, kq, useCharge)])
     N.call("ob_tpb_quad1", _h(bunch), float(length), float(kq), int(useCharge))
 
 
 def quad2(bunch, length):
+    if _batched(bunch):
+        return bunch._batch.trackSequence([("quad2", length)])
     N.call("ob_tpb_quad2", _h(bunch), float(length))
 
 
 def quadfringeIN(bunch, kq, useCharge=1):
+    if _batched(bunch):
+        return bunch._batch.trackSequence([("quadfringeIN", kq, useCharge)])
     N.call("ob_tpb_quadfringeIN", _h(bunch), float(kq), int(useCharge))
 
 
 def quadfringeOUT(bunch, kq, useCharge=1):
+    if _batched(bunch):
+        return bunch._batch.trackSequence([("quadfringeOUT", kq, useCharge)])
     N.call("ob_tpb_quadfringeOUT", _h(bunch), float(kq), int(useCharge))
 
 
