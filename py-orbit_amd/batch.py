@@ -15,6 +15,7 @@ tracking each bunch on its own.  While a bunch is in a batch its maps go through
 first applies the batch's pending work.
 """
 import ctypes as C
+import weakref
 
 import _obnative as N
 import teapot_base as TPB
@@ -91,10 +92,14 @@ class BunchBatch:
 
 
 class _BatchBunch:
-    """What the lattice nodes see: teapot_base maps queue on the batch."""
+    """What the lattice nodes see: teapot_base maps queue on the batch.
+
+    It refers to the batch weakly: the batch is then freed by reference counting, before
+    the bunches and calculators it holds, never by the cycle collector in arbitrary order.
+    """
 
     def __init__(self, batch):
-        self._batch = batch
+        self._batch = weakref.proxy(batch)
         self._h = None  # never passed to a single-bunch entry point
 
     def getSyncParticle(self):
@@ -122,7 +127,7 @@ class _BatchBunch:
 
 class _BatchCalc:
     def __init__(self, batch):
-        self._batch = batch
+        self._batch = weakref.proxy(batch)
 
     def trackBunch(self, bunch, length, pipe_radius):
         if bunch is not self._batch._proxy:

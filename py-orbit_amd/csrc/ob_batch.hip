@@ -244,6 +244,27 @@ void batch_flush_owner(ob_batch* B) {
   flush(B, false);
 }
 
+std::vector<ob_batch*> g_live_batches;
+
+void batch_forget_calc(ob_sc2p5drb* c) {
+  for (ob_batch* B : g_live_batches) {
+    if (std::find(B->calcs.begin(), B->calcs.end(), c) == B->calcs.end()) continue;
+    // The pending kick (and a captured graph) read this calculator's buffers: apply the
+    // kick now, then drop every calculator of the batch; SC tracking needs a new set.
+    std::string err;
+    try {
+      if (!B->capturing) flush(B, false);
+    } catch (const Error& e) {
+      err = e.msg;
+    }
+    B->calcs.clear();
+    B->kick = BatchKick{};
+    B->graph_version = -1;
+    B->built_chunk = 0;
+    OB_REQUIRE(err.empty(), err);
+  }
+}
+
 }  // namespace ob
 
 #define OB_API extern "C" __attribute__((visibility("default")))
@@ -273,6 +294,7 @@ OB_API int ob_batch_create(ob_bunch** bunches, int nb, ob_batch** out) {
     }
     OB_REQUIRE(B->leader != nullptr, "batch: all bunches are empty");
     for (ob_bunch* b : B->bunches) ob::bunch_set_owner(b, B);
+    ob::g_live_batches.push_back(B);
   } catch (...) {
     delete B;
     throw;
@@ -284,6 +306,8 @@ OB_API int ob_batch_create(ob_bunch** bunches, int nb, ob_batch** out) {
 OB_API int ob_batch_destroy(ob_batch* B) {
   OB_TRY if (!B) return 0;
   if (!B->capturing) ob::flush(B, false);  // nothing pending is lost
+  auto& lb = ob::g_live_batches;
+  lb.erase(std::remove(lb.begin(), lb.end(), B), lb.end());
   for (ob_bunch* b : B->bunches) ob::bunch_set_owner(b, nullptr);
   (void)hipStreamSynchronize(ob::stream());
   if (B->exec) (void)hipGraphExecDestroy(B->exec);

@@ -5,6 +5,7 @@
 // mirror used only by per-particle accessors and bulk copies.  teapot_base calls do not
 // launch anything: they append an Op (with host-precomputed constants) to the bunch's
 // queue, and the queue is applied by one fused kernel when the bunch is next needed.
+#include <algorithm>
 #include <cmath>
 #include <cstring>
 #include <mutex>
@@ -342,6 +343,29 @@ void bunch_set_sync_time(ob_bunch* b, double t) { b->time = t; }
 
 void bunch_set_owner(ob_bunch* b, ob_batch* owner) { b->owner = owner; }
 
+std::vector<ob_bunch*> g_live_bunches;
+
+// A calculator is about to free its buffers: apply every queued space-charge kick that
+// reads them (the kick belongs to a trackBunch call that has already happened).
+void flush_kicks_reading(const void* state) {
+  for (ob_bunch* b : g_live_bunches) {
+    for (const Op& op : b->queue) {
+      const double want = ptr_bits(state);
+      if ((op.kind == OP_SC2P5DRB_KICK || op.kind == OP_SC3D_KICK) &&
+          std::memcmp(&op.p[0], &want, sizeof(double)) == 0) {
+        try {
+          bunch_flush(b, nullptr);
+        } catch (const Error&) {
+          // The kick cannot be applied; it must not outlive the buffers it reads.
+          b->queue.clear();
+          throw;
+        }
+        break;
+      }
+    }
+  }
+}
+
 SyncInfo bunch_sync(ob_bunch* b) {
   return SyncInfo{b->mass, b->charge, b->r0, b->macrosize, b->T, b->p, b->beta, b->gamma, b->time};
 }
@@ -466,12 +490,15 @@ OB_API int ob_bunch_create(ob_bunch** out) {
   OB_TRY OB_REQUIRE(out != nullptr, "null output pointer");
   ob::ensure_init();
   *out = new ob_bunch();
+  ob::g_live_bunches.push_back(*out);
   OB_CATCH
 }
 
 OB_API int ob_bunch_destroy(ob_bunch* b) {
   OB_TRY if (!b) return 0;
   OB_REQUIRE(b->owner == nullptr, "bunch belongs to a batch: destroy the batch first");
+  auto& lb = ob::g_live_bunches;
+  lb.erase(std::remove(lb.begin(), lb.end(), b), lb.end());
   (void)hipStreamSynchronize(ob::stream());
   for (void* p : {(void*)b->d, (void*)b->d_msize, (void*)b->d_alt, (void*)b->d_msize_alt,
                   (void*)b->d_perm, (void*)b->d_perm_alt})

@@ -206,6 +206,11 @@ namespace ob {
 // A batched bunch's flush applies the whole batch's pending work (ob_batch.hip).
 void bunch_set_owner(ob_bunch* b, ob_batch* owner);
 void batch_flush_owner(ob_batch* B);
+// Live bunches; apply the queued SC kicks whose op p[0] is `state` (before it is freed).
+extern std::vector<ob_bunch*> g_live_bunches;
+void flush_kicks_reading(const void* state);
+// A 2.5D calculator is going away: batches using it apply their pending kicks and drop it.
+void batch_forget_calc(ob_sc2p5drb* c);
 // Queue one map of the public op list (ob_map_op) exactly as its entry point does.
 int queue_map_op(ob_bunch* b, const ob_map_op& o);
 // A kernel outside the map queue has written the device coordinates.

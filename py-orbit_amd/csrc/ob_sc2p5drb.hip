@@ -800,10 +800,19 @@ OB_API int ob_sc2p5drb_create(int nx, int ny, int nz, double xy_ratio, ob_sc2p5d
 
 OB_API int ob_sc2p5drb_destroy(ob_sc2p5drb* c) {
   OB_TRY if (!c) return 0;
+  // Pending kicks read st/phi/zg/zd: apply them before the buffers go.
+  std::string err;
+  try {
+    ob::batch_forget_calc(c);
+    ob::flush_kicks_reading(c->st);
+  } catch (const ob::Error& e) {
+    err = e.msg;
+  }
   (void)hipStreamSynchronize(ob::stream());
   (void)hipFree(c->rho); (void)hipFree(c->phi); (void)hipFree(c->zg); (void)hipFree(c->zd);
   (void)hipFree(c->spec); (void)hipFree(c->ghat_t); (void)hipFree(c->tw); (void)hipFree(c->partials); (void)hipFree(c->st);
   delete c;
+  OB_REQUIRE(err.empty(), err);
   OB_CATCH
 }
 

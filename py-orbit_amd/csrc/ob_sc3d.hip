@@ -692,12 +692,20 @@ OB_API int ob_sc3d_create(int nx, int ny, int nz, ob_sc3d** out) {
 
 OB_API int ob_sc3d_destroy(ob_sc3d* c) {
   OB_TRY if (!c) return 0;
+  // Pending kicks read phi: apply them before the buffers go.
+  std::string err;
+  try {
+    ob::flush_kicks_reading(c->phi);
+  } catch (const ob::Error& e) {
+    err = e.msg;
+  }
   (void)hipStreamSynchronize(ob::stream());
   (void)hipFree(c->rho); (void)hipFree(c->phi); (void)hipFree(c->spec); (void)hipFree(c->ghat);
   (void)hipFree(c->partials); (void)hipFree(c->ext_dev); (void)hipFree(c->tw);
   (void)hipHostFree(c->ext_host);
   c->work.release();
   delete c;
+  OB_REQUIRE(err.empty(), err);
   OB_CATCH
 }
 

@@ -46,6 +46,45 @@ def waterbag_coords(n, seed=SEED, rx=6e-3, rxp=1e-3, sz=0.3, sdE=1e-3):
     return c
 
 
+def kv_coords(n, seed=SEED, rx=6e-3, rxp=1e-3, zhalf=100.0, sdE=1e-3):
+    """Kapchinskij-Vladimirskij: uniform on the 4D ellipsoid surface
+    (x/rx)^2+(xp/rxp)^2+(y/rx)^2+(yp/rxp)^2 = 1 (uniform x-y projection)."""
+    r = np.random.default_rng(seed)
+    g = r.standard_normal((4, n))
+    g /= np.sqrt((g ** 2).sum(axis=0))
+    c = np.empty((6, n))
+    c[0], c[1], c[2], c[3] = rx * g[0], rxp * g[1], rx * g[2], rxp * g[3]
+    c[4] = r.uniform(-zhalf, zhalf, n)
+    c[5] = r.normal(0.0, sdE, n)
+    return c
+
+
+BATCH_BETA = 6.0  # m, beta function used to turn an emittance into beam sizes
+
+
+def batch_specs(nbunch=64, seed=SEED, nmin=50_000, nmax=2_000_000):
+    """Bunches of the batched workload: sizes uniform in [nmin, nmax], Gaussian or KV,
+    rms emittance 1-20 mm mrad, intensity 1e12-1e14, 2.5D grids 64^2, 128^2 or 256^2 with
+    64 z bins (all seeded)."""
+    r = np.random.default_rng(seed + 7)
+    specs = []
+    for k in range(nbunch):
+        n = int(r.integers(nmin, nmax + 1))
+        eps = float(r.uniform(1e-6, 20e-6))
+        kind = "gauss" if r.random() < 0.5 else "kv"
+        g = int(r.choice([64, 128, 256]))
+        specs.append({"n": n, "eps": eps, "kind": kind, "grid": (g, g, 64),
+                      "intensity": float(10 ** r.uniform(12, 14)), "seed": seed + 1000 + k})
+    return specs
+
+
+def batch_coords(spec):
+    s, e = np.sqrt(spec["eps"] * BATCH_BETA), np.sqrt(spec["eps"] / BATCH_BETA)
+    if spec["kind"] == "gauss":
+        return gaussian_coords(spec["n"], seed=spec["seed"], sx=s, sxp=e)
+    return kv_coords(spec["n"], seed=spec["seed"], rx=2 * s, rxp=2 * e)  # rms = edge / 2
+
+
 def make_bunch(coords, kin_energy=1.0, intensity=INTENSITY):
     b = Bunch()
     b.getSyncParticle().kinEnergy(kin_energy)

@@ -127,3 +127,47 @@ def test_batch_rejects_mixed_species_and_zero_extent():
     bb.calculator().trackBunch(bb.bunch(), 0.5, 0.1)
     with pytest.raises(OrbitError, match="zero extent"):
         bb.synchronize()
+
+
+def _destroy_now(calc, fn):
+    """Free a calculator's device state while work that reads it may still be queued."""
+    import _obnative as N
+    h, calc._h = calc._h, None
+    N.call(fn, h)
+
+
+def test_destroying_calculators_applies_pending_kicks():
+    """The last SC kick of a lattice pass is deferred to the next push; freeing the
+    calculator first must apply it (batch and single bunch), not drop it or read freed
+    buffers."""
+    from batch import BunchBatch
+    ref, _ = _single(1)
+    bunches, calcs = _setup()
+    bb = BunchBatch(bunches, calcs)
+    _lattice(bb.calculator(), 1).trackBunch(bb.bunch())
+    for c in calcs:
+        _destroy_now(c, "ob_sc2p5drb_destroy")
+    for k, b in enumerate(bunches):
+        out = b.getCoordinates()
+        for j in range(6):
+            assert rel_l2(out[j], ref[k][j]) < 1e-9, (k, j)
+    with pytest.raises(Exception):
+        bb.calculator().trackBunch(bb.bunch(), 0.1, 0.1)   # no calculators left
+    del bb
+
+    import spacecharge
+    from workloads import make_bunch
+    for k in range(2):
+        b1 = make_bunch(_coords(k, 5000), intensity=1e12)
+        b2 = make_bunch(_coords(k, 5000), intensity=1e12)
+        mk = (lambda: spacecharge.SpaceChargeCalc2p5Drb(32, 32, 8)) if k == 0 else \
+             (lambda: spacecharge.SpaceChargeCalc3D(16, 16, 16))
+        args = (0.3, 0.05) if k == 0 else (0.3,)
+        c1, c2 = mk(), mk()
+        c1.trackBunch(b1, *args)
+        ref1 = b1.getCoordinates()
+        c2.trackBunch(b2, *args)
+        _destroy_now(c2, "ob_sc2p5drb_destroy" if k == 0 else "ob_sc3d_destroy")
+        out = b2.getCoordinates()
+        for j in range(6):
+            assert rel_l2(out[j], ref1[j]) < 1e-9, (k, j)
