@@ -124,7 +124,9 @@ def kernel_bytes(workload, n, grid):
     else:
         H = M1 // 2 + 1
         b.update({
-            "sc2d_deposit": 24 * n + 8 * nx * ny + 8 * nz,
+            "sc2d_hist": 24 * n + 8 * nz,
+            "sc2d_scatter": 32 * n,
+            "sc2d_tdeposit": 16 * n + 8 * nx * ny,
             "fft_rows_r2c": 8 * nx * ny + 16 * nx * H,
             "fft_cols": 32 * nx * H + 16 * M0 * H,
             "fft_rows_c2r": 16 * nx * H + 8 * nx * ny,
@@ -238,7 +240,8 @@ def main_batch(args):
         prof[name] = (int(cnt), float(tot), float(mn), float(mx))
     grid_nodes = sum(sp["grid"][0] * sp["grid"][1] for sp in specs)
     kb = {"push_batch_extrema": 96 * n, "push_batch": 96 * n,
-          "sc2d_deposit_batch": 24 * n + 8 * grid_nodes}
+          "sc2d_hist_batch": 24 * n, "sc2d_scatter_batch": 32 * n,
+          "sc2d_tdeposit_batch": 16 * n + 8 * grid_nodes}
     dom = max((k for k in prof if k in kb), key=lambda k: prof[k][1])
     cnt, tot, _, _ = prof[dom]
     avg_s = tot / cnt / 1e3

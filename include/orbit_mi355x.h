@@ -183,10 +183,10 @@ int ob_batch_sc2p5drb_track(ob_batch* batch, double length, double pipe_radius);
 int ob_batch_flush(ob_batch* batch);
 /* Wait; fails if a bunch had zero extent in its last SC step (the reference aborts). */
 int ob_batch_synchronize(ob_batch* batch);
-/* Autotuning: particles per work item and the largest rho grid (bytes) deposited in LDS.
- * Tuned by timing on the first SC step unless set here. */
-int ob_batch_set_tuning(ob_batch* batch, int chunk, int lds_max_bytes);
-int ob_batch_get_tuning(ob_batch* batch, int* chunk, int* lds_max_bytes, int* tuned);
+/* Autotuning: particles per push/binning work item (chunk) and per tile-deposit work item
+ * (deposit_item).  Tuned by timing on the first SC step unless set here. */
+int ob_batch_set_tuning(ob_batch* batch, int chunk, int deposit_item);
+int ob_batch_get_tuning(ob_batch* batch, int* chunk, int* deposit_item, int* tuned);
 /* hipGraph: the batch calls between begin and end are recorded (not run) and replayed by
  * ob_batch_replay(count).  Pending work is applied before and at the end of the capture,
  * so a replay starts and ends with nothing pending.  Bunch sizes must not change between

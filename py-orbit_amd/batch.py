@@ -74,10 +74,11 @@ class BunchBatch:
     def tuning(self):
         c, l, t = C.c_int(), C.c_int(), C.c_int()
         N.call("ob_batch_get_tuning", self._h, C.byref(c), C.byref(l), C.byref(t))
-        return {"chunk": c.value, "lds_max_bytes": l.value, "tuned": bool(t.value)}
+        return {"chunk": c.value, "deposit_item": l.value, "tuned": bool(t.value)}
 
-    def setTuning(self, chunk, lds_max_bytes):
-        N.call("ob_batch_set_tuning", self._h, int(chunk), int(lds_max_bytes))
+    def setTuning(self, chunk, deposit_item):
+        """Particles per push/hist work item and per tile-deposit work item."""
+        N.call("ob_batch_set_tuning", self._h, int(chunk), int(deposit_item))
 
     def capture(self, fn):
         """Record the batch work done by fn() as a hipGraph (not run: replay runs it)."""

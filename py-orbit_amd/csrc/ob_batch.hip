@@ -6,12 +6,13 @@
 //   push     one workgroup per work item (a chunk of one bunch's particles): deferred SC
 //            kick, the map program, per-item extrema;
 //   setup    one workgroup per bunch (extrema of its items, grid sizing, clearing);
-//   deposit  one workgroup per work item (rho in LDS when the grid fits, else atomics);
+//   deposit  tile sort per bunch: hist and scatter one workgroup per work item, then one
+//            workgroup per tile work item accumulating rho in LDS;
 //   post     one workgroup per bunch; FFT row / column / row passes over all bunches'
 //            rows and columns (the FFT length is dispatched per workgroup).
 // The per-bunch records and the work items live in device memory and change only when a
 // bunch changes size, so a stretch of tracking (a lattice cell) can be captured once into
-// a hipGraph and replayed.  The work-item size and the LDS-deposit threshold are chosen
+// a hipGraph and replayed.  The work-item size and the tile-deposit item size are chosen
 // by timing the candidates on the batch's own data (autotune).
 //
 // Results are those of tracking each bunch alone: the same device bodies, the same host
@@ -30,7 +31,7 @@ struct ob_batch {
   std::vector<ob::Op> queue;           // maps waiting to be applied to all bunches
   ob::BatchKick kick{};                // deferred SC kicks of all bunches
   int chunk = 16384;                   // particles per work item
-  int lds_max = 64 * 1024;             // grids of at most this many bytes deposit in LDS
+  int item = 2048;                     // particles per tile-deposit work item
   bool tuned = false, fixed = false;   // fixed: tuning set by the caller
   // device records
   std::vector<ob::BatchBunch> h_bb;
@@ -42,7 +43,7 @@ struct ob_batch {
   ob::BatchShape shape{};
   std::vector<int64_t> seen_n;         // sizes / pointers the records were built for
   std::vector<const double*> seen_ptr;
-  int built_chunk = 0, built_lds = -1;
+  int built_chunk = 0, built_item = -1;
   int version = 0;                     // incremented by every rebuild of the records
   // graph
   int graph_version = -1;              // records version the graph was captured with
@@ -69,19 +70,23 @@ void check_same_species(const std::vector<ob_bunch*>& bs) {
 // (Re)build the device records when the bunches or the tuning changed.
 void build_records(ob_batch* B, bool force) {
   const int nb = (int)B->bunches.size();
-  bool same = !force && B->built_chunk == B->chunk && B->built_lds == B->lds_max &&
+  bool same = !force && B->built_chunk == B->chunk && B->built_item == B->item &&
               (int)B->seen_n.size() == nb;
   std::vector<DeviceBunchView> views(nb);
   for (int b = 0; b < nb; b++) {
     views[b] = bunch_device_view(B->bunches[b]);
     if (same && (B->seen_n[b] != views[b].n || B->seen_ptr[b] != views[b].c[0])) same = false;
+    // the calculator's tile buffers move if it tracked a larger bunch on its own
+    if (same && !B->calcs.empty() &&
+        !sc2d_tile_same(B->calcs[b], B->h_bb[b].tg, views[b].n, views[b].msize != nullptr))
+      same = false;
   }
   if (same) return;
   OB_REQUIRE(!B->capturing, "batch: bunch sizes changed during a graph capture");
   B->h_bb.assign(nb, BatchBunch{});
   B->h_items.clear();
-  int rows = 0, cols = 0, max_M = 16;
-  size_t lds = 0;
+  int rows = 0, cols = 0, max_M = 16, slots = 0;
+  size_t hist_lds = 0, dep_lds = 0;
   for (int b = 0; b < nb; b++) {
     BatchBunch& r = B->h_bb[b];
     std::memset(&r, 0, sizeof(r));
@@ -102,9 +107,12 @@ void build_records(ob_batch* B, bool force) {
       rows += (r.nx + 1) / 2;
       cols += r.H;
       max_M = std::max(max_M, std::max(r.M0, r.M1));
-      const size_t g = sizeof(double) * (size_t)r.nx * r.ny;
-      r.deposit_lds = g <= (size_t)B->lds_max ? 1 : 0;
-      if (r.deposit_lds) lds = std::max(lds, g);
+      r.tg = sc2d_tile_reserve(B->calcs[b], r.v.n, r.v.msize != nullptr, B->item);
+      r.slot_begin = slots;
+      slots += (int)(r.v.n / r.tg.item + r.tg.ntiles + 1);
+      r.slot_end = slots;
+      hist_lds = std::max(hist_lds, sizeof(double) * r.nz + sizeof(uint32_t) * r.tg.ntiles);
+      dep_lds = std::max(dep_lds, sizeof(double) * (r.tg.T + 2) * (r.tg.T + 2));
     }
   }
   if (nb > B->cap_bb) {
@@ -126,7 +134,7 @@ void build_records(ob_batch* B, bool force) {
     OB_HIP(hipMemcpyAsync(B->d_items, B->h_items.data(), sizeof(BatchItem) * ni,
                           hipMemcpyHostToDevice, s));
   OB_HIP(hipStreamSynchronize(s));  // host vectors may change before the copies run
-  B->shape = BatchShape{nb, ni, rows, cols, max_M, kBatchThreads, lds};
+  B->shape = BatchShape{nb, ni, rows, cols, max_M, slots, hist_lds, dep_lds};
   B->seen_n.resize(nb);
   B->seen_ptr.resize(nb);
   for (int b = 0; b < nb; b++) {
@@ -134,7 +142,7 @@ void build_records(ob_batch* B, bool force) {
     B->seen_ptr[b] = views[b].c[0];
   }
   B->built_chunk = B->chunk;
-  B->built_lds = B->lds_max;
+  B->built_item = B->item;
   B->version++;
 }
 
@@ -181,7 +189,7 @@ float time_ms(hipEvent_t a, hipEvent_t b) {
 }
 
 // Time the candidates on the batch's data: work-item size with an identity push (reads
-// and writes every particle unchanged), then the LDS-deposit threshold with repeated
+// and writes every particle unchanged), then the tile-deposit item size with repeated
 // setup + deposit (setup clears the grids, so the repeats are idempotent).
 void autotune(ob_batch* B) {
   hipStream_t s = stream();
@@ -212,9 +220,9 @@ void autotune(ob_batch* B) {
   B->chunk = best_chunk;
   if (!B->calcs.empty()) {
     best = 1e30f;
-    int best_lds = B->lds_max;
-    for (int lds : {0, 32 * 1024, 128 * 1024}) {
-      B->lds_max = lds;
+    int best_item = B->item;
+    for (int item : {1024, 4096}) {
+      B->item = item;
       build_records(B, false);
       launch_push_batch(B->d_bb, B->d_items, B->shape.nitems, idp, BatchKick{}, B->d_partials,
                         kBatchThreads, s);
@@ -226,10 +234,10 @@ void autotune(ob_batch* B) {
       const float t = time_ms(e0, e1);
       if (t < best) {
         best = t;
-        best_lds = lds;
+        best_item = item;
       }
     }
-    B->lds_max = best_lds;
+    B->item = best_item;
   }
   build_records(B, false);
   OB_HIP(hipEventDestroy(e0));
@@ -322,8 +330,10 @@ OB_API int ob_batch_set_sc2p5drb(ob_batch* B, ob_sc2p5drb** calcs) {
   OB_TRY OB_REQUIRE(B != nullptr && calcs != nullptr, "null handle");
   OB_REQUIRE(!B->capturing, "batch: cannot change calculators during a capture");
   std::vector<ob_sc2p5drb*> cs(calcs, calcs + B->bunches.size());
-  for (ob_sc2p5drb* c : cs) {
+  for (size_t k = 0; k < cs.size(); k++) {
+    ob_sc2p5drb* c = cs[k];
     OB_REQUIRE(c != nullptr, "null calculator handle");
+    for (size_t j = 0; j < k; j++) OB_REQUIRE(cs[j] != c, "a calculator appears twice");
     ob::BatchBunch r{};
     ob::sc2d_batch_record(c, r);
     OB_REQUIRE(r.M0 <= 1024 && r.M1 <= 1024, "batch: grids up to 512 points per axis");
@@ -384,20 +394,20 @@ OB_API int ob_batch_synchronize(ob_batch* B) {
   OB_CATCH
 }
 
-OB_API int ob_batch_set_tuning(ob_batch* B, int chunk, int lds_max_bytes) {
+OB_API int ob_batch_set_tuning(ob_batch* B, int chunk, int deposit_item) {
   OB_TRY OB_REQUIRE(B != nullptr, "null handle");
   OB_REQUIRE(chunk >= 256 && chunk <= (1 << 24), "work-item size out of range");
-  OB_REQUIRE(lds_max_bytes >= 0 && lds_max_bytes <= 128 * 1024, "LDS threshold out of range");
+  OB_REQUIRE(deposit_item >= 64 && deposit_item <= (1 << 20), "deposit work-item size out of range");
   B->chunk = chunk;
-  B->lds_max = lds_max_bytes;
+  B->item = deposit_item;
   B->tuned = B->fixed = true;
   OB_CATCH
 }
 
-OB_API int ob_batch_get_tuning(ob_batch* B, int* chunk, int* lds_max_bytes, int* tuned) {
+OB_API int ob_batch_get_tuning(ob_batch* B, int* chunk, int* deposit_item, int* tuned) {
   OB_TRY OB_REQUIRE(B != nullptr, "null handle");
   if (chunk) *chunk = B->chunk;
-  if (lds_max_bytes) *lds_max_bytes = B->lds_max;
+  if (deposit_item) *deposit_item = B->item;
   if (tuned) *tuned = B->tuned ? 1 : 0;
   OB_CATCH
 }

@@ -125,6 +125,17 @@ struct DeviceBunchView {
 int launch_push(const DeviceBunchView& v, const OpProgram& prog, double* ext_partials,
                 hipStream_t s);
 
+// Tile sort of a 2.5D deposit: T x T tiles of TSC centre nodes, work items of at most
+// `item` particles of one tile, scratch copies of the in-grid particles sorted by tile.
+struct Tile2D {
+  int32_t T, shift, tx, ty, ntiles, item, max_items, pad;  // T = 1 << shift
+  uint32_t* counts;   // [ntiles], zero between steps
+  uint32_t* cursor;   // [ntiles]
+  int4* items;        // [max_items] (tile, begin, end, 0)
+  uint32_t* nitems;   // [1]
+  double *sx, *sy, *sm;
+};
+
 // ---- batches of bunches, each with its own SpaceChargeCalc2p5Drb ----------------------
 // Device-resident record of one bunch of a batch.
 struct BatchBunch {
@@ -137,8 +148,8 @@ struct BatchBunch {
   int32_t nx, ny, nz, M0, M1, H, nmax, n_long_avg;
   double xy_ratio, dx_solver, macrosize, r0, mass;
   double gamma_D;  // (beta*beta)*pow(gamma,3) of the synchronous particle
-  int32_t deposit_lds;  // 1: accumulate rho of a work item in LDS (grid fits), 0: atomics
-  int32_t pad;
+  Tile2D tg;                      // deposit tile sort (buffers owned by the calculator)
+  int32_t slot_begin, slot_end;   // its blocks of the batched tile-deposit pass
 };
 struct BatchItem {  // particles [begin, end) of one bunch
   int32_t bunch, pad;
@@ -154,8 +165,8 @@ void launch_push_batch(const BatchBunch* bb, const BatchItem* items, int nitems,
                        int threads, hipStream_t s);
 // SpaceChargeCalc2p5Drb steps of all bunches after launch_push_batch with extrema.
 struct BatchShape {
-  int nb, nitems, row_blocks, col_blocks, max_M, deposit_threads;
-  size_t deposit_lds_bytes;
+  int nb, nitems, row_blocks, col_blocks, max_M, dep_slots;
+  size_t hist_lds, dep_lds;  // dynamic LDS of the hist/scatter and tile-deposit passes
 };
 void launch_sc2d_batch(const BatchBunch* bb, const BatchItem* items, const BatchShape& sh,
                        const double* partials, double length, double pipe_radius, hipStream_t s);
@@ -165,6 +176,11 @@ namespace ob {
 // Calculator fields of a batch record; mark a calculator as tracked; its status word
 // (1: the last step found a zero-extent bunch).
 void sc2d_batch_record(ob_sc2p5drb* c, BatchBunch& r);
+// Size the calculator's tile-sort buffers for a bunch of n particles (per-particle
+// macrosize or not) with work items of `item` particles; returns the tile sort.
+Tile2D sc2d_tile_reserve(ob_sc2p5drb* c, int64_t n, bool per_particle_m, int item);
+// Whether `t` still describes the calculator's tile buffers and fits such a bunch.
+bool sc2d_tile_same(ob_sc2p5drb* c, const Tile2D& t, int64_t n, bool per_particle_m);
 void sc2d_mark_tracked(ob_sc2p5drb* c);
 int sc2d_read_status(ob_sc2p5drb* c);
 int push_grid_blocks(int64_t n);

@@ -41,6 +41,7 @@ namespace {
 constexpr int kSetupThreads = 1024;
 constexpr int kDepositThreads = 256;
 constexpr int kMaxZBinsLDS = 4096;
+constexpr int kUnroll = 4;  // particles per thread whose loads are issued together
 constexpr int kFFTThreads = 256;
 
 struct SC2DParams {
@@ -125,49 +126,135 @@ __global__ __launch_bounds__(kSetupThreads) void sc2d_setup(SC2DParams P,
   sc2d_setup_body(P, partials, 0, P.nblk, st, rho, zg, zd);
 }
 
-// ---- 3: deposit -----------------------------------------------------------
-// Particles first, first+stride, ... < end of one bunch; the block's z line density is
-// summed in LDS and added to zg at the end.
-__device__ __forceinline__ void sc2d_deposit_body(const SC2DParams& P, const DeviceBunchView& v,
-                                                  const SC2DState* st, double* __restrict__ rho,
-                                                  double* __restrict__ zg, int64_t first,
-                                                  int64_t end, int64_t stride) {
-  __shared__ double zl[kMaxZBinsLDS];
+// ---- 3: deposit, tile-sorted ----------------------------------------------------
+// Binning particle by particle into the whole grid (Grid2D::binValue, Grid2D.cc:142-166)
+// piles every wave's 9 FP64 atomics onto the few nodes of the beam core.  Instead:
+//   a. hist    per block: LDS counts of the particles per T x T tile of TSC centre nodes,
+//              and the z line density (Grid1D::binValue, Grid1D.cc:422-440) in LDS; both
+//              merged with atomics; then one block scans the counts into scatter
+//              positions and work items of at most `item` particles of one tile;
+//   b. scatter counting-sort copy of (x, y[, macrosize]) of the in-grid particles into
+//              scratch arrays, tile by tile (the bunch itself is not reordered);
+//   c. deposit one workgroup per work item accumulates its particles in an LDS block of
+//              the tile plus a one-node halo, then adds the block to rho with atomics.
+// Each contribution is the reference's product; only the FP64 summation order differs.
+
+// Tile of the TSC centre node of (x, y), or -1 outside the grid (no charge deposited).
+__device__ __forceinline__ int tile2d_key(const Tile2D& tg, int nx, int ny, double xMin,
+                                          double xMax, double yMin, double yMax, double dx,
+                                          double dy, double x, double y) {
+  if (x < xMin || x > xMax || y < yMin || y > yMax) return -1;
+  int iX, iY;
+  double xF, yF;
+  node_ind_frac(x, xMin, dx, nx, iX, xF);
+  node_ind_frac(y, yMin, dy, ny, iY, yF);
+  return (iX >> tg.shift) * tg.ty + (iY >> tg.shift);
+}
+
+// Exclusive block scan of two counters (blockDim.x a multiple of 64, at most 1024).
+__device__ __forceinline__ void block_scan2(uint32_t& a, uint32_t& b, uint32_t& ta,
+                                           uint32_t& tb) {
+  __shared__ uint32_t wa[16], wb[16];
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6, nw = blockDim.x >> 6;
+  uint32_t ia = a, ib = b;
+  for (int o = 1; o < 64; o <<= 1) {
+    const uint32_t xa = __shfl_up(ia, o, 64), xb = __shfl_up(ib, o, 64);
+    if (lane >= o) {
+      ia += xa;
+      ib += xb;
+    }
+  }
+  if (lane == 63) {
+    wa[w] = ia;
+    wb[w] = ib;
+  }
+  __syncthreads();
+  uint32_t pa = 0, pb = 0;
+  ta = 0;
+  tb = 0;
+  for (int k = 0; k < nw; k++) {
+    if (k < w) {
+      pa += wa[k];
+      pb += wb[k];
+    }
+    ta += wa[k];
+    tb += wb[k];
+  }
+  a = pa + ia - a;
+  b = pb + ib - b;
+}
+
+// One block, after the hist pass: turns the counts into scatter positions and work items,
+// and clears the counts for the next step.  (A separate launch: a "last block scans"
+// hand-off inside the hist pass would need a device-scope fence per block, which writes
+// back the L2.)
+__device__ __forceinline__ void tile2d_scan_body(const Tile2D& tg) {
+  const int per = (tg.ntiles + blockDim.x - 1) / blockDim.x;
+  const int lo = min(tg.ntiles, (int)threadIdx.x * per), hi = min(tg.ntiles, lo + per);
+  uint32_t s = 0, c = 0;
+  for (int t = lo; t < hi; t++) {
+    const uint32_t k = tg.counts[t];
+    s += k;
+    c += (k + tg.item - 1) / tg.item;
+  }
+  uint32_t ts, tc;
+  block_scan2(s, c, ts, tc);
+  for (int t = lo; t < hi; t++) {
+    const uint32_t k = tg.counts[t];
+    tg.cursor[t] = s;
+    for (uint32_t c0 = 0; c0 < k; c0 += tg.item)
+      tg.items[c++] = make_int4(t, (int)(s + c0), (int)(s + min(k, c0 + (uint32_t)tg.item)), 0);
+    s += k;
+    tg.counts[t] = 0u;
+  }
+  if (threadIdx.x == 0) *tg.nitems = tc;
+}
+
+// LDS of the hist pass: nz doubles (z line density), then ntiles counters.
+__device__ __forceinline__ void sc2d_hist_body(const SC2DParams& P, const DeviceBunchView& v,
+                                               const SC2DState* st, const Tile2D& tg,
+                                               double* __restrict__ zg, int64_t first,
+                                               int64_t end, int64_t stride, double* zl,
+                                               uint32_t* lh) {
   const int nz = P.nz;
   for (int i = threadIdx.x; i < nz; i += blockDim.x) zl[i] = 0.0;
+  for (int t = threadIdx.x; t < tg.ntiles; t += blockDim.x) lh[t] = 0u;
   __syncthreads();
   if (!st->skip) {
     const double xMin = st->xMin, xMax = st->xMax, yMin = st->yMin, yMax = st->yMax;
     const double dx = st->dx, dy = st->dy, zMin = st->zMin, zMax = st->zMax, dz = st->dz;
-    const int nx = P.nx, ny = P.ny;
-    for (int64_t i = first; i < end; i += stride) {
-      const double x = v.c[0][i], y = v.c[2][i], z = v.c[4][i];
-      const double m = v.msize ? v.msize[i] : P.macrosize;
-      // Grid2D::binValue (Grid2D.cc:142-166)
-      if (!(x < xMin || x > xMax || y < yMin || y > yMax)) {
-        int iX, iY;
-        double xF, yF;
-        node_ind_frac(x, xMin, dx, nx, iX, xF);
-        node_ind_frac(y, yMin, dy, ny, iY, yF);
-        const double xF2 = xF * xF, yF2 = yF * yF;
-        const double wx[3] = {0.5 * (0.25 - xF + xF2), 0.75 - xF2, 0.5 * (0.25 + xF + xF2)};
-        const double wy[3] = {0.5 * (0.25 - yF + yF2), 0.75 - yF2, 0.5 * (0.25 + yF + yF2)};
-        double* base = rho + (int64_t)(iX - 1) * ny + (iY - 1);
+    const double* __restrict__ px = v.c[0];
+    const double* __restrict__ py = v.c[2];
+    const double* __restrict__ pz = v.c[4];
+    const double* __restrict__ pm = v.msize;
+    // kUnroll particles per thread per pass: their loads are all in flight together
+    for (int64_t i0 = first; i0 < end; i0 += kUnroll * stride) {
+      double x[kUnroll], y[kUnroll], z[kUnroll], m[kUnroll];
 #pragma unroll
-        for (int a = 0; a < 3; a++)
-#pragma unroll
-          for (int c = 0; c < 3; c++) atomicAdd(base + a * ny + c, wx[a] * wy[c] * m);
+      for (int u = 0; u < kUnroll; u++) {
+        const int64_t i = i0 + u * stride;
+        const bool ok = i < end;
+        x[u] = ok ? px[i] : 0.0;
+        y[u] = ok ? py[i] : 0.0;
+        z[u] = ok ? pz[i] : 0.0;
+        m[u] = pm ? (ok ? pm[i] : 0.0) : P.macrosize;
       }
-      // Grid1D::binValue (Grid1D.cc:422-440)
-      if (!(z < zMin || z > zMax)) {
-        int i0, ip;
-        double w0, wp;
-        grid1d_ind_wz(z, zMin, dz, nz, i0, ip, w0, wp);
-        if (nz > 1) {
-          atomicAdd(&zl[i0], w0 * m);
-          atomicAdd(&zl[ip], wp * m);
-        } else {
-          atomicAdd(&zl[0], m);
+#pragma unroll
+      for (int u = 0; u < kUnroll; u++) {
+        if (i0 + u * stride >= end) break;
+        const int k = tile2d_key(tg, P.nx, P.ny, xMin, xMax, yMin, yMax, dx, dy, x[u], y[u]);
+        if (k >= 0) atomicAdd(&lh[k], 1u);
+        // Grid1D::binValue (Grid1D.cc:422-440)
+        if (!(z[u] < zMin || z[u] > zMax)) {
+          int i0z, ip;
+          double w0, wp;
+          grid1d_ind_wz(z[u], zMin, dz, nz, i0z, ip, w0, wp);
+          if (nz > 1) {
+            atomicAdd(&zl[i0z], w0 * m[u]);
+            atomicAdd(&zl[ip], wp * m[u]);
+          } else {
+            atomicAdd(&zl[0], m[u]);
+          }
         }
       }
     }
@@ -175,14 +262,153 @@ __device__ __forceinline__ void sc2d_deposit_body(const SC2DParams& P, const Dev
   __syncthreads();
   for (int i = threadIdx.x; i < nz; i += blockDim.x)
     if (zl[i] != 0.0) atomicAdd(&zg[i], zl[i]);
+  for (int t = threadIdx.x; t < tg.ntiles; t += blockDim.x)
+    if (lh[t]) atomicAdd(&tg.counts[t], lh[t]);
 }
 
-__global__ __launch_bounds__(kDepositThreads) void sc2d_deposit(SC2DParams P, DeviceBunchView v,
-                                                                const SC2DState* st,
-                                                                double* __restrict__ rho,
-                                                                double* __restrict__ zg) {
-  sc2d_deposit_body(P, v, st, rho, zg, (int64_t)blockIdx.x * blockDim.x + threadIdx.x, v.n,
-                    (int64_t)gridDim.x * blockDim.x);
+// Particles [lo, hi) of one bunch; LDS: ntiles counters.
+__device__ __forceinline__ void sc2d_scatter_body(const SC2DParams& P, const DeviceBunchView& v,
+                                                  const SC2DState* st, const Tile2D& tg,
+                                                  int64_t lo, int64_t hi, uint32_t* lc) {
+  for (int t = threadIdx.x; t < tg.ntiles; t += blockDim.x) lc[t] = 0u;
+  __syncthreads();
+  const bool run = !st->skip;
+  const double xMin = st->xMin, xMax = st->xMax, yMin = st->yMin, yMax = st->yMax;
+  const double dx = st->dx, dy = st->dy;
+  const double* __restrict__ px = v.c[0];
+  const double* __restrict__ py = v.c[2];
+  const int64_t stride = blockDim.x;
+  if (run)
+    for (int64_t i0 = lo + threadIdx.x; i0 < hi; i0 += kUnroll * stride) {
+      double x[kUnroll], y[kUnroll];
+#pragma unroll
+      for (int u = 0; u < kUnroll; u++) {
+        const int64_t i = i0 + u * stride;
+        x[u] = i < hi ? px[i] : 0.0;
+        y[u] = i < hi ? py[i] : 0.0;
+      }
+#pragma unroll
+      for (int u = 0; u < kUnroll; u++) {
+        if (i0 + u * stride >= hi) break;
+        const int k = tile2d_key(tg, P.nx, P.ny, xMin, xMax, yMin, yMax, dx, dy, x[u], y[u]);
+        if (k >= 0) atomicAdd(&lc[k], 1u);
+      }
+    }
+  __syncthreads();
+  for (int t = threadIdx.x; t < tg.ntiles; t += blockDim.x) {
+    const uint32_t c = lc[t];
+    lc[t] = c ? atomicAdd(&tg.cursor[t], c) : 0u;
+  }
+  __syncthreads();
+  if (run)
+    for (int64_t i0 = lo + threadIdx.x; i0 < hi; i0 += kUnroll * stride) {
+      double x[kUnroll], y[kUnroll], m[kUnroll];
+#pragma unroll
+      for (int u = 0; u < kUnroll; u++) {
+        const int64_t i = i0 + u * stride;
+        x[u] = i < hi ? px[i] : 0.0;
+        y[u] = i < hi ? py[i] : 0.0;
+        m[u] = (v.msize && i < hi) ? v.msize[i] : 0.0;
+      }
+#pragma unroll
+      for (int u = 0; u < kUnroll; u++) {
+        if (i0 + u * stride >= hi) break;
+        const int k = tile2d_key(tg, P.nx, P.ny, xMin, xMax, yMin, yMax, dx, dy, x[u], y[u]);
+        if (k < 0) continue;
+        const uint32_t d = atomicAdd(&lc[k], 1u);
+        tg.sx[d] = x[u];
+        tg.sy[d] = y[u];
+        if (v.msize) tg.sm[d] = m[u];
+      }
+    }
+}
+
+// Grid2D::binValue (Grid2D.cc:142-166) of the particles of work item `it` (tile it.x,
+// scratch range [it.y, it.z)) into an LDS block of the tile plus halo, then into rho.
+__device__ __forceinline__ void sc2d_tdeposit_body(const SC2DParams& P, const SC2DState* st,
+                                                   const Tile2D& tg, int4 it, bool per_particle_m,
+                                                   double* __restrict__ rho, double* acc) {
+  const int W = tg.T + 2, cells = W * W;
+  const int x0 = (it.x / tg.ty) * tg.T - 1, y0 = (it.x % tg.ty) * tg.T - 1;
+  for (int c = threadIdx.x; c < cells; c += blockDim.x) acc[c] = 0.0;
+  __syncthreads();
+  const double xMin = st->xMin, yMin = st->yMin, dx = st->dx, dy = st->dy;
+  const int nx = P.nx, ny = P.ny;
+  const double* __restrict__ sx = tg.sx;
+  const double* __restrict__ sy = tg.sy;
+  const double* __restrict__ sm = tg.sm;
+  const int stride = blockDim.x;
+  for (int i0 = it.y + threadIdx.x; i0 < it.z; i0 += kUnroll * stride) {
+    double xs[kUnroll], ys[kUnroll], ms[kUnroll];
+#pragma unroll
+    for (int u = 0; u < kUnroll; u++) {
+      const int i = i0 + u * stride;
+      xs[u] = i < it.z ? sx[i] : 0.0;
+      ys[u] = i < it.z ? sy[i] : 0.0;
+      ms[u] = per_particle_m ? (i < it.z ? sm[i] : 0.0) : P.macrosize;
+    }
+#pragma unroll
+    for (int u = 0; u < kUnroll; u++) {
+      if (i0 + u * stride >= it.z) break;
+      const double x = xs[u], y = ys[u], m = ms[u];
+      int iX, iY;
+      double xF, yF;
+      node_ind_frac(x, xMin, dx, nx, iX, xF);
+      node_ind_frac(y, yMin, dy, ny, iY, yF);
+      const double xF2 = xF * xF, yF2 = yF * yF;
+      const double wx[3] = {0.5 * (0.25 - xF + xF2), 0.75 - xF2, 0.5 * (0.25 + xF + xF2)};
+      const double wy[3] = {0.5 * (0.25 - yF + yF2), 0.75 - yF2, 0.5 * (0.25 + yF + yF2)};
+      const int lx = iX - 1 - x0, ly = iY - 1 - y0;
+      if (lx >= 0 && lx + 2 < W && ly >= 0 && ly + 2 < W) {
+        double* base = acc + lx * W + ly;
+#pragma unroll
+        for (int a = 0; a < 3; a++)
+#pragma unroll
+          for (int c = 0; c < 3; c++) atomicAdd(base + a * W + c, wx[a] * wy[c] * m);
+      } else {  // not in this tile's block (cannot happen for a consistent sort): global
+        double* base = rho + (int64_t)(iX - 1) * ny + (iY - 1);
+        for (int a = 0; a < 3; a++)
+          for (int c = 0; c < 3; c++) atomicAdd(base + a * ny + c, wx[a] * wy[c] * m);
+      }
+    }
+  }
+  __syncthreads();
+  for (int c = threadIdx.x; c < cells; c += blockDim.x) {
+    const double val = acc[c];
+    if (val == 0.0) continue;
+    const int gx = x0 + c / W, gy = y0 + c % W;
+    if (gx < 0 || gx >= nx || gy < 0 || gy >= ny) continue;
+    atomicAdd(&rho[(int64_t)gx * ny + gy], val);
+  }
+}
+
+__global__ __launch_bounds__(kDepositThreads) void sc2d_hist(SC2DParams P, DeviceBunchView v,
+                                                             const SC2DState* st, Tile2D tg,
+                                                             double* zg) {
+  extern __shared__ double dyn[];
+  sc2d_hist_body(P, v, st, tg, zg, (int64_t)blockIdx.x * blockDim.x + threadIdx.x, v.n,
+                 (int64_t)gridDim.x * blockDim.x, dyn, (uint32_t*)(dyn + P.nz));
+}
+
+__global__ __launch_bounds__(kDepositThreads) void sc2d_tile_scan(Tile2D tg) {
+  tile2d_scan_body(tg);
+}
+
+__global__ __launch_bounds__(kDepositThreads) void sc2d_scatter(SC2DParams P, DeviceBunchView v,
+                                                                const SC2DState* st, Tile2D tg,
+                                                                int64_t per_block) {
+  extern __shared__ uint32_t lc[];
+  const int64_t lo = (int64_t)blockIdx.x * per_block;
+  sc2d_scatter_body(P, v, st, tg, lo, min(v.n, lo + per_block), lc);
+}
+
+__global__ __launch_bounds__(kDepositThreads) void sc2d_tdeposit(SC2DParams P,
+                                                                 const SC2DState* st, Tile2D tg,
+                                                                 int per_particle_m,
+                                                                 double* __restrict__ rho) {
+  if (blockIdx.x >= *tg.nitems) return;
+  extern __shared__ double acc[];
+  sc2d_tdeposit_body(P, st, tg, tg.items[blockIdx.x], per_particle_m != 0, rho, acc);
 }
 
 // ---- 4: post (one block) ---------------------------------------------------
@@ -480,25 +706,50 @@ __global__ __launch_bounds__(kSetupThreads) void sc2d_setup_batch(const BatchBun
   sc2d_setup_body(P, partials, B.item_begin, B.item_end, B.st, B.rho, B.zg, B.zd);
 }
 
-// One block per work item.  Bunches whose grid fits (deposit_lds) accumulate their item's
-// rho in LDS and add it to the grid with one atomic per touched node.
-__global__ __launch_bounds__(kDepositThreads) void sc2d_deposit_batch(const BatchBunch* bb,
-                                                                      const BatchItem* items) {
-  extern __shared__ double lrho[];
+// Tile-sorted deposit of every bunch: hist and scatter take one block per particle work
+// item; the last hist block of a bunch scans that bunch's counts.
+__global__ __launch_bounds__(kDepositThreads) void sc2d_hist_batch(const BatchBunch* bb,
+                                                                   const BatchItem* items) {
+  extern __shared__ double dyn[];
   const BatchItem it = items[blockIdx.x];
   const BatchBunch& B = bb[it.bunch];
   const SC2DParams P = batch_params(B, 0.0, 1.0);
-  if (B.deposit_lds) {
-    const int ng = B.nx * B.ny;
-    for (int i = threadIdx.x; i < ng; i += blockDim.x) lrho[i] = 0.0;
-    __syncthreads();
-    sc2d_deposit_body(P, B.v, B.st, lrho, B.zg, it.begin + threadIdx.x, it.end, blockDim.x);
-    __syncthreads();
-    for (int i = threadIdx.x; i < ng; i += blockDim.x)
-      if (lrho[i] != 0.0) atomicAdd(&B.rho[i], lrho[i]);
-  } else {
-    sc2d_deposit_body(P, B.v, B.st, B.rho, B.zg, it.begin + threadIdx.x, it.end, blockDim.x);
+  sc2d_hist_body(P, B.v, B.st, B.tg, B.zg, it.begin + threadIdx.x, it.end, blockDim.x, dyn,
+                 (uint32_t*)(dyn + P.nz));
+}
+
+__global__ __launch_bounds__(kDepositThreads) void sc2d_tile_scan_batch(const BatchBunch* bb) {
+  tile2d_scan_body(bb[blockIdx.x].tg);
+}
+
+__global__ __launch_bounds__(kDepositThreads) void sc2d_scatter_batch(const BatchBunch* bb,
+                                                                      const BatchItem* items) {
+  extern __shared__ uint32_t lc[];
+  const BatchItem it = items[blockIdx.x];
+  const BatchBunch& B = bb[it.bunch];
+  const SC2DParams P = batch_params(B, 0.0, 1.0);
+  sc2d_scatter_body(P, B.v, B.st, B.tg, it.begin, it.end, lc);
+}
+
+// Bunch whose deposit block range [slot_begin, slot_end) holds block blk.
+__device__ __forceinline__ int batch_find_slot(const BatchBunch* bb, int nb, int blk) {
+  int lo = 0, hi = nb - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (bb[mid].slot_begin <= blk) lo = mid; else hi = mid - 1;
   }
+  return lo;
+}
+
+__global__ __launch_bounds__(kDepositThreads) void sc2d_tdeposit_batch(const BatchBunch* bb,
+                                                                       int nb) {
+  extern __shared__ double acc[];
+  const int b = batch_find_slot(bb, nb, blockIdx.x);
+  const BatchBunch& B = bb[b];
+  const int j = blockIdx.x - B.slot_begin;
+  if (j >= (int)*B.tg.nitems) return;
+  const SC2DParams P = batch_params(B, 0.0, 1.0);
+  sc2d_tdeposit_body(P, B.st, B.tg, B.tg.items[j], B.v.msize != nullptr, B.rho, acc);
 }
 
 __global__ __launch_bounds__(kSetupThreads) void sc2d_post_batch(const BatchBunch* bb,
@@ -578,6 +829,11 @@ struct ob_sc2p5drb {
   double* partials = nullptr;
   ob::SC2DState* st = nullptr;
   bool tracked = false;
+  // tile-sorted deposit
+  ob::Tile2D tg{};
+  int64_t tile_ncap = 0;  // scratch particles allocated
+  bool tile_msize = false;
+  int tile_item = 2048;   // particles per deposit work item
 };
 
 namespace ob {
@@ -690,6 +946,48 @@ void sc2d_batch_record(ob_sc2p5drb* c, BatchBunch& r) {
   r.dx_solver = c->dx_solver;
 }
 
+Tile2D sc2d_tile_reserve(ob_sc2p5drb* c, int64_t n, bool per_particle_m, int item) {
+  Tile2D& g = c->tg;
+  if (g.T == 0) {
+    int T = 16;
+    while ((int64_t)((c->nx + T - 1) / T) * ((c->ny + T - 1) / T) > 4096) T *= 2;
+    g.T = T;
+    g.shift = __builtin_ctz((unsigned)T);
+    g.tx = (c->nx + T - 1) / T;
+    g.ty = (c->ny + T - 1) / T;
+    g.ntiles = g.tx * g.ty;
+    OB_HIP(hipMalloc((void**)&g.counts, sizeof(uint32_t) * (2 * g.ntiles + 1)));
+    g.cursor = g.counts + g.ntiles;
+    g.nitems = g.cursor + g.ntiles;
+    OB_HIP(hipMemset(g.counts, 0, sizeof(uint32_t) * (2 * g.ntiles + 1)));
+  }
+  const int64_t want_items = n / item + g.ntiles + 1;
+  OB_REQUIRE(want_items < (int64_t)1 << 31, "tile sort: too many work items");
+  if (n > c->tile_ncap || (per_particle_m && !c->tile_msize) || item != g.item ||
+      want_items > g.max_items) {
+    OB_HIP(hipStreamSynchronize(stream()));  // buffers may be in use by queued work
+    for (void* p : {(void*)g.sx, (void*)g.sy, (void*)g.sm, (void*)g.items})
+      if (p) OB_HIP(hipFree(p));
+    g.sx = g.sy = g.sm = nullptr;
+    int64_t cap = std::max<int64_t>(n, std::max<int64_t>(c->tile_ncap, 1024));
+    OB_HIP(hipMalloc((void**)&g.sx, sizeof(double) * cap));
+    OB_HIP(hipMalloc((void**)&g.sy, sizeof(double) * cap));
+    c->tile_msize = per_particle_m || c->tile_msize;
+    if (c->tile_msize) OB_HIP(hipMalloc((void**)&g.sm, sizeof(double) * cap));
+    c->tile_ncap = cap;
+    g.item = item;
+    g.max_items = (int)(cap / item + g.ntiles + 1);
+    OB_HIP(hipMalloc((void**)&g.items, sizeof(int4) * g.max_items));
+  }
+  return g;
+}
+
+bool sc2d_tile_same(ob_sc2p5drb* c, const Tile2D& t, int64_t n, bool per_particle_m) {
+  const Tile2D& g = c->tg;
+  return g.sx == t.sx && g.items == t.items && g.item == t.item && n <= c->tile_ncap &&
+         (!per_particle_m || g.sm != nullptr);
+}
+
 void sc2d_mark_tracked(ob_sc2p5drb* c) { c->tracked = true; }
 
 int sc2d_read_status(ob_sc2p5drb* c) {
@@ -701,13 +999,8 @@ int sc2d_read_status(ob_sc2p5drb* c) {
 void launch_sc2d_batch(const BatchBunch* bb, const BatchItem* items, const BatchShape& sh,
                        const double* partials, double length, double pipe_radius,
                        hipStream_t s) {
-  static bool ready = false;
-  if (!ready) {
-    OB_HIP(hipFuncSetAttribute((const void*)sc2d_deposit_batch,
-                               hipFuncAttributeMaxDynamicSharedMemorySize, 128 * 1024));
-    ready = true;
-  }
-  OB_REQUIRE(sh.deposit_lds_bytes <= 128 * 1024, "batch deposit: LDS block too large");
+  OB_REQUIRE(sh.hist_lds <= 64 * 1024 && sh.dep_lds <= 64 * 1024,
+             "batch deposit: LDS block too large");
   OB_REQUIRE(sh.max_M <= 1024, "batch: grids up to 512 points per axis");
   {
     ProfScope ps("sc2d_setup_batch");
@@ -715,9 +1008,23 @@ void launch_sc2d_batch(const BatchBunch* bb, const BatchItem* items, const Batch
                        length, pipe_radius);
   }
   {
-    ProfScope ps("sc2d_deposit_batch");
-    hipLaunchKernelGGL(sc2d_deposit_batch, dim3(sh.nitems), dim3(kDepositThreads),
-                       sh.deposit_lds_bytes, s, bb, items);
+    ProfScope ps("sc2d_hist_batch");
+    hipLaunchKernelGGL(sc2d_hist_batch, dim3(sh.nitems), dim3(kDepositThreads), sh.hist_lds, s,
+                       bb, items);
+  }
+  {
+    ProfScope ps("sc2d_tile_scan_batch");
+    hipLaunchKernelGGL(sc2d_tile_scan_batch, dim3(sh.nb), dim3(kDepositThreads), 0, s, bb);
+  }
+  {
+    ProfScope ps("sc2d_scatter_batch");
+    hipLaunchKernelGGL(sc2d_scatter_batch, dim3(sh.nitems), dim3(kDepositThreads), sh.hist_lds,
+                       s, bb, items);
+  }
+  {
+    ProfScope ps("sc2d_tdeposit_batch");
+    hipLaunchKernelGGL(sc2d_tdeposit_batch, dim3(sh.dep_slots), dim3(kDepositThreads),
+                       sh.dep_lds, s, bb, sh.nb);
   }
   {
     ProfScope ps("sc2d_post_batch");
@@ -811,6 +1118,9 @@ OB_API int ob_sc2p5drb_destroy(ob_sc2p5drb* c) {
   (void)hipStreamSynchronize(ob::stream());
   (void)hipFree(c->rho); (void)hipFree(c->phi); (void)hipFree(c->zg); (void)hipFree(c->zd);
   (void)hipFree(c->spec); (void)hipFree(c->ghat_t); (void)hipFree(c->tw); (void)hipFree(c->partials); (void)hipFree(c->st);
+  for (void* p : {(void*)c->tg.counts, (void*)c->tg.items, (void*)c->tg.sx, (void*)c->tg.sy,
+                  (void*)c->tg.sm})
+    if (p) (void)hipFree(p);
   delete c;
   OB_REQUIRE(err.empty(), err);
   OB_CATCH
@@ -838,13 +1148,31 @@ OB_API int ob_sc2p5drb_track(ob_sc2p5drb* c, ob_bunch* b, double length, double 
     hipLaunchKernelGGL(ob::sc2d_setup, dim3(1), dim3(ob::kSetupThreads), 0, s, P, c->partials,
                        c->st, c->rho, c->zg, c->zd);
   }
-  int dblk = (int)((n + ob::kDepositThreads - 1) / ob::kDepositThreads);
-  const int cap = 8 * ob::device_cu_count();
-  if (dblk > cap) dblk = cap;
+  const ob::Tile2D tg = ob::sc2d_tile_reserve(c, n, v.msize != nullptr, c->tile_item);
   {
-    ob::ProfScope ps_("sc2d_deposit");
-    hipLaunchKernelGGL(ob::sc2d_deposit, dim3(dblk), dim3(ob::kDepositThreads), 0, s, P, v, c->st,
-                       c->rho, c->zg);
+    ob::ProfScope ps_("sc2d_hist");
+    const int hb = (int)std::max<int64_t>(
+        1, std::min<int64_t>((n + 2047) / 2048, 4 * ob::device_cu_count()));
+    hipLaunchKernelGGL(ob::sc2d_hist, dim3(hb), dim3(ob::kDepositThreads),
+                       sizeof(double) * c->nz + sizeof(uint32_t) * tg.ntiles, s, P, v, c->st, tg,
+                       c->zg);
+  }
+  {
+    ob::ProfScope ps_("sc2d_tile_scan");
+    hipLaunchKernelGGL(ob::sc2d_tile_scan, dim3(1), dim3(ob::kDepositThreads), 0, s, tg);
+  }
+  {
+    ob::ProfScope ps_("sc2d_scatter");
+    const int64_t per = std::max<int64_t>(2048, (n + 1023) / 1024);
+    hipLaunchKernelGGL(ob::sc2d_scatter, dim3((unsigned)((n + per - 1) / per)),
+                       dim3(ob::kDepositThreads), sizeof(uint32_t) * tg.ntiles, s, P, v, c->st,
+                       tg, per);
+  }
+  {
+    ob::ProfScope ps_("sc2d_tdeposit");
+    hipLaunchKernelGGL(ob::sc2d_tdeposit, dim3((unsigned)(n / tg.item + tg.ntiles + 1)),
+                       dim3(ob::kDepositThreads), sizeof(double) * (tg.T + 2) * (tg.T + 2), s, P,
+                       c->st, tg, v.msize != nullptr ? 1 : 0, c->rho);
   }
   {
     ob::ProfScope ps_("sc2d_post");

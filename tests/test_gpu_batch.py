@@ -82,12 +82,12 @@ def test_batch_graph_replay():
 def test_batch_results_do_not_depend_on_tuning():
     from batch import BunchBatch
     outs = []
-    for chunk, lds in [(4096, 0), (65536, 128 * 1024)]:
+    for chunk, item in [(4096, 256), (65536, 8192)]:
         bunches, calcs = _setup()
         bb = BunchBatch(bunches, calcs)
-        bb.setTuning(chunk, lds)
+        bb.setTuning(chunk, item)
         _lattice(bb.calculator(), 1).trackBunch(bb.bunch())
-        assert bb.tuning() == {"chunk": chunk, "lds_max_bytes": lds, "tuned": True}
+        assert bb.tuning() == {"chunk": chunk, "deposit_item": item, "tuned": True}
         outs.append([b.getCoordinates() for b in bunches])
     for k in range(len(SIZES)):
         for j in range(6):
