@@ -37,8 +37,8 @@ TRAFFIC_FILES = {"ring3d": "profiles/ring3d_hbm_traffic.md",
                  "fodo2p5d": "profiles/fodo2p5d_hbm_traffic.md"}
 # profiler label -> kernel name (prefix) in the rocprofv3 summaries
 ROCPROF_NAME = {"fft3_strided_conv": "fft3_strided<2,", "fft3_strided_fwd": "fft3_strided<0,",
-                "fft3_strided_inv": "fft3_strided<1,", "push_extrema": "push_kernel<true>",
-                "push": "push_kernel<false>", "fft_cols": "fft_cols<false,",
+                "fft3_strided_inv": "fft3_strided<1,", "push_extrema": "push_kernel<true,",
+                "push": "push_kernel<false,", "fft_cols": "fft_cols<false,",
                 "fft3_z_r2c": "fft3_z_r2c<", "fft3_z_c2r": "fft3_z_c2r<"}
 
 

@@ -102,8 +102,14 @@ __device__ __forceinline__ void quadfringe(const Op& op, double& x, double& xp, 
   }
 }
 
+// FULL = false leaves out the register-hungry multipole fringe and 3D kick, so programs
+// without them run with about half the VGPRs (twice the waves per SIMD).
+template <bool FULL>
 __device__ __forceinline__ void apply_op(const Op& op, double& x, double& xp, double& y,
                                          double& yp, double& z, double& dE) {
+  if (!FULL && (op.kind == OP_MULTPFRINGE_IN || op.kind == OP_MULTPFRINGE_OUT ||
+                op.kind == OP_SC3D_KICK))
+    return;  // never reached: the host launches FULL for such programs
   switch (op.kind) {
     case OP_DRIFT: {  // teapotbase.cc:158-183. p: length, gamma2i, dp_p_coeff
       const double length = op.p[0], gamma2i = op.p[1];
@@ -160,7 +166,7 @@ __device__ __forceinline__ void apply_op(const Op& op, double& x, double& xp, do
     }
     case OP_MULTPFRINGE_IN:
     case OP_MULTPFRINGE_OUT:
-      multpfringe(op, x, xp, y, yp, z, dE);
+      if constexpr (FULL) multpfringe(op, x, xp, y, yp, z, dE);
       break;
     case OP_WRAPBUNCH: {  // teapotbase.cc:202-211. p[0]=length
       const double length = op.p[0];
@@ -186,11 +192,20 @@ __device__ __forceinline__ void apply_op(const Op& op, double& x, double& xp, do
       sc2p5drb_kick(op, x, xp, y, yp, z, dE);
       break;
     case OP_SC3D_KICK:
-      sc3d_kick(op, x, xp, y, yp, z, dE);
+      if constexpr (FULL) sc3d_kick(op, x, xp, y, yp, z, dE);
       break;
     default:
       break;
   }
+}
+
+// Whether a program holds an op that only the FULL kernels implement.
+bool program_needs_full(const OpProgram& prog) {
+  for (int k = 0; k < prog.nops; k++) {
+    const int kd = prog.ops[k].kind;
+    if (kd == OP_MULTPFRINGE_IN || kd == OP_MULTPFRINGE_OUT || kd == OP_SC3D_KICK) return true;
+  }
+  return false;
 }
 
 constexpr int kPushThreads = 256;
@@ -205,7 +220,19 @@ __device__ __forceinline__ double wave_max(double v) {
   return v;
 }
 
-template <bool EXTREMA>
+__device__ __forceinline__ void load6(const double* __restrict__ a, const double* __restrict__ b,
+                                      const double* __restrict__ c, const double* __restrict__ d,
+                                      const double* __restrict__ e, const double* __restrict__ f,
+                                      int64_t i, double q[6]) {
+  q[0] = a[i];
+  q[1] = b[i];
+  q[2] = c[i];
+  q[3] = d[i];
+  q[4] = e[i];
+  q[5] = f[i];
+}
+
+template <bool EXTREMA, bool FULL>
 __global__ __launch_bounds__(kPushThreads) void push_kernel(DeviceBunchView v, OpProgram prog,
                                                             double* __restrict__ partials) {
   double lo[3] = {DBL_MAX, DBL_MAX, DBL_MAX};
@@ -217,9 +244,15 @@ __global__ __launch_bounds__(kPushThreads) void push_kernel(DeviceBunchView v, O
   double* __restrict__ cyp = v.c[3];
   double* __restrict__ cz = v.c[4];
   double* __restrict__ cdE = v.c[5];
-  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < v.n; i += stride) {
-    double x = cx[i], xp = cxp[i], y = cy[i], yp = cyp[i], z = cz[i], dE = cdE[i];
-    for (int k = 0; k < prog.nops; k++) apply_op(prog.ops[k], x, xp, y, yp, z, dE);
+  // The next particle's coordinates are loaded before this one is computed, so their
+  // latency overlaps the map arithmetic (few waves fit: the map switch needs many VGPRs).
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  double q[6];
+  if (i < v.n) load6(cx, cxp, cy, cyp, cz, cdE, i, q);
+  for (; i < v.n; i += stride) {
+    double x = q[0], xp = q[1], y = q[2], yp = q[3], z = q[4], dE = q[5];
+    if (i + stride < v.n) load6(cx, cxp, cy, cyp, cz, cdE, i + stride, q);
+    for (int k = 0; k < prog.nops; k++) apply_op<FULL>(prog.ops[k], x, xp, y, yp, z, dE);
     cx[i] = x;
     cxp[i] = xp;
     cy[i] = y;
@@ -260,7 +293,7 @@ __global__ __launch_bounds__(kPushThreads) void push_kernel(DeviceBunchView v, O
 // bunch).  The bunch's deferred 2.5D kick comes first (its op is assembled from the
 // bunch's record and the batch's common factors), then the map program; extrema are
 // reduced per work item.
-template <bool EXTREMA>
+template <bool EXTREMA, bool FULL>
 __global__ __launch_bounds__(kPushThreads) void push_batch_kernel(
     const BatchBunch* __restrict__ bb, const BatchItem* __restrict__ items, OpProgram prog,
     BatchKick kick, double* __restrict__ partials) {
@@ -285,10 +318,14 @@ __global__ __launch_bounds__(kPushThreads) void push_batch_kernel(
   double* __restrict__ cyp = B.v.c[3];
   double* __restrict__ cz = B.v.c[4];
   double* __restrict__ cdE = B.v.c[5];
-  for (int64_t i = it.begin + threadIdx.x; i < it.end; i += kPushThreads) {
-    double x = cx[i], xp = cxp[i], y = cy[i], yp = cyp[i], z = cz[i], dE = cdE[i];
+  int64_t i = it.begin + threadIdx.x;
+  double q[6];
+  if (i < it.end) load6(cx, cxp, cy, cyp, cz, cdE, i, q);
+  for (; i < it.end; i += kPushThreads) {
+    double x = q[0], xp = q[1], y = q[2], yp = q[3], z = q[4], dE = q[5];
+    if (i + kPushThreads < it.end) load6(cx, cxp, cy, cyp, cz, cdE, i + kPushThreads, q);
     if (kick.active) sc2p5drb_kick(kop, x, xp, y, yp, z, dE);
-    for (int k = 0; k < prog.nops; k++) apply_op(prog.ops[k], x, xp, y, yp, z, dE);
+    for (int k = 0; k < prog.nops; k++) apply_op<FULL>(prog.ops[k], x, xp, y, yp, z, dE);
     cx[i] = x;
     cxp[i] = xp;
     cy[i] = y;
@@ -332,16 +369,25 @@ void launch_push_batch(const BatchBunch* bb, const BatchItem* items, int nitems,
                        int threads, hipStream_t s) {
   OB_REQUIRE(threads == kPushThreads, "batch push: unsupported block size");
   if (nitems == 0) return;
+  const bool full = program_needs_full(prog);
   if (prog.want_extrema) {
     OB_REQUIRE(partials != nullptr, "batch push: extrema requested without a buffer");
     ProfScope ps("push_batch_extrema");
-    hipLaunchKernelGGL(push_batch_kernel<true>, dim3(nitems), dim3(kPushThreads), 0, s, bb, items,
-                       prog, kick, partials);
+    if (full)
+      hipLaunchKernelGGL((push_batch_kernel<true, true>), dim3(nitems), dim3(kPushThreads), 0, s,
+                         bb, items, prog, kick, partials);
+    else
+      hipLaunchKernelGGL((push_batch_kernel<true, false>), dim3(nitems), dim3(kPushThreads), 0,
+                         s, bb, items, prog, kick, partials);
   } else {
     if (prog.nops == 0 && !kick.active) return;
     ProfScope ps("push_batch");
-    hipLaunchKernelGGL(push_batch_kernel<false>, dim3(nitems), dim3(kPushThreads), 0, s, bb,
-                       items, prog, kick, nullptr);
+    if (full)
+      hipLaunchKernelGGL((push_batch_kernel<false, true>), dim3(nitems), dim3(kPushThreads), 0,
+                         s, bb, items, prog, kick, nullptr);
+    else
+      hipLaunchKernelGGL((push_batch_kernel<false, false>), dim3(nitems), dim3(kPushThreads), 0,
+                         s, bb, items, prog, kick, nullptr);
   }
   OB_HIP(hipGetLastError());
 }
@@ -356,16 +402,25 @@ int push_grid_blocks(int64_t n) {
 int launch_push(const DeviceBunchView& v, const OpProgram& prog, double* ext_partials,
                 hipStream_t s) {
   const int blocks = push_grid_blocks(v.n);
+  const bool full = program_needs_full(prog);
   if (prog.want_extrema) {
     OB_REQUIRE(ext_partials != nullptr, "push: extrema requested without a buffer");
     ProfScope ps("push_extrema");
-    hipLaunchKernelGGL(push_kernel<true>, dim3(blocks), dim3(kPushThreads), 0, s, v, prog,
-                       ext_partials);
+    if (full)
+      hipLaunchKernelGGL((push_kernel<true, true>), dim3(blocks), dim3(kPushThreads), 0, s, v,
+                         prog, ext_partials);
+    else
+      hipLaunchKernelGGL((push_kernel<true, false>), dim3(blocks), dim3(kPushThreads), 0, s, v,
+                         prog, ext_partials);
   } else {
     if (prog.nops == 0 || v.n == 0) return 0;
     ProfScope ps("push");
-    hipLaunchKernelGGL(push_kernel<false>, dim3(blocks), dim3(kPushThreads), 0, s, v, prog,
-                       nullptr);
+    if (full)
+      hipLaunchKernelGGL((push_kernel<false, true>), dim3(blocks), dim3(kPushThreads), 0, s, v,
+                         prog, nullptr);
+    else
+      hipLaunchKernelGGL((push_kernel<false, false>), dim3(blocks), dim3(kPushThreads), 0, s, v,
+                         prog, nullptr);
   }
   OB_HIP(hipGetLastError());
   return blocks;
