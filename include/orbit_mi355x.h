@@ -195,6 +195,42 @@ int ob_batch_capture_begin(ob_batch* batch);
 int ob_batch_capture_end(ob_batch* batch);
 int ob_batch_replay(ob_batch* batch, int count);
 
+/* ---- learned denoiser of deposited charge density (no reference counterpart) ----
+ * A 3-level U-Net (16/32/64 channels, 191,841 FP32 master parameters, BF16 weights and
+ * activations, FP32 accumulation on MFMA) that maps a rho grid deposited from few
+ * macroparticles to the grid many more particles would deposit.  Images are FP32
+ * [B][H][W] normalized to mean 1; H and W multiples of 64.  Trained with Adam by the
+ * library's own forward and backward kernels. */
+typedef struct ob_denoiser ob_denoiser;
+int ob_dn_create(int H, int W, int max_batch, ob_denoiser** out);
+int ob_dn_destroy(ob_denoiser* dn);
+int ob_dn_num_params(ob_denoiser* dn, int64_t* n);
+/* Parameters as one FP32 vector: per layer W[9][Cin][Cout] then b[Cout] (layers c1a c1b
+ * c2a c2b c3a c3b c3c c3d c4a c4b c5a c5b), then the 1x1 head's 16 weights and bias.
+ * Setting them resets the optimizer. */
+int ob_dn_set_params(ob_denoiser* dn, const float* host, int64_t n);
+int ob_dn_get_params(ob_denoiser* dn, float* host, int64_t n);
+/* Inference on B images: host buffers, or device buffers (y_dev may equal x_dev). */
+int ob_dn_forward(ob_denoiser* dn, const float* x, float* y, int B);
+int ob_dn_forward_dev(ob_denoiser* dn, const float* x_dev, float* y_dev, int B);
+/* MSE loss and the gradient of every parameter for B host images and targets. */
+int ob_dn_loss_grad(ob_denoiser* dn, const float* x, const float* target, int B, float* loss,
+                    float* grad);
+/* One Adam step (lr) on B device-resident pairs; ob_dn_last_loss reads its loss. */
+int ob_dn_train_step_dev(ob_denoiser* dn, const float* x_dev, const float* target_dev, int B,
+                         float lr);
+int ob_dn_last_loss(ob_denoiser* dn, float* loss);
+/* Training pairs on the device: specs holds 5 floats per pair (family 0 Gaussian, 1 KV,
+ * 2 waterbag, 3 hollow shell; x and y scales; cos and sin of the rotation).  x_dev gets the
+ * n_low-particle TSC deposit, y_dev the n_high-particle one on the same grid (the square
+ * box around the low-N sample), both normalized to mean 1. */
+int ob_dn_make_pairs(int H, int W, int npairs, const float* specs, uint64_t seed,
+                     int64_t n_low, int64_t n_high, float* x_dev, float* y_dev);
+/* Device float buffers and copies (kind 0 host->device, 1 device->host, 2 device->device). */
+int ob_dn_alloc(int64_t nfloats, float** out);
+int ob_dn_free(float* p);
+int ob_dn_copy(float* dst, const float* src, int64_t nfloats, int kind);
+
 /* ---- kernel timing (HIP events on the library stream) ---- */
 /* Start (on != 0) or stop collecting per-kernel timings; clears earlier records. */
 int ob_profile_enable(int on);

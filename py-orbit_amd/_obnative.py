@@ -10,6 +10,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "liborbit_mi355x.so")
 
 _D = C.POINTER(C.c_double)
+_F = C.POINTER(C.c_float)
 _vp = C.c_void_p
 _i, _d, _i64 = C.c_int, C.c_double, C.c_int64
 
@@ -83,6 +84,20 @@ SIGNATURES = {
     "ob_batch_capture_begin": [_vp],
     "ob_batch_capture_end": [_vp],
     "ob_batch_replay": [_vp, _i],
+    "ob_dn_create": [_i, _i, _i, C.POINTER(_vp)],
+    "ob_dn_destroy": [_vp],
+    "ob_dn_num_params": [_vp, C.POINTER(_i64)],
+    "ob_dn_set_params": [_vp, _F, _i64],
+    "ob_dn_get_params": [_vp, _F, _i64],
+    "ob_dn_forward": [_vp, _F, _F, _i],
+    "ob_dn_forward_dev": [_vp, _vp, _vp, _i],
+    "ob_dn_loss_grad": [_vp, _F, _F, _i, _F, _F],
+    "ob_dn_train_step_dev": [_vp, _vp, _vp, _i, C.c_float],
+    "ob_dn_last_loss": [_vp, _F],
+    "ob_dn_make_pairs": [_i, _i, _i, _F, C.c_uint64, _i64, _i64, _vp, _vp],
+    "ob_dn_alloc": [_i64, C.POINTER(_vp)],
+    "ob_dn_free": [_vp],
+    "ob_dn_copy": [_vp, _vp, _i64, _i],
     "ob_profile_enable": [_i],
     "ob_profile_report": [C.c_char_p, _i],
     "ob_calibrate_copy": [_i64, _i, _D],
@@ -128,6 +143,10 @@ def call(name, *args):
 
 def dptr(a):
     return a.ctypes.data_as(_D)
+
+
+def fptr(a):
+    return a.ctypes.data_as(_F)
 
 
 def synchronize():

@@ -1,0 +1,200 @@
+"""denoiser -- learned denoiser of deposited charge density (no reference counterpart).
+
+    from denoiser import Denoiser, PairSet
+    dn = Denoiser(256, 256, max_batch=32)
+    dn.setParams(Denoiser.initParams(seed=1))
+    data = PairSet(256, 256, npairs=20000, n_low=100_000, n_high=16_000_000, seed=7)
+    dn.train(data, epochs=4, batch=32, lr=1e-3)
+    rho_clean = dn(rho_noisy)                        # [B, H, W] float32, mean-1 images
+
+The network (a 3-level U-Net, 16/32/64 channels, 191,841 parameters) runs on the device:
+BF16 weights and activations, FP32 accumulation on the matrix cores, FP32 master weights
+and Adam, all through the library's own kernels (ob_dn_* in include/orbit_mi355x.h).
+Training pairs are made on the device too: seeded particles of four distribution
+families (Gaussian, KV, waterbag, hollow shell) with random aspect ratio and rotation,
+deposited with the TSC weights of the 2.5D space-charge grid at a low and a high particle
+count on the same grid.
+"""
+import ctypes as C
+
+import numpy as np
+
+import _obnative as N
+
+# (Cin, Cout, level) of the conv3x3 layers, in parameter order
+LAYERS = [(1, 16, 0), (16, 16, 0), (16, 32, 1), (32, 32, 1), (32, 64, 2), (64, 64, 2),
+          (64, 64, 2), (64, 64, 2), (96, 32, 1), (32, 32, 1), (48, 16, 0), (16, 16, 0)]
+FAMILIES = ("gaussian", "kv", "waterbag", "hollow")
+
+
+def _f32(a):
+    return np.ascontiguousarray(a, dtype=np.float32)
+
+
+class DeviceArray:
+    """A float32 buffer in device memory owned by the library (ob_dn_alloc)."""
+
+    def __init__(self, n):
+        self.n = int(n)
+        p = C.c_void_p()
+        N.call("ob_dn_alloc", self.n, C.byref(p))
+        self.ptr = p
+
+    def upload(self, host, offset=0):
+        h = _f32(host).ravel()
+        dst = C.c_void_p(self.ptr.value + 4 * offset)
+        N.call("ob_dn_copy", dst, h.ctypes.data_as(C.c_void_p), h.size, 0)
+
+    def download(self, count=None, offset=0):
+        count = self.n - offset if count is None else count
+        h = np.empty(count, dtype=np.float32)
+        src = C.c_void_p(self.ptr.value + 4 * offset)
+        N.call("ob_dn_copy", h.ctypes.data_as(C.c_void_p), src, count, 1)
+        return h
+
+    def at(self, offset):
+        return C.c_void_p(self.ptr.value + 4 * offset)
+
+    def __del__(self):
+        p = getattr(self, "ptr", None)
+        if p is not None and p.value and N._lib is not None:
+            N._lib.ob_dn_free(p)
+            self.ptr = None
+
+
+def pair_specs(npairs, seed, families=(0, 1, 2, 3), aspect=(0.5, 2.0)):
+    """Per-pair distribution: family, x/y scales for an aspect ratio drawn log-uniformly in
+    `aspect`, rotation uniform in [0, pi).  Returns float32 [npairs, 5]."""
+    r = np.random.default_rng(seed)
+    fam = r.choice(np.asarray(families), npairs)
+    ar = np.exp(r.uniform(np.log(aspect[0]), np.log(aspect[1]), npairs))
+    th = r.uniform(0.0, np.pi, npairs)
+    return np.stack([fam, np.sqrt(ar), 1.0 / np.sqrt(ar), np.cos(th), np.sin(th)],
+                    axis=1).astype(np.float32)
+
+
+class PairSet:
+    """Training pairs resident on the device: x[k] (n_low-particle deposit) and y[k]
+    (n_high-particle deposit on the same grid), both [H][W] float32 of mean 1."""
+
+    def __init__(self, H, W, npairs, n_low, n_high, seed, chunk=256, **spec_kw):
+        self.H, self.W, self.npairs = H, W, npairs
+        self.n_low, self.n_high, self.seed = n_low, n_high, seed
+        self.specs = pair_specs(npairs, seed, **spec_kw)
+        img = H * W
+        self.x = DeviceArray(npairs * img)
+        self.y = DeviceArray(npairs * img)
+        for k0 in range(0, npairs, chunk):
+            k1 = min(npairs, k0 + chunk)
+            sp = _f32(self.specs[k0:k1])
+            # pair index k0 + j draws the particles of pair j of the chunk with seed + k0
+            N.call("ob_dn_make_pairs", H, W, k1 - k0, N.fptr(sp), int(seed + 7919 * k0) & (2**64 - 1),
+                   int(n_low), int(n_high), self.x.at(k0 * img), self.y.at(k0 * img))
+
+    def host(self, k0, k1):
+        img = self.H * self.W
+        xs = self.x.download((k1 - k0) * img, k0 * img).reshape(k1 - k0, self.H, self.W)
+        ys = self.y.download((k1 - k0) * img, k0 * img).reshape(k1 - k0, self.H, self.W)
+        return xs, ys
+
+
+class Denoiser:
+    def __init__(self, H=256, W=256, max_batch=32):
+        h = C.c_void_p()
+        N.call("ob_dn_create", int(H), int(W), int(max_batch), C.byref(h))
+        self._h = h
+        self.H, self.W, self.max_batch = H, W, max_batch
+        n = C.c_int64()
+        N.call("ob_dn_num_params", self._h, C.byref(n))
+        self.nparams = n.value
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h is not None and h.value and N._lib is not None:
+            N._lib.ob_dn_destroy(h)
+            self._h = None
+
+    @staticmethod
+    def initParams(seed=0):
+        """He-normal conv weights, zero biases and a zero head: the untrained net returns
+        its input unchanged."""
+        rng = np.random.default_rng(seed)
+        parts = []
+        for ci, co, _ in LAYERS:
+            parts.append(rng.normal(0.0, np.sqrt(2.0 / (9 * ci)), 9 * ci * co))
+            parts.append(np.zeros(co))
+        parts.append(np.zeros(17))
+        return np.concatenate(parts).astype(np.float32)
+
+    def setParams(self, p):
+        p = _f32(p)
+        if p.size != self.nparams:
+            raise ValueError(f"expected {self.nparams} parameters, got {p.size}")
+        N.call("ob_dn_set_params", self._h, N.fptr(p), p.size)
+
+    def getParams(self):
+        p = np.empty(self.nparams, dtype=np.float32)
+        N.call("ob_dn_get_params", self._h, N.fptr(p), p.size)
+        return p
+
+    def save(self, path):
+        np.savez(path, params=self.getParams(), H=self.H, W=self.W)
+
+    def load(self, path):
+        with np.load(path, allow_pickle=False) as f:
+            self.setParams(f["params"])
+
+    def __call__(self, x):
+        """Denoise mean-1 images x [B, H, W] (host arrays)."""
+        x = _f32(x)
+        single = x.ndim == 2
+        if single:
+            x = x[None]
+        out = np.empty_like(x)
+        for b0 in range(0, x.shape[0], self.max_batch):
+            xb = _f32(x[b0:b0 + self.max_batch])
+            yb = np.empty_like(xb)
+            N.call("ob_dn_forward", self._h, N.fptr(xb), N.fptr(yb), xb.shape[0])
+            out[b0:b0 + xb.shape[0]] = yb
+        return out[0] if single else out
+
+    def lossAndGrad(self, x, y):
+        """MSE loss and the gradient of every parameter for host batches x, y [B, H, W]."""
+        x, y = _f32(x), _f32(y)
+        loss = np.zeros(1, dtype=np.float32)
+        grad = np.empty(self.nparams, dtype=np.float32)
+        N.call("ob_dn_loss_grad", self._h, N.fptr(x), N.fptr(y), x.shape[0], N.fptr(loss),
+               N.fptr(grad))
+        return float(loss[0]), grad
+
+    def trainStep(self, data, k0, batch, lr):
+        img = self.H * self.W
+        N.call("ob_dn_train_step_dev", self._h, data.x.at(k0 * img), data.y.at(k0 * img),
+               int(batch), float(lr))
+
+    def lastLoss(self):
+        v = np.zeros(1, dtype=np.float32)
+        N.call("ob_dn_last_loss", self._h, N.fptr(v))
+        return float(v[0])
+
+    def train(self, data, epochs=1, batch=32, lr=1e-3, lr_final=None, log_every=0,
+              log=print, npairs=None):
+        """Adam over the pairs of `data` in order, `epochs` times, with the learning rate
+        decayed geometrically to lr_final (default lr / 10).  Returns the loss of every
+        log_every-th step (all steps if 0 is given as None)."""
+        npairs = data.npairs if npairs is None else npairs
+        steps_per_epoch = npairs // batch
+        total = epochs * steps_per_epoch
+        lr_final = lr / 10 if lr_final is None else lr_final
+        hist = []
+        step = 0
+        for ep in range(epochs):
+            for s in range(steps_per_epoch):
+                frac = step / max(1, total - 1)
+                self.trainStep(data, s * batch, batch, lr * (lr_final / lr) ** frac)
+                step += 1
+                if log_every and step % log_every == 0:
+                    hist.append((step, self.lastLoss()))
+                    if log:
+                        log(f"step {step}/{total} loss {hist[-1][1]:.5f}")
+        return hist
