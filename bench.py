@@ -47,7 +47,11 @@ def parse():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=24)
     ap.add_argument("--warmup", type=int, default=2)
-    ap.add_argument("--workload", default="ring3d", choices=["ring3d", "fodo2p5d", "batch64"])
+    ap.add_argument("--workload", default="ring3d",
+                    choices=["ring3d", "fodo2p5d", "batch64", "denoise"])
+    ap.add_argument("--pairs", type=int, default=20000, help="denoise: training pairs")
+    ap.add_argument("--epochs", type=int, default=2, help="denoise: training epochs")
+    ap.add_argument("--dn-scales", type=int, default=5, help="denoise: loss pyramid scales")
     ap.add_argument("--n", type=int, default=0, help="override particle count")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--cpu-threads", type=int, default=0)
@@ -273,10 +277,121 @@ def main_batch(args):
     print(json.dumps(out))
 
 
+def field_of(solver, rho, lims):
+    """E = -grad phi on the nodes, phi from the device FP64 Poisson solve."""
+    import numpy as np
+    import spacecharge
+    g = spacecharge.Grid2D(np.asarray(rho, dtype=np.float64), *lims)
+    phi = spacecharge.Grid2D(rho.shape[0], rho.shape[1], *lims)
+    solver.findPotential(g, phi)
+    ex, ey = np.gradient(-phi.array())
+    return np.stack([ex, ey])
+
+
+def main_denoise(args):
+    """Learned denoiser: train on (100k, 16M)-particle deposit pairs on 256 x 256 grids,
+    then the field error of denoised 100k deposits against the 16M ones (FP64 solves)."""
+    import numpy as np
+    import _obnative as N
+    import spacecharge
+    from denoiser import LAYERS, Denoiser, PairSet
+    H = W = 256
+    n_low, n_high = args.n or 100_000, 16_000_000
+    ntrain, ntest, batch = args.pairs, 64, 32
+    t0 = time.perf_counter()
+    train = PairSet(H, W, ntrain, n_low, n_high, seed=20261202)
+    test = PairSet(H, W, ntest, n_low, n_high, seed=777)
+    N.synchronize()
+    t_gen = time.perf_counter() - t0
+    dn = Denoiser(H, W, max_batch=batch)
+    dn.setParams(Denoiser.initParams(1))
+    dn.setLoss(args.dn_scales, 1.0)
+    steps_per_epoch = ntrain // batch
+    dn.train(train, epochs=1, batch=batch, lr=1e-3, npairs=batch * 4, log=None)  # warm-up
+    dn.setParams(Denoiser.initParams(1))
+    N.synchronize()
+    t0 = time.perf_counter()
+    hist = dn.train(train, epochs=args.epochs, batch=batch, lr=1e-3, lr_final=1e-4,
+                    log_every=max(1, steps_per_epoch // 4), log=None)
+    N.synchronize()
+    t_train = time.perf_counter() - t0
+    nsteps = args.epochs * steps_per_epoch
+    pix = [H * W >> (2 * lev) for _, _, lev in LAYERS]
+    fwd = sum(2 * 9 * ci * co * p for (ci, co, _), p in zip(LAYERS, pix))
+    dgrad = fwd - 2 * 9 * LAYERS[0][0] * LAYERS[0][1] * pix[0]
+    flop_sample = fwd + dgrad + fwd
+    tflops = flop_sample * batch * nsteps / t_train / 1e12
+    # inference
+    x, y = test.host(0, ntest)
+    out = dn(x)
+    xb = np.ascontiguousarray(x[:batch])
+    N.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(5):
+        dn(xb)
+    t_inf = (time.perf_counter() - t0) / 5 / batch
+    # field error against the high-N deposit, FP64 device Poisson solves on a unit box
+    lims = (-1.0, 1.0, -1.0, 1.0)
+    solver = spacecharge.PoissonSolverFFT2D(H, W, *lims)
+    ef, ep, rf, rp = [], [], [], []
+    for k in range(ntest):
+        ref = field_of(solver, y[k], lims)
+        ef.append(np.linalg.norm(field_of(solver, out[k], lims) - ref) / np.linalg.norm(ref))
+        ep.append(np.linalg.norm(field_of(solver, x[k], lims) - ref) / np.linalg.norm(ref))
+        rf.append(np.linalg.norm(out[k] - y[k]) / np.linalg.norm(y[k]))
+        rp.append(np.linalg.norm(x[k] - y[k]) / np.linalg.norm(y[k]))
+    # cost: one 2.5D SC step at 100k particles plus inference against one at 16M
+    from workloads import gaussian_coords, make_bunch
+    costs = {}
+    for n in (n_low, n_high):
+        b = make_bunch(gaussian_coords(n, seed=5))
+        calc = spacecharge.SpaceChargeCalc2p5Drb(H, W, 64)
+        if n == n_low:
+            calc.setDenoiser(dn)
+        for _ in range(2):
+            calc.trackBunch(b, 0.5, 0.1)
+        N.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(10):
+            calc.trackBunch(b, 0.5, 0.1)
+        N.synchronize()
+        costs[n] = (time.perf_counter() - t0) / 10
+    out_line = {
+        "metric": "denoised 100k-particle field rel-L2 vs the 16M-particle FP64 solve (256x256)",
+        "value": round(float(np.median(ef)), 5), "unit": "rel-L2", "n_gpus": 1,
+        "steps": nsteps, "warmup": 4, "higher_is_better": False, "dtype": "bf16",
+        "data": "synthetic", "vs_baseline": None,
+        "config": {"workload": "denoise", "grid": [H, W], "n_low": n_low, "n_high": n_high,
+                   "train_pairs": ntrain, "test_pairs": ntest, "epochs": args.epochs,
+                   "batch": batch, "params": dn.nparams, "loss_scales": args.dn_scales},
+        "field_rel_l2": {"plain_low_n_median": round(float(np.median(ep)), 5),
+                         "denoised_median": round(float(np.median(ef)), 5),
+                         "plain_low_n_p90": round(float(np.percentile(ep, 90)), 5),
+                         "denoised_p90": round(float(np.percentile(ef, 90)), 5)},
+        "rho_rel_l2": {"plain_low_n_median": round(float(np.median(rp)), 4),
+                       "denoised_median": round(float(np.median(rf)), 4)},
+        "train": {"pair_generation_s": round(t_gen, 2), "train_s": round(t_train, 2),
+                  "samples_per_s": round(batch * nsteps / t_train, 1),
+                  "loss_first": hist[0][1] if hist else None,
+                  "loss_last": hist[-1][1] if hist else None},
+        "roofline": {"bound": "mfma", "achieved": round(tflops, 2), "peak": 2500.0,
+                     "unit": "TFLOP/s", "frac": round(tflops / 2500.0, 5), "traffic": None,
+                     "kernel": "training step (conv fwd + data grad + weight grad)",
+                     "flop_per_sample": flop_sample},
+        "inference_ms_per_grid": round(t_inf * 1e3, 4),
+        "sc_step_ms": {"100k_plus_denoiser": round(costs[n_low] * 1e3, 3),
+                       "16M_plain": round(costs[n_high] * 1e3, 3)},
+        "cpu_baseline": None,
+    }
+    print(json.dumps(out_line))
+
+
 def main():
     args = parse()
     if args.workload == "batch64" and args.gpus == 1:
         return main_batch(args)
+    if args.workload == "denoise" and args.gpus == 1:
+        return main_denoise(args)
     if args.gpus != 1:
         print("bench.py: only one GPU is measured; runs on several GPUs are not supported "
               "(the path would shard by particles with one rho all-reduce per SC step)",

@@ -131,6 +131,11 @@ int ob_sc2p5drb_create(int nx, int ny, int nz, double xy_ratio, ob_sc2p5drb** ou
 int ob_sc2p5drb_destroy(ob_sc2p5drb* c);
 /* trackBunch(bunch, length, pipe_radius) (:110-168). */
 int ob_sc2p5drb_track(ob_sc2p5drb* c, ob_bunch* b, double length, double pipe_radius);
+/* PoissonSolverFFT2D(xSize, ySize, -xy_ratio, xy_ratio, -1, 1).findPotential(rho, phi)
+ * (PoissonSolverFFT2D.cc:174-256) with the calculator's solver: rho and phi are host
+ * [nx][ny] grids spanning [xMin, xMax] x [yMin, yMax] (same step ratio as the solver). */
+int ob_sc2p5drb_find_potential(ob_sc2p5drb* calc, const double* rho, double xMin, double xMax,
+                               double yMin, double yMax, double* phi);
 /* getRhoGrid / getPhiGrid / getLongGrid / getLongDerivativeGrid (wrap :54-72 and
  * following): which = 0 rho, 1 phi ([nx][ny] row-major), 2 z line density, 3 its
  * derivative ([nz]).  lims receives xMin,xMax,yMin,yMax (which 0,1) or zMin,zMax. */
@@ -220,6 +225,9 @@ int ob_dn_loss_grad(ob_denoiser* dn, const float* x, const float* target, int B,
 int ob_dn_train_step_dev(ob_denoiser* dn, const float* x_dev, const float* target_dev, int B,
                          float lr);
 int ob_dn_last_loss(ob_denoiser* dn, float* loss);
+/* Loss = mean (out - y)^2 + sum_{s=1..nscales} lam[s-1] * mean of the squared 2^s x 2^s
+ * block means of out - y (low spatial frequencies carry the field error).  0: plain MSE. */
+int ob_dn_set_loss(ob_denoiser* dn, int nscales, const float* lam);
 /* Training pairs on the device: specs holds 5 floats per pair (family 0 Gaussian, 1 KV,
  * 2 waterbag, 3 hollow shell; x and y scales; cos and sin of the rotation).  x_dev gets the
  * n_low-particle TSC deposit, y_dev the n_high-particle one on the same grid (the square
@@ -230,6 +238,11 @@ int ob_dn_make_pairs(int H, int W, int npairs, const float* specs, uint64_t seed
 int ob_dn_alloc(int64_t nfloats, float** out);
 int ob_dn_free(float* p);
 int ob_dn_copy(float* dst, const float* src, int64_t nfloats, int kind);
+
+/* SpaceChargeCalc2p5Drb with a denoiser: rho is replaced by the denoiser's estimate (same
+ * total charge) after each deposit, before the moments and the Poisson solve.  The image
+ * size must equal the grid; nullptr switches it off.  No reference counterpart. */
+int ob_sc2p5drb_set_denoiser(ob_sc2p5drb* calc, ob_denoiser* dn);
 
 /* ---- kernel timing (HIP events on the library stream) ---- */
 /* Start (on != 0) or stop collecting per-kernel timings; clears earlier records. */

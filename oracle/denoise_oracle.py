@@ -129,15 +129,23 @@ class Net:
         self.cache = (x, xin, A)
         return out, A
 
-    def loss_grad(self, x, y):
-        """MSE loss and the gradient vector (layout of the parameters)."""
+    def loss_grad(self, x, y, lam=()):
+        """Loss and the gradient vector (layout of the parameters).  Loss = mean e^2 +
+        sum_s lam[s-1] mean over 2^s x 2^s blocks of (block mean of e)^2, e = out - y."""
         out, A = self.forward(x)
         x, xin, _ = self.cache
         r = self.r
         npix = out.size
         e = out - y
         loss = float(np.sum(e * e) / npix)
-        g = 2.0 * e / npix
+        es = e.copy()
+        m = e
+        for s, l in enumerate(lam, start=1):
+            B, H, W = m.shape
+            m = m.reshape(B, H // 2, 2, W // 2, 2).mean(axis=(2, 4))
+            loss += l * float(np.mean(m * m))
+            es += l * np.repeat(np.repeat(m, 2 ** s, axis=1), 2 ** s, axis=2)
+        g = 2.0 * es / npix
         dhw = np.einsum("bhw,bhwc->c", g, A[11])
         dhb = g.sum()
         G = [None] * 12

@@ -84,6 +84,8 @@ SIGNATURES = {
     "ob_batch_capture_begin": [_vp],
     "ob_batch_capture_end": [_vp],
     "ob_batch_replay": [_vp, _i],
+    "ob_sc2p5drb_find_potential": [_vp, _D, _d, _d, _d, _d, _D],
+    "ob_sc2p5drb_set_denoiser": [_vp, _vp],
     "ob_dn_create": [_i, _i, _i, C.POINTER(_vp)],
     "ob_dn_destroy": [_vp],
     "ob_dn_num_params": [_vp, C.POINTER(_i64)],
@@ -94,6 +96,7 @@ SIGNATURES = {
     "ob_dn_loss_grad": [_vp, _F, _F, _i, _F, _F],
     "ob_dn_train_step_dev": [_vp, _vp, _vp, _i, C.c_float],
     "ob_dn_last_loss": [_vp, _F],
+    "ob_dn_set_loss": [_vp, _i, _F],
     "ob_dn_make_pairs": [_i, _i, _i, _F, C.c_uint64, _i64, _i64, _vp, _vp],
     "ob_dn_alloc": [_i64, C.POINTER(_vp)],
     "ob_dn_free": [_vp],

@@ -372,8 +372,13 @@ OB_API int ob_batch_sc2p5drb_track(ob_batch* B, double length, double pipe_radiu
     ob::autotune(B);
   }
   ob::flush(B, true);
+  std::vector<ob::DenoiseJob> jobs;
+  for (ob_sc2p5drb* c : B->calcs) {
+    ob::DenoiseJob j;
+    if (ob::sc2d_denoise_job(c, &j)) jobs.push_back(j);
+  }
   ob::launch_sc2d_batch(B->d_bb, B->d_items, B->shape, B->d_partials, length, pipe_radius,
-                        ob::stream());
+                        ob::stream(), jobs);
   const ob::SyncInfo sy = ob::bunch_sync(B->leader);
   B->kick.active = 1;
   B->kick.p4 = 2 * log(pipe_radius);                               // :139

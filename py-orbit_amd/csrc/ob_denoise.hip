@@ -335,15 +335,55 @@ __device__ __forceinline__ float block_sum(float v, float* sh) {
   return s;
 }
 
-// MSE loss over npix pixels and its gradient: dd1 = g hw, dhw += sum g d1, dhb += sum g,
-// with g = 2 (out - y) / npix.  loss += sum (out - y)^2 / npix.
+// Multi-scale error pyramid of the loss: m_s = mean of e = out - y over 2^s x 2^s blocks,
+// s = 1..S.  Low spatial frequencies of the density error dominate the field error, so
+// the loss adds lam_s * mean(m_s^2) for each scale.
+constexpr int kMaxScales = 8;
+struct Pyramid {
+  float* m[kMaxScales + 1];  // m[s]: [B][H >> s][W >> s]  (m[0] unused)
+  float lam[kMaxScales + 1];
+  int32_t S, H, W;
+};
+
+// Level s from level s - 1 (s = 1: from out - y); loss += lam_s * mean over blocks of m_s^2.
+__global__ __launch_bounds__(kThreads) void pyramid_level(const float* __restrict__ out,
+                                                          const float* __restrict__ y,
+                                                          Pyramid P, int s, int B,
+                                                          float* __restrict__ loss) {
+  __shared__ float sh[kThreads / 64];
+  const int h = P.H >> s, w = P.W >> s, W2 = P.W >> (s - 1);
+  const int64_t n = (int64_t)B * h * w;
+  float l = 0.f;
+  for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < n;
+       e += (int64_t)gridDim.x * blockDim.x) {
+    const int x = (int)(e % w);
+    const int64_t r = e / w;
+    const int yy = (int)(r % h);
+    const int64_t b = r / h;
+    const int64_t i0 = (b * (P.H >> (s - 1)) + 2 * yy) * W2 + 2 * x;
+    const int64_t idx[4] = {i0, i0 + 1, i0 + W2, i0 + W2 + 1};
+    float v = 0.f;
+#pragma unroll
+    for (int k = 0; k < 4; k++) v += s == 1 ? out[idx[k]] - y[idx[k]] : P.m[s - 1][idx[k]];
+    v *= 0.25f;
+    P.m[s][e] = v;
+    l += v * v;
+  }
+  const float t = block_sum(l, sh);
+  if (threadIdx.x == 0) atomicAdd(loss, P.lam[s] * t / (float)n);
+}
+
+// Loss over npix pixels and its gradient: dd1 = g hw, dhw += sum g d1, dhb += sum g, with
+// g = 2 (e + sum_s lam_s m_s(block of p)) / npix, e = out - y; loss += sum e^2 / npix
+// (the pyramid terms are added by pyramid_level).
 __global__ __launch_bounds__(kThreads) void head_bwd(const bf16* __restrict__ d1,
                                                      const float* __restrict__ out,
                                                      const float* __restrict__ y,
                                                      const float* __restrict__ hw,
                                                      bf16* __restrict__ dd1,
                                                      float* __restrict__ dhw,
-                                                     float* __restrict__ loss, int64_t npix) {
+                                                     float* __restrict__ loss, int64_t npix,
+                                                     Pyramid P) {
   __shared__ float sh[kThreads / 64];
   float acc[17];
 #pragma unroll
@@ -354,7 +394,16 @@ __global__ __launch_bounds__(kThreads) void head_bwd(const bf16* __restrict__ d1
        p += (int64_t)gridDim.x * blockDim.x) {
     const float e = out[p] - y[p];
     l += e * e;
-    const float g = 2.0f * e * inv;
+    float es = e;
+    if (P.S > 0) {
+      const int x = (int)(p % P.W);
+      const int64_t r = p / P.W;
+      const int yy = (int)(r % P.H);
+      const int64_t b = r / P.H;
+      for (int sc = 1; sc <= P.S; sc++)
+        es += P.lam[sc] * P.m[sc][(b * (P.H >> sc) + (yy >> sc)) * (P.W >> sc) + (x >> sc)];
+    }
+    const float g = 2.0f * es * inv;
     float f[16], o[16];
     unpack8(ld16(d1 + p * 16), f);
     unpack8(ld16(d1 + p * 16 + 8), f + 8);
@@ -472,6 +521,44 @@ __global__ __launch_bounds__(kThreads) void fill_f32(float* __restrict__ p, int6
   for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
        i += (int64_t)gridDim.x * blockDim.x)
     p[i] = v;
+}
+
+// ---- use inside a space-charge step -----------------------------------------------------
+// x = rho * n / total (mean 1), total = sum rho.  One block.
+__global__ __launch_bounds__(1024) void rho_to_image(const double* __restrict__ rho, int n,
+                                                     const int32_t* skip, float* __restrict__ x,
+                                                     double* __restrict__ total) {
+  if (*skip) return;
+  __shared__ double sh[16];
+  double s = 0.0;
+  for (int i = threadIdx.x; i < n; i += blockDim.x) s += rho[i];
+  for (int o = 32; o > 0; o >>= 1) s += __shfl_xor(s, o, 64);
+  if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = s;
+  __syncthreads();
+  double t = 0.0;
+  for (int w = 0; w < (int)(blockDim.x >> 6); w++) t += sh[w];
+  const double scale = t > 0.0 ? (double)n / t : 0.0;
+  for (int i = threadIdx.x; i < n; i += blockDim.x) x[i] = (float)(rho[i] * scale);
+  if (threadIdx.x == 0) *total = t;
+}
+
+// rho = out * total / sum(out): the denoised grid with the deposited total charge.
+__global__ __launch_bounds__(1024) void image_to_rho(const float* __restrict__ out, int n,
+                                                     const int32_t* skip,
+                                                     const double* __restrict__ total,
+                                                     double* __restrict__ rho) {
+  if (*skip) return;
+  __shared__ double sh[16];
+  double s = 0.0;
+  for (int i = threadIdx.x; i < n; i += blockDim.x) s += (double)out[i];
+  for (int o = 32; o > 0; o >>= 1) s += __shfl_xor(s, o, 64);
+  if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = s;
+  __syncthreads();
+  double t = 0.0;
+  for (int w = 0; w < (int)(blockDim.x >> 6); w++) t += sh[w];
+  if (t == 0.0) return;  // keep the deposited grid
+  const double scale = *total / t;
+  for (int i = threadIdx.x; i < n; i += blockDim.x) rho[i] = (double)out[i] * scale;
 }
 
 // ---- training pairs ------------------------------------------------------------------------
@@ -649,6 +736,11 @@ struct ob_denoiser {
   bf16* gtmp1 = nullptr;
   float* out = nullptr;   // head output, FP32
   float* loss = nullptr;  // device scalar
+  float* pyr = nullptr;   // error pyramid of the multi-scale loss
+  int nscales = 0;
+  float lam[9] = {};
+  float* xin = nullptr;   // one normalized rho image (space-charge use)
+  double* total = nullptr;  // its total charge
 };
 
 namespace ob {
@@ -751,11 +843,28 @@ void backward(ob_denoiser* d, const float* x, const float* target, int B, hipStr
   hipLaunchKernelGGL(fill_f32, dim3(grid_for(d->nparam)), dim3(kThreads), 0, s, d->g,
                      (int64_t)d->nparam, 0.f);
   hipLaunchKernelGGL(fill_f32, dim3(1), dim3(kThreads), 0, s, d->loss, (int64_t)1, 0.f);
+  Pyramid P;
+  std::memset(&P, 0, sizeof(P));
+  P.S = d->nscales;
+  P.H = d->H;
+  P.W = d->W;
+  size_t off = 0;
+  for (int sc = 1; sc <= P.S; sc++) {
+    P.m[sc] = d->pyr + off;
+    P.lam[sc] = d->lam[sc];
+    off += (size_t)B * (d->H >> sc) * (d->W >> sc);
+  }
+  for (int sc = 1; sc <= P.S; sc++) {
+    ProfScope ps("dn_loss_pyramid");
+    const int64_t n = (int64_t)B * (d->H >> sc) * (d->W >> sc);
+    hipLaunchKernelGGL(pyramid_level, dim3(grid_for(n)), dim3(kThreads), 0, s, d->out, target, P,
+                       sc, B, d->loss);
+  }
   {
     ProfScope ps("dn_head_bwd");
     hipLaunchKernelGGL(head_bwd, dim3(grid_for(npix)), dim3(kThreads), 0, s, d->act[11], d->out,
                        target, d->w + d->head_off, d->gact[11], d->g + d->head_off, d->loss,
-                       npix);
+                       npix, P);
   }
   // gradient w.r.t. each layer's output arrives in gact[l] (before its ReLU mask)
   for (int l = 11; l >= 0; l--) {
@@ -847,6 +956,21 @@ void check_batch(ob_denoiser* d, int B) {
 
 }  // namespace
 }  // namespace dn
+
+void dn_check_shape(ob_denoiser* d, int nx, int ny) {
+  OB_REQUIRE(d->H == nx && d->W == ny, "denoiser: image size differs from the grid");
+}
+
+void dn_denoise_rho(ob_denoiser* d, double* rho, int nx, int ny, const int32_t* skip) {
+  dn_check_shape(d, nx, ny);
+  hipStream_t s = stream();
+  const int n = nx * ny;
+  ProfScope ps("dn_sc_denoise");
+  hipLaunchKernelGGL(dn::rho_to_image, dim3(1), dim3(1024), 0, s, rho, n, skip, d->xin, d->total);
+  dn::forward(d, d->xin, 1, s);
+  hipLaunchKernelGGL(dn::image_to_rho, dim3(1), dim3(1024), 0, s, d->out, n, skip, d->total, rho);
+  OB_HIP(hipGetLastError());
+}
 }  // namespace ob
 
 #define OB_API extern "C" __attribute__((visibility("default")))
@@ -912,6 +1036,9 @@ OB_API int ob_dn_create(int H, int W, int max_batch, ob_denoiser** out) {
     OB_HIP(hipMalloc((void**)&d->gtmp1, sizeof(bf16) * (img / 16) * 32));
     OB_HIP(hipMalloc((void**)&d->out, sizeof(float) * img));
     OB_HIP(hipMalloc((void**)&d->loss, sizeof(float)));
+    OB_HIP(hipMalloc((void**)&d->xin, sizeof(float) * H * W));
+    OB_HIP(hipMalloc((void**)&d->pyr, sizeof(float) * (img / 3 + 1)));
+    OB_HIP(hipMalloc((void**)&d->total, sizeof(double)));
   } catch (...) {
     delete d;  // device buffers of a failed create are reclaimed at process exit
     throw;
@@ -925,7 +1052,8 @@ OB_API int ob_dn_destroy(ob_denoiser* d) {
   (void)hipStreamSynchronize(ob::stream());
   for (void* p : {(void*)d->w, (void*)d->g, (void*)d->m, (void*)d->v, (void*)d->pf, (void*)d->pt,
                   (void*)d->gcat4, (void*)d->gcat5, (void*)d->gtmp0, (void*)d->gtmp1,
-                  (void*)d->out, (void*)d->loss})
+                  (void*)d->out, (void*)d->loss, (void*)d->xin, (void*)d->total,
+                  (void*)d->pyr})
     if (p) (void)hipFree(p);
   for (int l = 0; l < 12; l++) {
     if (d->act[l]) (void)hipFree(d->act[l]);
@@ -1043,6 +1171,20 @@ OB_API int ob_dn_train_step_dev(ob_denoiser* d, const float* x_dev, const float*
   }
   ob::dn::pack_all(d, s);
   OB_HIP(hipGetLastError());
+  OB_CATCH
+}
+
+// Loss = mean (out - y)^2 + sum_{s=1..nscales} lam[s-1] * mean over 2^s x 2^s blocks of
+// (block mean of out - y)^2.  nscales = 0: plain MSE.
+OB_API int ob_dn_set_loss(ob_denoiser* d, int nscales, const float* lam) {
+  OB_TRY OB_REQUIRE(d != nullptr, "null handle");
+  OB_REQUIRE(nscales >= 0 && nscales <= ob::dn::kMaxScales && (nscales == 0 || lam != nullptr),
+             "denoiser: 0..8 loss scales");
+  OB_REQUIRE((d->H >> nscales) >= 1 && (d->W >> nscales) >= 1 && (d->H >> nscales) << nscales == d->H &&
+                 (d->W >> nscales) << nscales == d->W,
+             "denoiser: the image is not divisible into 2^nscales blocks");
+  d->nscales = nscales;
+  for (int sc = 1; sc <= nscales; sc++) d->lam[sc] = lam[sc - 1];
   OB_CATCH
 }
 

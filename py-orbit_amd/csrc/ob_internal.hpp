@@ -136,6 +136,9 @@ struct Tile2D {
   double *sx, *sy, *sm;
 };
 
+}  // namespace ob
+struct ob_denoiser;
+namespace ob {
 // ---- batches of bunches, each with its own SpaceChargeCalc2p5Drb ----------------------
 // Device-resident record of one bunch of a batch.
 struct BatchBunch {
@@ -168,8 +171,16 @@ struct BatchShape {
   int nb, nitems, row_blocks, col_blocks, max_M, dep_slots;
   size_t hist_lds, dep_lds;  // dynamic LDS of the hist/scatter and tile-deposit passes
 };
+// Denoiser applied to one bunch's rho between the deposit and the moments.
+struct DenoiseJob {
+  ob_denoiser* d;
+  double* rho;
+  int nx, ny;
+  const int32_t* skip;
+};
 void launch_sc2d_batch(const BatchBunch* bb, const BatchItem* items, const BatchShape& sh,
-                       const double* partials, double length, double pipe_radius, hipStream_t s);
+                       const double* partials, double length, double pipe_radius, hipStream_t s,
+                       const std::vector<DenoiseJob>& jobs = {});
 }  // namespace ob
 struct ob_sc2p5drb;
 namespace ob {
@@ -184,6 +195,12 @@ bool sc2d_tile_same(ob_sc2p5drb* c, const Tile2D& t, int64_t n, bool per_particl
 void sc2d_mark_tracked(ob_sc2p5drb* c);
 int sc2d_read_status(ob_sc2p5drb* c);
 int push_grid_blocks(int64_t n);
+// The calculator's denoiser job (false: none set).
+bool sc2d_denoise_job(ob_sc2p5drb* c, DenoiseJob* job);
+// Replace a [nx][ny] FP64 rho grid by the denoiser's estimate (same total charge); nothing
+// happens where *skip is set.  dn_check_shape: the denoiser's image size must be nx x ny.
+void dn_denoise_rho(ob_denoiser* d, double* rho, int nx, int ny, const int32_t* skip);
+void dn_check_shape(ob_denoiser* d, int nx, int ny);
 
 }  // namespace ob
 

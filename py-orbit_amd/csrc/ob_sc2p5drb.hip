@@ -834,6 +834,7 @@ struct ob_sc2p5drb {
   int64_t tile_ncap = 0;  // scratch particles allocated
   bool tile_msize = false;
   int tile_item = 2048;   // particles per deposit work item
+  ob_denoiser* denoiser = nullptr;  // applied to rho after each deposit when set
 };
 
 namespace ob {
@@ -996,9 +997,15 @@ int sc2d_read_status(ob_sc2p5drb* c) {
   return v;
 }
 
+bool sc2d_denoise_job(ob_sc2p5drb* c, DenoiseJob* job) {
+  if (!c->denoiser) return false;
+  *job = DenoiseJob{c->denoiser, c->rho, c->nx, c->ny, &c->st->skip};
+  return true;
+}
+
 void launch_sc2d_batch(const BatchBunch* bb, const BatchItem* items, const BatchShape& sh,
                        const double* partials, double length, double pipe_radius,
-                       hipStream_t s) {
+                       hipStream_t s, const std::vector<DenoiseJob>& jobs) {
   OB_REQUIRE(sh.hist_lds <= 64 * 1024 && sh.dep_lds <= 64 * 1024,
              "batch deposit: LDS block too large");
   OB_REQUIRE(sh.max_M <= 1024, "batch: grids up to 512 points per axis");
@@ -1026,6 +1033,7 @@ void launch_sc2d_batch(const BatchBunch* bb, const BatchItem* items, const Batch
     hipLaunchKernelGGL(sc2d_tdeposit_batch, dim3(sh.dep_slots), dim3(kDepositThreads),
                        sh.dep_lds, s, bb, sh.nb);
   }
+  for (const DenoiseJob& j : jobs) dn_denoise_rho(j.d, j.rho, j.nx, j.ny, j.skip);
   {
     ProfScope ps("sc2d_post_batch");
     hipLaunchKernelGGL(sc2d_post_batch, dim3(sh.nb), dim3(kSetupThreads), 0, s, bb, length,
@@ -1174,6 +1182,7 @@ OB_API int ob_sc2p5drb_track(ob_sc2p5drb* c, ob_bunch* b, double length, double 
                        dim3(ob::kDepositThreads), sizeof(double) * (tg.T + 2) * (tg.T + 2), s, P,
                        c->st, tg, v.msize != nullptr ? 1 : 0, c->rho);
   }
+  if (c->denoiser) ob::dn_denoise_rho(c->denoiser, c->rho, c->nx, c->ny, &c->st->skip);
   {
     ob::ProfScope ps_("sc2d_post");
     hipLaunchKernelGGL(ob::sc2d_post, dim3(1), dim3(ob::kSetupThreads), 0, s, P, c->st, c->rho,
@@ -1199,6 +1208,45 @@ OB_API int ob_sc2p5drb_track(ob_sc2p5drb* c, ob_bunch* b, double length, double 
   op.p[5] = -length * sy.r0 * sy.mass / (sy.gamma * sy.gamma);                // :141
   ob::bunch_queue(b, op);
   c->tracked = true;
+  OB_CATCH
+}
+
+// PoissonSolverFFT2D::findPotential (PoissonSolverFFT2D.cc:174-256) on a host rho grid
+// [nx][ny] spanning [xMin, xMax] x [yMin, yMax]: the calculator's solver (box
+// [-xy_ratio, xy_ratio] x [-1, 1]) rescaled to the rho grid's step.
+OB_API int ob_sc2p5drb_find_potential(ob_sc2p5drb* c, const double* rho, double xMin,
+                                      double xMax, double yMin, double yMax, double* phi) {
+  OB_TRY OB_REQUIRE(c != nullptr && rho != nullptr && phi != nullptr, "null handle");
+  const double dx = (xMax - xMin) / (c->nx - 1), dy = (yMax - yMin) / (c->ny - 1);
+  OB_REQUIRE(dx > 0.0 && dy > 0.0, "PoissonSolverFFT2D: empty grid");
+  OB_REQUIRE(fabs(c->dx_solver / c->dy_solver - dx / dy) <= 1e-8,  // :176-210
+             "PoissonSolverFFT2D: the grid sizes or shape are different");
+  hipStream_t s = ob::stream();
+  OB_HIP(hipStreamSynchronize(s));
+  ob::SC2DState h;
+  OB_HIP(hipMemcpy(&h, c->st, sizeof(h), hipMemcpyDeviceToHost));
+  h.xMin = xMin;
+  h.xMax = xMax;
+  h.yMin = yMin;
+  h.yMax = yMax;
+  h.dx = dx;
+  h.dy = dy;
+  h.skip = 0;
+  h.status = 0;
+  h.scale_coeff = log(c->dx_solver / dx);  // :216
+  OB_HIP(hipMemcpy(c->st, &h, sizeof(h), hipMemcpyHostToDevice));
+  OB_HIP(hipMemcpyAsync(c->rho, rho, sizeof(double) * c->nx * c->ny, hipMemcpyHostToDevice, s));
+  ob::launch_fft_pipeline(c, c->st);
+  OB_HIP(hipMemcpyAsync(phi, c->phi, sizeof(double) * c->nx * c->ny, hipMemcpyDeviceToHost, s));
+  OB_HIP(hipStreamSynchronize(s));
+  OB_CATCH
+}
+
+// Denoise rho after every deposit (nullptr: off).  The denoiser must outlive its use here.
+OB_API int ob_sc2p5drb_set_denoiser(ob_sc2p5drb* c, ob_denoiser* dn) {
+  OB_TRY OB_REQUIRE(c != nullptr, "null handle");
+  if (dn) ob::dn_check_shape(dn, c->nx, c->ny);
+  c->denoiser = dn;
   OB_CATCH
 }
 

@@ -158,6 +158,15 @@ class Denoiser:
             out[b0:b0 + xb.shape[0]] = yb
         return out[0] if single else out
 
+    def setLoss(self, scales=0, weight=1.0):
+        """Loss = mean e^2 + sum_{s=1..scales} weight * 4^s * mean of the squared 2^s x 2^s
+        block means of e = out - target.  Block means of white noise shrink as 2^-s, so
+        each scale then weighs like the pixel term: long wavelengths, which carry the
+        field error, count as much as short ones.  scales = 0: plain MSE."""
+        lam = np.array([weight * 4.0 ** s for s in range(1, scales + 1)], dtype=np.float32)
+        N.call("ob_dn_set_loss", self._h, int(scales), N.fptr(lam) if scales else None)
+        self.loss_lam = lam
+
     def lossAndGrad(self, x, y):
         """MSE loss and the gradient of every parameter for host batches x, y [B, H, W]."""
         x, y = _f32(x), _f32(y)

@@ -46,11 +46,33 @@ class Grid1D:
 
 
 class Grid2D:
-    """Snapshot of a 2D node-centred grid, arr[ix][iy] (src/spacecharge/Grid2D.cc)."""
+    """2D node-centred grid, arr[ix][iy] (src/spacecharge/Grid2D.cc).
 
-    def __init__(self, arr, xMin, xMax, yMin, yMax):
+    Grid2D(xSize, ySize[, xMin, xMax, yMin, yMax]) makes a zero grid as the reference's
+    constructor does (wrap_grid2D.cc; limits default to [-1, 1]); Grid2D(arr, xMin, xMax,
+    yMin, yMax) wraps an array (the snapshots the calculators return)."""
+
+    def __init__(self, arr, xMin=-1.0, xMax=1.0, yMin=-1.0, yMax=1.0, *rest):
+        if isinstance(arr, (int, np.integer)):
+            ySize = xMin
+            lims = (xMax, yMin, yMax) + rest if rest else None
+            self._a = np.zeros((int(arr), int(ySize)))
+            self._l = tuple(float(v) for v in lims) if lims and len(lims) == 4 else (-1.0, 1.0, -1.0, 1.0)
+            return
         self._a = np.array(arr, dtype=np.float64)
         self._l = (float(xMin), float(xMax), float(yMin), float(yMax))
+
+    def setGridX(self, xMin, xMax):
+        self._l = (float(xMin), float(xMax)) + self._l[2:]
+
+    def setGridY(self, yMin, yMax):
+        self._l = self._l[:2] + (float(yMin), float(yMax))
+
+    def setValue(self, value, ix, iy):
+        self._a[ix, iy] = value
+
+    def setZero(self):
+        self._a[:] = 0.0
 
     def getSizeX(self):
         return self._a.shape[0]
@@ -145,6 +167,62 @@ class Grid3D:
         return self._a.copy()
 
 
+class PoissonSolverFFT2D:
+    """PoissonSolverFFT2D(xSize, ySize[, xMin, xMax, yMin, yMax])
+    (wrap_poissonsolverfft2d.cc:37-47): free-space 2D Poisson solver on the device, FP64
+    Hockney convolution (PoissonSolverFFT2D.cc)."""
+
+    def __init__(self, xSize, ySize, xMin=-1.0, xMax=1.0, yMin=-1.0, yMax=1.0):
+        self._n = (int(xSize), int(ySize))
+        self._l = (float(xMin), float(xMax), float(yMin), float(yMax))
+        ratio = (self._l[1] - self._l[0]) / (self._l[3] - self._l[2])
+        h = C.c_void_p()
+        N.call("ob_sc2p5drb_create", self._n[0], self._n[1], 1, ratio, C.byref(h))
+        self._h = h
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h and N._lib is not None:
+            N._lib.ob_sc2p5drb_destroy(h)
+            self._h = None
+
+    def getSizeX(self):
+        return self._n[0]
+
+    def getSizeY(self):
+        return self._n[1]
+
+    def getMinX(self):
+        return self._l[0]
+
+    def getMaxX(self):
+        return self._l[1]
+
+    def getMinY(self):
+        return self._l[2]
+
+    def getMaxY(self):
+        return self._l[3]
+
+    def getStepX(self):
+        return (self._l[1] - self._l[0]) / (self._n[0] - 1)
+
+    def getStepY(self):
+        return (self._l[3] - self._l[2]) / (self._n[1] - 1)
+
+    def findPotential(self, rhoGrid2D, phiGrid2D):
+        """phiGrid2D <- potential of rhoGrid2D (PoissonSolverFFT2D.cc:174-256).  The grids
+        must have the solver's size and step ratio, and phi the rho grid's limits."""
+        if (rhoGrid2D.getSizeX(), rhoGrid2D.getSizeY()) != self._n or \
+                (phiGrid2D.getSizeX(), phiGrid2D.getSizeY()) != self._n or \
+                phiGrid2D._l != rhoGrid2D._l:
+            raise N.OrbitError("PoissonSolverFFT2D: the grid sizes or shape are different")
+        rho = np.ascontiguousarray(rhoGrid2D._a, dtype=np.float64)
+        phi = np.empty_like(rho)
+        N.call("ob_sc2p5drb_find_potential", self._h, N.dptr(rho), *rhoGrid2D._l, N.dptr(phi))
+        phiGrid2D._a[:] = phi
+
+
 class SpaceChargeCalc2p5Drb:
     """SpaceChargeCalc2p5Drb(nx, ny, nz[, xy_ratio]) (SpaceChargeCalc2p5Drb.cc:31-63)."""
 
@@ -160,6 +238,15 @@ class SpaceChargeCalc2p5Drb:
         if h and N._lib is not None:
             N._lib.ob_sc2p5drb_destroy(h)
             self._h = None
+
+    def setDenoiser(self, denoiser):
+        """Replace rho by the learned denoiser's estimate after every deposit (None: off).
+        No reference counterpart; the denoiser's image size must equal the grid."""
+        N.call("ob_sc2p5drb_set_denoiser", self._h, denoiser._h if denoiser is not None else None)
+        self._denoiser = denoiser  # kept alive while in use
+
+    def getDenoiser(self):
+        return getattr(self, "_denoiser", None)
 
     def trackBunch(self, bunch, length, pipe_radius):
         """trackBunch(bunch, length, pipe_radius) (:110-168)."""

@@ -53,8 +53,9 @@ def test_backward_matches_finite_differences():
         off += co
     x = r.gamma(2.0, 0.5, size=(2, 8, 8))
     y = r.gamma(2.0, 0.5, size=(2, 8, 8))
+    lam = (4.0, 16.0)  # the multi-scale terms too
     net = D.Net(p, bf16=False)
-    loss, g = net.loss_grad(x, y)
+    loss, g = net.loss_grad(x, y, lam)
     # a sample of parameters from every layer and the head
     idx, off = [], 0
     for ci, co, _ in D.LAYERS:
@@ -67,7 +68,8 @@ def test_backward_matches_finite_differences():
         pp, pm = p.copy(), p.copy()
         pp[i] += eps
         pm[i] -= eps
-        fd = (D.Net(pp, bf16=False).loss_grad(x, y)[0] - D.Net(pm, bf16=False).loss_grad(x, y)[0]) / (2 * eps)
+        fd = (D.Net(pp, bf16=False).loss_grad(x, y, lam)[0] -
+              D.Net(pm, bf16=False).loss_grad(x, y, lam)[0]) / (2 * eps)
         assert abs(fd - g[i]) <= 1e-7 + 1e-4 * abs(g[i]), (i, fd, g[i])
     assert loss > 0
 

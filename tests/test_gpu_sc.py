@@ -213,3 +213,53 @@ def test_sc3d_tiled_matches_particle_order_deposit(budget):
     ocalc.trackBunch(ob, 0.5)
     o1 = b1.getCoordinates()
     assert np.array_equal(o1[0], b1.getCoordinates()[0])
+
+
+def test_poisson_solver_fft2d_find_potential():
+    """PoissonSolverFFT2D.findPotential on a user grid (solver on its own box, rho on a
+    box of the same step ratio) against the oracle's PoissonSolverFFT2D."""
+    import spacecharge
+    r = np.random.default_rng(8)
+    nx, ny = 40, 24
+    solver_lims = (-2.0, 2.0, -1.0, 1.0)
+    # the rho grid's steps keep dx/dy of the solver: (xMax-xMin)/(yMax-yMin) = 2
+    rho_lims = (0.3, 0.3 + 0.05 * 2, -0.7, -0.7 + 0.05)
+    ps = spacecharge.PoissonSolverFFT2D(nx, ny, *solver_lims)
+    rho = spacecharge.Grid2D(nx, ny, *rho_lims)
+    phi = spacecharge.Grid2D(nx, ny, *rho_lims)
+    arr = r.normal(size=(nx, ny))
+    for ix in range(nx):
+        for iy in range(ny):
+            rho.setValue(arr[ix, iy], ix, iy)
+    ps.findPotential(rho, phi)
+    assert rel_l2(phi.array(), O.poisson2d(arr, solver_lims, rho_lims)) < 1e-12
+    bad = spacecharge.Grid2D(nx, ny, 0.0, 1.0, 0.0, 1.0)  # step ratio differs
+    with pytest.raises(Exception):
+        ps.findPotential(bad, spacecharge.Grid2D(nx, ny, 0.0, 1.0, 0.0, 1.0))
+
+
+def test_sc2p5drb_with_identity_denoiser_equals_plain():
+    """An untrained denoiser (zero head: output = input) changes rho only by its FP32
+    normalization; the kicks match the plain calculator to that level and the total
+    charge is kept exactly."""
+    import spacecharge
+    from denoiser import Denoiser
+    from workloads import gaussian_coords
+    c = gaussian_coords(20000, seed=12)
+    b1, _ = _bunches(c)
+    b2, _ = _bunches(c)
+    plain = spacecharge.SpaceChargeCalc2p5Drb(64, 64, 16)
+    calc = spacecharge.SpaceChargeCalc2p5Drb(64, 64, 16)
+    dn = Denoiser(64, 64, max_batch=1)
+    dn.setParams(Denoiser.initParams(1))
+    calc.setDenoiser(dn)
+    plain.trackBunch(b1, 0.5, 0.1)
+    calc.trackBunch(b2, 0.5, 0.1)
+    o1, o2 = b1.getCoordinates(), b2.getCoordinates()
+    r1, r2 = plain.getRhoGrid().array(), calc.getRhoGrid().array()
+    assert rel_l2(r2, r1) < 1e-6
+    assert r2.sum() == pytest.approx(r1.sum(), rel=1e-12)
+    for k in (1, 3):
+        assert rel_l2(o2[k] - c[k], o1[k] - c[k]) < 1e-5, k
+    calc.setDenoiser(None)
+    assert calc.getDenoiser() is None
