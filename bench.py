@@ -52,6 +52,7 @@ def parse():
     ap.add_argument("--pairs", type=int, default=20000, help="denoise: training pairs")
     ap.add_argument("--epochs", type=int, default=2, help="denoise: training epochs")
     ap.add_argument("--dn-scales", type=int, default=5, help="denoise: loss pyramid scales")
+    ap.add_argument("--turns", type=int, default=5, help="denoise: ring turns of the tracking test")
     ap.add_argument("--n", type=int, default=0, help="override particle count")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--cpu-threads", type=int, default=0)
@@ -356,6 +357,7 @@ def main_denoise(args):
             calc.trackBunch(b, 0.5, 0.1)
         N.synchronize()
         costs[n] = (time.perf_counter() - t0) / 10
+    tracking = track_with_denoiser(args, dn, H, W, n_low, n_high) if args.turns > 0 else None
     out_line = {
         "metric": "denoised 100k-particle field rel-L2 vs the 16M-particle FP64 solve (256x256)",
         "value": round(float(np.median(ef)), 5), "unit": "rel-L2", "n_gpus": 1,
@@ -381,9 +383,57 @@ def main_denoise(args):
         "inference_ms_per_grid": round(t_inf * 1e3, 4),
         "sc_step_ms": {"100k_plus_denoiser": round(costs[n_low] * 1e3, 3),
                        "16M_plain": round(costs[n_high] * 1e3, 3)},
+        "tracking": tracking,
         "cpu_baseline": None,
     }
     print(json.dumps(out_line))
+
+
+def track_with_denoiser(args, dn, H, W, n_low, n_high):
+    """The same Gaussian bunch (intensity 1e14) through `turns` turns of the 24-cell ring
+    with a SpaceChargeCalc2p5Drb(H, W, 64) node every ~0.5 m: plain at n_high (the
+    reference), plain at n_low, denoised at n_low.  rms emittance growth and the fraction
+    of particles beyond 4 rms in normalized transverse amplitude."""
+    import numpy as np
+    import _obnative as N
+    import spacecharge
+    from bunch import BunchTwissAnalysis
+    from orbit.space_charge.sc2p5d import setSC2p5DrbAccNodes
+    from workloads import PIPE_RADIUS, gaussian_coords, make_bunch, ring_lattice
+    res = {}
+    for name, n, use in (("plain_high_n", n_high, False), ("plain_low_n", n_low, False),
+                         ("denoised_low_n", n_low, True)):
+        b = make_bunch(gaussian_coords(n, seed=31), intensity=1e14)
+        lat = ring_lattice(24)
+        calc = spacecharge.SpaceChargeCalc2p5Drb(H, W, 64)
+        if use:
+            calc.setDenoiser(dn)
+        setSC2p5DrbAccNodes(lat, 0.49, calc, PIPE_RADIUS)
+        tw = BunchTwissAnalysis()
+        tw.analyzeBunch(b)
+        e0 = (tw.getEmittance(0), tw.getEmittance(1))
+        N.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(args.turns):
+            lat.trackBunch(b)
+        N.synchronize()
+        dt = time.perf_counter() - t0
+        tw.analyzeBunch(b)
+        e1 = (tw.getEmittance(0), tw.getEmittance(1))
+        c = b.getCoordinates()
+        ax = (c[0] - c[0].mean()) / c[0].std()
+        ay = (c[2] - c[2].mean()) / c[2].std()
+        res[name] = {"particles": n, "eps_growth_x": e1[0] / e0[0] - 1.0,
+                     "eps_growth_y": e1[1] / e0[1] - 1.0,
+                     "halo_beyond_4sigma": float(np.mean(ax * ax + ay * ay > 16.0)),
+                     "seconds": round(dt, 3)}
+    ref = res["plain_high_n"]
+    for name in ("plain_low_n", "denoised_low_n"):
+        r = res[name]
+        r["eps_growth_err_x"] = r["eps_growth_x"] - ref["eps_growth_x"]
+        r["eps_growth_err_y"] = r["eps_growth_y"] - ref["eps_growth_y"]
+    res["turns"] = args.turns
+    return res
 
 
 def main():
