@@ -53,6 +53,8 @@ def parse():
     ap.add_argument("--epochs", type=int, default=2, help="denoise: training epochs")
     ap.add_argument("--dn-scales", type=int, default=5, help="denoise: loss pyramid scales")
     ap.add_argument("--turns", type=int, default=5, help="denoise: ring turns of the tracking test")
+    ap.add_argument("--capture-turns", type=int, default=5,
+                    help="denoise: ring turns whose SC nodes add training pairs")
     ap.add_argument("--n", type=int, default=0, help="override particle count")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--cpu-threads", type=int, default=0)
@@ -299,21 +301,33 @@ def main_denoise(args):
     H = W = 256
     n_low, n_high = args.n or 100_000, 16_000_000
     ntrain, ntest, batch = args.pairs, 64, 32
+    def note(msg):
+        print(f"[denoise] {msg}", file=sys.stderr, flush=True)
+
     t0 = time.perf_counter()
-    train = PairSet(H, W, ntrain, n_low, n_high, seed=20261202)
+    cap_pairs = capture_count(args)
+    train = PairSet(H, W, capacity=ntrain + cap_pairs)
+    train.generate(ntrain, n_low, n_high, seed=20261202)
     test = PairSet(H, W, ntest, n_low, n_high, seed=777)
     N.synchronize()
     t_gen = time.perf_counter() - t0
+    note(f"{ntrain + ntest} generated pairs in {t_gen:.1f} s")
+    t0 = time.perf_counter()
+    if cap_pairs:
+        capture_tracking_pairs(args, train, H, W, n_low, n_high)
+    t_cap = time.perf_counter() - t0
+    note(f"{train.npairs - ntrain} pairs captured from tracking in {t_cap:.1f} s")
     dn = Denoiser(H, W, max_batch=batch)
     dn.setParams(Denoiser.initParams(1))
     dn.setLoss(args.dn_scales, 1.0)
-    steps_per_epoch = ntrain // batch
-    dn.train(train, epochs=1, batch=batch, lr=1e-3, npairs=batch * 4, log=None)  # warm-up
+    steps_per_epoch = train.npairs // batch
+    dn.train(train, epochs=1, batch=batch, lr=1e-3, npairs=batch * 4, log=None,
+             shuffle=True)  # warm-up
     dn.setParams(Denoiser.initParams(1))
     N.synchronize()
     t0 = time.perf_counter()
     hist = dn.train(train, epochs=args.epochs, batch=batch, lr=1e-3, lr_final=1e-4,
-                    log_every=max(1, steps_per_epoch // 4), log=None)
+                    log_every=max(1, steps_per_epoch // 4), log=note, shuffle=True, seed=9)
     N.synchronize()
     t_train = time.perf_counter() - t0
     nsteps = args.epochs * steps_per_epoch
@@ -364,7 +378,8 @@ def main_denoise(args):
         "steps": nsteps, "warmup": 4, "higher_is_better": False, "dtype": "bf16",
         "data": "synthetic", "vs_baseline": None,
         "config": {"workload": "denoise", "grid": [H, W], "n_low": n_low, "n_high": n_high,
-                   "train_pairs": ntrain, "test_pairs": ntest, "epochs": args.epochs,
+                   "train_pairs": ntrain, "tracking_pairs": train.npairs - ntrain,
+                   "test_pairs": ntest, "epochs": args.epochs,
                    "batch": batch, "params": dn.nparams, "loss_scales": args.dn_scales},
         "field_rel_l2": {"plain_low_n_median": round(float(np.median(ep)), 5),
                          "denoised_median": round(float(np.median(ef)), 5),
@@ -372,7 +387,8 @@ def main_denoise(args):
                          "denoised_p90": round(float(np.percentile(ef, 90)), 5)},
         "rho_rel_l2": {"plain_low_n_median": round(float(np.median(rp)), 4),
                        "denoised_median": round(float(np.median(rf)), 4)},
-        "train": {"pair_generation_s": round(t_gen, 2), "train_s": round(t_train, 2),
+        "train": {"pair_generation_s": round(t_gen, 2), "pair_capture_s": round(t_cap, 2),
+                  "train_s": round(t_train, 2),
                   "samples_per_s": round(batch * nsteps / t_train, 1),
                   "loss_first": hist[0][1] if hist else None,
                   "loss_last": hist[-1][1] if hist else None},
@@ -387,6 +403,28 @@ def main_denoise(args):
         "cpu_baseline": None,
     }
     print(json.dumps(out_line))
+
+
+def capture_count(args):
+    """Training pairs taken from tracking: one per SC node of args.capture_turns turns."""
+    return 320 * args.capture_turns
+
+
+def capture_tracking_pairs(args, pairs, H, W, n_low, n_high):
+    """Track an n_high-particle Gaussian bunch (intensity 1e14, another seed than the
+    tracking comparison) through the ring with the plain calculator and add a pair at
+    every SC node: every (n_high / n_low)-th particle against all of them."""
+    import spacecharge
+    from denoiser import CapturingCalc
+    from orbit.space_charge.sc2p5d import setSC2p5DrbAccNodes
+    from workloads import PIPE_RADIUS, gaussian_coords, make_bunch, ring_lattice
+    b = make_bunch(gaussian_coords(n_high, seed=41), intensity=1e14)
+    lat = ring_lattice(24)
+    calc = CapturingCalc(spacecharge.SpaceChargeCalc2p5Drb(H, W, 64), pairs,
+                         stride=max(1, n_high // n_low))
+    setSC2p5DrbAccNodes(lat, 0.49, calc, PIPE_RADIUS)
+    for _ in range(args.capture_turns):
+        lat.trackBunch(b)
 
 
 def track_with_denoiser(args, dn, H, W, n_low, n_high):

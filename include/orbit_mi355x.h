@@ -234,6 +234,13 @@ int ob_dn_set_loss(ob_denoiser* dn, int nscales, const float* lam);
  * box around the low-N sample), both normalized to mean 1. */
 int ob_dn_make_pairs(int H, int W, int npairs, const float* specs, uint64_t seed,
                      int64_t n_low, int64_t n_high, float* x_dev, float* y_dev);
+/* One training pair from a tracked bunch: x_dev gets the deposit of particles 0, stride,
+ * 2 stride, ... and y_dev that of all particles, on the square box around the sample. */
+int ob_dn_pair_from_bunch(int H, int W, ob_bunch* bunch, int64_t stride, float* x_dev,
+                          float* y_dev);
+/* dst image j <- src image idx[j] (< nsrc) for j < count (<= 64); images of n floats. */
+int ob_dn_gather(const float* src, int64_t nsrc, float* dst, int64_t n, const int* idx,
+                 int count);
 /* Device float buffers and copies (kind 0 host->device, 1 device->host, 2 device->device). */
 int ob_dn_alloc(int64_t nfloats, float** out);
 int ob_dn_free(float* p);

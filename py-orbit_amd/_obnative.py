@@ -98,6 +98,8 @@ SIGNATURES = {
     "ob_dn_last_loss": [_vp, _F],
     "ob_dn_set_loss": [_vp, _i, _F],
     "ob_dn_make_pairs": [_i, _i, _i, _F, C.c_uint64, _i64, _i64, _vp, _vp],
+    "ob_dn_pair_from_bunch": [_i, _i, _vp, _i64, _vp, _vp],
+    "ob_dn_gather": [_vp, _i64, _vp, _i64, C.POINTER(_i), _i],
     "ob_dn_alloc": [_i64, C.POINTER(_vp)],
     "ob_dn_free": [_vp],
     "ob_dn_copy": [_vp, _vp, _i64, _i],

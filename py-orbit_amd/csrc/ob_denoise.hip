@@ -561,6 +561,21 @@ __global__ __launch_bounds__(1024) void image_to_rho(const float* __restrict__ o
   for (int i = threadIdx.x; i < n; i += blockDim.x) rho[i] = (double)out[i] * scale;
 }
 
+// dst image j <- src image idx[j] (j < count), images of n floats.
+struct GatherIdx {
+  int32_t idx[64];
+};
+__global__ __launch_bounds__(kThreads) void gather_images(const float* __restrict__ src,
+                                                          float* __restrict__ dst, int64_t n,
+                                                          GatherIdx g) {
+  const int j = blockIdx.y;
+  const float* a = src + (int64_t)g.idx[j] * n;
+  float* b = dst + (int64_t)j * n;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x)
+    b[i] = a[i];
+}
+
 // ---- training pairs ------------------------------------------------------------------------
 // Counter-based uniform variates: splitmix64 of (seed, pair, particle, draw).
 __device__ __forceinline__ uint64_t mix64(uint64_t z) {
@@ -599,17 +614,37 @@ __device__ __forceinline__ void particle(const PairSpec& s, uint64_t seed, int k
   y = s.sn * px + s.cs * py;
 }
 
+// Particles of a pair come from the generator, or (bx != nullptr) are particles 0, stride,
+// 2 stride, ... of a bunch's x and y arrays.
+struct PairSource {
+  const PairSpec* spec;
+  uint64_t seed;
+  const double* bx;
+  const double* by;
+};
+
+__device__ __forceinline__ void source_particle(const PairSource& src, const PairSpec& s, int k,
+                                                int which, int64_t i, int64_t stride, float& x,
+                                                float& y) {
+  if (src.bx) {
+    x = (float)src.bx[i * stride];
+    y = (float)src.by[i * stride];
+  } else {
+    particle(s, src.seed, k, which, i, x, y);
+  }
+}
+
 // Extrema of the low-N sample of each pair: grid (pairs, blocks per pair).
-__global__ __launch_bounds__(kThreads) void pair_extrema(const PairSpec* __restrict__ spec,
-                                                         uint64_t seed, int64_t n,
+__global__ __launch_bounds__(kThreads) void pair_extrema(PairSource src, int64_t n,
+                                                         int64_t stride,
                                                          float* __restrict__ partial) {
   const int k = blockIdx.y;
-  const PairSpec s = spec[k];
+  const PairSpec s = src.bx ? PairSpec{} : src.spec[k];
   float lo[2] = {FLT_MAX, FLT_MAX}, hi[2] = {-FLT_MAX, -FLT_MAX};
   for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
        i += (int64_t)gridDim.x * blockDim.x) {
     float x, y;
-    particle(s, seed, k, 0, i, x, y);
+    source_particle(src, s, k, 0, i, stride, x, y);
     lo[0] = fminf(lo[0], x);
     hi[0] = fmaxf(hi[0], x);
     lo[1] = fminf(lo[1], y);
@@ -655,14 +690,14 @@ __global__ void pair_grid(const float* __restrict__ partial, int nblk, int npair
 // 1).  The grid is split into row slabs held in LDS; every slab block regenerates the
 // particles of its chunk and keeps the contributions that land in its slab.
 // grid: (chunks, slabs, pairs).
-__global__ __launch_bounds__(kThreads) void pair_deposit(const PairSpec* __restrict__ spec,
-                                                         uint64_t seed, int which, int64_t n,
+__global__ __launch_bounds__(kThreads) void pair_deposit(PairSource src, int which, int64_t n,
+                                                         int64_t stride,
                                                          const float* __restrict__ geo, int H,
                                                          int W, int slab_rows,
                                                          float* __restrict__ out) {
   extern __shared__ float acc[];
   const int k = blockIdx.z;
-  const PairSpec s = spec[k];
+  const PairSpec s = src.bx ? PairSpec{} : src.spec[k];
   const float xmin = geo[4 * k], dx = geo[4 * k + 1], ymin = geo[4 * k + 2], dy = geo[4 * k + 3];
   const int r0 = blockIdx.y * slab_rows, r1 = min(H, r0 + slab_rows);
   const int cells = (r1 - r0) * W;
@@ -673,7 +708,7 @@ __global__ __launch_bounds__(kThreads) void pair_deposit(const PairSpec* __restr
   const int64_t i0 = (int64_t)blockIdx.x * per, i1 = min(n, i0 + per);
   for (int64_t i = i0 + threadIdx.x; i < i1; i += blockDim.x) {
     float x, y;
-    particle(s, seed, k, which, i, x, y);
+    source_particle(src, s, k, which, i, stride, x, y);
     const float fx = (x - xmin) / dx, fy = (y - ymin) / dy;
     if (!(fx >= 0.f && fx <= (float)(H - 1) && fy >= 0.f && fy <= (float)(W - 1))) continue;
     int ix = (int)(fx + 0.5f), iy = (int)(fy + 0.5f);
@@ -1200,29 +1235,18 @@ OB_API int ob_dn_last_loss(ob_denoiser* d, float* loss) {
 // Writes x[k] (n_low particles) and y[k] (n_high particles) deposited on the grid of the
 // low-N sample, both normalized to mean 1 over the H x W grid.  x, y: device buffers of
 // npairs * H * W floats (may be allocated with ob_dn_alloc).
-OB_API int ob_dn_make_pairs(int H, int W, int npairs, const float* specs, uint64_t seed,
-                            int64_t n_low, int64_t n_high, float* x_dev, float* y_dev) {
-  OB_TRY OB_REQUIRE(specs != nullptr && x_dev != nullptr && y_dev != nullptr, "null pointer");
-  OB_REQUIRE(H >= 4 && W >= 4 && npairs >= 1 && n_low >= 2 && n_high >= 1, "invalid pair sizes");
-  ob::ensure_init();
+namespace {
+// Extrema -> square grid -> the two deposits, for npairs pairs from `src`.
+void make_pairs(int H, int W, int npairs, const ob::dn::PairSource& src, int64_t n_low,
+                int64_t low_stride, int64_t n_high, float* x_dev, float* y_dev) {
   hipStream_t s = ob::stream();
-  std::vector<ob::dn::PairSpec> hs(npairs);
-  for (int k = 0; k < npairs; k++) {
-    const float* p = specs + 5 * k;
-    hs[k] = ob::dn::PairSpec{(int32_t)p[0], p[1], p[2], p[3], p[4]};
-    OB_REQUIRE(hs[k].family >= 0 && hs[k].family <= 3, "invalid distribution family");
-  }
   const int nblk = 64;
-  ob::dn::PairSpec* dspec = nullptr;
   float *part = nullptr, *geo = nullptr;
-  OB_HIP(hipMalloc((void**)&dspec, sizeof(ob::dn::PairSpec) * npairs));
+  OB_HIP(hipMalloc((void**)&part, sizeof(float) * 4 * nblk * npairs));
   try {
-    OB_HIP(hipMalloc((void**)&part, sizeof(float) * 4 * nblk * npairs));
     OB_HIP(hipMalloc((void**)&geo, sizeof(float) * 4 * npairs));
-    OB_HIP(hipMemcpyAsync(dspec, hs.data(), sizeof(ob::dn::PairSpec) * npairs,
-                          hipMemcpyHostToDevice, s));
     hipLaunchKernelGGL(ob::dn::pair_extrema, dim3(nblk, npairs), dim3(ob::dn::kThreads), 0, s,
-                       dspec, seed, n_low, part);
+                       src, n_low, low_stride, part);
     hipLaunchKernelGGL(ob::dn::pair_grid, dim3((npairs + 63) / 64), dim3(64), 0, s, part, nblk,
                        npairs, H, W, geo);
     const size_t n = (size_t)npairs * H * W;
@@ -1233,22 +1257,79 @@ OB_API int ob_dn_make_pairs(int H, int W, int npairs, const float* specs, uint64
     const size_t lds = sizeof(float) * slab_rows * W;
     for (int which = 0; which < 2; which++) {
       const int64_t np = which ? n_high : n_low;
+      const int64_t stride = which ? 1 : low_stride;
       const int chunks = (int)std::max<int64_t>(1, std::min<int64_t>(np / 16384, 256));
-      hipLaunchKernelGGL(ob::dn::pair_deposit, dim3(chunks, slabs, npairs), dim3(ob::dn::kThreads),
-                         lds, s, dspec, seed, which, np, geo, H, W, slab_rows,
-                         which ? y_dev : x_dev);
+      hipLaunchKernelGGL(ob::dn::pair_deposit, dim3(chunks, slabs, npairs),
+                         dim3(ob::dn::kThreads), lds, s, src, which, np, stride, geo, H, W,
+                         slab_rows, which ? y_dev : x_dev);
     }
     OB_HIP(hipGetLastError());
     OB_HIP(hipStreamSynchronize(s));
   } catch (...) {
-    (void)hipFree(dspec);
     (void)hipFree(part);
     (void)hipFree(geo);
     throw;
   }
-  (void)hipFree(dspec);
   (void)hipFree(part);
   (void)hipFree(geo);
+}
+}  // namespace
+
+OB_API int ob_dn_make_pairs(int H, int W, int npairs, const float* specs, uint64_t seed,
+                            int64_t n_low, int64_t n_high, float* x_dev, float* y_dev) {
+  OB_TRY OB_REQUIRE(specs != nullptr && x_dev != nullptr && y_dev != nullptr, "null pointer");
+  OB_REQUIRE(H >= 4 && W >= 4 && npairs >= 1 && n_low >= 2 && n_high >= 1, "invalid pair sizes");
+  ob::ensure_init();
+  std::vector<ob::dn::PairSpec> hs(npairs);
+  for (int k = 0; k < npairs; k++) {
+    const float* p = specs + 5 * k;
+    hs[k] = ob::dn::PairSpec{(int32_t)p[0], p[1], p[2], p[3], p[4]};
+    OB_REQUIRE(hs[k].family >= 0 && hs[k].family <= 3, "invalid distribution family");
+  }
+  ob::dn::PairSpec* dspec = nullptr;
+  OB_HIP(hipMalloc((void**)&dspec, sizeof(ob::dn::PairSpec) * npairs));
+  try {
+    OB_HIP(hipMemcpy(dspec, hs.data(), sizeof(ob::dn::PairSpec) * npairs, hipMemcpyHostToDevice));
+    make_pairs(H, W, npairs, ob::dn::PairSource{dspec, seed, nullptr, nullptr}, n_low, 1, n_high,
+               x_dev, y_dev);
+  } catch (...) {
+    (void)hipFree(dspec);
+    throw;
+  }
+  (void)hipFree(dspec);
+  OB_CATCH
+}
+
+// One training pair from a bunch: x_dev gets particles 0, stride, 2 stride, ... (the
+// low-N sample), y_dev all particles, both on the square grid around the low-N sample
+// (as the generator does).  Pending maps of the bunch are applied first.
+OB_API int ob_dn_pair_from_bunch(int H, int W, ob_bunch* b, int64_t stride, float* x_dev,
+                                 float* y_dev) {
+  OB_TRY OB_REQUIRE(b != nullptr && x_dev != nullptr && y_dev != nullptr, "null pointer");
+  OB_REQUIRE(H >= 4 && W >= 4 && stride >= 1, "invalid pair sizes");
+  ob::bunch_flush(b, nullptr);
+  const ob::DeviceBunchView v = ob::bunch_device_view(b);
+  const int64_t n_low = (v.n + stride - 1) / stride;
+  OB_REQUIRE(n_low >= 2, "denoiser: the low-N sample needs at least 2 particles");
+  make_pairs(H, W, 1, ob::dn::PairSource{nullptr, 0, v.c[0], v.c[2]}, n_low, stride, v.n,
+             x_dev, y_dev);
+  OB_CATCH
+}
+
+// dst image j <- src image idx[j], j < count <= 64, images of n floats (device buffers);
+// src holds nsrc images.
+OB_API int ob_dn_gather(const float* src, int64_t nsrc, float* dst, int64_t n, const int* idx,
+                        int count) {
+  OB_TRY OB_REQUIRE(src != nullptr && dst != nullptr && idx != nullptr, "null pointer");
+  OB_REQUIRE(count >= 1 && count <= 64 && n >= 1, "gather: 1..64 images");
+  ob::dn::GatherIdx g;
+  for (int j = 0; j < count; j++) {
+    OB_REQUIRE(idx[j] >= 0 && idx[j] < nsrc, "gather: image index out of range");
+    g.idx[j] = idx[j];
+  }
+  hipLaunchKernelGGL(ob::dn::gather_images, dim3(std::max(1, (int)std::min<int64_t>(64, n / 4096 + 1)), count),
+                     dim3(ob::dn::kThreads), 0, ob::stream(), src, dst, n, g);
+  OB_HIP(hipGetLastError());
   OB_CATCH
 }
 

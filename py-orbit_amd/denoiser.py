@@ -74,28 +74,76 @@ def pair_specs(npairs, seed, families=(0, 1, 2, 3), aspect=(0.5, 2.0)):
 
 
 class PairSet:
-    """Training pairs resident on the device: x[k] (n_low-particle deposit) and y[k]
-    (n_high-particle deposit on the same grid), both [H][W] float32 of mean 1."""
+    """Training pairs resident on the device: x[k] (low-N deposit) and y[k] (high-N deposit
+    on the same grid), both [H][W] float32 of mean 1.
 
-    def __init__(self, H, W, npairs, n_low, n_high, seed, chunk=256, **spec_kw):
-        self.H, self.W, self.npairs = H, W, npairs
-        self.n_low, self.n_high, self.seed = n_low, n_high, seed
-        self.specs = pair_specs(npairs, seed, **spec_kw)
+    PairSet(H, W, npairs, n_low, n_high, seed) generates npairs pairs of the four
+    distribution families.  PairSet(H, W, capacity=C) makes an empty set to which
+    generate() adds generated pairs and capture() adds pairs taken from a bunch being
+    tracked (its every stride-th particle against all of them)."""
+
+    def __init__(self, H, W, npairs=0, n_low=None, n_high=None, seed=0, capacity=None,
+                 chunk=256, **spec_kw):
+        self.H, self.W = H, W
+        self.capacity = capacity if capacity is not None else npairs
+        if self.capacity < 1:
+            raise ValueError("a pair set needs room for at least one pair")
         img = H * W
-        self.x = DeviceArray(npairs * img)
-        self.y = DeviceArray(npairs * img)
+        self.x = DeviceArray(self.capacity * img)
+        self.y = DeviceArray(self.capacity * img)
+        self.npairs = 0
+        self.specs = np.zeros((0, 5), dtype=np.float32)
+        if npairs:
+            self.generate(npairs, n_low, n_high, seed, chunk=chunk, **spec_kw)
+
+    def _room(self, n):
+        if self.npairs + n > self.capacity:
+            raise ValueError(f"pair set full ({self.capacity} pairs)")
+
+    def generate(self, npairs, n_low, n_high, seed, chunk=256, **spec_kw):
+        self._room(npairs)
+        img = self.H * self.W
+        specs = pair_specs(npairs, seed, **spec_kw)
         for k0 in range(0, npairs, chunk):
             k1 = min(npairs, k0 + chunk)
-            sp = _f32(self.specs[k0:k1])
-            # pair index k0 + j draws the particles of pair j of the chunk with seed + k0
-            N.call("ob_dn_make_pairs", H, W, k1 - k0, N.fptr(sp), int(seed + 7919 * k0) & (2**64 - 1),
-                   int(n_low), int(n_high), self.x.at(k0 * img), self.y.at(k0 * img))
+            sp = _f32(specs[k0:k1])
+            at = (self.npairs + k0) * img
+            # pair j of a chunk draws its particles with (seed + 7919 k0, j)
+            N.call("ob_dn_make_pairs", self.H, self.W, k1 - k0, N.fptr(sp),
+                   int(seed + 7919 * k0) & (2 ** 64 - 1), int(n_low), int(n_high),
+                   self.x.at(at), self.y.at(at))
+        self.specs = np.concatenate([self.specs, specs])
+        self.npairs += npairs
+
+    def capture(self, bunch, stride):
+        """Add the pair (every stride-th particle, all particles) of a bunch as it is now."""
+        self._room(1)
+        at = self.npairs * self.H * self.W
+        N.call("ob_dn_pair_from_bunch", self.H, self.W, bunch._h, int(stride), self.x.at(at),
+               self.y.at(at))
+        self.specs = np.concatenate([self.specs, np.full((1, 5), -1, dtype=np.float32)])
+        self.npairs += 1
 
     def host(self, k0, k1):
         img = self.H * self.W
         xs = self.x.download((k1 - k0) * img, k0 * img).reshape(k1 - k0, self.H, self.W)
         ys = self.y.download((k1 - k0) * img, k0 * img).reshape(k1 - k0, self.H, self.W)
         return xs, ys
+
+
+class CapturingCalc:
+    """Stands in for a SpaceChargeCalc2p5Drb in a lattice: every `every`-th trackBunch call
+    first adds a training pair of the bunch to `pairs`, then runs the calculator."""
+
+    def __init__(self, calc, pairs, stride, every=1):
+        self.calc, self.pairs, self.stride, self.every = calc, pairs, stride, every
+        self.calls = 0
+
+    def trackBunch(self, bunch, *args):
+        if self.calls % self.every == 0 and self.pairs.npairs < self.pairs.capacity:
+            self.pairs.capture(bunch, self.stride)
+        self.calls += 1
+        return self.calc.trackBunch(bunch, *args)
 
 
 class Denoiser:
@@ -181,26 +229,45 @@ class Denoiser:
         N.call("ob_dn_train_step_dev", self._h, data.x.at(k0 * img), data.y.at(k0 * img),
                int(batch), float(lr))
 
+    def trainStepIndices(self, data, idx, lr):
+        """One Adam step on the pairs idx of `data` (gathered into a batch buffer)."""
+        img = self.H * self.W
+        b = len(idx)
+        if getattr(self, "_bx", None) is None:
+            self._bx = DeviceArray(self.max_batch * img)
+            self._by = DeviceArray(self.max_batch * img)
+        ii = np.ascontiguousarray(idx, dtype=np.int32)
+        pi = ii.ctypes.data_as(C.POINTER(C.c_int))
+        N.call("ob_dn_gather", data.x.ptr, data.npairs, self._bx.ptr, img, pi, b)
+        N.call("ob_dn_gather", data.y.ptr, data.npairs, self._by.ptr, img, pi, b)
+        N.call("ob_dn_train_step_dev", self._h, self._bx.ptr, self._by.ptr, b, float(lr))
+
     def lastLoss(self):
         v = np.zeros(1, dtype=np.float32)
         N.call("ob_dn_last_loss", self._h, N.fptr(v))
         return float(v[0])
 
     def train(self, data, epochs=1, batch=32, lr=1e-3, lr_final=None, log_every=0,
-              log=print, npairs=None):
-        """Adam over the pairs of `data` in order, `epochs` times, with the learning rate
-        decayed geometrically to lr_final (default lr / 10).  Returns the loss of every
-        log_every-th step (all steps if 0 is given as None)."""
+              log=print, npairs=None, shuffle=False, seed=0):
+        """Adam over the pairs of `data`, `epochs` times, with the learning rate decayed
+        geometrically to lr_final (default lr / 10); in order, or in a new random order
+        every epoch (shuffle).  Returns (step, loss) every log_every-th step."""
         npairs = data.npairs if npairs is None else npairs
         steps_per_epoch = npairs // batch
         total = epochs * steps_per_epoch
         lr_final = lr / 10 if lr_final is None else lr_final
+        rng = np.random.default_rng(seed)
         hist = []
         step = 0
         for ep in range(epochs):
+            order = rng.permutation(npairs) if shuffle else None
             for s in range(steps_per_epoch):
                 frac = step / max(1, total - 1)
-                self.trainStep(data, s * batch, batch, lr * (lr_final / lr) ** frac)
+                rate = lr * (lr_final / lr) ** frac
+                if shuffle:
+                    self.trainStepIndices(data, order[s * batch:(s + 1) * batch], rate)
+                else:
+                    self.trainStep(data, s * batch, batch, rate)
                 step += 1
                 if log_every and step % log_every == 0:
                     hist.append((step, self.lastLoss()))
