@@ -38,6 +38,9 @@ typedef float f4 __attribute__((ext_vector_type(4)));
 constexpr int kTile = 16;
 constexpr int kHalo = kTile + 2;
 constexpr int kThreads = 256;
+constexpr int kWgradBlocks = 512;                          // weight-gradient workgroups
+constexpr size_t kWpartFloats = (size_t)kWgradBlocks * 9 * 16 * 64 + 6 * 9 * 16 * 64;
+constexpr int kHeadBlocks = 512;                           // head / loss workgroups
 
 __device__ __forceinline__ float bf2f(bf16 h) { return __uint_as_float((uint32_t)h << 16); }
 // Round to nearest, ties to even (finite inputs).
@@ -73,6 +76,8 @@ struct ConvArgs {
   const bf16* y;       // the layer output (ReLU mask), channel stride Cout
   float* dw;           // [9][Cin][Cout]
   float* db;           // [Cout]
+  float* wpart;        // per-workgroup partial sums of the weight-gradient pass
+  float* bpart;
 };
 
 __device__ __forceinline__ void unpack8(const uint4& v, float f[8]) {
@@ -285,25 +290,55 @@ __global__ __launch_bounds__(kThreads) void conv3x3_wgrad(ConvArgs a, int ntiles
         }
       }
     }
-    if (a.db && cc == 0 && threadIdx.x < COUT * 4) {
+    if (cc == 0 && threadIdx.x < COUT * 4) {
       const int co = threadIdx.x >> 2, part = threadIdx.x & 3;
       for (int r = part * 4; r < part * 4 + 4; r++)
         for (int c = 0; c < kTile; c++) bsum += bf2f(ds[co][r][c]);
     }
   }
+  // Partial sums of this workgroup (every element written once: no atomics, which would
+  // pile up on the same few thousand addresses from every workgroup); conv3x3_wgrad_reduce
+  // adds them up.  Layout [chunk][workgroup][tap][16 ci][COUT].
+  float* wp = a.wpart + ((size_t)cc * gridDim.x + blockIdx.x) * (9 * 16 * COUT);
 #pragma unroll
   for (int k = 0; k < NP; k++) {
     const int p = wv + 4 * k;
     if (p < 9 * MT) {
       const int tap = p / MT, mt = p - tap * MT;
-      const int ci = cc * 16 + l16;
-      if (ci < a.Cin)
 #pragma unroll
-        for (int j = 0; j < 4; j++)
-          atomicAdd(&a.dw[((size_t)tap * a.Cin + ci) * a.Cout + mt * 16 + 4 * q + j], acc[k][j]);
+      for (int j = 0; j < 4; j++) wp[(tap * 16 + l16) * COUT + mt * 16 + 4 * q + j] = acc[k][j];
     }
   }
-  if (a.db && cc == 0 && threadIdx.x < COUT * 4) atomicAdd(&a.db[threadIdx.x >> 2], bsum);
+  if (cc == 0) {  // threads 4 co .. 4 co + 3 hold quarters of the bias sum of channel co
+    bsum += __shfl_xor(bsum, 1, 64);
+    bsum += __shfl_xor(bsum, 2, 64);
+    if (threadIdx.x < COUT * 4 && (threadIdx.x & 3) == 0)
+      a.bpart[(size_t)blockIdx.x * COUT + (threadIdx.x >> 2)] = bsum;
+  }
+}
+
+// dW[tap][ci][co] = sum over workgroups of the partials; db[co] likewise (chunk 0).
+__global__ __launch_bounds__(kThreads) void conv3x3_wgrad_reduce(const float* __restrict__ wpart,
+                                                                 const float* __restrict__ bpart,
+                                                                 int nblk, int Cin, int Cout,
+                                                                 float* __restrict__ dw,
+                                                                 float* __restrict__ db) {
+  const int nw = 9 * Cin * Cout;
+  for (int e = blockIdx.x * blockDim.x + threadIdx.x; e < nw + Cout; e += gridDim.x * blockDim.x) {
+    float s = 0.f;
+    if (e < nw) {
+      const int co = e % Cout, r = e / Cout;
+      const int ci = r % Cin, tap = r / Cin;
+      const int cc = ci / 16, l = ci % 16;
+      const float* p = wpart + (size_t)cc * nblk * (9 * 16 * Cout) + (tap * 16 + l) * Cout + co;
+      for (int b = 0; b < nblk; b++) s += p[(size_t)b * (9 * 16 * Cout)];
+      dw[e] = s;
+    } else {
+      const int co = e - nw;
+      for (int b = 0; b < nblk; b++) s += bpart[(size_t)b * Cout + co];
+      db[co] = s;
+    }
+  }
 }
 
 // ---- head, loss, gradient plumbing ----------------------------------------------------
@@ -381,8 +416,7 @@ __global__ __launch_bounds__(kThreads) void head_bwd(const bf16* __restrict__ d1
                                                      const float* __restrict__ y,
                                                      const float* __restrict__ hw,
                                                      bf16* __restrict__ dd1,
-                                                     float* __restrict__ dhw,
-                                                     float* __restrict__ loss, int64_t npix,
+                                                     float* __restrict__ part, int64_t npix,
                                                      Pyramid P) {
   __shared__ float sh[kThreads / 64];
   float acc[17];
@@ -416,12 +450,27 @@ __global__ __launch_bounds__(kThreads) void head_bwd(const bf16* __restrict__ d1
     *(uint4*)(dd1 + p * 16) = pack8(o);
     *(uint4*)(dd1 + p * 16 + 8) = pack8(o + 8);
   }
+  // per-workgroup partials [block][18]: 17 head gradients, then the loss
   for (int c = 0; c < 17; c++) {
     const float s = block_sum(acc[c], sh);
-    if (threadIdx.x == 0) atomicAdd(&dhw[c], s);
+    if (threadIdx.x == 0) part[blockIdx.x * 18 + c] = s;
   }
   const float s = block_sum(l, sh);
-  if (threadIdx.x == 0) atomicAdd(loss, s * inv);
+  if (threadIdx.x == 0) part[blockIdx.x * 18 + 17] = s * inv;
+}
+
+// dhw[c] += sum of the partials, loss += sum of the loss partials.  One block of 64.
+__global__ __launch_bounds__(64) void head_reduce(const float* __restrict__ part, int nblk,
+                                                  float* __restrict__ dhw,
+                                                  float* __restrict__ loss) {
+  const int c = threadIdx.x;
+  if (c >= 18) return;
+  float s = 0.f;
+  for (int b = 0; b < nblk; b++) s += part[b * 18 + c];
+  if (c < 17)
+    dhw[c] += s;
+  else
+    *loss += s;
 }
 
 // g[b, y, x, c] = 0.25 gp[b, y/2, x/2, c] + gs[b, y, x, soff + c]   (C channels, H x W)
@@ -771,6 +820,9 @@ struct ob_denoiser {
   bf16* gtmp1 = nullptr;
   float* out = nullptr;   // head output, FP32
   float* loss = nullptr;  // device scalar
+  float* wpart = nullptr;  // weight-gradient partials (kWpartFloats)
+  float* bpart = nullptr;  // bias-gradient partials (kWgradBlocks * 64)
+  float* hpart = nullptr;  // head / loss partials (kHeadBlocks * 18)
   float* pyr = nullptr;   // error pyramid of the multi-scale loss
   int nscales = 0;
   float lam[9] = {};
@@ -799,18 +851,31 @@ void launch_conv(const ConvArgs& a, int B, hipStream_t s) {
     hipLaunchKernelGGL(conv3x3_fwd<1>, grid, dim3(kThreads), 0, s, a);
 }
 
+// Workgroups of the weight-gradient pass: about two per CU in all, split over the chunks.
+int wgrad_blocks(int ntiles, int chunks) {
+  return std::max(1, std::min(ntiles, (kWgradBlocks + chunks - 1) / chunks));
+}
+
 void launch_wgrad(const ConvArgs& a, int B, hipStream_t s) {
   const int ntiles = (a.H / kTile) * (a.W / kTile) * B;
   const int chunks = (a.Cin + 15) / 16;
-  int blocks = std::max(1, std::min(ntiles, (4 * device_cu_count() + chunks - 1) / chunks));
+  const int blocks = wgrad_blocks(ntiles, chunks);
+  OB_REQUIRE((size_t)chunks * blocks * 9 * 16 * a.Cout <= kWpartFloats,
+             "denoiser: weight-gradient partials exceed their buffer");
   const dim3 grid(blocks, chunks);
-  ProfScope ps("dn_wgrad");
-  switch (a.Cout / 16) {
-    case 1: hipLaunchKernelGGL(conv3x3_wgrad<1>, grid, dim3(kThreads), 0, s, a, ntiles); break;
-    case 2: hipLaunchKernelGGL(conv3x3_wgrad<2>, grid, dim3(kThreads), 0, s, a, ntiles); break;
-    case 4: hipLaunchKernelGGL(conv3x3_wgrad<4>, grid, dim3(kThreads), 0, s, a, ntiles); break;
-    default: OB_REQUIRE(false, "denoiser: unsupported output channel count");
+  {
+    ProfScope ps("dn_wgrad");
+    switch (a.Cout / 16) {
+      case 1: hipLaunchKernelGGL(conv3x3_wgrad<1>, grid, dim3(kThreads), 0, s, a, ntiles); break;
+      case 2: hipLaunchKernelGGL(conv3x3_wgrad<2>, grid, dim3(kThreads), 0, s, a, ntiles); break;
+      case 4: hipLaunchKernelGGL(conv3x3_wgrad<4>, grid, dim3(kThreads), 0, s, a, ntiles); break;
+      default: OB_REQUIRE(false, "denoiser: unsupported output channel count");
+    }
   }
+  ProfScope ps("dn_wgrad_reduce");
+  const int ne = 9 * a.Cin * a.Cout + a.Cout;
+  hipLaunchKernelGGL(conv3x3_wgrad_reduce, dim3((ne + kThreads - 1) / kThreads), dim3(kThreads), 0,
+                     s, a.wpart, a.bpart, blocks, a.Cin, a.Cout, a.dw, a.db);
 }
 
 int grid_for(int64_t n) {
@@ -897,9 +962,11 @@ void backward(ob_denoiser* d, const float* x, const float* target, int B, hipStr
   }
   {
     ProfScope ps("dn_head_bwd");
-    hipLaunchKernelGGL(head_bwd, dim3(grid_for(npix)), dim3(kThreads), 0, s, d->act[11], d->out,
-                       target, d->w + d->head_off, d->gact[11], d->g + d->head_off, d->loss,
-                       npix, P);
+    const int hb = (int)std::min<int64_t>(kHeadBlocks, (npix + kThreads - 1) / kThreads);
+    hipLaunchKernelGGL(head_bwd, dim3(hb), dim3(kThreads), 0, s, d->act[11], d->out, target,
+                       d->w + d->head_off, d->gact[11], d->hpart, npix, P);
+    hipLaunchKernelGGL(head_reduce, dim3(1), dim3(64), 0, s, d->hpart, hb, d->g + d->head_off,
+                       d->loss);
   }
   // gradient w.r.t. each layer's output arrives in gact[l] (before its ReLU mask)
   for (int l = 11; l >= 0; l--) {
@@ -917,6 +984,8 @@ void backward(ob_denoiser* d, const float* x, const float* target, int B, hipStr
     a.y = d->act[l];
     a.dw = d->g + L.w_off;
     a.db = d->g + L.b_off;
+    a.wpart = d->wpart;
+    a.bpart = d->bpart;
     launch_wgrad(a, B, s);
     if (l == 0) break;
     // data gradient: conv of dY (masked by Y > 0) with the flipped transposed kernel
@@ -1073,6 +1142,9 @@ OB_API int ob_dn_create(int H, int W, int max_batch, ob_denoiser** out) {
     OB_HIP(hipMalloc((void**)&d->loss, sizeof(float)));
     OB_HIP(hipMalloc((void**)&d->xin, sizeof(float) * H * W));
     OB_HIP(hipMalloc((void**)&d->pyr, sizeof(float) * (img / 3 + 1)));
+    OB_HIP(hipMalloc((void**)&d->wpart, sizeof(float) * ob::dn::kWpartFloats));
+    OB_HIP(hipMalloc((void**)&d->bpart, sizeof(float) * ob::dn::kWgradBlocks * 64));
+    OB_HIP(hipMalloc((void**)&d->hpart, sizeof(float) * ob::dn::kHeadBlocks * 18));
     OB_HIP(hipMalloc((void**)&d->total, sizeof(double)));
   } catch (...) {
     delete d;  // device buffers of a failed create are reclaimed at process exit
@@ -1088,7 +1160,7 @@ OB_API int ob_dn_destroy(ob_denoiser* d) {
   for (void* p : {(void*)d->w, (void*)d->g, (void*)d->m, (void*)d->v, (void*)d->pf, (void*)d->pt,
                   (void*)d->gcat4, (void*)d->gcat5, (void*)d->gtmp0, (void*)d->gtmp1,
                   (void*)d->out, (void*)d->loss, (void*)d->xin, (void*)d->total,
-                  (void*)d->pyr})
+                  (void*)d->pyr, (void*)d->wpart, (void*)d->bpart, (void*)d->hpart})
     if (p) (void)hipFree(p);
   for (int l = 0; l < 12; l++) {
     if (d->act[l]) (void)hipFree(d->act[l]);
