@@ -272,4 +272,37 @@ __device__ __forceinline__ void sc3d_kick(const Op& op, double x, double& xp, do
   dE += -ez * op.p[10];
 }
 
+// Exclusive block scan of two counters (blockDim.x a multiple of 64, at most 1024).
+__device__ __forceinline__ void block_scan2(uint32_t& a, uint32_t& b, uint32_t& ta,
+                                           uint32_t& tb) {
+  __shared__ uint32_t wa[16], wb[16];
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6, nw = blockDim.x >> 6;
+  uint32_t ia = a, ib = b;
+  for (int o = 1; o < 64; o <<= 1) {
+    const uint32_t xa = __shfl_up(ia, o, 64), xb = __shfl_up(ib, o, 64);
+    if (lane >= o) {
+      ia += xa;
+      ib += xb;
+    }
+  }
+  if (lane == 63) {
+    wa[w] = ia;
+    wb[w] = ib;
+  }
+  __syncthreads();
+  uint32_t pa = 0, pb = 0;
+  ta = 0;
+  tb = 0;
+  for (int k = 0; k < nw; k++) {
+    if (k < w) {
+      pa += wa[k];
+      pb += wb[k];
+    }
+    ta += wa[k];
+    tb += wb[k];
+  }
+  a = pa + ia - a;
+  b = pb + ib - b;
+}
+
 }  // namespace ob

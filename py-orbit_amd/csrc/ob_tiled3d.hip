@@ -92,31 +92,17 @@ __global__ __launch_bounds__(1024) void tile_scan(uint32_t* __restrict__ counts,
                                                   uint32_t* __restrict__ cursor,
                                                   int4* __restrict__ items,
                                                   uint32_t* __restrict__ nitems) {
-  __shared__ uint32_t part[1024], cpart[1024];
   const int nb = ntiles + 1;
   const int per = (nb + blockDim.x - 1) / blockDim.x;
-  const int lo = threadIdx.x * per, hi = min(nb, lo + per);
-  uint32_t s = 0, cs = 0;
+  const int lo = min(nb, (int)threadIdx.x * per), hi = min(nb, lo + per);
+  uint32_t off = 0, item = 0;
   for (int t = lo; t < hi; t++) {
-    s += counts[t];
-    if (t < ntiles) cs += (counts[t] + kChunk - 1) / kChunk;
+    off += counts[t];
+    if (t < ntiles) item += (counts[t] + kChunk - 1) / kChunk;
   }
-  part[threadIdx.x] = s;
-  cpart[threadIdx.x] = cs;
-  __syncthreads();
-  if (threadIdx.x == 0) {  // serial scan of 1024 partial sums
-    uint32_t a = 0, c = 0;
-    for (int k = 0; k < (int)blockDim.x; k++) {
-      const uint32_t va = part[k], vc = cpart[k];
-      part[k] = a;
-      cpart[k] = c;
-      a += va;
-      c += vc;
-    }
-    *nitems = c;
-  }
-  __syncthreads();
-  uint32_t off = part[threadIdx.x], item = cpart[threadIdx.x];
+  uint32_t total, total_items;
+  block_scan2(off, item, total, total_items);
+  if (threadIdx.x == 0) *nitems = total_items;
   for (int t = lo; t < hi; t++) {
     const uint32_t cnt = counts[t];
     offsets[t] = off;
