@@ -22,6 +22,7 @@ void set_error(const std::string& msg) { t_last_error = msg; }
 namespace {
 int g_device = -1;
 hipStream_t g_stream = nullptr;
+hipStream_t g_aux_stream = nullptr;
 int g_cus = 0;
 
 struct Watched {
@@ -34,6 +35,7 @@ std::vector<Watched> g_watch;
 }  // namespace
 
 hipStream_t stream() { return g_stream; }
+hipStream_t aux_stream() { return g_aux_stream; }
 int device_cu_count() { return g_cus; }
 
 static void init_device(int device) {
@@ -47,6 +49,7 @@ static void init_device(int device) {
   OB_REQUIRE(device >= 0 && device < n, "ob_init: device index out of range");
   OB_HIP(hipSetDevice(device));
   OB_HIP(hipStreamCreateWithFlags(&g_stream, hipStreamNonBlocking));
+  OB_HIP(hipStreamCreateWithFlags(&g_aux_stream, hipStreamNonBlocking));
   hipDeviceProp_t prop;
   OB_HIP(hipGetDeviceProperties(&prop, device));
   g_cus = prop.multiProcessorCount;
@@ -81,16 +84,16 @@ hipEvent_t prof_event() {
 
 bool profiling() { return g_prof; }
 
-void prof_mark(const char* name, bool begin) {
+void prof_mark(const char* name, bool begin, hipStream_t s) {
   if (begin) {
     ProfRec r{name, prof_event(), nullptr};
-    OB_HIP(hipEventRecord(r.a, g_stream));
+    OB_HIP(hipEventRecord(r.a, s));
     g_prof_recs.push_back(r);
   } else {
     for (size_t i = g_prof_recs.size(); i-- > 0;)
       if (g_prof_recs[i].b == nullptr && g_prof_recs[i].name == name) {
         g_prof_recs[i].b = prof_event();
-        OB_HIP(hipEventRecord(g_prof_recs[i].b, g_stream));
+        OB_HIP(hipEventRecord(g_prof_recs[i].b, s));
         return;
       }
   }
@@ -434,6 +437,7 @@ OB_API int ob_init(int device) {
 OB_API int ob_synchronize(void) {
   OB_TRY ob::ensure_init();
   OB_HIP(hipStreamSynchronize(ob::stream()));
+  OB_HIP(hipStreamSynchronize(ob::aux_stream()));
   ob::check_statuses(true);
   OB_CATCH
 }
@@ -441,6 +445,7 @@ OB_API int ob_synchronize(void) {
 OB_API int ob_profile_enable(int on) {
   OB_TRY ob::ensure_init();
   OB_HIP(hipStreamSynchronize(ob::stream()));
+  OB_HIP(hipStreamSynchronize(ob::aux_stream()));
   for (auto& r : ob::g_prof_recs) {
     ob::g_prof_pool.push_back(r.a);
     if (r.b) ob::g_prof_pool.push_back(r.b);
@@ -453,6 +458,7 @@ OB_API int ob_profile_enable(int on) {
 OB_API int ob_profile_report(char* buf, int buflen) {
   OB_TRY ob::ensure_init();
   OB_HIP(hipStreamSynchronize(ob::stream()));
+  OB_HIP(hipStreamSynchronize(ob::aux_stream()));
   std::vector<std::string> names;
   std::vector<double> tot, mn, mx;
   std::vector<long> cnt;

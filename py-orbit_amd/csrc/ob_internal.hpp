@@ -31,20 +31,24 @@ struct Error {
   } while (0)
 
 hipStream_t stream();
+// A second stream (default priority) for work that overlaps the main stream; joined back
+// with events before its results are used.
+hipStream_t aux_stream();
 int device_cu_count();
 void ensure_init();
 
 // Kernel timing with HIP events on the library stream (ob_profile_enable/_report).  A
 // ProfScope brackets one launch when profiling is on and costs nothing otherwise.
 bool profiling();
-void prof_mark(const char* name, bool begin);
+void prof_mark(const char* name, bool begin, hipStream_t s);
 struct ProfScope {
   const char* name;
-  explicit ProfScope(const char* n) : name(n) {
-    if (profiling()) prof_mark(name, true);
+  hipStream_t s;
+  explicit ProfScope(const char* n, hipStream_t st = nullptr) : name(n), s(st ? st : stream()) {
+    if (profiling()) prof_mark(name, true, s);
   }
   ~ProfScope() {
-    if (profiling()) prof_mark(name, false);
+    if (profiling()) prof_mark(name, false, s);
   }
 };
 

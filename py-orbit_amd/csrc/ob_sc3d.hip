@@ -486,6 +486,8 @@ struct ob_sc3d {
   bool tiled = true;              // tile-sorted LDS deposit (else particle-order atomics)
   int lds_budget = 16 * 1024;     // LDS bytes for one deposit / kick tile
   ob::TileWork work;
+  hipEvent_t ev_main = nullptr, ev_green = nullptr;  // Green rebuild on the second stream
+  bool green_pending = false;
 };
 
 namespace ob {
@@ -539,12 +541,11 @@ void launch_strided(ob_sc3d* c, int n, int n_in, int n_out, int64_t line_stride,
 
 // PoissonSolverFFT3D::setGridXYZ + _defineGreenF (:136-247)
 void set_solver_grid(ob_sc3d* c, double xMin, double xMax, double yMin, double yMax, double zMin,
-                     double zMax) {
+                     double zMax, hipStream_t s) {
   c->sMin[0] = xMin; c->sMax[0] = xMax; c->sd[0] = (xMax - xMin) / (c->nx - 1);
   c->sMin[1] = yMin; c->sMax[1] = yMax; c->sd[1] = (yMax - yMin) / (c->ny - 1);
   c->sMin[2] = zMin; c->sMax[2] = zMax; c->sd[2] = (zMax - zMin) / c->nz;
   GreenGeom gg{c->M0, c->M1, c->M2, c->sd[0], c->sd[1], c->sd[2], c->nbunches, c->lambda_cm};
-  hipStream_t s = stream();
   dispatch_pow2(c->M2, [&](auto NC) {
     constexpr int N = decltype(NC)::value;
     if constexpr (N <= 2048) {
@@ -556,7 +557,7 @@ void set_solver_grid(ob_sc3d* c, double xMin, double xMax, double yMin, double y
       }
       const int64_t pairs = ((int64_t)c->qx * c->qy + 1) / 2;
       const int64_t blocks = (pairs + zpass_pairs(N) - 1) / zpass_pairs(N);
-      ob::ProfScope ps_("green3d_z");
+      ob::ProfScope ps_("green3d_z", s);
       hipLaunchKernelGGL(green3d_z<N>, dim3((unsigned)blocks), dim3(kThreads), lds, s, c->qx,
                          c->qy, c->H2p, c->ghat, gg, c->tw, c->nmax);
     }
@@ -573,7 +574,7 @@ void set_solver_grid(ob_sc3d* c, double xMin, double xMax, double yMin, double y
           ready = true;
         }
         const int tiles = (c->H2 + strided_lines(N) - 1) / strided_lines(N);
-        ob::ProfScope ps_("green3d_axis");
+        ob::ProfScope ps_("green3d_axis", s);
         hipLaunchKernelGGL(green3d_axis<N>, dim3(outers * tiles), dim3(kThreads), lds, s,
                            c->ghat, line_stride, outer_stride, c->H2, c->tw, c->nmax);
       }
@@ -584,6 +585,18 @@ void set_solver_grid(ob_sc3d* c, double xMin, double xMax, double yMin, double y
   axis(c->M1, H2p, (int64_t)c->qy * H2p, c->qx);  // y: lines (i, kz)
   axis(c->M0, (int64_t)c->qy * H2p, H2p, c->qy);  // x: lines (j, kz)
   c->rebuilds++;
+}
+
+// Rebuild the Green function on the second stream: it is first read by the x pass of the
+// next solve, so it overlaps the deposit and the first passes; the previous solves, which
+// read the old one, are enqueued before it starts.
+void set_solver_grid_async(ob_sc3d* c, double xMin, double xMax, double yMin, double yMax,
+                           double zMin, double zMax) {
+  OB_HIP(hipEventRecord(c->ev_main, stream()));
+  OB_HIP(hipStreamWaitEvent(aux_stream(), c->ev_main, 0));
+  set_solver_grid(c, xMin, xMax, yMin, yMax, zMin, zMax, aux_stream());
+  OB_HIP(hipEventRecord(c->ev_green, aux_stream()));
+  c->green_pending = true;
 }
 
 void solve(ob_sc3d* c, double denom) {
@@ -608,6 +621,10 @@ void solve(ob_sc3d* c, double denom) {
   OB_HIP(hipGetLastError());
   launch_strided<FWD>(c, c->M1, c->ny, c->M1, H2p, (int64_t)c->M1 * H2p, c->nx,
                       "fft3_strided_fwd");
+  if (c->green_pending) {
+    OB_HIP(hipStreamWaitEvent(s, c->ev_green, 0));
+    c->green_pending = false;
+  }
   launch_strided<CONV>(c, c->M0, c->nx, c->nx, (int64_t)c->M1 * H2p, H2p, c->M1,
                        "fft3_strided_conv");
   launch_strided<INV>(c, c->M1, c->M1, c->ny, H2p, (int64_t)c->M1 * H2p, c->nx,
@@ -678,7 +695,9 @@ OB_API int ob_sc3d_create(int nx, int ny, int nz, ob_sc3d** out) {
   c->tw = ob::make_twiddles(c->nmax);
   // SpaceChargeCalc3D(:32-44): solver on [-1,1]^3 (xy_ratio = xz_ratio = 1); rho/phi grids
   // take the solver limits.
-  ob::set_solver_grid(c, -1.0, 1.0, -1.0, 1.0, -1.0, 1.0);
+  OB_HIP(hipEventCreateWithFlags(&c->ev_main, hipEventDisableTiming));
+  OB_HIP(hipEventCreateWithFlags(&c->ev_green, hipEventDisableTiming));
+  ob::set_solver_grid(c, -1.0, 1.0, -1.0, 1.0, -1.0, 1.0, ob::stream());
   c->rebuilds = 0;
   for (int k = 0; k < 3; k++) {
     c->g[2 * k] = c->sMin[k];
@@ -700,6 +719,9 @@ OB_API int ob_sc3d_destroy(ob_sc3d* c) {
     err = e.msg;
   }
   (void)hipStreamSynchronize(ob::stream());
+  (void)hipStreamSynchronize(ob::aux_stream());
+  if (c->ev_main) (void)hipEventDestroy(c->ev_main);
+  if (c->ev_green) (void)hipEventDestroy(c->ev_green);
   (void)hipFree(c->rho); (void)hipFree(c->phi); (void)hipFree(c->spec); (void)hipFree(c->ghat);
   (void)hipFree(c->partials); (void)hipFree(c->ext_dev); (void)hipFree(c->tw);
   (void)hipHostFree(c->ext_host);
@@ -769,7 +791,7 @@ OB_API int ob_sc3d_track(ob_sc3d* c, ob_bunch* b, double length) {
         c->ratio_limit < xz_ratio_beam / c->xz_ratio || 1.0 / c->ratio_limit > xz_ratio_beam / c->xz_ratio) {
       c->xy_ratio = xy_ratio_beam;
       c->xz_ratio = xz_ratio_beam;
-      ob::set_solver_grid(c, -(xMax - xMin) / 2.0, (xMax - xMin) / 2.0, -(yMax - yMin) / 2.0,
+      ob::set_solver_grid_async(c, -(xMax - xMin) / 2.0, (xMax - xMin) / 2.0, -(yMax - yMin) / 2.0,
                           (yMax - yMin) / 2.0, -(zMax - zMin) / 2.0, (zMax - zMin) / 2.0);
     }
     const double scx = (xMax - xMin) / (c->sMax[0] - c->sMin[0]);
@@ -818,7 +840,7 @@ OB_API int ob_sc3d_track(ob_sc3d* c, ob_bunch* b, double length) {
   c->g[4] = rzMin; c->g[5] = rzMax; c->g[6] = g.dx; c->g[7] = g.dy; c->g[8] = rdz;
   if (wrapped) {  // the solver takes the rho grid and the image spacing (:326-330)
     c->lambda_cm = lambda_cm;
-    ob::set_solver_grid(c, g.xMin, g.xMax, g.yMin, g.yMax, rzMin, rzMax);
+    ob::set_solver_grid(c, g.xMin, g.xMax, g.yMin, g.yMax, rzMin, rzMax, ob::stream());
   }
   // findPotential (PoissonSolverFFT3D.cc:249-347)
   const double scale_coeff = c->sd[0] / g.dx;
