@@ -131,7 +131,7 @@ def kernel_bytes(workload, n, grid):
     else:
         H = M1 // 2 + 1
         b.update({
-            "sc2d_hist": 24 * n + 8 * nz,
+            "sc2d_bin": 24 * n + 8 * nx * ny,
             "sc2d_scatter": 32 * n,
             "sc2d_tdeposit": 16 * n + 8 * nx * ny,
             "fft_rows_r2c": 8 * nx * ny + 16 * nx * H,

@@ -80,6 +80,10 @@ void build_records(ob_batch* B, bool force) {
     if (same && !B->calcs.empty() &&
         !sc2d_tile_same(B->calcs[b], B->h_bb[b].tg, views[b].n, views[b].msize != nullptr))
       same = false;
+    DenoiseJob dj;  // a denoiser set or cleared since: the moments come from elsewhere
+    if (same && !B->calcs.empty() &&
+        (sc2d_denoise_job(B->calcs[b], &dj) ? 1 : 0) != B->h_bb[b].moments_from_rho)
+      same = false;
   }
   if (same) return;
   OB_REQUIRE(!B->capturing, "batch: bunch sizes changed during a graph capture");
@@ -111,7 +115,10 @@ void build_records(ob_batch* B, bool force) {
       r.slot_begin = slots;
       slots += (int)(r.v.n / r.tg.item + r.tg.ntiles + 1);
       r.slot_end = slots;
-      hist_lds = std::max(hist_lds, sizeof(double) * r.nz + sizeof(uint32_t) * r.tg.ntiles);
+      hist_lds = std::max(hist_lds, std::max(sizeof(double) * r.nz + sizeof(uint32_t) * r.tg.ntiles,
+                                             2 * sizeof(uint32_t) * r.tg.ntiles));
+      DenoiseJob dj;
+      r.moments_from_rho = sc2d_denoise_job(B->calcs[b], &dj) ? 1 : 0;
       dep_lds = std::max(dep_lds, sizeof(double) * (r.tg.T + 2) * (r.tg.T + 2));
     }
   }

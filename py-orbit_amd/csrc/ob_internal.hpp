@@ -132,12 +132,14 @@ int launch_push(const DeviceBunchView& v, const OpProgram& prog, double* ext_par
 // Tile sort of a 2.5D deposit: T x T tiles of TSC centre nodes, work items of at most
 // `item` particles of one tile, scratch copies of the in-grid particles sorted by tile.
 struct Tile2D {
-  int32_t T, shift, tx, ty, ntiles, item, max_items, pad;  // T = 1 << shift
+  int32_t T, shift, tx, ty, ntiles, item, max_items, zrows;  // T = 1 << shift
   uint32_t* counts;   // [ntiles], zero between steps
-  uint32_t* cursor;   // [ntiles]
+  uint32_t* cursor;   // [ntiles] scatter positions taken, cleared before each scatter
   int4* items;        // [max_items] (tile, begin, end, 0)
   uint32_t* nitems;   // [1]
   double *sx, *sy, *sm;
+  double* zpart;      // [zrows][nz] per-block z line densities (single-bunch path)
+  double* mpart;      // [max_items][5] rho moments per deposit item
 };
 
 }  // namespace ob
@@ -157,6 +159,7 @@ struct BatchBunch {
   double gamma_D;  // (beta*beta)*pow(gamma,3) of the synchronous particle
   Tile2D tg;                      // deposit tile sort (buffers owned by the calculator)
   int32_t slot_begin, slot_end;   // its blocks of the batched tile-deposit pass
+  int32_t moments_from_rho, pad;  // 1: rho changes after the deposit (denoiser)
 };
 struct BatchItem {  // particles [begin, end) of one bunch
   int32_t bunch, pad;
@@ -193,7 +196,8 @@ namespace ob {
 void sc2d_batch_record(ob_sc2p5drb* c, BatchBunch& r);
 // Size the calculator's tile-sort buffers for a bunch of n particles (per-particle
 // macrosize or not) with work items of `item` particles; returns the tile sort.
-Tile2D sc2d_tile_reserve(ob_sc2p5drb* c, int64_t n, bool per_particle_m, int item);
+Tile2D sc2d_tile_reserve(ob_sc2p5drb* c, int64_t n, bool per_particle_m, int item,
+                         int zrows = 1);
 // Whether `t` still describes the calculator's tile buffers and fits such a bunch.
 bool sc2d_tile_same(ob_sc2p5drb* c, const Tile2D& t, int64_t n, bool per_particle_m);
 void sc2d_mark_tracked(ob_sc2p5drb* c);

@@ -13,6 +13,7 @@
 //      FFT, column FFT x Green x inverse column FFT (zero-padding pruned: only the nx
 //      non-zero rows are read and only the nx needed rows written), row c2r;
 //   6. the kick (:146-167) is queued on the bunch and runs fused into the next map kernel.
+#include <algorithm>
 #include <cmath>
 #include <cstring>
 
@@ -54,13 +55,20 @@ struct SC2DParams {
   int n_long_avg;
 };
 
-// ---- 2: setup -------------------------------------------------------------
-// partials[pb..pe) hold per-block extrema of this bunch.  One block of kSetupThreads.
-__device__ __forceinline__ void sc2d_setup_body(const SC2DParams& P,
-                                                const double* __restrict__ partials, int pb,
-                                                int pe, SC2DState* st, double* rho, double* zg,
-                                                double* zd) {
-  __shared__ double red[kSetupThreads / 64][6];
+// ---- 2: grid of the step --------------------------------------------------------
+// bunchAnalysis grid sizing (SpaceChargeCalc2p5Drb.cc:170-213) from the extrema partials.
+struct Grid2Step {
+  double ext[6];
+  double xMin, xMax, yMin, yMax, dx, dy, zMin, zMax, dz;
+  int32_t skip;  // zero extent: no deposit, no kick (the reference aborts, :176-189)
+};
+
+// Every thread of the block gets the grid of partials [pb, pe) (blockDim.x <= 1024).
+__device__ __forceinline__ void sc2d_grid_body(const SC2DParams& P,
+                                               const double* __restrict__ partials, int pb,
+                                               int pe, Grid2Step& out) {
+  __shared__ double red[16][6];
+  __shared__ Grid2Step gs;
   double r[6] = {DBL_MAX, -DBL_MAX, DBL_MAX, -DBL_MAX, DBL_MAX, -DBL_MAX};
   for (int b = pb + threadIdx.x; b < pe; b += blockDim.x)
     for (int k = 0; k < 6; k++) {
@@ -76,17 +84,14 @@ __device__ __forceinline__ void sc2d_setup_body(const SC2DParams& P,
     for (int k = 0; k < 6; k++) red[threadIdx.x >> 6][k] = r[k];
   __syncthreads();
   if (threadIdx.x == 0) {
-    for (int w = 1; w < kSetupThreads / 64; w++)
+    for (int w = 1; w < (int)(blockDim.x >> 6); w++)
       for (int k = 0; k < 6; k++) r[k] = (k & 1) ? fmax(r[k], red[w][k]) : fmin(r[k], red[w][k]);
-    for (int k = 0; k < 6; k++) st->ext[k] = r[k];
+    Grid2Step g;
+    for (int k = 0; k < 6; k++) g.ext[k] = r[k];
     double xMin = r[0], xMax = r[1], yMin = r[2], yMax = r[3];
     const double zMin = r[4], zMax = r[5];
-    st->status = 0;
-    st->skip = 0;
-    if (xMin >= xMax || yMin >= yMax || zMin >= zMax) {  // :176-189 (reference aborts)
-      st->status = 1;
-      st->skip = 1;
-    } else {  // :191-203
+    g.skip = (xMin >= xMax || yMin >= yMax || zMin >= zMax) ? 1 : 0;
+    if (!g.skip) {  // :191-203
       const double xy_ratio_beam = (xMax - xMin) / (yMax - yMin);
       double width, center;
       if (xy_ratio_beam > P.xy_ratio) {
@@ -100,96 +105,107 @@ __device__ __forceinline__ void sc2d_setup_body(const SC2DParams& P,
         xMin = center - width;
         xMax = center + width;
       }
-      st->xMin = xMin;
-      st->xMax = xMax;
-      st->dx = (xMax - xMin) / (P.nx - 1);  // Grid2D.cc:409-413
-      st->yMin = yMin;
-      st->yMax = yMax;
-      st->dy = (yMax - yMin) / (P.ny - 1);
-      st->zMin = zMin;
-      st->zMax = zMax;
-      st->dz = (zMax - zMin) / P.nz;        // Grid1D.cc:88-93
     }
+    g.xMin = xMin;
+    g.xMax = xMax;
+    g.dx = (xMax - xMin) / (P.nx - 1);  // Grid2D.cc:409-413
+    g.yMin = yMin;
+    g.yMax = yMax;
+    g.dy = (yMax - yMin) / (P.ny - 1);
+    g.zMin = zMin;
+    g.zMax = zMax;
+    g.dz = (zMax - zMin) / P.nz;        // Grid1D.cc:88-93
+    gs = g;
   }
+  __syncthreads();
+  out = gs;
+}
+
+// The step's grid into the state; on a zero-extent bunch the grid of the last good step
+// stays (it is not used) and the status word reports it.
+__device__ __forceinline__ void sc2d_write_state(SC2DState* st, const Grid2Step& g) {
+  for (int k = 0; k < 6; k++) st->ext[k] = g.ext[k];
+  st->status = g.skip;
+  st->skip = g.skip;
+  if (!g.skip) {
+    st->xMin = g.xMin;
+    st->xMax = g.xMax;
+    st->dx = g.dx;
+    st->yMin = g.yMin;
+    st->yMax = g.yMax;
+    st->dy = g.dy;
+    st->zMin = g.zMin;
+    st->zMax = g.zMax;
+    st->dz = g.dz;
+  }
+}
+
+__device__ __forceinline__ Grid2Step sc2d_load_grid(const SC2DState* st) {
+  Grid2Step g;
+  g.xMin = st->xMin; g.xMax = st->xMax; g.dx = st->dx;
+  g.yMin = st->yMin; g.yMax = st->yMax; g.dy = st->dy;
+  g.zMin = st->zMin; g.zMax = st->zMax; g.dz = st->dz;
+  g.skip = st->skip;
+  return g;
+}
+
+// Batch: one block per bunch writes the grid and clears its grids and scatter cursors.
+__device__ __forceinline__ void sc2d_setup_body(const SC2DParams& P,
+                                                const double* __restrict__ partials, int pb,
+                                                int pe, SC2DState* st, const Tile2D& tg,
+                                                double* rho, double* zg, double* zd) {
+  Grid2Step g;
+  sc2d_grid_body(P, partials, pb, pe, g);
+  if (threadIdx.x == 0) sc2d_write_state(st, g);
   const int ng = P.nx * P.ny;
   for (int i = threadIdx.x; i < ng; i += blockDim.x) rho[i] = 0.0;
   for (int i = threadIdx.x; i < P.nz; i += blockDim.x) {
     zg[i] = 0.0;
     zd[i] = 0.0;
   }
-}
-
-__global__ __launch_bounds__(kSetupThreads) void sc2d_setup(SC2DParams P,
-                                                            const double* __restrict__ partials,
-                                                            SC2DState* st, double* rho,
-                                                            double* zg, double* zd) {
-  sc2d_setup_body(P, partials, 0, P.nblk, st, rho, zg, zd);
+  for (int t = threadIdx.x; t < tg.ntiles; t += blockDim.x) tg.cursor[t] = 0u;
 }
 
 // ---- 3: deposit, tile-sorted ----------------------------------------------------
 // Binning particle by particle into the whole grid (Grid2D::binValue, Grid2D.cc:142-166)
 // piles every wave's 9 FP64 atomics onto the few nodes of the beam core.  Instead:
-//   a. hist    per block: LDS counts of the particles per T x T tile of TSC centre nodes,
-//              and the z line density (Grid1D::binValue, Grid1D.cc:422-440) in LDS; both
-//              merged with atomics; then one block scans the counts into scatter
-//              positions and work items of at most `item` particles of one tile;
-//   b. scatter counting-sort copy of (x, y[, macrosize]) of the in-grid particles into
-//              scratch arrays, tile by tile (the bunch itself is not reordered);
-//   c. deposit one workgroup per work item accumulates its particles in an LDS block of
-//              the tile plus a one-node halo, then adds the block to rho with atomics.
+//   a. bin     per block: LDS counts of the particles per T x T tile of TSC centre nodes
+//              (merged with atomics), and the z line density (Grid1D::binValue,
+//              Grid1D.cc:422-440) in LDS, written out per block;
+//   b. scatter every block scans the counts, then copies (x, y[, macrosize]) of its
+//              in-grid particles into scratch arrays, tile by tile (counting sort; the
+//              bunch itself is not reordered); block 0 also writes the work items;
+//   c. deposit one workgroup per work item (at most `item` particles of one tile)
+//              accumulates its particles in an LDS block of the tile plus a one-node halo,
+//              adds the block to rho with atomics and writes the block's rho moments.
 // Each contribution is the reference's product; only the FP64 summation order differs.
+// The single-bunch path computes the grid inside the bin pass (every block from the
+// extrema partials, block 0 writes the state); the batch path has a setup launch.
 
 // Tile of the TSC centre node of (x, y), or -1 outside the grid (no charge deposited).
-__device__ __forceinline__ int tile2d_key(const Tile2D& tg, int nx, int ny, double xMin,
-                                          double xMax, double yMin, double yMax, double dx,
-                                          double dy, double x, double y) {
-  if (x < xMin || x > xMax || y < yMin || y > yMax) return -1;
+__device__ __forceinline__ int tile2d_key(const Tile2D& tg, int nx, int ny, const Grid2Step& g,
+                                          double x, double y) {
+  if (x < g.xMin || x > g.xMax || y < g.yMin || y > g.yMax) return -1;
   int iX, iY;
   double xF, yF;
-  node_ind_frac(x, xMin, dx, nx, iX, xF);
-  node_ind_frac(y, yMin, dy, ny, iY, yF);
+  node_ind_frac(x, g.xMin, g.dx, nx, iX, xF);
+  node_ind_frac(y, g.yMin, g.dy, ny, iY, yF);
   return (iX >> tg.shift) * tg.ty + (iY >> tg.shift);
 }
 
-// One block, after the hist pass: turns the counts into scatter positions and work items,
-// and clears the counts for the next step.  (A separate launch: a "last block scans"
-// hand-off inside the hist pass would need a device-scope fence per block, which writes
-// back the L2.)
-__device__ __forceinline__ void tile2d_scan_body(const Tile2D& tg) {
-  const int per = (tg.ntiles + blockDim.x - 1) / blockDim.x;
-  const int lo = min(tg.ntiles, (int)threadIdx.x * per), hi = min(tg.ntiles, lo + per);
-  uint32_t s = 0, c = 0;
-  for (int t = lo; t < hi; t++) {
-    const uint32_t k = tg.counts[t];
-    s += k;
-    c += (k + tg.item - 1) / tg.item;
-  }
-  uint32_t ts, tc;
-  block_scan2(s, c, ts, tc);
-  for (int t = lo; t < hi; t++) {
-    const uint32_t k = tg.counts[t];
-    tg.cursor[t] = s;
-    for (uint32_t c0 = 0; c0 < k; c0 += tg.item)
-      tg.items[c++] = make_int4(t, (int)(s + c0), (int)(s + min(k, c0 + (uint32_t)tg.item)), 0);
-    s += k;
-    tg.counts[t] = 0u;
-  }
-  if (threadIdx.x == 0) *tg.nitems = tc;
-}
-
-// LDS of the hist pass: nz doubles (z line density), then ntiles counters.
+// Particles first, first + stride, ... < end: tile counts (atomics into tg.counts) and the
+// z line density, added atomically into zg or written as this block's row zrow[nz].
+// LDS: zl[nz] doubles, lh[ntiles] counters.
 __device__ __forceinline__ void sc2d_hist_body(const SC2DParams& P, const DeviceBunchView& v,
-                                               const SC2DState* st, const Tile2D& tg,
-                                               double* __restrict__ zg, int64_t first,
-                                               int64_t end, int64_t stride, double* zl,
-                                               uint32_t* lh) {
+                                               const Grid2Step& g, const Tile2D& tg,
+                                               double* __restrict__ zg, double* __restrict__ zrow,
+                                               int64_t first, int64_t end, int64_t stride,
+                                               double* zl, uint32_t* lh) {
   const int nz = P.nz;
   for (int i = threadIdx.x; i < nz; i += blockDim.x) zl[i] = 0.0;
   for (int t = threadIdx.x; t < tg.ntiles; t += blockDim.x) lh[t] = 0u;
   __syncthreads();
-  if (!st->skip) {
-    const double xMin = st->xMin, xMax = st->xMax, yMin = st->yMin, yMax = st->yMax;
-    const double dx = st->dx, dy = st->dy, zMin = st->zMin, zMax = st->zMax, dz = st->dz;
+  if (!g.skip) {
     const double* __restrict__ px = v.c[0];
     const double* __restrict__ py = v.c[2];
     const double* __restrict__ pz = v.c[4];
@@ -209,13 +225,13 @@ __device__ __forceinline__ void sc2d_hist_body(const SC2DParams& P, const Device
 #pragma unroll
       for (int u = 0; u < kUnroll; u++) {
         if (i0 + u * stride >= end) break;
-        const int k = tile2d_key(tg, P.nx, P.ny, xMin, xMax, yMin, yMax, dx, dy, x[u], y[u]);
+        const int k = tile2d_key(tg, P.nx, P.ny, g, x[u], y[u]);
         if (k >= 0) atomicAdd(&lh[k], 1u);
         // Grid1D::binValue (Grid1D.cc:422-440)
-        if (!(z[u] < zMin || z[u] > zMax)) {
+        if (!(z[u] < g.zMin || z[u] > g.zMax)) {
           int i0z, ip;
           double w0, wp;
-          grid1d_ind_wz(z[u], zMin, dz, nz, i0z, ip, w0, wp);
+          grid1d_ind_wz(z[u], g.zMin, g.dz, nz, i0z, ip, w0, wp);
           if (nz > 1) {
             atomicAdd(&zl[i0z], w0 * m[u]);
             atomicAdd(&zl[ip], wp * m[u]);
@@ -227,21 +243,50 @@ __device__ __forceinline__ void sc2d_hist_body(const SC2DParams& P, const Device
     }
   }
   __syncthreads();
-  for (int i = threadIdx.x; i < nz; i += blockDim.x)
-    if (zl[i] != 0.0) atomicAdd(&zg[i], zl[i]);
+  for (int i = threadIdx.x; i < nz; i += blockDim.x) {
+    if (zrow)
+      zrow[i] = zl[i];
+    else if (zl[i] != 0.0)
+      atomicAdd(&zg[i], zl[i]);
+  }
   for (int t = threadIdx.x; t < tg.ntiles; t += blockDim.x)
     if (lh[t]) atomicAdd(&tg.counts[t], lh[t]);
 }
 
-// Particles [lo, hi) of one bunch; LDS: ntiles counters.
+// Exclusive scan of the tile counts into loff[ntiles] (LDS) by every block; the writer
+// block also writes the work items and their number.
+__device__ __forceinline__ void tile2d_scan_body(const Tile2D& tg, uint32_t* loff, bool writer) {
+  const int per = (tg.ntiles + blockDim.x - 1) / blockDim.x;
+  const int lo = min(tg.ntiles, (int)threadIdx.x * per), hi = min(tg.ntiles, lo + per);
+  uint32_t s = 0, c = 0;
+  for (int t = lo; t < hi; t++) {
+    const uint32_t k = tg.counts[t];
+    s += k;
+    c += (k + tg.item - 1) / tg.item;
+  }
+  uint32_t ts, tc;
+  block_scan2(s, c, ts, tc);
+  for (int t = lo; t < hi; t++) {
+    const uint32_t k = tg.counts[t];
+    loff[t] = s;
+    if (writer)
+      for (uint32_t c0 = 0; c0 < k; c0 += tg.item)
+        tg.items[c++] = make_int4(t, (int)(s + c0), (int)(s + min(k, c0 + (uint32_t)tg.item)), 0);
+    s += k;
+  }
+  if (writer && threadIdx.x == 0) *tg.nitems = tc;
+  __syncthreads();
+}
+
+// Particles [lo, hi) of one bunch.  LDS: loff[ntiles] (tile offsets), lc[ntiles].
 __device__ __forceinline__ void sc2d_scatter_body(const SC2DParams& P, const DeviceBunchView& v,
-                                                  const SC2DState* st, const Tile2D& tg,
-                                                  int64_t lo, int64_t hi, uint32_t* lc) {
+                                                  const Grid2Step& g, const Tile2D& tg,
+                                                  int64_t lo, int64_t hi, uint32_t* loff,
+                                                  uint32_t* lc, bool writer) {
+  tile2d_scan_body(tg, loff, writer);
   for (int t = threadIdx.x; t < tg.ntiles; t += blockDim.x) lc[t] = 0u;
   __syncthreads();
-  const bool run = !st->skip;
-  const double xMin = st->xMin, xMax = st->xMax, yMin = st->yMin, yMax = st->yMax;
-  const double dx = st->dx, dy = st->dy;
+  const bool run = !g.skip;
   const double* __restrict__ px = v.c[0];
   const double* __restrict__ py = v.c[2];
   const int64_t stride = blockDim.x;
@@ -257,14 +302,14 @@ __device__ __forceinline__ void sc2d_scatter_body(const SC2DParams& P, const Dev
 #pragma unroll
       for (int u = 0; u < kUnroll; u++) {
         if (i0 + u * stride >= hi) break;
-        const int k = tile2d_key(tg, P.nx, P.ny, xMin, xMax, yMin, yMax, dx, dy, x[u], y[u]);
+        const int k = tile2d_key(tg, P.nx, P.ny, g, x[u], y[u]);
         if (k >= 0) atomicAdd(&lc[k], 1u);
       }
     }
   __syncthreads();
   for (int t = threadIdx.x; t < tg.ntiles; t += blockDim.x) {
     const uint32_t c = lc[t];
-    lc[t] = c ? atomicAdd(&tg.cursor[t], c) : 0u;
+    lc[t] = c ? loff[t] + atomicAdd(&tg.cursor[t], c) : 0u;
   }
   __syncthreads();
   if (run)
@@ -280,7 +325,7 @@ __device__ __forceinline__ void sc2d_scatter_body(const SC2DParams& P, const Dev
 #pragma unroll
       for (int u = 0; u < kUnroll; u++) {
         if (i0 + u * stride >= hi) break;
-        const int k = tile2d_key(tg, P.nx, P.ny, xMin, xMax, yMin, yMax, dx, dy, x[u], y[u]);
+        const int k = tile2d_key(tg, P.nx, P.ny, g, x[u], y[u]);
         if (k < 0) continue;
         const uint32_t d = atomicAdd(&lc[k], 1u);
         tg.sx[d] = x[u];
@@ -290,17 +335,33 @@ __device__ __forceinline__ void sc2d_scatter_body(const SC2DParams& P, const Dev
     }
 }
 
+// rho moments of one node (SpaceChargeCalc2p5Drb.cc:229-238); quirk Q1: the y position
+// is getGridX(iy).
+__device__ __forceinline__ void moment_add(double s[5], double val, int gx, int gy, double xMin,
+                                           double dx) {
+  const double x = xMin + gx * dx;
+  const double y = xMin + gy * dx;
+  s[0] += val;
+  s[1] += x * val;
+  s[2] += x * x * val;
+  s[3] += y * val;
+  s[4] += y * y * val;
+}
+
 // Grid2D::binValue (Grid2D.cc:142-166) of the particles of work item `it` (tile it.x,
-// scratch range [it.y, it.z)) into an LDS block of the tile plus halo, then into rho.
-__device__ __forceinline__ void sc2d_tdeposit_body(const SC2DParams& P, const SC2DState* st,
+// scratch range [it.y, it.z)) into an LDS block of the tile plus halo, then into rho; the
+// block's share of the rho moments goes to mrow[5] (if given).
+__device__ __forceinline__ void sc2d_tdeposit_body(const SC2DParams& P, const Grid2Step& g,
                                                    const Tile2D& tg, int4 it, bool per_particle_m,
-                                                   double* __restrict__ rho, double* acc) {
+                                                   double* __restrict__ rho, double* acc,
+                                                   double* mrow) {
   const int W = tg.T + 2, cells = W * W;
   const int x0 = (it.x / tg.ty) * tg.T - 1, y0 = (it.x % tg.ty) * tg.T - 1;
   for (int c = threadIdx.x; c < cells; c += blockDim.x) acc[c] = 0.0;
   __syncthreads();
-  const double xMin = st->xMin, yMin = st->yMin, dx = st->dx, dy = st->dy;
+  const double xMin = g.xMin, yMin = g.yMin, dx = g.dx, dy = g.dy;
   const int nx = P.nx, ny = P.ny;
+  double mom[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
   const double* __restrict__ sx = tg.sx;
   const double* __restrict__ sy = tg.sy;
   const double* __restrict__ sm = tg.sm;
@@ -335,7 +396,11 @@ __device__ __forceinline__ void sc2d_tdeposit_body(const SC2DParams& P, const SC
       } else {  // not in this tile's block (cannot happen for a consistent sort): global
         double* base = rho + (int64_t)(iX - 1) * ny + (iY - 1);
         for (int a = 0; a < 3; a++)
-          for (int c = 0; c < 3; c++) atomicAdd(base + a * ny + c, wx[a] * wy[c] * m);
+          for (int c = 0; c < 3; c++) {
+            const double val = wx[a] * wy[c] * m;
+            atomicAdd(base + a * ny + c, val);
+            moment_add(mom, val, iX - 1 + a, iY - 1 + c, xMin, dx);
+          }
       }
     }
   }
@@ -346,36 +411,66 @@ __device__ __forceinline__ void sc2d_tdeposit_body(const SC2DParams& P, const SC
     const int gx = x0 + c / W, gy = y0 + c % W;
     if (gx < 0 || gx >= nx || gy < 0 || gy >= ny) continue;
     atomicAdd(&rho[(int64_t)gx * ny + gy], val);
+    moment_add(mom, val, gx, gy, xMin, dx);
+  }
+  if (mrow) {
+    __shared__ double red[kDepositThreads / 64][5];
+    for (int k = 0; k < 5; k++)
+      for (int o = 32; o > 0; o >>= 1) mom[k] += __shfl_xor(mom[k], o, 64);
+    if ((threadIdx.x & 63) == 0)
+      for (int k = 0; k < 5; k++) red[threadIdx.x >> 6][k] = mom[k];
+    __syncthreads();
+    if (threadIdx.x < 5) {
+      double s = 0.0;
+      for (int w = 0; w < (int)(blockDim.x >> 6); w++) s += red[w][threadIdx.x];
+      mrow[threadIdx.x] = s;
+    }
   }
 }
 
-__global__ __launch_bounds__(kDepositThreads) void sc2d_hist(SC2DParams P, DeviceBunchView v,
-                                                             const SC2DState* st, Tile2D tg,
-                                                             double* zg) {
+// Single bunch: the grid from the extrema partials in every block (block 0 writes the
+// state and clears the scatter cursors); blocks clear rho in turn (it is next written by
+// the deposit, two launches later) and write their z line density rows.
+__global__ __launch_bounds__(kDepositThreads) void sc2d_bin(SC2DParams P, DeviceBunchView v,
+                                                            const double* __restrict__ partials,
+                                                            SC2DState* st, Tile2D tg,
+                                                            double* __restrict__ rho) {
   extern __shared__ double dyn[];
-  sc2d_hist_body(P, v, st, tg, zg, (int64_t)blockIdx.x * blockDim.x + threadIdx.x, v.n,
+  Grid2Step g;
+  sc2d_grid_body(P, partials, 0, P.nblk, g);
+  if (blockIdx.x == 0) {
+    if (threadIdx.x == 0) sc2d_write_state(st, g);
+    for (int t = threadIdx.x; t < tg.ntiles; t += blockDim.x) tg.cursor[t] = 0u;
+  }
+  const int ng = P.nx * P.ny;
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < ng; i += gridDim.x * blockDim.x)
+    rho[i] = 0.0;
+  sc2d_hist_body(P, v, g, tg, nullptr, tg.zpart + (size_t)blockIdx.x * P.nz,
+                 (int64_t)blockIdx.x * blockDim.x + threadIdx.x, v.n,
                  (int64_t)gridDim.x * blockDim.x, dyn, (uint32_t*)(dyn + P.nz));
-}
-
-__global__ __launch_bounds__(kDepositThreads) void sc2d_tile_scan(Tile2D tg) {
-  tile2d_scan_body(tg);
 }
 
 __global__ __launch_bounds__(kDepositThreads) void sc2d_scatter(SC2DParams P, DeviceBunchView v,
                                                                 const SC2DState* st, Tile2D tg,
                                                                 int64_t per_block) {
-  extern __shared__ uint32_t lc[];
+  extern __shared__ uint32_t lds[];
   const int64_t lo = (int64_t)blockIdx.x * per_block;
-  sc2d_scatter_body(P, v, st, tg, lo, min(v.n, lo + per_block), lc);
+  sc2d_scatter_body(P, v, sc2d_load_grid(st), tg, lo, min(v.n, lo + per_block), lds,
+                    lds + tg.ntiles, blockIdx.x == 0);
 }
 
+// Block 0 also clears the tile counts for the next step (every scatter block has read them).
 __global__ __launch_bounds__(kDepositThreads) void sc2d_tdeposit(SC2DParams P,
                                                                  const SC2DState* st, Tile2D tg,
                                                                  int per_particle_m,
-                                                                 double* __restrict__ rho) {
+                                                                 double* __restrict__ rho,
+                                                                 int moments) {
+  if (blockIdx.x == 0)
+    for (int t = threadIdx.x; t < tg.ntiles; t += blockDim.x) tg.counts[t] = 0u;
   if (blockIdx.x >= *tg.nitems) return;
   extern __shared__ double acc[];
-  sc2d_tdeposit_body(P, st, tg, tg.items[blockIdx.x], per_particle_m != 0, rho, acc);
+  sc2d_tdeposit_body(P, sc2d_load_grid(st), tg, tg.items[blockIdx.x], per_particle_m != 0, rho,
+                     acc, moments ? tg.mpart + (size_t)blockIdx.x * 5 : nullptr);
 }
 
 // ---- 4: post (one block) ---------------------------------------------------
@@ -396,13 +491,42 @@ __device__ __forceinline__ double ipow(double x, int j) {  // exact for the smal
   return r;
 }
 
+// zg: when zpart is given, the sum of its zrows rows of per-block line densities.
+// Moments: the sum of the deposit items' partial moments mpart[nitems][5], or (mpart
+// nullptr, when rho was changed after the deposit) a pass over rho.
 __device__ __forceinline__ void sc2d_post_body(const SC2DParams& P, SC2DState* st,
                                                const double* __restrict__ rho,
-                                               const double* __restrict__ zg,
-                                               double* __restrict__ zd) {
+                                               double* __restrict__ zg,
+                                               double* __restrict__ zd,
+                                               const double* __restrict__ zpart, int zrows,
+                                               const double* __restrict__ mpart,
+                                               const uint32_t* nitems) {
   __shared__ double sh[kSetupThreads / 64];
-  if (st->skip) return;
   const int nz = P.nz;
+  if (zpart) {
+    // g = blockDim / nz threads per bin (when nz < blockDim) sum the rows in turn, then
+    // their partial sums meet in LDS
+    __shared__ double zs[kSetupThreads];
+    const int g = nz < (int)blockDim.x ? (int)blockDim.x / nz : 1;
+    for (int i0 = 0; i0 < nz; i0 += (int)blockDim.x / g) {
+      const int i = i0 + (int)threadIdx.x / g, r = (int)threadIdx.x % g;
+      double acc = 0.0;
+      if (i < nz && (int)threadIdx.x < g * ((int)blockDim.x / g))
+        for (int b = r; b < zrows; b += g) acc += zpart[(size_t)b * nz + i];
+      zs[threadIdx.x] = acc;
+      __syncthreads();
+      if (r == 0 && i < nz && (int)threadIdx.x < g * ((int)blockDim.x / g)) {
+        double t = 0.0;
+        for (int k = 0; k < g; k++) t += zs[threadIdx.x + k];
+        zg[i] = t;
+      }
+      __syncthreads();
+    }
+  }
+  if (st->skip) {
+    for (int i = threadIdx.x; i < nz; i += blockDim.x) zd[i] = 0.0;
+    return;
+  }
   const double z_step = st->dz;
   const double zMin = st->zMin;
   // calculateLongDerivative (:257-310)
@@ -462,16 +586,28 @@ __device__ __forceinline__ void sc2d_post_body(const SC2DParams& P, SC2DState* s
   const int nx = P.nx, ny = P.ny;
   const double gx0 = st->xMin, gdx = st->dx;
   double s0 = 0., s1 = 0., s2 = 0., s3 = 0., s4 = 0.;
-  for (int k = threadIdx.x; k < nx * ny; k += blockDim.x) {
-    const int ix = k / ny, iy = k - ix * ny;
-    const double val = rho[k];
-    const double x = gx0 + ix * gdx;
-    const double y = gx0 + iy * gdx;
-    s0 += val;
-    s1 += x * val;
-    s2 += x * x * val;
-    s3 += y * val;
-    s4 += y * y * val;
+  if (mpart) {
+    const int ni = (int)*nitems;
+    for (int j = threadIdx.x; j < ni; j += blockDim.x) {
+      const double* m = mpart + (size_t)j * 5;
+      s0 += m[0];
+      s1 += m[1];
+      s2 += m[2];
+      s3 += m[3];
+      s4 += m[4];
+    }
+  } else {
+    for (int k = threadIdx.x; k < nx * ny; k += blockDim.x) {
+      const int ix = k / ny, iy = k - ix * ny;
+      const double val = rho[k];
+      const double x = gx0 + ix * gdx;
+      const double y = gx0 + iy * gdx;
+      s0 += val;
+      s1 += x * val;
+      s2 += x * x * val;
+      s3 += y * val;
+      s4 += y * y * val;
+    }
   }
   s0 = block_sum(s0, sh);
   s1 = block_sum(s1, sh);
@@ -497,9 +633,10 @@ __device__ __forceinline__ void sc2d_post_body(const SC2DParams& P, SC2DState* s
 
 __global__ __launch_bounds__(kSetupThreads) void sc2d_post(SC2DParams P, SC2DState* st,
                                                            const double* __restrict__ rho,
-                                                           const double* __restrict__ zg,
-                                                           double* __restrict__ zd) {
-  sc2d_post_body(P, st, rho, zg, zd);
+                                                           double* __restrict__ zg,
+                                                           double* __restrict__ zd, Tile2D tg,
+                                                           int zrows, int moments) {
+  sc2d_post_body(P, st, rho, zg, zd, tg.zpart, zrows, moments ? tg.mpart : nullptr, tg.nitems);
 }
 
 // ---- 5: Poisson solve --------------------------------------------------------
@@ -670,32 +807,30 @@ __global__ __launch_bounds__(kSetupThreads) void sc2d_setup_batch(const BatchBun
                                                                   double pipe_radius) {
   const BatchBunch& B = bb[blockIdx.x];
   const SC2DParams P = batch_params(B, length, pipe_radius);
-  sc2d_setup_body(P, partials, B.item_begin, B.item_end, B.st, B.rho, B.zg, B.zd);
+  sc2d_setup_body(P, partials, B.item_begin, B.item_end, B.st, B.tg, B.rho, B.zg, B.zd);
 }
 
-// Tile-sorted deposit of every bunch: hist and scatter take one block per particle work
-// item; the last hist block of a bunch scans that bunch's counts.
+// Tile-sorted deposit of every bunch: bin and scatter take one block per particle work
+// item (every scatter block scans its bunch's counts; the bunch's first item writes the
+// work items).
 __global__ __launch_bounds__(kDepositThreads) void sc2d_hist_batch(const BatchBunch* bb,
                                                                    const BatchItem* items) {
   extern __shared__ double dyn[];
   const BatchItem it = items[blockIdx.x];
   const BatchBunch& B = bb[it.bunch];
   const SC2DParams P = batch_params(B, 0.0, 1.0);
-  sc2d_hist_body(P, B.v, B.st, B.tg, B.zg, it.begin + threadIdx.x, it.end, blockDim.x, dyn,
-                 (uint32_t*)(dyn + P.nz));
-}
-
-__global__ __launch_bounds__(kDepositThreads) void sc2d_tile_scan_batch(const BatchBunch* bb) {
-  tile2d_scan_body(bb[blockIdx.x].tg);
+  sc2d_hist_body(P, B.v, sc2d_load_grid(B.st), B.tg, B.zg, nullptr, it.begin + threadIdx.x,
+                 it.end, blockDim.x, dyn, (uint32_t*)(dyn + P.nz));
 }
 
 __global__ __launch_bounds__(kDepositThreads) void sc2d_scatter_batch(const BatchBunch* bb,
                                                                       const BatchItem* items) {
-  extern __shared__ uint32_t lc[];
+  extern __shared__ uint32_t lds[];
   const BatchItem it = items[blockIdx.x];
   const BatchBunch& B = bb[it.bunch];
   const SC2DParams P = batch_params(B, 0.0, 1.0);
-  sc2d_scatter_body(P, B.v, B.st, B.tg, it.begin, it.end, lc);
+  sc2d_scatter_body(P, B.v, sc2d_load_grid(B.st), B.tg, it.begin, it.end, lds,
+                    lds + B.tg.ntiles, (int)blockIdx.x == B.item_begin);
 }
 
 // Bunch whose deposit block range [slot_begin, slot_end) holds block blk.
@@ -714,9 +849,12 @@ __global__ __launch_bounds__(kDepositThreads) void sc2d_tdeposit_batch(const Bat
   const int b = batch_find_slot(bb, nb, blockIdx.x);
   const BatchBunch& B = bb[b];
   const int j = blockIdx.x - B.slot_begin;
+  if (j == 0)  // every scatter block of the bunch has read the counts: clear them
+    for (int t = threadIdx.x; t < B.tg.ntiles; t += blockDim.x) B.tg.counts[t] = 0u;
   if (j >= (int)*B.tg.nitems) return;
   const SC2DParams P = batch_params(B, 0.0, 1.0);
-  sc2d_tdeposit_body(P, B.st, B.tg, B.tg.items[j], B.v.msize != nullptr, B.rho, acc);
+  sc2d_tdeposit_body(P, sc2d_load_grid(B.st), B.tg, B.tg.items[j], B.v.msize != nullptr, B.rho,
+                     acc, B.moments_from_rho ? nullptr : B.tg.mpart + (size_t)j * 5);
 }
 
 __global__ __launch_bounds__(kSetupThreads) void sc2d_post_batch(const BatchBunch* bb,
@@ -724,7 +862,8 @@ __global__ __launch_bounds__(kSetupThreads) void sc2d_post_batch(const BatchBunc
                                                                  double pipe_radius) {
   const BatchBunch& B = bb[blockIdx.x];
   const SC2DParams P = batch_params(B, length, pipe_radius);
-  sc2d_post_body(P, B.st, B.rho, B.zg, B.zd);
+  sc2d_post_body(P, B.st, B.rho, B.zg, B.zd, nullptr, 0,
+                 B.moments_from_rho ? nullptr : B.tg.mpart, B.tg.nitems);
 }
 
 #define OB_BATCH_FFT_SIZES(X) X(8) X(16) X(32) X(64) X(128) X(256) X(512) X(1024)
@@ -914,7 +1053,8 @@ void sc2d_batch_record(ob_sc2p5drb* c, BatchBunch& r) {
   r.dx_solver = c->dx_solver;
 }
 
-Tile2D sc2d_tile_reserve(ob_sc2p5drb* c, int64_t n, bool per_particle_m, int item) {
+Tile2D sc2d_tile_reserve(ob_sc2p5drb* c, int64_t n, bool per_particle_m, int item,
+                         int zrows) {
   Tile2D& g = c->tg;
   if (g.T == 0) {
     int T = 16;
@@ -934,9 +1074,9 @@ Tile2D sc2d_tile_reserve(ob_sc2p5drb* c, int64_t n, bool per_particle_m, int ite
   if (n > c->tile_ncap || (per_particle_m && !c->tile_msize) || item != g.item ||
       want_items > g.max_items) {
     OB_HIP(hipStreamSynchronize(stream()));  // buffers may be in use by queued work
-    for (void* p : {(void*)g.sx, (void*)g.sy, (void*)g.sm, (void*)g.items})
+    for (void* p : {(void*)g.sx, (void*)g.sy, (void*)g.sm, (void*)g.items, (void*)g.mpart})
       if (p) OB_HIP(hipFree(p));
-    g.sx = g.sy = g.sm = nullptr;
+    g.sx = g.sy = g.sm = g.mpart = nullptr;
     int64_t cap = std::max<int64_t>(n, std::max<int64_t>(c->tile_ncap, 1024));
     OB_HIP(hipMalloc((void**)&g.sx, sizeof(double) * cap));
     OB_HIP(hipMalloc((void**)&g.sy, sizeof(double) * cap));
@@ -946,6 +1086,13 @@ Tile2D sc2d_tile_reserve(ob_sc2p5drb* c, int64_t n, bool per_particle_m, int ite
     g.item = item;
     g.max_items = (int)(cap / item + g.ntiles + 1);
     OB_HIP(hipMalloc((void**)&g.items, sizeof(int4) * g.max_items));
+    OB_HIP(hipMalloc((void**)&g.mpart, sizeof(double) * 5 * g.max_items));
+  }
+  if (zrows > g.zrows) {
+    OB_HIP(hipStreamSynchronize(stream()));
+    if (g.zpart) OB_HIP(hipFree(g.zpart));
+    OB_HIP(hipMalloc((void**)&g.zpart, sizeof(double) * zrows * c->nz));
+    g.zrows = zrows;
   }
   return g;
 }
@@ -985,10 +1132,6 @@ void launch_sc2d_batch(const BatchBunch* bb, const BatchItem* items, const Batch
     ProfScope ps("sc2d_hist_batch");
     hipLaunchKernelGGL(sc2d_hist_batch, dim3(sh.nitems), dim3(kDepositThreads), sh.hist_lds, s,
                        bb, items);
-  }
-  {
-    ProfScope ps("sc2d_tile_scan_batch");
-    hipLaunchKernelGGL(sc2d_tile_scan_batch, dim3(sh.nb), dim3(kDepositThreads), 0, s, bb);
   }
   {
     ProfScope ps("sc2d_scatter_batch");
@@ -1094,7 +1237,7 @@ OB_API int ob_sc2p5drb_destroy(ob_sc2p5drb* c) {
   (void)hipFree(c->rho); (void)hipFree(c->phi); (void)hipFree(c->zg); (void)hipFree(c->zd);
   (void)hipFree(c->spec); (void)hipFree(c->ghat_t); (void)hipFree(c->tw); (void)hipFree(c->partials); (void)hipFree(c->st);
   for (void* p : {(void*)c->tg.counts, (void*)c->tg.items, (void*)c->tg.sx, (void*)c->tg.sy,
-                  (void*)c->tg.sm})
+                  (void*)c->tg.sm, (void*)c->tg.zpart, (void*)c->tg.mpart})
     if (p) (void)hipFree(p);
   delete c;
   OB_REQUIRE(err.empty(), err);
@@ -1118,42 +1261,36 @@ OB_API int ob_sc2p5drb_track(ob_sc2p5drb* c, ob_bunch* b, double length, double 
   P.factor0 = 2 * length * sy.r0 / ((sy.beta * sy.beta) * pow(sy.gamma, 3));  // :129
   P.pipe_radius = pipe_radius;
   P.n_long_avg = c->n_long_avg;
+  // bin (grid + tile counts + z line density), scatter (scan + counting sort), deposit
+  const int hb = (int)std::max<int64_t>(
+      1, std::min<int64_t>({(n + 2047) / 2048, (int64_t)4 * ob::device_cu_count(),
+                            (int64_t)std::max(1, (1 << 18) / c->nz)}));
+  const ob::Tile2D tg = ob::sc2d_tile_reserve(c, n, v.msize != nullptr, c->tile_item, hb);
   {
-    ob::ProfScope ps_("sc2d_setup");
-    hipLaunchKernelGGL(ob::sc2d_setup, dim3(1), dim3(ob::kSetupThreads), 0, s, P, c->partials,
-                       c->st, c->rho, c->zg, c->zd);
-  }
-  const ob::Tile2D tg = ob::sc2d_tile_reserve(c, n, v.msize != nullptr, c->tile_item);
-  {
-    ob::ProfScope ps_("sc2d_hist");
-    const int hb = (int)std::max<int64_t>(
-        1, std::min<int64_t>((n + 2047) / 2048, 4 * ob::device_cu_count()));
-    hipLaunchKernelGGL(ob::sc2d_hist, dim3(hb), dim3(ob::kDepositThreads),
-                       sizeof(double) * c->nz + sizeof(uint32_t) * tg.ntiles, s, P, v, c->st, tg,
-                       c->zg);
-  }
-  {
-    ob::ProfScope ps_("sc2d_tile_scan");
-    hipLaunchKernelGGL(ob::sc2d_tile_scan, dim3(1), dim3(ob::kDepositThreads), 0, s, tg);
+    ob::ProfScope ps_("sc2d_bin");
+    hipLaunchKernelGGL(ob::sc2d_bin, dim3(hb), dim3(ob::kDepositThreads),
+                       sizeof(double) * c->nz + sizeof(uint32_t) * tg.ntiles, s, P, v,
+                       c->partials, c->st, tg, c->rho);
   }
   {
     ob::ProfScope ps_("sc2d_scatter");
     const int64_t per = std::max<int64_t>(2048, (n + 1023) / 1024);
     hipLaunchKernelGGL(ob::sc2d_scatter, dim3((unsigned)((n + per - 1) / per)),
-                       dim3(ob::kDepositThreads), sizeof(uint32_t) * tg.ntiles, s, P, v, c->st,
-                       tg, per);
+                       dim3(ob::kDepositThreads), 2 * sizeof(uint32_t) * tg.ntiles, s, P, v,
+                       c->st, tg, per);
   }
+  const int moments = c->denoiser ? 0 : 1;  // the denoiser changes rho after the deposit
   {
     ob::ProfScope ps_("sc2d_tdeposit");
     hipLaunchKernelGGL(ob::sc2d_tdeposit, dim3((unsigned)(n / tg.item + tg.ntiles + 1)),
                        dim3(ob::kDepositThreads), sizeof(double) * (tg.T + 2) * (tg.T + 2), s, P,
-                       c->st, tg, v.msize != nullptr ? 1 : 0, c->rho);
+                       c->st, tg, v.msize != nullptr ? 1 : 0, c->rho, moments);
   }
   if (c->denoiser) ob::dn_denoise_rho(c->denoiser, c->rho, c->nx, c->ny, &c->st->skip);
   {
     ob::ProfScope ps_("sc2d_post");
     hipLaunchKernelGGL(ob::sc2d_post, dim3(1), dim3(ob::kSetupThreads), 0, s, P, c->st, c->rho,
-                       c->zg, c->zd);
+                       c->zg, c->zd, tg, hb, moments);
   }
   ob::launch_fft_pipeline(c, c->st);
   OB_HIP(hipGetLastError());
