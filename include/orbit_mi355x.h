@@ -28,6 +28,7 @@ extern "C" {
 
 typedef struct ob_bunch ob_bunch;
 typedef struct ob_sc2p5drb ob_sc2p5drb;
+typedef struct ob_sc2p5d ob_sc2p5d;
 typedef struct ob_sc3d ob_sc3d;
 
 /* ---- library ---------------------------------------------------------- */
@@ -83,6 +84,12 @@ int ob_bunch_compress(ob_bunch* b);
  * n must equal getSize(); NULL removes the attribute. */
 int ob_bunch_set_macrosizes(ob_bunch* b, int64_t n, const double* m);
 int ob_bunch_get_macrosizes(ob_bunch* b, int64_t n, double* m);
+/* The sums BunchTwissAnalysis::analyzeBunch accumulates (BunchTwissAnalysis.cc:49-142),
+ * on the device after the queued maps: out[0] = sum w, out[1+k] = sum w c_k (k = 0..5),
+ * out[7..27] = sum w c_a c_b for a <= b row by row (a = 0: b = 0..5, a = 1: b = 1..5 ...),
+ * w = the macrosize attribute or 1.  Dead particles are counted until compress(), as the
+ * reference counts them until its own compress() call. */
+int ob_bunch_moments(ob_bunch* b, double* out);
 /* Device pointers of the SoA coordinate arrays (x, xp, y, yp, z, dE), valid until the
  * next size change; pending maps are applied first. */
 int ob_bunch_device_coords(ob_bunch* b, double** ptrs6);
@@ -145,6 +152,22 @@ int ob_sc2p5drb_get_moments(ob_sc2p5drb* c, double* out);
 /* set/getLongAveragingPointsN (:313-320). */
 int ob_sc2p5drb_set_long_avg_n(ob_sc2p5drb* c, int n);
 int ob_sc2p5drb_get_long_avg_n(ob_sc2p5drb* c, int* n);
+
+/* ---- SpaceChargeCalc2p5D (src/spacecharge/SpaceChargeCalc2p5D.cc), no boundary ---- */
+/* SpaceChargeCalc2p5D(nx, ny, nz[, xy_ratio]) (:25-43; wrap_spacechargecalc2p5d.cc:41). */
+int ob_sc2p5d_create(int nx, int ny, int nz, double xy_ratio, ob_sc2p5d** out);
+int ob_sc2p5d_destroy(ob_sc2p5d* c);
+/* trackBunch(bunch, length) (:77-120, bunchAnalysis :122-234): the grid keeps the
+ * calculator's x/y ratio while the beam's is within 25% of it, else follows the beam and
+ * the solver's Green function is rebuilt on the device; transverse kick only, queued on
+ * the bunch like every kick.  The boundary form (wrap :139) is not provided. */
+int ob_sc2p5d_track(ob_sc2p5d* c, ob_bunch* b, double length);
+/* getRhoGrid / getPhiGrid / getLongGrid (wrap :174-177): which = 0 rho, 1 phi, 2 z line
+ * density; lims as for ob_sc2p5drb_get_grid. */
+int ob_sc2p5d_get_grid(ob_sc2p5d* c, int which, double* out, double* lims);
+/* out[7] = xy_ratio, solver xMin, xMax, yMin, yMax, totalMacrosize of the last step,
+ * Green function rebuilds so far (the calculator's private state, :152-172, 206-216). */
+int ob_sc2p5d_get_state(ob_sc2p5d* c, double* out);
 
 /* ---- SpaceChargeCalc3D (src/spacecharge/SpaceChargeCalc3D.cc) ---- */
 int ob_sc3d_create(int nx, int ny, int nz, ob_sc3d** out);             /* :32-58 */

@@ -71,6 +71,10 @@ def lib():
             "orc_sc2p5drb_track": (i, [vp, bp, d, d]),
             "orc_sc2p5drb_get_grid": (i, [vp, i, _D, _D]),
             "orc_sc2p5drb_get_moments": (None, [vp, _D]),
+            "orc_sc2p5d_create": (vp, [i, i, i, d]), "orc_sc2p5d_destroy": (None, [vp]),
+            "orc_sc2p5d_track": (i, [vp, bp, d]),
+            "orc_sc2p5d_get_grid": (i, [vp, i, _D, _D]),
+            "orc_sc2p5d_state": (None, [vp, _D]),
             "orc_sc3d_create": (vp, [i, i, i]), "orc_sc3d_destroy": (None, [vp]),
             "orc_sc3d_set_ratio_limit": (None, [vp, d]), "orc_sc3d_track": (i, [vp, bp, d]),
             "orc_sc3d_get_grid": (C.c_int64, [vp, i, _D, _D]),
@@ -174,6 +178,39 @@ class SC2p5Drb:
         out = np.zeros(4)
         lib().orc_sc2p5drb_get_moments(self.h, _dp(out))
         return out
+
+
+class SC2p5D:
+    """SpaceChargeCalc2p5D without a boundary (SpaceChargeCalc2p5D.cc:77-234)."""
+
+    def __init__(self, nx, ny, nz, xy_ratio=1.0):
+        self.nx, self.ny, self.nz = nx, ny, nz
+        self.h = lib().orc_sc2p5d_create(nx, ny, nz, xy_ratio)
+
+    def __del__(self):
+        if getattr(self, "h", None) and _lib is not None:
+            _lib.orc_sc2p5d_destroy(self.h)
+            self.h = None
+
+    def trackBunch(self, bunch, length):
+        rc = lib().orc_sc2p5d_track(self.h, bunch.ref, length)
+        if rc != 0:
+            raise RuntimeError("oracle: bunch has zero extent")
+
+    def grid(self, which):
+        n = self.nx * self.ny if which in (0, 1) else self.nz
+        out = np.zeros(n)
+        lims = np.zeros(4)
+        lib().orc_sc2p5d_get_grid(self.h, which, _dp(out), _dp(lims))
+        if which in (0, 1):
+            return out.reshape(self.nx, self.ny), lims
+        return out, lims[:2]
+
+    def state(self):
+        """(xy_ratio, solver (xMin, xMax, yMin, yMax), totalMacrosize, Green rebuilds)."""
+        o = np.zeros(7)
+        lib().orc_sc2p5d_state(self.h, _dp(o))
+        return float(o[0]), tuple(o[1:5]), float(o[5]), int(o[6])
 
 
 class SC3D:

@@ -1192,6 +1192,128 @@ int orc_sc2p5drb_get_grid(const orc_sc2p5drb* c, int which, double* out, double*
 void orc_sc2p5drb_get_moments(const orc_sc2p5drb* c, double o[4]) { memcpy(o, c->mom, sizeof(c->mom)); }
 
 /* ------------------------------------------------------------------------ */
+/* SpaceChargeCalc2p5D: src/spacecharge/SpaceChargeCalc2p5D.cc (no boundary) */
+/* PoissonSolverFFT2D::setGridXY (PoissonSolverFFT2D.cc:103-111) */
+void orc_poisson2d_set_grid(orc_poisson2d* p, double xMin, double xMax, double yMin, double yMax) {
+  p->xMin = xMin; p->xMax = xMax; p->yMin = yMin; p->yMax = yMax;
+  p->dx = (xMax - xMin) / (p->nx - 1);
+  p->dy = (yMax - yMin) / (p->ny - 1);
+  poisson2d_define_green(p);
+}
+struct orc_sc2p5d {
+  double xy_ratio;
+  orc_poisson2d* solver;
+  orc_grid2d rho, phi;
+  orc_grid1d zg;
+  double total;
+  int rebuilds;
+};
+orc_sc2p5d* orc_sc2p5d_create(int nx, int ny, int nz, double xy_ratio) { /* :25-33 */
+  orc_sc2p5d* c = (orc_sc2p5d*)calloc(1, sizeof(orc_sc2p5d));
+  c->xy_ratio = xy_ratio;
+  c->solver = orc_poisson2d_create(nx, ny, -xy_ratio, xy_ratio, -1.0, 1.0);
+  orc_grid2d* gs[2] = {&c->rho, &c->phi};
+  for (int k = 0; k < 2; k++) {
+    gs[k]->nx = nx; gs[k]->ny = ny;
+    gs[k]->arr = (double*)calloc((size_t)nx * ny, sizeof(double));
+    orc_grid2d_set_x(gs[k], -1.0, 1.0);
+    orc_grid2d_set_y(gs[k], -1.0, 1.0);
+  }
+  c->zg.nz = nz;
+  c->zg.arr = (double*)calloc(nz, sizeof(double));
+  orc_grid1d_set_z(&c->zg, -0.5, 0.5);
+  return c;
+}
+void orc_sc2p5d_destroy(orc_sc2p5d* c) {
+  if (!c) return;
+  orc_poisson2d_destroy(c->solver);
+  free(c->rho.arr); free(c->phi.arr); free(c->zg.arr); free(c);
+}
+/* bunchAnalysis without a boundary (:113-225): the grid keeps the calculator's x/y ratio
+   while the beam's ratio stays within 25% of it; otherwise the ratio follows the beam and
+   the solver's Green function is rebuilt on the beam's box. */
+static int sc2p5d_analysis(orc_sc2p5d* c, orc_bunch* b) {
+  double e[6];
+  orc_extrema_xyz(b, e);
+  double xMin = e[0], xMax = e[1], yMin = e[2], yMax = e[3], zMin = e[4], zMax = e[5];
+  if (xMin >= xMax || yMin >= yMax || zMin >= zMax) return -1;
+  double xy_ratio_beam = (xMax - xMin) / (yMax - yMin);
+  if (fabs(xy_ratio_beam - c->xy_ratio) < 0.25 * c->xy_ratio) {
+    double width, center;
+    if (xy_ratio_beam > c->xy_ratio) {
+      center = (yMax + yMin) / 2.0;
+      width = ((yMax - yMin) * (xy_ratio_beam / c->xy_ratio)) / 2.0;
+      yMin = center - width;
+      yMax = center + width;
+    } else {
+      center = (xMax + xMin) / 2.0;
+      width = ((xMax - xMin) / (xy_ratio_beam / c->xy_ratio)) / 2.0;
+      xMin = center - width;
+      xMax = center + width;
+    }
+    c->xy_ratio = (xMax - xMin) / (yMax - yMin);
+  } else {
+    c->xy_ratio = xy_ratio_beam;
+    orc_poisson2d_set_grid(c->solver, xMin, xMax, yMin, yMax);
+    c->rebuilds++;
+  }
+  orc_grid2d_set_x(&c->rho, xMin, xMax); orc_grid2d_set_y(&c->rho, yMin, yMax);
+  orc_grid2d_set_x(&c->phi, xMin, xMax); orc_grid2d_set_y(&c->phi, yMin, yMax);
+  orc_grid1d_set_z(&c->zg, zMin, zMax);
+  {
+    const orc_poisson2d* p = c->solver; /* :204-212 */
+    if (fabs((p->xMax - p->xMin) / (p->yMax - p->yMin) - c->xy_ratio) > 0.00000001) {
+      orc_poisson2d_set_grid(c->solver, xMin, xMax, yMin, yMax);
+      c->rebuilds++;
+    }
+  }
+  memset(c->rho.arr, 0, sizeof(double) * c->rho.nx * c->rho.ny);
+  memset(c->zg.arr, 0, sizeof(double) * c->zg.nz);
+  orc_grid2d_bin_bunch(&c->rho, b);
+  orc_grid1d_bin_bunch(&c->zg, b);
+  c->total = 0.;
+  for (int iz = 0; iz < c->zg.nz; iz++) c->total += c->zg.arr[iz];
+  return 0;
+}
+/* trackBunch without a boundary (:76-111) */
+int orc_sc2p5d_track(orc_sc2p5d* c, orc_bunch* b, double length) {
+  if (b->n < 2) return 0;
+  if (sc2p5d_analysis(c, b) != 0) return -1;
+  double z_step = c->zg.dz;
+  orc_poisson2d_find_potential(c->solver, &c->rho, &c->phi);
+  double beta = b->s.beta, gamma = b->s.gamma;
+  double factor = 2 * length * b->s.classical_radius / ((beta * beta) * pow(gamma, 3));
+  factor = factor / (z_step * c->total);
+#pragma omp parallel for num_threads(g_threads) if (g_threads > 1)
+  for (int64_t i = 0; i < b->n; i++) {
+    double ex, ey;
+    orc_grid2d_gradient(&c->phi, b->x[i], b->y[i], &ex, &ey);
+    double Lfactor = -orc_grid1d_value(&c->zg, b->z[i]) * factor;
+    b->xp[i] += ex * Lfactor;
+    b->yp[i] += ey * Lfactor;
+  }
+  return 0;
+}
+int orc_sc2p5d_get_grid(const orc_sc2p5d* c, int which, double* out, double* lims) {
+  if (which == 0 || which == 1) {
+    const orc_grid2d* g = which == 0 ? &c->rho : &c->phi;
+    if (out) memcpy(out, g->arr, sizeof(double) * g->nx * g->ny);
+    if (lims) { lims[0] = g->xMin; lims[1] = g->xMax; lims[2] = g->yMin; lims[3] = g->yMax; }
+    return g->nx * g->ny;
+  }
+  if (out) memcpy(out, c->zg.arr, sizeof(double) * c->zg.nz);
+  if (lims) { lims[0] = c->zg.zMin; lims[1] = c->zg.zMax; }
+  return c->zg.nz;
+}
+/* out: xy_ratio, solver xMin xMax yMin yMax, totalMacrosize, Green rebuilds */
+void orc_sc2p5d_state(const orc_sc2p5d* c, double o[7]) {
+  o[0] = c->xy_ratio;
+  o[1] = c->solver->xMin; o[2] = c->solver->xMax; o[3] = c->solver->yMin; o[4] = c->solver->yMax;
+  o[5] = c->total;
+  o[6] = c->rebuilds;
+}
+
+/* ------------------------------------------------------------------------ */
 /* SpaceChargeCalc3D: src/spacecharge/SpaceChargeCalc3D.cc */
 struct orc_sc3d {
   double xy_ratio, xz_ratio, ratio_limit;

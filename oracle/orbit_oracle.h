@@ -146,6 +146,15 @@ int orc_sc2p5drb_get_grid(const orc_sc2p5drb* c, int which, double* out, double*
 /* x_c, y_c, a_bunch, totalMacrosize of the last bunchAnalysis */
 void orc_sc2p5drb_get_moments(const orc_sc2p5drb* c, double out4[4]);
 
+/* SpaceChargeCalc2p5D without a boundary (SpaceChargeCalc2p5D.cc) */
+typedef struct orc_sc2p5d orc_sc2p5d;
+void orc_poisson2d_set_grid(orc_poisson2d* p, double xMin, double xMax, double yMin, double yMax);
+orc_sc2p5d* orc_sc2p5d_create(int nx, int ny, int nz, double xy_ratio);
+void orc_sc2p5d_destroy(orc_sc2p5d* c);
+int orc_sc2p5d_track(orc_sc2p5d* c, orc_bunch* b, double length);
+int orc_sc2p5d_get_grid(const orc_sc2p5d* c, int which, double* out, double* lims);
+void orc_sc2p5d_state(const orc_sc2p5d* c, double out7[7]);
+
 /* ---- SpaceChargeCalc3D (src/spacecharge/SpaceChargeCalc3D.cc) ---- */
 typedef struct orc_sc3d orc_sc3d;
 orc_sc3d* orc_sc3d_create(int nx, int ny, int nz);

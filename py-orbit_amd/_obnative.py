@@ -39,6 +39,7 @@ SIGNATURES = {
     "ob_bunch_compress": [_vp],
     "ob_bunch_set_macrosizes": [_vp, _i64, _D],
     "ob_bunch_get_macrosizes": [_vp, _i64, _D],
+    "ob_bunch_moments": [_vp, _D],
     "ob_bunch_device_coords": [_vp, C.POINTER(_vp)],
     "ob_bunch_flush": [_vp],
     "ob_track_sequence": [_vp, _vp, _i],
@@ -53,6 +54,11 @@ SIGNATURES = {
     "ob_tpb_quad2": [_vp, _d],
     "ob_tpb_quadfringeIN": [_vp, _d, _i],
     "ob_tpb_quadfringeOUT": [_vp, _d, _i],
+    "ob_sc2p5d_create": [_i, _i, _i, _d, C.POINTER(_vp)],
+    "ob_sc2p5d_destroy": [_vp],
+    "ob_sc2p5d_track": [_vp, _vp, _d],
+    "ob_sc2p5d_get_grid": [_vp, _i, _D, _D],
+    "ob_sc2p5d_get_state": [_vp, _D],
     "ob_sc2p5drb_create": [_i, _i, _i, _d, C.POINTER(_vp)],
     "ob_sc2p5drb_destroy": [_vp],
     "ob_sc2p5drb_track": [_vp, _vp, _d, _d],

@@ -6,6 +6,8 @@ argument meaning and units (GeV, m, rad).  Coordinates live in HBM as six FP64 a
 per-particle accessors go through a host mirror and are correct but slow, so bulk code
 should use setCoordinates()/getCoordinates().
 """
+import sys
+
 import numpy as np
 
 import _obnative as N
@@ -481,30 +483,46 @@ class Bunch:
 
 
 class BunchTwissAnalysis:
-    """RMS moments and emittances (src/orbit/BunchDiagnostics/BunchTwissAnalysis.cc:49-142,
-    344-391).  Computed in FP64 on the downloaded coordinates."""
+    """RMS moments, emittances and Twiss parameters of a bunch
+    (src/orbit/BunchDiagnostics/BunchTwissAnalysis.cc:49-142, 200-420).
+
+    analyzeBunch compresses the bunch (BunchTwissAnalysis.cc:63) and reads the 28 weighted
+    sums of ob_bunch_moments, computed on the device after the queued maps: the
+    coordinates are not downloaded.  The getters derive everything from those sums with
+    the reference's formulas, including its choice of absolute values and the dispersive
+    corrections of the x and y planes."""
 
     def __init__(self):
         self._avg = np.zeros(6)
         self._corr = np.zeros((6, 6))
         self._count = 0
         self._total = 0.0
+        self._p = 0.0
         self._gb = (1.0, 1.0)
 
     def analyzeBunch(self, bunch):
-        c = bunch.getCoordinates()
-        n = c.shape[1]
-        if bunch.hasPartAttr("macrosize"):
-            w = np.array(bunch._msize)
-        else:
-            w = np.ones(n)
-        tot = float(w.sum())
-        self._count, self._total = n, tot
-        if tot != 0.0:
-            self._avg = (c * w).sum(axis=1) / tot
-            self._corr = (c * w) @ c.T / tot
+        bunch.compress()
+        bunch._push_attrs()
+        s = np.zeros(28)
+        N.call("ob_bunch_moments", bunch._h, N.dptr(s))
+        self._count = bunch.getSize()
+        self._total = float(s[0])
+        raw = np.zeros((6, 6))
+        k = 7
+        for a in range(6):
+            for b in range(a, 6):
+                raw[b, a] = s[k]  # the reference fills corr[i + 6 j] for j <= i only
+                k += 1
+        if abs(self._total) > 0.0:
+            self._avg = s[1:7] / self._total
+            low = raw / self._total
+            self._corr = low + np.tril(low, -1).T
+        else:  # BunchTwissAnalysis.cc:117-128 leaves the unnormalized sums in place
+            self._avg = s[1:7].copy()
+            self._corr = raw
         sp = bunch.getSyncParticle()
         self._gb = (sp.gamma(), sp.beta())
+        self._p = sp.momentum()
 
     def getGlobalCount(self):
         return self._count
@@ -513,17 +531,24 @@ class BunchTwissAnalysis:
         return self._total
 
     def getAverage(self, i):
+        if i < 0 or i > 5:
+            return 0.0
         return float(self._avg[i])
 
     def getCorrelation(self, i, j):
+        if i < 0 or i > 5 or j < 0 or j > 5:
+            return 0.0
         return float(self._corr[i, j] - self._avg[i] * self._avg[j])
+
+    def _plane(self, ic):
+        g = self.getCorrelation
+        return (abs(g(2 * ic, 2 * ic)), abs(g(2 * ic + 1, 2 * ic + 1)), g(2 * ic, 2 * ic + 1),
+                g(2 * ic, 5), g(2 * ic + 1, 5), abs(g(5, 5)))
 
     def getEmittance(self, ic):
         if ic < 0 or ic > 2:
             return 0.0
-        g = self.getCorrelation
-        x2, xp2, xxp = abs(g(2 * ic, 2 * ic)), abs(g(2 * ic + 1, 2 * ic + 1)), g(2 * ic, 2 * ic + 1)
-        xdE, xpdE, dE2 = g(2 * ic, 5), g(2 * ic + 1, 5), abs(g(5, 5))
+        x2, xp2, xxp, xdE, xpdE, dE2 = self._plane(ic)
         if ic == 2 or dE2 == 0:
             return float(np.sqrt(abs(x2 * xp2 - xxp * xxp)))
         a = x2 - xdE * xdE / dE2
@@ -532,6 +557,78 @@ class BunchTwissAnalysis:
         return float(np.sqrt(abs(a * b - c * c)))
 
     def getEmittanceNormalized(self, ic):
+        if ic < 0 or ic > 2:
+            return 0.0
         if ic == 2:
             return self.getEmittance(ic)
         return self.getEmittance(ic) * self._gb[0] * self._gb[1]
+
+    # Twiss parameters without the dispersive part for x and y (BunchTwissAnalysis.cc:257-306).
+    # The reference divides by <dE^2> and by the emittance unguarded; so does this (IEEE
+    # division: inf or nan where the reference's C arithmetic gives the same).
+    def getAlpha(self, ic):
+        if ic < 0 or ic > 2:
+            return 0.0
+        x2, xp2, xxp, xdE, xpdE, dE2 = self._plane(ic)
+        num = xxp if ic == 2 else xxp - _div(xdE * xpdE, dE2)
+        return -_div(num, self.getEmittance(ic))
+
+    def getBeta(self, ic):
+        if ic < 0 or ic > 2:
+            return 0.0
+        x2, xp2, xxp, xdE, xpdE, dE2 = self._plane(ic)
+        num = x2 if ic == 2 else x2 - _div(xdE * xdE, dE2)
+        return _div(num, self.getEmittance(ic))
+
+    def getGamma(self, ic):
+        if ic < 0 or ic > 2:
+            return 0.0
+        x2, xp2, xxp, xdE, xpdE, dE2 = self._plane(ic)
+        num = xp2 if ic == 2 else xp2 - _div(xpdE * xpdE, dE2)
+        return _div(num, self.getEmittance(ic))
+
+    def getDispersion(self, ic):
+        if ic < 0 or ic > 1:
+            return 0.0
+        return _div(self.getCorrelation(2 * ic, 5), abs(self.getCorrelation(5, 5))) \
+            * self._p * self._gb[1]
+
+    def getDispersionDerivative(self, ic):
+        if ic < 0 or ic > 1:
+            return 0.0
+        return _div(self.getCorrelation(2 * ic + 1, 5), abs(self.getCorrelation(5, 5))) \
+            * self._p * self._gb[1]
+
+    # Effective (dispersion-inclusive) values (BunchTwissAnalysis.cc:340-420).
+    def _eff(self, ic):
+        x2, xp2, xxp = self._plane(ic)[:3]
+        return x2, xp2, xxp, x2 * xp2 - xxp * xxp
+
+    def getEffectiveEmittance(self, ic):
+        if ic < 0 or ic > 2:
+            return 0.0
+        return float(np.sqrt(abs(self._eff(ic)[3])))
+
+    def getEffectiveAlpha(self, ic):
+        if ic < 0 or ic > 2:
+            return 0.0
+        x2, xp2, xxp, e2 = self._eff(ic)
+        return 0.0 if e2 <= 0.0 else -xxp / float(np.sqrt(e2))
+
+    def getEffectiveBeta(self, ic):
+        if ic < 0 or ic > 2:
+            return 0.0
+        x2, xp2, xxp, e2 = self._eff(ic)
+        return 0.0 if e2 <= 0.0 else x2 / float(np.sqrt(e2))
+
+    def getEffectiveGamma(self, ic):
+        if ic < 0 or ic > 2:
+            return 0.0
+        x2, xp2, xxp, e2 = self._eff(ic)
+        return sys.float_info.max if e2 <= 0.0 else xp2 / float(np.sqrt(e2))
+
+
+def _div(a, b):
+    """a / b with C's IEEE semantics (x/0 = +-inf, 0/0 = nan) instead of Python's error."""
+    with np.errstate(divide="ignore", invalid="ignore"):
+        return float(np.float64(a) / np.float64(b))

@@ -102,6 +102,14 @@ struct SC2DState {
   double lsf_in, a2;                     // longitudinal RB coefficients (:138-141)
   int32_t status;                        // 1: zero-size bunch (reference aborts, :176-189)
   int32_t skip;                          // 1: do not kick
+  // SpaceChargeCalc2p5D (SpaceChargeCalc2p5D.cc:152-172, 206-216): the calculator's x/y
+  // ratio and its solver's box change from step to step.  Each step reads entry [par] and
+  // its grid pass writes [par ^ 1], so every workgroup of that pass reads the same values.
+  double ratio[2];
+  double sol[2][4];                      // solver box xMin xMax yMin yMax
+  double sdx, sdy;                       // solver grid steps (PoissonSolverFFT2D dx_, dy_)
+  int32_t long_kick;                     // 1: 2p5Drb longitudinal kick, 0: transverse only
+  int32_t rebuilds;                      // Green function rebuilds so far
 };
 
 struct SC3DState {

@@ -53,20 +53,29 @@ struct SC2DParams {
   double factor0;             // 2 L r0 / (beta^2 gamma^3)            (:129)
   double pipe_radius;
   int n_long_avg;
+  int plain;                  // 1: SpaceChargeCalc2p5D (ratio rule, transverse kick only)
+  int par;                    // SpaceChargeCalc2p5D: state entry this step reads
 };
 
 // ---- 2: grid of the step --------------------------------------------------------
-// bunchAnalysis grid sizing (SpaceChargeCalc2p5Drb.cc:170-213) from the extrema partials.
+// bunchAnalysis grid sizing from the extrema partials: SpaceChargeCalc2p5Drb.cc:170-213
+// (the box always takes the calculator's x/y ratio) or SpaceChargeCalc2p5D.cc:122-216
+// (the ratio is kept only while the beam's is within 25% of it; otherwise it follows the
+// beam and the solver's Green function is rebuilt on the beam's box).
 struct Grid2Step {
   double ext[6];
   double xMin, xMax, yMin, yMax, dx, dy, zMin, zMax, dz;
-  int32_t skip;  // zero extent: no deposit, no kick (the reference aborts, :176-189)
+  int32_t skip;     // zero extent: no deposit, no kick (the reference aborts, :176-189)
+  int32_t rebuild;  // SpaceChargeCalc2p5D: rebuild the Green function on sol
+  double ratio;     // SpaceChargeCalc2p5D: the calculator's ratio after this step
+  double sol[4];    // SpaceChargeCalc2p5D: the solver box after this step
 };
 
 // Every thread of the block gets the grid of partials [pb, pe) (blockDim.x <= 1024).
+// st is read (entry P.par) only for SpaceChargeCalc2p5D.
 __device__ __forceinline__ void sc2d_grid_body(const SC2DParams& P,
                                                const double* __restrict__ partials, int pb,
-                                               int pe, Grid2Step& out) {
+                                               int pe, const SC2DState* st, Grid2Step& out) {
   __shared__ double red[16][6];
   __shared__ Grid2Step gs;
   double r[6] = {DBL_MAX, -DBL_MAX, DBL_MAX, -DBL_MAX, DBL_MAX, -DBL_MAX};
@@ -91,21 +100,41 @@ __device__ __forceinline__ void sc2d_grid_body(const SC2DParams& P,
     double xMin = r[0], xMax = r[1], yMin = r[2], yMax = r[3];
     const double zMin = r[4], zMax = r[5];
     g.skip = (xMin >= xMax || yMin >= yMax || zMin >= zMax) ? 1 : 0;
-    if (!g.skip) {  // :191-203
+    g.rebuild = 0;
+    double ratio = P.xy_ratio;
+    if (P.plain) {
+      ratio = st->ratio[P.par];
+      for (int k = 0; k < 4; k++) g.sol[k] = st->sol[P.par][k];
+    }
+    if (!g.skip) {
       const double xy_ratio_beam = (xMax - xMin) / (yMax - yMin);
-      double width, center;
-      if (xy_ratio_beam > P.xy_ratio) {
-        center = (yMax + yMin) / 2.0;
-        width = ((yMax - yMin) * (xy_ratio_beam / P.xy_ratio)) / 2.0;
-        yMin = center - width;
-        yMax = center + width;
-      } else {
-        center = (xMax + xMin) / 2.0;
-        width = ((xMax - xMin) / (xy_ratio_beam / P.xy_ratio)) / 2.0;
-        xMin = center - width;
-        xMax = center + width;
+      // 2p5Drb: always (:191-203); 2p5D: within 25% of the ratio (SpaceChargeCalc2p5D.cc:152-167)
+      if (!P.plain || fabs(xy_ratio_beam - ratio) < 0.25 * ratio) {
+        double width, center;
+        if (xy_ratio_beam > ratio) {
+          center = (yMax + yMin) / 2.0;
+          width = ((yMax - yMin) * (xy_ratio_beam / ratio)) / 2.0;
+          yMin = center - width;
+          yMax = center + width;
+        } else {
+          center = (xMax + xMin) / 2.0;
+          width = ((xMax - xMin) / (xy_ratio_beam / ratio)) / 2.0;
+          xMin = center - width;
+          xMax = center + width;
+        }
+        if (P.plain) ratio = (xMax - xMin) / (yMax - yMin);
+      } else {  // SpaceChargeCalc2p5D.cc:168-172
+        ratio = xy_ratio_beam;
+        g.sol[0] = xMin; g.sol[1] = xMax; g.sol[2] = yMin; g.sol[3] = yMax;
+        g.rebuild = 1;
+      }
+      // SpaceChargeCalc2p5D.cc:206-216: a solver box of another shape is rebuilt on this step's box
+      if (P.plain && fabs((g.sol[1] - g.sol[0]) / (g.sol[3] - g.sol[2]) - ratio) > 0.00000001) {
+        g.sol[0] = xMin; g.sol[1] = xMax; g.sol[2] = yMin; g.sol[3] = yMax;
+        g.rebuild = 1;
       }
     }
+    g.ratio = ratio;
     g.xMin = xMin;
     g.xMax = xMax;
     g.dx = (xMax - xMin) / (P.nx - 1);  // Grid2D.cc:409-413
@@ -140,6 +169,22 @@ __device__ __forceinline__ void sc2d_write_state(SC2DState* st, const Grid2Step&
   }
 }
 
+// SpaceChargeCalc2p5D: the ratio and solver box for the next step (entry par ^ 1), the
+// solver steps (PoissonSolverFFT2D::setGridXY, PoissonSolverFFT2D.cc:103-111) and whether
+// the Green function passes of this step run (gate->skip = 0).
+__device__ __forceinline__ void sc2d_write_solver(const SC2DParams& P, SC2DState* st,
+                                                  SC2DState* gate, const Grid2Step& g) {
+  const int nxt = P.par ^ 1;
+  st->ratio[nxt] = g.ratio;
+  for (int k = 0; k < 4; k++) st->sol[nxt][k] = g.sol[k];
+  if (g.rebuild) {
+    st->sdx = (g.sol[1] - g.sol[0]) / (P.nx - 1);
+    st->sdy = (g.sol[3] - g.sol[2]) / (P.ny - 1);
+    st->rebuilds += 1;
+  }
+  gate->skip = g.rebuild ? 0 : 1;
+}
+
 __device__ __forceinline__ Grid2Step sc2d_load_grid(const SC2DState* st) {
   Grid2Step g;
   g.xMin = st->xMin; g.xMax = st->xMax; g.dx = st->dx;
@@ -155,7 +200,7 @@ __device__ __forceinline__ void sc2d_setup_body(const SC2DParams& P,
                                                 int pe, SC2DState* st, const Tile2D& tg,
                                                 double* rho, double* zg, double* zd) {
   Grid2Step g;
-  sc2d_grid_body(P, partials, pb, pe, g);
+  sc2d_grid_body(P, partials, pb, pe, st, g);
   if (threadIdx.x == 0) sc2d_write_state(st, g);
   const int ng = P.nx * P.ny;
   for (int i = threadIdx.x; i < ng; i += blockDim.x) rho[i] = 0.0;
@@ -433,13 +478,16 @@ __device__ __forceinline__ void sc2d_tdeposit_body(const SC2DParams& P, const Gr
 // the deposit, two launches later) and write their z line density rows.
 __global__ __launch_bounds__(kDepositThreads) void sc2d_bin(SC2DParams P, DeviceBunchView v,
                                                             const double* __restrict__ partials,
-                                                            SC2DState* st, Tile2D tg,
-                                                            double* __restrict__ rho) {
+                                                            SC2DState* st, SC2DState* gate,
+                                                            Tile2D tg, double* __restrict__ rho) {
   extern __shared__ double dyn[];
   Grid2Step g;
-  sc2d_grid_body(P, partials, 0, P.nblk, g);
+  sc2d_grid_body(P, partials, 0, P.nblk, st, g);
   if (blockIdx.x == 0) {
-    if (threadIdx.x == 0) sc2d_write_state(st, g);
+    if (threadIdx.x == 0) {
+      sc2d_write_state(st, g);
+      if (P.plain) sc2d_write_solver(P, st, gate, g);
+    }
     for (int t = threadIdx.x; t < tg.ntiles; t += blockDim.x) tg.cursor[t] = 0u;
   }
   const int ng = P.nx * P.ny;
@@ -529,6 +577,19 @@ __device__ __forceinline__ void sc2d_post_body(const SC2DParams& P, SC2DState* s
   }
   const double z_step = st->dz;
   const double zMin = st->zMin;
+  if (P.plain) {
+    // SpaceChargeCalc2p5D.cc:229-233, 100: the total macrosize is the sum of the z line
+    // density; no derivative and no rho moments are used
+    double t = 0.0;
+    for (int i = threadIdx.x; i < nz; i += blockDim.x) t += zg[i];
+    t = block_sum(t, sh);
+    if (threadIdx.x == 0) {
+      st->total = t;
+      st->factor = P.factor0 / (z_step * t);
+      st->scale_coeff = log(st->sdx / st->dx);  // PoissonSolverFFT2D.cc:216
+    }
+    return;
+  }
   // calculateLongDerivative (:257-310)
   if (nz == 1) {
     if (threadIdx.x == 0) zd[0] = 0.;
@@ -787,6 +848,8 @@ __device__ __forceinline__ SC2DParams batch_params(const BatchBunch& B, double l
   P.factor0 = 2 * length * B.r0 / B.gamma_D;  // same operations as the single-bunch host path
   P.pipe_radius = pipe_radius;
   P.n_long_avg = B.n_long_avg;
+  P.plain = 0;
+  P.par = 0;
   return P;
 }
 
@@ -912,7 +975,8 @@ __global__ __launch_bounds__(kFFTThreads) void rows_c2r_batch(const BatchBunch* 
 }
 
 // Green function of PoissonSolverFFT2D::_defineGreenF (:115-172) on the padded grid.
-__global__ void green2d_fill(double* __restrict__ G, int M0, int M1, double dx, double dy) {
+__device__ __forceinline__ void green2d_fill_body(double* __restrict__ G, int M0, int M1,
+                                                  double dx, double dy) {
   const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= (int64_t)M0 * M1) return;
   const int i = (int)(t / M1), j = (int)(t - (int64_t)i * M1);
@@ -921,6 +985,18 @@ __global__ void green2d_fill(double* __restrict__ G, int M0, int M1, double dx, 
   const double rTransX = ii * dx, rTransY = jj * dy;
   const double rTot2 = rTransX * rTransX + rTransY * rTransY;
   G[t] = (ii != 0 || jj != 0) ? -log(rTot2) / 2 : 0.0;
+}
+
+__global__ void green2d_fill(double* __restrict__ G, int M0, int M1, double dx, double dy) {
+  green2d_fill_body(G, M0, M1, dx, dy);
+}
+
+// SpaceChargeCalc2p5D: the same on the solver steps of the state, when this step's grid
+// pass asked for a rebuild (gate->skip 0).
+__global__ void green2d_fill_state(double* __restrict__ G, int M0, int M1,
+                                   const SC2DState* st, const SC2DState* gate) {
+  if (gate->skip) return;
+  green2d_fill_body(G, M0, M1, st->sdx, st->sdy);
 }
 
 }  // namespace
@@ -941,7 +1017,16 @@ struct ob_sc2p5drb {
   bool tile_msize = false;
   int tile_item = 2048;   // particles per deposit work item
   ob_denoiser* denoiser = nullptr;  // applied to rho after each deposit when set
+  // SpaceChargeCalc2p5D: ratio rule and transverse-only kick; the Green function is
+  // rebuilt on the device (gate->skip 0) into ghat_t from gbuf when the grid pass asks
+  int plain = 0;
+  int par = 0;                      // state entry the next step reads
+  ob::SC2DState* gate = nullptr;
+  double* gbuf = nullptr;           // M0 x M1 Green function values
 };
+
+// SpaceChargeCalc2p5D (SpaceChargeCalc2p5D.cc): the same device object in plain mode.
+struct ob_sc2p5d : ob_sc2p5drb {};
 
 namespace ob {
 namespace {
@@ -1033,6 +1118,7 @@ void define_green(ob_sc2p5drb* c) {
 }  // namespace
 
 void sc2d_batch_record(ob_sc2p5drb* c, BatchBunch& r) {
+  OB_REQUIRE(!c->plain, "batches track SpaceChargeCalc2p5Drb calculators only");
   r.st = c->st;
   r.rho = c->rho;
   r.phi = c->phi;
@@ -1176,27 +1262,31 @@ void launch_sc2d_batch(const BatchBunch* bb, const BatchItem* items, const Batch
   }                                    \
   return 0;
 
-OB_API int ob_sc2p5drb_create(int nx, int ny, int nz, double xy_ratio, ob_sc2p5drb** out) {
-  OB_TRY OB_REQUIRE(out != nullptr, "null output pointer");
-  OB_REQUIRE(nx >= 3 && ny >= 3, "SpaceChargeCalc2p5Drb: grid size too small (should be >= 3)");
-  OB_REQUIRE(nz >= 1, "SpaceChargeCalc2p5Drb: number of z bins should be >= 1");
-  OB_REQUIRE(nz <= ob::kMaxZBinsLDS, "SpaceChargeCalc2p5Drb: more than 4096 z bins");
-  OB_REQUIRE(xy_ratio > 0.0, "SpaceChargeCalc2p5Drb: xy_ratio must be positive");
-  ob::ensure_init();
-  ob_sc2p5drb* c = new ob_sc2p5drb();
+namespace ob {
+namespace {
+
+// Calculator object of either kind: grids, FFT buffers, state, the solver Green function
+// on [-xy_ratio, xy_ratio] x [-1, 1] (SpaceChargeCalc2p5Drb.cc:31-46,
+// SpaceChargeCalc2p5D.cc:25-43, PoissonSolver2D.cc:31-32).
+template <typename Calc>
+Calc* sc2d_create(int nx, int ny, int nz, double xy_ratio, bool plain, const std::string& who) {
+  OB_REQUIRE(nx >= 3 && ny >= 3, who + ": grid size too small (should be >= 3)");
+  OB_REQUIRE(nz >= 1, who + ": number of z bins should be >= 1");
+  OB_REQUIRE(nz <= kMaxZBinsLDS, who + ": more than 4096 z bins");
+  OB_REQUIRE(xy_ratio > 0.0, who + ": xy_ratio must be positive");
+  OB_REQUIRE(pad_len(nx) <= 4096 && pad_len(ny) <= 4096,
+             who + ": grid larger than 2048 points per axis");
+  ensure_init();
+  Calc* c = new Calc();
+  c->plain = plain ? 1 : 0;
   c->nx = nx;
   c->ny = ny;
   c->nz = nz;
-  c->M0 = ob::pad_len(nx);
-  c->M1 = ob::pad_len(ny);
-  if (c->M0 > 4096 || c->M1 > 4096) {
-    delete c;
-    throw ob::Error{"SpaceChargeCalc2p5Drb: grid larger than 2048 points per axis"};
-  }
+  c->M0 = pad_len(nx);
+  c->M1 = pad_len(ny);
   c->H = c->M1 / 2 + 1;
   c->nmax = c->M0 > c->M1 ? c->M0 : c->M1;
   c->xy_ratio = xy_ratio;
-  // solver grid [-xy_ratio, xy_ratio] x [-1, 1] (:33, PoissonSolver2D.cc:31-32)
   c->dx_solver = (xy_ratio - -xy_ratio) / (nx - 1);
   c->dy_solver = (1.0 - -1.0) / (ny - 1);
   OB_HIP(hipMalloc((void**)&c->rho, sizeof(double) * nx * ny));
@@ -1206,54 +1296,69 @@ OB_API int ob_sc2p5drb_create(int nx, int ny, int nz, double xy_ratio, ob_sc2p5d
   OB_HIP(hipMalloc((void**)&c->spec, sizeof(double2) * (size_t)c->M0 * c->H));
   OB_HIP(hipMalloc((void**)&c->ghat_t, sizeof(double2) * (size_t)c->M0 * c->H));
   OB_HIP(hipMalloc((void**)&c->partials, sizeof(double) * 6 * 4096));
-  OB_HIP(hipMalloc((void**)&c->st, sizeof(ob::SC2DState)));
+  OB_HIP(hipMalloc((void**)&c->st, sizeof(SC2DState)));
   OB_HIP(hipMemset(c->rho, 0, sizeof(double) * nx * ny));
   OB_HIP(hipMemset(c->phi, 0, sizeof(double) * nx * ny));
   OB_HIP(hipMemset(c->zg, 0, sizeof(double) * nz));
   OB_HIP(hipMemset(c->zd, 0, sizeof(double) * nz));
-  ob::SC2DState init;
+  SC2DState init;
   std::memset(&init, 0, sizeof(init));
   init.xMin = -1.0; init.xMax = 1.0; init.yMin = -1.0; init.yMax = 1.0;  // Grid2D(nx, ny)
   init.dx = 2.0 / (nx - 1); init.dy = 2.0 / (ny - 1);
   init.zMin = -0.5; init.zMax = 0.5; init.dz = 1.0 / nz;                // Grid1D(nz)
+  for (int k = 0; k < 2; k++) {
+    init.ratio[k] = xy_ratio;
+    init.sol[k][0] = -xy_ratio; init.sol[k][1] = xy_ratio;
+    init.sol[k][2] = -1.0; init.sol[k][3] = 1.0;
+  }
+  init.sdx = c->dx_solver;
+  init.sdy = c->dy_solver;
+  init.long_kick = plain ? 0 : 1;
   OB_HIP(hipMemcpy(c->st, &init, sizeof(init), hipMemcpyHostToDevice));
-  c->tw = ob::make_twiddles(c->nmax);
-  ob::define_green(c);
-  *out = c;
-  OB_CATCH
+  if (plain) {
+    OB_HIP(hipMalloc((void**)&c->gate, sizeof(SC2DState)));
+    SC2DState g;
+    std::memset(&g, 0, sizeof(g));
+    g.skip = 1;
+    OB_HIP(hipMemcpy(c->gate, &g, sizeof(g), hipMemcpyHostToDevice));
+    OB_HIP(hipMalloc((void**)&c->gbuf, sizeof(double) * c->M0 * c->M1));
+  }
+  c->tw = make_twiddles(c->nmax);
+  define_green(c);
+  return c;
 }
 
-OB_API int ob_sc2p5drb_destroy(ob_sc2p5drb* c) {
-  OB_TRY if (!c) return 0;
+void sc2d_destroy(ob_sc2p5drb* c) {
   // Pending kicks read st/phi/zg/zd: apply them before the buffers go.
   std::string err;
   try {
-    ob::batch_forget_calc(c);
-    ob::flush_kicks_reading(c->st);
-  } catch (const ob::Error& e) {
+    batch_forget_calc(c);
+    flush_kicks_reading(c->st);
+  } catch (const Error& e) {
     err = e.msg;
   }
-  (void)hipStreamSynchronize(ob::stream());
-  (void)hipFree(c->rho); (void)hipFree(c->phi); (void)hipFree(c->zg); (void)hipFree(c->zd);
-  (void)hipFree(c->spec); (void)hipFree(c->ghat_t); (void)hipFree(c->tw); (void)hipFree(c->partials); (void)hipFree(c->st);
-  for (void* p : {(void*)c->tg.counts, (void*)c->tg.items, (void*)c->tg.sx, (void*)c->tg.sy,
-                  (void*)c->tg.sm, (void*)c->tg.zpart, (void*)c->tg.mpart})
+  (void)hipStreamSynchronize(stream());
+  for (void* p : {(void*)c->rho, (void*)c->phi, (void*)c->zg, (void*)c->zd, (void*)c->spec,
+                  (void*)c->ghat_t, (void*)c->tw, (void*)c->partials, (void*)c->st,
+                  (void*)c->gate, (void*)c->gbuf, (void*)c->tg.counts, (void*)c->tg.items,
+                  (void*)c->tg.sx, (void*)c->tg.sy, (void*)c->tg.sm, (void*)c->tg.zpart,
+                  (void*)c->tg.mpart})
     if (p) (void)hipFree(p);
   delete c;
   OB_REQUIRE(err.empty(), err);
-  OB_CATCH
 }
 
-OB_API int ob_sc2p5drb_track(ob_sc2p5drb* c, ob_bunch* b, double length, double pipe_radius) {
-  OB_TRY OB_REQUIRE(c != nullptr && b != nullptr, "null handle");
-  ob::check_statuses(false);
-  const int64_t n = ob::bunch_size(b);
-  if (n < 2) return 0;  // :112-113
-  hipStream_t s = ob::stream();
-  const int nblk = ob::bunch_flush(b, c->partials);
-  ob::DeviceBunchView v = ob::bunch_device_view(b);
-  const ob::SyncInfo sy = ob::bunch_sync(b);
-  ob::SC2DParams P;
+// One trackBunch: SpaceChargeCalc2p5Drb.cc:110-168, or SpaceChargeCalc2p5D.cc:76-111
+// without a boundary when c->plain.
+void sc2d_track(ob_sc2p5drb* c, ob_bunch* b, double length, double pipe_radius) {
+  check_statuses(false);
+  const int64_t n = bunch_size(b);
+  if (n < 2) return;  // :112-113 (2p5D :79-80)
+  hipStream_t s = stream();
+  const int nblk = bunch_flush(b, c->partials);
+  DeviceBunchView v = bunch_device_view(b);
+  const SyncInfo sy = bunch_sync(b);
+  SC2DParams P;
   P.nx = c->nx; P.ny = c->ny; P.nz = c->nz; P.nblk = nblk;
   P.xy_ratio = c->xy_ratio;
   P.dx_solver = c->dx_solver;
@@ -1261,57 +1366,156 @@ OB_API int ob_sc2p5drb_track(ob_sc2p5drb* c, ob_bunch* b, double length, double 
   P.factor0 = 2 * length * sy.r0 / ((sy.beta * sy.beta) * pow(sy.gamma, 3));  // :129
   P.pipe_radius = pipe_radius;
   P.n_long_avg = c->n_long_avg;
+  P.plain = c->plain;
+  P.par = c->par;
   // bin (grid + tile counts + z line density), scatter (scan + counting sort), deposit
   const int hb = (int)std::max<int64_t>(
-      1, std::min<int64_t>({(n + 2047) / 2048, (int64_t)4 * ob::device_cu_count(),
+      1, std::min<int64_t>({(n + 2047) / 2048, (int64_t)4 * device_cu_count(),
                             (int64_t)std::max(1, (1 << 18) / c->nz)}));
-  const ob::Tile2D tg = ob::sc2d_tile_reserve(c, n, v.msize != nullptr, c->tile_item, hb);
+  const Tile2D tg = sc2d_tile_reserve(c, n, v.msize != nullptr, c->tile_item, hb);
   {
-    ob::ProfScope ps_("sc2d_bin");
-    hipLaunchKernelGGL(ob::sc2d_bin, dim3(hb), dim3(ob::kDepositThreads),
+    ProfScope ps_("sc2d_bin");
+    hipLaunchKernelGGL(sc2d_bin, dim3(hb), dim3(kDepositThreads),
                        sizeof(double) * c->nz + sizeof(uint32_t) * tg.ntiles, s, P, v,
-                       c->partials, c->st, tg, c->rho);
+                       c->partials, c->st, c->gate, tg, c->rho);
+  }
+  if (c->plain) {
+    // PoissonSolverFFT2D::setGridXY (PoissonSolverFFT2D.cc:103-111) when the grid pass
+    // asked for it; each pass returns at once otherwise.  spec is free until the solve.
+    const int64_t tot = (int64_t)c->M0 * c->M1;
+    {
+      ProfScope ps_("green2d_fill_state");
+      hipLaunchKernelGGL(green2d_fill_state, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0,
+                         s, c->gbuf, c->M0, c->M1, c->st, c->gate);
+    }
+    launch_rows_r2c(c, c->gbuf, c->M0, c->M1, c->M1, c->gate);
+    launch_cols<true>(c, c->M0, c->gate, 0.0);
+    c->par ^= 1;
   }
   {
-    ob::ProfScope ps_("sc2d_scatter");
+    ProfScope ps_("sc2d_scatter");
     const int64_t per = std::max<int64_t>(2048, (n + 1023) / 1024);
-    hipLaunchKernelGGL(ob::sc2d_scatter, dim3((unsigned)((n + per - 1) / per)),
-                       dim3(ob::kDepositThreads), 2 * sizeof(uint32_t) * tg.ntiles, s, P, v,
+    hipLaunchKernelGGL(sc2d_scatter, dim3((unsigned)((n + per - 1) / per)),
+                       dim3(kDepositThreads), 2 * sizeof(uint32_t) * tg.ntiles, s, P, v,
                        c->st, tg, per);
   }
-  const int moments = c->denoiser ? 0 : 1;  // the denoiser changes rho after the deposit
+  // rb: the rho moments come from the deposit unless the denoiser changes rho after it
+  const int moments = (c->denoiser || c->plain) ? 0 : 1;
   {
-    ob::ProfScope ps_("sc2d_tdeposit");
-    hipLaunchKernelGGL(ob::sc2d_tdeposit, dim3((unsigned)(n / tg.item + tg.ntiles + 1)),
-                       dim3(ob::kDepositThreads), sizeof(double) * (tg.T + 2) * (tg.T + 2), s, P,
+    ProfScope ps_("sc2d_tdeposit");
+    hipLaunchKernelGGL(sc2d_tdeposit, dim3((unsigned)(n / tg.item + tg.ntiles + 1)),
+                       dim3(kDepositThreads), sizeof(double) * (tg.T + 2) * (tg.T + 2), s, P,
                        c->st, tg, v.msize != nullptr ? 1 : 0, c->rho, moments);
   }
-  if (c->denoiser) ob::dn_denoise_rho(c->denoiser, c->rho, c->nx, c->ny, &c->st->skip);
+  if (c->denoiser) dn_denoise_rho(c->denoiser, c->rho, c->nx, c->ny, &c->st->skip);
   {
-    ob::ProfScope ps_("sc2d_post");
-    hipLaunchKernelGGL(ob::sc2d_post, dim3(1), dim3(ob::kSetupThreads), 0, s, P, c->st, c->rho,
-                       c->zg, c->zd, tg, hb, moments);
+    ProfScope ps_("sc2d_post");
+    hipLaunchKernelGGL(sc2d_post, dim3(1), dim3(kSetupThreads), 0, s, P, c->st, c->rho, c->zg,
+                       c->zd, tg, hb, moments);
   }
-  ob::launch_fft_pipeline(c, c->st);
+  launch_fft_pipeline(c, c->st);
   OB_HIP(hipGetLastError());
-  ob::watch_status(&c->st->status,
-                   "SpaceChargeCalc2p5Drb::bunchAnalysis: the bunch min and max sizes are wrong "
-                   "(zero extent); cannot calculate space charge");
-  // kick (:146-167), fused into the next map kernel
-  ob::Op op;
+  watch_status(&c->st->status,
+               c->plain ? "SpaceChargeCalc2p5D::bunchAnalysis: the bunch min and max sizes are "
+                          "wrong (zero extent); cannot calculate space charge"
+                        : "SpaceChargeCalc2p5Drb::bunchAnalysis: the bunch min and max sizes are "
+                          "wrong (zero extent); cannot calculate space charge");
+  // kick (:146-167; 2p5D :105-119), fused into the next map kernel
+  Op op;
   std::memset(&op, 0, sizeof(op));
-  op.kind = ob::OP_SC2P5DRB_KICK;
+  op.kind = OP_SC2P5DRB_KICK;
   op.i0 = c->nx;
   op.i1 = c->ny;
   op.i2 = c->nz;
-  op.p[0] = ob::ptr_bits(c->st);
-  op.p[1] = ob::ptr_bits(c->phi);
-  op.p[2] = ob::ptr_bits(c->zg);
-  op.p[3] = ob::ptr_bits(c->zd);
-  op.p[4] = 2 * log(pipe_radius);                                             // :139
-  op.p[5] = -length * sy.r0 * sy.mass / (sy.gamma * sy.gamma);                // :141
-  ob::bunch_queue(b, op);
+  op.p[0] = ptr_bits(c->st);
+  op.p[1] = ptr_bits(c->phi);
+  op.p[2] = ptr_bits(c->zg);
+  op.p[3] = ptr_bits(c->zd);
+  op.p[4] = c->plain ? 0.0 : 2 * log(pipe_radius);                            // :139
+  op.p[5] = c->plain ? 0.0 : -length * sy.r0 * sy.mass / (sy.gamma * sy.gamma);  // :141
+  bunch_queue(b, op);
   c->tracked = true;
+}
+
+void sc2d_get_grid(ob_sc2p5drb* c, int which, double* out, double* lims) {
+  hipStream_t s = stream();
+  SC2DState h;
+  OB_HIP(hipMemcpyAsync(&h, c->st, sizeof(h), hipMemcpyDeviceToHost, s));
+  const double* src = which == 0 ? c->rho : which == 1 ? c->phi : which == 2 ? c->zg : c->zd;
+  const size_t cnt = which <= 1 ? (size_t)c->nx * c->ny : (size_t)c->nz;
+  if (out) OB_HIP(hipMemcpyAsync(out, src, sizeof(double) * cnt, hipMemcpyDeviceToHost, s));
+  OB_HIP(hipStreamSynchronize(s));
+  check_statuses(true);
+  if (lims) {
+    if (which <= 1) {
+      lims[0] = h.xMin; lims[1] = h.xMax; lims[2] = h.yMin; lims[3] = h.yMax;
+    } else {
+      lims[0] = h.zMin; lims[1] = h.zMax;
+    }
+  }
+}
+}  // namespace
+}  // namespace ob
+
+OB_API int ob_sc2p5drb_create(int nx, int ny, int nz, double xy_ratio, ob_sc2p5drb** out) {
+  OB_TRY OB_REQUIRE(out != nullptr, "null output pointer");
+  *out = ob::sc2d_create<ob_sc2p5drb>(nx, ny, nz, xy_ratio, false, "SpaceChargeCalc2p5Drb");
+  OB_CATCH
+}
+
+OB_API int ob_sc2p5drb_destroy(ob_sc2p5drb* c) {
+  OB_TRY if (!c) return 0;
+  OB_REQUIRE(!c->plain, "not a SpaceChargeCalc2p5Drb handle");
+  ob::sc2d_destroy(c);
+  OB_CATCH
+}
+
+OB_API int ob_sc2p5drb_track(ob_sc2p5drb* c, ob_bunch* b, double length, double pipe_radius) {
+  OB_TRY OB_REQUIRE(c != nullptr && b != nullptr, "null handle");
+  OB_REQUIRE(!c->plain, "not a SpaceChargeCalc2p5Drb handle");
+  ob::sc2d_track(c, b, length, pipe_radius);
+  OB_CATCH
+}
+
+OB_API int ob_sc2p5d_create(int nx, int ny, int nz, double xy_ratio, ob_sc2p5d** out) {
+  OB_TRY OB_REQUIRE(out != nullptr, "null output pointer");
+  *out = ob::sc2d_create<ob_sc2p5d>(nx, ny, nz, xy_ratio, true, "SpaceChargeCalc2p5D");
+  OB_CATCH
+}
+
+OB_API int ob_sc2p5d_destroy(ob_sc2p5d* c) {
+  OB_TRY if (!c) return 0;
+  OB_REQUIRE(c->plain, "not a SpaceChargeCalc2p5D handle");
+  ob::sc2d_destroy(c);
+  OB_CATCH
+}
+
+OB_API int ob_sc2p5d_track(ob_sc2p5d* c, ob_bunch* b, double length) {
+  OB_TRY OB_REQUIRE(c != nullptr && b != nullptr, "null handle");
+  OB_REQUIRE(c->plain, "not a SpaceChargeCalc2p5D handle");
+  ob::sc2d_track(c, b, length, 0.0);
+  OB_CATCH
+}
+
+OB_API int ob_sc2p5d_get_grid(ob_sc2p5d* c, int which, double* out, double* lims) {
+  OB_TRY OB_REQUIRE(c != nullptr, "null handle");
+  OB_REQUIRE(which >= 0 && which <= 2, "grid selector must be 0..2");
+  ob::sc2d_get_grid(c, which, out, lims);
+  OB_CATCH
+}
+
+// out: xy_ratio, solver box (xMin, xMax, yMin, yMax), totalMacrosize of the last step,
+// number of Green function rebuilds.
+OB_API int ob_sc2p5d_get_state(ob_sc2p5d* c, double* out) {
+  OB_TRY OB_REQUIRE(c != nullptr && out != nullptr, "null handle");
+  ob::SC2DState h;
+  OB_HIP(hipMemcpyAsync(&h, c->st, sizeof(h), hipMemcpyDeviceToHost, ob::stream()));
+  OB_HIP(hipStreamSynchronize(ob::stream()));
+  ob::check_statuses(true);
+  out[0] = h.ratio[c->par];
+  for (int k = 0; k < 4; k++) out[1 + k] = h.sol[c->par][k];
+  out[5] = h.total;
+  out[6] = h.rebuilds;
   OB_CATCH
 }
 
@@ -1357,21 +1561,7 @@ OB_API int ob_sc2p5drb_set_denoiser(ob_sc2p5drb* c, ob_denoiser* dn) {
 OB_API int ob_sc2p5drb_get_grid(ob_sc2p5drb* c, int which, double* out, double* lims) {
   OB_TRY OB_REQUIRE(c != nullptr, "null handle");
   OB_REQUIRE(which >= 0 && which <= 3, "grid selector must be 0..3");
-  hipStream_t s = ob::stream();
-  ob::SC2DState h;
-  OB_HIP(hipMemcpyAsync(&h, c->st, sizeof(h), hipMemcpyDeviceToHost, s));
-  const double* src = which == 0 ? c->rho : which == 1 ? c->phi : which == 2 ? c->zg : c->zd;
-  const size_t cnt = which <= 1 ? (size_t)c->nx * c->ny : (size_t)c->nz;
-  if (out) OB_HIP(hipMemcpyAsync(out, src, sizeof(double) * cnt, hipMemcpyDeviceToHost, s));
-  OB_HIP(hipStreamSynchronize(s));
-  ob::check_statuses(true);
-  if (lims) {
-    if (which <= 1) {
-      lims[0] = h.xMin; lims[1] = h.xMax; lims[2] = h.yMin; lims[3] = h.yMax;
-    } else {
-      lims[0] = h.zMin; lims[1] = h.zMax;
-    }
-  }
+  ob::sc2d_get_grid(c, which, out, lims);
   OB_CATCH
 }
 

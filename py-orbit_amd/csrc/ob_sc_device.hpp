@@ -223,25 +223,27 @@ __device__ __forceinline__ void grid3d_gradient(const double* __restrict__ a, in
   grid3d_gradient_at(v, nz, iX, iY, iZ, xF, yF, zF, dx, dy, dz, gX, gY, gZ);
 }
 
-// SpaceChargeCalc2p5Drb kick loop body (SpaceChargeCalc2p5Drb.cc:146-167).
+// SpaceChargeCalc2p5Drb kick loop body (SpaceChargeCalc2p5Drb.cc:146-167), and, with
+// st->long_kick 0, the SpaceChargeCalc2p5D one (SpaceChargeCalc2p5D.cc:105-119).
 // op: p[0]=SC2DState*, p[1]=phi, p[2]=z line density, p[3]=its derivative,
 //     p[4]=2*log(pipe_radius), p[5]=-L*r0*m/gamma^2; i0=nx, i1=ny, i2=nz.
 __device__ __forceinline__ void sc2p5drb_kick(const Op& op, double x, double& xp, double y,
                                               double& yp, double z, double& dE) {
   const SC2DState* st = op_ptr<const SC2DState*>(op.p[0]);
   if (st->skip) return;
-  const double xc = st->x_c, yc = st->y_c;
-  const double r2 = (x - xc) * (x - xc) + (y - yc) * (y - yc);
   double ex, ey;
   grid2d_gradient(op_ptr<const double*>(op.p[1]), op.i0, op.i1, st->xMin, st->dx, st->yMin,
                   st->dy, x, y, ex, ey);
-  const double ez =
-      grid1d_value(op_ptr<const double*>(op.p[3]), z, st->zMin, st->zMax, st->dz, op.i2);
   const double Lfactor =
       -grid1d_value(op_ptr<const double*>(op.p[2]), z, st->zMin, st->zMax, st->dz, op.i2) *
       st->factor;
   xp += ex * Lfactor;
   yp += ey * Lfactor;
+  if (!st->long_kick) return;  // SpaceChargeCalc2p5D.cc:105-119: transverse only
+  const double xc = st->x_c, yc = st->y_c;
+  const double r2 = (x - xc) * (x - xc) + (y - yc) * (y - yc);
+  const double ez =
+      grid1d_value(op_ptr<const double*>(op.p[3]), z, st->zMin, st->zMax, st->dz, op.i2);
   double coeff;
   if (r2 <= st->a2)
     coeff = st->lsf_in - r2 / st->a2;

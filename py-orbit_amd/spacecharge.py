@@ -292,6 +292,58 @@ class SpaceChargeCalc2p5Drb:
         return tuple(float(v) for v in out)
 
 
+class SpaceChargeCalc2p5D:
+    """SpaceChargeCalc2p5D(nx, ny, nz[, xy_ratio]) (SpaceChargeCalc2p5D.cc:25-43,
+    wrap_spacechargecalc2p5d.cc:41).  trackBunch(bunch, length) without a boundary."""
+
+    def __init__(self, xSize, ySize, zSize, xy_ratio=1.0):
+        self._n = (int(xSize), int(ySize), int(zSize))
+        h = C.c_void_p()
+        N.call("ob_sc2p5d_create", self._n[0], self._n[1], self._n[2], float(xy_ratio),
+               C.byref(h))
+        self._h = h
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h and N._lib is not None:
+            N._lib.ob_sc2p5d_destroy(h)
+            self._h = None
+
+    def trackBunch(self, bunch, length, boundary=None):
+        """trackBunch(bunch, length) (SpaceChargeCalc2p5D.cc:77-120).  The boundary form
+        (wrap_spacechargecalc2p5d.cc:139) is not provided."""
+        if boundary is not None:
+            raise NotImplementedError("SpaceChargeCalc2p5D.trackBunch with a boundary")
+        bunch._push_attrs()
+        N.call("ob_sc2p5d_track", self._h, bunch._h, float(length))
+
+    def _grid(self, which):
+        nx, ny, nz = self._n
+        out = np.zeros(nx * ny if which <= 1 else nz)
+        lims = np.zeros(4)
+        N.call("ob_sc2p5d_get_grid", self._h, which, N.dptr(out), N.dptr(lims))
+        return out, lims
+
+    def getRhoGrid(self):
+        a, l = self._grid(0)
+        return Grid2D(a.reshape(self._n[0], self._n[1]), *l)
+
+    def getPhiGrid(self):
+        a, l = self._grid(1)
+        return Grid2D(a.reshape(self._n[0], self._n[1]), *l)
+
+    def getLongGrid(self):
+        a, l = self._grid(2)
+        return Grid1D(a, l[0], l[1])
+
+    def getState(self):
+        """(xy_ratio, solver box (xMin, xMax, yMin, yMax), totalMacrosize of the last step,
+        Green function rebuilds): the calculator's private state, for diagnostics."""
+        o = np.zeros(7)
+        N.call("ob_sc2p5d_get_state", self._h, N.dptr(o))
+        return float(o[0]), tuple(float(v) for v in o[1:5]), float(o[5]), int(o[6])
+
+
 class SpaceChargeCalc3D:
     """SpaceChargeCalc3D(nx, ny, nz) (SpaceChargeCalc3D.cc:32-58)."""
 

@@ -287,6 +287,88 @@ def test_sc2p5drb_longitudinal_kick_linear_ramp():
     assert np.median(ratio) == pytest.approx(1.0, abs=0.05)
 
 
+def test_sc2p5d_transverse_kick_uniform_disk():
+    """SpaceChargeCalc2p5D: the same defocusing kick of a uniform round beam; dE is left
+    alone and the total macrosize is the sum of the z line density (all particles)."""
+    n, R, Lz, L = 400000, 5e-3, 20.0, 0.5
+    x, y = _uniform_disk(n, R, 5)
+    c = np.zeros((6, n))
+    c[0], c[2] = x, y
+    c[4] = np.random.default_rng(6).uniform(-Lz / 2, Lz / 2, n)
+    c[5] = 1e-4
+    ob = O.Bunch(c, macrosize=1e13 / n, kin_energy=1.0)
+    calc = O.SC2p5D(128, 128, 16)
+    calc.trackBunch(ob, L)
+    s = ob.sync
+    K = 2 * 1.534698e-18 * (1e13 / Lz) * L / (s["beta"] ** 2 * s["gamma"] ** 3 * R ** 2)
+    inside = np.hypot(x, y) < 0.7 * R
+    dxp = ob.coords[1] - c[1]
+    dyp = ob.coords[3] - c[3]
+    assert np.median(dxp[inside] / (K * x[inside])) == pytest.approx(1.0, abs=0.03)
+    assert np.median(dyp[inside] / (K * y[inside])) == pytest.approx(1.0, abs=0.03)
+    assert np.all(ob.coords[5] == c[5])
+    assert calc.state()[2] == pytest.approx(1e13, rel=1e-12)
+
+
+def _beam(n, ax, ay, seed):
+    r = np.random.default_rng(seed)
+    c = np.zeros((6, n))
+    c[0] = r.normal(0, ax, n)
+    c[1] = r.normal(0, 1e-3, n)
+    c[2] = r.normal(0, ay, n)
+    c[3] = r.normal(0, 1e-3, n)
+    c[4] = r.uniform(-5, 5, n)
+    c[5] = r.normal(0, 1e-3, n)
+    return c
+
+
+def test_sc2p5d_ratio_rule_and_green_rebuild():
+    """Beam ratio within 25% of the calculator's: the box keeps the ratio and contains the
+    beam, no rebuild.  Outside: the box is the beam's own, the ratio follows it and the
+    solver is rebuilt on that box once; a repeat of the same beam rebuilds nothing."""
+    calc = O.SC2p5D(64, 64, 8, 1.0)
+    c = _beam(20000, 2e-3, 1.8e-3, 1)
+    ob = O.Bunch(c, macrosize=1e9, kin_energy=1.0)
+    calc.trackBunch(ob, 0.1)
+    ratio, sol, total, rebuilds = calc.state()
+    _, lims = calc.grid(0)
+    e = c[[0, 0, 2, 2]].copy()
+    assert rebuilds == 0 and sol == (-1.0, 1.0, -1.0, 1.0)
+    assert ratio == pytest.approx(1.0, rel=1e-14)
+    assert (lims[1] - lims[0]) / (lims[3] - lims[2]) == pytest.approx(1.0, rel=1e-14)
+    assert lims[0] <= c[0].min() and lims[1] >= c[0].max()
+    assert lims[2] <= c[2].min() and lims[3] >= c[2].max()
+    c2 = _beam(20000, 4e-3, 1.5e-3, 2)
+    ob2 = O.Bunch(c2, macrosize=1e9, kin_energy=1.0)
+    calc.trackBunch(ob2, 0.1)
+    ratio, sol, total, rebuilds = calc.state()
+    _, lims = calc.grid(0)
+    want = (c2[0].min(), c2[0].max(), c2[2].min(), c2[2].max())
+    assert rebuilds == 1
+    assert tuple(lims) == pytest.approx(want, rel=0, abs=0)
+    assert sol == pytest.approx(want, rel=0, abs=0)
+    assert ratio == (want[1] - want[0]) / (want[3] - want[2])
+    calc.trackBunch(O.Bunch(c2, macrosize=1e9, kin_energy=1.0), 0.1)
+    assert calc.state()[3] == 1
+
+
+def test_sc2p5d_transverse_kick_equals_rb_on_the_same_box():
+    """With the calculator's ratio set to the beam's, both calculators use the same box;
+    the potential differs only by a constant (the solver box scale), so the transverse
+    kicks agree to rounding (the rb kick is pinned by the uniform-disk test)."""
+    c = _beam(30000, 3e-3, 2e-3, 3)
+    ratio = (c[0].max() - c[0].min()) / (c[2].max() - c[2].min())
+    a = O.Bunch(c, macrosize=1e9, kin_energy=1.0)
+    b = O.Bunch(c, macrosize=1e9, kin_energy=1.0)
+    O.SC2p5D(64, 64, 16, ratio).trackBunch(a, 0.3)
+    rb = O.SC2p5Drb(64, 64, 16, ratio)
+    rb.trackBunch(b, 0.3, 0.1)
+    for k in (1, 3):
+        d2, drb = a.coords[k] - c[k], b.coords[k] - c[k]
+        assert rel_l2(d2, drb) < 1e-10
+    assert np.all(a.coords[5] == c[5])
+
+
 def test_sc3d_kick_uniform_sphere():
     """Uniform sphere of radius R in the rest frame (z semi-axis R/gamma in the lab): inside,
     x' kick = Q r0 L x / (R^3 beta^2 gamma^2), dE kick = Q r0 m L z_rest / R^3."""
