@@ -110,6 +110,27 @@ int ob_tpb_quad1(ob_bunch* b, double length, double kq, int useCharge); /* :620 
 int ob_tpb_quad2(ob_bunch* b, double length);                           /* :719 */
 int ob_tpb_quadfringeIN(ob_bunch* b, double kq, int useCharge);         /* :784 */
 int ob_tpb_quadfringeOUT(ob_bunch* b, double kq, int useCharge);        /* :854 */
+int ob_tpb_quad3(ob_bunch* b, double length, double kq, int useCharge); /* :761 (no-op) */
+/* drifti(bunch, i, length) (:120): one particle, through the host copy (slow). */
+int ob_tpb_drifti(ob_bunch* b, int64_t i, double length);
+/* Bends, wedges, solenoid and ring RF (:923-1566).  bend1 and soln advance the
+ * synchronous time as drift does; RingRF with phase_s != 0 changes the synchronous
+ * momentum.  wedgebendCF takes n <= 8 multipole terms with poles 0..15 (the device op
+ * carries them inline). */
+int ob_tpb_wedgerotate(ob_bunch* b, double e, int frinout);             /* :923  */
+int ob_tpb_wedgedrift(ob_bunch* b, double e, int inout);                /* :986  */
+int ob_tpb_wedgebend(ob_bunch* b, double e, int inout, double rho, int nsteps); /* :1037 */
+int ob_tpb_bend1(ob_bunch* b, double length, double th);                /* :1099 */
+int ob_tpb_bend2(ob_bunch* b, double length);                           /* :1164 */
+int ob_tpb_bend3(ob_bunch* b, double th);                               /* :1208 */
+int ob_tpb_bend4(ob_bunch* b, double th);                               /* :1251 */
+int ob_tpb_bendfringeIN(ob_bunch* b, double rho);                       /* :1292 */
+int ob_tpb_bendfringeOUT(ob_bunch* b, double rho);                      /* :1329 */
+int ob_tpb_soln(ob_bunch* b, double length, double B, int useCharge);   /* :1367 */
+int ob_tpb_wedgebendCF(ob_bunch* b, double e, int inout, double rho, int n, const int* poles,
+                       const double* kls, const int* skews, int nsteps, int useCharge); /* :1447 */
+int ob_tpb_RingRF(ob_bunch* b, double ring_length, int harmonic_numb, double voltage,
+                  double phase_s, int useCharge);                       /* :1527 */
 /* Apply the queued maps now. */
 int ob_bunch_flush(ob_bunch* b);
 
@@ -119,16 +140,27 @@ int ob_bunch_flush(ob_bunch* b);
  * library stream).  Arguments per kind (a, b, c real; ia, ib, ic integer):
  *   ROTATEXY a=angle | DRIFT a=length | WRAPBUNCH a=ring length | KICK a,b,c=kx,ky,kE ic=useCharge
  *   MULTP / MULTPFRINGE_IN / _OUT  ia=pole a=kl ib=skew ic=useCharge
- *   QUAD1 a=length b=kq ic=useCharge | QUAD2 a=length | QUADFRINGE_IN / _OUT b=kq ic=useCharge */
+ *   QUAD1 a=length b=kq ic=useCharge | QUAD2 a=length | QUADFRINGE_IN / _OUT b=kq ic=useCharge
+ *   BEND1 a=length b=th | BEND2 a=length | BEND3 / BEND4 a=th | BENDFRINGE_IN / _OUT a=rho
+ *   WEDGEROTATE a=e ia=frinout | WEDGEDRIFT a=e ia=inout | WEDGEBEND a=e ia=inout b=rho ib=nsteps
+ *   WEDGEBENDCF as WEDGEBEND plus n, poles, kls, skews, ic=useCharge
+ *   SOLN a=length b=B ic=useCharge | RINGRF a=ring length ia=harmonic b=voltage c=phase_s
+ *   ic=useCharge */
 enum {
   OB_MAP_ROTATEXY = 1, OB_MAP_DRIFT, OB_MAP_WRAPBUNCH, OB_MAP_KICK, OB_MAP_MULTP,
   OB_MAP_MULTPFRINGE_IN, OB_MAP_MULTPFRINGE_OUT, OB_MAP_QUAD1, OB_MAP_QUAD2,
-  OB_MAP_QUADFRINGE_IN, OB_MAP_QUADFRINGE_OUT
+  OB_MAP_QUADFRINGE_IN, OB_MAP_QUADFRINGE_OUT, OB_MAP_BEND1, OB_MAP_BEND2, OB_MAP_BEND3,
+  OB_MAP_BEND4, OB_MAP_BENDFRINGE_IN, OB_MAP_BENDFRINGE_OUT, OB_MAP_WEDGEROTATE,
+  OB_MAP_WEDGEDRIFT, OB_MAP_WEDGEBEND, OB_MAP_WEDGEBENDCF, OB_MAP_SOLN, OB_MAP_RINGRF
 };
 typedef struct ob_map_op {
   int32_t kind;
   int32_t ia, ib, ic;
   double a, b, c;
+  int32_t n, pad;       /* WEDGEBENDCF: number of multipole terms */
+  const int32_t* poles; /* WEDGEBENDCF: n poles, kls, skews */
+  const double* kls;
+  const int32_t* skews;
 } ob_map_op;
 int ob_track_sequence(ob_bunch* b, const ob_map_op* ops, int nops);
 

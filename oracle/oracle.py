@@ -63,6 +63,13 @@ def lib():
             "orc_multpfringeOUT": (None, [bp, i, d, i]), "orc_quad1": (None, [bp, d, d]),
             "orc_quad2": (None, [bp, d]), "orc_quadfringeIN": (None, [bp, d]),
             "orc_quadfringeOUT": (None, [bp, d]), "orc_extrema_xyz": (None, [bp, _D]),
+            "orc_wedgerotate": (None, [bp, d, i]), "orc_wedgedrift": (None, [bp, d, i]),
+            "orc_wedgebend": (None, [bp, d, i, d, i]),
+            "orc_wedgebendCF": (None, [bp, d, i, d, i, _I, _D, _I, i]),
+            "orc_bend1": (None, [bp, d, d]), "orc_bend2": (None, [bp, d]),
+            "orc_bend3": (None, [bp, d]), "orc_bend4": (None, [bp, d]),
+            "orc_bendfringeIN": (None, [bp, d]), "orc_bendfringeOUT": (None, [bp, d]),
+            "orc_soln": (None, [bp, d, d]), "orc_ringrf": (None, [bp, d, i, d, d, i]),
             "orc_fft_c2c": (None, [_D, _D, i, i]),
             "orc_fft_r2c_2d": (None, [_D, i, i, _D]), "orc_fft_c2r_2d": (None, [_D, i, i, _D]),
             "orc_fft_r2c_3d": (None, [_D, i, i, i, _D]), "orc_fft_c2r_3d": (None, [_D, i, i, i, _D]),
@@ -132,7 +139,18 @@ class Bunch:
             "multp": L.orc_multp, "multpfringeIN": L.orc_multpfringeIN,
             "multpfringeOUT": L.orc_multpfringeOUT, "wrapbunch": L.orc_wrapbunch,
             "rotatexy": L.orc_rotatexy, "kick": L.orc_kick,
+            "wedgerotate": L.orc_wedgerotate, "wedgedrift": L.orc_wedgedrift,
+            "wedgebend": L.orc_wedgebend, "bend1": L.orc_bend1, "bend2": L.orc_bend2,
+            "bend3": L.orc_bend3, "bend4": L.orc_bend4, "bendfringeIN": L.orc_bendfringeIN,
+            "bendfringeOUT": L.orc_bendfringeOUT, "soln": L.orc_soln, "RingRF": L.orc_ringrf,
         }
+        if op == "wedgebendCF":  # (e, inout, rho, vecnum, poles, kls, skews, nsteps)
+            e, inout, rho, n, poles, kls, skews, nsteps = args[:8]
+            pa = np.ascontiguousarray(list(poles) or [0], dtype=np.intc)
+            ka = np.ascontiguousarray(list(kls) or [0.0], dtype=np.float64)
+            sa = np.ascontiguousarray(list(skews) or [0], dtype=np.intc)
+            L.orc_wedgebendCF(self.ref, e, inout, rho, n, _ip(pa), _dp(ka), _ip(sa), nsteps)
+            return
         table[op](self.ref, *args)
 
     def extrema(self):

@@ -290,6 +290,249 @@ static void quadfringe(orc_bunch* b, double kq, int out) {
 void orc_quadfringeIN(orc_bunch* b, double kq) { quadfringe(b, kq, 0); }
 void orc_quadfringeOUT(orc_bunch* b, double kq) { quadfringe(b, kq, 1); }
 
+/* ---- bends, wedges, solenoid, ring RF: teapotbase.cc:923-1569 ---- */
+/* One particle's drift, teapotbase.cc:120-141 (drifti). */
+static void drift_one(orc_bunch* b, int64_t i, double length, double gamma2i, double dp_p_coeff) {
+  double dp_p = b->dE[i] * dp_p_coeff;
+  double KNL = 1.0 / (1.0 + dp_p);
+  b->x[i] += KNL * length * b->xp[i];
+  b->y[i] += KNL * length * b->yp[i];
+  double phifac = (b->xp[i] * b->xp[i] + b->yp[i] * b->yp[i] + dp_p * dp_p * gamma2i) / 2.0;
+  phifac = (phifac * KNL - dp_p * gamma2i) * KNL;
+  b->z[i] -= length * phifac;
+}
+/* One particle's multipole kick, teapotbase.cc:280-315 (multpi). */
+static void multp_one(orc_bunch* b, int64_t i, int pole, double kl, int skew) {
+  if (b->s.charge == 0.) return;
+  double kl1 = kl / g_factorial[pole];
+  cplx z = {b->x[i], b->y[i]}, zn = {1.0, 0.0};
+  for (int k = 0; k < pole; k++) zn = cmul(zn, z);
+  if (skew) {
+    b->xp[i] += kl1 * zn.im;
+    b->yp[i] += kl1 * zn.re;
+  } else {
+    b->xp[i] -= kl1 * zn.re;
+    b->yp[i] += kl1 * zn.im;
+  }
+}
+/* :923-966 */
+void orc_wedgerotate(orc_bunch* b, double e, int frinout) {
+  double dp_p_coeff = 1.0 / (b->s.p * b->s.beta);
+  double cs = cos(e), sn = sin(e);
+#pragma omp parallel for num_threads(g_threads) if (g_threads > 1)
+  for (int64_t i = 0; i < b->n; i++) {
+    double dp_p, xp_temp, p0_temp, p0;
+    if (frinout == 0) {
+      dp_p = b->dE[i] * dp_p_coeff;
+      xp_temp = b->xp[i];
+      p0_temp = 1.0 + dp_p;
+      b->x[i] /= cs;
+      b->xp[i] = xp_temp * cs + p0_temp * sn;
+      p0 = -xp_temp * sn + p0_temp * cs;
+      dp_p = p0 - 1.0;
+      b->z[i] = (-b->x[i] * sn + b->z[i]) * cs;
+    } else {
+      dp_p = b->dE[i] * dp_p_coeff;
+      p0 = 1.0 + dp_p;
+      b->z[i] = b->x[i] * sn + b->z[i] / cs;
+      b->x[i] *= cs;
+      xp_temp = b->xp[i] * cs - p0 * sn;
+      p0_temp = b->xp[i] * sn + p0 * cs;
+      b->xp[i] = xp_temp;
+      dp_p = p0_temp - 1.0;
+    }
+    b->dE[i] = dp_p / dp_p_coeff;
+  }
+}
+/* :986-1012 */
+void orc_wedgedrift(orc_bunch* b, double e, int inout) {
+  double gamma2i = 1.0 / (b->s.gamma * b->s.gamma);
+  double dp_p_coeff = 1.0 / (b->s.p * b->s.beta);
+  double ct = cos(e) / sin(e);
+#pragma omp parallel for num_threads(g_threads) if (g_threads > 1)
+  for (int64_t i = 0; i < b->n; i++) {
+    double s;
+    if (inout == 0) {
+      double dp_p = b->dE[i] * dp_p_coeff;
+      double tn = b->xp[i] / (1.0 + dp_p);
+      s = b->x[i] / (ct - tn);
+    } else {
+      s = b->x[i] / ct;
+    }
+    drift_one(b, i, s, gamma2i, dp_p_coeff);
+  }
+}
+/* :1447-1509 (wedgebendCF); wedgebend (:1037-1080) is the case with no multipoles. */
+void orc_wedgebendCF(orc_bunch* b, double e, int inout, double rho, int vecnum, const int* pole,
+                     const double* kl, const int* skew, int nsteps) {
+  init_factorial();
+  double gamma2i = 1.0 / (b->s.gamma * b->s.gamma);
+  double dp_p_coeff = 1.0 / (b->s.p * b->s.beta);
+  int nst = nsteps / 2;
+  if (nst < 1) nst = 1;
+  double ct = cos(e) / sin(e);
+#pragma omp parallel for num_threads(g_threads) if (g_threads > 1)
+  for (int64_t i = 0; i < b->n; i++) {
+    double s;
+    if (inout == 0) {
+      s = -b->x[i] / ct;
+    } else {
+      double dp_p = b->dE[i] * dp_p_coeff;
+      double tn = b->xp[i] / (1.0 + dp_p);
+      s = -b->x[i] / (ct + tn);
+    }
+    double sm = s / nst, sm2 = sm / 2.0;
+    drift_one(b, i, sm2, gamma2i, dp_p_coeff);
+    b->xp[i] -= sm / rho;
+    for (int l = 0; l < vecnum; l++) multp_one(b, i, pole[l], kl[l] * sm, skew[l]);
+    for (int j = 1; j < nst; j++) {
+      drift_one(b, i, sm, gamma2i, dp_p_coeff);
+      b->xp[i] -= sm / rho;
+      for (int l = 0; l < vecnum; l++) multp_one(b, i, pole[l], kl[l] * sm, skew[l]);
+    }
+    drift_one(b, i, sm2, gamma2i, dp_p_coeff);
+  }
+}
+void orc_wedgebend(orc_bunch* b, double e, int inout, double rho, int nsteps) {
+  orc_wedgebendCF(b, e, inout, rho, 0, NULL, NULL, NULL, nsteps);
+}
+/* :1099-1145 */
+void orc_bend1(orc_bunch* b, double length, double th) {
+  orc_sync* s = &b->s;
+  double v = 2.99792458e+8 * s->beta;
+  if (length > 0.) s->time = s->time + length / v;
+  double betasq = s->beta * s->beta;
+  double dp_p_coeff = 1.0 / (s->p * s->beta);
+  double rho = length / th, cx = cos(th), sx = sin(th);
+  double m11 = cx, m12 = rho * sx, m16 = rho * (1.0 - cx);
+  double m21 = -sx / rho, m22 = cx, m26 = sx;
+  double m51 = -sx, m52 = -rho * (1.0 - cx), m56 = -betasq * length + rho * sx;
+#pragma omp parallel for num_threads(g_threads) if (g_threads > 1)
+  for (int64_t i = 0; i < b->n; i++) {
+    double dp_p = b->dE[i] * dp_p_coeff;
+    double x0 = b->x[i], xp0 = b->xp[i];
+    b->x[i] = x0 * m11 + xp0 * m12 + dp_p * m16;
+    b->xp[i] = x0 * m21 + xp0 * m22 + dp_p * m26;
+    b->y[i] += length * b->yp[i];
+    b->z[i] += x0 * m51 + xp0 * m52 + dp_p * m56;
+  }
+}
+/* :1164-1190 (same arithmetic as quad2) */
+void orc_bend2(orc_bunch* b, double length) { orc_quad2(b, length); }
+/* :1208-1233 */
+void orc_bend3(orc_bunch* b, double th) {
+  double gamma2i = 1.0 / (b->s.gamma * b->s.gamma);
+  double dp_p_coeff = 1.0 / (b->s.p * b->s.beta);
+#pragma omp parallel for num_threads(g_threads) if (g_threads > 1)
+  for (int64_t i = 0; i < b->n; i++) {
+    double dp_p = b->dE[i] * dp_p_coeff;
+    double KNL = 1.0 / (1.0 + dp_p);
+    double phifac = (b->yp[i] * b->yp[i] + dp_p * dp_p * gamma2i) / 2.0;
+    b->xp[i] -= phifac * KNL * th;
+    b->y[i] += KNL * b->yp[i] * b->x[i] * th;
+    phifac = (phifac * KNL - dp_p * gamma2i) * KNL;
+    b->z[i] -= th * phifac * b->x[i];
+  }
+}
+/* :1251-1274 */
+void orc_bend4(orc_bunch* b, double th) {
+  double dp_p_coeff = 1.0 / (b->s.p * b->s.beta);
+#pragma omp parallel for num_threads(g_threads) if (g_threads > 1)
+  for (int64_t i = 0; i < b->n; i++) {
+    double dp_p = b->dE[i] * dp_p_coeff;
+    double KNL = 1.0 / (1.0 + dp_p);
+    double xfac = 1.0 + KNL * b->xp[i] * th / 2.0;
+    double phifac = KNL * KNL * b->xp[i] * b->xp[i] / 2.0;
+    b->x[i] *= xfac * xfac;
+    b->xp[i] /= xfac;
+    b->z[i] -= th * phifac * b->x[i];
+  }
+}
+/* :1292-1311 (IN) and :1329-1348 (OUT) */
+static void bendfringe(orc_bunch* b, double rho, int out) {
+  double dp_p_coeff = 1.0 / (b->s.p * b->s.beta);
+#pragma omp parallel for num_threads(g_threads) if (g_threads > 1)
+  for (int64_t i = 0; i < b->n; i++) {
+    double dp_p = b->dE[i] * dp_p_coeff;
+    double KNL = 1.0 / (1.0 + dp_p);
+    if (!out) {
+      b->x[i] += KNL * b->y[i] * b->y[i] / (2. * rho);
+      b->yp[i] -= KNL * b->xp[i] * b->y[i] / rho;
+      b->z[i] += KNL * KNL * b->xp[i] * b->y[i] * b->y[i] / (2. * rho);
+    } else {
+      b->x[i] -= KNL * b->y[i] * b->y[i] / (2. * rho);
+      b->yp[i] += KNL * b->xp[i] * b->y[i] / rho;
+      b->z[i] -= KNL * KNL * b->xp[i] * b->y[i] * b->y[i] / (2. * rho);
+    }
+  }
+}
+void orc_bendfringeIN(orc_bunch* b, double rho) { bendfringe(b, rho, 0); }
+void orc_bendfringeOUT(orc_bunch* b, double rho) { bendfringe(b, rho, 1); }
+/* :1367-1425 */
+void orc_soln(orc_bunch* b, double length, double B) {
+  if (fabs(B) < 1.0e-100 || b->s.charge == 0.) {
+    orc_drift(b, length);
+    return;
+  }
+  orc_sync* s = &b->s;
+  double Bc = B * s->charge;
+  double v = 2.99792458e+8 * s->beta;
+  if (length > 0.) s->time = s->time + length / v;
+  double gamma2i = 1.0 / (s->gamma * s->gamma);
+  double dp_p_coeff = 1.0 / (s->p * s->beta);
+#pragma omp parallel for num_threads(g_threads) if (g_threads > 1)
+  for (int64_t i = 0; i < b->n; i++) {
+    double dp_p = b->dE[i] * dp_p_coeff;
+    double KNL = 1.0 / (1.0 + dp_p);
+    double cu = b->y[i] / 2. - b->xp[i] / Bc;
+    double cpu = b->x[i] * Bc / 2. + b->yp[i];
+    double u_init = b->y[i] / 2. + b->xp[i] / Bc;
+    double pu_init = -b->x[i] * Bc / 2. + b->yp[i];
+    double phase = KNL * Bc * length;
+    double cs = cos(phase), sn = sin(phase);
+    double u = u_init * cs + pu_init * sn / Bc;
+    double pu = -u_init * Bc * sn + pu_init * cs;
+    b->x[i] = (-pu + cpu) / Bc;
+    b->xp[i] = 0.5 * (u - cu) * Bc;
+    b->y[i] = u + cu;
+    b->yp[i] = 0.5 * (pu + cpu);
+    double phifac = (pu_init * pu_init + Bc * Bc * u_init * u_init + dp_p * dp_p * gamma2i) / 2.0;
+    phifac = (phifac * KNL - dp_p * gamma2i) * KNL;
+    b->z[i] -= length * phifac;
+  }
+}
+/* :1527-1566 */
+void orc_ringrf(orc_bunch* b, double ring_length, int harmonic_numb, double voltage,
+                double phase_s, int useCharge) {
+  double charge = +1.0;
+  if (useCharge == 1) charge = b->s.charge;
+  double coeff = charge;
+  double Factor = 0.;
+  if (ring_length > 0.) Factor = 2.0 * 3.14159265358979323846 / ring_length;
+  orc_sync* s = &b->s;
+  double p_synch_in = s->p;
+  if (phase_s != 0.) {
+    double kin_e = s->T;
+    kin_e += coeff * voltage * sin(phase_s);
+    double m = s->mass; /* setMomentum(energyToMomentum(kin_e)), SyncPart.cc:133-138, 314-317 */
+    double pz = sqrt(kin_e * (kin_e + 2.0 * m));
+    double p2 = 0.0 * 0.0 + 0.0 * 0.0 + pz * pz;
+    double p_abs = sqrt(p2), m2 = m * m, w2 = m2 + p2, w = sqrt(w2);
+    s->T = p2 / (sqrt(m2 + p2) + m);
+    s->p = p_abs;
+    s->beta = p_abs / w;
+    s->gamma = w / m;
+  }
+  double xp_yp_coeff = p_synch_in / s->p;
+#pragma omp parallel for num_threads(g_threads) if (g_threads > 1)
+  for (int64_t i = 0; i < b->n; i++) {
+    double deltaV = voltage * (sin(harmonic_numb * Factor * b->z[i] + phase_s));
+    b->dE[i] += coeff * deltaV;
+    b->xp[i] *= xp_yp_coeff;
+    b->yp[i] *= xp_yp_coeff;
+  }
+}
+
 /* ------------------------------------------------------------------------ */
 /* BunchExtremaCalculator.cc:37-83 */
 void orc_extrema_xyz(const orc_bunch* b, double o[6]) {

@@ -55,6 +55,21 @@ void orc_quad1(orc_bunch* b, double length, double kq);
 void orc_quad2(orc_bunch* b, double length);
 void orc_quadfringeIN(orc_bunch* b, double kq);
 void orc_quadfringeOUT(orc_bunch* b, double kq);
+/* bends, wedges, solenoid, ring RF (teapotbase.cc:923-1569) */
+void orc_wedgerotate(orc_bunch* b, double e, int frinout);
+void orc_wedgedrift(orc_bunch* b, double e, int inout);
+void orc_wedgebend(orc_bunch* b, double e, int inout, double rho, int nsteps);
+void orc_wedgebendCF(orc_bunch* b, double e, int inout, double rho, int vecnum, const int* pole,
+                     const double* kl, const int* skew, int nsteps);
+void orc_bend1(orc_bunch* b, double length, double th);
+void orc_bend2(orc_bunch* b, double length);
+void orc_bend3(orc_bunch* b, double th);
+void orc_bend4(orc_bunch* b, double th);
+void orc_bendfringeIN(orc_bunch* b, double rho);
+void orc_bendfringeOUT(orc_bunch* b, double rho);
+void orc_soln(orc_bunch* b, double length, double B);
+void orc_ringrf(orc_bunch* b, double ring_length, int harmonic_numb, double voltage,
+                double phase_s, int useCharge);
 
 /* ---- extrema (src/utils/bunch/BunchExtremaCalculator.cc:37-83) ---- */
 void orc_extrema_xyz(const orc_bunch* b, double out6[6]);

@@ -42,6 +42,20 @@ SIGNATURES = {
     "ob_bunch_moments": [_vp, _D],
     "ob_bunch_device_coords": [_vp, C.POINTER(_vp)],
     "ob_bunch_flush": [_vp],
+    "ob_tpb_quad3": [_vp, _d, _d, _i],
+    "ob_tpb_drifti": [_vp, _i64, _d],
+    "ob_tpb_wedgerotate": [_vp, _d, _i],
+    "ob_tpb_wedgedrift": [_vp, _d, _i],
+    "ob_tpb_wedgebend": [_vp, _d, _i, _d, _i],
+    "ob_tpb_bend1": [_vp, _d, _d],
+    "ob_tpb_bend2": [_vp, _d],
+    "ob_tpb_bend3": [_vp, _d],
+    "ob_tpb_bend4": [_vp, _d],
+    "ob_tpb_bendfringeIN": [_vp, _d],
+    "ob_tpb_bendfringeOUT": [_vp, _d],
+    "ob_tpb_soln": [_vp, _d, _d, _i],
+    "ob_tpb_wedgebendCF": [_vp, _d, _i, _d, _i, C.POINTER(_i), _D, C.POINTER(_i), _i, _i],
+    "ob_tpb_RingRF": [_vp, _d, _i, _d, _d, _i],
     "ob_track_sequence": [_vp, _vp, _i],
     "ob_tpb_rotatexy": [_vp, _d],
     "ob_tpb_drift": [_vp, _d],

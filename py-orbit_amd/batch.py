@@ -61,7 +61,8 @@ class BunchBatch:
 
     def trackSequence(self, ops):
         """Queue maps for every bunch, e.g. [("drift", 0.5), ("quad1", 0.25, 0.5)]."""
-        arr = (TPB._MapOp * len(ops))(*[TPB._map_op(op[0], *op[1:]) for op in ops])
+        keep = []
+        arr = (TPB._MapOp * len(ops))(*[TPB._map_op(op[0], *op[1:], keep=keep) for op in ops])
         N.call("ob_batch_track_sequence", self._h, arr, len(ops))
 
     def flush(self):

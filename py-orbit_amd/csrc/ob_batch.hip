@@ -80,6 +80,10 @@ void build_records(ob_batch* B, bool force) {
     if (same && !B->calcs.empty() &&
         !sc2d_tile_same(B->calcs[b], B->h_bb[b].tg, views[b].n, views[b].msize != nullptr))
       same = false;
+    if (same) {  // RingRF changed the synchronous momentum: new kick factors
+      const SyncInfo sy = bunch_sync(B->bunches[b]);
+      if (B->h_bb[b].gamma_D != (sy.beta * sy.beta) * pow(sy.gamma, 3)) same = false;
+    }
     DenoiseJob dj;  // a denoiser set or cleared since: the moments come from elsewhere
     if (same && !B->calcs.empty() &&
         (sc2d_denoise_job(B->calcs[b], &dj) ? 1 : 0) != B->h_bb[b].moments_from_rho)
@@ -157,9 +161,10 @@ void build_records(ob_batch* B, bool force) {
 void collect(ob_batch* B) {
   std::vector<Op> q = bunch_take_queue(B->leader);
   B->queue.insert(B->queue.end(), q.begin(), q.end());
-  const double t = bunch_sync(B->leader).time;
+  // the maps advance the synchronous time (drift, bend1, soln) and RingRF may change the
+  // synchronous momentum: every bunch of the batch follows the leader
   for (ob_bunch* b : B->bunches)
-    if (b != B->leader) bunch_set_sync_time(b, t);
+    if (b != B->leader) bunch_copy_kinematics(b, B->leader);
 }
 
 // Apply the pending kick and maps to all bunches (extrema per work item if asked).
@@ -355,6 +360,9 @@ OB_API int ob_batch_track_sequence(ob_batch* B, const ob_map_op* ops, int nops) 
   OB_TRY OB_REQUIRE(B != nullptr, "null handle");
   OB_REQUIRE(nops >= 0 && (nops == 0 || ops != nullptr), "invalid op list");
   for (int k = 0; k < nops; k++) {
+    // a graph replays the recorded constants: a momentum change would not be repeated
+    OB_REQUIRE(!(B->capturing && ops[k].kind == OB_MAP_RINGRF && ops[k].c != 0.),
+               "batch: RingRF with a non-zero synchronous phase cannot be captured");
     const int rc = ob::queue_map_op(B->leader, ops[k]);
     if (rc != 0) return rc;
     ob::collect(B);

@@ -343,6 +343,13 @@ std::vector<Op> bunch_take_queue(ob_bunch* b) {
 }
 
 void bunch_set_sync_time(ob_bunch* b, double t) { b->time = t; }
+void bunch_copy_kinematics(ob_bunch* dst, const ob_bunch* src) {
+  dst->T = src->T;
+  dst->p = src->p;
+  dst->beta = src->beta;
+  dst->gamma = src->gamma;
+  dst->time = src->time;
+}
 
 void bunch_set_owner(ob_bunch* b, ob_batch* owner) { b->owner = owner; }
 
@@ -894,6 +901,194 @@ OB_API int ob_tpb_quadfringeOUT(ob_bunch* b, double kq, int) {
   OB_CATCH
 }
 
+// quad3 (teapotbase.cc:761-764) does nothing.
+OB_API int ob_tpb_quad3(ob_bunch* b, double, double, int) {
+  OB_TRY OB_CHECK_B(b);
+  OB_CATCH
+}
+
+// drifti (teapotbase.cc:120-141): one particle, through the host mirror.
+OB_API int ob_tpb_drifti(ob_bunch* b, int64_t i, double length) {
+  OB_TRY OB_CHECK_B(b);
+  OB_REQUIRE(i >= 0 && i < b->n, "particle index out of range");
+  ob::download(b);
+  const double gamma2i = 1.0 / (b->gamma * b->gamma);
+  const double dp_p_coeff = 1.0 / (b->p * b->beta);
+  double* a[6];
+  for (int k = 0; k < 6; k++) a[k] = b->h[k].data();
+  const double dp_p = a[5][i] * dp_p_coeff;
+  const double KNL = 1.0 / (1.0 + dp_p);
+  a[0][i] += KNL * length * a[1][i];
+  a[2][i] += KNL * length * a[3][i];
+  double phifac = (a[1][i] * a[1][i] + a[3][i] * a[3][i] + dp_p * dp_p * gamma2i) / 2.0;
+  phifac = (phifac * KNL - dp_p * gamma2i) * KNL;
+  a[4][i] -= length * phifac;
+  ob::host_write(b);
+  OB_CATCH
+}
+
+OB_API int ob_tpb_wedgerotate(ob_bunch* b, double e, int frinout) {  // teapotbase.cc:923-966
+  OB_TRY OB_CHECK_B(b);
+  ob::Op op = mkop(ob::OP_WEDGEROTATE);
+  op.i0 = frinout;
+  op.p[0] = cos(e);
+  op.p[1] = sin(e);
+  op.p[2] = 1.0 / (b->p * b->beta);
+  ob::bunch_queue(b, op);
+  OB_CATCH
+}
+
+OB_API int ob_tpb_wedgedrift(ob_bunch* b, double e, int inout) {  // teapotbase.cc:986-1012
+  OB_TRY OB_CHECK_B(b);
+  ob::Op op = mkop(ob::OP_WEDGEDRIFT);
+  op.i0 = inout;
+  op.p[0] = cos(e) / sin(e);
+  op.p[1] = 1.0 / (b->p * b->beta);
+  op.p[2] = 1.0 / (b->gamma * b->gamma);
+  ob::bunch_queue(b, op);
+  OB_CATCH
+}
+
+// wedgebendCF (teapotbase.cc:1447-1509); wedgebend (:1037-1080) is the case n = 0.
+OB_API int ob_tpb_wedgebendCF(ob_bunch* b, double e, int inout, double rho, int n,
+                              const int* poles, const double* kls, const int* skews,
+                              int nsteps, int) {
+  OB_TRY OB_CHECK_B(b);
+  OB_REQUIRE(n >= 0 && n <= 8, "wedgebendCF: at most 8 multipole terms");
+  OB_REQUIRE(n == 0 || (poles && kls && skews), "wedgebendCF: null multipole arrays");
+  int nst = nsteps / 2;
+  if (nst < 1) nst = 1;
+  ob::Op op = mkop(ob::OP_WEDGEBEND);
+  const int nm = b->charge == 0. ? 0 : n;  // multpi does nothing for neutral particles
+  int skewbits = 0, polebits = 0;
+  for (int l = 0; l < nm; l++) {
+    OB_REQUIRE(poles[l] >= 0 && poles[l] < 16, "wedgebendCF: pole out of range (0..15)");
+    polebits |= poles[l] << (4 * l);
+    if (skews[l]) skewbits |= 1 << l;
+    op.p[4 + l] = kls[l];
+  }
+  op.i0 = (inout ? 1 : 0) | (nm << 1) | (skewbits << 8);
+  op.i1 = nst;
+  op.i2 = polebits;
+  op.p[0] = cos(e) / sin(e);
+  op.p[1] = rho;
+  op.p[2] = 1.0 / (b->p * b->beta);
+  op.p[3] = 1.0 / (b->gamma * b->gamma);
+  ob::bunch_queue(b, op);
+  OB_CATCH
+}
+
+OB_API int ob_tpb_wedgebend(ob_bunch* b, double e, int inout, double rho, int nsteps) {
+  return ob_tpb_wedgebendCF(b, e, inout, rho, 0, nullptr, nullptr, nullptr, nsteps, 1);
+}
+
+OB_API int ob_tpb_bend1(ob_bunch* b, double length, double th) {  // teapotbase.cc:1099-1145
+  OB_TRY OB_CHECK_B(b);
+  const double v = 2.99792458e+8 * b->beta;
+  if (length > 0.) b->time = b->time + length / v;
+  const double betasq = b->beta * b->beta;
+  const double rho = length / th, cx = cos(th), sx = sin(th);
+  ob::Op op = mkop(ob::OP_BEND1);
+  op.p[0] = cx;                             // m11
+  op.p[1] = rho * sx;                       // m12
+  op.p[2] = rho * (1.0 - cx);               // m16
+  op.p[3] = -sx / rho;                      // m21
+  op.p[4] = cx;                             // m22
+  op.p[5] = sx;                             // m26
+  op.p[6] = -sx;                            // m51
+  op.p[7] = -rho * (1.0 - cx);              // m52
+  op.p[8] = -betasq * length + rho * sx;    // m56
+  op.p[9] = length;
+  op.p[10] = 1.0 / (b->p * b->beta);
+  ob::bunch_queue(b, op);
+  OB_CATCH
+}
+
+OB_API int ob_tpb_bend2(ob_bunch* b, double length) {  // teapotbase.cc:1164-1190 (= quad2)
+  return ob_tpb_quad2(b, length);
+}
+
+OB_API int ob_tpb_bend3(ob_bunch* b, double th) {  // teapotbase.cc:1208-1233
+  OB_TRY OB_CHECK_B(b);
+  ob::Op op = mkop(ob::OP_BEND3);
+  op.p[0] = th;
+  op.p[1] = 1.0 / (b->gamma * b->gamma);
+  op.p[2] = 1.0 / (b->p * b->beta);
+  ob::bunch_queue(b, op);
+  OB_CATCH
+}
+
+OB_API int ob_tpb_bend4(ob_bunch* b, double th) {  // teapotbase.cc:1251-1274
+  OB_TRY OB_CHECK_B(b);
+  ob::Op op = mkop(ob::OP_BEND4);
+  op.p[0] = th;
+  op.p[1] = 1.0 / (b->p * b->beta);
+  ob::bunch_queue(b, op);
+  OB_CATCH
+}
+
+static void bendfringe(ob_bunch* b, double rho, bool out) {  // teapotbase.cc:1292-1348
+  ob::Op op = mkop(out ? ob::OP_BENDFRINGE_OUT : ob::OP_BENDFRINGE_IN);
+  op.p[0] = rho;
+  op.p[1] = 1.0 / (b->p * b->beta);
+  ob::bunch_queue(b, op);
+}
+OB_API int ob_tpb_bendfringeIN(ob_bunch* b, double rho) {
+  OB_TRY OB_CHECK_B(b);
+  bendfringe(b, rho, false);
+  OB_CATCH
+}
+OB_API int ob_tpb_bendfringeOUT(ob_bunch* b, double rho) {
+  OB_TRY OB_CHECK_B(b);
+  bendfringe(b, rho, true);
+  OB_CATCH
+}
+
+// soln (teapotbase.cc:1367-1425).  The zero-field test is on |B| as a double (abs
+// resolves to the double overload there).
+OB_API int ob_tpb_soln(ob_bunch* b, double length, double B, int) {
+  OB_TRY OB_CHECK_B(b);
+  if (fabs(B) < 1.0e-100 || b->charge == 0.) {
+    drift_op(b, length);
+    return 0;
+  }
+  const double v = 2.99792458e+8 * b->beta;
+  if (length > 0.) b->time = b->time + length / v;
+  ob::Op op = mkop(ob::OP_SOLN);
+  op.p[0] = B * b->charge;
+  op.p[1] = length;
+  op.p[2] = 1.0 / (b->gamma * b->gamma);
+  op.p[3] = 1.0 / (b->p * b->beta);
+  ob::bunch_queue(b, op);
+  OB_CATCH
+}
+
+// RingRF (teapotbase.cc:1527-1566): a non-zero synchronous phase changes the synchronous
+// momentum by the energy gain (SyncPart::setMomentum, SyncPart.cc:133-138, 314-317).
+OB_API int ob_tpb_RingRF(ob_bunch* b, double ring_length, int harmonic_numb, double voltage,
+                         double phase_s, int useCharge) {
+  OB_TRY OB_CHECK_B(b);
+  double charge = +1.0;
+  if (useCharge == 1) charge = b->charge;
+  const double coeff = charge;
+  double Factor = 0.;
+  if (ring_length > 0.) Factor = 2.0 * 3.14159265358979323846 / ring_length;
+  const double p_synch_in = b->p;
+  if (phase_s != 0.) {
+    double kin_e = b->T;
+    kin_e += coeff * voltage * sin(phase_s);
+    ob::sync_set_momentum(b, sqrt(kin_e * (kin_e + 2.0 * b->mass)));
+  }
+  ob::Op op = mkop(ob::OP_RINGRF);
+  op.p[0] = harmonic_numb * Factor;
+  op.p[1] = phase_s;
+  op.p[2] = voltage;
+  op.p[3] = coeff;
+  op.p[4] = p_synch_in / b->p;
+  ob::bunch_queue(b, op);
+  OB_CATCH
+}
+
 namespace ob {
 // Queue one map of the public op list through its single entry point (host constants).
 int queue_map_op(ob_bunch* b, const ob_map_op& o) {
@@ -909,6 +1104,19 @@ int queue_map_op(ob_bunch* b, const ob_map_op& o) {
     case OB_MAP_QUAD2: return ob_tpb_quad2(b, o.a);
     case OB_MAP_QUADFRINGE_IN: return ob_tpb_quadfringeIN(b, o.b, o.ic);
     case OB_MAP_QUADFRINGE_OUT: return ob_tpb_quadfringeOUT(b, o.b, o.ic);
+    case OB_MAP_BEND1: return ob_tpb_bend1(b, o.a, o.b);
+    case OB_MAP_BEND2: return ob_tpb_bend2(b, o.a);
+    case OB_MAP_BEND3: return ob_tpb_bend3(b, o.a);
+    case OB_MAP_BEND4: return ob_tpb_bend4(b, o.a);
+    case OB_MAP_BENDFRINGE_IN: return ob_tpb_bendfringeIN(b, o.a);
+    case OB_MAP_BENDFRINGE_OUT: return ob_tpb_bendfringeOUT(b, o.a);
+    case OB_MAP_WEDGEROTATE: return ob_tpb_wedgerotate(b, o.a, o.ia);
+    case OB_MAP_WEDGEDRIFT: return ob_tpb_wedgedrift(b, o.a, o.ia);
+    case OB_MAP_WEDGEBEND: return ob_tpb_wedgebend(b, o.a, o.ia, o.b, o.ib);
+    case OB_MAP_WEDGEBENDCF:
+      return ob_tpb_wedgebendCF(b, o.a, o.ia, o.b, o.n, o.poles, o.kls, o.skews, o.ib, o.ic);
+    case OB_MAP_SOLN: return ob_tpb_soln(b, o.a, o.b, o.ic);
+    case OB_MAP_RINGRF: return ob_tpb_RingRF(b, o.a, o.ia, o.b, o.c, o.ic);
     default: throw Error{"unknown map kind " + std::to_string(o.kind)};
   }
 }

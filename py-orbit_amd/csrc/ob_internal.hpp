@@ -74,6 +74,16 @@ enum OpKind : int32_t {
   OP_KICK,
   OP_SC2P5DRB_KICK,
   OP_SC3D_KICK,
+  OP_BEND1,
+  OP_BEND3,
+  OP_BEND4,
+  OP_BENDFRINGE_IN,
+  OP_BENDFRINGE_OUT,
+  OP_WEDGEROTATE,
+  OP_WEDGEDRIFT,
+  OP_WEDGEBEND,  // wedgebend and wedgebendCF
+  OP_SOLN,
+  OP_RINGRF,
 };
 
 struct Op {
@@ -249,6 +259,8 @@ SyncInfo bunch_sync(ob_bunch* b);
 std::vector<Op> bunch_take_queue(ob_bunch* b);
 // Host-side synchronous time (advanced by the drift maps when queued).
 void bunch_set_sync_time(ob_bunch* b, double t);
+// Synchronous particle kinematics (T, p, beta, gamma, time) of src onto dst.
+void bunch_copy_kinematics(ob_bunch* dst, const ob_bunch* src);
 }  // namespace ob
 struct ob_batch;
 namespace ob {

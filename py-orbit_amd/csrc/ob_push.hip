@@ -102,14 +102,96 @@ __device__ __forceinline__ void quadfringe(const Op& op, double& x, double& xp, 
   }
 }
 
-// FULL = false leaves out the register-hungry multipole fringe and 3D kick, so programs
+// One particle's drift (drifti, teapotbase.cc:120-141).
+__device__ __forceinline__ void drift_one(double length, double gamma2i, double dp_p_coeff,
+                                          double& x, double xp, double& y, double yp,
+                                          double& z, double dE) {
+  const double dp_p = dE * dp_p_coeff;
+  const double KNL = 1.0 / (1.0 + dp_p);
+  x += KNL * length * xp;
+  y += KNL * length * yp;
+  double phifac = (xp * xp + yp * yp + dp_p * dp_p * gamma2i) / 2.0;
+  phifac = (phifac * KNL - dp_p * gamma2i) * KNL;
+  z -= length * phifac;
+}
+
+// wedgebend / wedgebendCF (teapotbase.cc:1037-1080, 1447-1509).  p[0]=ct, p[1]=rho,
+// p[2]=dp_p_coeff, p[3]=gamma2i, p[4+l]=kl of multipole l; i0 = inout | n << 1 |
+// skew bits << 8, i1 = nst, i2 = poles, 4 bits each.  The multipole kick of each step is
+// multpi (:280-315) with kl * sm, divided by pole! per particle as there.
+__device__ __forceinline__ void wedgebend(const Op& op, double& x, double& xp, double& y,
+                                          double& yp, double& z, double dE) {
+  const double ct = op.p[0], rho = op.p[1], dpc = op.p[2], gamma2i = op.p[3];
+  const int inout = op.i0 & 1, nm = (op.i0 >> 1) & 15, skews = (op.i0 >> 8) & 255;
+  const int nst = op.i1;
+  double s;
+  if (inout == 0) {
+    s = -x / ct;
+  } else {
+    const double dp_p = dE * dpc;
+    const double tn = xp / (1.0 + dp_p);
+    s = -x / (ct + tn);
+  }
+  const double sm = s / nst, sm2 = sm / 2.0;
+  for (int j = 0; j < nst; j++) {
+    drift_one(j == 0 ? sm2 : sm, gamma2i, dpc, x, xp, y, yp, z, dE);
+    xp -= sm / rho;
+    for (int l = 0; l < nm; l++) {
+      const int pole = (op.i2 >> (4 * l)) & 15;
+      double fact = 1.0;  // teapotbase.cc:45-57
+      for (int k = 1; k <= pole; k++) fact = k * fact;
+      const double kl1 = (op.p[4 + l] * sm) / fact;
+      const cplx zz = {x, y};
+      cplx zn = {1.0, 0.0};
+      for (int k = 0; k < pole; k++) zn = cmul(zn, zz);
+      if ((skews >> l) & 1) {
+        xp += kl1 * zn.im;
+        yp += kl1 * zn.re;
+      } else {
+        xp -= kl1 * zn.re;
+        yp += kl1 * zn.im;
+      }
+    }
+  }
+  drift_one(sm2, gamma2i, dpc, x, xp, y, yp, z, dE);
+}
+
+// soln (teapotbase.cc:1367-1425).  p: Bc, length, gamma2i, dp_p_coeff.
+__device__ __forceinline__ void soln(const Op& op, double& x, double& xp, double& y,
+                                     double& yp, double& z, double dE) {
+  const double Bc = op.p[0], length = op.p[1], gamma2i = op.p[2];
+  const double dp_p = dE * op.p[3];
+  const double KNL = 1.0 / (1.0 + dp_p);
+  const double cu = y / 2. - xp / Bc;
+  const double cpu = x * Bc / 2. + yp;
+  const double u_init = y / 2. + xp / Bc;
+  const double pu_init = -x * Bc / 2. + yp;
+  const double phase = KNL * Bc * length;
+  const double cs = cos(phase), sn = sin(phase);
+  const double u = u_init * cs + pu_init * sn / Bc;
+  const double pu = -u_init * Bc * sn + pu_init * cs;
+  x = (-pu + cpu) / Bc;
+  xp = 0.5 * (u - cu) * Bc;
+  y = u + cu;
+  yp = 0.5 * (pu + cpu);
+  double phifac = (pu_init * pu_init + Bc * Bc * u_init * u_init + dp_p * dp_p * gamma2i) / 2.0;
+  phifac = (phifac * KNL - dp_p * gamma2i) * KNL;
+  z -= length * phifac;
+}
+
+// Whether an op kind is implemented only by the FULL kernels.
+__host__ __device__ constexpr bool full_only(int kind) {
+  return kind == OP_MULTPFRINGE_IN || kind == OP_MULTPFRINGE_OUT || kind == OP_SC3D_KICK ||
+         kind == OP_WEDGEBEND || kind == OP_SOLN || kind == OP_RINGRF;
+}
+
+// FULL = false leaves out the register-hungry ops (multipole fringe, 3D kick, wedge bend,
+// solenoid, ring RF), so programs
 // without them run with about half the VGPRs (twice the waves per SIMD).
 template <bool FULL>
 __device__ __forceinline__ void apply_op(const Op& op, double& x, double& xp, double& y,
                                          double& yp, double& z, double& dE) {
-  if (!FULL && (op.kind == OP_MULTPFRINGE_IN || op.kind == OP_MULTPFRINGE_OUT ||
-                op.kind == OP_SC3D_KICK))
-    return;  // never reached: the host launches FULL for such programs
+  if (!FULL && full_only(op.kind)) return;  // never reached: the host launches FULL
   switch (op.kind) {
     case OP_DRIFT: {  // teapotbase.cc:158-183. p: length, gamma2i, dp_p_coeff
       const double length = op.p[0], gamma2i = op.p[1];
@@ -194,6 +276,106 @@ __device__ __forceinline__ void apply_op(const Op& op, double& x, double& xp, do
     case OP_SC3D_KICK:
       if constexpr (FULL) sc3d_kick(op, x, xp, y, yp, z, dE);
       break;
+    case OP_BEND1: {  // teapotbase.cc:1099-1145. p: m11 m12 m16 m21 m22 m26 m51 m52 m56,
+                      // length, dp_p_coeff
+      const double dp_p = dE * op.p[10];
+      const double x0 = x, xp0 = xp;
+      x = x0 * op.p[0] + xp0 * op.p[1] + dp_p * op.p[2];
+      xp = x0 * op.p[3] + xp0 * op.p[4] + dp_p * op.p[5];
+      y += op.p[9] * yp;
+      z += x0 * op.p[6] + xp0 * op.p[7] + dp_p * op.p[8];
+      break;
+    }
+    case OP_BEND3: {  // teapotbase.cc:1208-1233. p: th, gamma2i, dp_p_coeff
+      const double th = op.p[0], gamma2i = op.p[1];
+      const double dp_p = dE * op.p[2];
+      const double KNL = 1.0 / (1.0 + dp_p);
+      double phifac = (yp * yp + dp_p * dp_p * gamma2i) / 2.0;
+      xp -= phifac * KNL * th;
+      y += KNL * yp * x * th;
+      phifac = (phifac * KNL - dp_p * gamma2i) * KNL;
+      z -= th * phifac * x;
+      break;
+    }
+    case OP_BEND4: {  // teapotbase.cc:1251-1274. p: th, dp_p_coeff
+      const double th = op.p[0];
+      const double dp_p = dE * op.p[1];
+      const double KNL = 1.0 / (1.0 + dp_p);
+      const double xfac = 1.0 + KNL * xp * th / 2.0;
+      const double phifac = KNL * KNL * xp * xp / 2.0;
+      x *= xfac * xfac;
+      xp /= xfac;
+      z -= th * phifac * x;
+      break;
+    }
+    case OP_BENDFRINGE_IN:    // teapotbase.cc:1292-1311. p: rho, dp_p_coeff
+    case OP_BENDFRINGE_OUT: {  // teapotbase.cc:1329-1348
+      const double rho = op.p[0];
+      const double dp_p = dE * op.p[1];
+      const double KNL = 1.0 / (1.0 + dp_p);
+      if (op.kind == OP_BENDFRINGE_IN) {
+        x += KNL * y * y / (2. * rho);
+        yp -= KNL * xp * y / rho;
+        z += KNL * KNL * xp * y * y / (2. * rho);
+      } else {
+        x -= KNL * y * y / (2. * rho);
+        yp += KNL * xp * y / rho;
+        z -= KNL * KNL * xp * y * y / (2. * rho);
+      }
+      break;
+    }
+    case OP_WEDGEROTATE: {  // teapotbase.cc:923-966. p: cos e, sin e, dp_p_coeff; i0=frinout
+      const double cs = op.p[0], sn = op.p[1], dpc = op.p[2];
+      double dp_p, xp_temp, p0_temp, p0;
+      if (op.i0 == 0) {
+        dp_p = dE * dpc;
+        xp_temp = xp;
+        p0_temp = 1.0 + dp_p;
+        x /= cs;
+        xp = xp_temp * cs + p0_temp * sn;
+        p0 = -xp_temp * sn + p0_temp * cs;
+        dp_p = p0 - 1.0;
+        z = (-x * sn + z) * cs;
+      } else {
+        dp_p = dE * dpc;
+        p0 = 1.0 + dp_p;
+        z = x * sn + z / cs;
+        x *= cs;
+        xp_temp = xp * cs - p0 * sn;
+        p0_temp = xp * sn + p0 * cs;
+        xp = xp_temp;
+        dp_p = p0_temp - 1.0;
+      }
+      dE = dp_p / dpc;
+      break;
+    }
+    case OP_WEDGEDRIFT: {  // teapotbase.cc:986-1012. p: ct, dp_p_coeff, gamma2i; i0=inout
+      const double ct = op.p[0], dpc = op.p[1];
+      double s;
+      if (op.i0 == 0) {
+        const double dp_p = dE * dpc;
+        const double tn = xp / (1.0 + dp_p);
+        s = x / (ct - tn);
+      } else {
+        s = x / ct;
+      }
+      drift_one(s, op.p[2], dpc, x, xp, y, yp, z, dE);
+      break;
+    }
+    case OP_WEDGEBEND:
+      if constexpr (FULL) wedgebend(op, x, xp, y, yp, z, dE);
+      break;
+    case OP_SOLN:
+      if constexpr (FULL) soln(op, x, xp, y, yp, z, dE);
+      break;
+    case OP_RINGRF:  // teapotbase.cc:1527-1566. p: h*2pi/L, phase_s, voltage, coeff, p_in/p_out
+      if constexpr (FULL) {
+        const double deltaV = op.p[2] * (sin(op.p[0] * z + op.p[1]));
+        dE += op.p[3] * deltaV;
+        xp *= op.p[4];
+        yp *= op.p[4];
+      }
+      break;
     default:
       break;
   }
@@ -201,10 +383,8 @@ __device__ __forceinline__ void apply_op(const Op& op, double& x, double& xp, do
 
 // Whether a program holds an op that only the FULL kernels implement.
 bool program_needs_full(const OpProgram& prog) {
-  for (int k = 0; k < prog.nops; k++) {
-    const int kd = prog.ops[k].kind;
-    if (kd == OP_MULTPFRINGE_IN || kd == OP_MULTPFRINGE_OUT || kd == OP_SC3D_KICK) return true;
-  }
+  for (int k = 0; k < prog.nops; k++)
+    if (full_only(prog.ops[k].kind)) return true;
   return false;
 }
 

@@ -1,8 +1,8 @@
 """TEAPOT lattice elements (reference: py/orbit/teapot/teapot.py).
 
 Each element's track() issues the same teapot_base map calls, in the same order and with
-the same arguments, as the reference element; the maps run on the device.
-Elements outside the hot path (bends, solenoids, RF, apertures, monitors) are not built.
+the same arguments, as the reference element; the maps run on the device.  Apertures
+(which need the reference's aperture module) are not built.
 """
 import os
 
@@ -381,3 +381,179 @@ class KickTEAPOT(NodeTEAPOT):
 
     def setWaveform(self, waveform):
         self.waveform = waveform
+
+
+class BendTEAPOT(NodeTEAPOT):
+    """Combined-function bend: wedge edges and bend fringes around the bend1..4 splitting
+    (:978-1150)."""
+
+    def __init__(self, name="bend no name"):
+        NodeTEAPOT.__init__(self, name)
+        self.addParam("poles", [])
+        self.addParam("kls", [])
+        self.addParam("skews", [])
+        self.addParam("ea1", 0.0)
+        self.addParam("ea2", 0.0)
+        self.addParam("rho", 0.0)
+        self.addParam("theta", 1.0e-36)
+        self.setnParts(2)
+
+        def multipole_fringes(node, paramsDict, fn, length):
+            if length == 0.0:
+                return
+            bunch = paramsDict["bunch"]
+            uc = _use_charge(paramsDict)
+            for pole, kl, skew in zip(node.getParam("poles")[:], node.getParam("kls")[:],
+                                      node.getParam("skews")[:]):
+                fn(bunch, pole, kl / length, skew, uc)
+
+        def fringeIN(node, paramsDict):
+            e = node.getParam("ea1")
+            rho = node.getParam("rho")
+            parent = paramsDict["parentNode"]
+            length = parent.getLength()
+            bunch = paramsDict["bunch"]
+            if e != 0.0:
+                TPB.wedgedrift(bunch, e, 0)
+                if node.getUsage():
+                    TPB.wedgerotate(bunch, e, 0)
+                    TPB.bendfringeIN(bunch, rho)
+                    multipole_fringes(node, paramsDict, TPB.multpfringeIN, length)
+                    TPB.wedgerotate(bunch, e, 1)
+                poles = node.getParam("poles")[:]
+                TPB.wedgebendCF(bunch, e, 0, rho, len(poles), poles, node.getParam("kls")[:],
+                                node.getParam("skews")[:], parent.getnParts() - 1,
+                                _use_charge(paramsDict))
+            elif node.getUsage():
+                TPB.bendfringeIN(bunch, rho)
+                multipole_fringes(node, paramsDict, TPB.multpfringeIN, length)
+
+        def fringeOUT(node, paramsDict):
+            e = node.getParam("ea2")
+            rho = node.getParam("rho")
+            parent = paramsDict["parentNode"]
+            length = parent.getLength()
+            bunch = paramsDict["bunch"]
+            if e != 0.0:
+                poles = node.getParam("poles")[:]
+                TPB.wedgebendCF(bunch, e, 1, rho, len(poles), poles, node.getParam("kls")[:],
+                                node.getParam("skews")[:], parent.getnParts() - 1,
+                                _use_charge(paramsDict))
+                if node.getUsage():
+                    TPB.wedgerotate(bunch, -e, 0)
+                    TPB.bendfringeOUT(bunch, rho)
+                    multipole_fringes(node, paramsDict, TPB.multpfringeOUT, length)
+                    TPB.wedgerotate(bunch, -e, 1)
+                TPB.wedgedrift(bunch, e, 1)
+            elif node.getUsage():
+                TPB.bendfringeOUT(bunch, rho)
+                multipole_fringes(node, paramsDict, TPB.multpfringeOUT, length)
+
+        self.setFringeFieldFunctionIN(fringeIN)
+        self.setFringeFieldFunctionOUT(fringeOUT)
+        self.setType("bend teapot")
+
+    def initialize(self):
+        if self.getnParts() < 2:
+            orbitFinalize("The Bend Combined Functions TEAPOT class instance should have more "
+                          "than 2 parts!" + os.linesep + "Method initialize():" + os.linesep
+                          + "Name of element=" + self.getName() + os.linesep
+                          + "Type of element=" + self.getType() + os.linesep
+                          + "nParts =" + str(self.getnParts()))
+        self.addParam("rho", self.getLength() / self.getParam("theta"))
+        _split_parts(self, "Bend Combined Functions")
+
+    def track(self, paramsDict):
+        nParts = self.getnParts()
+        index = self.getActivePartIndex()
+        length = self.getLength(index)
+        bunch = paramsDict["bunch"]
+        uc = _use_charge(paramsDict)
+        theta = self.getParam("theta") / (nParts - 1)
+        if index == 0:
+            TPB.bend1(bunch, length, theta / 2.0)
+            return
+        last = index == nParts - 1
+        step = length if last else length / 2.0
+        TPB.bend2(bunch, step)
+        TPB.bend3(bunch, theta / 2.0)
+        TPB.bend4(bunch, theta / 2.0)
+        for pole, kl, skew in zip(self.getParam("poles"), self.getParam("kls"),
+                                  self.getParam("skews")):
+            TPB.multp(bunch, pole, kl / (nParts - 1), skew, uc)
+        TPB.bend4(bunch, theta / 2.0)
+        TPB.bend3(bunch, theta / 2.0)
+        TPB.bend2(bunch, step)
+        TPB.bend1(bunch, length, theta / 2.0 if last else theta)
+
+
+class SolenoidTEAPOT(NodeTEAPOT):
+    """Solenoid (:648-682); B in 1/m, scaled by an optional waveform."""
+
+    def __init__(self, name="solenoid no name"):
+        NodeTEAPOT.__init__(self, name)
+        self.setType("solenoid teapot")
+        self.addParam("B", 0.0)
+        self.waveform = None
+
+    def track(self, paramsDict):
+        length = self.getLength(self.getActivePartIndex())
+        strength = self.waveform.getStrength() if self.waveform else 1.0
+        TPB.soln(paramsDict["bunch"], length, strength * self.getParam("B"),
+                 _use_charge(paramsDict))
+
+    def setWaveform(self, waveform):
+        self.waveform = waveform
+
+
+class RingRFTEAPOT(NodeTEAPOT):
+    """Ring RF cavity: one RingRF call per harmonic (:1153-1219)."""
+
+    def __init__(self, name="RingRF no name"):
+        NodeTEAPOT.__init__(self, name)
+        self.addParam("harmonics", [])
+        self.addParam("voltages", [])
+        self.addParam("phases", [])
+        self.addParam("ring_length", 0.0)
+        self.setType("RingRF teapot")
+        self.setnParts(1)
+
+    def addRF(self, harmonic, voltage, phase):
+        self.getParam("harmonics").append(harmonic)
+        self.getParam("voltages").append(voltage)
+        self.getParam("phases").append(phase)
+
+    def initialize(self):
+        if self.getnParts() > 1:
+            orbitFinalize("The Ring RF TEAPOT class instance should not have more than 1 part!"
+                          + os.linesep + "Method initialize():" + os.linesep
+                          + "Name of element=" + self.getName() + os.linesep
+                          + "Type of element=" + self.getType() + os.linesep
+                          + "nParts =" + str(self.getnParts()) + os.linesep
+                          + "length =" + str(self.getLength()))
+
+    def track(self, paramsDict):
+        bunch = paramsDict["bunch"]
+        uc = _use_charge(paramsDict)
+        ring_length = self.getParam("ring_length")
+        for h, v, ph in zip(self.getParam("harmonics"), self.getParam("voltages"),
+                            self.getParam("phases")):
+            TPB.RingRF(bunch, ring_length, h, v, ph, uc)
+
+
+class MonitorTEAPOT(NodeTEAPOT):
+    """Beam-position monitor: records the bunch centroid (:601-623), from the device
+    moments of BunchTwissAnalysis."""
+
+    def __init__(self, name="Monitor no name"):
+        from bunch import BunchTwissAnalysis
+        NodeTEAPOT.__init__(self, name)
+        self.setType("monitor teapot")
+        self.twiss = BunchTwissAnalysis()
+
+    def track(self, paramsDict):
+        self.twiss.analyzeBunch(paramsDict["bunch"])
+        self.addParam("xAvg", self.twiss.getAverage(0))
+        self.addParam("xpAvg", self.twiss.getAverage(1))
+        self.addParam("yAvg", self.twiss.getAverage(2))
+        self.addParam("ypAvg", self.twiss.getAverage(3))

@@ -57,10 +57,10 @@ def replay_oracle(calls, obunch, calc):
         if name in ("sc2p5drb", "sc2p5d", "sc3d"):
             calc.trackBunch(obunch, *args)
         else:
-            if name in ("quad1", "kick", "multp", "multpfringeIN", "multpfringeOUT",
+            if name in ("quad1", "kick", "multp", "multpfringeIN", "multpfringeOUT", "soln",
                         "quadfringeIN", "quadfringeOUT"):
                 # drop useCharge (ignored by the reference maps)
-                nargs = {"quad1": 2, "kick": 3, "multp": 3, "multpfringeIN": 3,
+                nargs = {"quad1": 2, "kick": 3, "multp": 3, "multpfringeIN": 3, "soln": 2,
                          "multpfringeOUT": 3, "quadfringeIN": 1, "quadfringeOUT": 1}[name]
                 args = args[:nargs]
             obunch.apply(name, *args)

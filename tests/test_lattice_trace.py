@@ -85,3 +85,87 @@ def test_orbit_finalize_raises():
     q = QuadTEAPOT("Q")
     with pytest.raises(OrbitFinalizeError):
         q.setnParts(1)
+
+
+def _trace_nodes(monkeypatch, *nodes):
+    import orbit.teapot.teapot as tt
+    from helpers import TraceTPB
+    from orbit.teapot import TEAPOT_Lattice
+    tpb = TraceTPB()
+    monkeypatch.setattr(tt, "TPB", tpb)
+    lat = TEAPOT_Lattice()
+    for nd in nodes:
+        lat.addNode(nd)
+    lat.initialize()
+
+    class B:
+        pass
+    lat.trackBunch(B())
+    return tpb.calls
+
+
+def test_bend_with_edges_and_sextupole(monkeypatch):
+    """BendTEAPOT, 3 parts, wedge angles on both ends, one sextupole component
+    (teapot.py:999-1150): wedge drift, rotate, bend fringe, multipole fringe with kl/L,
+    rotate back, wedge bend with nParts-1 steps; parts 0 / 1 / 2 of the bend1..4 splitting
+    with theta/(nParts-1) and kl/(nParts-1); the exit mirrored with -e."""
+    from orbit.teapot import BendTEAPOT
+    b = BendTEAPOT("B")
+    b.setLength(2.0)
+    b.setnParts(3)
+    b.setParam("theta", 0.2)
+    b.setParam("ea1", 0.05)
+    b.setParam("ea2", 0.07)
+    b.setParam("poles", [2])
+    b.setParam("kls", [0.5])
+    b.setParam("skews", [0])
+    calls = _trace_nodes(monkeypatch, b)
+    rho = 2.0 / 0.2
+    th = 0.1  # theta / (nParts - 1)
+    L0, L1 = 0.5, 1.0  # part lengths: half steps at the ends
+    expected = [("wedgedrift", 0.05, 0), ("wedgerotate", 0.05, 0), ("bendfringeIN", rho),
+                ("multpfringeIN", 2, 0.25, 0, 1), ("wedgerotate", 0.05, 1),
+                ("wedgebendCF", 0.05, 0, rho, 1, [2], [0.5], [0], 2, 1),
+                ("bend1", L0, th / 2),
+                ("bend2", L1 / 2), ("bend3", th / 2), ("bend4", th / 2), ("multp", 2, 0.25, 0, 1),
+                ("bend4", th / 2), ("bend3", th / 2), ("bend2", L1 / 2), ("bend1", L1, th),
+                ("bend2", L0), ("bend3", th / 2), ("bend4", th / 2), ("multp", 2, 0.25, 0, 1),
+                ("bend4", th / 2), ("bend3", th / 2), ("bend2", L0), ("bend1", L0, th / 2),
+                ("wedgebendCF", 0.07, 1, rho, 1, [2], [0.5], [0], 2, 1),
+                ("wedgerotate", -0.07, 0), ("bendfringeOUT", rho),
+                ("multpfringeOUT", 2, 0.25, 0, 1), ("wedgerotate", -0.07, 1),
+                ("wedgedrift", 0.07, 1)]
+    assert calls == expected
+
+
+def test_bend_without_edges_solenoid_rf_monitor(monkeypatch):
+    from orbit.teapot import BendTEAPOT, SolenoidTEAPOT, RingRFTEAPOT
+    b = BendTEAPOT("B")
+    b.setLength(1.0)
+    b.setParam("theta", 0.1)
+    s = SolenoidTEAPOT("S")
+    s.setLength(0.4)
+    s.setParam("B", 0.8)
+    rf = RingRFTEAPOT("RF")
+    rf.setParam("ring_length", 248.0)
+    rf.addRF(1, 1e-5, 0.0)
+    rf.addRF(2, -5e-6, 0.1)
+    b.setUsageFringeFieldOUT(False)
+    calls = _trace_nodes(monkeypatch, b, s, rf)
+    rho = 1.0 / 0.1
+    assert calls == [("bendfringeIN", rho), ("bend1", 0.5, 0.05),
+                     ("bend2", 0.5), ("bend3", 0.05), ("bend4", 0.05), ("bend4", 0.05),
+                     ("bend3", 0.05), ("bend2", 0.5), ("bend1", 0.5, 0.05),
+                     ("soln", 0.4, 0.8, 1),
+                     ("RingRF", 248.0, 1, 1e-5, 0.0, 1), ("RingRF", 248.0, 2, -5e-6, 0.1, 1)]
+
+
+def test_bend_and_rf_part_rules():
+    from orbit.teapot import BendTEAPOT, RingRFTEAPOT
+    from orbit.utils import OrbitFinalizeError
+    b = BendTEAPOT("B")
+    with pytest.raises(OrbitFinalizeError):
+        b.setnParts(1)
+    rf = RingRFTEAPOT("RF")
+    with pytest.raises(OrbitFinalizeError):
+        rf.setnParts(2)

@@ -204,7 +204,9 @@ def test_drift_composes():
 MAPS = [("drift", (0.7,)), ("quad1", (0.25, 0.5)), ("quad1", (0.25, -0.5)), ("quad2", (0.25,)),
         ("quadfringeIN", (0.5,)), ("quadfringeOUT", (-0.5,)), ("multp", (2, 0.4, 0)),
         ("multpfringeIN", (2, 2.0, 0)), ("multpfringeOUT", (1, -0.7, 1)),
-        ("rotatexy", (0.3,)), ("kick", (1e-4, 2e-4, 1e-5))]
+        ("rotatexy", (0.3,)), ("kick", (1e-4, 2e-4, 1e-5)), ("bend1", (0.5, 0.1)),
+        ("bend1", (1.2, -0.3)), ("bend2", (0.5,)), ("bend3", (0.05,)), ("bend4", (0.05,)),
+        ("soln", (0.5, 0.8)), ("soln", (0.3, -2.0)), ("RingRF", (100.0, 3, 1e-4, 0.0, 1))]
 
 
 @pytest.mark.parametrize("op,args", MAPS, ids=[m[0] + str(i) for i, m in enumerate(MAPS)])
@@ -228,6 +230,109 @@ def test_maps_are_symplectic(op, args):
             ob.apply(op, *args)
             J[:, j] = (ob.coords[:, 0] - ob.coords[:, 1]) / (2 * h)
         assert np.abs(J.T @ Om @ J - Om).max() < 1e-6
+
+
+def _rand6(n, seed, scale=1.0):
+    r = np.random.default_rng(seed)
+    c = np.empty((6, n))
+    c[0] = r.normal(0, 3e-3, n) * scale
+    c[1] = r.normal(0, 1e-3, n) * scale
+    c[2] = r.normal(0, 3e-3, n) * scale
+    c[3] = r.normal(0, 1e-3, n) * scale
+    c[4] = r.uniform(-1, 1, n)
+    c[5] = r.normal(0, 2e-3, n) * scale
+    return c
+
+
+def _apply(c, *ops):
+    ob = O.Bunch(c, kin_energy=1.0)
+    for op in ops:
+        ob.apply(op[0], *op[1:])
+    return ob
+
+
+@pytest.mark.parametrize("pair", [(("wedgerotate", 0.13, 0), ("wedgerotate", 0.13, 1)),
+                                  (("bendfringeIN", 4.0), ("bendfringeOUT", 4.0))])
+def test_edge_maps_undo_each_other(pair):
+    """wedgerotate(e, 1) after wedgerotate(e, 0) is a rotation there and back, and the
+    bend fringe OUT kick cancels the IN kick (the pair BendTEAPOT applies around the
+    multipole fringes)."""
+    c = _rand6(2000, 1)
+    # dE goes through dp/p = p0 - 1: absolute rounding ~1e-16 on dE ~1e-3
+    assert np.allclose(_apply(c, *pair).coords, c, rtol=1e-13, atol=1e-14)
+
+
+def test_wedgedrift_lands_on_the_edge():
+    """inout 0: each particle drifts (in its own straight line, on momentum) to the edge
+    line s = x tan(e); the drift length recovered from y is x_new tan(e)."""
+    c = _rand6(2000, 2)
+    c[5] = 0.0
+    e = 0.2
+    out = _apply(c, ("wedgedrift", e, 0)).coords
+    s = (out[2] - c[2]) / c[3]
+    assert np.allclose(s, out[0] * np.tan(e), rtol=1e-9, atol=1e-15)
+
+
+def test_wedgebend_is_stepped_drift_and_curvature_kick():
+    """Straight infinite-radius steps reduce to one drift of length -x tan(e) (inout 0);
+    with a radius the total x' change is -s/rho, since the drifts leave x' alone."""
+    c = _rand6(2000, 3)
+    e, n = 0.15, 6
+    flat = _apply(c, ("wedgebend", e, 0, 1e300, n)).coords
+    s = -c[0] * np.tan(e)
+    ref = np.empty_like(c)
+    for i in range(c.shape[1]):
+        ob = O.Bunch(c[:, i:i + 1], kin_energy=1.0)
+        ob.apply("drift", s[i])
+        ref[:, i] = ob.coords[:, 0]
+    assert np.allclose(flat, ref, rtol=1e-12, atol=1e-16)
+    rho = 7.0
+    bent = _apply(c, ("wedgebend", e, 0, rho, n)).coords
+    assert np.allclose(bent[1] - c[1], -s / rho, rtol=1e-12, atol=1e-16)
+    # wedgebendCF with no multipoles is wedgebend; with a zero strength too
+    cf = _apply(c, ("wedgebendCF", e, 0, rho, 1, [2], [0.0], [0], n)).coords
+    assert np.array_equal(cf, bent)
+
+
+def test_bend1_composes():
+    """The linear sector-bend map: two halves (same radius) equal the whole."""
+    c = _rand6(2000, 4)
+    half = _apply(c, ("bend1", 0.6, 0.05), ("bend1", 0.6, 0.05)).coords
+    whole = _apply(c, ("bend1", 1.2, 0.1)).coords
+    assert np.allclose(half, whole, rtol=1e-12, atol=1e-15)
+
+
+def test_soln_composes_and_tends_to_a_drift():
+    c = _rand6(2000, 5)
+    two = _apply(c, ("soln", 0.2, 1.3), ("soln", 0.5, 1.3)).coords
+    one = _apply(c, ("soln", 0.7, 1.3)).coords
+    assert np.allclose(two, one, rtol=1e-11, atol=1e-15)
+    # the departure from a drift shrinks linearly with the field
+    drift = _apply(c, ("drift", 0.7)).coords
+    d3 = np.abs(_apply(c, ("soln", 0.7, 1e-3)).coords - drift).max()
+    d4 = np.abs(_apply(c, ("soln", 0.7, 1e-4)).coords - drift).max()
+    assert d3 / d4 == pytest.approx(10.0, rel=0.05)
+    # below 1e-100 the reference switches to drift exactly
+    assert np.array_equal(_apply(c, ("soln", 0.7, 1e-101)).coords, drift)
+
+
+def test_ringrf_closed_form():
+    """dE += V sin(2 pi h z / L + phi); a non-zero phi raises the synchronous energy by
+    V sin(phi), and x', y' scale by p_in / p_out."""
+    c = _rand6(500, 6)
+    L, h, V, phi = 248.0, 4, 2e-5, 0.3
+    ob = O.Bunch(c, kin_energy=1.0)
+    s0 = ob.sync
+    ob.apply("RingRF", L, h, V, phi, 1)
+    s1 = ob.sync
+    m = s0["mass"]
+    T1 = s0["T"] + V * math.sin(phi)
+    p1 = math.sqrt(T1 * (T1 + 2 * m))
+    assert s1["T"] == pytest.approx(T1, rel=1e-14)
+    assert s1["p"] == pytest.approx(p1, rel=1e-14)
+    assert np.allclose(ob.coords[5], c[5] + V * np.sin(2 * np.pi * h * c[4] / L + phi),
+                       rtol=1e-13, atol=1e-19)
+    assert np.allclose(ob.coords[1], c[1] * s0["p"] / p1, rtol=1e-14, atol=0)
 
 
 # ---------------------------------------------------------------- space-charge kicks
