@@ -29,6 +29,7 @@ extern "C" {
 typedef struct ob_bunch ob_bunch;
 typedef struct ob_sc2p5drb ob_sc2p5drb;
 typedef struct ob_sc2p5d ob_sc2p5d;
+typedef struct ob_boundary2d ob_boundary2d;
 typedef struct ob_sc3d ob_sc3d;
 
 /* ---- library ---------------------------------------------------------- */
@@ -192,14 +193,42 @@ int ob_sc2p5d_destroy(ob_sc2p5d* c);
 /* trackBunch(bunch, length) (:77-120, bunchAnalysis :122-234): the grid keeps the
  * calculator's x/y ratio while the beam's is within 25% of it, else follows the beam and
  * the solver's Green function is rebuilt on the device; transverse kick only, queued on
- * the bunch like every kick.  The boundary form (wrap :139) is not provided. */
+ * the bunch like every kick. */
 int ob_sc2p5d_track(ob_sc2p5d* c, ob_bunch* b, double length);
+/* trackBunch(bunch, length, boundary) (:77-120, 174-234; wrap :139): the grid is the
+ * boundary's box, the boundary's potential (BaseBoundary2D.cc:349-458) is added to phi on
+ * the device, and particles outside the boundary are not kicked.  bnd NULL = no boundary. */
+int ob_sc2p5d_track_boundary(ob_sc2p5d* c, ob_bunch* b, double length, ob_boundary2d* bnd);
 /* getRhoGrid / getPhiGrid / getLongGrid (wrap :174-177): which = 0 rho, 1 phi, 2 z line
  * density; lims as for ob_sc2p5drb_get_grid. */
 int ob_sc2p5d_get_grid(ob_sc2p5d* c, int which, double* out, double* lims);
 /* out[7] = xy_ratio, solver xMin, xMax, yMin, yMax, totalMacrosize of the last step,
  * Green function rebuilds so far (the calculator's private state, :152-172, 206-216). */
 int ob_sc2p5d_get_state(ob_sc2p5d* c, double* out);
+
+/* ---- Boundary2D (src/spacecharge/BaseBoundary2D.cc, ShapedBoundary2D.cc;
+ *      wrap_boundary2d.cc:38-75, 208-220) ---- */
+/* Boundary2D(nPoints, nModes): points set one by one, then initialize (2*nModes+1 must
+ * be below nPoints, BaseBoundary2D.cc:57-70). */
+int ob_boundary2d_create(int nPoints, int nModes, ob_boundary2d** out);
+/* Boundary2D(nPoints, nModes, shape, xDim, yDim): shape "Circle", "Ellipse" or
+ * "Rectangle" (ShapedBoundary2D.cc:9-127); initialized on return. */
+int ob_boundary2d_create_shaped(int nPoints, int nModes, const char* shape, double xDim,
+                                double yDim, ob_boundary2d** out);
+int ob_boundary2d_destroy(ob_boundary2d* b);
+int ob_boundary2d_set_point(ob_boundary2d* b, int i, double x, double y);   /* :160 */
+int ob_boundary2d_get_point(ob_boundary2d* b, int i, double* x, double* y); /* :150 */
+/* initialize (initializeBPs, :188-230): limits and the least-squares projector. */
+int ob_boundary2d_initialize(ob_boundary2d* b);
+/* lims[4] = xMin, xMax, yMin, yMax; info[4] = nPoints, nModes, initialized, shape type
+ * (0 no shape, 1 Circle, 2 Ellipse, 3 Rectangle). */
+int ob_boundary2d_info(ob_boundary2d* b, double* lims, int* info);
+/* isInside(x, y): *out = +1 inside, -1 outside. */
+int ob_boundary2d_is_inside(ob_boundary2d* b, double x, double y, int* out);
+/* addBoundaryPotential(rhoGrid, phiGrid) (:349-458) on a host phi grid [nx][ny] spanning
+ * lims[4] (computed on the device). */
+int ob_boundary2d_add_potential(ob_boundary2d* b, double* phi, int nx, int ny,
+                                const double* lims);
 
 /* ---- SpaceChargeCalc3D (src/spacecharge/SpaceChargeCalc3D.cc) ---- */
 int ob_sc3d_create(int nx, int ny, int nz, ob_sc3d** out);             /* :32-58 */

@@ -82,6 +82,16 @@ def lib():
             "orc_sc2p5d_track": (i, [vp, bp, d]),
             "orc_sc2p5d_get_grid": (i, [vp, i, _D, _D]),
             "orc_sc2p5d_state": (None, [vp, _D]),
+            "orc_sc2p5d_track_boundary": (i, [vp, bp, d, vp]),
+            "orc_boundary2d_create": (vp, [i, i]),
+            "orc_boundary2d_create_shaped": (vp, [i, i, i, d, d]),
+            "orc_boundary2d_destroy": (None, [vp]),
+            "orc_boundary2d_set_point": (None, [vp, i, d, d]),
+            "orc_boundary2d_initialize": (i, [vp]),
+            "orc_boundary2d_is_inside": (i, [vp, d, d]),
+            "orc_boundary2d_add_potential": (None, [vp, vp]),
+            "orc_boundary2d_info": (None, [vp, _I, _D]),
+            "orc_boundary2d_points": (None, [vp, _D, _D]),
             "orc_sc3d_create": (vp, [i, i, i]), "orc_sc3d_destroy": (None, [vp]),
             "orc_sc3d_set_ratio_limit": (None, [vp, d]), "orc_sc3d_track": (i, [vp, bp, d]),
             "orc_sc3d_get_grid": (C.c_int64, [vp, i, _D, _D]),
@@ -210,8 +220,11 @@ class SC2p5D:
             _lib.orc_sc2p5d_destroy(self.h)
             self.h = None
 
-    def trackBunch(self, bunch, length):
-        rc = lib().orc_sc2p5d_track(self.h, bunch.ref, length)
+    def trackBunch(self, bunch, length, boundary=None):
+        if boundary is None:
+            rc = lib().orc_sc2p5d_track(self.h, bunch.ref, length)
+        else:
+            rc = lib().orc_sc2p5d_track_boundary(self.h, bunch.ref, length, boundary.h)
         if rc != 0:
             raise RuntimeError("oracle: bunch has zero extent")
 
@@ -229,6 +242,74 @@ class SC2p5D:
         o = np.zeros(7)
         lib().orc_sc2p5d_state(self.h, _dp(o))
         return float(o[0]), tuple(o[1:5]), float(o[5]), int(o[6])
+
+
+class _Grid2DC(C.Structure):
+    _fields_ = [("nx", C.c_int), ("ny", C.c_int)] + [
+        (n, C.c_double) for n in ("xMin", "xMax", "yMin", "yMax", "dx", "dy")] + [("arr", _D)]
+
+
+SHAPES = {"Circle": 1, "Ellipse": 2, "Rectangle": 3}
+
+
+class Boundary2D:
+    """Boundary2D(nPoints, nModes) or Boundary2D(nPoints, nModes, shape, xDim[, yDim])
+    (BaseBoundary2D.cc, ShapedBoundary2D.cc)."""
+
+    def __init__(self, nPoints, nModes, shape=None, xDim=0.0, yDim=None):
+        L = lib()
+        if shape is None:
+            self.h = L.orc_boundary2d_create(nPoints, nModes)
+        else:
+            self.h = L.orc_boundary2d_create_shaped(nPoints, nModes, SHAPES[shape], xDim,
+                                                    xDim if yDim is None else yDim)
+        if not self.h:
+            raise RuntimeError("oracle: invalid boundary")
+
+    def __del__(self):
+        if getattr(self, "h", None) and _lib is not None:
+            _lib.orc_boundary2d_destroy(self.h)
+            self.h = None
+
+    def setBoundaryPoint(self, i, x, y):
+        lib().orc_boundary2d_set_point(self.h, i, x, y)
+
+    def initialize(self):
+        if lib().orc_boundary2d_initialize(self.h) != 0:
+            raise RuntimeError("oracle: singular boundary matrix")
+
+    def isInside(self, x, y):
+        return lib().orc_boundary2d_is_inside(self.h, x, y)
+
+    def info(self):
+        n = C.c_int()
+        lims = np.zeros(4)
+        lib().orc_boundary2d_info(self.h, C.byref(n), _dp(lims))
+        x, y = np.zeros(n.value), np.zeros(n.value)
+        lib().orc_boundary2d_points(self.h, _dp(x), _dp(y))
+        return x, y, lims
+
+    def addBoundaryPotential(self, phi, lims):
+        """phi [nx][ny] on the grid spanning lims; returns the corrected copy."""
+        phi = np.ascontiguousarray(phi, dtype=np.float64).copy()
+        nx, ny = phi.shape
+        g = _Grid2DC(nx, ny, lims[0], lims[1], lims[2], lims[3],
+                     (lims[1] - lims[0]) / (nx - 1), (lims[3] - lims[2]) / (ny - 1), _dp(phi))
+        lib().orc_boundary2d_add_potential(self.h, C.byref(g))
+        return phi
+
+
+def Grid2DValue(arr, lims, x, y):
+    """Grid2D::getValue (Grid2D.cc:82-108) of a [nx][ny] array spanning lims, at (x, y)."""
+    arr = np.ascontiguousarray(arr, dtype=np.float64)
+    nx, ny = arr.shape
+    g = _Grid2DC(nx, ny, lims[0], lims[1], lims[2], lims[3],
+                 (lims[1] - lims[0]) / (nx - 1), (lims[3] - lims[2]) / (ny - 1), _dp(arr))
+    L = lib()
+    f = L.orc_grid2d_value
+    f.restype = C.c_double
+    f.argtypes = [C.c_void_p, C.c_double, C.c_double]
+    return f(C.byref(g), x, y)
 
 
 class SC3D:
@@ -359,6 +440,36 @@ def fft_r2c_3d(a):
 # and in the reference Grid build (ref_*, oracle/_ref, present only where the
 # reference tree was available at build time; tests that need it skip
 # otherwise).
+
+
+def ref_boundary2d(shape, nPoints, nModes, xDim=0.0, yDim=0.0, points=None, tests=None,
+                   phi=None, glims=None):
+    """The reference's Base/ShapedBoundary2D (oracle/_ref): points, limits, isInside of
+    the test points and, with phi, addBoundaryPotential on the grid spanning glims."""
+    L = C.CDLL(os.path.join(HERE, "_ref", "libref_grid.so"))
+    f = L.ref_boundary2d
+    f.restype = C.c_int
+    f.argtypes = [C.c_int, C.c_int, C.c_int, C.c_double, C.c_double, _D, _D, _D, _D, _D,
+                  C.c_long, _D, _D, _I, C.c_int, C.c_int, _D, _D]
+    px = np.ascontiguousarray(points[0] if points is not None else np.zeros(1), dtype=np.float64)
+    py = np.ascontiguousarray(points[1] if points is not None else np.zeros(1), dtype=np.float64)
+    bx, by = np.zeros(max(nPoints, 1)), np.zeros(max(nPoints, 1))
+    lims = np.zeros(4)
+    tx = np.ascontiguousarray(tests[0] if tests is not None else np.zeros(1), dtype=np.float64)
+    ty = np.ascontiguousarray(tests[1] if tests is not None else np.zeros(1), dtype=np.float64)
+    n = len(tests[0]) if tests is not None else 0
+    inside = np.zeros(max(n, 1), dtype=np.intc)
+    out = None
+    nx = ny = 0
+    gl = np.zeros(4)
+    if phi is not None:
+        out = np.ascontiguousarray(phi, dtype=np.float64).copy()
+        nx, ny = out.shape
+        gl = np.ascontiguousarray(glims, dtype=np.float64)
+    np_ = f(SHAPES.get(shape, 0) if shape else 0, nPoints, nModes, xDim, yDim, _dp(px), _dp(py),
+            _dp(bx), _dp(by), _dp(lims), n, _dp(tx), _dp(ty), _ip(inside), nx, ny, _dp(gl),
+            _dp(out) if out is not None else _D())
+    return bx[:np_], by[:np_], lims, inside[:n], out
 
 
 def ref_available():

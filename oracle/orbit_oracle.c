@@ -1435,6 +1435,236 @@ int orc_sc2p5drb_get_grid(const orc_sc2p5drb* c, int which, double* out, double*
 void orc_sc2p5drb_get_moments(const orc_sc2p5drb* c, double o[4]) { memcpy(o, c->mom, sizeof(c->mom)); }
 
 /* ------------------------------------------------------------------------ */
+/* Boundary2D: src/spacecharge/BaseBoundary2D.cc, ShapedBoundary2D.cc */
+struct orc_boundary2d {
+  int np, nm, shape;        /* points, free-space modes, 0 none / 1 circle / 2 ellipse / 3 rect */
+  double *bx, *by;
+  double xMin, xMax, yMin, yMax, rnorm;
+  double p0, p1;            /* circle r | ellipse a, b | rectangle a, b */
+  double* lsq;              /* [(2 nm + 1)][np] */
+  double* f;                /* [(2 nm + 1)] scratch */
+};
+/* the mode functions 1, (r/R)^k cos k phi, (r/R)^k sin k phi at (x, y) (:316-345) */
+static void bnd_functions(const orc_boundary2d* b, double x, double y, double* f) {
+  int nf = 2 * b->nm + 1;
+  double r2 = x * x + y * y;
+  if (r2 == 0.) {
+    f[0] = 1.0;
+    for (int i = 1; i < nf; i++) f[i] = 0.0;
+    return;
+  }
+  double r = sqrt(r2);
+  double sn = y / r, cs = x / r, snk = sn, csk = cs;
+  double q = r / b->rnorm, rk = 1.0;
+  f[0] = 1.;
+  for (int k = 0; k < b->nm; k++) {
+    rk *= q;
+    f[2 * k + 1] = rk * csk;
+    f[2 * k + 2] = rk * snk;
+    double s1 = sn * csk + cs * snk;
+    double c1 = cs * csk - sn * snk;
+    snk = s1;
+    csk = c1;
+  }
+}
+/* In-place inverse by Gauss-Jordan elimination with full pivoting, columns swapped back
+   at the end (:233-313, the Numerical Recipes scheme). 0 on success. */
+static int bnd_gauss_jordan_inverse(double* a, int n) {
+  int* pivoted = (int*)calloc(n, sizeof(int));
+  int* prow = (int*)malloc(sizeof(int) * n);
+  int* pcol = (int*)malloc(sizeof(int) * n);
+  int rc = 0;
+  for (int step = 0; step < n && rc == 0; step++) {
+    double big = 0.0;
+    int irow = 0, icol = 0;
+    for (int j = 0; j < n; j++) {
+      if (pivoted[j] == 1) continue;
+      for (int k = 0; k < n; k++) {
+        if (pivoted[k] == 0) {
+          if (fabs(a[j * n + k]) >= big) {
+            big = fabs(a[j * n + k]);
+            irow = j;
+            icol = k;
+          }
+        } else if (pivoted[k] > 1) {
+          rc = 1;
+        }
+      }
+    }
+    if (rc) break;
+    pivoted[icol] += 1;
+    if (irow != icol)
+      for (int l = 0; l < n; l++) {
+        double t = a[irow * n + l];
+        a[irow * n + l] = a[icol * n + l];
+        a[icol * n + l] = t;
+      }
+    prow[step] = irow;
+    pcol[step] = icol;
+    if (a[icol * n + icol] == 0.0) { rc = 2; break; }
+    double inv = 1.0 / a[icol * n + icol];
+    a[icol * n + icol] = 1.0;
+    for (int l = 0; l < n; l++) a[icol * n + l] *= inv;
+    for (int r = 0; r < n; r++) {
+      if (r == icol) continue;
+      double d = a[r * n + icol];
+      a[r * n + icol] = 0.0;
+      for (int l = 0; l < n; l++) a[r * n + l] -= a[icol * n + l] * d;
+    }
+  }
+  if (rc == 0)
+    for (int l = n - 1; l >= 0; l--)
+      if (prow[l] != pcol[l])
+        for (int k = 0; k < n; k++) {
+          double t = a[k * n + prow[l]];
+          a[k * n + prow[l]] = a[k * n + pcol[l]];
+          a[k * n + pcol[l]] = t;
+        }
+  free(pivoted); free(prow); free(pcol);
+  return rc;
+}
+orc_boundary2d* orc_boundary2d_create(int np, int nm) { /* :22-95 */
+  if (2 * nm + 1 >= np) return NULL;
+  orc_boundary2d* b = (orc_boundary2d*)calloc(1, sizeof(orc_boundary2d));
+  b->np = np; b->nm = nm; b->shape = 0;
+  b->bx = (double*)malloc(sizeof(double) * np);
+  b->by = (double*)malloc(sizeof(double) * np);
+  for (int i = 0; i < np; i++) b->bx[i] = b->by[i] = DBL_MAX;
+  b->lsq = (double*)calloc((size_t)(2 * nm + 1) * np, sizeof(double));
+  b->f = (double*)calloc(2 * nm + 1, sizeof(double));
+  b->xMin = DBL_MAX; b->xMax = -DBL_MAX; b->yMin = DBL_MAX; b->yMax = -DBL_MAX;
+  return b;
+}
+void orc_boundary2d_destroy(orc_boundary2d* b) {
+  if (!b) return;
+  free(b->bx); free(b->by); free(b->lsq); free(b->f); free(b);
+}
+void orc_boundary2d_set_point(orc_boundary2d* b, int i, double x, double y) {
+  if (i >= 0 && i < b->np) { b->bx[i] = x; b->by[i] = y; }
+}
+/* initializeBPs (:188-230): limits, norm radius, the least-squares projector
+   LSQ = (F^T F)^-1 F^T with F[j][k] = f_k(point j). */
+int orc_boundary2d_initialize(orc_boundary2d* b) {
+  int nf = 2 * b->nm + 1;
+  b->xMin = DBL_MAX; b->xMax = -DBL_MAX; b->yMin = DBL_MAX; b->yMax = -DBL_MAX;
+  for (int i = 0; i < b->np; i++) {
+    double x = b->bx[i], y = b->by[i];
+    if (b->xMin > x) b->xMin = x;
+    if (b->xMax < x) b->xMax = x;
+    if (b->yMin > y) b->yMin = y;
+    if (b->yMax < y) b->yMax = y;
+  }
+  b->rnorm = sqrt((b->xMin - b->xMax) * (b->xMin - b->xMax) + (b->yMin - b->yMax) * (b->yMin - b->yMax));
+  double* m = (double*)calloc((size_t)nf * nf, sizeof(double));
+  for (int p = 0; p < b->np; p++) {
+    bnd_functions(b, b->bx[p], b->by[p], b->f);
+    for (int i = 0; i < nf; i++)
+      for (int j = 0; j < nf; j++) m[i * nf + j] += b->f[i] * b->f[j];
+  }
+  int rc = bnd_gauss_jordan_inverse(m, nf);
+  for (int i = 0; i < nf; i++)
+    for (int j = 0; j < b->np; j++) {
+      double acc = 0.0;
+      bnd_functions(b, b->bx[j], b->by[j], b->f);
+      for (int k = 0; k < nf; k++) acc += m[i * nf + k] * b->f[k];
+      b->lsq[(size_t)i * b->np + j] = acc;
+    }
+  free(m);
+  return rc;
+}
+/* ShapedBoundary2D(nPoints, nModes, shape, xDim, yDim) (ShapedBoundary2D.cc:9-127) */
+orc_boundary2d* orc_boundary2d_create_shaped(int np_in, int nm, int shape, double xDim, double yDim) {
+  const double PI = 3.14159265358979323846;
+  int np = 4 * (np_in / 4);
+  orc_boundary2d* b = orc_boundary2d_create(np, nm);
+  if (!b) return NULL;
+  b->shape = shape;
+  if (shape == 1) {
+    double r = xDim / 2;
+    b->p0 = r;
+    for (int i = 0; i < np; i++) orc_boundary2d_set_point(b, i, r * sin(2 * i * PI / np), r * cos(2 * i * PI / np));
+  } else if (shape == 2) {
+    /* points of equal arc length: iterate the first quadrant until it spans pi/2 */
+    double a = xDim / 2, bb = yDim / 2;
+    b->p0 = a; b->p1 = bb;
+    double ds = 2 * PI * sqrt(a * bb) / np, resid = 1.0;
+    double* th_arr = (double*)calloc(np + 1, sizeof(double));
+    while (resid > 1.0e-08 || resid < -1.0e-08) {
+      double denom = bb;
+      double dth = ds / denom;
+      double th = dth / 2.;
+      th_arr[0] = 0.;
+      for (int i = 0; i < np / 4; i++) {
+        denom = bb * bb + (a * a - bb * bb) * sin(th) * sin(th);
+        denom = sqrt(denom);
+        dth = ds / denom;
+        th_arr[i + 1] = th_arr[i] + dth;
+        th += dth;
+      }
+      resid = th_arr[np / 4] - PI / 2.;
+      ds *= PI / (2. * th_arr[np / 4]);
+    }
+    for (int i = 0; i < np / 4; i++) th_arr[np / 2 - i] = PI - th_arr[i];
+    for (int i = 0; i < np / 2; i++) th_arr[np / 2 + i] = PI + th_arr[i];
+    for (int i = 0; i < np; i++) orc_boundary2d_set_point(b, i, a * cos(th_arr[i]), bb * sin(th_arr[i]));
+    free(th_arr);
+  } else if (shape == 3) {
+    double a = xDim, bb = yDim;
+    b->p0 = a; b->p1 = bb;
+    double dx = a / (np / 4), dy = bb / (np / 4);
+    int k = 0;
+    for (int i = 0; i < np / 4; i++) orc_boundary2d_set_point(b, k++, (-a / 2.0) + i * dx, bb / 2.0);
+    for (int i = 0; i < np / 4; i++) orc_boundary2d_set_point(b, k++, a / 2.0, bb / 2.0 - i * dy);
+    for (int i = 0; i < np / 4; i++) orc_boundary2d_set_point(b, k++, a / 2.0 - i * dx, -bb / 2.0);
+    for (int i = 0; i < np / 4; i++) orc_boundary2d_set_point(b, k++, -a / 2.0, (-bb / 2.0) + i * dy);
+  } else {
+    orc_boundary2d_destroy(b);
+    return NULL;
+  }
+  orc_boundary2d_initialize(b);
+  return b;
+}
+/* isInside: BaseBoundary2D.cc:139-147 (open box) / ShapedBoundary2D.cc:284-304 */
+int orc_boundary2d_is_inside(const orc_boundary2d* b, double x, double y) {
+  int in = 0;
+  if (b->shape == 0) in = x > b->xMin && x < b->xMax && y > b->yMin && y < b->yMax;
+  else if (b->shape == 1) in = x * x + y * y < b->p0 * b->p0;
+  else if (b->shape == 2) {
+    double xx = x / b->p0, yy = y / b->p1;
+    in = (xx * xx + yy * yy) < 1.0;
+  } else in = fabs(x / b->p0) < 0.5 && fabs(y / b->p1) < 0.5;
+  return in ? 1 : -1;
+}
+/* addBoundaryPotential (:349-458): phi on the boundary points, least-squares mode
+   coefficients, and their field subtracted at every node. */
+void orc_boundary2d_add_potential(orc_boundary2d* b, orc_grid2d* phi) {
+  int nf = 2 * b->nm + 1;
+  double* bphi = (double*)malloc(sizeof(double) * b->np);
+  double* coef = (double*)malloc(sizeof(double) * nf);
+  for (int j = 0; j < b->np; j++) bphi[j] = orc_grid2d_value(phi, b->bx[j], b->by[j]);
+  for (int i = 0; i < nf; i++) {
+    coef[i] = 0.0;
+    for (int j = 0; j < b->np; j++) coef[i] += b->lsq[(size_t)i * b->np + j] * bphi[j];
+  }
+  for (int i = 0; i < phi->nx; i++)
+    for (int j = 0; j < phi->ny; j++) {
+      double v = 0.;
+      bnd_functions(b, phi->xMin + i * phi->dx, phi->yMin + j * phi->dy, b->f);
+      for (int k = 0; k < nf; k++) v += coef[k] * b->f[k];
+      phi->arr[i * phi->ny + j] -= v;
+    }
+  free(bphi); free(coef);
+}
+void orc_boundary2d_info(const orc_boundary2d* b, int* np, double lims[4]) {
+  *np = b->np;
+  lims[0] = b->xMin; lims[1] = b->xMax; lims[2] = b->yMin; lims[3] = b->yMax;
+}
+void orc_boundary2d_points(const orc_boundary2d* b, double* x, double* y) {
+  memcpy(x, b->bx, sizeof(double) * b->np);
+  memcpy(y, b->by, sizeof(double) * b->np);
+}
+
+/* ------------------------------------------------------------------------ */
 /* SpaceChargeCalc2p5D: src/spacecharge/SpaceChargeCalc2p5D.cc (no boundary) */
 /* PoissonSolverFFT2D::setGridXY (PoissonSolverFFT2D.cc:103-111) */
 void orc_poisson2d_set_grid(orc_poisson2d* p, double xMin, double xMax, double yMin, double yMax) {
@@ -1516,6 +1746,56 @@ static int sc2p5d_analysis(orc_sc2p5d* c, orc_bunch* b) {
   orc_grid1d_bin_bunch(&c->zg, b);
   c->total = 0.;
   for (int iz = 0; iz < c->zg.nz; iz++) c->total += c->zg.arr[iz];
+  return 0;
+}
+int orc_sc2p5d_track(orc_sc2p5d* c, orc_bunch* b, double length);
+/* bunchAnalysis with a boundary (:174-234): the grid is the boundary's box, only z comes
+   from the bunch */
+static int sc2p5d_analysis_boundary(orc_sc2p5d* c, orc_bunch* b, const orc_boundary2d* bd) {
+  double e[6];
+  orc_extrema_xyz(b, e);
+  double xMin = bd->xMin, xMax = bd->xMax, yMin = bd->yMin, yMax = bd->yMax;
+  double zMin = e[4], zMax = e[5];
+  c->xy_ratio = (xMax - xMin) / (yMax - yMin);
+  if (zMin >= zMax) return -1;
+  orc_grid2d_set_x(&c->rho, xMin, xMax); orc_grid2d_set_y(&c->rho, yMin, yMax);
+  orc_grid2d_set_x(&c->phi, xMin, xMax); orc_grid2d_set_y(&c->phi, yMin, yMax);
+  orc_grid1d_set_z(&c->zg, zMin, zMax);
+  {
+    const orc_poisson2d* p = c->solver;
+    if (fabs((p->xMax - p->xMin) / (p->yMax - p->yMin) - c->xy_ratio) > 0.00000001) {
+      orc_poisson2d_set_grid(c->solver, xMin, xMax, yMin, yMax);
+      c->rebuilds++;
+    }
+  }
+  memset(c->rho.arr, 0, sizeof(double) * c->rho.nx * c->rho.ny);
+  memset(c->zg.arr, 0, sizeof(double) * c->zg.nz);
+  orc_grid2d_bin_bunch(&c->rho, b);
+  orc_grid1d_bin_bunch(&c->zg, b);
+  c->total = 0.;
+  for (int iz = 0; iz < c->zg.nz; iz++) c->total += c->zg.arr[iz];
+  return 0;
+}
+/* trackBunch (:77-120), with a boundary when bd is not NULL: its potential is added to
+   phi and particles outside it get no kick */
+int orc_sc2p5d_track_boundary(orc_sc2p5d* c, orc_bunch* b, double length, orc_boundary2d* bd) {
+  if (b->n < 2) return 0;
+  if (!bd) return orc_sc2p5d_track(c, b, length);
+  if (sc2p5d_analysis_boundary(c, b, bd) != 0) return -1;
+  double z_step = c->zg.dz;
+  orc_poisson2d_find_potential(c->solver, &c->rho, &c->phi);
+  orc_boundary2d_add_potential(bd, &c->phi);
+  double beta = b->s.beta, gamma = b->s.gamma;
+  double factor = 2 * length * b->s.classical_radius / ((beta * beta) * pow(gamma, 3));
+  factor = factor / (z_step * c->total);
+  for (int64_t i = 0; i < b->n; i++) {
+    if (orc_boundary2d_is_inside(bd, b->x[i], b->y[i]) != 1) continue;
+    double ex, ey;
+    orc_grid2d_gradient(&c->phi, b->x[i], b->y[i], &ex, &ey);
+    double Lfactor = -orc_grid1d_value(&c->zg, b->z[i]) * factor;
+    b->xp[i] += ex * Lfactor;
+    b->yp[i] += ey * Lfactor;
+  }
   return 0;
 }
 /* trackBunch without a boundary (:76-111) */

@@ -161,12 +161,27 @@ int orc_sc2p5drb_get_grid(const orc_sc2p5drb* c, int which, double* out, double*
 /* x_c, y_c, a_bunch, totalMacrosize of the last bunchAnalysis */
 void orc_sc2p5drb_get_moments(const orc_sc2p5drb* c, double out4[4]);
 
+/* Base/ShapedBoundary2D (BaseBoundary2D.cc, ShapedBoundary2D.cc); shape 1 circle,
+ * 2 ellipse, 3 rectangle. */
+typedef struct orc_boundary2d orc_boundary2d;
+orc_boundary2d* orc_boundary2d_create(int nPoints, int nModes);
+orc_boundary2d* orc_boundary2d_create_shaped(int nPoints, int nModes, int shape, double xDim,
+                                             double yDim);
+void orc_boundary2d_destroy(orc_boundary2d* b);
+void orc_boundary2d_set_point(orc_boundary2d* b, int i, double x, double y);
+int orc_boundary2d_initialize(orc_boundary2d* b);
+int orc_boundary2d_is_inside(const orc_boundary2d* b, double x, double y);
+void orc_boundary2d_add_potential(orc_boundary2d* b, orc_grid2d* phi);
+void orc_boundary2d_info(const orc_boundary2d* b, int* np, double lims[4]);
+void orc_boundary2d_points(const orc_boundary2d* b, double* x, double* y);
+
 /* SpaceChargeCalc2p5D without a boundary (SpaceChargeCalc2p5D.cc) */
 typedef struct orc_sc2p5d orc_sc2p5d;
 void orc_poisson2d_set_grid(orc_poisson2d* p, double xMin, double xMax, double yMin, double yMax);
 orc_sc2p5d* orc_sc2p5d_create(int nx, int ny, int nz, double xy_ratio);
 void orc_sc2p5d_destroy(orc_sc2p5d* c);
 int orc_sc2p5d_track(orc_sc2p5d* c, orc_bunch* b, double length);
+int orc_sc2p5d_track_boundary(orc_sc2p5d* c, orc_bunch* b, double length, orc_boundary2d* bd);
 int orc_sc2p5d_get_grid(const orc_sc2p5d* c, int which, double* out, double* lims);
 void orc_sc2p5d_state(const orc_sc2p5d* c, double out7[7]);
 

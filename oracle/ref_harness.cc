@@ -1,5 +1,5 @@
-// ref_harness.cc -- C entry points over the reference's own Grid1D/Grid2D/Grid3D
-// classes, compiled from the reference sources where they lie (see
+// ref_harness.cc -- C entry points over the reference's own Grid1D/Grid2D/Grid3D and
+// Base/ShapedBoundary2D classes, compiled from the reference sources where they lie (see
 // oracle/Makefile.ref).  TEST INFRASTRUCTURE ONLY: it writes golden vectors and
 // checks the oracle's restatement of the same functions bit for bit.
 //
@@ -10,6 +10,7 @@
 #include "Grid1D.hh"
 #include "Grid2D.hh"
 #include "Grid3D.hh"
+#include "ShapedBoundary2D.hh"
 
 #define REF_API extern "C" __attribute__((visibility("default")))
 
@@ -143,4 +144,43 @@ REF_API void ref_grid3d_grad_wrapped(int nx, int ny, int nz, const double* lims,
     for (int ix = 0; ix < nx; ix++)
       for (int iy = 0; iy < ny; iy++) g.setValue(arr[((long)iz * nx + ix) * ny + iy], ix, iy, iz);
   for (long i = 0; i < n; i++) g.calcGradient(x[i], gx[i], y[i], gy[i], z[i], gz[i]);
+}
+
+// Base/ShapedBoundary2D (BaseBoundary2D.cc, ShapedBoundary2D.cc): shape 0 is a
+// BaseBoundary2D whose nPoints points come from px, py; 1/2/3 is a ShapedBoundary2D
+// Circle/Ellipse/Rectangle of dimensions xDim, yDim.  Writes the boundary points
+// (bx, by: nPoints after the shape's rounding to a multiple of 4), the limits
+// lims[4], isInside for n test points, and phi after addBoundaryPotential(rho, phi) on a
+// [nx][ny] grid spanning glims.  Returns the number of points.
+REF_API int ref_boundary2d(int shape, int nPoints, int nModes, double xDim, double yDim,
+                           const double* px, const double* py, double* bx, double* by,
+                           double* lims, long n, const double* tx, const double* ty,
+                           int* inside, int nx, int ny, const double* glims, double* phi) {
+  static const char* names[4] = {"", "Circle", "Ellipse", "Rectangle"};
+  BaseBoundary2D* b;
+  if (shape == 0) {
+    b = new BaseBoundary2D(nPoints, nModes);
+    for (int i = 0; i < nPoints; i++) b->setBoundaryPoint(i, px[i], py[i]);
+    b->initializeBPs();
+  } else {
+    b = new ShapedBoundary2D(nPoints, nModes, std::string(names[shape]), xDim, yDim);
+  }
+  const int np = b->getNumberOfPoints();
+  for (int i = 0; i < np; i++) {
+    bx[i] = b->getBoundaryPointX(i);
+    by[i] = b->getBoundaryPointY(i);
+  }
+  lims[0] = b->getMinX(); lims[1] = b->getMaxX(); lims[2] = b->getMinY(); lims[3] = b->getMaxY();
+  for (long i = 0; i < n; i++) inside[i] = b->isInside(tx[i], ty[i]);
+  if (phi) {
+    Grid2D rho(nx, ny, glims[0], glims[1], glims[2], glims[3]);
+    Grid2D ph(nx, ny, glims[0], glims[1], glims[2], glims[3]);
+    for (int ix = 0; ix < nx; ix++)
+      for (int iy = 0; iy < ny; iy++) ph.setValue(phi[ix * ny + iy], ix, iy);
+    b->addBoundaryPotential(&rho, &ph);
+    for (int ix = 0; ix < nx; ix++)
+      for (int iy = 0; iy < ny; iy++) phi[ix * ny + iy] = ph.getValueOnGrid(ix, iy);
+  }
+  delete b;
+  return np;
 }

@@ -73,6 +73,16 @@ SIGNATURES = {
     "ob_sc2p5d_track": [_vp, _vp, _d],
     "ob_sc2p5d_get_grid": [_vp, _i, _D, _D],
     "ob_sc2p5d_get_state": [_vp, _D],
+    "ob_sc2p5d_track_boundary": [_vp, _vp, _d, _vp],
+    "ob_boundary2d_create": [_i, _i, C.POINTER(_vp)],
+    "ob_boundary2d_create_shaped": [_i, _i, C.c_char_p, _d, _d, C.POINTER(_vp)],
+    "ob_boundary2d_destroy": [_vp],
+    "ob_boundary2d_set_point": [_vp, _i, _d, _d],
+    "ob_boundary2d_get_point": [_vp, _i, _D, _D],
+    "ob_boundary2d_initialize": [_vp],
+    "ob_boundary2d_info": [_vp, _D, C.POINTER(_i)],
+    "ob_boundary2d_is_inside": [_vp, _d, _d, C.POINTER(_i)],
+    "ob_boundary2d_add_potential": [_vp, _D, _i, _i, _D],
     "ob_sc2p5drb_create": [_i, _i, _i, _d, C.POINTER(_vp)],
     "ob_sc2p5drb_destroy": [_vp],
     "ob_sc2p5drb_track": [_vp, _vp, _d, _d],

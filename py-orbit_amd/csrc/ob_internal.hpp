@@ -120,7 +120,26 @@ struct SC2DState {
   double sdx, sdy;                       // solver grid steps (PoissonSolverFFT2D dx_, dy_)
   int32_t long_kick;                     // 1: 2p5Drb longitudinal kick, 0: transverse only
   int32_t rebuilds;                      // Green function rebuilds so far
+  // SpaceChargeCalc2p5D with a boundary: only particles inside it are kicked
+  // (SpaceChargeCalc2p5D.cc:109).  Type 0 none, 1 point-set box, 2 circle, 3 ellipse,
+  // 4 rectangle; bnd = box xMin xMax yMin yMax, then the shape's two sizes.
+  int32_t bnd_type, pad2;
+  double bnd[6];
 };
+
+// Boundary2D inside test (BaseBoundary2D.cc:139-147, ShapedBoundary2D.cc:284-304).
+__host__ __device__ inline bool boundary_inside(int type, const double* b, double x, double y) {
+  switch (type) {
+    case 1: return x > b[0] && x < b[1] && y > b[2] && y < b[3];
+    case 2: return x * x + y * y < b[4] * b[4];
+    case 3: {
+      const double xx = x / b[4], yy = y / b[5];
+      return (xx * xx + yy * yy) < 1.0;
+    }
+    case 4: return fabs(x / b[4]) < 0.5 && fabs(y / b[5]) < 0.5;
+    default: return true;
+  }
+}
 
 struct SC3DState {
   double ext[6];
@@ -206,6 +225,16 @@ struct DenoiseJob {
 void launch_sc2d_batch(const BatchBunch* bb, const BatchItem* items, const BatchShape& sh,
                        const double* partials, double length, double pipe_radius, hipStream_t s,
                        const std::vector<DenoiseJob>& jobs = {});
+}  // namespace ob
+struct ob_boundary2d;
+namespace ob {
+// Boundary2D::addBoundaryPotential (BaseBoundary2D.cc:349-458) on a device phi grid whose
+// geometry is in st (skipped when st->skip); the boundary must be initialized.
+void boundary_add_potential(ob_boundary2d* b, double* phi, int nx, int ny, const SC2DState* st,
+                            hipStream_t s);
+// Type (1 point set, 2 circle, 3 ellipse, 4 rectangle) and the six numbers of
+// SC2DState::bnd for an initialized boundary.
+int boundary_describe(ob_boundary2d* b, double* bnd6);
 }  // namespace ob
 struct ob_sc2p5drb;
 namespace ob {

@@ -55,6 +55,8 @@ struct SC2DParams {
   int n_long_avg;
   int plain;                  // 1: SpaceChargeCalc2p5D (ratio rule, transverse kick only)
   int par;                    // SpaceChargeCalc2p5D: state entry this step reads
+  int bnd_type;               // SpaceChargeCalc2p5D boundary (SC2DState::bnd_type), 0: none
+  double bnd[6];              // its box and sizes (SC2DState::bnd)
 };
 
 // ---- 2: grid of the step --------------------------------------------------------
@@ -106,7 +108,19 @@ __device__ __forceinline__ void sc2d_grid_body(const SC2DParams& P,
       ratio = st->ratio[P.par];
       for (int k = 0; k < 4; k++) g.sol[k] = st->sol[P.par][k];
     }
-    if (!g.skip) {
+    if (P.plain && P.bnd_type) {
+      // with a boundary the grid is its box; only z comes from the bunch (:174-195)
+      xMin = P.bnd[0];
+      xMax = P.bnd[1];
+      yMin = P.bnd[2];
+      yMax = P.bnd[3];
+      ratio = (xMax - xMin) / (yMax - yMin);
+      g.skip = zMin >= zMax ? 1 : 0;
+      if (!g.skip && fabs((g.sol[1] - g.sol[0]) / (g.sol[3] - g.sol[2]) - ratio) > 0.00000001) {
+        g.sol[0] = xMin; g.sol[1] = xMax; g.sol[2] = yMin; g.sol[3] = yMax;
+        g.rebuild = 1;
+      }
+    } else if (!g.skip) {
       const double xy_ratio_beam = (xMax - xMin) / (yMax - yMin);
       // 2p5Drb: always (:191-203); 2p5D: within 25% of the ratio (SpaceChargeCalc2p5D.cc:152-167)
       if (!P.plain || fabs(xy_ratio_beam - ratio) < 0.25 * ratio) {
@@ -183,6 +197,8 @@ __device__ __forceinline__ void sc2d_write_solver(const SC2DParams& P, SC2DState
     st->rebuilds += 1;
   }
   gate->skip = g.rebuild ? 0 : 1;
+  st->bnd_type = P.bnd_type;
+  for (int k = 0; k < 6; k++) st->bnd[k] = P.bnd[k];
 }
 
 __device__ __forceinline__ Grid2Step sc2d_load_grid(const SC2DState* st) {
@@ -850,6 +866,8 @@ __device__ __forceinline__ SC2DParams batch_params(const BatchBunch& B, double l
   P.n_long_avg = B.n_long_avg;
   P.plain = 0;
   P.par = 0;
+  P.bnd_type = 0;
+  for (int k = 0; k < 6; k++) P.bnd[k] = 0.0;
   return P;
 }
 
@@ -1348,9 +1366,10 @@ void sc2d_destroy(ob_sc2p5drb* c) {
   OB_REQUIRE(err.empty(), err);
 }
 
-// One trackBunch: SpaceChargeCalc2p5Drb.cc:110-168, or SpaceChargeCalc2p5D.cc:76-111
-// without a boundary when c->plain.
-void sc2d_track(ob_sc2p5drb* c, ob_bunch* b, double length, double pipe_radius) {
+// One trackBunch: SpaceChargeCalc2p5Drb.cc:110-168, or SpaceChargeCalc2p5D.cc:77-120
+// (with the boundary bnd, if not null) when c->plain.
+void sc2d_track(ob_sc2p5drb* c, ob_bunch* b, double length, double pipe_radius,
+                ob_boundary2d* bnd = nullptr) {
   check_statuses(false);
   const int64_t n = bunch_size(b);
   if (n < 2) return;  // :112-113 (2p5D :79-80)
@@ -1368,6 +1387,8 @@ void sc2d_track(ob_sc2p5drb* c, ob_bunch* b, double length, double pipe_radius) 
   P.n_long_avg = c->n_long_avg;
   P.plain = c->plain;
   P.par = c->par;
+  for (int k = 0; k < 6; k++) P.bnd[k] = 0.0;
+  P.bnd_type = bnd ? boundary_describe(bnd, P.bnd) : 0;
   // bin (grid + tile counts + z line density), scatter (scan + counting sort), deposit
   const int hb = (int)std::max<int64_t>(
       1, std::min<int64_t>({(n + 2047) / 2048, (int64_t)4 * device_cu_count(),
@@ -1414,6 +1435,7 @@ void sc2d_track(ob_sc2p5drb* c, ob_bunch* b, double length, double pipe_radius) 
                        c->zd, tg, hb, moments);
   }
   launch_fft_pipeline(c, c->st);
+  if (bnd) boundary_add_potential(bnd, c->phi, c->nx, c->ny, c->st, s);  // :90-94
   OB_HIP(hipGetLastError());
   watch_status(&c->st->status,
                c->plain ? "SpaceChargeCalc2p5D::bunchAnalysis: the bunch min and max sizes are "
@@ -1494,6 +1516,14 @@ OB_API int ob_sc2p5d_track(ob_sc2p5d* c, ob_bunch* b, double length) {
   OB_TRY OB_REQUIRE(c != nullptr && b != nullptr, "null handle");
   OB_REQUIRE(c->plain, "not a SpaceChargeCalc2p5D handle");
   ob::sc2d_track(c, b, length, 0.0);
+  OB_CATCH
+}
+
+OB_API int ob_sc2p5d_track_boundary(ob_sc2p5d* c, ob_bunch* b, double length,
+                                   ob_boundary2d* bnd) {
+  OB_TRY OB_REQUIRE(c != nullptr && b != nullptr, "null handle");
+  OB_REQUIRE(c->plain, "not a SpaceChargeCalc2p5D handle");
+  ob::sc2d_track(c, b, length, 0.0, bnd);
   OB_CATCH
 }
 

@@ -231,6 +231,7 @@ __device__ __forceinline__ void sc2p5drb_kick(const Op& op, double x, double& xp
                                               double& yp, double z, double& dE) {
   const SC2DState* st = op_ptr<const SC2DState*>(op.p[0]);
   if (st->skip) return;
+  if (st->bnd_type && !boundary_inside(st->bnd_type, st->bnd, x, y)) return;  // 2p5D :109
   double ex, ey;
   grid2d_gradient(op_ptr<const double*>(op.p[1]), op.i0, op.i1, st->xMin, st->dx, st->yMin,
                   st->dy, x, y, ex, ey);

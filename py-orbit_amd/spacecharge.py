@@ -292,6 +292,92 @@ class SpaceChargeCalc2p5Drb:
         return tuple(float(v) for v in out)
 
 
+class Boundary2D:
+    """Boundary2D(nPoints, nModes) or Boundary2D(nPoints, nModes, shape, xDim[, yDim])
+    with shape "Circle", "Ellipse" or "Rectangle" (wrap_boundary2d.cc:38-75;
+    BaseBoundary2D.cc, ShapedBoundary2D.cc)."""
+
+    _SHAPES = ("no shape", "Circle", "Ellipse", "Rectangle")
+
+    def __init__(self, nPoints, nModes, shape=None, xDim=None, yDim=None):
+        h = C.c_void_p()
+        if shape is None:
+            N.call("ob_boundary2d_create", int(nPoints), int(nModes), C.byref(h))
+        else:
+            if xDim is None:
+                raise TypeError("Boundary2D(nPoints, nModes, shape, xDim[, yDim])")
+            yDim = xDim if yDim is None else yDim
+            N.call("ob_boundary2d_create_shaped", int(nPoints), int(nModes),
+                   str(shape).encode(), float(xDim), float(yDim), C.byref(h))
+        self._h = h
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h and N._lib is not None:
+            N._lib.ob_boundary2d_destroy(h)
+            self._h = None
+
+    def _info(self):
+        lims = np.zeros(4)
+        info = (C.c_int * 4)()
+        N.call("ob_boundary2d_info", self._h, N.dptr(lims), info)
+        return lims, list(info)
+
+    def getMinX(self):
+        return float(self._info()[0][0])
+
+    def getMaxX(self):
+        return float(self._info()[0][1])
+
+    def getMinY(self):
+        return float(self._info()[0][2])
+
+    def getMaxY(self):
+        return float(self._info()[0][3])
+
+    def getNumberOfPoints(self):
+        return self._info()[1][0]
+
+    def getNumberOfModes(self):
+        return self._info()[1][1]
+
+    def isInitialized(self):
+        return self._info()[1][2]
+
+    def getShapeName(self):
+        return self._SHAPES[self._info()[1][3]]
+
+    def initialize(self):
+        N.call("ob_boundary2d_initialize", self._h)
+
+    def getBoundaryPoint(self, i):
+        x, y = C.c_double(), C.c_double()
+        N.call("ob_boundary2d_get_point", self._h, int(i), C.byref(x), C.byref(y))
+        return (x.value, y.value)
+
+    def setBoundaryPoint(self, i, x, y):
+        N.call("ob_boundary2d_set_point", self._h, int(i), float(x), float(y))
+
+    def isInside(self, x, y):
+        v = C.c_int()
+        N.call("ob_boundary2d_is_inside", self._h, float(x), float(y), C.byref(v))
+        return v.value
+
+    def addBoundaryPotential(self, rhoGrid, phiGrid):
+        """Subtract the boundary's field from phiGrid in place (BaseBoundary2D.cc:349-458);
+        rhoGrid must span the same grid."""
+        if (rhoGrid.getSizeX(), rhoGrid.getSizeY(), rhoGrid.getMinX(), rhoGrid.getMinY(),
+                rhoGrid.getStepX(), rhoGrid.getStepY()) != (
+                phiGrid.getSizeX(), phiGrid.getSizeY(), phiGrid.getMinX(), phiGrid.getMinY(),
+                phiGrid.getStepX(), phiGrid.getStepY()):
+            raise N.OrbitError("Boundary2D addBoundaryPotential: the rho and phi grids differ")
+        a = np.ascontiguousarray(phiGrid._a, dtype=np.float64)
+        lims = np.array(phiGrid._l, dtype=np.float64)
+        N.call("ob_boundary2d_add_potential", self._h, N.dptr(a), a.shape[0], a.shape[1],
+               N.dptr(lims))
+        phiGrid._a = a
+
+
 class SpaceChargeCalc2p5D:
     """SpaceChargeCalc2p5D(nx, ny, nz[, xy_ratio]) (SpaceChargeCalc2p5D.cc:25-43,
     wrap_spacechargecalc2p5d.cc:41).  trackBunch(bunch, length) without a boundary."""
@@ -310,12 +396,15 @@ class SpaceChargeCalc2p5D:
             self._h = None
 
     def trackBunch(self, bunch, length, boundary=None):
-        """trackBunch(bunch, length) (SpaceChargeCalc2p5D.cc:77-120).  The boundary form
-        (wrap_spacechargecalc2p5d.cc:139) is not provided."""
-        if boundary is not None:
-            raise NotImplementedError("SpaceChargeCalc2p5D.trackBunch with a boundary")
+        """trackBunch(bunch, length[, boundary]) (SpaceChargeCalc2p5D.cc:77-120,
+        wrap_spacechargecalc2p5d.cc:128-150)."""
         bunch._push_attrs()
-        N.call("ob_sc2p5d_track", self._h, bunch._h, float(length))
+        if boundary is None:
+            N.call("ob_sc2p5d_track", self._h, bunch._h, float(length))
+        else:
+            if not isinstance(boundary, Boundary2D):
+                raise TypeError("SpaceChargeCalc2p5D.trackBunch: boundary must be a Boundary2D")
+            N.call("ob_sc2p5d_track_boundary", self._h, bunch._h, float(length), boundary._h)
 
     def _grid(self, which):
         nx, ny, nz = self._n

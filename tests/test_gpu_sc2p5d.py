@@ -103,7 +103,7 @@ def test_sc2p5d_macrosize_attribute():
     assert calc.getState()[2] == pytest.approx(m.sum(), rel=1e-12)
 
 
-def test_sc2p5d_zero_extent_raises_and_boundary_refused():
+def test_sc2p5d_zero_extent_raises_and_bad_boundary_refused():
     import spacecharge
     from _obnative import OrbitError
     c = np.zeros((6, 10))
@@ -113,7 +113,7 @@ def test_sc2p5d_zero_extent_raises_and_boundary_refused():
     calc.trackBunch(b, 0.5)
     with pytest.raises(OrbitError, match="zero extent"):
         b.getCoordinates()
-    with pytest.raises(NotImplementedError):
+    with pytest.raises(TypeError):
         calc.trackBunch(b, 0.5, object())
 
 
@@ -153,3 +153,105 @@ def test_fodo_2p5d_tracking(monkeypatch):
     replay_oracle(calls, ob, O.SC2p5D(*grid))
     for k in range(6):
         assert rel_l2(out[k], ob.coords[k]) < 1e-9, (k, rel_l2(out[k], ob.coords[k]))
+
+
+BOUNDS = [("Circle", 128, 10, 0.02, 0.02), ("Ellipse", 128, 12, 0.03, 0.018),
+          ("Rectangle", 120, 8, 0.04, 0.025)]
+
+
+def _both_boundaries(shape, npts, nm, xd, yd):
+    import spacecharge
+    return (spacecharge.Boundary2D(npts, nm, shape, xd, yd), O.Boundary2D(npts, nm, shape, xd, yd))
+
+
+@pytest.mark.parametrize("shape,npts,nm,xd,yd", BOUNDS, ids=[b[0] for b in BOUNDS])
+def test_boundary2d_matches_oracle(shape, npts, nm, xd, yd):
+    """Points, limits and isInside exactly; addBoundaryPotential (on the device) to 1e-14
+    of the potential's scale."""
+    import spacecharge
+    b, ob = _both_boundaries(shape, npts, nm, xd, yd)
+    bx, by, lims = ob.info()
+    assert b.getNumberOfPoints() == len(bx) and b.getNumberOfModes() == nm
+    assert b.getShapeName() == shape and b.isInitialized() == 1
+    assert [b.getMinX(), b.getMaxX(), b.getMinY(), b.getMaxY()] == list(lims)
+    assert all(b.getBoundaryPoint(i) == (bx[i], by[i]) for i in range(len(bx)))
+    r = np.random.default_rng(3)
+    for x, y in zip(r.uniform(-0.03, 0.03, 300), r.uniform(-0.03, 0.03, 300)):
+        assert b.isInside(x, y) == ob.isInside(x, y)
+    phi = r.normal(size=(40, 33))
+    g = spacecharge.Grid2D(phi, *lims)
+    rho = spacecharge.Grid2D(np.zeros_like(phi), *lims)
+    b.addBoundaryPotential(rho, g)
+    want = ob.addBoundaryPotential(phi, lims)
+    assert np.abs(g.array() - want).max() <= 1e-14 * np.abs(want).max()
+
+
+def test_boundary2d_point_set_and_errors():
+    import spacecharge
+    from _obnative import OrbitError
+    with pytest.raises(OrbitError):
+        spacecharge.Boundary2D(10, 5)  # 2*nModes+1 >= nPoints
+    with pytest.raises(OrbitError):
+        spacecharge.Boundary2D(64, 4, "Circle", 0.02, 0.03)
+    with pytest.raises(OrbitError):
+        spacecharge.Boundary2D(64, 4, "Hexagon", 0.02)
+    shaped = spacecharge.Boundary2D(64, 4, "Circle", 0.02)
+    with pytest.raises(OrbitError, match="shape has been defined"):
+        shaped.setBoundaryPoint(0, 0.0, 0.0)
+    b = spacecharge.Boundary2D(48, 5)
+    ob = O.Boundary2D(48, 5)
+    t = np.linspace(0, 2 * np.pi, 48, endpoint=False)
+    for i in range(48):
+        x, y = 0.012 * np.cos(t[i]) + 1e-3, 0.008 * np.sin(t[i]) * (1 + 0.1 * np.cos(t[i]))
+        b.setBoundaryPoint(i, x, y)
+        ob.setBoundaryPoint(i, x, y)
+    assert b.isInitialized() == 0
+    g = spacecharge.Grid2D(np.ones((16, 16)), -0.02, 0.02, -0.02, 0.02)
+    with pytest.raises(OrbitError, match="not initialized"):
+        b.addBoundaryPotential(g, g)
+    b.initialize()
+    ob.initialize()
+    _, _, lims = ob.info()
+    assert [b.getMinX(), b.getMaxX(), b.getMinY(), b.getMaxY()] == list(lims)
+    phi = np.random.default_rng(4).normal(size=(24, 20))
+    gg = spacecharge.Grid2D(phi, *lims)
+    b.addBoundaryPotential(gg, gg)
+    want = ob.addBoundaryPotential(phi, lims)
+    assert np.abs(gg.array() - want).max() <= 1e-14 * np.abs(want).max()
+
+
+@pytest.mark.parametrize("shape,npts,nm,xd,yd", BOUNDS, ids=[b[0] for b in BOUNDS])
+def test_sc2p5d_with_boundary(shape, npts, nm, xd, yd):
+    """trackBunch with a boundary: the grid is the boundary's box, phi carries the
+    boundary's correction, particles outside the boundary keep their angles exactly."""
+    import spacecharge
+    c = _coords(30000, 4e-3, 3e-3, seed=40)
+    c[0, :50] = 0.018  # some particles outside every boundary
+    c[2, :50] = 0.013
+    b, ob = _bunches(c)
+    bnd, obnd = _both_boundaries(shape, npts, nm, xd, yd)
+    calc = spacecharge.SpaceChargeCalc2p5D(64, 64, 16)
+    ocalc = O.SC2p5D(64, 64, 16)
+    for step in range(2):
+        c0 = b.getCoordinates()
+        calc.trackBunch(b, 0.5, bnd)
+        ocalc.trackBunch(ob, 0.5, obnd)
+        out = b.getCoordinates()
+        rho, orho = calc.getRhoGrid(), ocalc.grid(0)
+        assert [rho.getMinX(), rho.getMaxX(), rho.getMinY(), rho.getMaxY()] == list(orho[1])
+        assert rel_l2(rho.array(), orho[0]) < 1e-12
+        assert rel_l2(calc.getPhiGrid().array(), ocalc.grid(1)[0]) < 1e-12
+        st, ost = calc.getState(), ocalc.state()
+        assert st[0] == ost[0] and st[1] == ost[1] and st[3] == ost[3]
+        for k in (1, 3):
+            assert rel_l2(out[k] - c0[k], ob.coords[k] - c0[k]) < 1e-10, (step, k)
+        outside = np.array([obnd.isInside(x, y) != 1 for x, y in zip(c0[0], c0[2])])
+        assert outside.sum() >= 50
+        for k in (1, 3):
+            assert np.array_equal(out[k][outside], c0[k][outside])
+    # the same calculator without the boundary afterwards: ratio rule from the boundary ratio
+    c2 = _coords(20000, 3e-3, 3e-3, seed=41)
+    b2, ob2 = _bunches(c2)
+    calc.trackBunch(b2, 0.5)
+    ocalc.trackBunch(ob2, 0.5)
+    _compare(calc, ocalc, b2, ob2, c2)

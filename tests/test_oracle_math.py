@@ -415,6 +415,41 @@ def test_sc2p5d_transverse_kick_uniform_disk():
     assert calc.state()[2] == pytest.approx(1e13, rel=1e-12)
 
 
+def test_boundary_potential_is_constant_on_a_conducting_pipe():
+    """An off-centre line charge inside a circular pipe: after addBoundaryPotential the
+    potential on the pipe is (nearly) constant, i.e. the pipe is an equipotential."""
+    bnd = O.Boundary2D(128, 16, "Circle", 0.04)
+    bx, by, lims = bnd.info()
+    nx = ny = 129
+    xs = np.linspace(lims[0], lims[1], nx)
+    ys = np.linspace(lims[2], lims[3], ny)
+    X, Y = np.meshgrid(xs, ys, indexing="ij")
+    phi = -0.5 * np.log((X - 0.006) ** 2 + (Y + 0.004) ** 2 + 1e-12)  # free space, off centre
+    out = bnd.addBoundaryPotential(phi, lims)
+
+    def on_pipe(a):
+        return np.array([O.Grid2DValue(a, lims, x, y) for x, y in zip(bx, by)])
+    before, after = on_pipe(phi), on_pipe(out)
+    assert np.ptp(after) < 1e-2 * np.ptp(before)
+
+
+def test_sc2p5d_centred_beam_in_round_pipe_keeps_its_field():
+    """A centred round beam in a round pipe: the pipe only adds a constant potential, so
+    the kick is the free-space uniform-disk kick 2 r0 lambda L x / (beta^2 gamma^3 R^2)."""
+    n, R = 200000, 5e-3
+    x, y = _uniform_disk(n, R, 9)
+    c = np.zeros((6, n))
+    c[0], c[2] = x, y
+    c[4] = np.random.default_rng(10).uniform(-5, 5, n)
+    bnd = O.Boundary2D(128, 12, "Circle", 0.02)
+    a = O.Bunch(c, macrosize=1e9, kin_energy=1.0)
+    O.SC2p5D(64, 64, 8).trackBunch(a, 0.5, bnd)
+    s = a.sync
+    K = 2 * 1.534698e-18 * (1e9 * n / 10.0) * 0.5 / (s["beta"] ** 2 * s["gamma"] ** 3 * R ** 2)
+    inside = np.hypot(x, y) < 0.7 * R
+    assert np.median((a.coords[1] - c[1])[inside] / (K * x[inside])) == pytest.approx(1.0, abs=0.04)
+
+
 def _beam(n, ax, ay, seed):
     r = np.random.default_rng(seed)
     c = np.zeros((6, n))

@@ -92,3 +92,42 @@ def test_grid3d_long_wrapping(nz):
     for u, v in zip(ref.grid3d_grad_wrapped(nx, ny, nz, lims, arr, x, y, zz),
                     orc.grid3d_grad_wrapped(nx, ny, nz, lims, arr, x, y, zz)):
         assert np.array_equal(u, v)
+
+
+BOUNDARIES = [("Circle", 128, 10, 0.02, 0.02), ("Ellipse", 130, 12, 0.03, 0.018),
+              ("Rectangle", 100, 8, 0.04, 0.025), (None, 64, 6, 0.0, 0.0)]
+
+
+@pytest.mark.parametrize("shape,npts,nmodes,xd,yd", BOUNDARIES,
+                         ids=[str(b[0]) for b in BOUNDARIES])
+def test_boundary2d(shape, npts, nmodes, xd, yd):
+    """Base/ShapedBoundary2D: boundary points, limits, isInside and addBoundaryPotential
+    bit for bit against the reference build, for the three shapes and a user-defined
+    polygon."""
+    r = np.random.default_rng(7)
+    pts = None
+    if shape is None:  # a lopsided polygon of user points
+        t = np.linspace(0, 2 * np.pi, npts, endpoint=False)
+        rad = 0.015 * (1 + 0.2 * np.cos(3 * t))
+        pts = (rad * np.cos(t) + 1e-3, rad * np.sin(t) - 2e-3)
+    tx = r.uniform(-0.03, 0.03, 3000)
+    ty = r.uniform(-0.03, 0.03, 3000)
+    nx, ny = 40, 33
+    rbx, rby, rlims, rin, _ = O.ref_boundary2d(shape, npts, nmodes, xd, yd, points=pts,
+                                                tests=(tx, ty))
+    glims = np.array([rlims[0], rlims[1], rlims[2], rlims[3]])
+    phi = r.normal(size=(nx, ny))
+    _, _, _, _, rphi = O.ref_boundary2d(shape, npts, nmodes, xd, yd, points=pts,
+                                         tests=(tx, ty), phi=phi, glims=glims)
+    if shape is None:
+        b = O.Boundary2D(npts, nmodes)
+        for i in range(npts):
+            b.setBoundaryPoint(i, pts[0][i], pts[1][i])
+        b.initialize()
+    else:
+        b = O.Boundary2D(npts, nmodes, shape, xd, yd)
+    bx, by, lims = b.info()
+    assert np.array_equal(bx, rbx) and np.array_equal(by, rby)
+    assert np.array_equal(lims, rlims)
+    assert [b.isInside(x, y) for x, y in zip(tx, ty)] == list(rin)
+    assert np.array_equal(b.addBoundaryPotential(phi, glims), rphi)
