@@ -25,13 +25,11 @@ hipStream_t g_stream = nullptr;
 hipStream_t g_aux_stream = nullptr;
 int g_cus = 0;
 
-struct Watched {
-  int* dev;
+struct Box {
   int* host;  // pinned
-  hipEvent_t ev;
   std::string what;
 };
-std::vector<Watched> g_watch;
+std::vector<Box> g_boxes;
 }  // namespace
 
 hipStream_t stream() { return g_stream; }
@@ -99,42 +97,36 @@ void prof_mark(const char* name, bool begin, hipStream_t s) {
   }
 }
 
-void watch_status(int* dev_status, const char* what) {
-  Watched w;
-  w.dev = dev_status;
-  OB_HIP(hipHostMalloc((void**)&w.host, sizeof(int), hipHostMallocDefault));
-  *w.host = 0;
-  OB_HIP(hipEventCreateWithFlags(&w.ev, hipEventDisableTiming));
-  OB_HIP(hipMemcpyAsync(w.host, dev_status, sizeof(int), hipMemcpyDeviceToHost, g_stream));
-  OB_HIP(hipEventRecord(w.ev, g_stream));
-  w.what = what;
-  g_watch.push_back(w);
+StatusBox status_box_create(const char* what) {
+  ensure_init();
+  StatusBox b;
+  OB_HIP(hipHostMalloc((void**)&b.host, sizeof(int), hipHostMallocDefault));
+  *b.host = 0;
+  OB_HIP(hipHostGetDevicePointer((void**)&b.dev, b.host, 0));
+  g_boxes.push_back(Box{b.host, what});
+  return b;
+}
+
+void status_box_destroy(StatusBox b) {
+  for (size_t i = 0; i < g_boxes.size(); i++)
+    if (g_boxes[i].host == b.host) {
+      g_boxes.erase(g_boxes.begin() + i);
+      break;
+    }
+  (void)hipHostFree(b.host);
 }
 
 void check_statuses(bool wait) {
+  if (g_boxes.empty()) return;
+  if (wait) OB_HIP(hipStreamSynchronize(g_stream));
   std::string err;
-  size_t keep = 0;
-  for (size_t i = 0; i < g_watch.size(); i++) {
-    Watched& w = g_watch[i];
-    bool done;
-    if (wait) {
-      OB_HIP(hipEventSynchronize(w.ev));
-      done = true;
-    } else {
-      hipError_t q = hipEventQuery(w.ev);
-      OB_REQUIRE(q == hipSuccess || q == hipErrorNotReady,
-                 std::string("status query failed: ") + hipGetErrorString(q));
-      done = (q == hipSuccess);
-    }
-    if (done) {
-      if (*w.host != 0 && err.empty()) err = w.what;
-      (void)hipEventDestroy(w.ev);
-      (void)hipHostFree(w.host);
-    } else {
-      g_watch[keep++] = w;
+  for (Box& b : g_boxes) {
+    volatile int* v = b.host;
+    if (*v != 0) {
+      *v = 0;
+      if (err.empty()) err = b.what;
     }
   }
-  g_watch.resize(keep);
   OB_REQUIRE(err.empty(), err);
 }
 

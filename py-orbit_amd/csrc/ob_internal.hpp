@@ -52,9 +52,16 @@ struct ProfScope {
   }
 };
 
-// Pending device-side status words: a kernel writes a non-zero code when it finds a
-// condition the reference treats as fatal.  The host reads them at sync points.
-void watch_status(int* dev_status, const char* what);
+// Status words: a pinned host int that a kernel sets to 1 (through its device address)
+// when it finds a condition the reference treats as fatal.  The host reads and clears the
+// words at sync points (check_statuses(true) waits for the stream first); nothing is
+// copied or allocated per step.
+struct StatusBox {
+  int* host;
+  int* dev;
+};
+StatusBox status_box_create(const char* what);
+void status_box_destroy(StatusBox b);
 void check_statuses(bool wait);
 
 // ---------------------------------------------------------------------------

@@ -495,7 +495,8 @@ __device__ __forceinline__ void sc2d_tdeposit_body(const SC2DParams& P, const Gr
 __global__ __launch_bounds__(kDepositThreads) void sc2d_bin(SC2DParams P, DeviceBunchView v,
                                                             const double* __restrict__ partials,
                                                             SC2DState* st, SC2DState* gate,
-                                                            Tile2D tg, double* __restrict__ rho) {
+                                                            Tile2D tg, double* __restrict__ rho,
+                                                            int* status) {
   extern __shared__ double dyn[];
   Grid2Step g;
   sc2d_grid_body(P, partials, 0, P.nblk, st, g);
@@ -503,6 +504,7 @@ __global__ __launch_bounds__(kDepositThreads) void sc2d_bin(SC2DParams P, Device
     if (threadIdx.x == 0) {
       sc2d_write_state(st, g);
       if (P.plain) sc2d_write_solver(P, st, gate, g);
+      if (g.skip) *status = 1;  // zero extent: the reference aborts (:176-189)
     }
     for (int t = threadIdx.x; t < tg.ntiles; t += blockDim.x) tg.cursor[t] = 0u;
   }
@@ -1041,6 +1043,7 @@ struct ob_sc2p5drb {
   int par = 0;                      // state entry the next step reads
   ob::SC2DState* gate = nullptr;
   double* gbuf = nullptr;           // M0 x M1 Green function values
+  ob::StatusBox status{nullptr, nullptr};  // set when a step finds a zero-extent bunch
 };
 
 // SpaceChargeCalc2p5D (SpaceChargeCalc2p5D.cc): the same device object in plain mode.
@@ -1343,6 +1346,11 @@ Calc* sc2d_create(int nx, int ny, int nz, double xy_ratio, bool plain, const std
   }
   c->tw = make_twiddles(c->nmax);
   define_green(c);
+  c->status = status_box_create(
+      plain ? "SpaceChargeCalc2p5D::bunchAnalysis: the bunch min and max sizes are wrong "
+              "(zero extent); cannot calculate space charge"
+            : "SpaceChargeCalc2p5Drb::bunchAnalysis: the bunch min and max sizes are wrong "
+              "(zero extent); cannot calculate space charge");
   return c;
 }
 
@@ -1356,6 +1364,7 @@ void sc2d_destroy(ob_sc2p5drb* c) {
     err = e.msg;
   }
   (void)hipStreamSynchronize(stream());
+  if (c->status.host) status_box_destroy(c->status);
   for (void* p : {(void*)c->rho, (void*)c->phi, (void*)c->zg, (void*)c->zd, (void*)c->spec,
                   (void*)c->ghat_t, (void*)c->tw, (void*)c->partials, (void*)c->st,
                   (void*)c->gate, (void*)c->gbuf, (void*)c->tg.counts, (void*)c->tg.items,
@@ -1398,7 +1407,7 @@ void sc2d_track(ob_sc2p5drb* c, ob_bunch* b, double length, double pipe_radius,
     ProfScope ps_("sc2d_bin");
     hipLaunchKernelGGL(sc2d_bin, dim3(hb), dim3(kDepositThreads),
                        sizeof(double) * c->nz + sizeof(uint32_t) * tg.ntiles, s, P, v,
-                       c->partials, c->st, c->gate, tg, c->rho);
+                       c->partials, c->st, c->gate, tg, c->rho, c->status.dev);
   }
   if (c->plain) {
     // PoissonSolverFFT2D::setGridXY (PoissonSolverFFT2D.cc:103-111) when the grid pass
@@ -1437,11 +1446,6 @@ void sc2d_track(ob_sc2p5drb* c, ob_bunch* b, double length, double pipe_radius,
   launch_fft_pipeline(c, c->st);
   if (bnd) boundary_add_potential(bnd, c->phi, c->nx, c->ny, c->st, s);  // :90-94
   OB_HIP(hipGetLastError());
-  watch_status(&c->st->status,
-               c->plain ? "SpaceChargeCalc2p5D::bunchAnalysis: the bunch min and max sizes are "
-                          "wrong (zero extent); cannot calculate space charge"
-                        : "SpaceChargeCalc2p5Drb::bunchAnalysis: the bunch min and max sizes are "
-                          "wrong (zero extent); cannot calculate space charge");
   // kick (:146-167; 2p5D :105-119), fused into the next map kernel
   Op op;
   std::memset(&op, 0, sizeof(op));
