@@ -30,6 +30,7 @@ typedef struct ob_bunch ob_bunch;
 typedef struct ob_sc2p5drb ob_sc2p5drb;
 typedef struct ob_sc2p5d ob_sc2p5d;
 typedef struct ob_boundary2d ob_boundary2d;
+typedef struct ob_slice2d ob_slice2d;
 typedef struct ob_sc3d ob_sc3d;
 
 /* ---- library ---------------------------------------------------------- */
@@ -205,6 +206,20 @@ int ob_sc2p5d_get_grid(ob_sc2p5d* c, int which, double* out, double* lims);
 /* out[7] = xy_ratio, solver xMin, xMax, yMin, yMax, totalMacrosize of the last step,
  * Green function rebuilds so far (the calculator's private state, :152-172, 206-216). */
 int ob_sc2p5d_get_state(ob_sc2p5d* c, double* out);
+
+/* ---- SpaceChargeCalcSliceBySlice2D (src/spacecharge/SpaceChargeCalcSliceBySlice2D.cc;
+ *      wrap_spacechargecalc_slicebyslice_2D.cc:41, 107-155) ---- */
+/* SpaceChargeCalcSliceBySlice2D(nx, ny, nz[, xy_ratio]) (:15-32). */
+int ob_slice2d_create(int nx, int ny, int nz, double xy_ratio, ob_slice2d** out);
+int ob_slice2d_destroy(ob_slice2d* c);
+/* trackBunch(bunch, length[, boundary]) (:66-113, bunchAnalysis :116-218): 3D TSC deposit,
+ * one 2D Poisson solve per z slice (and the boundary's potential per slice), transverse
+ * kick xp -= ex*factor, yp -= ey*factor from the 3D gradient of phi; bnd NULL = none. */
+int ob_slice2d_track(ob_slice2d* c, ob_bunch* b, double length, ob_boundary2d* bnd);
+/* getRhoGrid / getPhiGrid: which 0 rho, 1 phi as [nz][nx][ny]; lims[6] x, y, z limits. */
+int ob_slice2d_get_grid(ob_slice2d* c, int which, double* out, double* lims);
+/* out[6] = xy_ratio, solver xMin, xMax, yMin, yMax, Green function rebuilds. */
+int ob_slice2d_get_state(ob_slice2d* c, double* out);
 
 /* ---- Boundary2D (src/spacecharge/BaseBoundary2D.cc, ShapedBoundary2D.cc;
  *      wrap_boundary2d.cc:38-75, 208-220) ---- */

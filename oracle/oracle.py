@@ -83,6 +83,10 @@ def lib():
             "orc_sc2p5d_get_grid": (i, [vp, i, _D, _D]),
             "orc_sc2p5d_state": (None, [vp, _D]),
             "orc_sc2p5d_track_boundary": (i, [vp, bp, d, vp]),
+            "orc_slice2d_create": (vp, [i, i, i, d]), "orc_slice2d_destroy": (None, [vp]),
+            "orc_slice2d_track": (i, [vp, bp, d, vp]),
+            "orc_slice2d_get_grid": (None, [vp, i, _D, _D]),
+            "orc_slice2d_state": (None, [vp, _D]),
             "orc_boundary2d_create": (vp, [i, i]),
             "orc_boundary2d_create_shaped": (vp, [i, i, i, d, d]),
             "orc_boundary2d_destroy": (None, [vp]),
@@ -310,6 +314,38 @@ def Grid2DValue(arr, lims, x, y):
     f.restype = C.c_double
     f.argtypes = [C.c_void_p, C.c_double, C.c_double]
     return f(C.byref(g), x, y)
+
+
+class SliceBySlice2D:
+    """SpaceChargeCalcSliceBySlice2D (SpaceChargeCalcSliceBySlice2D.cc:15-218)."""
+
+    def __init__(self, nx, ny, nz, xy_ratio=1.0):
+        self.nx, self.ny, self.nz = nx, ny, nz
+        self.h = lib().orc_slice2d_create(nx, ny, nz, xy_ratio)
+
+    def __del__(self):
+        if getattr(self, "h", None) and _lib is not None:
+            _lib.orc_slice2d_destroy(self.h)
+            self.h = None
+
+    def trackBunch(self, bunch, length, boundary=None):
+        rc = lib().orc_slice2d_track(self.h, bunch.ref, length,
+                                     boundary.h if boundary is not None else None)
+        if rc != 0:
+            raise RuntimeError("oracle: bunch has zero extent")
+
+    def grid(self, which):
+        """(array [nz][nx][ny], limits x, y, z)."""
+        out = np.zeros(self.nx * self.ny * self.nz)
+        lims = np.zeros(6)
+        lib().orc_slice2d_get_grid(self.h, which, _dp(out), _dp(lims))
+        return out.reshape(self.nz, self.nx, self.ny), lims
+
+    def state(self):
+        """(xy_ratio, solver box, Green rebuilds)."""
+        o = np.zeros(6)
+        lib().orc_slice2d_state(self.h, _dp(o))
+        return float(o[0]), tuple(o[1:5]), int(o[5])
 
 
 class SC3D:

@@ -1837,6 +1837,122 @@ void orc_sc2p5d_state(const orc_sc2p5d* c, double o[7]) {
 }
 
 /* ------------------------------------------------------------------------ */
+/* SpaceChargeCalcSliceBySlice2D: src/spacecharge/SpaceChargeCalcSliceBySlice2D.cc */
+struct orc_slice2d {
+  double xy_ratio;
+  orc_poisson2d* solver;
+  orc_grid3d rho, phi;
+  int rebuilds;
+};
+orc_slice2d* orc_slice2d_create(int nx, int ny, int nz, double xy_ratio) { /* :15-32 */
+  orc_slice2d* c = (orc_slice2d*)calloc(1, sizeof(orc_slice2d));
+  c->xy_ratio = xy_ratio;
+  c->solver = orc_poisson2d_create(nx, ny, -xy_ratio, xy_ratio, -1.0, 1.0);
+  orc_grid3d* gs[2] = {&c->rho, &c->phi};
+  for (int k = 0; k < 2; k++) {
+    gs[k]->nx = nx; gs[k]->ny = ny; gs[k]->nz = nz;
+    gs[k]->arr = (double*)calloc((size_t)nx * ny * nz, sizeof(double));
+  }
+  return c;
+}
+void orc_slice2d_destroy(orc_slice2d* c) {
+  if (!c) return;
+  orc_poisson2d_destroy(c->solver);
+  free(c->rho.arr); free(c->phi.arr); free(c);
+}
+/* bunchAnalysis (:116-218): the 2p5D ratio rule, or the boundary's box; the z grid spans
+   the bunch */
+static int slice2d_analysis(orc_slice2d* c, orc_bunch* b, const orc_boundary2d* bd) {
+  double e[6];
+  orc_extrema_xyz(b, e);
+  double xMin = e[0], xMax = e[1], yMin = e[2], yMax = e[3], zMin = e[4], zMax = e[5];
+  if (!bd) {
+    if (xMin >= xMax || yMin >= yMax || zMin >= zMax) return -1;
+    double xy_ratio_beam = (xMax - xMin) / (yMax - yMin);
+    if (fabs(xy_ratio_beam - c->xy_ratio) < 0.25 * c->xy_ratio) {
+      double width, center;
+      if (xy_ratio_beam > c->xy_ratio) {
+        center = (yMax + yMin) / 2.0;
+        width = ((yMax - yMin) * (xy_ratio_beam / c->xy_ratio)) / 2.0;
+        yMin = center - width;
+        yMax = center + width;
+      } else {
+        center = (xMax + xMin) / 2.0;
+        width = ((xMax - xMin) / (xy_ratio_beam / c->xy_ratio)) / 2.0;
+        xMin = center - width;
+        xMax = center + width;
+      }
+      c->xy_ratio = (xMax - xMin) / (yMax - yMin);
+    } else {
+      c->xy_ratio = xy_ratio_beam;
+      orc_poisson2d_set_grid(c->solver, xMin, xMax, yMin, yMax);
+      c->rebuilds++;
+    }
+  } else {
+    xMin = bd->xMin; xMax = bd->xMax; yMin = bd->yMin; yMax = bd->yMax;
+    c->xy_ratio = (xMax - xMin) / (yMax - yMin);
+    if (zMin >= zMax) return -1;
+  }
+  orc_grid3d* gs[2] = {&c->rho, &c->phi};
+  for (int k = 0; k < 2; k++) {
+    orc_grid3d_set_x(gs[k], xMin, xMax);
+    orc_grid3d_set_y(gs[k], yMin, yMax);
+    orc_grid3d_set_z(gs[k], zMin, zMax);
+  }
+  const orc_poisson2d* p = c->solver; /* :203-209 */
+  if (fabs((p->xMax - p->xMin) / (p->yMax - p->yMin) - c->xy_ratio) > 0.00000001) {
+    orc_poisson2d_set_grid(c->solver, xMin, xMax, yMin, yMax);
+    c->rebuilds++;
+  }
+  memset(c->rho.arr, 0, sizeof(double) * c->rho.nx * c->rho.ny * c->rho.nz);
+  orc_grid3d_bin_bunch(&c->rho, b);
+  return 0;
+}
+static orc_grid2d slice_view(orc_grid3d* g, int iz) {
+  orc_grid2d s;
+  s.nx = g->nx; s.ny = g->ny;
+  s.xMin = g->xMin; s.xMax = g->xMax; s.yMin = g->yMin; s.yMax = g->yMax;
+  s.dx = g->dx; s.dy = g->dy;
+  s.arr = g->arr + (size_t)iz * g->nx * g->ny;
+  return s;
+}
+/* trackBunch (:66-113): a 2D Poisson solve per z slice (plus the boundary's potential),
+   then the transverse kick from the 3D gradient of phi */
+int orc_slice2d_track(orc_slice2d* c, orc_bunch* b, double length, orc_boundary2d* bd) {
+  if (b->n < 2) return 0;
+  if (slice2d_analysis(c, b, bd) != 0) return -1;
+  for (int iz = 0; iz < c->rho.nz; iz++) {
+    orc_grid2d r = slice_view(&c->rho, iz), f = slice_view(&c->phi, iz);
+    orc_poisson2d_find_potential(c->solver, &r, &f);
+    if (bd) orc_boundary2d_add_potential(bd, &f);
+  }
+  double beta = b->s.beta, gamma = b->s.gamma;
+  double factor = 2 * length / c->rho.dz * b->s.classical_radius / (pow(beta, 2) * pow(gamma, 3));
+  for (int64_t i = 0; i < b->n; i++) {
+    if (bd && orc_boundary2d_is_inside(bd, b->x[i], b->y[i]) != 1) continue;
+    double ex, ey, ez;
+    orc_grid3d_gradient(&c->phi, b->x[i], b->y[i], b->z[i], &ex, &ey, &ez);
+    b->xp[i] -= ex * factor;
+    b->yp[i] -= ey * factor;
+  }
+  return 0;
+}
+/* which 0 rho, 1 phi ([nz][nx][ny]); lims = x, y, z limits */
+void orc_slice2d_get_grid(const orc_slice2d* c, int which, double* out, double* lims) {
+  const orc_grid3d* g = which == 0 ? &c->rho : &c->phi;
+  if (out) memcpy(out, g->arr, sizeof(double) * g->nx * g->ny * g->nz);
+  if (lims) {
+    lims[0] = g->xMin; lims[1] = g->xMax; lims[2] = g->yMin;
+    lims[3] = g->yMax; lims[4] = g->zMin; lims[5] = g->zMax;
+  }
+}
+void orc_slice2d_state(const orc_slice2d* c, double o[6]) {
+  o[0] = c->xy_ratio;
+  o[1] = c->solver->xMin; o[2] = c->solver->xMax; o[3] = c->solver->yMin; o[4] = c->solver->yMax;
+  o[5] = c->rebuilds;
+}
+
+/* ------------------------------------------------------------------------ */
 /* SpaceChargeCalc3D: src/spacecharge/SpaceChargeCalc3D.cc */
 struct orc_sc3d {
   double xy_ratio, xz_ratio, ratio_limit;

@@ -186,6 +186,14 @@ int orc_sc2p5d_get_grid(const orc_sc2p5d* c, int which, double* out, double* lim
 void orc_sc2p5d_state(const orc_sc2p5d* c, double out7[7]);
 
 /* ---- SpaceChargeCalc3D (src/spacecharge/SpaceChargeCalc3D.cc) ---- */
+/* SpaceChargeCalcSliceBySlice2D (SpaceChargeCalcSliceBySlice2D.cc); bd may be NULL */
+typedef struct orc_slice2d orc_slice2d;
+orc_slice2d* orc_slice2d_create(int nx, int ny, int nz, double xy_ratio);
+void orc_slice2d_destroy(orc_slice2d* c);
+int orc_slice2d_track(orc_slice2d* c, orc_bunch* b, double length, orc_boundary2d* bd);
+void orc_slice2d_get_grid(const orc_slice2d* c, int which, double* out, double* lims);
+void orc_slice2d_state(const orc_slice2d* c, double out6[6]);
+
 typedef struct orc_sc3d orc_sc3d;
 orc_sc3d* orc_sc3d_create(int nx, int ny, int nz);
 void orc_sc3d_destroy(orc_sc3d* c);

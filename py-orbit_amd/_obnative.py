@@ -74,6 +74,11 @@ SIGNATURES = {
     "ob_sc2p5d_get_grid": [_vp, _i, _D, _D],
     "ob_sc2p5d_get_state": [_vp, _D],
     "ob_sc2p5d_track_boundary": [_vp, _vp, _d, _vp],
+    "ob_slice2d_create": [_i, _i, _i, _d, C.POINTER(_vp)],
+    "ob_slice2d_destroy": [_vp],
+    "ob_slice2d_track": [_vp, _vp, _d, _vp],
+    "ob_slice2d_get_grid": [_vp, _i, _D, _D],
+    "ob_slice2d_get_state": [_vp, _D],
     "ob_boundary2d_create": [_i, _i, C.POINTER(_vp)],
     "ob_boundary2d_create_shaped": [_i, _i, C.c_char_p, _d, _d, C.POINTER(_vp)],
     "ob_boundary2d_destroy": [_vp],

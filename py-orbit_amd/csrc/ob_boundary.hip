@@ -27,6 +27,8 @@ struct ob_boundary2d {
   double xMin = DBL_MAX, xMax = -DBL_MAX, yMin = DBL_MAX, yMax = -DBL_MAX, rnorm = DBL_MAX;
   std::vector<double> bx, by, lsq;  // lsq: [(2 nm + 1)][np]
   double *d_bx = nullptr, *d_by = nullptr, *d_lsq = nullptr, *d_work = nullptr;
+  double* d_work3 = nullptr;  // per-slice work of a stack of grids
+  int64_t work3_cap = 0;
   ob::SC2DState* d_geo = nullptr;  // grid geometry of a user grid (add_potential)
   int d_np = 0;
 };
@@ -93,6 +95,75 @@ __global__ __launch_bounds__(256) void bnd_apply(double* __restrict__ phi, int n
   } else {
     const double r = sqrt(r2);
     const double sn = y / r, cs = x / r, q = r / rnorm;
+    double snk = sn, csk = cs, rk = 1.0;
+    v += coef[0] * 1.;
+    for (int k = 0; k < nm; k++) {
+      rk *= q;
+      v += coef[2 * k + 1] * (rk * csk);
+      v += coef[2 * k + 2] * (rk * snk);
+      const double s1 = sn * csk + cs * snk;
+      const double c1 = cs * csk - sn * snk;
+      snk = s1;
+      csk = c1;
+    }
+  }
+  phi[t] -= v;
+}
+
+// A stack of z-fastest slices phi[(ix*ny + iy)*nz + iz]: one workgroup per slice iz.
+// work = [nz][np] phi at the points, then [nz][2 nm + 1] coefficients.
+__global__ __launch_bounds__(kCoefThreads) void bnd_coef_slices(
+    const double* __restrict__ phi, int nx, int ny, int nz, const SC2DState* st,
+    const double* __restrict__ bx, const double* __restrict__ by, int np,
+    const double* __restrict__ lsq, int nf, double* __restrict__ work) {
+  if (st->skip) return;
+  const int iz = blockIdx.x;
+  const double xMin = st->xMin, dx = st->dx, yMin = st->yMin, dy = st->dy;
+  double* bphi = work + (size_t)iz * np;
+  for (int j = threadIdx.x; j < np; j += blockDim.x) {
+    int iX, iY;
+    double xF, yF;
+    node_ind_frac(bx[j], xMin, dx, nx, iX, xF);
+    node_ind_frac(by[j], yMin, dy, ny, iY, yF);
+    const double xF2 = xF * xF, yF2 = yF * yF;
+    const double Wxm = 0.5 * (0.25 - xF + xF2), Wx0 = 0.75 - xF2, Wxp = 0.5 * (0.25 + xF + xF2);
+    const double Wym = 0.5 * (0.25 - yF + yF2), Wy0 = 0.75 - yF2, Wyp = 0.5 * (0.25 + yF + yF2);
+    auto a = [&](int ix, int iy) { return phi[((int64_t)ix * ny + iy) * nz + iz]; };
+    bphi[j] = Wxm * Wym * a(iX - 1, iY - 1) + Wxm * Wy0 * a(iX - 1, iY) +
+              Wxm * Wyp * a(iX - 1, iY + 1) + Wx0 * Wym * a(iX, iY - 1) + Wx0 * Wy0 * a(iX, iY) +
+              Wx0 * Wyp * a(iX, iY + 1) + Wxp * Wym * a(iX + 1, iY - 1) +
+              Wxp * Wy0 * a(iX + 1, iY) + Wxp * Wyp * a(iX + 1, iY + 1);
+  }
+  __syncthreads();
+  double* coef = work + (size_t)nz * np + (size_t)iz * nf;
+  for (int i = threadIdx.x; i < nf; i += blockDim.x) {
+    double c = 0.0;
+    const double* row = lsq + (size_t)i * np;
+    for (int j = 0; j < np; j++) c += row[j] * bphi[j];
+    coef[i] = c;
+  }
+}
+
+__global__ __launch_bounds__(256) void bnd_apply_slices(double* __restrict__ phi, int nx, int ny,
+                                                        int nz, const SC2DState* st,
+                                                        const double* __restrict__ coefs,
+                                                        int nm, double rnorm) {
+  if (st->skip) return;
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= (int64_t)nx * ny * nz) return;
+  const int iz = (int)(t % nz);
+  const int64_t r = t / nz;
+  const int i = (int)(r / ny), j = (int)(r - (int64_t)i * ny);
+  const double* coef = coefs + (size_t)iz * (2 * nm + 1);
+  const double x = st->xMin + i * st->dx, y = st->yMin + j * st->dy;
+  const double r2 = x * x + y * y;
+  double v = 0.;
+  if (r2 == 0.) {
+    v += coef[0] * 1.0;
+    for (int k = 1; k < 2 * nm + 1; k++) v += coef[k] * 0.0;
+  } else {
+    const double rr = sqrt(r2);
+    const double sn = y / rr, cs = x / rr, q = rr / rnorm;
     double snk = sn, csk = cs, rk = 1.0;
     v += coef[0] * 1.;
     for (int k = 0; k < nm; k++) {
@@ -249,6 +320,32 @@ void boundary_add_potential(ob_boundary2d* b, double* phi, int nx, int ny, const
   OB_HIP(hipGetLastError());
 }
 
+void boundary_add_potential_slices(ob_boundary2d* b, double* phi, int nx, int ny, int nz,
+                                   const SC2DState* st, hipStream_t s) {
+  OB_REQUIRE(b->initialized,
+             "Boundary2D addBoundaryPotential: the boundary was not initialized");
+  const int nf = 2 * b->nm + 1;
+  const int64_t need = (int64_t)nz * (b->np + nf);
+  if (need > b->work3_cap) {
+    OB_HIP(hipStreamSynchronize(s));
+    if (b->d_work3) OB_HIP(hipFree(b->d_work3));
+    OB_HIP(hipMalloc((void**)&b->d_work3, sizeof(double) * need));
+    b->work3_cap = need;
+  }
+  {
+    ProfScope ps("bnd_coef_slices", s);
+    hipLaunchKernelGGL(bnd_coef_slices, dim3(nz), dim3(kCoefThreads), 0, s, phi, nx, ny, nz, st,
+                       b->d_bx, b->d_by, b->np, b->d_lsq, nf, b->d_work3);
+  }
+  {
+    ProfScope ps("bnd_apply_slices", s);
+    const int64_t tot = (int64_t)nx * ny * nz;
+    hipLaunchKernelGGL(bnd_apply_slices, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, s,
+                       phi, nx, ny, nz, st, b->d_work3 + (size_t)nz * b->np, b->nm, b->rnorm);
+  }
+  OB_HIP(hipGetLastError());
+}
+
 int boundary_describe(ob_boundary2d* b, double* bnd6) {
   OB_REQUIRE(b->initialized, "Boundary2D: the boundary was not initialized");
   bnd6[0] = b->xMin; bnd6[1] = b->xMax; bnd6[2] = b->yMin; bnd6[3] = b->yMax;
@@ -372,7 +469,7 @@ OB_API int ob_boundary2d_destroy(ob_boundary2d* b) {
   OB_TRY if (!b) return 0;
   (void)hipStreamSynchronize(ob::stream());  // queued steps may read its arrays
   for (void* p : {(void*)b->d_bx, (void*)b->d_by, (void*)b->d_lsq, (void*)b->d_work,
-                  (void*)b->d_geo})
+                  (void*)b->d_geo, (void*)b->d_work3})
     if (p) (void)hipFree(p);
   delete b;
   OB_CATCH

@@ -239,6 +239,9 @@ namespace ob {
 // geometry is in st (skipped when st->skip); the boundary must be initialized.
 void boundary_add_potential(ob_boundary2d* b, double* phi, int nx, int ny, const SC2DState* st,
                             hipStream_t s);
+// The same for each z slice of a z-fastest [nx][ny][nz] stack (its own coefficients).
+void boundary_add_potential_slices(ob_boundary2d* b, double* phi, int nx, int ny, int nz,
+                                   const SC2DState* st, hipStream_t s);
 // Type (1 point set, 2 circle, 3 ellipse, 4 rectangle) and the six numbers of
 // SC2DState::bnd for an initialized boundary.
 int boundary_describe(ob_boundary2d* b, double* bnd6);

@@ -35,7 +35,9 @@ __host__ __device__ constexpr int zpass_pairs(int n) { return n >= 256 ? (n >= 2
 __host__ __device__ constexpr int strided_lines(int n) { return n >= 2048 ? 4 : 8; }
 __host__ __device__ constexpr int strided_threads(int n) { return n >= 256 ? 512 : 256; }
 
-enum StridedMode { FWD = 0, INV = 1, CONV = 2 };
+// CONV2: the x pass of a stack of independent 2D solves (SpaceChargeCalcSliceBySlice2D):
+// the Green spectrum is 2D, ghat[kx * qy + ky], the same for every line.
+enum StridedMode { FWD = 0, INV = 1, CONV = 2, CONV2 = 3 };
 
 // LDS doubles per line (re or im) of a strided block: the larger of the forward and the
 // reversed plan's padded layouts.
@@ -307,7 +309,7 @@ __device__ __forceinline__ void strided_compute(
   using P0 = StridedIn<MODE, N>;
   double ur[P0::BL][P0::RL], ui[P0::BL][P0::RL];
   plan_regs<N, REV0, L, NT, SIGN0>(re, im, nl, tw, nmax, vr, vi, ur, ui);
-  if constexpr (MODE == CONV) {
+  if constexpr (MODE == CONV || MODE == CONV2) {
     using PI = RegPlan<N, true, L, NT>;
 #pragma unroll
     for (int q = 0; q < P0::BL; q++)
@@ -354,7 +356,7 @@ template <int MODE, int N>
 __global__ __launch_bounds__(strided_threads(N)) void fft3_strided(
     double2* __restrict__ spec, int n_in, int n_out, int64_t line_stride, int64_t outer_stride,
     int outers, int H2, const double* __restrict__ ghat, int qy, int H2p, int M1,
-    const double2* __restrict__ tw, int nmax) {
+    const double2* __restrict__ tw, int nmax, double s_coeff, double npad) {
   constexpr int L = strided_lines(N), NT = strided_threads(N);
   using PF = StridedFwd<N>;
   using P0 = StridedIn<MODE, N>;
@@ -367,7 +369,7 @@ __global__ __launch_bounds__(strided_threads(N)) void fft3_strided(
   const int nl = min(L, H2 - k0);
   int r0, r1;
   bool two;
-  if constexpr (MODE == CONV) {
+  if constexpr (MODE == CONV || MODE == CONV2) {
     r0 = o;
     r1 = M1 - o;
     two = (o != 0 && 2 * o != M1);
@@ -393,6 +395,22 @@ __global__ __launch_bounds__(strided_threads(N)) void fft3_strided(
         const int p = j + m * PF::NBL;
         const int gx = (p <= N / 2) ? p : N - p;
         g[q][m] = ok ? ghat[((int64_t)gx * qy + o) * H2p + k0 + line] : 0.0;
+      }
+    }
+  }
+  if constexpr (MODE == CONV2) {  // PoissonSolverFFT2D.cc:215-228: rescaled to the rho step
+#pragma unroll
+    for (int q = 0; q < PF::BL; q++) {
+      const int b = (int)threadIdx.x + q * NT;
+      const int j = b / L;
+      const bool ok = b < L * PF::NBL;
+#pragma unroll
+      for (int m = 0; m < PF::RL; m++) {
+        const int p = j + m * PF::NBL;
+        const int gx = (p <= N / 2) ? p : N - p;
+        double v = ok ? ghat[(int64_t)gx * qy + o] : 0.0;
+        if (gx == 0 && o == 0) v = v + s_coeff * npad;
+        g[q][m] = v - s_coeff;
       }
     }
   }
@@ -523,7 +541,7 @@ void launch_strided_n(ob_sc3d* c, int n_in, int n_out, int64_t line_stride,
   ob::ProfScope ps_(label);
   hipLaunchKernelGGL((fft3_strided<MODE, N>), dim3(blocks_outer * tiles), dim3(strided_threads(N)),
                      lds, stream(), c->spec, n_in, n_out, line_stride, outer_stride, outers, c->H2,
-                     c->ghat, c->qy, c->H2p, c->M1, c->tw, c->nmax);
+                     c->ghat, c->qy, c->H2p, c->M1, c->tw, c->nmax, 0.0, 0.0);
   OB_HIP(hipGetLastError());
 }
 
@@ -936,5 +954,373 @@ OB_API int ob_sc3d_set_deposit_mode(ob_sc3d* c, int tiled, int lds_budget_bytes)
              "LDS budget must be 0 (keep) or 1 KiB..150 KiB");
   c->tiled = tiled != 0;
   if (lds_budget_bytes) c->lds_budget = lds_budget_bytes;
+  OB_CATCH
+}
+
+// ---------------------------------------------------------------------------------------
+// SpaceChargeCalcSliceBySlice2D (SpaceChargeCalcSliceBySlice2D.cc): the 3D TSC deposit of
+// SpaceChargeCalc3D (lab frame), then one 2D Hockney solve per z slice
+// (PoissonSolverFFT2D.cc:174-256), then the transverse kick from the 3D gradient of phi.
+// The nz slice solves run as one stack: the real slices 2l and 2l+1 are the real and
+// imaginary parts of complex line l (the 2D Green spectrum is real and even, so the two
+// solutions stay apart), and the stack goes through the batched y / x / y passes of the
+// 3D solver with a 2D Green spectrum.
+struct ob_slice2d {
+  int nx, ny, nz, M0, M1, L2, H2p, qx, qy, nmax;
+  double xy_ratio;
+  double sol[4];         // solver box (PoissonSolverFFT2D xMin_, xMax_, yMin_, yMax_)
+  double sdx, sdy;       // solver steps
+  int rebuilds = 0;
+  bool green_valid = false;
+  double g[9];           // last grid: xMin, xMax, yMin, yMax, zMin, zMax, dx, dy, dz
+  double *rho = nullptr, *phi = nullptr, *ghat = nullptr, *partials = nullptr;
+  double *ext_dev = nullptr, *ext_host = nullptr;
+  double2 *spec = nullptr, *tw = nullptr;
+  ob::SC2DState* geo = nullptr;    // grid geometry for the boundary kernels (device)
+  ob::SC2DState* geo_host = nullptr;  // pinned staging of it
+  ob::TileWork work;
+  int lds_budget = 16 * 1024;
+};
+
+namespace ob {
+namespace {
+
+// Quadrant i <= M0/2, j <= M1/2 of the 2D Green function -ln(r^2)/2 on the solver steps
+// (PoissonSolverFFT2D::_defineGreenF, :115-150), stored ghat[i * qy + j].
+__global__ void green2d_quadrant(double* __restrict__ q, int qx, int qy, double dx, double dy) {
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= (int64_t)qx * qy) return;
+  const int i = (int)(t / qy), j = (int)(t - (int64_t)i * qy);
+  const double rTransX = i * dx, rTransY = j * dy;
+  const double rTot2 = rTransX * rTransX + rTransY * rTransY;
+  q[t] = (i != 0 || j != 0) ? -log(rTot2) / 2 : 0.0;
+}
+
+// rho slices 2l, 2l+1 -> complex line l of spec[x][y][l] (x < nx, y < ny).
+__global__ void slice_pack(const double* __restrict__ rho, int nx, int ny, int nz, int L2,
+                           int H2p, int M1, double2* __restrict__ spec) {
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= (int64_t)nx * ny * L2) return;
+  const int l = (int)(t % L2);
+  const int64_t r = t / L2;
+  const int x = (int)(r / ny), y = (int)(r - (int64_t)x * ny);
+  const double* src = rho + r * nz;
+  const double a = src[2 * l];
+  const double b = (2 * l + 1 < nz) ? src[2 * l + 1] : 0.0;
+  spec[((int64_t)x * M1 + y) * H2p + l] = make_double2(a, b);
+}
+
+// complex line l -> phi slices 2l, 2l+1, scaled by 1 / (M0 M1) (:252-255).
+__global__ void slice_unpack(const double2* __restrict__ spec, int nx, int ny, int nz, int L2,
+                             int H2p, int M1, double denom, double* __restrict__ phi) {
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= (int64_t)nx * ny * L2) return;
+  const int l = (int)(t % L2);
+  const int64_t r = t / L2;
+  const int x = (int)(r / ny), y = (int)(r - (int64_t)x * ny);
+  const double2 v = spec[((int64_t)x * M1 + y) * H2p + l];
+  double* dst = phi + r * nz;
+  dst[2 * l] = denom * v.x;
+  if (2 * l + 1 < nz) dst[2 * l + 1] = denom * v.y;
+}
+
+template <int MODE>
+void slice_strided(ob_slice2d* c, int n, int n_in, int n_out, int64_t line_stride,
+                   int64_t outer_stride, int outers, double s_coeff, const char* label) {
+  dispatch_pow2(n, [&](auto NC) {
+    constexpr int N = decltype(NC)::value;
+    if constexpr (N >= 8 && N <= 2048) {
+      static bool ready = false;
+      const size_t lds = strided_lds<N>();
+      if (!ready) {
+        allow_lds(fft3_strided<MODE, N>, lds);
+        ready = true;
+      }
+      const int tiles = (c->L2 + strided_lines(N) - 1) / strided_lines(N);
+      const int blocks_outer = (MODE == CONV2) ? c->M1 / 2 + 1 : (outers + 1) / 2;
+      ProfScope ps_(label);
+      hipLaunchKernelGGL((fft3_strided<MODE, N>), dim3(blocks_outer * tiles),
+                         dim3(strided_threads(N)), lds, stream(), c->spec, n_in, n_out,
+                         line_stride, outer_stride, outers, c->L2, c->ghat, c->qy, c->H2p, c->M1,
+                         c->tw, c->nmax, s_coeff, (double)c->M0 * (double)c->M1);
+    } else {
+      throw Error{"SpaceChargeCalcSliceBySlice2D: transverse FFT length out of range"};
+    }
+  });
+  OB_HIP(hipGetLastError());
+}
+
+// PoissonSolverFFT2D::setGridXY (:103-111): the Green spectrum of the new box, as the
+// even-symmetric transform of its quadrant along y, then along x.
+void slice_set_solver(ob_slice2d* c, double xMin, double xMax, double yMin, double yMax) {
+  c->sol[0] = xMin; c->sol[1] = xMax; c->sol[2] = yMin; c->sol[3] = yMax;
+  c->sdx = (xMax - xMin) / (c->nx - 1);
+  c->sdy = (yMax - yMin) / (c->ny - 1);
+  hipStream_t s = stream();
+  const int64_t tot = (int64_t)c->qx * c->qy;
+  {
+    ProfScope ps_("green2d_quadrant");
+    hipLaunchKernelGGL(green2d_quadrant, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, s,
+                       c->ghat, c->qx, c->qy, c->sdx, c->sdy);
+  }
+  auto axis = [&](int n, int64_t line_stride, int64_t outer_stride, int outers, int lines) {
+    dispatch_pow2(n, [&](auto NC) {
+      constexpr int N = decltype(NC)::value;
+      if constexpr (N >= 8 && N <= 2048) {
+        static bool ready = false;
+        const size_t lds = sizeof(double) * 2 * (strided_lines(N) / 2) * (N + 1);
+        if (!ready) {
+          allow_lds(green3d_axis<N>, lds);
+          ready = true;
+        }
+        const int tiles = (lines + strided_lines(N) - 1) / strided_lines(N);
+        ProfScope ps_("green2d_axis");
+        hipLaunchKernelGGL(green3d_axis<N>, dim3(outers * tiles), dim3(kThreads), lds, s,
+                           c->ghat, line_stride, outer_stride, lines, c->tw, c->nmax);
+      }
+    });
+    OB_HIP(hipGetLastError());
+  };
+  axis(c->M1, 1, c->qy, c->qx, 1);    // y: one line per i
+  axis(c->M0, c->qy, 0, 1, c->qy);    // x: lines j
+  c->green_valid = true;
+}
+
+// One trackBunch (SpaceChargeCalcSliceBySlice2D.cc:66-113), boundary bnd or none.
+void slice_track(ob_slice2d* c, ob_bunch* b, double length, ob_boundary2d* bnd) {
+  check_statuses(false);
+  const int64_t n = bunch_size(b);
+  if (n < 2) return;  // :69-70
+  hipStream_t s = stream();
+  const int nblk = bunch_flush(b, c->partials);
+  {
+    ProfScope ps_("reduce_extrema");
+    hipLaunchKernelGGL(reduce_extrema, dim3(1), dim3(1024), 0, s, c->partials, nblk, c->ext_dev);
+  }
+  OB_HIP(hipGetLastError());
+  OB_HIP(hipMemcpyAsync(c->ext_host, c->ext_dev, sizeof(double) * 6, hipMemcpyDeviceToHost, s));
+  OB_HIP(hipStreamSynchronize(s));
+  const SyncInfo sy = bunch_sync(b);
+  double xMin = c->ext_host[0], xMax = c->ext_host[1], yMin = c->ext_host[2];
+  double yMax = c->ext_host[3], zMin = c->ext_host[4], zMax = c->ext_host[5];
+  double bnd6[6] = {0, 0, 0, 0, 0, 0};
+  int bnd_type = 0;
+  // bunchAnalysis (:116-218), scalar arithmetic as in the reference
+  if (!bnd) {
+    OB_REQUIRE(!(xMin >= xMax || yMin >= yMax || zMin >= zMax),
+               "SpaceChargeCalcSliceBySlice2D::bunchAnalysis: the bunch min and max sizes are "
+               "wrong (zero extent); cannot calculate space charge");
+    const double xy_ratio_beam = (xMax - xMin) / (yMax - yMin);
+    if (fabs(xy_ratio_beam - c->xy_ratio) < 0.25 * c->xy_ratio) {
+      double width, center;
+      if (xy_ratio_beam > c->xy_ratio) {
+        center = (yMax + yMin) / 2.0;
+        width = ((yMax - yMin) * (xy_ratio_beam / c->xy_ratio)) / 2.0;
+        yMin = center - width;
+        yMax = center + width;
+      } else {
+        center = (xMax + xMin) / 2.0;
+        width = ((xMax - xMin) / (xy_ratio_beam / c->xy_ratio)) / 2.0;
+        xMin = center - width;
+        xMax = center + width;
+      }
+      c->xy_ratio = (xMax - xMin) / (yMax - yMin);
+    } else {
+      c->xy_ratio = xy_ratio_beam;
+      slice_set_solver(c, xMin, xMax, yMin, yMax);
+      c->rebuilds++;
+    }
+  } else {
+    bnd_type = boundary_describe(bnd, bnd6);
+    xMin = bnd6[0]; xMax = bnd6[1]; yMin = bnd6[2]; yMax = bnd6[3];
+    c->xy_ratio = (xMax - xMin) / (yMax - yMin);
+    OB_REQUIRE(!(zMin >= zMax),
+               "SpaceChargeCalcSliceBySlice2D::bunchAnalysis: the bunch min and max sizes are "
+               "wrong (zero extent); cannot calculate space charge");
+  }
+  Grid3 g;
+  g.nx = c->nx; g.ny = c->ny; g.nz = c->nz;
+  g.xMin = xMin; g.xMax = xMax; g.dx = (xMax - xMin) / (c->nx - 1);  // Grid3D.cc:141-158
+  g.yMin = yMin; g.yMax = yMax; g.dy = (yMax - yMin) / (c->ny - 1);
+  g.zMin = zMin; g.zMax = zMax; g.dz = (zMax - zMin) / c->nz;         // :165-170
+  if (fabs((c->sol[1] - c->sol[0]) / (c->sol[3] - c->sol[2]) - c->xy_ratio) > 0.00000001) {
+    slice_set_solver(c, xMin, xMax, yMin, yMax);  // :203-209
+    c->rebuilds++;
+  }
+  c->g[0] = g.xMin; c->g[1] = g.xMax; c->g[2] = g.yMin; c->g[3] = g.yMax;
+  c->g[4] = g.zMin; c->g[5] = g.zMax; c->g[6] = g.dx; c->g[7] = g.dy; c->g[8] = g.dz;
+  // Grid3D::binBunch (Grid3D.cc:281-305)
+  const int64_t ng = (int64_t)c->nx * c->ny * c->nz;
+  {
+    ProfScope ps_("fill_zero");
+    hipLaunchKernelGGL(fill_zero, dim3(1024), dim3(256), 0, s, c->rho, ng);
+  }
+  const TileGeom tg = make_tile_geom(g.nx, g.ny, g.nz, g.xMin, g.xMax, g.yMin, g.yMax, g.zMin,
+                                     g.zMax, g.dx, g.dy, g.dz, c->lds_budget);
+  const bool tiled = tg.T > 0;
+  OB_REQUIRE(tiled || !bnd, "SpaceChargeCalcSliceBySlice2D: grid too large for the tiled kick "
+                            "with a boundary");
+  if (tiled) {
+    sort_and_deposit_3d(b, tg, sy.macrosize, c->rho, c->work);
+  } else {
+    DeviceBunchView v = bunch_device_view(b);
+    int dblk = (int)((n + kThreads - 1) / kThreads);
+    const int cap = 16 * device_cu_count();
+    if (dblk > cap) dblk = cap;
+    ProfScope ps_("deposit3d");
+    hipLaunchKernelGGL(deposit3d, dim3(dblk), dim3(kThreads), 0, s, v, g, sy.macrosize, 0.0,
+                       c->rho);
+  }
+  OB_HIP(hipGetLastError());
+  // one findPotential per slice (:77-90), as a stack
+  const int64_t np = (int64_t)c->nx * c->ny * c->L2;
+  {
+    ProfScope ps_("slice_pack");
+    hipLaunchKernelGGL(slice_pack, dim3((unsigned)((np + 255) / 256)), dim3(256), 0, s, c->rho,
+                       c->nx, c->ny, c->nz, c->L2, c->H2p, c->M1, c->spec);
+  }
+  const int64_t H2p = c->H2p;
+  const double scale_coeff = log(c->sdx / g.dx);  // PoissonSolverFFT2D.cc:216
+  slice_strided<FWD>(c, c->M1, c->ny, c->M1, H2p, (int64_t)c->M1 * H2p, c->nx, 0.0,
+                     "slice_fft_fwd");
+  slice_strided<CONV2>(c, c->M0, c->nx, c->nx, (int64_t)c->M1 * H2p, H2p, c->M1, scale_coeff,
+                       "slice_fft_conv");
+  slice_strided<INV>(c, c->M1, c->M1, c->ny, H2p, (int64_t)c->M1 * H2p, c->nx, 0.0,
+                     "slice_fft_inv");
+  {
+    ProfScope ps_("slice_unpack");
+    hipLaunchKernelGGL(slice_unpack, dim3((unsigned)((np + 255) / 256)), dim3(256), 0, s,
+                       c->spec, c->nx, c->ny, c->nz, c->L2, c->H2p, c->M1,
+                       1.0 / ((double)c->M0 * c->M1), c->phi);
+  }
+  OB_HIP(hipGetLastError());
+  if (bnd) {  // :86-88
+    SC2DState& h = *c->geo_host;
+    h.xMin = g.xMin; h.dx = g.dx; h.yMin = g.yMin; h.dy = g.dy; h.skip = 0;
+    OB_HIP(hipMemcpyAsync(c->geo, c->geo_host, sizeof(SC2DState), hipMemcpyHostToDevice, s));
+    boundary_add_potential_slices(bnd, c->phi, c->nx, c->ny, c->nz, c->geo, s);
+  }
+  // kick (:96-112): xp -= ex factor, yp -= ey factor, factor = 2 L / dz r0 / (beta^2 gamma^3)
+  const double factor = 2 * length / g.dz * sy.r0 / (pow(sy.beta, 2) * pow(sy.gamma, 3));
+  if (tiled) {
+    Kick3D k{c->phi, g.zMin, g.dz, 0.0, 1.0, factor, 0.0};
+    k.bnd_type = bnd_type;
+    for (int i = 0; i < 6; i++) k.bnd[i] = bnd6[i];
+    tile_kick_3d(b, tg, k, c->work);
+    return;
+  }
+  Op op;  // the 3D kick op with gamma 1 and z centre 0 (z unchanged), no dE kick
+  std::memset(&op, 0, sizeof(op));
+  op.kind = OP_SC3D_KICK;
+  op.i0 = c->nx; op.i1 = c->ny; op.i2 = c->nz;
+  op.p[0] = ptr_bits(c->phi);
+  op.p[1] = g.xMin; op.p[2] = g.dx; op.p[3] = g.yMin; op.p[4] = g.dy;
+  op.p[5] = g.zMin; op.p[6] = g.dz;
+  op.p[7] = 0.0;
+  op.p[8] = 1.0;
+  op.p[9] = factor;
+  op.p[10] = 0.0;
+  op.p[11] = 0.0;
+  bunch_queue(b, op);
+}
+}  // namespace
+}  // namespace ob
+
+// SpaceChargeCalcSliceBySlice2D(nx, ny, nz[, xy_ratio]) (:15-32).
+OB_API int ob_slice2d_create(int nx, int ny, int nz, double xy_ratio, ob_slice2d** out) {
+  OB_TRY OB_REQUIRE(out != nullptr, "null output pointer");
+  OB_REQUIRE(nx >= 3 && ny >= 3,
+             "SpaceChargeCalcSliceBySlice2D: grid size too small (should be >= 3)");
+  OB_REQUIRE(nz >= 1, "SpaceChargeCalcSliceBySlice2D: number of z slices should be >= 1");
+  OB_REQUIRE(xy_ratio > 0.0, "SpaceChargeCalcSliceBySlice2D: xy_ratio must be positive");
+  OB_REQUIRE(ob::pad_len(nx) <= 2048 && ob::pad_len(ny) <= 2048,
+             "SpaceChargeCalcSliceBySlice2D: grid larger than 1024 points per axis");
+  ob::ensure_init();
+  ob_slice2d* c = new ob_slice2d();
+  c->nx = nx; c->ny = ny; c->nz = nz;
+  c->M0 = ob::pad_len(nx);
+  c->M1 = ob::pad_len(ny);
+  c->L2 = (nz + 1) / 2;
+  c->H2p = (c->L2 + 7) & ~7;
+  c->qx = c->M0 / 2 + 1;
+  c->qy = c->M1 / 2 + 1;
+  c->nmax = std::max(c->M0, c->M1);
+  c->xy_ratio = xy_ratio;
+  for (int k = 0; k < 9; k++) c->g[k] = 0.0;  // Grid3D(nx, ny, nz) limits (Grid3D.cc:37-39)
+  const size_t ng = (size_t)nx * ny * nz;
+  OB_HIP(hipMalloc((void**)&c->rho, sizeof(double) * ng));
+  OB_HIP(hipMalloc((void**)&c->phi, sizeof(double) * ng));
+  OB_HIP(hipMalloc((void**)&c->spec, sizeof(double2) * (size_t)nx * c->M1 * c->H2p));
+  OB_HIP(hipMalloc((void**)&c->ghat, sizeof(double) * (size_t)c->qx * c->qy));
+  OB_HIP(hipMalloc((void**)&c->partials, sizeof(double) * 6 * 4096));
+  OB_HIP(hipMalloc((void**)&c->ext_dev, sizeof(double) * 6));
+  OB_HIP(hipHostMalloc((void**)&c->ext_host, sizeof(double) * 6, hipHostMallocDefault));
+  OB_HIP(hipMalloc((void**)&c->geo, sizeof(ob::SC2DState)));
+  OB_HIP(hipHostMalloc((void**)&c->geo_host, sizeof(ob::SC2DState), hipHostMallocDefault));
+  std::memset(c->geo_host, 0, sizeof(ob::SC2DState));
+  OB_HIP(hipMemset(c->rho, 0, sizeof(double) * ng));
+  OB_HIP(hipMemset(c->phi, 0, sizeof(double) * ng));
+  OB_HIP(hipMemset(c->spec, 0, sizeof(double2) * (size_t)nx * c->M1 * c->H2p));
+  c->tw = ob::make_twiddles(c->nmax);
+  ob::slice_set_solver(c, -xy_ratio, xy_ratio, -1.0, 1.0);  // PoissonSolverFFT2D(:19)
+  OB_HIP(hipStreamSynchronize(ob::stream()));
+  *out = c;
+  OB_CATCH
+}
+
+OB_API int ob_slice2d_destroy(ob_slice2d* c) {
+  OB_TRY if (!c) return 0;
+  std::string err;
+  try {
+    ob::flush_kicks_reading(c->phi);
+  } catch (const ob::Error& e) {
+    err = e.msg;
+  }
+  (void)hipStreamSynchronize(ob::stream());
+  for (void* p : {(void*)c->rho, (void*)c->phi, (void*)c->spec, (void*)c->ghat,
+                  (void*)c->partials, (void*)c->ext_dev, (void*)c->tw, (void*)c->geo})
+    if (p) (void)hipFree(p);
+  (void)hipHostFree(c->ext_host);
+  (void)hipHostFree(c->geo_host);
+  c->work.release();
+  delete c;
+  OB_REQUIRE(err.empty(), err);
+  OB_CATCH
+}
+
+// trackBunch(bunch, length[, boundary]) (:66-113); bnd NULL = no boundary.
+OB_API int ob_slice2d_track(ob_slice2d* c, ob_bunch* b, double length, ob_boundary2d* bnd) {
+  OB_TRY OB_REQUIRE(c != nullptr && b != nullptr, "null handle");
+  ob::slice_track(c, b, length, bnd);
+  OB_CATCH
+}
+
+// getRhoGrid / getPhiGrid: which 0 rho, 1 phi as the reference's Arr3D[iz][ix][iy];
+// lims[6] = x, y, z limits of the last step.
+OB_API int ob_slice2d_get_grid(ob_slice2d* c, int which, double* out, double* lims) {
+  OB_TRY OB_REQUIRE(c != nullptr, "null handle");
+  OB_REQUIRE(which == 0 || which == 1, "grid selector must be 0 (rho) or 1 (phi)");
+  const size_t ng = (size_t)c->nx * c->ny * c->nz;
+  if (out) {
+    std::vector<double> tmp(ng);
+    OB_HIP(hipMemcpyAsync(tmp.data(), which == 0 ? c->rho : c->phi, sizeof(double) * ng,
+                          hipMemcpyDeviceToHost, ob::stream()));
+    OB_HIP(hipStreamSynchronize(ob::stream()));
+    for (int ix = 0; ix < c->nx; ix++)
+      for (int iy = 0; iy < c->ny; iy++)
+        for (int iz = 0; iz < c->nz; iz++)
+          out[((size_t)iz * c->nx + ix) * c->ny + iy] = tmp[((size_t)ix * c->ny + iy) * c->nz + iz];
+  }
+  if (lims)
+    for (int k = 0; k < 6; k++) lims[k] = c->g[k];
+  OB_CATCH
+}
+
+// out[6] = xy_ratio, solver box (xMin, xMax, yMin, yMax), Green rebuilds.
+OB_API int ob_slice2d_get_state(ob_slice2d* c, double* out) {
+  OB_TRY OB_REQUIRE(c != nullptr && out != nullptr, "null handle");
+  out[0] = c->xy_ratio;
+  for (int k = 0; k < 4; k++) out[1 + k] = c->sol[k];
+  out[5] = c->rebuilds;
   OB_CATCH
 }

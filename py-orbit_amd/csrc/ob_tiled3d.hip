@@ -227,10 +227,13 @@ __global__ __launch_bounds__(kThreads) void tile_deposit(const double* __restric
 
 // SpaceChargeCalc3D kick of particle i, gathering phi through view v (which must hold the
 // 27 nodes) or the whole grid.
+// BND: particles outside the boundary in k are not kicked.
+template <bool BND>
 __device__ __forceinline__ void kick_particle(double* __restrict__ cur, int64_t cap, int64_t i,
                                               const TileGeom& g, const Kick3D& k,
                                               const double* lds, const TileBox* box) {
   const double x = cur[i], y = cur[2 * cap + i], z = cur[4 * cap + i];
+  if (BND && !boundary_inside(k.bnd_type, k.bnd, x, y)) return;
   const double zr = (z - k.zc) * k.gamma + k.zc;
   int iX, iY, iZ;
   double xF, yF, zF;
@@ -254,6 +257,7 @@ __device__ __forceinline__ void kick_particle(double* __restrict__ cur, int64_t 
 // Blocks 0..nitems-1: work items; block nitems: the particles outside the grid (last in the
 // sorted order, from offsets[ntiles]), kicked from the whole grid like the reference.
 // At least 4 waves per SIMD: caps the kick at 128 VGPRs (no spills) instead of 155.
+template <bool BND>
 __global__ __launch_bounds__(kThreads, 4) void tile_kick(double* __restrict__ cur, int64_t cap,
                                                       int64_t n, TileGeom g, Kick3D k,
                                                       const int4* __restrict__ items,
@@ -263,7 +267,7 @@ __global__ __launch_bounds__(kThreads, 4) void tile_kick(double* __restrict__ cu
   if (blockIdx.x > ni) return;
   if (blockIdx.x == ni) {
     for (int64_t i = (int64_t)offsets[g.ntiles] + threadIdx.x; i < n; i += blockDim.x)
-      kick_particle(cur, cap, i, g, k, nullptr, nullptr);
+      kick_particle<BND>(cur, cap, i, g, k, nullptr, nullptr);
     return;
   }
   extern __shared__ double sphi[];
@@ -278,7 +282,8 @@ __global__ __launch_bounds__(kThreads, 4) void tile_kick(double* __restrict__ cu
     sphi[c] = in ? k.phi[gx * gsx + (int64_t)gy * g.nz + gz] : 0.0;
   }
   __syncthreads();
-  for (int i = it.y + threadIdx.x; i < it.z; i += blockDim.x) kick_particle(cur, cap, i, g, k, sphi, &box);
+  for (int i = it.y + threadIdx.x; i < it.z; i += blockDim.x)
+    kick_particle<BND>(cur, cap, i, g, k, sphi, &box);
 }
 
 }  // namespace
@@ -358,7 +363,9 @@ void sort_and_deposit_3d(ob_bunch* b, const TileGeom& g, double macrosize, doubl
                                hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024));
     OB_HIP(hipFuncSetAttribute((const void*)tile_scatter,
                                hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024));
-    OB_HIP(hipFuncSetAttribute((const void*)tile_kick,
+    OB_HIP(hipFuncSetAttribute((const void*)tile_kick<false>,
+                               hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024));
+    OB_HIP(hipFuncSetAttribute((const void*)tile_kick<true>,
                                hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024));
     attr_done = true;
   }
@@ -409,9 +416,14 @@ void tile_kick_3d(ob_bunch* b, const TileGeom& g, const Kick3D& k, TileWork& w) 
   const int64_t max_items = n / kChunk + g.ntiles + 1;
   {
     ProfScope ps("tile_kick");
-    hipLaunchKernelGGL(tile_kick, dim3((unsigned)(max_items + 1)), dim3(kThreads),
-                       tile_lds_bytes(g), stream(), v.c[0], (int64_t)(v.c[1] - v.c[0]), n, g, k,
-                       w.items, w.nitems, w.offsets);
+    if (k.bnd_type)
+      hipLaunchKernelGGL(tile_kick<true>, dim3((unsigned)(max_items + 1)), dim3(kThreads),
+                         tile_lds_bytes(g), stream(), v.c[0], (int64_t)(v.c[1] - v.c[0]), n, g, k,
+                         w.items, w.nitems, w.offsets);
+    else
+      hipLaunchKernelGGL(tile_kick<false>, dim3((unsigned)(max_items + 1)), dim3(kThreads),
+                         tile_lds_bytes(g), stream(), v.c[0], (int64_t)(v.c[1] - v.c[0]), n, g, k,
+                         w.items, w.nitems, w.offsets);
   }
   OB_HIP(hipGetLastError());
   bunch_device_written(b);

@@ -44,6 +44,10 @@ void sort_and_deposit_3d(ob_bunch* b, const TileGeom& g, double macrosize, doubl
 struct Kick3D {
   const double* phi;
   double rzMin, rdz, zc, gamma, trans_factor, long_factor;
+  // SpaceChargeCalcSliceBySlice2D with a boundary: particles outside it are not kicked
+  // (SpaceChargeCalcSliceBySlice2D.cc:104); type 0 = no boundary (SC2DState::bnd_type).
+  int bnd_type;
+  double bnd[6];
 };
 void tile_kick_3d(ob_bunch* b, const TileGeom& g, const Kick3D& k, TileWork& w);
 

@@ -433,6 +433,52 @@ class SpaceChargeCalc2p5D:
         return float(o[0]), tuple(float(v) for v in o[1:5]), float(o[5]), int(o[6])
 
 
+class SpaceChargeCalcSliceBySlice2D:
+    """SpaceChargeCalcSliceBySlice2D(nx, ny, nz[, xy_ratio])
+    (SpaceChargeCalcSliceBySlice2D.cc:15-32, wrap_spacechargecalc_slicebyslice_2D.cc:41)."""
+
+    def __init__(self, xSize, ySize, zSize, xy_ratio=1.0):
+        self._n = (int(xSize), int(ySize), int(zSize))
+        h = C.c_void_p()
+        N.call("ob_slice2d_create", self._n[0], self._n[1], self._n[2], float(xy_ratio),
+               C.byref(h))
+        self._h = h
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h and N._lib is not None:
+            N._lib.ob_slice2d_destroy(h)
+            self._h = None
+
+    def trackBunch(self, bunch, length, boundary=None):
+        """trackBunch(bunch, length[, boundary]) (:66-113)."""
+        if boundary is not None and not isinstance(boundary, Boundary2D):
+            raise TypeError("SpaceChargeCalcSliceBySlice2D.trackBunch: boundary must be a "
+                            "Boundary2D")
+        bunch._push_attrs()
+        N.call("ob_slice2d_track", self._h, bunch._h, float(length),
+               boundary._h if boundary is not None else None)
+
+    def _grid(self, which):
+        nx, ny, nz = self._n
+        out = np.zeros(nx * ny * nz)
+        lims = np.zeros(6)
+        N.call("ob_slice2d_get_grid", self._h, which, N.dptr(out), N.dptr(lims))
+        return Grid3D(out.reshape(nz, nx, ny), lims)
+
+    def getRhoGrid(self):
+        return self._grid(0)
+
+    def getPhiGrid(self):
+        return self._grid(1)
+
+    def getState(self):
+        """(xy_ratio, solver box (xMin, xMax, yMin, yMax), Green function rebuilds)."""
+        o = np.zeros(6)
+        N.call("ob_slice2d_get_state", self._h, N.dptr(o))
+        return float(o[0]), tuple(float(v) for v in o[1:5]), int(o[5])
+
+
 class SpaceChargeCalc3D:
     """SpaceChargeCalc3D(nx, ny, nz) (SpaceChargeCalc3D.cc:32-58)."""
 

@@ -54,7 +54,7 @@ def replay_oracle(calls, obunch, calc):
     """Apply a recorded call list to an oracle bunch."""
     for c in calls:
         name, args = c[0], c[1:]
-        if name in ("sc2p5drb", "sc2p5d", "sc3d"):
+        if name in ("sc2p5drb", "sc2p5d", "sc3d", "slice2d"):
             calc.trackBunch(obunch, *args)
         else:
             if name in ("quad1", "kick", "multp", "multpfringeIN", "multpfringeOUT", "soln",

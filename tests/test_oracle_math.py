@@ -450,6 +450,45 @@ def test_sc2p5d_centred_beam_in_round_pipe_keeps_its_field():
     assert np.median((a.coords[1] - c[1])[inside] / (K * x[inside])) == pytest.approx(1.0, abs=0.04)
 
 
+def test_slice2d_kick_uniform_disk_and_local_line_density():
+    """SpaceChargeCalcSliceBySlice2D: a uniform round beam whose line density rises
+    linearly in z is kicked like the uniform disk of the local density,
+    2 r0 lambda(z) L x / (beta^2 gamma^3 R^2); dE is left alone."""
+    n, R, L = 600000, 5e-3, 0.5
+    x, y = _uniform_disk(n, R, 11)
+    r = np.random.default_rng(12)
+    s_ = 0.5  # density proportional to 1 + s z on [-1, 1]
+    u = r.random(n)
+    z = (-1 + np.sqrt((1 - s_) ** 2 + 4 * s_ * u)) / s_
+    c = np.zeros((6, n))
+    c[0], c[2], c[4] = x, y, z
+    c[5] = 1e-4
+    Q = 1e13
+    ob = O.Bunch(c, macrosize=Q / n, kin_energy=1.0)
+    calc = O.SliceBySlice2D(64, 64, 32)
+    calc.trackBunch(ob, L)
+    sy = ob.sync
+    lam = Q / 2.0 * (1 + s_ * z)
+    K = 2 * 1.534698e-18 * lam * L / (sy["beta"] ** 2 * sy["gamma"] ** 3 * R ** 2)
+    core = (np.hypot(x, y) < 0.7 * R) & (np.abs(z) < 0.8)
+    dxp = ob.coords[1] - c[1]
+    assert np.median(dxp[core] / (K[core] * x[core])) == pytest.approx(1.0, abs=0.04)
+    assert np.all(ob.coords[5] == c[5]) and np.all(ob.coords[4] == c[4])
+
+
+def test_slice2d_ratio_rule_matches_2p5d():
+    """The slice calculator sizes its transverse grid like SpaceChargeCalc2p5D: the same
+    beams give the same box, ratio and rebuild count."""
+    a, b = O.SliceBySlice2D(32, 32, 8, 1.0), O.SC2p5D(32, 32, 8, 1.0)
+    for k, (ax, ay) in enumerate([(2e-3, 1.8e-3), (5e-3, 1.5e-3), (5e-3, 1.5e-3), (2e-3, 2e-3)]):
+        c = _beam(5000, ax, ay, 50 + k)
+        a.trackBunch(O.Bunch(c, macrosize=1e9, kin_energy=1.0), 0.1)
+        b.trackBunch(O.Bunch(c, macrosize=1e9, kin_energy=1.0), 0.1)
+        sa, sb = a.state(), b.state()
+        assert sa[0] == sb[0] and sa[1] == sb[1] and sa[2] == sb[3]
+        assert tuple(a.grid(0)[1][:4]) == tuple(b.grid(0)[1])
+
+
 def _beam(n, ax, ay, seed):
     r = np.random.default_rng(seed)
     c = np.zeros((6, n))
