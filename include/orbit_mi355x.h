@@ -92,6 +92,23 @@ int ob_bunch_get_macrosizes(ob_bunch* b, int64_t n, double* m);
  * w = the macrosize attribute or 1.  Dead particles are counted until compress(), as the
  * reference counts them until its own compress() call. */
 int ob_bunch_moments(ob_bunch* b, double* out);
+/* BunchExtremaCalculator::getExtremaXYZ and getExtremaXpYpdE
+ * (src/utils/bunch/BunchExtremaCalculator.cc:37-120, wrap_bunch_extrema_calculator.cc) in
+ * one device pass: out[12] = (min, max) of x, xp, y, yp, z, dE; DBL_MAX, -DBL_MAX for an
+ * empty bunch. */
+int ob_bunch_extrema(ob_bunch* b, double* out);
+/* Grid1D::binBunch(bunch, axis) and binBunchByParticle(bunch, axis)
+ * (src/spacecharge/Grid1D.cc:241-275, 335-352; wrap_grid1D.cc:252-325) on the device, for a
+ * grid of nz cells over [zMin, zMax]: the binned values are added to the host array
+ * arr[nz].  by_particle != 0 weighs every particle 1. */
+int ob_grid1d_bin_bunch(ob_bunch* b, int axis, int by_particle, int nz, double zMin,
+                        double zMax, double* arr);
+/* The particle sums of BunchTwissAnalysis::computeBunchMoments
+ * (BunchTwissAnalysis.cc:145-337), 1 <= order <= 12.  prm[8] = <x>, <y>, D_x,
+ * E_kin + mass, beta^2, xb, yb, dispersion flag; out[order (order + 3) / 2 + 2] = the sums
+ * for j = 0..order-1, i = 0..order-j, then the stray [2][order] entry of a bunch without
+ * macrosizes (order >= 2), then the total weight. */
+int ob_bunch_xy_moments(ob_bunch* b, int order, const double* prm, double* out);
 /* Device pointers of the SoA coordinate arrays (x, xp, y, yp, z, dE), valid until the
  * next size change; pending maps are applied first. */
 int ob_bunch_device_coords(ob_bunch* b, double** ptrs6);

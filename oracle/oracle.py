@@ -63,6 +63,9 @@ def lib():
             "orc_multpfringeOUT": (None, [bp, i, d, i]), "orc_quad1": (None, [bp, d, d]),
             "orc_quad2": (None, [bp, d]), "orc_quadfringeIN": (None, [bp, d]),
             "orc_quadfringeOUT": (None, [bp, d]), "orc_extrema_xyz": (None, [bp, _D]),
+            "orc_extrema6": (None, [bp, _D]),
+            "orc_flat_bunch_bin_axis": (None, [bp, i, d, d, i, i, _D]),
+            "orc_compute_bunch_moments": (None, [bp, i, i, d, d, d, d, d, d, d, _D]),
             "orc_wedgerotate": (None, [bp, d, i]), "orc_wedgedrift": (None, [bp, d, i]),
             "orc_wedgebend": (None, [bp, d, i, d, i]),
             "orc_wedgebendCF": (None, [bp, d, i, d, i, _I, _D, _I, i]),
@@ -171,6 +174,27 @@ class Bunch:
         out = np.zeros(6)
         lib().orc_extrema_xyz(self.ref, _dp(out))
         return out
+
+    def extrema6(self):
+        """(min, max) of x, xp, y, yp, z, dE (BunchExtremaCalculator.cc:37-120)."""
+        out = np.zeros(12)
+        lib().orc_extrema6(self.ref, _dp(out))
+        return out
+
+    def bin_axis(self, nz, zmin, zmax, axis, by_particle=False):
+        """Grid1D(nz, zmin, zmax).binBunch(bunch, axis) (Grid1D.cc:241-275, 335-352)."""
+        out = np.zeros(nz)
+        lib().orc_flat_bunch_bin_axis(self.ref, nz, zmin, zmax, axis, int(by_particle),
+                                      _dp(out))
+        return out
+
+    def compute_moments(self, order, dispflag, xAvg, yAvg, D, Em, b2, xb, yb):
+        """computeBunchMoments' momentXY[i][j] (BunchTwissAnalysis.cc:145-337) for the
+        given analyzeBunch results."""
+        out = np.zeros((order + 1) * (order + 1))
+        lib().orc_compute_bunch_moments(self.ref, order, dispflag, xAvg, yAvg, D, Em, b2, xb,
+                                        yb, _dp(out))
+        return out.reshape(order + 1, order + 1)
 
     def emittances(self):
         out = np.zeros(3)

@@ -548,6 +548,69 @@ void orc_extrema_xyz(const orc_bunch* b, double o[6]) {
   memcpy(o, l, sizeof(l));
 }
 
+/* getExtremaXpYpdE (BunchExtremaCalculator.cc:85-120): the same scan over xp, yp, dE.
+   o[12] = (min, max) of x, xp, y, yp, z, dE. */
+void orc_extrema6(const orc_bunch* b, double o[12]) {
+  double xyz[6];
+  orc_extrema_xyz(b, xyz);
+  double l[6] = {DBL_MAX, -DBL_MAX, DBL_MAX, -DBL_MAX, DBL_MAX, -DBL_MAX};
+  for (int64_t i = 0; i < b->n; i++) {
+    if (b->xp[i] < l[0]) l[0] = b->xp[i];
+    if (b->xp[i] > l[1]) l[1] = b->xp[i];
+    if (b->yp[i] < l[2]) l[2] = b->yp[i];
+    if (b->yp[i] > l[3]) l[3] = b->yp[i];
+    if (b->dE[i] < l[4]) l[4] = b->dE[i];
+    if (b->dE[i] > l[5]) l[5] = b->dE[i];
+  }
+  o[0] = xyz[0]; o[1] = xyz[1]; o[2] = l[0]; o[3] = l[1];
+  o[4] = xyz[2]; o[5] = xyz[3]; o[6] = l[2]; o[7] = l[3];
+  o[8] = xyz[4]; o[9] = xyz[5]; o[10] = l[4]; o[11] = l[5];
+}
+
+/* BunchTwissAnalysis::computeBunchMoments (BunchTwissAnalysis.cc:145-337), after the
+   caller's analyzeBunch: xAvg = getAverage(0) (the dispersion branch at :166 is dead, its
+   test reads a local that is still 0), yAvg = getAverage(2), D = getDispersion(0),
+   Em = E_kin + mass, b2 = beta^2, xb and yb the normalising terms (:238-245).
+   mom[(order+1)^2], row-major [i][j]. */
+void orc_compute_bunch_moments(const orc_bunch* b, int order, int dispflag, double xAvg,
+                               double yAvg, double D, double Em, double b2, double xb,
+                               double yb, double* mom) {
+  const int o1 = order + 1;
+  double momX[64], momY[64];
+  for (int k = 0; k < o1 * o1; k++) mom[k] = 0.;
+  for (int n = 0; n < o1; n++) momX[n] = momY[n] = 0.;
+  momX[0] = 1.0;
+  momY[0] = 1.0;
+  double total = 0.;
+  double dispterm = 0.;
+  int i = 0, j = 0;
+  for (int64_t ip = 0; ip < b->n; ip++) {
+    const double m = b->msize ? b->msize[ip] : 1.0;
+    if (b->msize) total += m;
+    if (dispflag > 0) dispterm = D * b->dE[ip] / Em / b2;
+    for (i = 0; i < order; i++)
+      momX[i + 1] = b->msize ? momX[i] * m * ((b->x[ip] - dispterm) - xAvg)
+                             : momX[i] * ((b->x[ip] - dispterm) - xAvg);
+    for (i = 0; i < order; i++)
+      momY[i + 1] = b->msize ? momY[i] * m * (b->y[ip] - yAvg) : momY[i] * (b->y[ip] - yAvg);
+    for (j = 0; j < order; j++)
+      for (i = 0; i < o1 - j; i++) {
+        mom[i * o1 + j] += momX[i] / pow(xb, (double)i) * momY[j] / pow(yb, (double)j);
+        if (b->msize) mom[i * o1 + j] += momX[i] * momY[j];
+      }
+    /* without macrosizes the second statement sits after the loops (:276-280): i = 2,
+       j = order; for order 1 the reference reads and writes outside its arrays */
+    if (!b->msize && order >= 2) mom[2 * o1 + order] += momX[2] * momY[order];
+  }
+  if (!b->msize) total += (double)b->n * 1.0;
+  if (fabs(total) > 0.)
+    for (i = 0; i < o1; i++)
+      for (j = 0; j < o1 - i; j++) mom[i * o1 + j] /= total;
+  mom[0] = 1.;
+  mom[1 * o1 + 0] = xAvg;
+  mom[0 * o1 + 1] = yAvg;
+}
+
 /* ------------------------------------------------------------------------ */
 /* Grid2D: src/spacecharge/Grid2D.cc */
 void orc_grid2d_set_x(orc_grid2d* g, double a, double b) { /* :409-413 */
@@ -732,6 +795,14 @@ void orc_grid1d_bin_bunch(orc_grid1d* g, const orc_bunch* b) {
   for (int t = 0; t < nt; t++)
     for (int k = 0; k < g->nz; k++) g->arr[k] += priv[(size_t)g->nz * t + k];
   free(priv);
+}
+/* binBunch(bunch, axis) / binBunchByParticle(bunch, axis) (:241-275, 335-352) */
+void orc_grid1d_bin_axis(orc_grid1d* g, const orc_bunch* b, int axis, int by_particle) {
+  const double* c[6] = {b->x, b->xp, b->y, b->yp, b->z, b->dE};
+  for (int64_t i = 0; i < b->n; i++) {
+    const double v = by_particle ? 1.0 : (b->msize ? b->msize[i] : b->s.macrosize);
+    orc_grid1d_bin_value(g, v, c[axis][i]);
+  }
 }
 /* :191-210 (quirk Q7: zero outside [zMin,zMax]) */
 double orc_grid1d_value(const orc_grid1d* g, double z) {
@@ -2198,6 +2269,14 @@ void orc_flat_grid1d_bin(int nz, double zmin, double zmax, long n, const double*
   orc_grid1d_set_z(&g, zmin, zmax);
   memset(out, 0, sizeof(double) * nz);
   for (long i = 0; i < n; i++) orc_grid1d_bin_value(&g, m[i], z[i]);
+}
+void orc_flat_bunch_bin_axis(const orc_bunch* b, int nz, double zmin, double zmax, int axis,
+                             int by_particle, double* out) {
+  orc_grid1d g;
+  g.nz = nz; g.arr = out;
+  orc_grid1d_set_z(&g, zmin, zmax);
+  memset(out, 0, sizeof(double) * nz);
+  orc_grid1d_bin_axis(&g, b, axis, by_particle);
 }
 void orc_flat_grid1d_value(int nz, double zmin, double zmax, const double* arr, long n,
                            const double* z, double* val, double* grad) {

@@ -73,6 +73,11 @@ void orc_ringrf(orc_bunch* b, double ring_length, int harmonic_numb, double volt
 
 /* ---- extrema (src/utils/bunch/BunchExtremaCalculator.cc:37-83) ---- */
 void orc_extrema_xyz(const orc_bunch* b, double out6[6]);
+void orc_extrema6(const orc_bunch* b, double out12[12]);
+/* ---- computeBunchMoments (src/orbit/BunchDiagnostics/BunchTwissAnalysis.cc:145-337) ---- */
+void orc_compute_bunch_moments(const orc_bunch* b, int order, int dispflag, double xAvg,
+                               double yAvg, double D, double Em, double b2, double xb,
+                               double yb, double* mom);
 
 /* ---- Grid2D (src/spacecharge/Grid2D.cc) ----  arr is row-major [nx][ny] */
 typedef struct {
@@ -99,6 +104,7 @@ void orc_grid1d_set_z(orc_grid1d* g, double zMin, double zMax);
 void orc_grid1d_ind_wz(const orc_grid1d* g, double z, int* i0, int* ip, double* w0, double* wp);
 void orc_grid1d_bin_value(orc_grid1d* g, double value, double z);
 void orc_grid1d_bin_bunch(orc_grid1d* g, const orc_bunch* b);
+void orc_grid1d_bin_axis(orc_grid1d* g, const orc_bunch* b, int axis, int by_particle);
 double orc_grid1d_value(const orc_grid1d* g, double z);
 void orc_grid1d_gradient(const orc_grid1d* g, double z, double* ez);
 
@@ -220,6 +226,8 @@ void orc_flat_grid2d_indfrac(int nx, int ny, const double* l, long n, const doub
                              const double* y, int* ix, double* fx, int* iy, double* fy);
 void orc_flat_grid2d_grad(int nx, int ny, const double* l, const double* arr, long n,
                           const double* x, const double* y, double* ex, double* ey, double* val);
+void orc_flat_bunch_bin_axis(const orc_bunch* b, int nz, double zmin, double zmax, int axis,
+                             int by_particle, double* out);
 void orc_flat_grid1d_bin(int nz, double zmin, double zmax, long n, const double* z, const double* m,
                          double* out);
 void orc_flat_grid1d_value(int nz, double zmin, double zmax, const double* arr, long n,

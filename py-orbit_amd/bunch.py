@@ -180,6 +180,11 @@ class Bunch:
         N.call("ob_bunch_get_size", self._h, C.byref(n))
         return int(n.value)
 
+    def getMPIComm(self):
+        """The bunch's communicator (Bunch.cc getMPI_Comm_Local): one process holds the whole
+        bunch, so there is none."""
+        return None
+
     def getSizeGlobal(self):
         """One process holds the whole bunch: equals getSize() (Bunch.cc:850-861)."""
         return self.getSize()
@@ -523,6 +528,67 @@ class BunchTwissAnalysis:
         sp = bunch.getSyncParticle()
         self._gb = (sp.gamma(), sp.beta())
         self._p = sp.momentum()
+        self._ek_m = (sp.kinEnergy(), sp.mass())
+
+    def computeBunchMoments(self, bunch, order, dispersionflag, emitnormflag):
+        """computeBunchMoments(bunch, order, dispterm, emitnormterm)
+        (BunchTwissAnalysis.cc:145-337, wrap_bunch_twiss_analysis.cc:56-75).
+
+        Runs analyzeBunch, then sums on the device, for j < order and i <= order - j,
+        (x - D_x dE / (E_kin + M) / beta^2 - <x>)^i (y - <y>)^j / (xb^i yb^j) per particle
+        (the dispersion term only for dispersionflag > 0; xb, yb = sqrt(beta) or, for
+        emitnormflag > 0, sqrt(beta emittance)).  The reference's own arithmetic is kept:
+        with the macrosize attribute each factor carries the particle's macrosize and every
+        term is added a second time without the xb, yb normalisation; without it entry
+        [2][order] collects (x - <x>)^2 (y - <y>)^order undivided.  Entries with
+        i + j <= order are divided by the total macrosize; [0][0] = 1, [1][0] = <x>,
+        [0][1] = <y>, and (y - <y>)^order stays 0.  Orders 1 to 12 (order 0 makes the
+        reference write outside its array)."""
+        order = int(order)
+        if order < 1 or order > 12:
+            raise N.OrbitError("BunchTwissAnalysis.computeBunchMoments: order must be "
+                               "between 1 and 12")
+        self.analyzeBunch(bunch)
+        x_avg, y_avg = self.getAverage(0), self.getAverage(2)
+        with np.errstate(invalid="ignore"):
+            if emitnormflag > 0:
+                xb = float(np.sqrt(np.float64(self.getBeta(0) * self.getEmittance(0))))
+                yb = float(np.sqrt(np.float64(self.getBeta(1) * self.getEmittance(1))))
+            else:
+                xb = float(np.sqrt(np.float64(self.getBeta(0))))
+                yb = float(np.sqrt(np.float64(self.getBeta(1))))
+        beta = self._gb[1]
+        prm = np.array([x_avg, y_avg, self.getDispersion(0), self._ek_m[0] + self._ek_m[1],
+                        beta * beta, xb, yb, 1.0 if dispersionflag > 0 else 0.0])
+        out = np.zeros(order * (order + 3) // 2 + 2)
+        N.call("ob_bunch_xy_moments", bunch._h, order, N.dptr(prm), N.dptr(out))
+        mom = np.zeros((order + 1, order + 1))
+        k = 0
+        for j in range(order):
+            for i in range(order + 1 - j):
+                mom[i, j] = out[k]
+                k += 1
+        if order >= 2:
+            mom[2, order] = out[k]
+        total = out[k + 1]
+        if abs(total) > 0.0:
+            for i in range(order + 1):
+                mom[i, :order + 1 - i] /= total
+        mom[0, 0] = 1.0
+        mom[1, 0] = x_avg
+        mom[0, 1] = y_avg
+        self._mom = mom
+
+    def getBunchMoment(self, i, j):
+        """momentXY[i][j] of the last computeBunchMoments (BunchTwissAnalysis.cc:350-353);
+        0 outside 0..order."""
+        mom = getattr(self, "_mom", None)
+        if mom is None:
+            return 0.0
+        o = mom.shape[0] - 1
+        if i < 0 or i > o or j < 0 or j > o:
+            return 0.0
+        return float(mom[i, j])
 
     def getGlobalCount(self):
         return self._count

@@ -1,12 +1,19 @@
-// ob_diag.hip -- bunch moments on the device for BunchTwissAnalysis
-// (src/orbit/BunchDiagnostics/BunchTwissAnalysis.cc:49-142).
+// ob_diag.hip -- bunch diagnostics on the device: the moments of BunchTwissAnalysis
+// (src/orbit/BunchDiagnostics/BunchTwissAnalysis.cc:49-142), its x-y power moments
+// (computeBunchMoments, :145-337), the six coordinate extrema of BunchExtremaCalculator
+// (src/utils/bunch/BunchExtremaCalculator.cc:37-120) and Grid1D::binBunch along any axis
+// (src/spacecharge/Grid1D.cc:241-275, 328-352), which the profile diagnostics use.
 //
 // analyzeBunch sums, over the particles, the weight (macrosize attribute or 1), the
 // weighted coordinates and the weighted products of coordinate pairs.  Here each
 // workgroup sums its particles in registers and writes its 28 partial sums; one workgroup
 // adds the partials in a fixed order.  The host derives averages, correlations and
 // emittances exactly as the reference does, without downloading the 6 x N coordinates.
+#include <algorithm>
+#include <cmath>
+
 #include "ob_internal.hpp"
+#include "ob_sc_device.hpp"
 
 namespace ob {
 namespace {
@@ -46,6 +53,7 @@ __global__ __launch_bounds__(kThreads) void moment_partials(DeviceBunchView v,
 #pragma unroll
       for (int b = a; b < 6; b++) s[p++] += w * c[a] * c[b];
   }
+#pragma unroll
   for (int k = 0; k < kSums; k++) {
     const double t = block_sum(s[k], sh);
     if (threadIdx.x == 0) part[(size_t)blockIdx.x * kSums + k] = t;
@@ -58,6 +66,181 @@ __global__ __launch_bounds__(64) void moment_reduce(const double* __restrict__ p
   if (k >= kSums) return;
   double s = 0.0;
   for (int b = 0; b < nblk; b++) s += part[(size_t)b * kSums + k];
+  out[k] = s;
+}
+
+// ---- extrema of the six coordinates -------------------------------------------------
+// Per workgroup: min and max of x, xp, y, yp, z, dE (12 values, min/max interleaved);
+// comparisons as in the reference (a value replaces the extremum only if strictly beyond).
+__device__ __forceinline__ double lo(double a, double b) { return b < a ? b : a; }
+__device__ __forceinline__ double hi(double a, double b) { return b > a ? b : a; }
+
+__global__ __launch_bounds__(kThreads) void extrema6_partials(DeviceBunchView v,
+                                                              double* __restrict__ part) {
+  __shared__ double sh[kThreads / 64][12];
+  double e[12];
+#pragma unroll
+  for (int k = 0; k < 6; k++) {
+    e[2 * k] = DBL_MAX;
+    e[2 * k + 1] = -DBL_MAX;
+  }
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < v.n;
+       i += (int64_t)gridDim.x * blockDim.x) {
+#pragma unroll
+    for (int k = 0; k < 6; k++) {
+      const double c = v.c[k][i];
+      e[2 * k] = lo(e[2 * k], c);
+      e[2 * k + 1] = hi(e[2 * k + 1], c);
+    }
+  }
+#pragma unroll
+  for (int k = 0; k < 6; k++)
+    for (int o = 32; o > 0; o >>= 1) {
+      e[2 * k] = lo(e[2 * k], __shfl_xor(e[2 * k], o, 64));
+      e[2 * k + 1] = hi(e[2 * k + 1], __shfl_xor(e[2 * k + 1], o, 64));
+    }
+  if ((threadIdx.x & 63) == 0)
+    for (int k = 0; k < 12; k++) sh[threadIdx.x >> 6][k] = e[k];
+  __syncthreads();
+  if (threadIdx.x < 12) {
+    const int k = threadIdx.x;
+    double r = sh[0][k];
+    for (int w = 1; w < kThreads / 64; w++) r = (k & 1) ? hi(r, sh[w][k]) : lo(r, sh[w][k]);
+    part[(size_t)blockIdx.x * 12 + k] = r;
+  }
+}
+
+__global__ __launch_bounds__(64) void extrema6_reduce(const double* __restrict__ part, int nblk,
+                                                      double* __restrict__ out) {
+  const int k = threadIdx.x;
+  if (k >= 12) return;
+  double r = (k & 1) ? -DBL_MAX : DBL_MAX;
+  for (int b = 0; b < nblk; b++) r = (k & 1) ? hi(r, part[(size_t)b * 12 + k]) : lo(r, part[(size_t)b * 12 + k]);
+  out[k] = r;
+}
+
+// ---- Grid1D::binBunch along one coordinate --------------------------------------------
+// Linear (cloud-in-cell) weights of Grid1D::binValue (Grid1D.cc:422-440) with the grid's
+// periodic wrap at both ends.  Grids up to kBinLds bins accumulate a private histogram per
+// workgroup in LDS and write it out; a second pass adds the workgroups' histograms bin by
+// bin in a fixed order.  Larger grids add into one global histogram with atomics.
+constexpr int kBinLds = 4096;  // 32 KiB of FP64 bins per workgroup
+
+struct Bin1D {
+  int axis, by_particle, nz;
+  double value;  // bunch macrosize when the bunch has no macrosize attribute
+  double zMin, zMax, dz;
+};
+
+template <bool LDS>
+__global__ __launch_bounds__(kThreads) void bin1d_partials(DeviceBunchView v, Bin1D g,
+                                                           double* __restrict__ part) {
+  extern __shared__ double hist[];
+  double* h = LDS ? hist : part;
+  if (LDS) {
+    for (int k = threadIdx.x; k < g.nz; k += blockDim.x) hist[k] = 0.0;
+    __syncthreads();
+  }
+  const double* __restrict__ zc = v.c[g.axis];
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < v.n;
+       i += (int64_t)gridDim.x * blockDim.x) {
+    const double z = zc[i];
+    if (z < g.zMin || z > g.zMax) continue;
+    const double val = g.by_particle ? 1.0 : (v.msize ? v.msize[i] : g.value);
+    int i0, ip;
+    double w0, wp;
+    grid1d_ind_wz(z, g.zMin, g.dz, g.nz, i0, ip, w0, wp);
+    if (g.nz > 1) {
+      atomicAdd(&h[i0], w0 * val);
+      atomicAdd(&h[ip], wp * val);
+    } else {
+      atomicAdd(&h[i0], val);
+    }
+  }
+  if (LDS) {
+    __syncthreads();
+    for (int k = threadIdx.x; k < g.nz; k += blockDim.x)
+      part[(size_t)blockIdx.x * g.nz + k] = hist[k];
+  }
+}
+
+__global__ __launch_bounds__(kThreads) void bin1d_reduce(const double* __restrict__ part,
+                                                         int nblk, int nz,
+                                                         double* __restrict__ out) {
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= nz) return;
+  double s = 0.0;
+  for (int b = 0; b < nblk; b++) s += part[(size_t)b * nz + k];
+  out[k] = s;
+}
+
+// ---- computeBunchMoments: sums of x-y power products --------------------------------
+// BunchTwissAnalysis::computeBunchMoments (BunchTwissAnalysis.cc:145-337) accumulates, for
+// j < order and i <= order - j, momX[i] / xb^i * momY[j] / yb^j over the particles, with
+// momX[i] = prod (m (x - D dE / (E + M) / beta^2 - <x>)) and momY[j] = prod (m (y - <y>))
+// (m = 1 without the macrosize attribute); with the attribute each term also adds
+// momX[i] momY[j].  Without the attribute the statement after the loops adds
+// momX[2] momY[order] to entry [2][order] once per particle (it sits outside the braces).
+// Each lane keeps its sums in an LDS column (one wave per workgroup); the lanes' sums are
+// added in lane order, then the workgroups' in workgroup order.
+constexpr int kMaxMomOrder = 12;
+constexpr int kMomLanes = 64;
+
+struct XYMom {
+  int order, nterms, disp, stray;  // nterms = order (order + 3) / 2
+  double xAvg, yAvg, D, Em, b2;    // Em = E_kin + mass, b2 = beta^2
+  double px[kMaxMomOrder + 1], py[kMaxMomOrder + 1];  // xb^i, yb^j
+};
+
+__global__ __launch_bounds__(kMomLanes) void xy_moment_partials(DeviceBunchView v, XYMom p,
+                                                                double* __restrict__ part) {
+  extern __shared__ double acc[];  // [nterms + 2][kMomLanes]
+  __shared__ double spx[kMaxMomOrder + 1], spy[kMaxMomOrder + 1];
+  const int t = threadIdx.x;
+  const int nt = p.nterms + 2;
+  for (int k = 0; k < nt; k++) acc[k * kMomLanes + t] = 0.0;
+  if (t <= p.order) {
+    spx[t] = p.px[t];
+    spy[t] = p.py[t];
+  }
+  __syncthreads();
+  const bool ms = v.msize != nullptr;
+  for (int64_t i = (int64_t)blockIdx.x * kMomLanes + t; i < v.n;
+       i += (int64_t)gridDim.x * kMomLanes) {
+    const double m = ms ? v.msize[i] : 1.0;
+    const double dispterm = p.disp ? p.D * v.c[5][i] / p.Em / p.b2 : 0.0;
+    const double dx = (v.c[0][i] - dispterm) - p.xAvg;
+    const double dy = v.c[2][i] - p.yAvg;
+    double mY = 1.0;
+    int k = 0;
+    for (int j = 0; j < p.order; j++) {
+      double mX = 1.0;
+      for (int a = 0; a <= p.order - j; a++, k++) {
+        double s = acc[k * kMomLanes + t];
+        s += mX / spx[a] * mY / spy[j];
+        if (ms) s += mX * mY;
+        acc[k * kMomLanes + t] = s;
+        mX = ms ? mX * m * dx : mX * dx;
+      }
+      mY = ms ? mY * m * dy : mY * dy;
+    }
+    if (p.stray) acc[k * kMomLanes + t] += dx * dx * mY;  // momX[2] momY[order]
+    acc[(k + 1) * kMomLanes + t] += m;
+  }
+  __syncthreads();
+  for (int k = t; k < nt; k += kMomLanes) {
+    double s = 0.0;
+    for (int l = 0; l < kMomLanes; l++) s += acc[k * kMomLanes + l];
+    part[(size_t)blockIdx.x * nt + k] = s;
+  }
+}
+
+__global__ __launch_bounds__(128) void xy_moment_reduce(const double* __restrict__ part, int nblk,
+                                                        int nt, double* __restrict__ out) {
+  const int k = threadIdx.x;
+  if (k >= nt) return;
+  double s = 0.0;
+  for (int b = 0; b < nblk; b++) s += part[(size_t)b * nt + k];
   out[k] = s;
 }
 
@@ -98,6 +281,153 @@ OB_API int ob_bunch_moments(ob_bunch* b, double* out) {
                        buf + (size_t)nblk * ob::kSums);
     OB_HIP(hipGetLastError());
     OB_HIP(hipMemcpyAsync(out, buf + (size_t)nblk * ob::kSums, sizeof(double) * ob::kSums,
+                          hipMemcpyDeviceToHost, s));
+    OB_HIP(hipStreamSynchronize(s));
+  } catch (...) {
+    (void)hipFree(buf);
+    throw;
+  }
+  OB_HIP(hipFree(buf));
+  OB_CATCH
+}
+
+// BunchExtremaCalculator::getExtremaXYZ and getExtremaXpYpdE (BunchExtremaCalculator.cc:37-120)
+// in one pass: out[12] = min, max of x, xp, y, yp, z, dE; DBL_MAX, -DBL_MAX for an empty
+// bunch, as in the reference.
+OB_API int ob_bunch_extrema(ob_bunch* b, double* out) {
+  OB_TRY OB_REQUIRE(b != nullptr && out != nullptr, "null handle");
+  ob::bunch_flush(b, nullptr);
+  const ob::DeviceBunchView v = ob::bunch_device_view(b);
+  hipStream_t s = ob::stream();
+  if (v.n == 0) {
+    for (int k = 0; k < 6; k++) {
+      out[2 * k] = DBL_MAX;
+      out[2 * k + 1] = -DBL_MAX;
+    }
+    return 0;
+  }
+  const int nblk =
+      (int)std::min<int64_t>(ob::kMaxBlocks, (v.n + ob::kThreads - 1) / ob::kThreads);
+  double* buf = nullptr;
+  OB_HIP(hipMalloc((void**)&buf, sizeof(double) * (nblk + 1) * 12));
+  try {
+    {
+      ob::ProfScope ps("extrema6_partials");
+      hipLaunchKernelGGL(ob::extrema6_partials, dim3(nblk), dim3(ob::kThreads), 0, s, v, buf);
+    }
+    hipLaunchKernelGGL(ob::extrema6_reduce, dim3(1), dim3(64), 0, s, buf, nblk,
+                       buf + (size_t)nblk * 12);
+    OB_HIP(hipGetLastError());
+    OB_HIP(hipMemcpyAsync(out, buf + (size_t)nblk * 12, sizeof(double) * 12,
+                          hipMemcpyDeviceToHost, s));
+    OB_HIP(hipStreamSynchronize(s));
+  } catch (...) {
+    (void)hipFree(buf);
+    throw;
+  }
+  OB_HIP(hipFree(buf));
+  OB_CATCH
+}
+
+// Grid1D::binBunch(bunch, axis) / binBunchByParticle(bunch, axis) (Grid1D.cc:241-275,
+// 335-352) on a grid of nz cells over [zMin, zMax]: the binned values are ADDED to the
+// host array arr[nz], as the reference adds into its grid.  by_particle = 1 gives each
+// particle weight 1; otherwise its macrosize attribute or the bunch macrosize.
+OB_API int ob_grid1d_bin_bunch(ob_bunch* b, int axis, int by_particle, int nz, double zMin,
+                               double zMax, double* arr) {
+  OB_TRY OB_REQUIRE(b != nullptr && arr != nullptr, "null handle");
+  OB_REQUIRE(axis >= 0 && axis <= 5, "Grid1D::binBunch: the axis index should be between 0 and 5");
+  OB_REQUIRE(nz >= 1, "Grid1D: the number of bins should be at least 1");
+  ob::bunch_flush(b, nullptr);
+  const ob::DeviceBunchView v = ob::bunch_device_view(b);
+  if (v.n == 0) return 0;
+  const ob::SyncInfo sy = ob::bunch_sync(b);
+  hipStream_t s = ob::stream();
+  ob::Bin1D g;
+  g.axis = axis;
+  g.by_particle = by_particle ? 1 : 0;
+  g.nz = nz;
+  g.value = sy.macrosize;
+  g.zMin = zMin;
+  g.zMax = zMax;
+  g.dz = (zMax - zMin) / nz;  // Grid1D::setGridZ (Grid1D.cc:89-94)
+  const bool lds = nz <= ob::kBinLds;
+  const int64_t want = (v.n + ob::kThreads - 1) / ob::kThreads;
+  const int nblk = (int)std::min<int64_t>(lds ? 512 : ob::kMaxBlocks, want);
+  const size_t part_n = lds ? (size_t)nblk * nz : (size_t)nz;
+  double* buf = nullptr;
+  OB_HIP(hipMalloc((void**)&buf, sizeof(double) * (part_n + nz)));
+  std::vector<double> h((size_t)nz);
+  try {
+    double* res = buf + part_n;
+    if (lds) {
+      ob::ProfScope ps("bin1d_partials");
+      hipLaunchKernelGGL(ob::bin1d_partials<true>, dim3(nblk), dim3(ob::kThreads),
+                         sizeof(double) * nz, s, v, g, buf);
+      OB_HIP(hipGetLastError());
+      hipLaunchKernelGGL(ob::bin1d_reduce, dim3((nz + ob::kThreads - 1) / ob::kThreads),
+                         dim3(ob::kThreads), 0, s, buf, nblk, nz, res);
+    } else {
+      OB_HIP(hipMemsetAsync(buf, 0, sizeof(double) * nz, s));
+      ob::ProfScope ps("bin1d_partials");
+      hipLaunchKernelGGL(ob::bin1d_partials<false>, dim3(nblk), dim3(ob::kThreads), 0, s, v, g,
+                         buf);
+      res = buf;
+    }
+    OB_HIP(hipGetLastError());
+    OB_HIP(hipMemcpyAsync(h.data(), res, sizeof(double) * nz, hipMemcpyDeviceToHost, s));
+    OB_HIP(hipStreamSynchronize(s));
+  } catch (...) {
+    (void)hipFree(buf);
+    throw;
+  }
+  OB_HIP(hipFree(buf));
+  for (int k = 0; k < nz; k++) arr[k] += h[(size_t)k];
+  OB_CATCH
+}
+
+// BunchTwissAnalysis::computeBunchMoments (BunchTwissAnalysis.cc:145-337), the particle
+// sums.  prm[8] = xAvg, yAvg, D_x, E_kin + mass, beta^2, xb, yb, dispersion flag (> 0:
+// subtract D_x dE / (E_kin + mass) / beta^2 from x); xb, yb are sqrt(beta) or
+// sqrt(beta emittance) as the caller chose.  out[order (order + 3) / 2 + 2] = the sums
+// for j = 0 .. order-1, i = 0 .. order-j (j outer), then the [2][order] entry of a bunch
+// without the macrosize attribute (order >= 2; else 0), then the total weight.
+OB_API int ob_bunch_xy_moments(ob_bunch* b, int order, const double* prm, double* out) {
+  OB_TRY OB_REQUIRE(b != nullptr && prm != nullptr && out != nullptr, "null handle");
+  OB_REQUIRE(order >= 1 && order <= ob::kMaxMomOrder,
+             "computeBunchMoments: order must be between 1 and 12");
+  ob::bunch_flush(b, nullptr);
+  const ob::DeviceBunchView v = ob::bunch_device_view(b);
+  hipStream_t s = ob::stream();
+  ob::XYMom p;
+  p.order = order;
+  p.nterms = order * (order + 3) / 2;
+  p.disp = prm[7] > 0 ? 1 : 0;
+  p.stray = (v.msize == nullptr && order >= 2) ? 1 : 0;
+  p.xAvg = prm[0]; p.yAvg = prm[1]; p.D = prm[2]; p.Em = prm[3]; p.b2 = prm[4];
+  for (int k = 0; k <= ob::kMaxMomOrder; k++) {
+    p.px[k] = pow(prm[5], (double)k);
+    p.py[k] = pow(prm[6], (double)k);
+  }
+  const int nt = p.nterms + 2;
+  if (v.n == 0) {
+    for (int k = 0; k < nt; k++) out[k] = 0.0;
+    return 0;
+  }
+  const int nblk = (int)std::min<int64_t>(1024, (v.n + ob::kMomLanes - 1) / ob::kMomLanes);
+  double* buf = nullptr;
+  OB_HIP(hipMalloc((void**)&buf, sizeof(double) * (size_t)(nblk + 1) * nt));
+  try {
+    {
+      ob::ProfScope ps("xy_moment_partials");
+      hipLaunchKernelGGL(ob::xy_moment_partials, dim3(nblk), dim3(ob::kMomLanes),
+                         sizeof(double) * nt * ob::kMomLanes, s, v, p, buf);
+    }
+    OB_HIP(hipGetLastError());
+    hipLaunchKernelGGL(ob::xy_moment_reduce, dim3(1), dim3(128), 0, s, buf, nblk, nt,
+                       buf + (size_t)nblk * nt);
+    OB_HIP(hipGetLastError());
+    OB_HIP(hipMemcpyAsync(out, buf + (size_t)nblk * nt, sizeof(double) * nt,
                           hipMemcpyDeviceToHost, s));
     OB_HIP(hipStreamSynchronize(s));
   } catch (...) {

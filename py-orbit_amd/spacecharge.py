@@ -14,10 +14,35 @@ import _obnative as N
 
 
 class Grid1D:
-    """Snapshot of a 1D cell-centred grid (src/spacecharge/Grid1D.cc)."""
+    """Grid1D(nZ[, length]) or Grid1D(nZ, zMin, zMax): a 1D cell-centred grid held on the
+    host (src/spacecharge/Grid1D.cc, wrap_grid1D.cc:43-65).  Grid1D(nZ) spans [-1, 1] and
+    Grid1D(nZ, length) spans [0, length], as the reference's wrapper builds them.
+    binBunch / binBunchByParticle bin the bunch on the device (ob_grid1d_bin_bunch) and add
+    the result to the grid; the calculators also return their line densities as Grid1D."""
 
-    def __init__(self, arr, zMin, zMax):
-        self._a = np.array(arr, dtype=np.float64)
+    def __init__(self, nZ, *lims):
+        nZ = int(nZ)
+        if nZ < 1:
+            raise N.OrbitError("Grid1D: the number of bins should be at least 1")
+        if len(lims) == 0:
+            zmin, zmax = -1.0, 1.0
+        elif len(lims) == 1:
+            zmin, zmax = 0.0, float(lims[0])
+        elif len(lims) == 2:
+            zmin, zmax = float(lims[0]), float(lims[1])
+        else:
+            raise TypeError("Grid1D(nZ[, length]) or Grid1D(nZ, zMin, zMax)")
+        self._a = np.zeros(nZ)
+        self._zmin, self._zmax = zmin, zmax
+
+    @classmethod
+    def _of(cls, arr, zMin, zMax):
+        g = cls.__new__(cls)
+        g._a = np.array(arr, dtype=np.float64)
+        g._zmin, g._zmax = float(zMin), float(zMax)
+        return g
+
+    def setGridZ(self, zMin, zMax):
         self._zmin, self._zmax = float(zMin), float(zMax)
 
     def getSizeZ(self):
@@ -33,13 +58,94 @@ class Grid1D:
         return (self._zmax - self._zmin) / len(self._a)
 
     def getGridZ(self, i):
+        if i < 0 or i >= len(self._a):
+            raise N.OrbitError("Grid1D.getGridZ: index out of range")
         return self._zmin + (i + 0.5) * self.getStepZ()
 
     def getValueOnGrid(self, i):
+        if i < 0 or i >= len(self._a):
+            raise N.OrbitError("Grid1D.getValueOnGrid: index out of range")
         return float(self._a[i])
 
+    def setValue(self, value, i):
+        if i < 0 or i >= len(self._a):
+            raise N.OrbitError("Grid1D.setValue: index out of range")
+        self._a[i] = value
+
+    def setZero(self):
+        self._a[:] = 0.0
+
+    def multiply(self, coeff):
+        self._a *= coeff
+
     def getSum(self):
-        return float(self._a.sum())
+        s = 0.0
+        for v in self._a:  # Grid1D.cc:139-146, in index order
+            s += float(v)
+        return s
+
+    def isInside(self, z):
+        """1 inside [zMin, zMax], else 0 (Grid1D.cc:150-155)."""
+        return 0 if (z < self._zmin or z > self._zmax) else 1
+
+    def _ind_wz(self, z):
+        # Grid1D::getIndAndWZ (Grid1D.cc:568-626): periodic at both ends
+        n, dz = len(self._a), self.getStepZ()
+        if n == 1:
+            return 0, 0, 1.0, 0.0
+        i0 = int(((z - self._zmin) / dz) - 0.5)
+        frac = (z - (self._zmin + (i0 + 0.5) * dz)) / dz
+        if i0 == 0 and frac < 0.0:
+            return n - 1, 0, -frac, 1.0 + frac
+        if i0 == n - 1:
+            if frac > 0.0:
+                return i0, 0, 1.0 - frac, frac
+            return n - 2, i0, -frac, 1.0 + frac
+        return i0, i0 + 1, 1.0 - frac, frac
+
+    def binValue(self, value, z):
+        """Grid1D::binValue (Grid1D.cc:422-440)."""
+        if z < self._zmin or z > self._zmax:
+            return
+        i0, ip, w0, wp = self._ind_wz(z)
+        if len(self._a) > 1:
+            self._a[i0] += w0 * value
+            self._a[ip] += wp * value
+        else:
+            self._a[i0] += value
+
+    def getValue(self, z):
+        """Grid1D::getValue (Grid1D.cc:191-210): 0 outside [zMin, zMax]."""
+        if z < self._zmin or z > self._zmax:
+            return 0.0
+        i0, ip, w0, wp = self._ind_wz(z)
+        if len(self._a) > 1:
+            return float(w0 * self._a[i0] + wp * self._a[ip])
+        return float(self._a[0])
+
+    def _bin(self, bunch, axis, by_particle):
+        if not hasattr(bunch, "_h"):
+            raise TypeError("Grid1D.binBunch: the first parameter must be a Bunch")
+        bunch.compress()
+        bunch._push_attrs()
+        a = np.ascontiguousarray(self._a)
+        N.call("ob_grid1d_bin_bunch", bunch._h, int(axis), int(by_particle), len(a),
+               self._zmin, self._zmax, N.dptr(a))
+        self._a = a
+
+    def binBunch(self, bunch, axis=4):
+        """binBunch(bunch[, axis]) (Grid1D.cc:241-275): the macrosize of each particle
+        (attribute, else the bunch's) along coordinate axis (0..5 = x, xp, y, yp, z, dE),
+        added to the grid."""
+        self._bin(bunch, axis, False)
+
+    def binBunchByParticle(self, bunch, axis=4):
+        """binBunchByParticle(bunch[, axis]) (Grid1D.cc:335-352): weight 1 per particle."""
+        self._bin(bunch, axis, True)
+
+    def synchronizeMPI(self, comm=None):
+        """One process holds the whole bunch: nothing to add across ranks."""
+        return None
 
     def array(self):
         return self._a.copy()
@@ -271,11 +377,11 @@ class SpaceChargeCalc2p5Drb:
 
     def getLongGrid(self):
         a, l = self._grid(2)
-        return Grid1D(a, l[0], l[1])
+        return Grid1D._of(a, l[0], l[1])
 
     def getLongDerivativeGrid(self):
         a, l = self._grid(3)
-        return Grid1D(a, l[0], l[1])
+        return Grid1D._of(a, l[0], l[1])
 
     def setLongAveragingPointsN(self, n):
         N.call("ob_sc2p5drb_set_long_avg_n", self._h, int(n))
@@ -423,7 +529,7 @@ class SpaceChargeCalc2p5D:
 
     def getLongGrid(self):
         a, l = self._grid(2)
-        return Grid1D(a, l[0], l[1])
+        return Grid1D._of(a, l[0], l[1])
 
     def getState(self):
         """(xy_ratio, solver box (xMin, xMax, yMin, yMax), totalMacrosize of the last step,

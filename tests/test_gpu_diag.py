@@ -124,3 +124,207 @@ def test_empty_bunch():
     assert tw.getGlobalCount() == 0
     assert tw.getGlobalMacrosize() == 0.0
     assert tw.getAverage(0) == 0.0 and tw.getCorrelation(0, 0) == 0.0
+
+
+# ---- extrema, Grid1D binning, computeBunchMoments, diagnostics nodes, profiles ----------
+
+def _oracle_bunch(c, w=None, macrosize=1.0):
+    import oracle as O
+    return O.Bunch(c, macrosize=macrosize, kin_energy=1.0, msize=w)
+
+
+@pytest.mark.parametrize("n", [1, 777, 1_000_003])
+def test_extrema_match_oracle_exactly(n):
+    import teapot_base as TPB
+    from orbit_utils import BunchExtremaCalculator
+    from workloads import make_bunch
+    c = _coords(n, seed=n + 3)
+    b = make_bunch(c)
+    TPB.drift(b, 0.7)  # queued: applied before the scan
+    calc = BunchExtremaCalculator()
+    xyz, pde = calc.extremaXYZ(b), calc.extremaXpYpdE(b)
+    ob = _oracle_bunch(c)
+    ob.apply("drift", 0.7)
+    e = ob.extrema6()
+    assert xyz == tuple(e[[0, 1, 4, 5, 8, 9]]) and pde == tuple(e[[2, 3, 6, 7, 10, 11]])
+    assert calc.extremaZ(b) == (e[8], e[9])
+
+
+def test_extrema_of_empty_bunch():
+    from bunch import Bunch
+    from orbit_utils import BunchExtremaCalculator
+    big = np.finfo(float).max
+    assert BunchExtremaCalculator().extremaXYZ(Bunch()) == (big, -big) * 3
+
+
+@pytest.mark.parametrize("nz", [1, 2, 100, 4096, 4097])
+@pytest.mark.parametrize("mode", ["macrosize", "attribute", "particle"])
+def test_grid1d_bin_bunch_matches_oracle(nz, mode):
+    import spacecharge
+    from workloads import make_bunch
+    n = 300_000
+    c = _coords(n, seed=nz)
+    b = make_bunch(c, intensity=1e10)
+    w = None
+    if mode == "attribute":
+        w = np.random.default_rng(nz).uniform(0.5, 2.0, n)
+        b.setMacroSizes(w)
+    ob = _oracle_bunch(c, w, macrosize=1e10 / n)
+    for axis in range(6):
+        lo, hi = np.percentile(c[axis], [3, 96])  # particles outside are skipped
+        g = spacecharge.Grid1D(nz, lo, hi)
+        g.setValue(5.0, nz - 1)  # binning adds to what the grid holds
+        if mode == "particle":
+            g.binBunchByParticle(b, axis)
+        else:
+            g.binBunch(b, axis)
+        want = ob.bin_axis(nz, lo, hi, axis, by_particle=(mode == "particle"))
+        want[nz - 1] += 5.0
+        got = g.array()
+        # the oracle adds up to n values into one bin in sequence: ~n eps relative
+        assert np.all(np.abs(got - want) <= 1e-10 * np.abs(want) + 1e-300), (axis, nz, mode)
+
+
+def _moment_scale(c, w, order, prm):
+    """sum |terms| / total weight for every entry: the bound summation order is relative to."""
+    x_avg, y_avg, D, em, b2, xb, yb, disp = prm
+    dx = np.abs((c[0] - (D * c[5] / em / b2 if disp > 0 else 0.0)) - x_avg)
+    dy = np.abs(c[2] - y_avg)
+    m = np.ones(c.shape[1]) if w is None else w
+    s = np.zeros((order + 1, order + 1))
+    for j in range(order):
+        for i in range(order + 1 - j):
+            t = (m ** (i + j) if w is not None else 1.0) * dx ** i * dy ** j
+            t = t / (xb ** i * yb ** j) + (t if w is not None else 0.0)
+            s[i, j] = t.sum() / m.sum()
+    if w is None and order >= 2:
+        s[2, order] = np.sum(dx * dx * dy ** order)
+    return s
+
+
+@pytest.mark.parametrize("n,order,weights,disp,norm", [
+    (100_000, 4, False, -1, -1), (100_000, 6, True, 1, 1), (1_000_001, 12, False, 1, -1),
+    (50_000, 1, True, -1, 1), (50_000, 2, False, 1, 1), (1, 3, False, -1, -1)])
+def test_compute_bunch_moments_matches_oracle(n, order, weights, disp, norm):
+    import teapot_base as TPB
+    from bunch import BunchTwissAnalysis
+    from workloads import make_bunch
+    c = _coords(n, seed=order)
+    c[0] *= 1e3  # mm-scale coordinates keep high powers inside FP64's range
+    c[2] *= 1e3
+    b = make_bunch(c, kin_energy=1.3)
+    w = np.random.default_rng(3).uniform(0.5, 1.5, n) if weights else None
+    if weights:
+        b.setMacroSizes(w)
+    TPB.drift(b, 0.2)
+    tw = BunchTwissAnalysis()
+    tw.computeBunchMoments(b, order, disp, norm)
+    after = b.getCoordinates()
+    sp = b.getSyncParticle()
+    em = sp.kinEnergy() + sp.mass()
+    if norm > 0:
+        xb = np.sqrt(tw.getBeta(0) * tw.getEmittance(0))
+        yb = np.sqrt(tw.getBeta(1) * tw.getEmittance(1))
+    else:
+        xb, yb = np.sqrt(tw.getBeta(0)), np.sqrt(tw.getBeta(1))
+    prm = (tw.getAverage(0), tw.getAverage(2), tw.getDispersion(0), em, sp.beta() ** 2, xb, yb,
+           disp)
+    ob = _oracle_bunch(after, w)
+    want = ob.compute_moments(order, disp, *prm[:7])
+    got = np.array([[tw.getBunchMoment(i, j) for j in range(order + 1)]
+                    for i in range(order + 1)])
+    tol = 1e-11 * _moment_scale(after, w, order, prm) + 1e-300
+    tol[0, 0] = tol[1, 0] = tol[0, 1] = 0.0
+    ok = (np.abs(got - want) <= tol) | (np.isnan(got) & np.isnan(want))  # n = 1: beta is 0/0
+    assert np.all(ok), (got[~ok], want[~ok])
+    assert tw.getBunchMoment(order + 1, 0) == 0.0 and tw.getBunchMoment(-1, 0) == 0.0
+
+
+def test_compute_bunch_moments_refuses_order_zero():
+    from _obnative import OrbitError
+    from bunch import BunchTwissAnalysis
+    from workloads import make_bunch
+    with pytest.raises(OrbitError):
+        BunchTwissAnalysis().computeBunchMoments(make_bunch(_coords(10, 1)), 0, 1, 1)
+
+
+def _diag_lattice():
+    from orbit.teapot import TEAPOT_Lattice, DriftTEAPOT, QuadTEAPOT
+    lat = TEAPOT_Lattice()
+    d1, q, d2 = DriftTEAPOT("d1"), QuadTEAPOT("q"), DriftTEAPOT("d2")
+    d1.setLength(1.0)
+    q.setLength(0.4)
+    q.addParam("kq", 0.6)
+    d2.setLength(2.0)
+    for nd in (d1, q, d2):
+        lat.addNode(nd)
+    lat.initialize()
+    return lat, q
+
+
+def test_diagnostics_nodes_in_a_lattice(tmp_path):
+    """StatLats at 0.5 m (in the first drift), Moments at 2.4 m (in the second), a BPM at
+    the quad's entrance (after the quad's entrance fringe, as the reference orders entrance
+    children): each sees the bunch as the oracle has it at that point."""
+    from orbit.diagnostics import (TeapotStatLatsNode, TeapotMomentsNode, TeapotBPMSignalNode,
+                                   addTeapotDiagnosticsNode, addTeapotDiagnosticsNodeAsChild)
+    from workloads import make_bunch
+    lat, q = _diag_lattice()
+    st = TeapotStatLatsNode(str(tmp_path / "stat.dat"))
+    mo = TeapotMomentsNode(str(tmp_path / "mom.dat"), 3)
+    bpm = TeapotBPMSignalNode()
+    addTeapotDiagnosticsNode(lat, 0.5, st)
+    addTeapotDiagnosticsNode(lat, 2.4, mo)
+    addTeapotDiagnosticsNodeAsChild(lat, q, bpm)
+    assert [nd.getName() for nd in lat.getNodes()] == \
+        ["d1", "statlats no name", "d1", "q", "d2", "moments no name", "d2"]
+    c = _coords(50_000, seed=77)
+    b = make_bunch(c)
+    lat.trackBunch(b)
+    st.closeStatLats()
+    mo.closeMoments()
+    sp = b.getSyncParticle()
+    ob = _oracle_bunch(c)
+    ob.apply("drift", 0.5)
+    t1 = T.Twiss(ob.coords, p=sp.momentum(), beta=sp.beta(), gamma=sp.gamma())
+    row = [float(v) for v in open(tmp_path / "stat.dat").read().split()]
+    assert len(row) == 10 and row[0] == 0.5
+    want = [t1.emittance(0), t1.emittance(1), t1.beta_fn(0), t1.beta_fn(1), t1.alpha(0),
+            t1.alpha(1), t1.dispersion(0), t1.dispersion_prime(0)]
+    np.testing.assert_allclose(row[2:], want, rtol=1e-8, atol=1e-14)
+    ob.apply("drift", 0.5)
+    ob.apply("quadfringeIN", 0.6)  # entrance children run after the quad's own fringe
+    xb_yb = T.Twiss(ob.coords, p=sp.momentum(), beta=sp.beta(), gamma=sp.gamma())
+    assert bpm.getSignal() == pytest.approx((xb_yb.avg[0], xb_yb.avg[2]), rel=1e-10, abs=1e-16)
+    lines = open(tmp_path / "mom.dat").read().splitlines()
+    assert len(lines) == 1 and lines[0].endswith("\t")
+    vals = [float(v) for v in lines[0].split()]
+    assert vals[0] == 2.4 and len(vals) == 2 + 10
+    assert vals[2] == 1.0  # <x^0 y^0>
+
+
+def test_profiles_match_oracle(tmp_path):
+    from orbit.diagnostics import profiles
+    from orbit.utils import OrbitFinalizeError
+    from workloads import make_bunch
+    c = _coords(200_000, seed=4)
+    b = make_bunch(c, intensity=2e12)
+    ob = _oracle_bunch(c, macrosize=2e12 / 200_000)
+    e = ob.extrema6()
+    g = profiles(b, "px", str(tmp_path / "h.dat"), steps=64)
+    want = ob.bin_axis(64, e[2], e[3], 1)
+    assert (g.getMinZ(), g.getMaxZ()) == (e[2], e[3])
+    assert np.all(np.abs(g.array() - want) <= 1e-12 * want + 1e-300)
+    lines = open(tmp_path / "h.dat").read().splitlines()
+    assert len(lines) == 65 and lines[0].startswith("%  Min, Max =")
+    assert float(lines[0].split("hist_sum =")[1]) == pytest.approx(want.sum(), rel=1e-7)
+    z, v = (float(t) for t in lines[10].split())
+    assert z == pytest.approx(g.getGridZ(9), rel=1e-7) and v == pytest.approx(want[9], rel=1e-7)
+    g2 = profiles(b, "dE", str(tmp_path / "h2.dat"), steps=10, Min=-1e-3, Max=1e-3)
+    assert np.all(np.abs(g2.array() - ob.bin_axis(10, -1e-3, 1e-3, 5)) <=
+                  1e-12 * ob.bin_axis(10, -1e-3, 1e-3, 5) + 1e-300)
+    with pytest.raises(OrbitFinalizeError):
+        profiles(b, "t", str(tmp_path / "h3.dat"))
+    one = make_bunch(c[:, :1])
+    profiles(one, "x", str(tmp_path / "h4.dat"))
+    assert open(tmp_path / "h4.dat").read().startswith("bunch has only number of particles =1")
