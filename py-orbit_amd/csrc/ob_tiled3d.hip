@@ -229,10 +229,10 @@ __global__ __launch_bounds__(kThreads) void tile_deposit(const double* __restric
 // 27 nodes) or the whole grid.
 // BND: particles outside the boundary in k are not kicked.
 template <bool BND>
-__device__ __forceinline__ void kick_particle(double* __restrict__ cur, int64_t cap, int64_t i,
-                                              const TileGeom& g, const Kick3D& k,
-                                              const double* lds, const TileBox* box) {
-  const double x = cur[i], y = cur[2 * cap + i], z = cur[4 * cap + i];
+__device__ __forceinline__ void kick_particle_xyz(double* __restrict__ cur, int64_t cap,
+                                                  int64_t i, double x, double y, double z,
+                                                  const TileGeom& g, const Kick3D& k,
+                                                  const double* lds, const TileBox* box) {
   if (BND && !boundary_inside(k.bnd_type, k.bnd, x, y)) return;
   const double zr = (z - k.zc) * k.gamma + k.zc;
   int iX, iY, iZ;
@@ -252,6 +252,14 @@ __device__ __forceinline__ void kick_particle(double* __restrict__ cur, int64_t 
   cur[cap + i] += -ex * k.trans_factor;
   cur[3 * cap + i] += -ey * k.trans_factor;
   cur[5 * cap + i] += -ez * k.long_factor;
+}
+
+template <bool BND>
+__device__ __forceinline__ void kick_particle(double* __restrict__ cur, int64_t cap, int64_t i,
+                                              const TileGeom& g, const Kick3D& k,
+                                              const double* lds, const TileBox* box) {
+  kick_particle_xyz<BND>(cur, cap, i, cur[i], cur[2 * cap + i], cur[4 * cap + i], g, k, lds,
+                         box);
 }
 
 // Blocks 0..nitems-1: work items; block nitems: the particles outside the grid (last in the
@@ -275,14 +283,39 @@ __global__ __launch_bounds__(kThreads, 4) void tile_kick(double* __restrict__ cu
   const TileBox box(g, it.x);
   const int cells = box.cells();
   const int64_t gsx = (int64_t)g.ny * g.nz;
-  for (int c = threadIdx.x; c < cells; c += blockDim.x) {
+  // The first particle's coordinates are requested with the block of phi, and all of a
+  // thread's block loads are issued before any is stored (a masked lane loads phi[0] and
+  // stores 0), so these latencies overlap instead of adding up.
+  const int i0 = it.y + (int)threadIdx.x;
+  const int ia = i0 < it.z ? i0 : it.y;
+  const double px = cur[ia], py = cur[2 * cap + ia], pz = cur[4 * cap + ia];
+  constexpr int KB = 8;  // cells <= KB * kThreads for blocks of up to 16 KiB
+  double bv[KB];
+  uint32_t inmask = 0;
+#pragma unroll
+  for (int u = 0; u < KB; u++) {
+    const int c = (int)threadIdx.x + u * kThreads;
+    const int lz = c % box.Wz, r = c / box.Wz, ly = r % box.W, lx = r / box.W;
+    const int gx = box.x0 + lx, gy = box.y0 + ly, gz = box.z0 + lz;
+    const bool in = c < cells && gx >= 0 && gx < g.nx && gy >= 0 && gy < g.ny && gz >= 0 &&
+                    gz < g.nz;
+    bv[u] = k.phi[in ? gx * gsx + (int64_t)gy * g.nz + gz : 0];
+    inmask |= (in ? 1u : 0u) << u;
+  }
+#pragma unroll
+  for (int u = 0; u < KB; u++) {
+    const int c = (int)threadIdx.x + u * kThreads;
+    if (c < cells) sphi[c] = ((inmask >> u) & 1u) ? bv[u] : 0.0;
+  }
+  for (int c = (int)threadIdx.x + KB * kThreads; c < cells; c += kThreads) {  // larger blocks
     const int lz = c % box.Wz, r = c / box.Wz, ly = r % box.W, lx = r / box.W;
     const int gx = box.x0 + lx, gy = box.y0 + ly, gz = box.z0 + lz;
     const bool in = gx >= 0 && gx < g.nx && gy >= 0 && gy < g.ny && gz >= 0 && gz < g.nz;
     sphi[c] = in ? k.phi[gx * gsx + (int64_t)gy * g.nz + gz] : 0.0;
   }
   __syncthreads();
-  for (int i = it.y + threadIdx.x; i < it.z; i += blockDim.x)
+  if (i0 < it.z) kick_particle_xyz<BND>(cur, cap, i0, px, py, pz, g, k, sphi, &box);
+  for (int i = i0 + kThreads; i < it.z; i += kThreads)
     kick_particle<BND>(cur, cap, i, g, k, sphi, &box);
 }
 
