@@ -176,16 +176,31 @@ __global__ __launch_bounds__(kThreads) void fft3_z_r2c(const double* __restrict_
   double* re = sm;
   double* im = sm + P * ls;
   const int64_t pair0 = (int64_t)blockIdx.x * P;
-  for (int idx = threadIdx.x; idx < P * N; idx += kThreads) {
+  // All of a thread's loads are issued before any is stored (a masked lane loads rho[0]
+  // and stores 0), so their latencies overlap instead of adding up.
+  constexpr int IT = (P * N + kThreads - 1) / kThreads;  // <= 8
+  double a0[IT], a1[IT];
+  uint32_t m0 = 0, m1 = 0;
+#pragma unroll
+  for (int u = 0; u < IT; u++) {
+    const int idx = (int)threadIdx.x + u * kThreads;
     const int lp = idx / N, k = idx - lp * N;
     const int64_t q0 = 2 * (pair0 + lp), q1 = q0 + 1;
-    double v0 = 0.0, v1 = 0.0;
-    if (k < nz_in) {
-      if (q0 < nlines) v0 = rho[q0 * nz_in + k];
-      if (q1 < nlines) v1 = rho[q1 * nz_in + k];
+    const bool ok = idx < P * N && k < nz_in;
+    const bool ok0 = ok && q0 < nlines, ok1 = ok && q1 < nlines;
+    a0[u] = rho[ok0 ? q0 * nz_in + k : 0];
+    a1[u] = rho[ok1 ? q1 * nz_in + k : 0];
+    m0 |= (ok0 ? 1u : 0u) << u;
+    m1 |= (ok1 ? 1u : 0u) << u;
+  }
+#pragma unroll
+  for (int u = 0; u < IT; u++) {
+    const int idx = (int)threadIdx.x + u * kThreads;
+    if (idx < P * N) {
+      const int lp = idx / N, k = idx - lp * N;
+      re[lp * ls + k] = ((m0 >> u) & 1u) ? a0[u] : 0.0;
+      im[lp * ls + k] = ((m1 >> u) & 1u) ? a1[u] : 0.0;
     }
-    re[lp * ls + k] = v0;
-    im[lp * ls + k] = v1;
   }
   __syncthreads();
   lds_fft<N, P, kThreads, -1>(re, im, P, ls, tw, nmax);
@@ -470,12 +485,28 @@ __global__ __launch_bounds__(kThreads) void green3d_axis(double* __restrict__ gh
   const int k0 = (blockIdx.x - o * tiles) * L;
   const int nl = min(L, H2 - k0);
   double* base = ghat + (int64_t)o * outer_stride + k0;
-  for (int idx = threadIdx.x; idx < L * Q; idx += kThreads) {
+  // loads issued together before the stores, as in fft3_z_r2c
+  constexpr int IT = (L * Q + kThreads - 1) / kThreads;
+  double av[IT];
+  uint32_t msk = 0;
+#pragma unroll
+  for (int u = 0; u < IT; u++) {
+    const int idx = (int)threadIdx.x + u * kThreads;
     const int p = idx / L, l = idx - p * L;
-    const double v = (l < nl) ? base[(int64_t)p * line_stride + l] : 0.0;
-    double* d = ((l & 1) ? im : re) + (l >> 1) * ls;
-    d[p] = v;
-    if (p > 0 && p < N / 2) d[N - p] = v;
+    const bool ok = idx < L * Q && l < nl;
+    av[u] = base[ok ? (int64_t)p * line_stride + l : 0];
+    msk |= (ok ? 1u : 0u) << u;
+  }
+#pragma unroll
+  for (int u = 0; u < IT; u++) {
+    const int idx = (int)threadIdx.x + u * kThreads;
+    if (idx < L * Q) {
+      const int p = idx / L, l = idx - p * L;
+      const double v = ((msk >> u) & 1u) ? av[u] : 0.0;
+      double* d = ((l & 1) ? im : re) + (l >> 1) * ls;
+      d[p] = v;
+      if (p > 0 && p < N / 2) d[N - p] = v;
+    }
   }
   __syncthreads();
   lds_fft<N, C, kThreads, -1>(re, im, (nl + 1) >> 1, ls, tw, nmax);
