@@ -232,18 +232,29 @@ __global__ __launch_bounds__(kThreads) void fft3_z_c2r(const double2* __restrict
   double* re = sm;
   double* im = sm + P * ls;
   const int64_t pair0 = (int64_t)blockIdx.x * P;
-  for (int idx = threadIdx.x; idx < P * H2; idx += kThreads) {
+  // loads issued together before the stores, as in fft3_z_r2c
+  constexpr int IT = (P * H2 + kThreads - 1) / kThreads;
+  double2 av[IT], bv[IT];
+  uint32_t ma = 0, mb = 0;
+#pragma unroll
+  for (int u = 0; u < IT; u++) {
+    const int idx = (int)threadIdx.x + u * kThreads;
     const int lp = idx / H2, k = idx - lp * H2;
     const int64_t q0 = 2 * (pair0 + lp), q1 = q0 + 1;
-    double2 a = make_double2(0.0, 0.0), b = make_double2(0.0, 0.0);
-    if (q0 < nlines) {
-      const int64_t i0 = q0 / NY;
-      a = spec[(i0 * M1 + (q0 - i0 * NY)) * H2p + k];
-    }
-    if (q1 < nlines) {
-      const int64_t i1 = q1 / NY;
-      b = spec[(i1 * M1 + (q1 - i1 * NY)) * H2p + k];
-    }
+    const bool ok0 = idx < P * H2 && q0 < nlines, ok1 = idx < P * H2 && q1 < nlines;
+    const int64_t i0 = q0 / NY, i1 = q1 / NY;
+    av[u] = spec[ok0 ? (i0 * M1 + (q0 - i0 * NY)) * H2p + k : 0];
+    bv[u] = spec[ok1 ? (i1 * M1 + (q1 - i1 * NY)) * H2p + k : 0];
+    ma |= (ok0 ? 1u : 0u) << u;
+    mb |= (ok1 ? 1u : 0u) << u;
+  }
+#pragma unroll
+  for (int u = 0; u < IT; u++) {
+    const int idx = (int)threadIdx.x + u * kThreads;
+    if (idx >= P * H2) continue;
+    const int lp = idx / H2, k = idx - lp * H2;
+    double2 a = ((ma >> u) & 1u) ? av[u] : make_double2(0.0, 0.0);
+    double2 b = ((mb >> u) & 1u) ? bv[u] : make_double2(0.0, 0.0);
     if (k == 0 || k == N / 2) {  // c2r ignores the imaginary DC/Nyquist parts
       a.y = 0.0;
       b.y = 0.0;

@@ -75,7 +75,26 @@ __global__ __launch_bounds__(kThreads) void tile_hist(const double* __restrict__
   for (int t = threadIdx.x; t < nb; t += blockDim.x) lh[t] = 0;
   __syncthreads();
   const int64_t stride = (int64_t)gridDim.x * blockDim.x;
-  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  // four particles per round: their 12 loads are in flight together
+  constexpr int U = 4;
+  for (; i + (U - 1) * stride < n; i += U * stride) {
+    double x[U], y[U], z[U];
+#pragma unroll
+    for (int u = 0; u < U; u++) {
+      const int64_t j = i + u * stride;
+      x[u] = cur[j];
+      y[u] = cur[2 * cap + j];
+      z[u] = cur[4 * cap + j];
+    }
+#pragma unroll
+    for (int u = 0; u < U; u++) {
+      const uint32_t k = tile_key(g, x[u], y[u], z[u]);
+      key[i + u * stride] = k;
+      atomicAdd(&lh[k], 1u);
+    }
+  }
+  for (; i < n; i += stride) {
     const uint32_t k = tile_key(g, cur[i], cur[2 * cap + i], cur[4 * cap + i]);
     key[i] = k;
     atomicAdd(&lh[k], 1u);
@@ -409,7 +428,8 @@ void sort_and_deposit_3d(ob_bunch* b, const TileGeom& g, double macrosize, doubl
   hipStream_t s = stream();
   SortBuffers sb = bunch_sort_begin(b);
   const size_t hl = sizeof(uint32_t) * (g.ntiles + 1);
-  int hb = (int)std::min<int64_t>((n + kThreads - 1) / kThreads, 4 * device_cu_count());
+  // two workgroups per CU: each merges up to ntiles + 1 counters into the global counts
+  int hb = (int)std::min<int64_t>((n + kThreads - 1) / kThreads, 2 * device_cu_count());
   if (hb < 1) hb = 1;
   {
     ProfScope ps("tile_hist");
