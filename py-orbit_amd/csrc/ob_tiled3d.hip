@@ -114,16 +114,31 @@ __global__ __launch_bounds__(1024) void tile_scan(uint32_t* __restrict__ counts,
   const int nb = ntiles + 1;
   const int per = (nb + blockDim.x - 1) / blockDim.x;
   const int lo = min(nb, (int)threadIdx.x * per), hi = min(nb, lo + per);
+  // a thread's counters (at most kPer: ntiles + 1 <= 16384 over 1024 threads) are loaded
+  // once, all in flight together
+  constexpr int kPer = (kMaxTiles + 1) / 1024;
+  uint32_t cv[kPer];
+#pragma unroll
+  for (int u = 0; u < kPer; u++) {
+    const int t = lo + u;
+    const uint32_t v = counts[t < hi ? t : 0];
+    cv[u] = t < hi ? v : 0u;
+  }
   uint32_t off = 0, item = 0;
-  for (int t = lo; t < hi; t++) {
-    off += counts[t];
-    if (t < ntiles) item += (counts[t] + kChunk - 1) / kChunk;
+#pragma unroll
+  for (int u = 0; u < kPer; u++) {
+    const int t = lo + u;
+    off += cv[u];
+    if (t < ntiles) item += (cv[u] + kChunk - 1) / kChunk;
   }
   uint32_t total, total_items;
   block_scan2(off, item, total, total_items);
   if (threadIdx.x == 0) *nitems = total_items;
-  for (int t = lo; t < hi; t++) {
-    const uint32_t cnt = counts[t];
+#pragma unroll
+  for (int u = 0; u < kPer; u++) {
+    const int t = lo + u;
+    if (t >= hi) break;
+    const uint32_t cnt = cv[u];
     offsets[t] = off;
     cursor[t] = off;
     if (t < ntiles) {
