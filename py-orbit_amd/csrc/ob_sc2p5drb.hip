@@ -769,10 +769,23 @@ __device__ __forceinline__ void cols_body(double2* __restrict__ spec, int nrows_
   extern __shared__ double sm[];
   double* re = sm;
   double* im = sm + N;
-  for (int i = threadIdx.x; i < N; i += kFFTThreads) {
-    const double2 v = (i < nrows_in) ? spec[(int64_t)i * H + k] : make_double2(0.0, 0.0);
-    re[i] = v.x;
-    im[i] = v.y;
+  // The column and (for a solve) its Green values are requested together before the
+  // transform; a masked lane loads element 0 and drops it.
+  constexpr int CI = (N + kFFTThreads - 1) / kFFTThreads;
+  double2 cv[CI], gv[CI];
+#pragma unroll
+  for (int u = 0; u < CI; u++) {
+    const int i = (int)threadIdx.x + u * kFFTThreads;
+    cv[u] = spec[(i < nrows_in && i < N) ? (int64_t)i * H + k : (int64_t)k];
+    if constexpr (!GREEN) gv[u] = ghat_t[(int64_t)k * N + (i < N ? i : 0)];
+  }
+#pragma unroll
+  for (int u = 0; u < CI; u++) {
+    const int i = (int)threadIdx.x + u * kFFTThreads;
+    if (i < N) {
+      re[i] = i < nrows_in ? cv[u].x : 0.0;
+      im[i] = i < nrows_in ? cv[u].y : 0.0;
+    }
   }
   __syncthreads();
   lds_fft<N, 1, kFFTThreads, -1>(re, im, 1, N, tw, nmax);
@@ -783,8 +796,11 @@ __device__ __forceinline__ void cols_body(double2* __restrict__ spec, int nrows_
   } else {
     // PoissonSolverFFT2D.cc:215-236: Green spectrum rescaled to the rho grid step
     const double s = st->scale_coeff;
-    for (int i = threadIdx.x; i < N; i += kFFTThreads) {
-      const double2 g = ghat_t[(int64_t)k * N + i];
+#pragma unroll
+    for (int u = 0; u < CI; u++) {
+      const int i = (int)threadIdx.x + u * kFFTThreads;
+      if (i >= N) continue;
+      const double2 g = gv[u];
       const double gr_re = ((i == 0 && k == 0) ? (g.x + s * npad) : g.x) - s;
       const double gr_im = g.y;
       const double a = re[i], b = im[i];
