@@ -262,6 +262,15 @@ __global__ __launch_bounds__(kThreads) void tile_deposit(const double* __restric
 // SpaceChargeCalc3D kick of particle i, gathering phi through view v (which must hold the
 // 27 nodes) or the whole grid.
 // BND: particles outside the boundary in k are not kicked.
+// The kick of particle i at (x, y, z) with momenta (xp, yp, dE) already read: writes
+// xp, yp, dE (the rest of kick_particle).
+template <bool BND>
+__device__ __forceinline__ void kick_particle_all(double* __restrict__ cur, int64_t cap,
+                                                  int64_t i, double x, double y, double z,
+                                                  double xp, double yp, double dE,
+                                                  const TileGeom& g, const Kick3D& k,
+                                                  const double* lds, const TileBox* box);
+
 template <bool BND>
 __device__ __forceinline__ void kick_particle_xyz(double* __restrict__ cur, int64_t cap,
                                                   int64_t i, double x, double y, double z,
@@ -286,6 +295,33 @@ __device__ __forceinline__ void kick_particle_xyz(double* __restrict__ cur, int6
   cur[cap + i] += -ex * k.trans_factor;
   cur[3 * cap + i] += -ey * k.trans_factor;
   cur[5 * cap + i] += -ez * k.long_factor;
+}
+
+template <bool BND>
+__device__ __forceinline__ void kick_particle_all(double* __restrict__ cur, int64_t cap,
+                                                  int64_t i, double x, double y, double z,
+                                                  double xp, double yp, double dE,
+                                                  const TileGeom& g, const Kick3D& k,
+                                                  const double* lds, const TileBox* box) {
+  if (BND && !boundary_inside(k.bnd_type, k.bnd, x, y)) return;
+  const double zr = (z - k.zc) * k.gamma + k.zc;
+  int iX, iY, iZ;
+  double xF, yF, zF;
+  node_ind_frac(x, g.xMin, g.dx, g.nx, iX, xF);
+  node_ind_frac(y, g.yMin, g.dy, g.ny, iY, yF);
+  grid3d_z_ind_frac(zr, k.rzMin, k.rdz, g.nz, iZ, zF);
+  const int zlo = g.nz >= 3 ? iZ - 1 : 0, zhi = g.nz >= 3 ? iZ + 1 : g.nz - 1;
+  double ex, ey, ez;
+  if (box->holds(iX, iY, zlo, zhi)) {
+    const Grid3View v{lds, (int64_t)box->W * box->Wz, box->Wz, box->x0, box->y0, box->z0};
+    grid3d_gradient_at(v, g.nz, iX, iY, iZ, xF, yF, zF, g.dx, g.dy, k.rdz, ex, ey, ez);
+  } else {
+    const Grid3View v{k.phi, (int64_t)g.ny * g.nz, g.nz, 0, 0, 0};
+    grid3d_gradient_at(v, g.nz, iX, iY, iZ, xF, yF, zF, g.dx, g.dy, k.rdz, ex, ey, ez);
+  }
+  cur[cap + i] = xp + -ex * k.trans_factor;
+  cur[3 * cap + i] = yp + -ey * k.trans_factor;
+  cur[5 * cap + i] = dE + -ez * k.long_factor;
 }
 
 template <bool BND>
@@ -323,6 +359,7 @@ __global__ __launch_bounds__(kThreads, 4) void tile_kick(double* __restrict__ cu
   const int i0 = it.y + (int)threadIdx.x;
   const int ia = i0 < it.z ? i0 : it.y;
   const double px = cur[ia], py = cur[2 * cap + ia], pz = cur[4 * cap + ia];
+  const double pxp = cur[cap + ia], pyp = cur[3 * cap + ia], pdE = cur[5 * cap + ia];
   constexpr int KB = 8;  // cells <= KB * kThreads for blocks of up to 16 KiB
   double bv[KB];
   uint32_t inmask = 0;
@@ -348,7 +385,7 @@ __global__ __launch_bounds__(kThreads, 4) void tile_kick(double* __restrict__ cu
     sphi[c] = in ? k.phi[gx * gsx + (int64_t)gy * g.nz + gz] : 0.0;
   }
   __syncthreads();
-  if (i0 < it.z) kick_particle_xyz<BND>(cur, cap, i0, px, py, pz, g, k, sphi, &box);
+  if (i0 < it.z) kick_particle_all<BND>(cur, cap, i0, px, py, pz, pxp, pyp, pdE, g, k, sphi, &box);
   for (int i = i0 + kThreads; i < it.z; i += kThreads)
     kick_particle<BND>(cur, cap, i, g, k, sphi, &box);
 }
