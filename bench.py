@@ -22,6 +22,7 @@ bounded sample of the same workload, timed on this host).
 """
 import argparse
 import json
+import math
 import os
 import sys
 import time
@@ -93,6 +94,26 @@ def build_workload(args):
         nsc = sum(1 for nd in nodes[i:i + cell_len] for ch in nd.getBodyChildren() if id(ch) in ids)
         cells.append((i, i + cell_len - 1, nsc))
     return b, lat, calc, cells, n, grid, coords
+
+
+def beam_stats(b):
+    """Beam quality after the run, from device reductions without downloading the bunch:
+    rms emittances (BunchTwissAnalysis) and the fraction of particles beyond 4 rms sizes
+    from the mean in x and in y (a one-cell Grid1D over mean +- 4 sigma counts the
+    particles inside)."""
+    import spacecharge
+    from bunch import BunchTwissAnalysis
+    tw = BunchTwissAnalysis()
+    tw.analyzeBunch(b)
+    n = b.getSize()
+    out = {"emit_x": tw.getEmittance(0), "emit_y": tw.getEmittance(1)}
+    for name, ax in (("x", 0), ("y", 2)):
+        mu = tw.getAverage(ax)
+        sig = math.sqrt(max(tw.getCorrelation(ax, ax), 0.0))
+        g = spacecharge.Grid1D(1, mu - 4 * sig, mu + 4 * sig)
+        g.binBunchByParticle(b, ax)
+        out["halo_4sigma_" + name] = (n - g.getSum()) / n if n else 0.0
+    return out
 
 
 def run_cells(lat, b, cells, k0, count):
@@ -538,6 +559,7 @@ def main():
         "roofline": roofline,
         "cpu_baseline": cpu,
         "kernels": kernels,
+        "beam": beam_stats(b),
     }
     print(json.dumps(out))
 
