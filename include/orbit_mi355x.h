@@ -98,9 +98,9 @@ int ob_bunch_moments(ob_bunch* b, double* out);
  * empty bunch. */
 int ob_bunch_extrema(ob_bunch* b, double* out);
 /* Grid1D::binBunch(bunch, axis) and binBunchByParticle(bunch, axis)
- * (src/spacecharge/Grid1D.cc:241-275, 335-352; wrap_grid1D.cc:252-325) on the device, for a
- * grid of nz cells over [zMin, zMax]: the binned values are added to the host array
- * arr[nz].  by_particle != 0 weighs every particle 1. */
+ * (src/spacecharge/Grid1D.cc:241-275, 335-352; wrap_grid1D.cc:244-325) on the device, for a
+ * grid of nz cells (1 <= nz <= 2^26) over [zMin, zMax]: the binned values are added to the
+ * host array arr[nz].  by_particle != 0 weighs every particle 1. */
 int ob_grid1d_bin_bunch(ob_bunch* b, int axis, int by_particle, int nz, double zMin,
                         double zMax, double* arr);
 /* The particle sums of BunchTwissAnalysis::computeBunchMoments

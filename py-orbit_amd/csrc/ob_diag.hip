@@ -337,7 +337,8 @@ OB_API int ob_grid1d_bin_bunch(ob_bunch* b, int axis, int by_particle, int nz, d
                                double zMax, double* arr) {
   OB_TRY OB_REQUIRE(b != nullptr && arr != nullptr, "null handle");
   OB_REQUIRE(axis >= 0 && axis <= 5, "Grid1D::binBunch: the axis index should be between 0 and 5");
-  OB_REQUIRE(nz >= 1, "Grid1D: the number of bins should be at least 1");
+  OB_REQUIRE(nz >= 1 && nz <= (1 << 26),
+             "Grid1D.binBunch: the number of bins should be between 1 and 2^26");
   ob::bunch_flush(b, nullptr);
   const ob::DeviceBunchView v = ob::bunch_device_view(b);
   if (v.n == 0) return 0;

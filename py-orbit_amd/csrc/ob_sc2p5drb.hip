@@ -577,8 +577,18 @@ __device__ __forceinline__ void sc2d_post_body(const SC2DParams& P, SC2DState* s
     for (int i0 = 0; i0 < nz; i0 += (int)blockDim.x / g) {
       const int i = i0 + (int)threadIdx.x / g, r = (int)threadIdx.x % g;
       double acc = 0.0;
-      if (i < nz && (int)threadIdx.x < g * ((int)blockDim.x / g))
-        for (int b = r; b < zrows; b += g) acc += zpart[(size_t)b * nz + i];
+      if (i < nz && (int)threadIdx.x < g * ((int)blockDim.x / g)) {
+        // eight rows' loads in flight per round, added in the same order as one by one
+        int b = r;
+        for (; b + 7 * g < zrows; b += 8 * g) {
+          double v[8];
+#pragma unroll
+          for (int u = 0; u < 8; u++) v[u] = zpart[(size_t)(b + u * g) * nz + i];
+#pragma unroll
+          for (int u = 0; u < 8; u++) acc += v[u];
+        }
+        for (; b < zrows; b += g) acc += zpart[(size_t)b * nz + i];
+      }
       zs[threadIdx.x] = acc;
       __syncthreads();
       if (r == 0 && i < nz && (int)threadIdx.x < g * ((int)blockDim.x / g)) {
