@@ -97,6 +97,10 @@ int ob_bunch_moments(ob_bunch* b, double* out);
  * one device pass: out[12] = (min, max) of x, xp, y, yp, z, dE; DBL_MAX, -DBL_MAX for an
  * empty bunch. */
 int ob_bunch_extrema(ob_bunch* b, double* out);
+/* BunchExtremaCalculator::getExtremaR (BunchExtremaCalculator.cc:167-208,
+ * wrap_bunch_extrema_calculator.cc:95-109): *rmax = the largest sqrt((x - <x>)^2 +
+ * (y - <y>)^2) over the particles, <.> the unweighted mean; 0 for an empty bunch. */
+int ob_bunch_extrema_r(ob_bunch* b, double* rmax);
 /* Grid1D::binBunch(bunch, axis) and binBunchByParticle(bunch, axis)
  * (src/spacecharge/Grid1D.cc:241-275, 335-352; wrap_grid1D.cc:244-325) on the device, for a
  * grid of nz cells (1 <= nz <= 2^26) over [zMin, zMax]: the binned values are added to the

@@ -64,6 +64,7 @@ def lib():
             "orc_quad2": (None, [bp, d]), "orc_quadfringeIN": (None, [bp, d]),
             "orc_quadfringeOUT": (None, [bp, d]), "orc_extrema_xyz": (None, [bp, _D]),
             "orc_extrema6": (None, [bp, _D]),
+            "orc_extrema_r": (d, [bp]),
             "orc_flat_bunch_bin_axis": (None, [bp, i, d, d, i, i, _D]),
             "orc_compute_bunch_moments": (None, [bp, i, i, d, d, d, d, d, d, d, _D]),
             "orc_wedgerotate": (None, [bp, d, i]), "orc_wedgedrift": (None, [bp, d, i]),
@@ -180,6 +181,10 @@ class Bunch:
         out = np.zeros(12)
         lib().orc_extrema6(self.ref, _dp(out))
         return out
+
+    def extrema_r(self):
+        """Largest distance from the unweighted mean (x, y) (BunchExtremaCalculator.cc:167-208)."""
+        return lib().orc_extrema_r(self.ref)
 
     def bin_axis(self, nz, zmin, zmax, axis, by_particle=False):
         """Grid1D(nz, zmin, zmax).binBunch(bunch, axis) (Grid1D.cc:241-275, 335-352)."""

@@ -567,6 +567,26 @@ void orc_extrema6(const orc_bunch* b, double o[12]) {
   o[8] = xyz[4]; o[9] = xyz[5]; o[10] = l[4]; o[11] = l[5];
 }
 
+/* getExtremaR (BunchExtremaCalculator.cc:167-208): unweighted mean of x and y, then the
+   largest sqrt(pow(x - <x>, 2) + pow(y - <y>, 2)). */
+double orc_extrema_r(const orc_bunch* b) {
+  double xa = 0., ya = 0.;
+  for (int64_t i = 0; i < b->n; i++) {
+    xa += b->x[i];
+    ya += b->y[i];
+  }
+  if (b->n > 0) {
+    xa = xa / (double)b->n;
+    ya = ya / (double)b->n;
+  }
+  double rl = 0.;
+  for (int64_t i = 0; i < b->n; i++) {
+    const double r = sqrt(pow(b->x[i] - xa, 2.0) + pow(b->y[i] - ya, 2.0));
+    if (r > rl) rl = r;
+  }
+  return rl;
+}
+
 /* BunchTwissAnalysis::computeBunchMoments (BunchTwissAnalysis.cc:145-337), after the
    caller's analyzeBunch: xAvg = getAverage(0) (the dispersion branch at :166 is dead, its
    test reads a local that is still 0), yAvg = getAverage(2), D = getDispersion(0),

@@ -74,6 +74,7 @@ void orc_ringrf(orc_bunch* b, double ring_length, int harmonic_numb, double volt
 /* ---- extrema (src/utils/bunch/BunchExtremaCalculator.cc:37-83) ---- */
 void orc_extrema_xyz(const orc_bunch* b, double out6[6]);
 void orc_extrema6(const orc_bunch* b, double out12[12]);
+double orc_extrema_r(const orc_bunch* b);
 /* ---- computeBunchMoments (src/orbit/BunchDiagnostics/BunchTwissAnalysis.cc:145-337) ---- */
 void orc_compute_bunch_moments(const orc_bunch* b, int order, int dispflag, double xAvg,
                                double yAvg, double D, double Em, double b2, double xb,

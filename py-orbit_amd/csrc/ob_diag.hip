@@ -119,6 +119,65 @@ __global__ __launch_bounds__(64) void extrema6_reduce(const double* __restrict__
   out[k] = r;
 }
 
+// ---- extremaR: largest distance from the mean transverse position -----------------------
+// Pass 1: per-workgroup sums of x and y (unweighted, as the reference's mean); pass 2: each
+// workgroup adds the sums in a fixed order, forms the mean and its particles' largest
+// distance; pass 3: the largest of the workgroups'.
+__global__ __launch_bounds__(kThreads) void xy_sum_partials(DeviceBunchView v,
+                                                            double* __restrict__ part) {
+  __shared__ double sh[kThreads / 64];
+  double sx = 0.0, sy = 0.0;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < v.n;
+       i += (int64_t)gridDim.x * blockDim.x) {
+    sx += v.c[0][i];
+    sy += v.c[2][i];
+  }
+  sx = block_sum(sx, sh);
+  __syncthreads();
+  sy = block_sum(sy, sh);
+  if (threadIdx.x == 0) {
+    part[2 * blockIdx.x] = sx;
+    part[2 * blockIdx.x + 1] = sy;
+  }
+}
+
+__global__ __launch_bounds__(kThreads) void rmax_partials(DeviceBunchView v,
+                                                          const double* __restrict__ sums,
+                                                          int nblk, double* __restrict__ part) {
+  __shared__ double sh[kThreads / 64];
+  double sx = 0.0, sy = 0.0;
+  for (int b = 0; b < nblk; b++) {
+    sx += sums[2 * b];
+    sy += sums[2 * b + 1];
+  }
+  const double xa = sx / (double)v.n, ya = sy / (double)v.n;
+  double r = 0.0;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < v.n;
+       i += (int64_t)gridDim.x * blockDim.x) {
+    const double dx = v.c[0][i] - xa, dy = v.c[2][i] - ya;
+    const double ri = sqrt(dx * dx + dy * dy);
+    r = ri > r ? ri : r;
+  }
+  for (int o = 32; o > 0; o >>= 1) {
+    const double t = __shfl_xor(r, o, 64);
+    r = t > r ? t : r;
+  }
+  if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = r;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    for (int w = 1; w < kThreads / 64; w++) r = sh[w] > r ? sh[w] : r;
+    part[blockIdx.x] = r;
+  }
+}
+
+__global__ __launch_bounds__(64) void max_reduce(const double* __restrict__ part, int nblk,
+                                                 double* __restrict__ out) {
+  if (threadIdx.x != 0) return;
+  double r = 0.0;
+  for (int b = 0; b < nblk; b++) r = part[b] > r ? part[b] : r;
+  *out = r;
+}
+
 // ---- Grid1D::binBunch along one coordinate --------------------------------------------
 // Linear (cloud-in-cell) weights of Grid1D::binValue (Grid1D.cc:422-440) with the grid's
 // periodic wrap at both ends.  Grids up to kBinLds bins accumulate a private histogram per
@@ -430,6 +489,45 @@ OB_API int ob_bunch_xy_moments(ob_bunch* b, int order, const double* prm, double
     OB_HIP(hipGetLastError());
     OB_HIP(hipMemcpyAsync(out, buf + (size_t)nblk * nt, sizeof(double) * nt,
                           hipMemcpyDeviceToHost, s));
+    OB_HIP(hipStreamSynchronize(s));
+  } catch (...) {
+    (void)hipFree(buf);
+    throw;
+  }
+  OB_HIP(hipFree(buf));
+  OB_CATCH
+}
+
+// BunchExtremaCalculator::getExtremaR (BunchExtremaCalculator.cc:167-208): the largest
+// distance sqrt((x - <x>)^2 + (y - <y>)^2) from the unweighted mean position; 0 for an
+// empty bunch.
+OB_API int ob_bunch_extrema_r(ob_bunch* b, double* rmax) {
+  OB_TRY OB_REQUIRE(b != nullptr && rmax != nullptr, "null handle");
+  ob::bunch_flush(b, nullptr);
+  const ob::DeviceBunchView v = ob::bunch_device_view(b);
+  hipStream_t s = ob::stream();
+  if (v.n == 0) {
+    *rmax = 0.0;
+    return 0;
+  }
+  const int nblk =
+      (int)std::min<int64_t>(ob::kMaxBlocks, (v.n + ob::kThreads - 1) / ob::kThreads);
+  double* buf = nullptr;
+  OB_HIP(hipMalloc((void**)&buf, sizeof(double) * (3 * (size_t)nblk + 1)));
+  try {
+    double* sums = buf;
+    double* rpart = buf + 2 * (size_t)nblk;
+    {
+      ob::ProfScope ps("rmax_partials");
+      hipLaunchKernelGGL(ob::xy_sum_partials, dim3(nblk), dim3(ob::kThreads), 0, s, v, sums);
+      hipLaunchKernelGGL(ob::rmax_partials, dim3(nblk), dim3(ob::kThreads), 0, s, v, sums,
+                         nblk, rpart);
+    }
+    hipLaunchKernelGGL(ob::max_reduce, dim3(1), dim3(64), 0, s, rpart, nblk,
+                       buf + 3 * (size_t)nblk);
+    OB_HIP(hipGetLastError());
+    OB_HIP(hipMemcpyAsync(rmax, buf + 3 * (size_t)nblk, sizeof(double), hipMemcpyDeviceToHost,
+                          s));
     OB_HIP(hipStreamSynchronize(s));
   } catch (...) {
     (void)hipFree(buf);

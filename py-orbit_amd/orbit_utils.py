@@ -1,5 +1,5 @@
 """BunchExtremaCalculator under the reference's module name `orbit_utils`
-(src/utils/bunch/BunchExtremaCalculator.cc:37-163, wrap_bunch_extrema_calculator.cc:44-92).
+(src/utils/bunch/BunchExtremaCalculator.cc:37-208, wrap_bunch_extrema_calculator.cc:44-109).
 
 The extrema are found on the device (ob_bunch_extrema) after the bunch's queued maps;
 the coordinates are not downloaded.  An empty bunch gives (DBL_MAX, -DBL_MAX) pairs, as in
@@ -31,3 +31,11 @@ class BunchExtremaCalculator:
         """(zMin, zMax)."""
         e = self._all(bunch)
         return float(e[8]), float(e[9])
+
+    def extremaR(self, bunch):
+        """Largest distance of a particle from the unweighted mean (x, y)."""
+        if not hasattr(bunch, "_h"):
+            raise TypeError("BunchExtremaCalculator: the input parameter is not a Bunch")
+        out = np.zeros(1)
+        N.call("ob_bunch_extrema_r", bunch._h, N.dptr(out))
+        return float(out[0])

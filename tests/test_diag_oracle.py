@@ -85,6 +85,19 @@ def test_extrema6_and_empty():
     assert np.all(e0[0::2] == np.finfo(float).max) and np.all(e0[1::2] == -np.finfo(float).max)
 
 
+def test_extrema_r_closed_form():
+    """Points on a circle of radius R around (a, b) plus points inside it: the largest
+    distance from the mean is R (the mean is the centre for the symmetric set)."""
+    R, a, b = 3e-3, 1e-3, -2e-3
+    t = np.arange(16) * (2 * np.pi / 16)
+    x = np.concatenate([a + R * np.cos(t), a + 0.5 * R * np.cos(t + 0.1)])
+    y = np.concatenate([b + R * np.sin(t), b + 0.5 * R * np.sin(t + 0.1)])
+    c = np.zeros((6, x.size))
+    c[0], c[2] = x, y
+    assert O.Bunch(c).extrema_r() == pytest.approx(R, rel=1e-13)
+    assert O.Bunch(np.zeros((6, 0))).extrema_r() == 0.0
+
+
 @pytest.mark.parametrize("axis", range(6))
 def test_bin_axis_is_grid1d_bin_of_that_column(axis):
     r = np.random.default_rng(axis)

@@ -148,6 +148,7 @@ def test_extrema_match_oracle_exactly(n):
     e = ob.extrema6()
     assert xyz == tuple(e[[0, 1, 4, 5, 8, 9]]) and pde == tuple(e[[2, 3, 6, 7, 10, 11]])
     assert calc.extremaZ(b) == (e[8], e[9])
+    assert calc.extremaR(b) == pytest.approx(ob.extrema_r(), rel=1e-14)
 
 
 def test_extrema_of_empty_bunch():
@@ -155,6 +156,7 @@ def test_extrema_of_empty_bunch():
     from orbit_utils import BunchExtremaCalculator
     big = np.finfo(float).max
     assert BunchExtremaCalculator().extremaXYZ(Bunch()) == (big, -big) * 3
+    assert BunchExtremaCalculator().extremaR(Bunch()) == 0.0
 
 
 @pytest.mark.parametrize("nz", [1, 2, 100, 4096, 4097])
