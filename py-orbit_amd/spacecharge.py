@@ -123,6 +123,16 @@ class Grid1D:
             return float(w0 * self._a[i0] + wp * self._a[ip])
         return float(self._a[0])
 
+    def calcGradient(self, z):
+        """Grid1D::calcGradient (Grid1D.cc:520-534): (arr[ip] - arr[i0]) / dz between the
+        two nodes z is binned to; 0 outside [zMin, zMax] and for a one-cell grid."""
+        if z < self._zmin or z > self._zmax:
+            return 0.0
+        if len(self._a) == 1:
+            return 0.0
+        i0, ip, _, _ = self._ind_wz(z)
+        return float((self._a[ip] - self._a[i0]) / self.getStepZ())
+
     def _bin(self, bunch, axis, by_particle):
         if not hasattr(bunch, "_h"):
             raise TypeError("Grid1D.binBunch: the first parameter must be a Bunch")

@@ -130,8 +130,9 @@ def test_grid1d_host_methods_match_oracle(nz):
         g.binValue(vv, zz)
     orc = O.orc_grid()
     assert np.array_equal(g.array(), orc.grid1d_bin(nz, zmin, zmax, z, v))
-    val, _ = orc.grid1d_value(nz, zmin, zmax, g.array(), z)
+    val, grad = orc.grid1d_value(nz, zmin, zmax, g.array(), z)
     assert np.array_equal(np.array([g.getValue(zz) for zz in z]), val)
+    assert np.array_equal(np.array([g.calcGradient(zz) for zz in z]), grad)
     assert [g.isInside(zz) for zz in z] == [int(zmin <= zz <= zmax) for zz in z]
     assert g.getGridZ(0) == zmin + 0.5 * (zmax - zmin) / nz and g.getSizeZ() == nz
 
