@@ -304,6 +304,10 @@ class Bunch:
         self._msize_dirty = True
         self._push_attrs()
 
+    def getMacroSizes(self):
+        """Bulk read of the per-particle macrosize attribute (None without it)."""
+        return None if self._msize is None else np.array(self._msize)
+
     def _push_attrs(self):
         if self._msize_dirty and self._msize is not None:
             N.call("ob_bunch_set_macrosizes", self._h, len(self._msize), N.dptr(self._msize))
@@ -325,8 +329,24 @@ class Bunch:
             other.setMacroSizes(self._msize)
 
     def addParticlesTo(self, other):
+        """Append this bunch's particles to `other` (Bunch.cc:339-402).
+
+        Both bunches must carry the same particle attributes (the reference aborts
+        otherwise, :352-388).  Attribute values follow the reference exactly: the appended
+        particles start at 0 (attrInit, :618-635, ParticleAttributes.cc:67-71), then the
+        source's value of particle i is written to the target's particle i for i < the
+        source's size (:390-401).  For an empty target that is a plain copy."""
+        if self.getPartAttrNames() != other.getPartAttrNames():
+            raise N.OrbitError("Bunch::addParticlesTo: the particles attributes have "
+                               "different structures for two bunches")
+        src_m = None if self._msize is None else np.array(self._msize)
+        tgt_m = None if other._msize is None else np.array(other._msize)
         a = other.getCoordinates()
         other.setCoordinates(np.concatenate([a, self.getCoordinates()], axis=1))
+        if src_m is not None:
+            m = np.concatenate([tgt_m, np.zeros(len(src_m))])
+            m[:len(src_m)] = src_m
+            other.setMacroSizes(m)
 
     def flush(self):
         """Apply queued maps now (they are otherwise applied lazily)."""

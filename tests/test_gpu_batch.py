@@ -171,3 +171,117 @@ def test_destroying_calculators_applies_pending_kicks():
         out = b2.getCoordinates()
         for j in range(6):
             assert rel_l2(out[j], ref1[j]) < 1e-9, (k, j)
+
+
+def _replay_each(bunch_coords, specs, cells, calls):
+    """Track each bunch alone on the oracle through the recorded map/kick sequence."""
+    import oracle as O
+    from helpers import replay_oracle
+    outs = []
+    for c, sp in zip(bunch_coords, specs):
+        ob = O.Bunch(c, macrosize=sp["intensity"] / c.shape[1], kin_energy=1.0)
+        replay_oracle(calls, ob, O.SC2p5Drb(*sp["grid"]))
+        outs.append(ob.coords)
+    return outs
+
+
+def _c3_specs():
+    """Bunches of the 64-bunch workload's kinds: Gaussian and KV, 50k to 2M particles,
+    grids 64^2, 128^2 and 256^2 with 64 z bins, intensities 1e12 to 1e14."""
+    from workloads import batch_specs
+    specs = batch_specs(64)
+    pick = []
+    for g in (64, 128, 256):
+        for kind in ("gauss", "kv"):
+            pick.append(next(s for s in specs if s["grid"][0] == g and s["kind"] == kind
+                             and all(s is not p for p in pick)))
+    sizes = [50_000, 2_000_000, 300_000, 120_000, 800_000, 1_500_000]
+    for s, n in zip(pick, sizes):
+        s["n"] = n
+    return pick
+
+
+def test_batch_c3_shapes_match_oracle(monkeypatch):
+    """Six bunches spanning every grid size and the size range of the 64-bunch workload,
+    batched, tracked two FODO cells (24 SC steps, hipGraph replay of the second) and
+    compared with the oracle tracking each bunch alone: 1e-9 relative L2."""
+    import oracle as O
+    import spacecharge
+    from batch import BunchBatch
+    from helpers import trace_lattice
+    from workloads import batch_coords, make_bunch
+    O.set_threads(16)
+    specs = _c3_specs()
+    calls = trace_lattice(monkeypatch, lambda calc: _lattice(calc, 1), "sc2p5drb")
+    monkeypatch.undo()
+    coords = [batch_coords(s) for s in specs]
+    bunches = [make_bunch(c, intensity=s["intensity"]) for c, s in zip(coords, specs)]
+    calcs = [spacecharge.SpaceChargeCalc2p5Drb(*s["grid"]) for s in specs]
+    bb = BunchBatch(bunches, calcs)
+    lat = _lattice(bb.calculator(), 1)
+    lat.trackBunch(bb.bunch())
+    bb.capture(lambda: lat.trackBunch(bb.bunch()))
+    bb.replay(1)
+    bb.synchronize()
+    ref = _replay_each(coords, specs, 2, calls + calls)
+    for k, (b, s) in enumerate(zip(bunches, specs)):
+        out = b.getCoordinates()
+        errs = [rel_l2(out[j], ref[k][j]) for j in range(6)]
+        print(f"bunch {k} {s['kind']} n={s['n']} grid={s['grid']}: "
+              + " ".join(f"{e:.1e}" for e in errs), flush=True)
+        for j in range(6):
+            assert errs[j] < 1e-9, (k, j, errs[j])
+
+
+def _dn_params(seed):
+    """Parameters with non-zero biases and head: the net changes rho visibly."""
+    import denoise_oracle as D
+    r = np.random.default_rng(seed)
+    p = D.he_init(seed)
+    off = 0
+    for ci, co, _ in D.LAYERS:
+        off += 9 * ci * co
+        p[off:off + co] = r.normal(0, 0.05, co)
+        off += co
+    p[-17:-1] = r.normal(0, 0.3, 16)
+    p[-1] = 0.1
+    return p.astype(np.float32)
+
+
+def test_batch_with_denoiser_equals_bunches_alone():
+    """Batched + denoised tracking (one denoiser in every SC node of every bunch, 256 x 256
+    x 64 grids) equals each bunch tracked alone with the same denoiser; the denoiser does
+    change the result (it differs from plain tracking)."""
+    import spacecharge
+    from batch import BunchBatch
+    from denoiser import Denoiser
+    from workloads import gaussian_coords, kv_coords, make_bunch
+    sizes = [200_000, 500_000, 120_000]
+    mk = [lambda n, s: gaussian_coords(n, seed=s), lambda n, s: kv_coords(n, seed=s),
+          lambda n, s: gaussian_coords(n, seed=s, sx=2e-3)]
+    coords = [f(n, 300 + k) for k, (f, n) in enumerate(zip(mk, sizes))]
+    dn = Denoiser(256, 256, max_batch=4)
+    dn.setParams(_dn_params(9))
+
+    def run(batched, denoise):
+        bunches = [make_bunch(c, intensity=1e13) for c in coords]
+        calcs = [spacecharge.SpaceChargeCalc2p5Drb(256, 256, 64) for _ in coords]
+        if denoise:
+            for c in calcs:
+                c.setDenoiser(dn)
+        if batched:
+            bb = BunchBatch(bunches, calcs)
+            _lattice(bb.calculator(), 1).trackBunch(bb.bunch())
+            bb.synchronize()
+        else:
+            for b, c in zip(bunches, calcs):
+                _lattice(c, 1).trackBunch(b)
+        return [b.getCoordinates() for b in bunches]
+
+    alone, batched, plain = run(False, True), run(True, True), run(True, False)
+    for k in range(len(coords)):
+        for j in range(6):
+            assert rel_l2(batched[k][j], alone[k][j]) < 1e-9, (k, j)
+        # the net's correction is visible in the tracked coordinates
+        dk = rel_l2(batched[k][1], plain[k][1])
+        assert dk > 1e-7, (k, dk)

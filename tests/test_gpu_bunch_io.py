@@ -82,3 +82,31 @@ def test_binary_checkpoint_is_exact(tmp_path):
     assert b2.getSyncParticle().momentum() == b.getSyncParticle().momentum()
     assert b2.getSyncParticle().kinEnergy() == b.getSyncParticle().kinEnergy()
     assert b2.bunchAttrDouble("ring_length") == 248.0
+
+
+def test_add_particles_to_keeps_reference_attribute_semantics():
+    """addParticlesTo (Bunch.cc:339-402): coordinates are appended; the appended particles'
+    macrosizes start at 0 and the source's value i lands on the target's particle i; an
+    empty target therefore gets a plain copy.  Different attribute sets are refused."""
+    from bunch import Bunch
+    from _obnative import OrbitError
+    src, cs = _bunch(n=5, seed=1)
+    dst, cd = _bunch(n=7, seed=2)
+    ms, md = np.array(src.getMacroSizes()), np.array(dst.getMacroSizes())
+    src.addParticlesTo(dst)
+    assert np.array_equal(dst.getCoordinates(), np.concatenate([cd, cs], axis=1))
+    want = np.concatenate([md, np.zeros(5)])
+    want[:5] = ms
+    assert np.array_equal(dst.getMacroSizes(), want)
+    empty = Bunch()
+    empty.addPartAttr("macrosize")
+    src.addParticlesTo(empty)
+    assert np.array_equal(empty.getCoordinates(), cs)
+    assert np.array_equal(empty.getMacroSizes(), ms)
+    plain, _ = _bunch(n=4, seed=3, msize=False)
+    with pytest.raises(OrbitError, match="different structures"):
+        src.addParticlesTo(plain)
+    n0 = plain.getSize()
+    plain2, c2 = _bunch(n=3, seed=4, msize=False)
+    plain2.addParticlesTo(plain)
+    assert plain.getSize() == n0 + 3 and not plain.hasPartAttr("macrosize")

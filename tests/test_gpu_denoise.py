@@ -30,11 +30,13 @@ def _images(seed, B=2, H=64, W=64):
     return x, y
 
 
-def test_forward_matches_oracle():
+@pytest.mark.parametrize("H,W,B", [(64, 64, 2), (256, 256, 2), (128, 192, 1)])
+def test_forward_matches_oracle(H, W, B):
+    """(256, 256) is the size of the denoised workloads' grids."""
     from denoiser import Denoiser
     p = _params(1)
-    x, _ = _images(2)
-    dn = Denoiser(64, 64, max_batch=2)
+    x, _ = _images(2, B=B, H=H, W=W)
+    dn = Denoiser(H, W, max_batch=B)
     dn.setParams(p)
     out = dn(x)
     ref, _ = D.Net(p, bf16=True).forward(x)
@@ -43,12 +45,12 @@ def test_forward_matches_oracle():
     assert np.max(np.abs(out - ref)) < 0.05 * np.max(np.abs(ref - x))
 
 
-@pytest.mark.parametrize("scales", [0, 3])
-def test_loss_and_gradients_match_oracle(scales):
+@pytest.mark.parametrize("scales,H,B", [(0, 64, 2), (3, 64, 2), (3, 256, 1), (0, 256, 2)])
+def test_loss_and_gradients_match_oracle(scales, H, B):
     from denoiser import Denoiser
     p = _params(3)
-    x, y = _images(4)
-    dn = Denoiser(64, 64, max_batch=2)
+    x, y = _images(4, B=B, H=H, W=H)
+    dn = Denoiser(H, H, max_batch=B)
     dn.setParams(p)
     dn.setLoss(scales, 0.5)
     loss, g = dn.lossAndGrad(x, y)
@@ -63,19 +65,21 @@ def test_loss_and_gradients_match_oracle(scales):
     assert rel_l2(g[off:], rg[off:]) < 1e-3  # head: FP32 throughout
 
 
-def test_pair_generator_matches_oracle():
+@pytest.mark.parametrize("H,n_low,n_high", [(64, 3000, 6000), (256, 100_000, 200_000)])
+def test_pair_generator_matches_oracle(H, n_low, n_high):
+    """(256, 100k) is the low-N side of the denoiser's training pairs."""
     from denoiser import PairSet
-    H = W = 64
-    ps = PairSet(H, W, npairs=4, n_low=3000, n_high=6000, seed=21,
+    W = H
+    ps = PairSet(H, W, npairs=4, n_low=n_low, n_high=n_high, seed=21,
                  families=(0, 1, 2, 3))
     xs, ys = ps.host(0, 4)
     for k in range(4):
         spec = tuple(float(v) for v in ps.specs[k])
         spec = (int(spec[0]),) + spec[1:]
-        lx, ly = D.pair_particles(spec, 21, k, 0, 3000)
+        lx, ly = D.pair_particles(spec, 21, k, 0, n_low)
         geo = D.square_grid(lx, ly, H, W)
         want_x = D.tsc_deposit(lx, ly, H, W, geo)
-        hx, hy = D.pair_particles(spec, 21, k, 1, 6000)
+        hx, hy = D.pair_particles(spec, 21, k, 1, n_high)
         want_y = D.tsc_deposit(hx, hy, H, W, geo)
         # FP32 device arithmetic: a few particles may fall on the other side of a node
         assert rel_l2(xs[k], want_x) < 3e-2, k

@@ -40,12 +40,16 @@ def sparse_coords(n_side=6, spacing_cells=5, nx=64, seed=1):
     return c
 
 
-@pytest.mark.parametrize("grid", [(16, 16, 8), (32, 32, 16), (33, 20, 5), (128, 128, 64)])
-def test_sc2p5drb_step(grid):
+@pytest.mark.parametrize("grid,n", [((16, 16, 8), 20000), ((32, 32, 16), 20000),
+                                    ((33, 20, 5), 20000), ((128, 128, 64), 20000),
+                                    ((256, 256, 64), 1_000_000), ((256, 128, 64), 500_000)])
+def test_sc2p5drb_step(grid, n):
+    """One SpaceChargeCalc2p5Drb step; (256, 256, 64) pads the solve to 512 x 512, the
+    largest grid of the batched and denoised workloads."""
     import spacecharge
     from workloads import gaussian_coords
     nx, ny, nz = grid
-    c = gaussian_coords(20000, seed=11)
+    c = gaussian_coords(n, seed=11)
     b, ob = _bunches(c)
     calc = spacecharge.SpaceChargeCalc2p5Drb(nx, ny, nz)
     ocalc = O.SC2p5Drb(nx, ny, nz)
@@ -148,6 +152,47 @@ def test_sc3d_step(grid):
         ob.coords[:] = out
 
 
+def test_sc3d_full_size_256x256x128():
+    """SpaceChargeCalc3D at its largest workload: 4M waterbag particles on 256 x 256 x 128
+    (the solve pads to 512 x 512 x 256, the Green function's real quadrant is
+    257 x 257 x 129).  Three steps, each from identical inputs on both sides: the first
+    builds the Green function, the second (after a drift) keeps it, the third (x widened
+    1.5 times: the xy ratio leaves the 1.2 limit) rebuilds it."""
+    import spacecharge
+    import teapot_base as TPB
+    from workloads import waterbag_coords
+    O.set_threads(16)
+    n = 4_000_000
+    c = waterbag_coords(n, seed=23)
+    b, ob = _bunches(c, macrosize=1e13 / n)
+    calc = spacecharge.SpaceChargeCalc3D(256, 256, 128)
+    ocalc = O.SC3D(256, 256, 128)
+    for step in range(3):
+        if step == 1:
+            TPB.drift(b, 0.5)
+        if step == 2:
+            cc = b.getCoordinates()
+            cc[0] *= 1.5
+            b.setCoordinates(cc)
+        before = b.getCoordinates()
+        ob.coords[:] = before
+        calc.trackBunch(b, 0.5)
+        ocalc.trackBunch(ob, 0.5)
+        after = b.getCoordinates()
+        rho, orho = calc.getRhoGrid(), ocalc.grid(0)
+        lims = [rho.getMinX(), rho.getMaxX(), rho.getMinY(), rho.getMaxY(), rho.getMinZ(),
+                rho.getMaxZ()]
+        assert np.array_equal(lims, orho[1]), step
+        assert rel_l2(rho.array(), orho[0]) < 1e-12, step
+        assert rel_l2(calc.getPhiGrid().array(), ocalc.grid(1)[0]) < 1e-12, step
+        assert calc.getGreenRebuilds() == ocalc.green_rebuilds() == [1, 1, 2][step]
+        for k in (1, 3, 5):
+            assert rel_l2(after[k] - before[k], ob.coords[k] - before[k]) < 1e-10, (step, k)
+        for k in (0, 2, 4):
+            assert np.array_equal(after[k], ob.coords[k]), (step, k)
+        print(f"3D 256x256x128 step {step}: rebuilds {calc.getGreenRebuilds()}", flush=True)
+
+
 @pytest.mark.parametrize("zscale,nb", [(0.05, 4), (1.0, 3)])
 def test_sc3d_external_bunches(zscale, nb):
     """numExtBunches > 0 (wrappedBunchAnalysis): z wrapped to one bunch spacing, image bunches
@@ -202,17 +247,23 @@ def test_sc3d_tiled_matches_particle_order_deposit(budget):
     c2 = spacecharge.SpaceChargeCalc3D(32, 32, 32)
     c1.setDepositMode(True, budget)
     c2.setDepositMode(False)
+    ocalc = O.SC3D(32, 32, 32)
     for _ in range(3):
+        before = b1.getCoordinates()
+        ob.coords[:] = before  # the oracle steps from the tiled path's input every time
         c1.trackBunch(b1, 0.5)
         c2.trackBunch(b2, 0.5)
+        ocalc.trackBunch(ob, 0.5)
         assert rel_l2(c1.getRhoGrid().array(), c2.getRhoGrid().array()) < 1e-13
         o1, o2 = b1.getCoordinates(), b2.getCoordinates()
         for k in range(6):
             assert rel_l2(o1[k], o2[k]) < 1e-12
-    ocalc = O.SC3D(32, 32, 32)
-    ocalc.trackBunch(ob, 0.5)
-    o1 = b1.getCoordinates()
-    assert np.array_equal(o1[0], b1.getCoordinates()[0])
+        assert rel_l2(c1.getRhoGrid().array(), ocalc.grid(0)[0]) < 1e-12
+        assert rel_l2(c1.getPhiGrid().array(), ocalc.grid(1)[0]) < 1e-12
+        for k in (1, 3, 5):
+            assert rel_l2(o1[k] - before[k], ob.coords[k] - before[k]) < 1e-10, k
+        for k in (0, 2, 4):
+            assert np.array_equal(o1[k], ob.coords[k])
 
 
 def test_poisson_solver_fft2d_find_potential():

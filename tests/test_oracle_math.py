@@ -582,3 +582,89 @@ def test_extrema_and_emittance_vs_numpy():
     for k in range(2):
         cov = np.cov(c[2 * k], c[2 * k + 1], bias=True)
         assert em[k] == pytest.approx(math.sqrt(np.linalg.det(cov)), rel=1e-9)
+
+
+# ---------------------------------------------------------------- quirk pins
+
+def test_sc2p5drb_moments_use_grid_x_for_y():
+    """SpaceChargeCalc2p5Drb.cc:229-254 forms the y moments with getGridX(iy) (:238): node
+    iy sits at xMin + iy dx for them.  The TSC weights of a particle at fraction f from
+    its node reproduce its position (sum w k = f) and add a quarter cell to its variance
+    (sum w k^2 = 1/4 + f^2), also at the clamped edge nodes, so for a beam inside the grid
+        x_c = <x>,    y_c = xMin + (dx/dy) (<y> - yMin),
+        a^2 = 2 (var x + dx^2/4 + (dx/dy)^2 (var y + dy^2/4)).
+    An off-centre beam on a 64 x 40 grid (dx != dy, xMin != yMin) tells this apart from
+    y_c = <y> by millimetres."""
+    c = _beam(20000, 2e-3, 1e-3, 7)
+    c[0] += 3e-3
+    c[2] -= 4e-3
+    calc = O.SC2p5Drb(64, 40, 8, 1.0)
+    calc.trackBunch(O.Bunch(c, macrosize=1e9, kin_energy=1.0), 0.1, 0.1)
+    _, (xmin, xmax, ymin, ymax) = calc.grid(0)
+    dx, dy = (xmax - xmin) / 63, (ymax - ymin) / 39
+    assert abs(dx / dy - 1.0) > 0.3 and abs(xmin - ymin) > 1e-3
+    xm, ym = c[0].mean(), c[2].mean()
+    x_c, y_c, a, total = calc.moments()
+    assert total == pytest.approx(20000 * 1e9, rel=1e-13)
+    assert x_c == pytest.approx(xm, rel=0, abs=1e-14 * (xmax - xmin))
+    want_yc = xmin + dx / dy * (ym - ymin)
+    assert y_c == pytest.approx(want_yc, rel=0, abs=1e-14 * (xmax - xmin))
+    assert abs(y_c - ym) > 1e-3
+    want_a2 = 2 * (c[0].var() + dx * dx / 4 + (dx / dy) ** 2 * (c[2].var() + dy * dy / 4))
+    assert a * a == pytest.approx(want_a2, rel=1e-10)
+
+
+def _box_beam(ax, ay, az_rest, seed, n=4000):
+    """A beam whose extrema are exactly +-ax, +-ay and +-az_rest / gamma in lab z."""
+    g = O.Bunch(np.zeros((6, 1)), kin_energy=1.0).sync["gamma"]
+    r = np.random.default_rng(seed)
+    c = np.zeros((6, n))
+    c[0] = r.uniform(-ax, ax, n)
+    c[2] = r.uniform(-ay, ay, n)
+    c[4] = r.uniform(-az_rest / g, az_rest / g, n)
+    c[1] = r.normal(0, 1e-3, n)
+    c[3] = r.normal(0, 1e-3, n)
+    c[5] = r.normal(0, 1e-3, n)
+    for k, (i, a) in enumerate([(0, ax), (2, ay), (4, az_rest / g)]):
+        c[i, 2 * k], c[i, 2 * k + 1] = -a, a
+    return c
+
+
+def test_sc3d_green_rebuild_sequence():
+    """SpaceChargeCalc3D.cc:186-203 rebuilds the Green function when the beam's x/y or x/z
+    (rest frame) extent ratio leaves [1/limit, limit] times the ratio of the last rebuild;
+    the solver starts at ratio 1 with limit 1.2 (:51).  Beams with hand-chosen extents and
+    the rebuild count each must give, derived by hand:
+      (2, 2, 2) mm       xy 1, xz 1 against 1, 1: inside                        -> 0
+      (2.2, 2, 2)        1.1, 1.1: inside                                        -> 0
+      (2.6, 2, 2)        1.3 > 1.2: rebuild, ratios now 1.3, 1.3                 -> 1
+      (2.6, 1.8, 2.3)    1.444/1.3 = 1.11, 1.130/1.3 = 0.87 > 1/1.2: inside      -> 1
+      (2.6, 2, 2.9)      xz 0.897/1.3 = 0.69 < 0.833: rebuild (xz alone)         -> 2
+      limit 2, (2.6, 1.2, 2.9)  xy 2.167/1.3 = 1.67 < 2: inside                 -> 2
+      limit 2, (2.6, 0.6, 2.9)  xy 4.333/1.3 = 3.33 > 2: rebuild                -> 3
+    Right after a rebuild the rho grid is the beam's own box; otherwise it keeps the
+    solver's aspect and covers the beam."""
+    g = O.Bunch(np.zeros((6, 1)), kin_energy=1.0).sync["gamma"]
+    calc = O.SC3D(16, 16, 16)
+    seq = [((2.0, 2.0, 2.0), 1.2, 0), ((2.2, 2.0, 2.0), 1.2, 0), ((2.6, 2.0, 2.0), 1.2, 1),
+           ((2.6, 1.8, 2.3), 1.2, 1), ((2.6, 2.0, 2.9), 1.2, 2), ((2.6, 1.2, 2.9), 2.0, 2),
+           ((2.6, 0.6, 2.9), 2.0, 3)]
+    last_box = (2.0, 2.0, 2.0)  # the initial solver box has unit ratios
+    for k, ((ax, ay, az), limit, want) in enumerate(seq):
+        calc.setRatioLimit(limit)
+        c = _box_beam(ax * 1e-3, ay * 1e-3, az * 1e-3, 60 + k)
+        calc.trackBunch(O.Bunch(c, macrosize=1e9, kin_energy=1.0), 0.1)
+        assert calc.green_rebuilds() == want, k
+        _, L = calc.grid(0)
+        wx, wy, wz = L[1] - L[0], L[3] - L[2], L[5] - L[4]
+        rebuilt = k == 0 and want == 1 or k > 0 and want != seq[k - 1][2]
+        if rebuilt:
+            last_box = (ax, ay, az)
+            assert (wx, wy, wz) == pytest.approx((2 * ax * 1e-3, 2 * ay * 1e-3, 2 * az * 1e-3),
+                                                 rel=1e-12)
+        else:
+            assert wx / wy == pytest.approx(last_box[0] / last_box[1], rel=1e-12)
+            assert wx / wz == pytest.approx(last_box[0] / last_box[2], rel=1e-12)
+            assert wx >= 2 * ax * 1e-3 * (1 - 1e-12) and wy >= 2 * ay * 1e-3 * (1 - 1e-12)
+            assert wz >= 2 * az * 1e-3 * (1 - 1e-12)
+    assert g > 2
