@@ -70,7 +70,11 @@ def test_sc2p5drb_step(grid, n):
     assert np.array_equal(zd, O.long_derivative(zg.array(), zg.getMinZ(), zg.getMaxZ()))
     assert rel_l2(zd, ocalc.grid(3)[0]) < 1e-9
     m, om = calc.getMoments(), ocalc.moments()
-    assert np.allclose(m, om, rtol=1e-12, atol=1e-18)
+    # x_c, y_c are sums of terms of the grid's size: the rounding scales with the span,
+    # not with the (near-zero) centre of a centred beam
+    span = orho[1][1] - orho[1][0]
+    assert np.allclose(m[:2], om[:2], rtol=0, atol=1e-13 * span)
+    assert np.allclose(m[2:], om[2:], rtol=1e-12, atol=0)
     for k in (1, 3):  # transverse kicks: from phi, bar 1e-10
         assert rel_l2(out[k] - c[k], ob.coords[k] - c[k]) < 1e-10, k
     # longitudinal kick is proportional to dlambda/dz (conditioning above)
@@ -156,7 +160,7 @@ def test_sc3d_full_size_256x256x128():
     """SpaceChargeCalc3D at its largest workload: 4M waterbag particles on 256 x 256 x 128
     (the solve pads to 512 x 512 x 256, the Green function's real quadrant is
     257 x 257 x 129).  Three steps, each from identical inputs on both sides: the first
-    builds the Green function, the second (after a drift) keeps it, the third (x widened
+    builds the Green function, the second (after a short drift) keeps it, the third (x widened
     1.5 times: the xy ratio leaves the 1.2 limit) rebuilds it."""
     import spacecharge
     import teapot_base as TPB
@@ -169,7 +173,7 @@ def test_sc3d_full_size_256x256x128():
     ocalc = O.SC3D(256, 256, 128)
     for step in range(3):
         if step == 1:
-            TPB.drift(b, 0.5)
+            TPB.drift(b, 0.05)  # x and y grow ~3%: both ratios stay within 1.2
         if step == 2:
             cc = b.getCoordinates()
             cc[0] *= 1.5
