@@ -92,6 +92,14 @@ int ob_bunch_get_macrosizes(ob_bunch* b, int64_t n, double* m);
  * w = the macrosize attribute or 1.  Dead particles are counted until compress(), as the
  * reference counts them until its own compress() call. */
 int ob_bunch_moments(ob_bunch* b, double* out);
+/* ob_bunch_moments without any device allocation (no counterpart in the reference): for
+ * callers that cannot allocate (code under HIP graph capture, tight per-turn monitoring
+ * loops, harnesses that forbid hipMalloc).  Same kernels and the same out[28] bit for bit;
+ * the partial sums use a fixed per-device scratch of 229,600 bytes that the library holds
+ * until the process ends, which fits every bunch size (at most 1024 workgroup partials).
+ * Calls that overlap on one device race on that scratch.  Refuses a bunch that belongs to
+ * a batch (use ob_bunch_moments). */
+int ob_bunch_moments_noalloc(ob_bunch* b, double* out);
 /* BunchExtremaCalculator::getExtremaXYZ and getExtremaXpYpdE
  * (src/utils/bunch/BunchExtremaCalculator.cc:37-120, wrap_bunch_extrema_calculator.cc) in
  * one device pass: out[12] = (min, max) of x, xp, y, yp, z, dE; DBL_MAX, -DBL_MAX for an

@@ -40,6 +40,7 @@ SIGNATURES = {
     "ob_bunch_set_macrosizes": [_vp, _i64, _D],
     "ob_bunch_get_macrosizes": [_vp, _i64, _D],
     "ob_bunch_moments": [_vp, _D],
+    "ob_bunch_moments_noalloc": [_vp, _D],
     "ob_bunch_extrema": [_vp, _D],
     "ob_bunch_extrema_r": [_vp, _D],
     "ob_grid1d_bin_bunch": [_vp, _i, _i, _i, _d, _d, _D],

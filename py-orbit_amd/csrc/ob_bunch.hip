@@ -51,6 +51,7 @@ static void init_device(int device) {
   hipDeviceProp_t prop;
   OB_HIP(hipGetDeviceProperties(&prop, device));
   g_cus = prop.multiProcessorCount;
+  diag_scratch_setup(device);
   g_device = device;
 }
 
@@ -297,6 +298,7 @@ DeviceBunchView bunch_device_view(ob_bunch* b) {
 }
 
 int64_t bunch_size(ob_bunch* b) { return b->n; }
+bool bunch_in_batch(ob_bunch* b) { return b->owner != nullptr; }
 
 SortBuffers bunch_sort_begin(ob_bunch* b) {
   upload(b);

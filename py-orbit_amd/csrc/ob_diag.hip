@@ -303,7 +303,52 @@ __global__ __launch_bounds__(128) void xy_moment_reduce(const double* __restrict
   out[k] = s;
 }
 
+// Scratch of ob_bunch_moments_noalloc: the per-workgroup partial sums (at most kMaxBlocks
+// rows of kSums) and their total, 229,600 bytes whatever the bunch size.
+constexpr size_t kScratchDoubles = (size_t)(kMaxBlocks + 1) * kSums;
+
 }  // namespace
+}  // namespace ob
+
+// Held by every device the library runs on until the process ends.  In host code this name
+// is a host-side stand-in, not a device address: host code uses the address looked up by
+// diag_scratch_setup for the current device.
+static __device__ __align__(256) double scratch[ob::kScratchDoubles];
+
+namespace ob {
+namespace {
+constexpr int kMaxDevices = 64;
+double* g_scratch_dev[kMaxDevices] = {};
+
+// Partial sums into buf[0 .. nblk * kSums), their total at buf + nblk * kSums, copied to
+// the host array out[kSums]; waits for the copy.  Every partial row and the total are
+// written before they are read, so buf needs no clearing.  timed: record the partials
+// kernel in the kernel profiler (which creates HIP events when it is on).
+void moments_into(const DeviceBunchView& v, int nblk, double* buf, double* out, hipStream_t s,
+                  bool timed) {
+  if (timed) {
+    ProfScope ps("moment_partials");
+    hipLaunchKernelGGL(moment_partials, dim3(nblk), dim3(kThreads), 0, s, v, buf);
+  } else {
+    hipLaunchKernelGGL(moment_partials, dim3(nblk), dim3(kThreads), 0, s, v, buf);
+  }
+  hipLaunchKernelGGL(moment_reduce, dim3(1), dim3(64), 0, s, buf, nblk,
+                     buf + (size_t)nblk * kSums);
+  OB_HIP(hipGetLastError());
+  OB_HIP(hipMemcpyAsync(out, buf + (size_t)nblk * kSums, sizeof(double) * kSums,
+                        hipMemcpyDeviceToHost, s));
+  OB_HIP(hipStreamSynchronize(s));
+}
+}  // namespace
+
+void diag_scratch_setup(int device) {
+  OB_REQUIRE(device >= 0 && device < kMaxDevices, "device index out of range");
+  if (g_scratch_dev[device] != nullptr) return;
+  double* p = nullptr;
+  OB_HIP(hipGetSymbolAddress((void**)&p, HIP_SYMBOL(scratch)));
+  OB_REQUIRE(p != nullptr, "no device address for the moments scratch");
+  g_scratch_dev[device] = p;
+}
 }  // namespace ob
 
 #define OB_API extern "C" __attribute__((visibility("default")))
@@ -332,21 +377,40 @@ OB_API int ob_bunch_moments(ob_bunch* b, double* out) {
   double* buf = nullptr;
   OB_HIP(hipMalloc((void**)&buf, sizeof(double) * (nblk + 1) * ob::kSums));
   try {
-    {
-      ob::ProfScope ps("moment_partials");
-      hipLaunchKernelGGL(ob::moment_partials, dim3(nblk), dim3(ob::kThreads), 0, s, v, buf);
-    }
-    hipLaunchKernelGGL(ob::moment_reduce, dim3(1), dim3(64), 0, s, buf, nblk,
-                       buf + (size_t)nblk * ob::kSums);
-    OB_HIP(hipGetLastError());
-    OB_HIP(hipMemcpyAsync(out, buf + (size_t)nblk * ob::kSums, sizeof(double) * ob::kSums,
-                          hipMemcpyDeviceToHost, s));
-    OB_HIP(hipStreamSynchronize(s));
+    ob::moments_into(v, nblk, buf, out, s, true);
   } catch (...) {
     (void)hipFree(buf);
     throw;
   }
   OB_HIP(hipFree(buf));
+  OB_CATCH
+}
+
+// ob_bunch_moments without any allocation: the partial sums go to the library's fixed
+// device scratch (229,600 bytes per device).  Calls that overlap on one device race on
+// that scratch.  A batched bunch is refused (its flush may rebuild the batch's records).
+OB_API int ob_bunch_moments_noalloc(ob_bunch* b, double* out) {
+  OB_TRY OB_REQUIRE(b != nullptr && out != nullptr, "null handle");
+  OB_REQUIRE(!ob::bunch_in_batch(b),
+             "ob_bunch_moments_noalloc: the bunch belongs to a batch; use ob_bunch_moments");
+  int dev = -1;
+  OB_HIP(hipGetDevice(&dev));
+  OB_REQUIRE(dev >= 0 && dev < ob::kMaxDevices && ob::g_scratch_dev[dev] != nullptr,
+             "ob_bunch_moments_noalloc: the library is not initialised on this device");
+  ob::bunch_flush(b, nullptr);
+  const ob::DeviceBunchView v = ob::bunch_device_view(b);
+  if (v.n == 0) {
+    for (int k = 0; k < ob::kSums; k++) out[k] = 0.0;
+    return 0;
+  }
+  const int nblk =
+      (int)std::min<int64_t>(ob::kMaxBlocks, (v.n + ob::kThreads - 1) / ob::kThreads);
+  const size_t need = (size_t)(nblk + 1) * ob::kSums;
+  OB_REQUIRE(need <= ob::kScratchDoubles,
+             "ob_bunch_moments_noalloc: " + std::to_string(need) +
+                 " doubles of scratch needed, the maximum is " +
+                 std::to_string(ob::kScratchDoubles));
+  ob::moments_into(v, nblk, ob::g_scratch_dev[dev], out, ob::stream(), false);
   OB_CATCH
 }
 

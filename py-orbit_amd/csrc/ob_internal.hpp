@@ -36,6 +36,9 @@ hipStream_t stream();
 hipStream_t aux_stream();
 int device_cu_count();
 void ensure_init();
+// Look up, once per device and with that device current, the device address of the
+// fixed scratch array of the no-allocation entry points (ob_diag.hip).
+void diag_scratch_setup(int device);
 
 // Kernel timing with HIP events on the library stream (ob_profile_enable/_report).  A
 // ProfScope brackets one launch when profiling is on and costs nothing otherwise.
@@ -278,6 +281,8 @@ void bunch_queue(ob_bunch* b, const Op& op);
 // Apply queued ops; if ext_partials != nullptr compute partial extrema too.
 int bunch_flush(ob_bunch* b, double* ext_partials);
 int64_t bunch_size(ob_bunch* b);
+// Whether the bunch belongs to a batch (its flush then applies the batch's work).
+bool bunch_in_batch(ob_bunch* b);
 // Tile sorting of the device arrays: begin returns the current and spare buffers (the
 // permutation is initialised to the identity when the bunch is in user order); the caller
 // writes the reordered particles into the spare buffers and commits, which swaps them in.
