@@ -58,6 +58,8 @@ def parse():
                     help="denoise: ring turns whose SC nodes add training pairs")
     ap.add_argument("--n", type=int, default=0, help="override particle count")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
+    ap.add_argument("--no-accuracy", action="store_true",
+                    help="ring3d: skip the 4N reference run (emittance growth, field error)")
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--profile-steps", type=int, default=2)
     return ap.parse_args()
@@ -190,48 +192,105 @@ def profile(N, lat, b, cells, steps):
     return rows
 
 
+# teapot_base maps whose trailing useCharge argument the reference ignores (the oracle
+# takes the arguments without it)
+_ORACLE_NARGS = {"quad1": 2, "kick": 3, "multp": 3, "multpfringeIN": 3, "multpfringeOUT": 3,
+                 "quadfringeIN": 1, "quadfringeOUT": 1, "soln": 2}
+
+
+def lattice_program(workload):
+    """The workload's SC intervals as the lattice layer issues them: a list of
+    (maps, sc_args), maps being the teapot_base calls (name, *args) in order and sc_args the
+    calculator's trackBunch arguments.  Recorded once by running the lattice with stand-ins
+    for teapot_base and the calculator (no device work)."""
+    import orbit.teapot.teapot as tt
+    from orbit.space_charge.sc2p5d import setSC2p5DrbAccNodes
+    from orbit.space_charge.sc3d import setSC3DAccNodes
+    from workloads import PIPE_RADIUS, fodo_lattice, ring_lattice
+    calls = []
+
+    class Maps:
+        def __getattr__(self, name):
+            return lambda bunch, *a: calls.append((name,) + a)
+
+    class Calc:
+        def trackBunch(self, bunch, *a):
+            calls.append(("SC",) + a)
+
+    class Stand:
+        def hasBunchAttrInt(self, name):
+            return 0
+
+    saved = tt.TPB
+    tt.TPB = Maps()
+    try:
+        if workload == "ring3d":
+            lat = ring_lattice(24)
+            setSC3DAccNodes(lat, 0.49, Calc())
+        else:
+            lat = fodo_lattice(1)
+            setSC2p5DrbAccNodes(lat, 0.49, Calc(), PIPE_RADIUS)
+        lat.trackBunch(Stand())
+    finally:
+        tt.TPB = saved
+    out, cur = [], []
+    for c in calls:
+        if c[0] == "SC":
+            out.append((cur, c[1:]))
+            cur = []
+        else:
+            cur.append((c[0],) + tuple(c[1:1 + _ORACLE_NARGS.get(c[0], len(c) - 1)]))
+    return out
+
+
 def cpu_baseline(args, n, grid, coords):
-    """The CPU oracle on a bounded sample of the same workload: SC intervals (the maps of
-    one interval plus one SC step) repeated until about 10 s of CPU work, after one
-    untimed step that builds the Green function.  Threads: all host cores (OpenMP,
-    particle loops and FFT lines)."""
+    """The CPU oracle on a bounded sample of the same workload: the lattice's SC intervals
+    in order (every TEAPOT map of the interval, then the SC step), after one untimed
+    interval that builds the Green function, until about 10 s of CPU work.  Threads: all
+    host cores (OpenMP particle loops); the 3D transforms run through scipy.fft
+    (pocketfft, the same threads) in place of the oracle's radix-2 FFT."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
-    import numpy as np
     import oracle as O
     threads = args.cpu_threads or len(os.sched_getaffinity(0))
     O.set_threads(threads)
+    prog = lattice_program(args.workload)
+    ob = O.Bunch(coords.copy(), macrosize=1e13 / n, kin_energy=1.0)
     if args.workload == "ring3d":
-        ns = n
-        ob = O.Bunch(coords[:, :ns].copy(), macrosize=1e13 / ns, kin_energy=1.0)
         calc = O.SC3D(*grid)
-        ob.apply("drift", 0.5)
-        calc.trackBunch(ob, 0.5)
-        t0 = time.perf_counter()
-        reps = 0
-        while reps < 30 and (reps == 0 or time.perf_counter() - t0 < 10.0):
-            ob.apply("drift", 0.5)
-            calc.trackBunch(ob, 0.5)
-            reps += 1
-        total = time.perf_counter() - t0
-        dt = total / reps
-        sample = (f"{ns} particles, {reps} SC intervals (each drift 0.5 m + SpaceChargeCalc3D"
-                  f"{grid} step incl. 2x{2*grid[0]}x{2*grid[1]}x{2*grid[2]} FFTs), "
-                  f"after one untimed step")
+        fft = O.use_library_fft3(threads)
+        desc = f"SpaceChargeCalc3D{tuple(grid)}"
     else:
-        ns = n
-        ob = O.Bunch(coords[:, :ns].copy(), macrosize=1e13 / ns, kin_energy=1.0)
         calc = O.SC2p5Drb(*grid)
-        calc.trackBunch(ob, 0.5, 0.1)
-        t0 = time.perf_counter()
-        reps = 20
-        for _ in range(reps):
-            ob.apply("drift", 0.5)
-            calc.trackBunch(ob, 0.5, 0.1)
-        total = time.perf_counter() - t0
-        dt = total / reps
-        sample = f"{ns} particles, 20 SC intervals (drift 0.5 m + SpaceChargeCalc2p5Drb{grid} step)"
-    return {"value": ns / dt, "unit": "particle-pushes/s", "cores": threads, "kind": "port",
-            "sample": sample, "seconds": total}
+        fft = None
+        desc = f"SpaceChargeCalc2p5Drb{tuple(grid)}"
+
+    def interval(k):
+        maps, sc = prog[k % len(prog)]
+        for m in maps:
+            ob.apply(*m)
+        calc.trackBunch(ob, *sc)
+        return len(maps)
+
+    interval(0)
+    if fft is not None:
+        fft.reset()
+    t0 = time.perf_counter()
+    reps, nmaps = 0, 0
+    while reps < 20000 and (reps < 2 or time.perf_counter() - t0 < 10.0):
+        nmaps += interval(1 + reps)
+        reps += 1
+    total = time.perf_counter() - t0
+    out = {"value": n * reps / total, "unit": "particle-pushes/s", "cores": threads,
+           "kind": "port",
+           "sample": (f"{n} particles, {reps} consecutive SC intervals of the lattice "
+                      f"({nmaps} TEAPOT maps and {reps} {desc} steps) after one untimed "
+                      f"interval"),
+           "seconds": round(total, 3)}
+    if fft is not None:
+        out["fft"] = f"scipy.fft (pocketfft), {threads} workers"
+        out["fft_share"] = round(fft.seconds / total, 3)
+        O.use_library_fft3(0)
+    return out
 
 
 def main_batch(args):
@@ -501,6 +560,56 @@ def track_with_denoiser(args, dn, H, W, n_low, n_high):
     return res
 
 
+def emittances(b):
+    from bunch import BunchTwissAnalysis
+    tw = BunchTwissAnalysis()
+    tw.analyzeBunch(b)
+    return tw.getEmittance(0), tw.getEmittance(1)
+
+
+def accuracy_vs_high_n(args, n, grid, ncells, e0, e1, halo):
+    """The accuracy half of the metric: the same workload at 4N particles (a waterbag of the
+    same seed and parameters, FP64, the same number of ring cells) is the reference.
+      - rms emittance growth over the tracked cells at N and at 4N, and their difference;
+      - field at one SC node: the 4N bunch after its cells and every 4th of its particles
+        (an N-particle bunch of the same total charge) each get one SpaceChargeCalc3D kick;
+        the kicks of the common particles (proportional to the field there) are compared,
+        rel-L2 per component;
+      - the fraction beyond 4 rms sizes at N and at 4N."""
+    import numpy as np
+    import spacecharge
+    from orbit.space_charge.sc3d import setSC3DAccNodes
+    from workloads import make_bunch, ring_lattice, waterbag_coords
+    nh = 4 * n
+    bh = make_bunch(waterbag_coords(nh))
+    lat = ring_lattice(24)
+    setSC3DAccNodes(lat, 0.49, spacecharge.SpaceChargeCalc3D(*grid))
+    f0 = emittances(bh)
+    for c in range(ncells):
+        i0 = 6 * (c % 24)
+        lat.trackBunch(bh, index_start=i0, index_stop=i0 + 5)
+    f1 = emittances(bh)
+    halo_h = beam_stats(bh)
+    ch = bh.getCoordinates()
+    del bh
+    kicks = []
+    for cc in (ch, np.ascontiguousarray(ch[:, ::4])):
+        b = make_bunch(cc)
+        spacecharge.SpaceChargeCalc3D(*grid).trackBunch(b, 0.5)
+        kicks.append(b.getCoordinates()[[1, 3, 5]] - cc[[1, 3, 5]])
+        del b
+    ref = kicks[0][:, ::4]
+    field = {k: float(np.linalg.norm(kicks[1][j] - ref[j]) / np.linalg.norm(ref[j]))
+             for j, k in enumerate(("x", "y", "z"))}
+    g = {"N": [e1[0] / e0[0] - 1.0, e1[1] / e0[1] - 1.0],
+         "4N": [f1[0] / f0[0] - 1.0, f1[1] / f0[1] - 1.0]}
+    g["err"] = [g["N"][0] - g["4N"][0], g["N"][1] - g["4N"][1]]
+    return {"reference": f"same workload, {nh} particles, FP64", "cells": ncells,
+            "eps_growth_xy": g, "field_rel_l2_vs_4N": field,
+            "halo_4sigma_xy": {"N": [halo["halo_4sigma_x"], halo["halo_4sigma_y"]],
+                               "4N": [halo_h["halo_4sigma_x"], halo_h["halo_4sigma_y"]]}}
+
+
 def main():
     args = parse()
     if args.workload == "batch64" and args.gpus == 1:
@@ -514,6 +623,7 @@ def main():
         sys.exit(2)
     import _obnative as N
     b, lat, calc, cells, n, grid, coords = build_workload(args)
+    e0 = emittances(b)
     N.synchronize()
     run_cells(lat, b, cells, 0, args.warmup)
     N.synchronize()
@@ -536,6 +646,12 @@ def main():
     kernels = {k: {"count": v[0], "avg_us": round(v[1] / v[0] * 1e3, 2),
                    "share": round(v[1] / sum(x[1] for x in prof.values()), 4)}
                for k, v in sorted(prof.items(), key=lambda kv: -kv[1][1])}
+    e1 = emittances(b)
+    beam = beam_stats(b)
+    acc = None
+    if args.workload == "ring3d" and not args.no_accuracy:
+        acc = accuracy_vs_high_n(args, n, grid, args.warmup + args.steps + args.profile_steps,
+                                 e0, e1, beam)
     cpu = None
     if not args.no_cpu:
         cpu = cpu_baseline(args, n, grid, coords)
@@ -559,7 +675,8 @@ def main():
         "roofline": roofline,
         "cpu_baseline": cpu,
         "kernels": kernels,
-        "beam": beam_stats(b),
+        "beam": dict(beam, emit0_x=e0[0], emit0_y=e0[1]),
+        "accuracy": acc,
     }
     print(json.dumps(out))
 

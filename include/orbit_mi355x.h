@@ -387,6 +387,11 @@ int ob_profile_report(char* buf, int buflen);
 /* A plain streaming copy of n doubles (8n bytes read, 8n written) timed over reps launches
  * (no reference counterpart): the yardstick for the hardware byte counters. */
 int ob_calibrate_copy(int64_t n, int reps, double* ms_per_copy);
+/* The best streaming copy this library reaches (no reference counterpart): 16-byte loads,
+ * 4 or 8 in flight per thread, plain or non-temporal stores, grids of 1024-8192
+ * workgroups; ms per copy of n doubles (n even) and the variant index
+ * 4 * grid + 2 * non-temporal + (8 in flight). */
+int ob_calibrate_copy_wide(int64_t n, int reps, double* ms_per_copy, int* variant);
 
 #ifdef __cplusplus
 }

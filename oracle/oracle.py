@@ -8,6 +8,7 @@ measured.  The product (py-orbit_amd/) never imports this module.
 import ctypes as C
 import os
 import subprocess
+import time
 
 import numpy as np
 
@@ -77,6 +78,7 @@ def lib():
             "orc_fft_c2c": (None, [_D, _D, i, i]),
             "orc_fft_r2c_2d": (None, [_D, i, i, _D]), "orc_fft_c2r_2d": (None, [_D, i, i, _D]),
             "orc_fft_r2c_3d": (None, [_D, i, i, i, _D]), "orc_fft_c2r_3d": (None, [_D, i, i, i, _D]),
+            "orc_set_fft3_hook": (None, [C.c_void_p, C.c_void_p]),
             "orc_sc2p5drb_create": (vp, [i, i, i, d]), "orc_sc2p5drb_destroy": (None, [vp]),
             "orc_sc2p5drb_set_long_avg_n": (None, [vp, i]),
             "orc_sc2p5drb_track": (i, [vp, bp, d, d]),
@@ -490,6 +492,55 @@ def fft_c2r_2d(s, n1):
     out = np.zeros((n0, n1))
     lib().orc_fft_c2r_2d(_dp(buf), n0, n1, _dp(out))
     return out
+
+
+_FFT3 = C.CFUNCTYPE(None, _D, C.c_int, C.c_int, C.c_int, _D)
+_fft3_keep = None
+
+
+class _LibraryFFT3:
+    """3D transforms through scipy.fft (pocketfft) with `threads` workers; `seconds`
+    accumulates the time spent in them."""
+
+    def __init__(self, threads):
+        import scipy.fft as sf
+        self.seconds = 0.0
+
+        def r2c(src, n0, n1, n2, dst):
+            t = time.perf_counter()
+            a = np.ctypeslib.as_array(src, shape=(n0, n1, n2))
+            out = np.ctypeslib.as_array(dst, shape=(n0, n1, n2 // 2 + 1, 2))
+            out.view(np.complex128)[..., 0] = sf.rfftn(a, workers=threads)
+            self.seconds += time.perf_counter() - t
+
+        def c2r(src, n0, n1, n2, dst):
+            t = time.perf_counter()
+            z = np.ctypeslib.as_array(src, shape=(n0, n1, n2 // 2 + 1, 2))
+            out = np.ctypeslib.as_array(dst, shape=(n0, n1, n2))
+            out[...] = sf.irfftn(z.view(np.complex128)[..., 0], s=(n0, n1, n2), workers=threads)
+            out *= float(n0) * n1 * n2
+            self.seconds += time.perf_counter() - t
+
+        self.fns = (_FFT3(r2c), _FFT3(c2r))
+
+    def reset(self):
+        self.seconds = 0.0
+
+
+def use_library_fft3(threads):
+    """Run the oracle's 3D transforms through scipy.fft (pocketfft, `threads` workers) in
+    place of its own radix-2 FFT and return the hook (its `seconds` is the time spent in
+    the transforms); threads = 0 restores the oracle's FFT.  For timing the CPU baseline
+    only: every check of the oracle runs its own FFT."""
+    global _fft3_keep
+    if not threads:
+        lib().orc_set_fft3_hook(None, None)
+        _fft3_keep = None
+        return None
+    _fft3_keep = _LibraryFFT3(threads)
+    lib().orc_set_fft3_hook(C.cast(_fft3_keep.fns[0], C.c_void_p),
+                            C.cast(_fft3_keep.fns[1], C.c_void_p))
+    return _fft3_keep
 
 
 def fft_r2c_3d(a):

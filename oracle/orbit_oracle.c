@@ -1145,13 +1145,30 @@ void orc_fft_c2r_2d(const double* in, int n0, int n1, double* out) {
   c2r_lines(tmp, n0, n1, out);
   free(tmp);
 }
+/* Optional replacement of the two 3D transforms (same layout and conventions: FFTW's
+ * row-major r2c/c2r, unnormalised).  Only the CPU-baseline timing installs one, to time the
+ * oracle with a multi-threaded library FFT of the FFTW class in place of fft_core; the
+ * tests never do. */
+static orc_fft3_fn g_r2c3_hook = NULL, g_c2r3_hook = NULL;
+void orc_set_fft3_hook(orc_fft3_fn r2c, orc_fft3_fn c2r) {
+  g_r2c3_hook = r2c;
+  g_c2r3_hook = c2r;
+}
 void orc_fft_r2c_3d(const double* in, int n0, int n1, int n2, double* out) {
+  if (g_r2c3_hook) {
+    g_r2c3_hook(in, n0, n1, n2, out);
+    return;
+  }
   int h = n2 / 2 + 1;
   r2c_lines(in, (int64_t)n0 * n1, n2, out);
   c2c_strided(out, n0, h, n1, 0);     /* along axis 1 */
   c2c_strided(out, 1, (int64_t)n1 * h, n0, 0); /* along axis 0 */
 }
 void orc_fft_c2r_3d(const double* in, int n0, int n1, int n2, double* out) {
+  if (g_c2r3_hook) {
+    g_c2r3_hook(in, n0, n1, n2, out);
+    return;
+  }
   int h = n2 / 2 + 1;
   size_t sz = (size_t)n0 * n1 * h * 2;
   double* tmp = (double*)malloc(sizeof(double) * sz);

@@ -133,6 +133,10 @@ void orc_fft_r2c_2d(const double* in, int n0, int n1, double* out_cplx /* [n0][n
 void orc_fft_c2r_2d(const double* in_cplx, int n0, int n1, double* out);
 void orc_fft_r2c_3d(const double* in, int n0, int n1, int n2, double* out_cplx);
 void orc_fft_c2r_3d(const double* in_cplx, int n0, int n1, int n2, double* out);
+/* Replace both 3D transforms by caller functions with the same signature and conventions
+ * (NULL, NULL restores the oracle's own FFT). */
+typedef void (*orc_fft3_fn)(const double* in, int n0, int n1, int n2, double* out);
+void orc_set_fft3_hook(orc_fft3_fn r2c, orc_fft3_fn c2r);
 
 /* ---- PoissonSolverFFT2D (src/spacecharge/PoissonSolverFFT2D.cc) ---- */
 typedef struct orc_poisson2d orc_poisson2d;

@@ -146,6 +146,7 @@ SIGNATURES = {
     "ob_profile_enable": [_i],
     "ob_profile_report": [C.c_char_p, _i],
     "ob_calibrate_copy": [_i64, _i, _D],
+    "ob_calibrate_copy_wide": [_i64, _i, _D, C.POINTER(C.c_int)],
 }
 _RET = {"ob_last_error": C.c_char_p, "ob_version": C.c_char_p}
 

@@ -1139,7 +1139,85 @@ __global__ __launch_bounds__(256) void copy_kernel(const double* __restrict__ sr
   for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride)
     dst[i] = src[i];
 }
+// The wider yardstick: 16-byte loads, U of them in flight per thread before the stores;
+// NT: non-temporal stores (they bypass the L2 allocate on write).
+template <int U, bool NT>
+__global__ __launch_bounds__(256) void copy_wide_kernel(const double2* __restrict__ src,
+                                                        double2* __restrict__ dst, int64_t n2) {
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  for (; i + (U - 1) * stride < n2; i += U * stride) {
+    double2 v[U];
+#pragma unroll
+    for (int u = 0; u < U; u++) v[u] = src[i + u * stride];
+#pragma unroll
+    for (int u = 0; u < U; u++) {
+      if constexpr (NT) {
+        __builtin_nontemporal_store(v[u].x, &dst[i + u * stride].x);
+        __builtin_nontemporal_store(v[u].y, &dst[i + u * stride].y);
+      } else {
+        dst[i + u * stride] = v[u];
+      }
+    }
+  }
+  for (; i < n2; i += stride) dst[i] = src[i];
+}
 }  // namespace
+
+// Best of the wide copy variants over a sweep of grid sizes: ms per copy of n doubles
+// (n even) and the variant that reached it (*variant = 4 * grid-size index + 2 * NT + (U==8)).
+OB_API int ob_calibrate_copy_wide(int64_t n, int reps, double* ms_per_copy, int* variant) {
+  OB_TRY OB_REQUIRE(n > 0 && n % 2 == 0 && reps > 0, "n must be positive and even, reps > 0");
+  ob::ensure_init();
+  double *a = nullptr, *b2 = nullptr;
+  OB_HIP(hipMalloc((void**)&a, sizeof(double) * n));
+  OB_HIP(hipMalloc((void**)&b2, sizeof(double) * n));
+  hipStream_t s = ob::stream();
+  const int64_t n2 = n / 2;
+  hipLaunchKernelGGL(copy_kernel, dim3(1024), dim3(256), 0, s, b2, a, n);  // touch
+  hipEvent_t e0, e1;
+  OB_HIP(hipEventCreate(&e0));
+  OB_HIP(hipEventCreate(&e1));
+  const int grids[4] = {1024, 2048, 4096, 8192};
+  double best = 1e30;
+  int best_v = -1;
+  for (int gi = 0; gi < 4; gi++)
+    for (int v = 0; v < 4; v++) {
+      const int blocks = (int)std::min<int64_t>((n2 + 255) / 256, grids[gi]);
+      auto launch = [&]() {
+        const double2* src = (const double2*)a;
+        double2* dst = (double2*)b2;
+        switch (v) {
+          case 0: hipLaunchKernelGGL((copy_wide_kernel<4, false>), dim3(blocks), dim3(256), 0, s, src, dst, n2); break;
+          case 1: hipLaunchKernelGGL((copy_wide_kernel<8, false>), dim3(blocks), dim3(256), 0, s, src, dst, n2); break;
+          case 2: hipLaunchKernelGGL((copy_wide_kernel<4, true>), dim3(blocks), dim3(256), 0, s, src, dst, n2); break;
+          default: hipLaunchKernelGGL((copy_wide_kernel<8, true>), dim3(blocks), dim3(256), 0, s, src, dst, n2); break;
+        }
+      };
+      launch();  // warm
+      OB_HIP(hipEventRecord(e0, s));
+      for (int r = 0; r < reps; r++) {
+        ob::ProfScope ps("copy_wide_kernel");
+        launch();
+      }
+      OB_HIP(hipEventRecord(e1, s));
+      OB_HIP(hipEventSynchronize(e1));
+      float ms = 0.f;
+      OB_HIP(hipEventElapsedTime(&ms, e0, e1));
+      if (ms / reps < best) {
+        best = ms / reps;
+        best_v = 4 * gi + v;
+      }
+    }
+  OB_HIP(hipGetLastError());
+  *ms_per_copy = best;
+  *variant = best_v;
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  (void)hipFree(a);
+  (void)hipFree(b2);
+  OB_CATCH
+}
 
 OB_API int ob_calibrate_copy(int64_t n, int reps, double* ms_per_copy) {
   OB_TRY OB_REQUIRE(n > 0 && reps > 0, "n and reps must be positive");
