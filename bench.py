@@ -57,6 +57,8 @@ def parse():
     ap.add_argument("--capture-turns", type=int, default=5,
                     help="denoise: ring turns whose SC nodes add training pairs")
     ap.add_argument("--n", type=int, default=0, help="override particle count")
+    ap.add_argument("--intensity", type=float, default=0.0,
+                    help="protons per bunch (default: the workload's)")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--no-accuracy", action="store_true",
                     help="ring3d: skip the 4N reference run (emittance growth, field error)")
@@ -69,7 +71,7 @@ def build_workload(args):
     import spacecharge
     from orbit.space_charge.sc2p5d import setSC2p5DrbAccNodes
     from orbit.space_charge.sc3d import setSC3DAccNodes
-    from workloads import (PIPE_RADIUS, fodo_lattice, gaussian_coords, make_bunch,
+    from workloads import (INTENSITY, PIPE_RADIUS, fodo_lattice, gaussian_coords, make_bunch,
                            ring_lattice, waterbag_coords)
     if args.workload == "ring3d":
         n = args.n or 4_000_000
@@ -87,7 +89,7 @@ def build_workload(args):
         calc = spacecharge.SpaceChargeCalc2p5Drb(*grid)
         scn = setSC2p5DrbAccNodes(lat, 0.49, calc, PIPE_RADIUS)
         cell_len = 4
-    b = make_bunch(coords)
+    b = make_bunch(coords, intensity=args.intensity or INTENSITY)
     # SC nodes per cell (children of the cell's first-level nodes)
     nodes = lat.getNodes()
     cells = []

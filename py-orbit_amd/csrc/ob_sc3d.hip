@@ -37,7 +37,12 @@ __host__ __device__ constexpr int strided_threads(int n) { return n >= 256 ? 512
 
 // CONV2: the x pass of a stack of independent 2D solves (SpaceChargeCalcSliceBySlice2D):
 // the Green spectrum is 2D, ghat[kx * qy + ky], the same for every line.
-enum StridedMode { FWD = 0, INV = 1, CONV = 2, CONV2 = 3 };
+// GREEN: an even-symmetric transform along one axis of the real Green quadrant.  Two
+// adjacent kz lines (2c, 2c+1) read as one double2 are the complex line a + i b; the
+// N/2+1 stored samples are extended evenly to N points, and the first N/2+1 outputs
+// A + i B (A, B real: the transforms of even real lines) go back as the same double2.
+// n_in = the number of real kz lines; a partner line at or past it is padding, zero.
+enum StridedMode { FWD = 0, INV = 1, CONV = 2, CONV2 = 3, GREEN = 4 };
 
 // LDS doubles per line (re or im) of a strided block: the larger of the forward and the
 // reversed plan's padded layouts.
@@ -298,11 +303,11 @@ using StridedFwd = RegPlan<N, false, strided_lines(N), strided_threads(N)>;
 // Global -> registers in the first-stage layout of the row's plan.
 template <int MODE, int N>
 __device__ __forceinline__ void strided_load(const double2* __restrict__ base, int nl, int n_in,
-                                             int64_t line_stride,
+                                             int pair0, int64_t line_stride,
                                              double (&vr)[StridedIn<MODE, N>::B0][StridedIn<MODE, N>::R0],
                                              double (&vi)[StridedIn<MODE, N>::B0][StridedIn<MODE, N>::R0]) {
   constexpr int L = strided_lines(N), NT = strided_threads(N);
-  constexpr bool HALF_IN = (MODE != INV);
+  constexpr bool HALF_IN = (MODE != INV && MODE != GREEN);
   using P0 = StridedIn<MODE, N>;
 #pragma unroll
   for (int q = 0; q < P0::B0; q++) {
@@ -313,7 +318,16 @@ __device__ __forceinline__ void strided_load(const double2* __restrict__ base, i
     for (int m = 0; m < P0::R0; m++) {
       const int p = j + m * P0::NB0;
       double2 v = make_double2(0.0, 0.0);
-      if (!(HALF_IN && 2 * m >= P0::R0) && ok && p < n_in) v = base[(int64_t)p * line_stride + line];
+      if constexpr (MODE == GREEN) {  // even extension: point p holds sample min(p, N - p)
+        const int pp = p <= N / 2 ? p : N - p;
+        if (ok) v = base[(int64_t)pp * line_stride + line];
+        // n_in = real lines: a partner line past the last one is padding, read as zero
+        // (else the rounding noise left there would grow by ~N^2 with every rebuild)
+        if (2 * (pair0 + line) + 1 >= n_in) v.y = 0.0;
+      } else {
+        if (!(HALF_IN && 2 * m >= P0::R0) && ok && p < n_in)
+          v = base[(int64_t)p * line_stride + line];
+      }
       vr[q][m] = v.x;
       vi[q][m] = v.y;
     }
@@ -356,6 +370,21 @@ __device__ __forceinline__ void strided_compute(
         for (int m = 0; m < PI::RL / 2; m++) {
           const int p = j + m * PI::NBL;
           if (p < n_out) base[(int64_t)p * line_stride + line] = make_double2(wr[q][m], wi[q][m]);
+        }
+      }
+    }
+  } else if constexpr (MODE == GREEN) {  // outputs p <= N/2
+#pragma unroll
+    for (int q = 0; q < P0::BL; q++) {
+      const int b = (int)threadIdx.x + q * NT;
+      const int line = b % L, j = b / L;
+      if (b < L * P0::NBL && line < nl) {
+        const bool pad = 2 * line + 1 >= n_out;  // the partner line stays zero
+#pragma unroll
+        for (int m = 0; m <= P0::RL / 2; m++) {
+          const int p = j + m * P0::NBL;
+          if (p <= N / 2)
+            base[(int64_t)p * line_stride + line] = make_double2(ur[q][m], pad ? 0.0 : ui[q][m]);
         }
       }
     }
@@ -407,8 +436,8 @@ __global__ __launch_bounds__(strided_threads(N)) void fft3_strided(
   double2* base0 = spec + (int64_t)r0 * outer_stride + k0;
   double2* base1 = spec + (int64_t)r1 * outer_stride + k0;
   double ar[P0::B0][P0::R0], ai[P0::B0][P0::R0], br[P0::B0][P0::R0], bi[P0::B0][P0::R0];
-  strided_load<MODE, N>(base0, nl, n_in, line_stride, ar, ai);
-  if (two) strided_load<MODE, N>(base1, nl, n_in, line_stride, br, bi);
+  strided_load<MODE, N>(base0, nl, n_in, k0, line_stride, ar, ai);
+  if (two) strided_load<MODE, N>(base1, nl, n_in, k0, line_stride, br, bi);
   double g[PF::BL][PF::RL];
   if constexpr (MODE == CONV) {  // Green values of (ky = o, this kz tile), shared by both rows
 #pragma unroll
@@ -440,10 +469,12 @@ __global__ __launch_bounds__(strided_threads(N)) void fft3_strided(
       }
     }
   }
-  strided_compute<MODE, N>(re, im, base0, nl, n_out, line_stride, tw, nmax, g, ar, ai);
+  // GREEN: n_out passes the real lines from this tile's first pair on
+  const int nout = MODE == GREEN ? n_in - 2 * k0 : n_out;
+  strided_compute<MODE, N>(re, im, base0, nl, nout, line_stride, tw, nmax, g, ar, ai);
   if (two) {
     __syncthreads();  // LDS reads of the first row are done
-    strided_compute<MODE, N>(re, im, base1, nl, n_out, line_stride, tw, nmax, g, br, bi);
+    strided_compute<MODE, N>(re, im, base1, nl, nout, line_stride, tw, nmax, g, br, bi);
   }
 }
 
@@ -531,7 +562,7 @@ __global__ __launch_bounds__(kThreads) void green3d_axis(double* __restrict__ gh
 }  // namespace ob
 
 struct ob_sc3d {
-  int nx, ny, nz, M0, M1, M2, H2, H2p, qx, qy, nmax;
+  int nx, ny, nz, M0, M1, M2, H2, H2p, H2g, qx, qy, nmax;
   double ratio_limit = 1.2, xy_ratio = 1.0, xz_ratio = 1.0;
   double sMin[3], sMax[3], sd[3];  // solver grid limits and steps (PoissonSolver3D.cc)
   double g[9];                     // rho/phi grid: xMin,xMax,yMin,yMax,zMin,zMax (rest), dx,dy,dz
@@ -583,7 +614,7 @@ void launch_strided_n(ob_sc3d* c, int n_in, int n_out, int64_t line_stride,
   ob::ProfScope ps_(label);
   hipLaunchKernelGGL((fft3_strided<MODE, N>), dim3(blocks_outer * tiles), dim3(strided_threads(N)),
                      lds, stream(), c->spec, n_in, n_out, line_stride, outer_stride, outers, c->H2,
-                     c->ghat, c->qy, c->H2p, c->M1, c->tw, c->nmax, 0.0, 0.0);
+                     c->ghat, c->qy, c->H2g, c->M1, c->tw, c->nmax, 0.0, 0.0);
   OB_HIP(hipGetLastError());
 }
 
@@ -619,31 +650,36 @@ void set_solver_grid(ob_sc3d* c, double xMin, double xMax, double yMin, double y
       const int64_t blocks = (pairs + zpass_pairs(N) - 1) / zpass_pairs(N);
       ob::ProfScope ps_("green3d_z", s);
       hipLaunchKernelGGL(green3d_z<N>, dim3((unsigned)blocks), dim3(kThreads), lds, s, c->qx,
-                         c->qy, c->H2p, c->ghat, gg, c->tw, c->nmax);
+                         c->qy, c->H2g, c->ghat, gg, c->tw, c->nmax);
     }
   });
   OB_HIP(hipGetLastError());
+  // The y and x axis transforms: complex lines are kz pairs, strides in double2 units
+  // (H2g is a multiple of 16, so a pair never straddles a row; the padding is zero).
   auto axis = [&](int n, int64_t line_stride, int64_t outer_stride, int outers) {
     dispatch_pow2(n, [&](auto NC) {
       constexpr int N = decltype(NC)::value;
       if constexpr (N >= 8 && N <= 2048) {
         static bool ready = false;
-        const size_t lds = sizeof(double) * 2 * (strided_lines(N) / 2) * (N + 1);
+        const size_t lds = strided_lds<N>();
         if (!ready) {
-          allow_lds(green3d_axis<N>, lds);
+          allow_lds(fft3_strided<GREEN, N>, lds);
           ready = true;
         }
-        const int tiles = (c->H2 + strided_lines(N) - 1) / strided_lines(N);
+        const int pairs = (c->H2 + 1) / 2;
+        const int tiles = (pairs + strided_lines(N) - 1) / strided_lines(N);
         ob::ProfScope ps_("green3d_axis", s);
-        hipLaunchKernelGGL(green3d_axis<N>, dim3(outers * tiles), dim3(kThreads), lds, s,
-                           c->ghat, line_stride, outer_stride, c->H2, c->tw, c->nmax);
+        hipLaunchKernelGGL((fft3_strided<GREEN, N>), dim3(((outers + 1) / 2) * tiles),
+                           dim3(strided_threads(N)), lds, s, (double2*)c->ghat, c->H2, 0,
+                           line_stride, outer_stride, outers, pairs, nullptr, 0, 0, 0, c->tw,
+                           c->nmax, 0.0, 0.0);
       }
     });
     OB_HIP(hipGetLastError());
   };
-  const int64_t H2p = c->H2p;
-  axis(c->M1, H2p, (int64_t)c->qy * H2p, c->qx);  // y: lines (i, kz)
-  axis(c->M0, (int64_t)c->qy * H2p, H2p, c->qy);  // x: lines (j, kz)
+  const int64_t H2q = c->H2g / 2;
+  axis(c->M1, H2q, (int64_t)c->qy * H2q, c->qx);  // y: lines (i, kz pair) along j
+  axis(c->M0, (int64_t)c->qy * H2q, H2q, c->qy);  // x: lines (j, kz pair) along i
   c->rebuilds++;
 }
 
@@ -736,6 +772,7 @@ OB_API int ob_sc3d_create(int nx, int ny, int nz, ob_sc3d** out) {
   }
   c->H2 = c->M2 / 2 + 1;
   c->H2p = (c->H2 + 7) & ~7;  // kz rows padded to 8 complex: 128-byte aligned tile rows
+  c->H2g = (c->H2 + 15) & ~15;  // Green rows padded to 16 reals: 128-byte aligned as well
   c->qx = c->M0 / 2 + 1;
   c->qy = c->M1 / 2 + 1;
   c->nmax = std::max(c->M0, std::max(c->M1, c->M2));
@@ -744,14 +781,14 @@ OB_API int ob_sc3d_create(int nx, int ny, int nz, ob_sc3d** out) {
   OB_HIP(hipMalloc((void**)&c->phi, sizeof(double) * ng));
   // spectrum after the y pass: x < nx only (the x pass reads and writes just those planes)
   OB_HIP(hipMalloc((void**)&c->spec, sizeof(double2) * (size_t)nx * c->M1 * c->H2p));
-  OB_HIP(hipMalloc((void**)&c->ghat, sizeof(double) * (size_t)c->qx * c->qy * c->H2p));
+  OB_HIP(hipMalloc((void**)&c->ghat, sizeof(double) * (size_t)c->qx * c->qy * c->H2g));
   OB_HIP(hipMalloc((void**)&c->partials, sizeof(double) * 6 * 4096));
   OB_HIP(hipMalloc((void**)&c->ext_dev, sizeof(double) * 6));
   OB_HIP(hipHostMalloc((void**)&c->ext_host, sizeof(double) * 6, hipHostMallocDefault));
   OB_HIP(hipMemset(c->rho, 0, sizeof(double) * ng));
   OB_HIP(hipMemset(c->phi, 0, sizeof(double) * ng));
   OB_HIP(hipMemset(c->spec, 0, sizeof(double2) * (size_t)nx * c->M1 * c->H2p));
-  OB_HIP(hipMemset(c->ghat, 0, sizeof(double) * (size_t)c->qx * c->qy * c->H2p));
+  OB_HIP(hipMemset(c->ghat, 0, sizeof(double) * (size_t)c->qx * c->qy * c->H2g));
   c->tw = ob::make_twiddles(c->nmax);
   // SpaceChargeCalc3D(:32-44): solver on [-1,1]^3 (xy_ratio = xz_ratio = 1); rho/phi grids
   // take the solver limits.
