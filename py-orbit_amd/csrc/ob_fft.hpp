@@ -113,6 +113,28 @@ __device__ __forceinline__ void dft(double* r, double* i) {
     dft2<SIGN>(r, i);
 }
 
+// v[m] *= w^m (m = 1..R-1) with w = tw[i1] (conjugated for SIGN > 0): one table load per
+// butterfly, the higher powers by complex products (a few ulp from the table's values).
+template <int R, int SIGN>
+__device__ __forceinline__ void twiddle_powers(double* vr, double* vi,
+                                               const double2* __restrict__ tw, int i1) {
+#pragma clang fp contract(fast)
+  const double2 w = tw[i1];
+  const double wr = w.x, wi = SIGN < 0 ? w.y : -w.y;
+  double pr = wr, pi = wi;
+#pragma unroll
+  for (int m = 1; m < R; m++) {
+    const double a = vr[m], c = vi[m];
+    vr[m] = a * pr - c * pi;
+    vi[m] = a * pi + c * pr;
+    if (m + 1 < R) {
+      const double t = pr * wr - pi * wi;
+      pi = pr * wi + pi * wr;
+      pr = t;
+    }
+  }
+}
+
 // One Stockham stage S of an N-point transform over `lines` (<= LMAX) lines, NT threads.
 template <int N, int S, int LMAX, int NT, int SIGN>
 __device__ __forceinline__ void fft_stage(double* __restrict__ re, double* __restrict__ im,
@@ -288,14 +310,7 @@ __device__ __forceinline__ void plan_stage_ll(double* __restrict__ re, double* _
         vr[q][m] = re[a];
         vi[q][m] = im[a];
       }
-#pragma unroll
-      for (int m = 1; m < R; m++) {
-        const double2 w = tw[m * k * tstep];
-        const double wr = w.x, wi = SIGN < 0 ? w.y : -w.y;
-        const double a = vr[q][m], c = vi[q][m];
-        vr[q][m] = a * wr - c * wi;
-        vi[q][m] = a * wi + c * wr;
-      }
+      twiddle_powers<R, SIGN>(vr[q], vi[q], tw, k * tstep);
       dft<R, SIGN>(vr[q], vi[q]);
     }
   }
@@ -378,14 +393,7 @@ __device__ __forceinline__ void plan_last_regs(
         vr[q][m] = re[a];
         vi[q][m] = im[a];
       }
-#pragma unroll
-      for (int m = 1; m < R; m++) {  // k = j: the span of the last stage is N/R
-        const double2 w = tw[m * j * tstep];
-        const double wr = w.x, wi = SIGN < 0 ? w.y : -w.y;
-        const double a = vr[q][m], c = vi[q][m];
-        vr[q][m] = a * wr - c * wi;
-        vi[q][m] = a * wi + c * wr;
-      }
+      twiddle_powers<R, SIGN>(vr[q], vi[q], tw, j * tstep);  // k = j: span N/R
       dft<R, SIGN>(vr[q], vi[q]);
     }
   }
