@@ -59,6 +59,8 @@ def parse():
     ap.add_argument("--n", type=int, default=0, help="override particle count")
     ap.add_argument("--intensity", type=float, default=0.0,
                     help="protons per bunch (default: the workload's)")
+    ap.add_argument("--lds-budget", type=int, default=0,
+                    help="ring3d: LDS bytes per deposit/kick tile (0: the calculator's default)")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--no-accuracy", action="store_true",
                     help="ring3d: skip the 4N reference run (emittance growth, field error)")
@@ -79,6 +81,8 @@ def build_workload(args):
         coords = waterbag_coords(n)
         lat = ring_lattice(24)
         calc = spacecharge.SpaceChargeCalc3D(*grid)
+        if args.lds_budget:
+            calc.setDepositMode(True, args.lds_budget)
         scn = setSC3DAccNodes(lat, 0.49, calc)
         cell_len = 6  # QF S1 D1 QD S2 D2 (the ring's turn counter node follows the cells)
     else:

@@ -9,7 +9,8 @@
 // FFTW: forward exponent -1, unnormalised.
 //
 // Twiddles come from a table tw[q] = exp(-2 pi i q / nmax), q < nmax, computed on the
-// host in long double from the exact rational angle and rounded once.
+// host in long double from the exact rational angle and rounded once; a butterfly loads
+// its first twiddle w and forms w^2 .. w^(R-1) by complex products.
 #pragma once
 
 #include <type_traits>
@@ -160,16 +161,9 @@ __device__ __forceinline__ void fft_stage(double* __restrict__ re, double* __res
         vr[q][m] = lr[m * NB];
         vi[q][m] = li[m * NB];
       }
-      if constexpr (Ns > 1) {
+      if constexpr (Ns > 1) {  // v[m] *= exp(SIGN 2 pi i m k / (Ns R))
         const int k = j & (Ns - 1);
-#pragma unroll
-        for (int m = 1; m < R; m++) {  // v[m] *= exp(SIGN 2 pi i m k / (Ns R))
-          const double2 w = tw[m * k * tstep];
-          const double wr = w.x, wi = SIGN < 0 ? w.y : -w.y;
-          const double a = vr[q][m], c = vi[q][m];
-          vr[q][m] = a * wr - c * wi;
-          vi[q][m] = a * wi + c * wr;
-        }
+        twiddle_powers<R, SIGN>(vr[q], vi[q], tw, k * tstep);
       }
       dft<R, SIGN>(vr[q], vi[q]);
     }
