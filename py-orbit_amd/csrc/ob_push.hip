@@ -179,19 +179,23 @@ __device__ __forceinline__ void soln(const Op& op, double& x, double& xp, double
   z -= length * phifac;
 }
 
-// Whether an op kind is implemented only by the FULL kernels.
-__host__ __device__ constexpr bool full_only(int kind) {
-  return kind == OP_MULTPFRINGE_IN || kind == OP_MULTPFRINGE_OUT || kind == OP_SC3D_KICK ||
-         kind == OP_WEDGEBEND || kind == OP_SOLN || kind == OP_RINGRF;
+// Kernel level an op kind needs: 0 = the lite kernels, 1 = with the multipole fringes
+// (a ring's sextupoles), 2 = with every register-hungry op (3D kick, wedge bend, solenoid,
+// ring RF).  Each level holds the ones below it; a lower level runs with fewer VGPRs and
+// so more waves per SIMD.
+__host__ __device__ constexpr int op_level(int kind) {
+  return (kind == OP_MULTPFRINGE_IN || kind == OP_MULTPFRINGE_OUT)
+             ? 1
+             : ((kind == OP_SC3D_KICK || kind == OP_WEDGEBEND || kind == OP_SOLN ||
+                 kind == OP_RINGRF)
+                    ? 2
+                    : 0);
 }
 
-// FULL = false leaves out the register-hungry ops (multipole fringe, 3D kick, wedge bend,
-// solenoid, ring RF), so programs
-// without them run with about half the VGPRs (twice the waves per SIMD).
-template <bool FULL>
+template <int LEVEL>
 __device__ __forceinline__ void apply_op(const Op& op, double& x, double& xp, double& y,
                                          double& yp, double& z, double& dE) {
-  if (!FULL && full_only(op.kind)) return;  // never reached: the host launches FULL
+  if (op_level(op.kind) > LEVEL) return;  // never reached: the host picks the level
   switch (op.kind) {
     case OP_DRIFT: {  // teapotbase.cc:158-183. p: length, gamma2i, dp_p_coeff
       const double length = op.p[0], gamma2i = op.p[1];
@@ -248,7 +252,7 @@ __device__ __forceinline__ void apply_op(const Op& op, double& x, double& xp, do
     }
     case OP_MULTPFRINGE_IN:
     case OP_MULTPFRINGE_OUT:
-      if constexpr (FULL) multpfringe(op, x, xp, y, yp, z, dE);
+      if constexpr (LEVEL >= 1) multpfringe(op, x, xp, y, yp, z, dE);
       break;
     case OP_WRAPBUNCH: {  // teapotbase.cc:202-211. p[0]=length
       const double length = op.p[0];
@@ -274,7 +278,7 @@ __device__ __forceinline__ void apply_op(const Op& op, double& x, double& xp, do
       sc2p5drb_kick(op, x, xp, y, yp, z, dE);
       break;
     case OP_SC3D_KICK:
-      if constexpr (FULL) sc3d_kick(op, x, xp, y, yp, z, dE);
+      if constexpr (LEVEL >= 2) sc3d_kick(op, x, xp, y, yp, z, dE);
       break;
     case OP_BEND1: {  // teapotbase.cc:1099-1145. p: m11 m12 m16 m21 m22 m26 m51 m52 m56,
                       // length, dp_p_coeff
@@ -363,13 +367,13 @@ __device__ __forceinline__ void apply_op(const Op& op, double& x, double& xp, do
       break;
     }
     case OP_WEDGEBEND:
-      if constexpr (FULL) wedgebend(op, x, xp, y, yp, z, dE);
+      if constexpr (LEVEL >= 2) wedgebend(op, x, xp, y, yp, z, dE);
       break;
     case OP_SOLN:
-      if constexpr (FULL) soln(op, x, xp, y, yp, z, dE);
+      if constexpr (LEVEL >= 2) soln(op, x, xp, y, yp, z, dE);
       break;
     case OP_RINGRF:  // teapotbase.cc:1527-1566. p: h*2pi/L, phase_s, voltage, coeff, p_in/p_out
-      if constexpr (FULL) {
+      if constexpr (LEVEL >= 2) {
         const double deltaV = op.p[2] * (sin(op.p[0] * z + op.p[1]));
         dE += op.p[3] * deltaV;
         xp *= op.p[4];
@@ -381,11 +385,11 @@ __device__ __forceinline__ void apply_op(const Op& op, double& x, double& xp, do
   }
 }
 
-// Whether a program holds an op that only the FULL kernels implement.
-bool program_needs_full(const OpProgram& prog) {
-  for (int k = 0; k < prog.nops; k++)
-    if (full_only(prog.ops[k].kind)) return true;
-  return false;
+// The kernel level a program needs (the highest of its ops).
+int program_level(const OpProgram& prog) {
+  int l = 0;
+  for (int k = 0; k < prog.nops; k++) l = std::max(l, op_level(prog.ops[k].kind));
+  return l;
 }
 
 constexpr int kPushThreads = 256;
@@ -412,7 +416,7 @@ __device__ __forceinline__ void load6(const double* __restrict__ a, const double
   q[5] = f[i];
 }
 
-template <bool EXTREMA, bool FULL>
+template <bool EXTREMA, int LEVEL>
 __global__ __launch_bounds__(kPushThreads) void push_kernel(DeviceBunchView v, OpProgram prog,
                                                             double* __restrict__ partials) {
   double lo[3] = {DBL_MAX, DBL_MAX, DBL_MAX};
@@ -432,7 +436,7 @@ __global__ __launch_bounds__(kPushThreads) void push_kernel(DeviceBunchView v, O
   for (; i < v.n; i += stride) {
     double x = q[0], xp = q[1], y = q[2], yp = q[3], z = q[4], dE = q[5];
     if (i + stride < v.n) load6(cx, cxp, cy, cyp, cz, cdE, i + stride, q);
-    for (int k = 0; k < prog.nops; k++) apply_op<FULL>(prog.ops[k], x, xp, y, yp, z, dE);
+    for (int k = 0; k < prog.nops; k++) apply_op<LEVEL>(prog.ops[k], x, xp, y, yp, z, dE);
     cx[i] = x;
     cxp[i] = xp;
     cy[i] = y;
@@ -505,7 +509,7 @@ __global__ __launch_bounds__(kPushThreads) void push_batch_kernel(
     double x = q[0], xp = q[1], y = q[2], yp = q[3], z = q[4], dE = q[5];
     if (i + kPushThreads < it.end) load6(cx, cxp, cy, cyp, cz, cdE, i + kPushThreads, q);
     if (kick.active) sc2p5drb_kick(kop, x, xp, y, yp, z, dE);
-    for (int k = 0; k < prog.nops; k++) apply_op<FULL>(prog.ops[k], x, xp, y, yp, z, dE);
+    for (int k = 0; k < prog.nops; k++) apply_op<FULL ? 2 : 0>(prog.ops[k], x, xp, y, yp, z, dE);
     cx[i] = x;
     cxp[i] = xp;
     cy[i] = y;
@@ -549,7 +553,7 @@ void launch_push_batch(const BatchBunch* bb, const BatchItem* items, int nitems,
                        int threads, hipStream_t s) {
   OB_REQUIRE(threads == kPushThreads, "batch push: unsupported block size");
   if (nitems == 0) return;
-  const bool full = program_needs_full(prog);
+  const bool full = program_level(prog) > 0;
   if (prog.want_extrema) {
     OB_REQUIRE(partials != nullptr, "batch push: extrema requested without a buffer");
     ProfScope ps("push_batch_extrema");
@@ -582,25 +586,28 @@ int push_grid_blocks(int64_t n) {
 int launch_push(const DeviceBunchView& v, const OpProgram& prog, double* ext_partials,
                 hipStream_t s) {
   const int blocks = push_grid_blocks(v.n);
-  const bool full = program_needs_full(prog);
+  const int level = program_level(prog);
+  auto go = [&](auto kernel, double* part) {
+    hipLaunchKernelGGL(kernel, dim3(blocks), dim3(kPushThreads), 0, s, v, prog, part);
+  };
   if (prog.want_extrema) {
     OB_REQUIRE(ext_partials != nullptr, "push: extrema requested without a buffer");
     ProfScope ps("push_extrema");
-    if (full)
-      hipLaunchKernelGGL((push_kernel<true, true>), dim3(blocks), dim3(kPushThreads), 0, s, v,
-                         prog, ext_partials);
+    if (level == 2)
+      go(push_kernel<true, 2>, ext_partials);
+    else if (level == 1)
+      go(push_kernel<true, 1>, ext_partials);
     else
-      hipLaunchKernelGGL((push_kernel<true, false>), dim3(blocks), dim3(kPushThreads), 0, s, v,
-                         prog, ext_partials);
+      go(push_kernel<true, 0>, ext_partials);
   } else {
     if (prog.nops == 0 || v.n == 0) return 0;
     ProfScope ps("push");
-    if (full)
-      hipLaunchKernelGGL((push_kernel<false, true>), dim3(blocks), dim3(kPushThreads), 0, s, v,
-                         prog, nullptr);
+    if (level == 2)
+      go(push_kernel<false, 2>, nullptr);
+    else if (level == 1)
+      go(push_kernel<false, 1>, nullptr);
     else
-      hipLaunchKernelGGL((push_kernel<false, false>), dim3(blocks), dim3(kPushThreads), 0, s, v,
-                         prog, nullptr);
+      go(push_kernel<false, 0>, nullptr);
   }
   OB_HIP(hipGetLastError());
   return blocks;

@@ -84,11 +84,21 @@ __global__ __launch_bounds__(1024) void reduce_extrema(const double* __restrict_
                                                        int nblk, double* __restrict__ out) {
   __shared__ double red[16][6];
   double r[6] = {DBL_MAX, -DBL_MAX, DBL_MAX, -DBL_MAX, DBL_MAX, -DBL_MAX};
-  for (int b = threadIdx.x; b < nblk; b += blockDim.x)
-    for (int k = 0; k < 6; k++) {
-      const double v = partials[b * 6 + k];
-      r[k] = (k & 1) ? fmax(r[k], v) : fmin(r[k], v);
+  // four rows per thread and round, their loads in flight together
+  constexpr int U = 4;
+  for (int b0 = threadIdx.x; b0 < nblk; b0 += U * blockDim.x) {
+    double v[U][6];
+#pragma unroll
+    for (int u = 0; u < U; u++) {
+      const int b = b0 + u * (int)blockDim.x;
+#pragma unroll
+      for (int k = 0; k < 6; k++) v[u][k] = b < nblk ? partials[b * 6 + k] : r[k];
     }
+#pragma unroll
+    for (int u = 0; u < U; u++)
+#pragma unroll
+      for (int k = 0; k < 6; k++) r[k] = (k & 1) ? fmax(r[k], v[u][k]) : fmin(r[k], v[u][k]);
+  }
   for (int k = 0; k < 6; k++)
     for (int o = 32; o > 0; o >>= 1) {
       const double v = __shfl_xor(r[k], o, 64);

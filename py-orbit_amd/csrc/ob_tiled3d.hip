@@ -231,9 +231,24 @@ __global__ __launch_bounds__(kThreads) void tile_deposit(const double* __restric
   for (int c = threadIdx.x; c < cells; c += blockDim.x) acc[c] = 0.0;
   __syncthreads();
   const int64_t gsx = (int64_t)g.ny * nz;
-  for (int i = it.y + threadIdx.x; i < it.z; i += blockDim.x) {
-    const double x = cur[i], y = cur[2 * cap + i], z = cur[4 * cap + i];
-    const double m = msize ? msize[i] : macrosize;
+  // the next particle's coordinates are requested before this one is deposited
+  int i = it.y + threadIdx.x;
+  double qx = 0.0, qy = 0.0, qz = 0.0, qm = macrosize;
+  if (i < it.z) {
+    qx = cur[i];
+    qy = cur[2 * cap + i];
+    qz = cur[4 * cap + i];
+    if (msize) qm = msize[i];
+  }
+  for (; i < it.z; i += blockDim.x) {
+    const double x = qx, y = qy, z = qz, m = qm;
+    const int inext = i + (int)blockDim.x;
+    if (inext < it.z) {
+      qx = cur[inext];
+      qy = cur[2 * cap + inext];
+      qz = cur[4 * cap + inext];
+      if (msize) qm = msize[inext];
+    }
     int iX, iY, iZ;
     double xF, yF, zF;
     node_ind_frac(x, g.xMin, g.dx, g.nx, iX, xF);
@@ -493,7 +508,8 @@ void sort_and_deposit_3d(ob_bunch* b, const TileGeom& g, double macrosize, doubl
     hipLaunchKernelGGL(tile_scan, dim3(1), dim3(1024), 0, s, w.counts, g.ntiles, w.offsets,
                        w.cursor, w.items, w.nitems);
   }
-  const int64_t per = std::max<int64_t>(2048, (n + 1023) / 1024);
+  // about 4096 workgroups: measured fastest for the scatter and for the deposit after it
+  const int64_t per = std::max<int64_t>(512, (n + 4095) / 4096);
   const int sb_blocks = (int)((n + per - 1) / per);
   if (n) {
     ProfScope ps("tile_scatter");
