@@ -213,7 +213,7 @@ void autotune(ob_batch* B) {
   idp.want_extrema = 1;
   float best = 1e30f;
   int best_chunk = B->chunk;
-  for (int c : {4096, 16384, 65536}) {
+  for (int c : {2048, 4096, 16384, 65536}) {
     B->chunk = c;
     build_records(B, false);
     launch_push_batch(B->d_bb, B->d_items, B->shape.nitems, idp, BatchKick{}, B->d_partials,

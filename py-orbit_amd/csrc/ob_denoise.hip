@@ -610,6 +610,50 @@ __global__ __launch_bounds__(1024) void image_to_rho(const float* __restrict__ o
   for (int i = threadIdx.x; i < n; i += blockDim.x) rho[i] = (double)out[i] * scale;
 }
 
+// The same for the rho grids of up to 64 space-charge calculators at once: workgroup j
+// handles grid j (image j of x, total[j]).
+struct RhoJobs {
+  double* rho[64];
+  const int32_t* skip[64];
+};
+__global__ __launch_bounds__(1024) void rho_to_images(RhoJobs jobs, int n, float* __restrict__ x,
+                                                      double* __restrict__ total) {
+  const int j = blockIdx.x;
+  if (*jobs.skip[j]) return;
+  const double* __restrict__ rho = jobs.rho[j];
+  float* __restrict__ xj = x + (size_t)j * n;
+  __shared__ double sh[16];
+  double s = 0.0;
+  for (int i = threadIdx.x; i < n; i += blockDim.x) s += rho[i];
+  for (int o = 32; o > 0; o >>= 1) s += __shfl_xor(s, o, 64);
+  if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = s;
+  __syncthreads();
+  double t = 0.0;
+  for (int w = 0; w < (int)(blockDim.x >> 6); w++) t += sh[w];
+  const double scale = t > 0.0 ? (double)n / t : 0.0;
+  for (int i = threadIdx.x; i < n; i += blockDim.x) xj[i] = (float)(rho[i] * scale);
+  if (threadIdx.x == 0) total[j] = t;
+}
+__global__ __launch_bounds__(1024) void images_to_rho(const float* __restrict__ out, int n,
+                                                      RhoJobs jobs,
+                                                      const double* __restrict__ total) {
+  const int j = blockIdx.x;
+  if (*jobs.skip[j]) return;
+  const float* __restrict__ oj = out + (size_t)j * n;
+  double* __restrict__ rho = jobs.rho[j];
+  __shared__ double sh[16];
+  double s = 0.0;
+  for (int i = threadIdx.x; i < n; i += blockDim.x) s += (double)oj[i];
+  for (int o = 32; o > 0; o >>= 1) s += __shfl_xor(s, o, 64);
+  if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = s;
+  __syncthreads();
+  double t = 0.0;
+  for (int w = 0; w < (int)(blockDim.x >> 6); w++) t += sh[w];
+  if (t == 0.0) return;  // keep the deposited grid
+  const double scale = total[j] / t;
+  for (int i = threadIdx.x; i < n; i += blockDim.x) rho[i] = (double)oj[i] * scale;
+}
+
 // dst image j <- src image idx[j] (j < count), images of n floats.
 struct GatherIdx {
   int32_t idx[64];
@@ -826,7 +870,7 @@ struct ob_denoiser {
   float* pyr = nullptr;   // error pyramid of the multi-scale loss
   int nscales = 0;
   float lam[9] = {};
-  float* xin = nullptr;   // one normalized rho image (space-charge use)
+  float* xin = nullptr;   // normalized rho images (space-charge use), max_batch of them
   double* total = nullptr;  // its total charge
 };
 
@@ -1065,6 +1109,30 @@ void dn_check_shape(ob_denoiser* d, int nx, int ny) {
   OB_REQUIRE(d->H == nx && d->W == ny, "denoiser: image size differs from the grid");
 }
 
+void dn_denoise_rhos(const std::vector<DenoiseJob>& jobs) {
+  hipStream_t s = stream();
+  size_t a = 0;
+  while (a < jobs.size()) {
+    ob_denoiser* d = jobs[a].d;
+    dn_check_shape(d, jobs[a].nx, jobs[a].ny);
+    const int cap = std::min(64, d->max_batch);
+    dn::RhoJobs rj;
+    int count = 0;
+    while (a < jobs.size() && jobs[a].d == d && count < cap) {
+      rj.rho[count] = jobs[a].rho;
+      rj.skip[count] = jobs[a].skip;
+      count++;
+      a++;
+    }
+    const int n = d->H * d->W;
+    ProfScope ps("dn_sc_denoise_batch");
+    hipLaunchKernelGGL(dn::rho_to_images, dim3(count), dim3(1024), 0, s, rj, n, d->xin, d->total);
+    dn::forward(d, d->xin, count, s);
+    hipLaunchKernelGGL(dn::images_to_rho, dim3(count), dim3(1024), 0, s, d->out, n, rj, d->total);
+    OB_HIP(hipGetLastError());
+  }
+}
+
 void dn_denoise_rho(ob_denoiser* d, double* rho, int nx, int ny, const int32_t* skip) {
   dn_check_shape(d, nx, ny);
   hipStream_t s = stream();
@@ -1140,12 +1208,12 @@ OB_API int ob_dn_create(int H, int W, int max_batch, ob_denoiser** out) {
     OB_HIP(hipMalloc((void**)&d->gtmp1, sizeof(bf16) * (img / 16) * 32));
     OB_HIP(hipMalloc((void**)&d->out, sizeof(float) * img));
     OB_HIP(hipMalloc((void**)&d->loss, sizeof(float)));
-    OB_HIP(hipMalloc((void**)&d->xin, sizeof(float) * H * W));
+    OB_HIP(hipMalloc((void**)&d->xin, sizeof(float) * img));
     OB_HIP(hipMalloc((void**)&d->pyr, sizeof(float) * (img / 3 + 1)));
     OB_HIP(hipMalloc((void**)&d->wpart, sizeof(float) * ob::dn::kWpartFloats));
     OB_HIP(hipMalloc((void**)&d->bpart, sizeof(float) * ob::dn::kWgradBlocks * 64));
     OB_HIP(hipMalloc((void**)&d->hpart, sizeof(float) * ob::dn::kHeadBlocks * 18));
-    OB_HIP(hipMalloc((void**)&d->total, sizeof(double)));
+    OB_HIP(hipMalloc((void**)&d->total, sizeof(double) * max_batch));
   } catch (...) {
     delete d;  // device buffers of a failed create are reclaimed at process exit
     throw;

@@ -235,6 +235,9 @@ struct DenoiseJob {
 void launch_sc2d_batch(const BatchBunch* bb, const BatchItem* items, const BatchShape& sh,
                        const double* partials, double length, double pipe_radius, hipStream_t s,
                        const std::vector<DenoiseJob>& jobs = {});
+// Consecutive jobs with the same denoiser run as one batched inference (up to its
+// max_batch, at most 64 images per launch).
+void dn_denoise_rhos(const std::vector<DenoiseJob>& jobs);
 }  // namespace ob
 struct ob_boundary2d;
 namespace ob {

@@ -477,7 +477,7 @@ __global__ __launch_bounds__(kPushThreads) void push_kernel(DeviceBunchView v, O
 // bunch).  The bunch's deferred 2.5D kick comes first (its op is assembled from the
 // bunch's record and the batch's common factors), then the map program; extrema are
 // reduced per work item.
-template <bool EXTREMA, bool FULL>
+template <bool EXTREMA, int LEVEL>
 __global__ __launch_bounds__(kPushThreads) void push_batch_kernel(
     const BatchBunch* __restrict__ bb, const BatchItem* __restrict__ items, OpProgram prog,
     BatchKick kick, double* __restrict__ partials) {
@@ -509,7 +509,7 @@ __global__ __launch_bounds__(kPushThreads) void push_batch_kernel(
     double x = q[0], xp = q[1], y = q[2], yp = q[3], z = q[4], dE = q[5];
     if (i + kPushThreads < it.end) load6(cx, cxp, cy, cyp, cz, cdE, i + kPushThreads, q);
     if (kick.active) sc2p5drb_kick(kop, x, xp, y, yp, z, dE);
-    for (int k = 0; k < prog.nops; k++) apply_op<FULL ? 2 : 0>(prog.ops[k], x, xp, y, yp, z, dE);
+    for (int k = 0; k < prog.nops; k++) apply_op<LEVEL>(prog.ops[k], x, xp, y, yp, z, dE);
     cx[i] = x;
     cxp[i] = xp;
     cy[i] = y;
@@ -553,25 +553,29 @@ void launch_push_batch(const BatchBunch* bb, const BatchItem* items, int nitems,
                        int threads, hipStream_t s) {
   OB_REQUIRE(threads == kPushThreads, "batch push: unsupported block size");
   if (nitems == 0) return;
-  const bool full = program_level(prog) > 0;
+  const int level = program_level(prog);
+  auto go = [&](auto kernel, double* part) {
+    hipLaunchKernelGGL(kernel, dim3(nitems), dim3(kPushThreads), 0, s, bb, items, prog, kick,
+                       part);
+  };
   if (prog.want_extrema) {
     OB_REQUIRE(partials != nullptr, "batch push: extrema requested without a buffer");
     ProfScope ps("push_batch_extrema");
-    if (full)
-      hipLaunchKernelGGL((push_batch_kernel<true, true>), dim3(nitems), dim3(kPushThreads), 0, s,
-                         bb, items, prog, kick, partials);
+    if (level == 2)
+      go(push_batch_kernel<true, 2>, partials);
+    else if (level == 1)
+      go(push_batch_kernel<true, 1>, partials);
     else
-      hipLaunchKernelGGL((push_batch_kernel<true, false>), dim3(nitems), dim3(kPushThreads), 0,
-                         s, bb, items, prog, kick, partials);
+      go(push_batch_kernel<true, 0>, partials);
   } else {
     if (prog.nops == 0 && !kick.active) return;
     ProfScope ps("push_batch");
-    if (full)
-      hipLaunchKernelGGL((push_batch_kernel<false, true>), dim3(nitems), dim3(kPushThreads), 0,
-                         s, bb, items, prog, kick, nullptr);
+    if (level == 2)
+      go(push_batch_kernel<false, 2>, nullptr);
+    else if (level == 1)
+      go(push_batch_kernel<false, 1>, nullptr);
     else
-      hipLaunchKernelGGL((push_batch_kernel<false, false>), dim3(nitems), dim3(kPushThreads), 0,
-                         s, bb, items, prog, kick, nullptr);
+      go(push_batch_kernel<false, 0>, nullptr);
   }
   OB_HIP(hipGetLastError());
 }

@@ -1276,7 +1276,7 @@ void launch_sc2d_batch(const BatchBunch* bb, const BatchItem* items, const Batch
     hipLaunchKernelGGL(sc2d_tdeposit_batch, dim3(sh.dep_slots), dim3(kDepositThreads),
                        sh.dep_lds, s, bb, sh.nb);
   }
-  for (const DenoiseJob& j : jobs) dn_denoise_rho(j.d, j.rho, j.nx, j.ny, j.skip);
+  if (!jobs.empty()) dn_denoise_rhos(jobs);
   {
     ProfScope ps("sc2d_post_batch");
     hipLaunchKernelGGL(sc2d_post_batch, dim3(sh.nb), dim3(kSetupThreads), 0, s, bb, length,
