@@ -49,7 +49,8 @@ def parse():
     ap.add_argument("--steps", type=int, default=24)
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--workload", default="ring3d",
-                    choices=["ring3d", "fodo2p5d", "batch64", "denoise", "batch_denoise"])
+                    choices=["ring3d", "fodo2p5d", "fodo2p5d_graph", "batch64", "denoise",
+                             "batch_denoise"])
     ap.add_argument("--pairs", type=int, default=20000, help="denoise: training pairs")
     ap.add_argument("--bunches", type=int, default=32, help="batch_denoise: bunches")
     ap.add_argument("--dn-pairs", type=int, default=4096,
@@ -311,8 +312,14 @@ def main_batch(args):
     from batch import BunchBatch
     from orbit.space_charge.sc2p5d import setSC2p5DrbAccNodes
     from workloads import PIPE_RADIUS, batch_coords, batch_specs, fodo_lattice, make_bunch
-    specs = batch_specs(64)
-    bunches = [make_bunch(batch_coords(sp), intensity=sp["intensity"]) for sp in specs]
+    if args.workload == "fodo2p5d_graph":  # the C1 bunch alone through the graph path
+        from workloads import INTENSITY, gaussian_coords
+        n1 = args.n or 100_000
+        specs = [{"n": n1, "grid": (128, 128, 64), "intensity": INTENSITY}]
+        bunches = [make_bunch(gaussian_coords(n1), intensity=INTENSITY)]
+    else:
+        specs = batch_specs(64)
+        bunches = [make_bunch(batch_coords(sp), intensity=sp["intensity"]) for sp in specs]
     calcs = [spacecharge.SpaceChargeCalc2p5Drb(*sp["grid"]) for sp in specs]
     bb = BunchBatch(bunches, calcs)
     lat = fodo_lattice(1)
@@ -357,7 +364,7 @@ def main_batch(args):
                    "share": round(v[1] / sum(x[1] for x in prof.values()), 4)}
                for k, v in sorted(prof.items(), key=lambda kv: -kv[1][1])}
     cpu = None
-    if not args.no_cpu:
+    if not args.no_cpu and args.workload == "batch64":
         med = sorted(specs, key=lambda sp: sp["n"])[len(specs) // 2]
         a2 = argparse.Namespace(**vars(args))
         a2.workload = "fodo2p5d"
@@ -368,7 +375,7 @@ def main_batch(args):
         "steps": args.steps, "warmup": args.warmup,
         "ms_per_step": round(dt / args.steps * 1e3, 3), "higher_is_better": True,
         "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": "synthetic",
-        "config": {"workload": "batch64", "bunches": len(specs), "particles": n,
+        "config": {"workload": args.workload, "bunches": len(specs), "particles": n,
                    "grids": sorted({sp["grid"][0] for sp in specs}), "z_bins": 64,
                    "sc_intervals_per_step": nsc, "step": "one FODO cell (hipGraph replay)",
                    "tuning": bb.tuning(), "parallelism": "1 GPU"},
@@ -646,9 +653,10 @@ def main_batch_denoise(args):
     def note(msg):
         print(f"[batch_denoise] {msg}", file=sys.stderr, flush=True)
 
-    gens = [lambda m, sd: gaussian_coords(m, seed=sd),
-            lambda m, sd: kv_coords(m, seed=sd, rx=4e-3, rxp=0.7e-3),
-            lambda m, sd: waterbag_coords(m, seed=sd, sz=30.0)]
+    # z inside the 144 m ring (the ring's bunch wrap keeps z within +-72 m)
+    gens = [lambda m, sd: gaussian_coords(m, seed=sd, zhalf=60.0),
+            lambda m, sd: kv_coords(m, seed=sd, rx=4e-3, rxp=0.7e-3, zhalf=60.0),
+            lambda m, sd: waterbag_coords(m, seed=sd, sz=20.0)]
 
     def coords(k, m):
         return gens[k % 3](m, 5000 + k)
@@ -683,27 +691,30 @@ def main_batch_denoise(args):
         c.setDenoiser(dn)
     e0 = [emit(b) for b in bunches[:args.ref_bunches]]
     bb = BunchBatch(bunches, calcs)
-    lat = ring_lattice(1)
+    lat = ring_lattice(24)
     scn = setSC2p5DrbAccNodes(lat, 0.49, bb.calculator(), PIPE_RADIUS)
-    nsc = len(scn)
+    nsc = len(scn) // 24
+    # every cell of the ring is the same, so one cell (nodes 0-5) captured and replayed
+    # tracks the bunches through successive cells
+    cell = lambda: lat.trackBunch(bb.bunch(), index_start=0, index_stop=5)
     for _ in range(max(1, args.warmup)):
-        lat.trackBunch(bb.bunch())
+        cell()
     bb.synchronize()
     graph = True
     try:
-        bb.capture(lambda: lat.trackBunch(bb.bunch()))
+        bb.capture(cell)
         bb.replay(1)
     except Exception as e:  # the direct path is the fallback, and is reported
         note(f"graph capture failed ({e}); tracking cells directly")
         graph = False
-        lat.trackBunch(bb.bunch())
+        cell()
     bb.synchronize()
     t0 = time.perf_counter()
     if graph:
         bb.replay(args.steps)
     else:
         for _ in range(args.steps):
-            lat.trackBunch(bb.bunch())
+            cell()
     bb.synchronize()
     dt = time.perf_counter() - t0
     cells = max(1, args.warmup) + 1 + args.steps
@@ -711,7 +722,7 @@ def main_batch_denoise(args):
     # kernel times of one cell tracked directly (graph replays are not instrumented)
     import ctypes as C
     N.call("ob_profile_enable", 1)
-    lat.trackBunch(bb.bunch())
+    cell()
     bb.synchronize()
     cells += 1
     buf = C.create_string_buffer(1 << 16)
@@ -743,11 +754,12 @@ def main_batch_denoise(args):
         for name, m in (("plain_n", n), ("plain_16M", n_high)):
             b = make_bunch(coords(k, m), intensity=1e14)
             c = spacecharge.SpaceChargeCalc2p5Drb(H, W, 64)
-            lr = ring_lattice(1)
+            lr = ring_lattice(24)
             setSC2p5DrbAccNodes(lr, 0.49, c, PIPE_RADIUS)
             f0 = emit(b)
-            for _ in range(cells):
-                lr.trackBunch(b)
+            for j in range(cells):
+                i0 = 6 * (j % 24)
+                lr.trackBunch(b, index_start=i0, index_stop=i0 + 5)
             f1 = emit(b)
             row[name] = {"particles": m, "eps_growth_xy": [f1[0] / f0[0] - 1,
                                                            f1[1] / f0[1] - 1],
@@ -767,7 +779,7 @@ def main_batch_denoise(args):
         "scaling": "weak", "vs_baseline": None, "dtype": "f64 (denoiser bf16)",
         "data": "synthetic",
         "config": {"workload": "batch_denoise", "bunches": nb, "particles_per_bunch": n,
-                   "grid": [H, W, 64], "lattice": "24-cell ring, one cell per step",
+                   "grid": [H, W, 64], "lattice": "24-cell ring (144 m), one cell per step",
                    "sc_intervals_per_step": nsc, "graph": graph,
                    "tuning": bb.tuning(),
                    "train_pairs": args.dn_pairs, "epochs": args.epochs,
@@ -785,7 +797,7 @@ def main():
     args = parse()
     if args.workload == "batch_denoise" and args.gpus == 1:
         return main_batch_denoise(args)
-    if args.workload == "batch64" and args.gpus == 1:
+    if args.workload in ("batch64", "fodo2p5d_graph") and args.gpus == 1:
         return main_batch(args)
     if args.workload == "denoise" and args.gpus == 1:
         return main_denoise(args)
