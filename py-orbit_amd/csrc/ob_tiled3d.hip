@@ -170,12 +170,32 @@ __global__ __launch_bounds__(kThreads) void tile_scatter(
     lc[t] = c ? atomicAdd(&cursor[t], c) : 0u;
   }
   __syncthreads();
-  for (int64_t i = lo + threadIdx.x; i < hi; i += blockDim.x) {
-    const int64_t d = atomicAdd(&lc[key[i]], 1u);
+  // U particles per thread and round: all their loads are issued before any store
+  constexpr int U = 4;
+  for (int64_t i0 = lo + threadIdx.x; i0 < hi; i0 += U * (int64_t)blockDim.x) {
+    double v[U][6], m[U];
+    uint32_t kk[U];
+    int32_t pp[U];
 #pragma unroll
-    for (int k = 0; k < 6; k++) alt[k * cap + d] = cur[k * cap + i];
-    perm_alt[d] = perm[i];
-    if (msize) msize_alt[d] = msize[i];
+    for (int u = 0; u < U; u++) {
+      const int64_t i = i0 + u * (int64_t)blockDim.x;
+      const int64_t j = i < hi ? i : lo;  // a masked lane reloads the first particle
+      kk[u] = key[j];
+#pragma unroll
+      for (int k = 0; k < 6; k++) v[u][k] = cur[k * cap + j];
+      pp[u] = perm[j];
+      m[u] = msize ? msize[j] : 0.0;
+    }
+#pragma unroll
+    for (int u = 0; u < U; u++) {
+      const int64_t i = i0 + u * (int64_t)blockDim.x;
+      if (i >= hi) break;
+      const int64_t d = atomicAdd(&lc[kk[u]], 1u);
+#pragma unroll
+      for (int k = 0; k < 6; k++) alt[k * cap + d] = v[u][k];
+      perm_alt[d] = pp[u];
+      if (msize) msize_alt[d] = m[u];
+    }
   }
 }
 
