@@ -26,8 +26,8 @@ namespace ob {
 namespace {
 
 constexpr int kThreads = 256;
-constexpr int kChunk = 4096;
-constexpr int kMaxTiles = 16383;  // (ntiles + 1) 32-bit counters in 64 KiB of LDS
+constexpr int kChunk = 1024;
+constexpr int kMaxTiles = 16383;  // (ntiles + 1) 32-bit counters in 64 KiB of LDS (the scan: 2x)
 
 __device__ __forceinline__ bool inside(const TileGeom& g, double x, double y, double z) {
   return !(x < g.xMin || x > g.xMax || y < g.yMin || y > g.yMax || z < g.zMin || z > g.zMax);
@@ -104,50 +104,70 @@ __global__ __launch_bounds__(kThreads) void tile_hist(const double* __restrict__
     if (lh[t]) atomicAdd(&counts[t], lh[t]);
 }
 
-// One block.  counts[0..ntiles] -> offsets (exclusive), cursor = offsets, work items;
-// counts are cleared for the next step.
+// One block.  counts[0..ntiles] (the last = particles outside the grid) -> offsets
+// (exclusive) and cursor = offsets; work items of at most kChunk particles per tile, in tile
+// order; counts are cleared for the next step.  The counts are staged through LDS: thread i
+// loads tiles i, i + 1024, ... (coalesced, all its loads in flight together), the scan runs
+// over contiguous ranges read back from LDS, and the stores go out tile-strided again, so
+// neighbouring lanes write neighbouring offsets and items.  LDS: 2 * (ntiles + 2) counters.
 __global__ __launch_bounds__(1024) void tile_scan(uint32_t* __restrict__ counts, int ntiles,
                                                   uint32_t* __restrict__ offsets,
                                                   uint32_t* __restrict__ cursor,
                                                   int4* __restrict__ items,
                                                   uint32_t* __restrict__ nitems) {
+  extern __shared__ uint32_t ls[];
   const int nb = ntiles + 1;
-  const int per = (nb + blockDim.x - 1) / blockDim.x;
-  const int lo = min(nb, (int)threadIdx.x * per), hi = min(nb, lo + per);
-  // a thread's counters (at most kPer: ntiles + 1 <= 16384 over 1024 threads) are loaded
-  // once, all in flight together
+  uint32_t* loff = ls;            // [nb + 1]: counts, then exclusive particle offsets
+  uint32_t* litem = ls + nb + 1;  // [nb + 1]: exclusive item offsets
   constexpr int kPer = (kMaxTiles + 1) / 1024;
   uint32_t cv[kPer];
 #pragma unroll
   for (int u = 0; u < kPer; u++) {
-    const int t = lo + u;
-    const uint32_t v = counts[t < hi ? t : 0];
-    cv[u] = t < hi ? v : 0u;
+    const int t = (int)threadIdx.x + u * 1024;
+    cv[u] = counts[t < nb ? t : 0];
   }
-  uint32_t off = 0, item = 0;
 #pragma unroll
   for (int u = 0; u < kPer; u++) {
-    const int t = lo + u;
-    off += cv[u];
-    if (t < ntiles) item += (cv[u] + kChunk - 1) / kChunk;
+    const int t = (int)threadIdx.x + u * 1024;
+    if (t < nb) loff[t] = cv[u];
+  }
+  __syncthreads();
+  const int per = (nb + blockDim.x - 1) / blockDim.x;
+  const int lo = min(nb, (int)threadIdx.x * per), hi = min(nb, lo + per);
+  uint32_t off = 0, item = 0;
+  for (int t = lo; t < hi; t++) {
+    const uint32_t k = loff[t];
+    off += k;
+    if (t < ntiles) item += (k + kChunk - 1) / kChunk;
   }
   uint32_t total, total_items;
   block_scan2(off, item, total, total_items);
-  if (threadIdx.x == 0) *nitems = total_items;
+  // each thread rewrites only the entries it alone has read
+  for (int t = lo; t < hi; t++) {
+    const uint32_t k = loff[t];
+    loff[t] = off;
+    litem[t] = item;
+    off += k;
+    if (t < ntiles) item += (k + kChunk - 1) / kChunk;
+  }
+  if (threadIdx.x == 0) {
+    loff[nb] = total;
+    *nitems = total_items;
+  }
+  __syncthreads();
 #pragma unroll
   for (int u = 0; u < kPer; u++) {
-    const int t = lo + u;
-    if (t >= hi) break;
-    const uint32_t cnt = cv[u];
-    offsets[t] = off;
-    cursor[t] = off;
-    if (t < ntiles) {
-      for (uint32_t c0 = 0; c0 < cnt; c0 += kChunk) {
-        items[item++] = make_int4(t, (int)(off + c0), (int)(off + min(cnt, c0 + kChunk)), 0);
-      }
-    }
-    off += cnt;
+    const int t = (int)threadIdx.x + u * 1024;
+    if (t >= nb) break;
+    const uint32_t o = loff[t], e = loff[t + 1];
+    offsets[t] = o;
+    cursor[t] = o;
     counts[t] = 0;
+    if (t < ntiles) {
+      uint32_t j = litem[t];
+      for (uint32_t b0 = o; b0 < e; b0 += kChunk)
+        items[j++] = make_int4(t, (int)b0, (int)min(e, b0 + kChunk), 0);
+    }
   }
 }
 
@@ -502,6 +522,8 @@ void sort_and_deposit_3d(ob_bunch* b, const TileGeom& g, double macrosize, doubl
                                hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024));
     OB_HIP(hipFuncSetAttribute((const void*)tile_scatter,
                                hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024));
+    OB_HIP(hipFuncSetAttribute((const void*)tile_scan,
+                               hipFuncAttributeMaxDynamicSharedMemorySize, 136 * 1024));
     OB_HIP(hipFuncSetAttribute((const void*)tile_kick<false>,
                                hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024));
     OB_HIP(hipFuncSetAttribute((const void*)tile_kick<true>,
@@ -525,7 +547,8 @@ void sort_and_deposit_3d(ob_bunch* b, const TileGeom& g, double macrosize, doubl
   }
   {
     ProfScope ps("tile_scan");
-    hipLaunchKernelGGL(tile_scan, dim3(1), dim3(1024), 0, s, w.counts, g.ntiles, w.offsets,
+    hipLaunchKernelGGL(tile_scan, dim3(1), dim3(1024), sizeof(uint32_t) * 2 * (g.ntiles + 2), s,
+                       w.counts, g.ntiles, w.offsets,
                        w.cursor, w.items, w.nitems);
   }
   // about 4096 workgroups: measured fastest for the scatter and for the deposit after it
