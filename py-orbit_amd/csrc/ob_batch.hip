@@ -115,7 +115,8 @@ void build_records(ob_batch* B, bool force) {
       rows += (r.nx + 1) / 2;
       cols += r.H;
       max_M = std::max(max_M, std::max(r.M0, r.M1));
-      r.tg = sc2d_tile_reserve(B->calcs[b], r.v.n, r.v.msize != nullptr, B->item);
+      r.tg = sc2d_tile_reserve(B->calcs[b], r.v.n, r.v.msize != nullptr, B->item, 1,
+                               r.item_end - r.item_begin);
       r.slot_begin = slots;
       slots += (int)(r.v.n / r.tg.item + r.tg.ntiles + 1);
       r.slot_end = slots;

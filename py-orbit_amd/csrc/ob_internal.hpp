@@ -187,6 +187,8 @@ struct Tile2D {
   double *sx, *sy, *sm;
   double* zpart;      // [zrows][nz] per-block z line densities (single-bunch path)
   double* mpart;      // [max_items][5] rho moments per deposit item
+  uint32_t* icounts;  // [hist_rows][ntiles] tile counts per batch work item (batch path)
+  int32_t hist_rows, pad;
 };
 
 }  // namespace ob
@@ -259,8 +261,9 @@ namespace ob {
 void sc2d_batch_record(ob_sc2p5drb* c, BatchBunch& r);
 // Size the calculator's tile-sort buffers for a bunch of n particles (per-particle
 // macrosize or not) with work items of `item` particles; returns the tile sort.
+// hist_rows: batch work items of the bunch, each of which keeps its row of tile counts.
 Tile2D sc2d_tile_reserve(ob_sc2p5drb* c, int64_t n, bool per_particle_m, int item,
-                         int zrows = 1);
+                         int zrows = 1, int hist_rows = 0);
 // Whether `t` still describes the calculator's tile buffers and fits such a bunch.
 bool sc2d_tile_same(ob_sc2p5drb* c, const Tile2D& t, int64_t n, bool per_particle_m);
 void sc2d_mark_tracked(ob_sc2p5drb* c);

