@@ -233,6 +233,10 @@ template <int MT>
 __global__ __launch_bounds__(kThreads) void conv3x3_wgrad(ConvArgs a, int ntiles) {
   constexpr int COUT = MT * 16;
   constexpr int NP = (9 * MT + 3) / 4;
+  // a thread's share of one tile: NH halo pixel-halves and ND 8-channel runs of dY and Y
+  constexpr int NH = (kHalo * kHalo * 2 + kThreads - 1) / kThreads;
+  constexpr int ND = (kTile * kTile * COUT / 8) / kThreads;
+  static_assert(ND * kThreads == kTile * kTile * COUT / 8, "dY tile split");
   __shared__ __align__(16) bf16 xs[3][16][kHalo][kTile];
   __shared__ __align__(16) bf16 ds[COUT][kTile][kTile];
   const int cc = blockIdx.y;
@@ -241,41 +245,73 @@ __global__ __launch_bounds__(kThreads) void conv3x3_wgrad(ConvArgs a, int ntiles
   f4 acc[NP];
 #pragma unroll
   for (int k = 0; k < NP; k++) acc[k] = f4{0.f, 0.f, 0.f, 0.f};
-  float bsum = 0.f;
-  for (int t = blockIdx.x; t < ntiles; t += gridDim.x) {
+  // bias gradient (chunk 0): a thread's dY runs are always channels w8 * 8 .. w8 * 8 + 7
+  // (kThreads is a multiple of COUT / 8), summed in registers as they are staged
+  float bsum[8] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
+  // The tile's global loads go to registers first, all in flight together; the next tile's
+  // are issued before this tile's MFMAs, so their latency is hidden behind them.
+  uint4 hv[NH], dv[ND], yv[ND];
+  auto fetch = [&](int t) {
     const int b = t / tiles_img, ti = t - b * tiles_img;
     const int ty0 = (ti / tiles_x) * kTile, tx0 = (ti % tiles_x) * kTile;
-    __syncthreads();
-    for (int e = threadIdx.x; e < kHalo * kHalo * 2; e += kThreads) {
+#pragma unroll
+    for (int u = 0; u < NH; u++) {
+      const int e = (int)threadIdx.x + u * kThreads;
       const int pix = e >> 1, h = e & 1;
       const int hy = pix / kHalo, hx = pix - hy * kHalo;
-      const uint4 v = load8(a, b, ty0 + hy - 1, tx0 + hx - 1, cc * 16 + h * 8);
-      const uint32_t w4[4] = {v.x, v.y, v.z, v.w};
+      hv[u] = e < kHalo * kHalo * 2 ? load8(a, b, ty0 + hy - 1, tx0 + hx - 1, cc * 16 + h * 8)
+                                    : make_uint4(0, 0, 0, 0);
+    }
+#pragma unroll
+    for (int u = 0; u < ND; u++) {
+      const int e = (int)threadIdx.x + u * kThreads;
+      const int pix = e / (COUT / 8), w8 = e - pix * (COUT / 8);
+      const int py = pix / kTile, px = pix % kTile;
+      const size_t g = (((size_t)b * a.H + ty0 + py) * a.W + tx0 + px) * COUT + w8 * 8;
+      dv[u] = ld16(a.dy + g);
+      yv[u] = ld16(a.y + g);
+    }
+  };
+  int t = blockIdx.x;
+  if (t < ntiles) fetch(t);
+  for (; t < ntiles; t += gridDim.x) {
+    __syncthreads();  // the previous tile's MFMAs are done with xs and ds
+#pragma unroll
+    for (int u = 0; u < NH; u++) {
+      const int e = (int)threadIdx.x + u * kThreads;
+      if (e >= kHalo * kHalo * 2) break;
+      const int pix = e >> 1, h = e & 1;
+      const int hy = pix / kHalo, hx = pix - hy * kHalo;
+      const uint32_t w4[4] = {hv[u].x, hv[u].y, hv[u].z, hv[u].w};
 #pragma unroll
       for (int j = 0; j < 8; j++) {
         const bf16 val = (bf16)((j & 1) ? (w4[j >> 1] >> 16) : (w4[j >> 1] & 0xFFFFu));
 #pragma unroll
-        for (int s = 0; s < 3; s++) {
-          const int c = hx - s;
-          if (c >= 0 && c < kTile) xs[s][h * 8 + j][hy][c] = val;
+        for (int sh = 0; sh < 3; sh++) {
+          const int c = hx - sh;
+          if (c >= 0 && c < kTile) xs[sh][h * 8 + j][hy][c] = val;
         }
       }
     }
-    for (int e = threadIdx.x; e < kTile * kTile * COUT / 8; e += kThreads) {
-      const int pix = e / (COUT / 8), u = e - pix * (COUT / 8);
+#pragma unroll
+    for (int u = 0; u < ND; u++) {
+      const int e = (int)threadIdx.x + u * kThreads;
+      const int pix = e / (COUT / 8), w8 = e - pix * (COUT / 8);
       const int py = pix / kTile, px = pix % kTile;
-      const size_t g = (((size_t)b * a.H + ty0 + py) * a.W + tx0 + px) * COUT + u * 8;
-      const uint4 dv = ld16(a.dy + g), yv = ld16(a.y + g);
-      const uint32_t d4[4] = {dv.x, dv.y, dv.z, dv.w}, y4[4] = {yv.x, yv.y, yv.z, yv.w};
+      const uint32_t d4[4] = {dv[u].x, dv[u].y, dv[u].z, dv[u].w};
+      const uint32_t y4[4] = {yv[u].x, yv[u].y, yv[u].z, yv[u].w};
 #pragma unroll
       for (int j = 0; j < 8; j++) {
         const uint32_t sh = (j & 1) ? 16 : 0;
         const bf16 d = (bf16)((d4[j >> 1] >> sh) & 0xFFFFu);
         const bf16 yy = (bf16)((y4[j >> 1] >> sh) & 0xFFFFu);
-        ds[u * 8 + j][py][px] = positive(yy) ? d : (bf16)0;
+        const bf16 dm = positive(yy) ? d : (bf16)0;
+        ds[w8 * 8 + j][py][px] = dm;
+        bsum[j] += bf2f(dm);
       }
     }
     __syncthreads();
+    if (t + (int)gridDim.x < ntiles) fetch(t + gridDim.x);
 #pragma unroll
     for (int k = 0; k < NP; k++) {
       const int p = wv + 4 * k;
@@ -289,11 +325,6 @@ __global__ __launch_bounds__(kThreads) void conv3x3_wgrad(ConvArgs a, int ntiles
           acc[k] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(af, bfr, acc[k], 0, 0, 0);
         }
       }
-    }
-    if (cc == 0 && threadIdx.x < COUT * 4) {
-      const int co = threadIdx.x >> 2, part = threadIdx.x & 3;
-      for (int r = part * 4; r < part * 4 + 4; r++)
-        for (int c = 0; c < kTile; c++) bsum += bf2f(ds[co][r][c]);
     }
   }
   // Partial sums of this workgroup (every element written once: no atomics, which would
@@ -309,35 +340,75 @@ __global__ __launch_bounds__(kThreads) void conv3x3_wgrad(ConvArgs a, int ntiles
       for (int j = 0; j < 4; j++) wp[(tap * 16 + l16) * COUT + mt * 16 + 4 * q + j] = acc[k][j];
     }
   }
-  if (cc == 0) {  // threads 4 co .. 4 co + 3 hold quarters of the bias sum of channel co
-    bsum += __shfl_xor(bsum, 1, 64);
-    bsum += __shfl_xor(bsum, 2, 64);
-    if (threadIdx.x < COUT * 4 && (threadIdx.x & 3) == 0)
-      a.bpart[(size_t)blockIdx.x * COUT + (threadIdx.x >> 2)] = bsum;
+  if (cc == 0) {
+    // threads with equal threadIdx % (COUT / 8) hold the same 8 channels: sum them over
+    // the wave by shuffles, then over the 4 waves in LDS (xs is free by now)
+    constexpr int G = COUT / 8;
+#pragma unroll
+    for (int j = 0; j < 8; j++)
+      for (int o = G; o < 64; o <<= 1) bsum[j] += __shfl_xor(bsum[j], o, 64);
+    __syncthreads();
+    float* red = (float*)&xs[0][0][0][0];  // [4 waves][COUT]
+    if (lane < G)
+#pragma unroll
+      for (int j = 0; j < 8; j++) red[wv * COUT + lane * 8 + j] = bsum[j];
+    __syncthreads();
+    if (threadIdx.x < COUT) {
+      const float v = red[threadIdx.x] + red[COUT + threadIdx.x] + red[2 * COUT + threadIdx.x] +
+                      red[3 * COUT + threadIdx.x];
+      a.bpart[(size_t)blockIdx.x * COUT + threadIdx.x] = v;
+    }
   }
 }
 
-// dW[tap][ci][co] = sum over workgroups of the partials; db[co] likewise (chunk 0).
-__global__ __launch_bounds__(kThreads) void conv3x3_wgrad_reduce(const float* __restrict__ wpart,
-                                                                 const float* __restrict__ bpart,
-                                                                 int nblk, int Cin, int Cout,
-                                                                 float* __restrict__ dw,
-                                                                 float* __restrict__ db) {
+// dW[tap][ci][co] = sum over workgroups of the partials; db[co] likewise (chunk 0).  A
+// workgroup of 1024 threads takes 64 consecutive outputs: its 16 waves sum the
+// workgroups' partials in 16 interleaved groups (lanes read 64 consecutive outputs, eight
+// partials in flight per lane), and wave 0 adds the 16 group sums in a fixed order.
+constexpr int kReduceGroups = 16;
+__global__ __launch_bounds__(1024) void conv3x3_wgrad_reduce(const float* __restrict__ wpart,
+                                                             const float* __restrict__ bpart,
+                                                             int nblk, int Cin, int Cout,
+                                                             float* __restrict__ dw,
+                                                             float* __restrict__ db) {
+  __shared__ float red[kReduceGroups][64];
   const int nw = 9 * Cin * Cout;
-  for (int e = blockIdx.x * blockDim.x + threadIdx.x; e < nw + Cout; e += gridDim.x * blockDim.x) {
-    float s = 0.f;
+  const int e = blockIdx.x * 64 + (threadIdx.x & 63), grp = threadIdx.x >> 6;
+  float s = 0.f;
+  if (e < nw + Cout) {
+    const float* p;
+    size_t stride;
     if (e < nw) {
       const int co = e % Cout, r = e / Cout;
       const int ci = r % Cin, tap = r / Cin;
       const int cc = ci / 16, l = ci % 16;
-      const float* p = wpart + (size_t)cc * nblk * (9 * 16 * Cout) + (tap * 16 + l) * Cout + co;
-      for (int b = 0; b < nblk; b++) s += p[(size_t)b * (9 * 16 * Cout)];
-      dw[e] = s;
+      p = wpart + (size_t)cc * nblk * (9 * 16 * Cout) + (tap * 16 + l) * Cout + co;
+      stride = (size_t)9 * 16 * Cout;
     } else {
-      const int co = e - nw;
-      for (int b = 0; b < nblk; b++) s += bpart[(size_t)b * Cout + co];
-      db[co] = s;
+      p = bpart + (e - nw);
+      stride = (size_t)Cout;
     }
+    constexpr int U = 8;
+    int b = grp;
+    for (; b + (U - 1) * kReduceGroups < nblk; b += U * kReduceGroups) {
+      float v[U];
+#pragma unroll
+      for (int u = 0; u < U; u++) v[u] = p[(size_t)(b + u * kReduceGroups) * stride];
+#pragma unroll
+      for (int u = 0; u < U; u++) s += v[u];
+    }
+    for (; b < nblk; b += kReduceGroups) s += p[(size_t)b * stride];
+  }
+  red[grp][threadIdx.x & 63] = s;
+  __syncthreads();
+  if (grp == 0 && e < nw + Cout) {
+    float t = 0.f;
+#pragma unroll
+    for (int g = 0; g < kReduceGroups; g++) t += red[g][threadIdx.x];
+    if (e < nw)
+      dw[e] = t;
+    else
+      db[e - nw] = t;
   }
 }
 
@@ -918,8 +989,8 @@ void launch_wgrad(const ConvArgs& a, int B, hipStream_t s) {
   }
   ProfScope ps("dn_wgrad_reduce");
   const int ne = 9 * a.Cin * a.Cout + a.Cout;
-  hipLaunchKernelGGL(conv3x3_wgrad_reduce, dim3((ne + kThreads - 1) / kThreads), dim3(kThreads), 0,
-                     s, a.wpart, a.bpart, blocks, a.Cin, a.Cout, a.dw, a.db);
+  hipLaunchKernelGGL(conv3x3_wgrad_reduce, dim3((ne + 63) / 64), dim3(1024), 0, s, a.wpart,
+                     a.bpart, blocks, a.Cin, a.Cout, a.dw, a.db);
 }
 
 int grid_for(int64_t n) {
