@@ -237,8 +237,13 @@ __global__ __launch_bounds__(kThreads) void conv3x3_wgrad(ConvArgs a, int ntiles
   constexpr int NH = (kHalo * kHalo * 2 + kThreads - 1) / kThreads;
   constexpr int ND = (kTile * kTile * COUT / 8) / kThreads;
   static_assert(ND * kThreads == kTile * kTile * COUT / 8, "dY tile split");
-  __shared__ __align__(16) bf16 xs[3][16][kHalo][kTile];
-  __shared__ __align__(16) bf16 ds[COUT][kTile][kTile];
+  // Channel planes padded to 148 (xs) and 132 (ds) dwords, both 4 mod 8: the 16 channels of
+  // an MFMA operand read start on 16 distinct multiples of 4 banks, so the 64 lanes' 8-byte
+  // reads fall two to a bank, and the two 8-channel halves of a staging write are 32 banks
+  // apart.
+  constexpr int SX = kHalo * kTile + 8, SD = kTile * kTile + 8;
+  __shared__ __align__(16) bf16 xs[3][16][SX];
+  __shared__ __align__(16) bf16 ds[COUT][SD];
   const int cc = blockIdx.y;
   const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63, q = lane >> 4, l16 = lane & 15;
   const int tiles_x = a.W / kTile, tiles_img = tiles_x * (a.H / kTile);
@@ -289,7 +294,7 @@ __global__ __launch_bounds__(kThreads) void conv3x3_wgrad(ConvArgs a, int ntiles
 #pragma unroll
         for (int sh = 0; sh < 3; sh++) {
           const int c = hx - sh;
-          if (c >= 0 && c < kTile) xs[sh][h * 8 + j][hy][c] = val;
+          if (c >= 0 && c < kTile) xs[sh][h * 8 + j][hy * kTile + c] = val;
         }
       }
     }
@@ -306,7 +311,7 @@ __global__ __launch_bounds__(kThreads) void conv3x3_wgrad(ConvArgs a, int ntiles
         const bf16 d = (bf16)((d4[j >> 1] >> sh) & 0xFFFFu);
         const bf16 yy = (bf16)((y4[j >> 1] >> sh) & 0xFFFFu);
         const bf16 dm = positive(yy) ? d : (bf16)0;
-        ds[w8 * 8 + j][py][px] = dm;
+        ds[w8 * 8 + j][py * kTile + px] = dm;
         bsum[j] += bf2f(dm);
       }
     }
@@ -320,8 +325,8 @@ __global__ __launch_bounds__(kThreads) void conv3x3_wgrad(ConvArgs a, int ntiles
         const int ky = tap / 3, kx = tap - 3 * ky;
 #pragma unroll 4
         for (int r = 0; r < kTile; r++) {
-          const s4 af = *(const s4*)&ds[mt * 16 + l16][r][4 * q];
-          const s4 bfr = *(const s4*)&xs[kx][l16][r + ky][4 * q];
+          const s4 af = *(const s4*)&ds[mt * 16 + l16][r * kTile + 4 * q];
+          const s4 bfr = *(const s4*)&xs[kx][l16][(r + ky) * kTile + 4 * q];
           acc[k] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(af, bfr, acc[k], 0, 0, 0);
         }
       }
@@ -348,7 +353,7 @@ __global__ __launch_bounds__(kThreads) void conv3x3_wgrad(ConvArgs a, int ntiles
     for (int j = 0; j < 8; j++)
       for (int o = G; o < 64; o <<= 1) bsum[j] += __shfl_xor(bsum[j], o, 64);
     __syncthreads();
-    float* red = (float*)&xs[0][0][0][0];  // [4 waves][COUT]
+    float* red = (float*)&xs[0][0][0];  // [4 waves][COUT]
     if (lane < G)
 #pragma unroll
       for (int j = 0; j < 8; j++) red[wv * COUT + lane * 8 + j] = bsum[j];
