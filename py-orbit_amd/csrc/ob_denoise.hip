@@ -152,8 +152,14 @@ __global__ __launch_bounds__(kThreads) void conv3x3_fwd(ConvArgs a) {
   constexpr int NCO = NT * 16;
   constexpr int XS = kHalo * kHalo * 16;  // bf16 elements of the halo tile
   constexpr int WS = 9 * NCO * 16;        // weights of one 16-channel chunk
-  constexpr int OS = kTile * kTile * NCO; // staged output
+  // staged output: pixel rows of NCO + 8 channels, so that the four pixel groups of an
+  // accumulator write (4 pixels apart) start 16 banks apart
+  constexpr int OP = NCO + 8;
+  constexpr int OS = kTile * kTile * OP;
   constexpr int SMEM = (XS + WS) > OS ? (XS + WS) : OS;
+  // a thread's share of one chunk: NH halo pixel-halves and NW weight runs of 8 channels
+  constexpr int NH = (kHalo * kHalo * 2 + kThreads - 1) / kThreads;
+  constexpr int NW = (9 * NCO * 2 + kThreads - 1) / kThreads;
   __shared__ __align__(16) bf16 smem[SMEM];
   bf16* sx = smem;
   bf16* sw = smem + XS;
@@ -168,21 +174,61 @@ __global__ __launch_bounds__(kThreads) void conv3x3_fwd(ConvArgs a) {
 #pragma unroll
     for (int n = 0; n < NT; n++) acc[m][n] = f4{0.f, 0.f, 0.f, 0.f};
   const int nchunks = (a.Cin + 15) / 16;
-  for (int cc = 0; cc < nchunks; cc++) {
-    __syncthreads();
-    for (int e = threadIdx.x; e < kHalo * kHalo * 2; e += kThreads) {
+  // With two or more output tiles, a chunk's global loads go to registers first, all in
+  // flight together, and the next chunk's are issued before this chunk's MFMAs, so their
+  // latency is hidden behind them.  With one, the registers that takes are worth more as
+  // occupancy (120 against 64 VGPRs, measured slower): its loads go straight to LDS.
+  constexpr bool kPrefetch = NT >= 2;
+  uint4 hv[kPrefetch ? NH : 1], wvv[kPrefetch ? NW : 1];
+  auto fetch = [&](int cc) {
+    if constexpr (!kPrefetch) return;
+#pragma unroll
+    for (int u = 0; u < NH; u++) {
+      const int e = (int)threadIdx.x + u * kThreads;
       const int pix = e >> 1, h = e & 1;
       const int hy = pix / kHalo, hx = pix - hy * kHalo;
-      *(uint4*)&sx[pix * 16 + h * 8] = load8(a, b, ty0 + hy - 1, tx0 + hx - 1, cc * 16 + h * 8);
+      hv[u] = e < kHalo * kHalo * 2 ? load8(a, b, ty0 + hy - 1, tx0 + hx - 1, cc * 16 + h * 8)
+                                    : make_uint4(0, 0, 0, 0);
     }
     const bf16* wsrc = a.w + (size_t)cc * 9 * a.Cout * 16;
-    for (int e = threadIdx.x; e < 9 * NCO * 2; e += kThreads) {
+#pragma unroll
+    for (int u = 0; u < NW; u++) {
+      const int e = (int)threadIdx.x + u * kThreads;
       const int tap = e / (NCO * 2), r = e - tap * (NCO * 2);
       const int n = r >> 1, h = r & 1;
-      *(uint4*)&sw[(tap * NCO + n) * 16 + h * 8] =
-          ld16(wsrc + ((size_t)tap * a.Cout + co0 + n) * 16 + h * 8);
+      wvv[u] = e < 9 * NCO * 2 ? ld16(wsrc + ((size_t)tap * a.Cout + co0 + n) * 16 + h * 8)
+                               : make_uint4(0, 0, 0, 0);
+    }
+  };
+  fetch(0);
+  for (int cc = 0; cc < nchunks; cc++) {
+    __syncthreads();  // the previous chunk's MFMAs are done with sx and sw
+    if constexpr (kPrefetch) {
+#pragma unroll
+      for (int u = 0; u < NH; u++) {
+        const int e = (int)threadIdx.x + u * kThreads;
+        if (e < kHalo * kHalo * 2) *(uint4*)&sx[e * 8] = hv[u];
+      }
+#pragma unroll
+      for (int u = 0; u < NW; u++) {
+        const int e = (int)threadIdx.x + u * kThreads;
+        if (e < 9 * NCO * 2) *(uint4*)&sw[e * 8] = wvv[u];
+      }
+    } else {
+      for (int e = threadIdx.x; e < kHalo * kHalo * 2; e += kThreads) {
+        const int pix = e >> 1, h = e & 1;
+        const int hy = pix / kHalo, hx = pix - hy * kHalo;
+        *(uint4*)&sx[e * 8] = load8(a, b, ty0 + hy - 1, tx0 + hx - 1, cc * 16 + h * 8);
+      }
+      const bf16* wsrc = a.w + (size_t)cc * 9 * a.Cout * 16;
+      for (int e = threadIdx.x; e < 9 * NCO * 2; e += kThreads) {
+        const int tap = e / (NCO * 2), r = e - tap * (NCO * 2);
+        const int n = r >> 1, h = r & 1;
+        *(uint4*)&sw[e * 8] = ld16(wsrc + ((size_t)tap * a.Cout + co0 + n) * 16 + h * 8);
+      }
     }
     __syncthreads();
+    if (cc + 1 < nchunks) fetch(cc + 1);
 #pragma unroll 1
     for (int tap = 0; tap < 9; tap++) {
       const int ky = tap / 3, kx = tap - 3 * ky;
@@ -200,7 +246,7 @@ __global__ __launch_bounds__(kThreads) void conv3x3_fwd(ConvArgs a) {
     }
   }
   __syncthreads();
-  bf16* so = smem;  // [pixel][NCO]
+  bf16* so = smem;  // [pixel][OP]
 #pragma unroll
   for (int m = 0; m < 4; m++)
 #pragma unroll
@@ -211,7 +257,7 @@ __global__ __launch_bounds__(kThreads) void conv3x3_fwd(ConvArgs a) {
       for (int j = 0; j < 4; j++) {
         float v = acc[m][n][j] + bias;
         if (a.relu) v = fmaxf(v, 0.f);
-        so[((wv * 4 + m) * kTile + 4 * q + j) * NCO + ch] = f2bf(v);
+        so[((wv * 4 + m) * kTile + 4 * q + j) * OP + ch] = f2bf(v);
       }
     }
   __syncthreads();
@@ -219,7 +265,7 @@ __global__ __launch_bounds__(kThreads) void conv3x3_fwd(ConvArgs a) {
     const int pix = e / (NCO / 8), u = e - pix * (NCO / 8);
     const int y = ty0 + pix / kTile, x = tx0 + pix % kTile;
     *(uint4*)&a.out[(((size_t)b * a.H + y) * a.W + x) * a.ocs + a.ocoff + co0 + u * 8] =
-        *(const uint4*)&so[pix * NCO + u * 8];
+        *(const uint4*)&so[pix * OP + u * 8];
   }
 }
 
