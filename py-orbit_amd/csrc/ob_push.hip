@@ -473,10 +473,16 @@ __global__ __launch_bounds__(kPushThreads) void push_kernel(DeviceBunchView v, O
   }
 }
 
+// Potentials of at most this many nodes (64 x 64) are copied into LDS by each workgroup
+// of the batch push: the kick's nine gathers per particle then read LDS, where 64
+// random addresses cost a few cycles, instead of the L1/L2 path, where they cost a
+// cache-line lookup per lane.
+constexpr int kBatchPhiLds = 4096;
+
 // All bunches of a batch in one launch: block = one work item (a particle range of one
 // bunch).  The bunch's deferred 2.5D kick comes first (its op is assembled from the
 // bunch's record and the batch's common factors), then the map program; extrema are
-// reduced per work item.
+// reduced per work item.  Dynamic LDS: kBatchPhiLds doubles when the kick is active.
 template <bool EXTREMA, int LEVEL>
 __global__ __launch_bounds__(kPushThreads) void push_batch_kernel(
     const BatchBunch* __restrict__ bb, const BatchItem* __restrict__ items, OpProgram prog,
@@ -502,13 +508,34 @@ __global__ __launch_bounds__(kPushThreads) void push_batch_kernel(
   double* __restrict__ cyp = B.v.c[3];
   double* __restrict__ cz = B.v.c[4];
   double* __restrict__ cdE = B.v.c[5];
+  extern __shared__ double sphi[];
+  const bool lds_phi = kick.active && B.nx * B.ny <= kBatchPhiLds;
+  if (lds_phi) {  // one load per thread and node slot, all in flight together
+    constexpr int KP = kBatchPhiLds / kPushThreads;
+    const int ng = B.nx * B.ny;
+    double v[KP];
+#pragma unroll
+    for (int u = 0; u < KP; u++) {
+      const int k = (int)threadIdx.x + u * kPushThreads;
+      v[u] = B.phi[k < ng ? k : 0];
+    }
+#pragma unroll
+    for (int u = 0; u < KP; u++) {
+      const int k = (int)threadIdx.x + u * kPushThreads;
+      if (k < ng) sphi[k] = v[u];
+    }
+    __syncthreads();
+  }
   int64_t i = it.begin + threadIdx.x;
   double q[6];
   if (i < it.end) load6(cx, cxp, cy, cyp, cz, cdE, i, q);
   for (; i < it.end; i += kPushThreads) {
     double x = q[0], xp = q[1], y = q[2], yp = q[3], z = q[4], dE = q[5];
     if (i + kPushThreads < it.end) load6(cx, cxp, cy, cyp, cz, cdE, i + kPushThreads, q);
-    if (kick.active) sc2p5drb_kick(kop, x, xp, y, yp, z, dE);
+    if (lds_phi)
+      sc2p5drb_kick(kop, sphi, x, xp, y, yp, z, dE);
+    else if (kick.active)
+      sc2p5drb_kick(kop, x, xp, y, yp, z, dE);
     for (int k = 0; k < prog.nops; k++) apply_op<LEVEL>(prog.ops[k], x, xp, y, yp, z, dE);
     cx[i] = x;
     cxp[i] = xp;
@@ -554,8 +581,9 @@ void launch_push_batch(const BatchBunch* bb, const BatchItem* items, int nitems,
   OB_REQUIRE(threads == kPushThreads, "batch push: unsupported block size");
   if (nitems == 0) return;
   const int level = program_level(prog);
+  const size_t lds = kick.active ? sizeof(double) * kBatchPhiLds : 0;
   auto go = [&](auto kernel, double* part) {
-    hipLaunchKernelGGL(kernel, dim3(nitems), dim3(kPushThreads), 0, s, bb, items, prog, kick,
+    hipLaunchKernelGGL(kernel, dim3(nitems), dim3(kPushThreads), lds, s, bb, items, prog, kick,
                        part);
   };
   if (prog.want_extrema) {

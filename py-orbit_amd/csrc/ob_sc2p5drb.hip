@@ -574,6 +574,8 @@ __device__ __forceinline__ void sc2d_post_body(const SC2DParams& P, SC2DState* s
                                                const double* __restrict__ mpart,
                                                const uint32_t* nitems) {
   __shared__ double sh[kSetupThreads / 64];
+  // the z line density, read below many times per bin, is kept in LDS
+  __shared__ double zl[kMaxZBinsLDS];
   const int nz = P.nz;
   if (zpart) {
     // g = blockDim / nz threads per bin (when nz < blockDim) sum the rows in turn, then
@@ -601,9 +603,13 @@ __device__ __forceinline__ void sc2d_post_body(const SC2DParams& P, SC2DState* s
         double t = 0.0;
         for (int k = 0; k < g; k++) t += zs[threadIdx.x + k];
         zg[i] = t;
+        zl[i] = t;
       }
       __syncthreads();
     }
+  } else {
+    for (int i = threadIdx.x; i < nz; i += blockDim.x) zl[i] = zg[i];
+    __syncthreads();
   }
   if (st->skip) {
     for (int i = threadIdx.x; i < nz; i += blockDim.x) zd[i] = 0.0;
@@ -615,7 +621,7 @@ __device__ __forceinline__ void sc2d_post_body(const SC2DParams& P, SC2DState* s
     // SpaceChargeCalc2p5D.cc:229-233, 100: the total macrosize is the sum of the z line
     // density; no derivative and no rho moments are used
     double t = 0.0;
-    for (int i = threadIdx.x; i < nz; i += blockDim.x) t += zg[i];
+    for (int i = threadIdx.x; i < nz; i += blockDim.x) t += zl[i];
     t = block_sum(t, sh);
     if (threadIdx.x == 0) {
       st->total = t;
@@ -636,7 +642,7 @@ __device__ __forceinline__ void sc2d_post_body(const SC2DParams& P, SC2DState* s
         int i0, ip;
         double w0, wp;
         grid1d_ind_wz(zmid, zMin, z_step, nz, i0, ip, w0, wp);
-        zDeriv = (zg[ip] - zg[i0]) / z_step;
+        zDeriv = (zl[ip] - zl[i0]) / z_step;
       }
       zd[0] = zDeriv / z_step;
       zd[1] = zDeriv / z_step;
@@ -656,7 +662,7 @@ __device__ __forceinline__ void sc2d_post_body(const SC2DParams& P, SC2DState* s
           double acc = 0.;
           for (int i = iStart; i < iStop; i++) {
             const double x = 1.0 * (i - iStart + 1);
-            const double y = zg[i];
+            const double y = zl[i];
             acc += ipow(x, j) * ipow(y, k);
           }
           S[j][k] = acc;

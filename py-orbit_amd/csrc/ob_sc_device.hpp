@@ -227,14 +227,15 @@ __device__ __forceinline__ void grid3d_gradient(const double* __restrict__ a, in
 // st->long_kick 0, the SpaceChargeCalc2p5D one (SpaceChargeCalc2p5D.cc:105-119).
 // op: p[0]=SC2DState*, p[1]=phi, p[2]=z line density, p[3]=its derivative,
 //     p[4]=2*log(pipe_radius), p[5]=-L*r0*m/gamma^2; i0=nx, i1=ny, i2=nz.
-__device__ __forceinline__ void sc2p5drb_kick(const Op& op, double x, double& xp, double y,
-                                              double& yp, double z, double& dE) {
+// phi: the potential op.p[1] points to, or a copy of it (in LDS).
+__device__ __forceinline__ void sc2p5drb_kick(const Op& op, const double* __restrict__ phi,
+                                              double x, double& xp, double y, double& yp,
+                                              double z, double& dE) {
   const SC2DState* st = op_ptr<const SC2DState*>(op.p[0]);
   if (st->skip) return;
   if (st->bnd_type && !boundary_inside(st->bnd_type, st->bnd, x, y)) return;  // 2p5D :109
   double ex, ey;
-  grid2d_gradient(op_ptr<const double*>(op.p[1]), op.i0, op.i1, st->xMin, st->dx, st->yMin,
-                  st->dy, x, y, ex, ey);
+  grid2d_gradient(phi, op.i0, op.i1, st->xMin, st->dx, st->yMin, st->dy, x, y, ex, ey);
   const double Lfactor =
       -grid1d_value(op_ptr<const double*>(op.p[2]), z, st->zMin, st->zMax, st->dz, op.i2) *
       st->factor;
@@ -251,6 +252,10 @@ __device__ __forceinline__ void sc2p5drb_kick(const Op& op, double x, double& xp
   else
     coeff = op.p[4] - log(r2);
   dE += ez * op.p[5] * coeff;
+}
+__device__ __forceinline__ void sc2p5drb_kick(const Op& op, double x, double& xp, double y,
+                                              double& yp, double z, double& dE) {
+  sc2p5drb_kick(op, op_ptr<const double*>(op.p[1]), x, xp, y, yp, z, dE);
 }
 
 // SpaceChargeCalc3D kick loop body (SpaceChargeCalc3D.cc:132-147).
