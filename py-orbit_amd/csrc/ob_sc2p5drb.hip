@@ -740,6 +740,7 @@ __global__ __launch_bounds__(kSetupThreads) void sc2d_post(SC2DParams P, SC2DSta
   sc2d_post_body(P, st, rho, zg, zd, tg.zpart, zrows, moments ? tg.mpart : nullptr, tg.nitems);
 }
 
+
 // ---- 5: Poisson solve --------------------------------------------------------
 // (a) two real rows per complex FFT along y, r2c split.  rows >= nrows_in are zero;
 // columns >= ncols_in are zero (padding).  out: spectrum [*][H] complex (H = N/2+1).
@@ -777,6 +778,23 @@ __global__ __launch_bounds__(kFFTThreads) void fft_rows_r2c(const double* __rest
                                                             const double2* __restrict__ tw,
                                                             int nmax, const SC2DState* st) {
   rows_r2c_body<N>(in, nrows_in, ncols_in, ld_in, out, tw, nmax, st, blockIdx.x);
+}
+
+// The single-bunch step's row transforms of rho and its post pass in one launch:
+// workgroups 0 .. rows/2 - 1 transform row pairs, the last one runs the post pass.  Both
+// only read rho; the post pass writes the state's kick factors, z line density and
+// derivative, which the column pass after this launch (scale_coeff) and the next map
+// kernel read.  One launch fewer per step.
+template <int N>
+__global__ __launch_bounds__(kFFTThreads) void fft_rows_r2c_post(
+    const double* __restrict__ rho, int nrows_in, int ncols_in, double2* __restrict__ out,
+    const double2* __restrict__ tw, int nmax, SC2DParams P, SC2DState* st,
+    double* __restrict__ zg, double* __restrict__ zd, Tile2D tg, int zrows, int moments) {
+  if ((int)blockIdx.x < (nrows_in + 1) / 2) {
+    rows_r2c_body<N>(rho, nrows_in, ncols_in, ncols_in, out, tw, nmax, st, blockIdx.x);
+    return;
+  }
+  sc2d_post_body(P, st, rho, zg, zd, tg.zpart, zrows, moments ? tg.mpart : nullptr, tg.nitems);
 }
 
 // (b) column transforms along x (N = M0).  GREEN: full forward transform of all N rows,
@@ -1140,8 +1158,34 @@ void launch_cols(ob_sc2p5drb* c, int nrows, const SC2DState* st, double npad) {
   OB_HIP(hipGetLastError());
 }
 
-void launch_fft_pipeline(ob_sc2p5drb* c, const SC2DState* st) {
-  launch_rows_r2c(c, c->rho, c->nx, c->ny, c->ny, st);
+// post: the post pass's arguments (single-bunch SpaceChargeCalc2p5Drb step), run with the
+// row transforms in one launch; nullptr when it already ran.
+struct PostArgs {
+  SC2DParams P;
+  Tile2D tg;
+  int zrows, moments;
+};
+
+void launch_fft_pipeline(ob_sc2p5drb* c, SC2DState* st, const PostArgs* post = nullptr) {
+  if (post) {
+    dispatch_pow2(c->M1, [&](auto NC) {
+      constexpr int N = decltype(NC)::value;
+      if constexpr (N >= 8) {
+        static bool ready = false;
+        if (!ready) {
+          allow_lds(fft_rows_r2c_post<N>, fft_lds<N>());
+          ready = true;
+        }
+        ob::ProfScope ps_("fft_rows_r2c_post");
+        hipLaunchKernelGGL(fft_rows_r2c_post<N>, dim3((c->nx + 1) / 2 + 1), dim3(kFFTThreads),
+                           fft_lds<N>(), stream(), c->rho, c->nx, c->ny, c->spec, c->tw, c->nmax,
+                           post->P, st, c->zg, c->zd, post->tg, post->zrows, post->moments);
+      }
+    });
+    OB_HIP(hipGetLastError());
+  } else {
+    launch_rows_r2c(c, c->rho, c->nx, c->ny, c->ny, st);
+  }
   launch_cols<false>(c, c->nx, st, (double)c->M0 * (double)c->M1);
   const double denom = 1.0 / (c->M0 * c->M1);
   dispatch_pow2(c->M1, [&](auto NC) {
@@ -1487,12 +1531,17 @@ void sc2d_track(ob_sc2p5drb* c, ob_bunch* b, double length, double pipe_radius,
                        c->st, tg, v.msize != nullptr ? 1 : 0, c->rho, moments);
   }
   if (c->denoiser) dn_denoise_rho(c->denoiser, c->rho, c->nx, c->ny, &c->st->skip);
-  {
-    ProfScope ps_("sc2d_post");
-    hipLaunchKernelGGL(sc2d_post, dim3(1), dim3(kSetupThreads), 0, s, P, c->st, c->rho, c->zg,
-                       c->zd, tg, hb, moments);
+  if (c->plain || bnd) {  // their solve reads the post pass's results: it runs first
+    {
+      ProfScope ps_("sc2d_post");
+      hipLaunchKernelGGL(sc2d_post, dim3(1), dim3(kSetupThreads), 0, s, P, c->st, c->rho,
+                         c->zg, c->zd, tg, hb, moments);
+    }
+    launch_fft_pipeline(c, c->st);
+  } else {
+    const PostArgs pa{P, tg, hb, moments};
+    launch_fft_pipeline(c, c->st, &pa);
   }
-  launch_fft_pipeline(c, c->st);
   if (bnd) boundary_add_potential(bnd, c->phi, c->nx, c->ny, c->st, s);  // :90-94
   OB_HIP(hipGetLastError());
   // kick (:146-167; 2p5D :105-119), fused into the next map kernel
