@@ -203,7 +203,9 @@ float time_ms(hipEvent_t a, hipEvent_t b) {
 
 // Time the candidates on the batch's data: work-item size with an identity push (reads
 // and writes every particle unchanged), then the tile-deposit item size with repeated
-// setup + deposit (setup clears the grids, so the repeats are idempotent).
+// setup + deposit (setup clears the grids, so the repeats are idempotent).  The small
+// candidates serve batches of few, small bunches (a single 100k bunch: 512 particles per
+// item gives it 196 push workgroups instead of 49).
 void autotune(ob_batch* B) {
   hipStream_t s = stream();
   hipEvent_t e0, e1;
@@ -214,7 +216,7 @@ void autotune(ob_batch* B) {
   idp.want_extrema = 1;
   float best = 1e30f;
   int best_chunk = B->chunk;
-  for (int c : {2048, 4096, 16384, 65536}) {
+  for (int c : {512, 1024, 2048, 4096, 16384, 65536}) {
     B->chunk = c;
     build_records(B, false);
     launch_push_batch(B->d_bb, B->d_items, B->shape.nitems, idp, BatchKick{}, B->d_partials,
@@ -234,7 +236,7 @@ void autotune(ob_batch* B) {
   if (!B->calcs.empty()) {
     best = 1e30f;
     int best_item = B->item;
-    for (int item : {1024, 4096}) {
+    for (int item : {256, 1024, 4096}) {
       B->item = item;
       build_records(B, false);
       launch_push_batch(B->d_bb, B->d_items, B->shape.nitems, idp, BatchKick{}, B->d_partials,
