@@ -39,7 +39,8 @@ constexpr int kTile = 16;
 constexpr int kHalo = kTile + 2;
 constexpr int kThreads = 256;
 constexpr int kWgradBlocks = 1024;                         // weight-gradient workgroups
-constexpr size_t kWpartFloats = (size_t)kWgradBlocks * 9 * 16 * 64 + 6 * 9 * 16 * 64;
+// the largest layer need: 3 chunks per workgroup x 32 output channels (or 1 x 64, 2 x 32)
+constexpr size_t kWpartFloats = (size_t)kWgradBlocks * 9 * 16 * 96 + 6 * 9 * 16 * 64;
 constexpr int kHeadBlocks = 512;                           // head / loss workgroups
 
 __device__ __forceinline__ float bf2f(bf16 h) { return __uint_as_float((uint32_t)h << 16); }
@@ -271,15 +272,18 @@ __global__ __launch_bounds__(kThreads) void conv3x3_fwd(ConvArgs a) {
 
 // ---- weight gradient ------------------------------------------------------------------
 // dW[tap][ci][co] += sum_p X[p + off(tap)][ci] * dY'[p][co], dY' = dY where Y > 0.
-// grid: (blocks over all tiles of the batch, Cin chunks of 16).  Pixels are the reduction
-// dimension: the halo tile is stored channel-major in three copies shifted by kx, so every
-// MFMA operand is an aligned 4-pixel run.  Wave w accumulates the (tap, co-tile) pairs
-// w, w + 4, ...; chunk 0 also sums the bias gradient.
-template <int MT>
+// grid: (blocks over all tiles of the batch, groups of CH Cin chunks of 16).  Pixels are the
+// reduction dimension: the halo tile is stored channel-major in three copies shifted by kx,
+// so every MFMA operand is an aligned 4-pixel run.  Wave w accumulates the (tap, co-tile)
+// pairs w, w + 4, ... of each of its CH chunks; dY' is staged once per tile for all of them
+// (a workgroup per chunk would read dY and Y once per chunk).  Chunk 0 also sums the bias
+// gradient.
+template <int MT, int CH>
 __global__ __launch_bounds__(kThreads) void conv3x3_wgrad(ConvArgs a, int ntiles) {
   constexpr int COUT = MT * 16;
   constexpr int NP = (9 * MT + 3) / 4;
-  // a thread's share of one tile: NH halo pixel-halves and ND 8-channel runs of dY and Y
+  // a thread's share of one tile: NH halo pixel-halves per chunk and ND 8-channel runs of
+  // dY and Y
   constexpr int NH = (kHalo * kHalo * 2 + kThreads - 1) / kThreads;
   constexpr int ND = (kTile * kTile * COUT / 8) / kThreads;
   static_assert(ND * kThreads == kTile * kTile * COUT / 8, "dY tile split");
@@ -290,29 +294,34 @@ __global__ __launch_bounds__(kThreads) void conv3x3_wgrad(ConvArgs a, int ntiles
   constexpr int SX = kHalo * kTile + 8, SD = kTile * kTile + 8;
   __shared__ __align__(16) bf16 xs[3][16][SX];
   __shared__ __align__(16) bf16 ds[COUT][SD];
-  const int cc = blockIdx.y;
+  const int cc0 = blockIdx.y * CH;
   const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63, q = lane >> 4, l16 = lane & 15;
   const int tiles_x = a.W / kTile, tiles_img = tiles_x * (a.H / kTile);
-  f4 acc[NP];
+  f4 acc[CH][NP];
 #pragma unroll
-  for (int k = 0; k < NP; k++) acc[k] = f4{0.f, 0.f, 0.f, 0.f};
+  for (int h = 0; h < CH; h++)
+#pragma unroll
+    for (int k = 0; k < NP; k++) acc[h][k] = f4{0.f, 0.f, 0.f, 0.f};
   // bias gradient (chunk 0): a thread's dY runs are always channels w8 * 8 .. w8 * 8 + 7
   // (kThreads is a multiple of COUT / 8), summed in registers as they are staged
   float bsum[8] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
   // The tile's global loads go to registers first, all in flight together; the next tile's
-  // are issued before this tile's MFMAs, so their latency is hidden behind them.
-  uint4 hv[NH], dv[ND], yv[ND];
+  // are issued before this tile's last chunk of MFMAs, so their latency is hidden behind it.
+  uint4 hv[CH][NH], dv[ND], yv[ND];
   auto fetch = [&](int t) {
     const int b = t / tiles_img, ti = t - b * tiles_img;
     const int ty0 = (ti / tiles_x) * kTile, tx0 = (ti % tiles_x) * kTile;
 #pragma unroll
-    for (int u = 0; u < NH; u++) {
-      const int e = (int)threadIdx.x + u * kThreads;
-      const int pix = e >> 1, h = e & 1;
-      const int hy = pix / kHalo, hx = pix - hy * kHalo;
-      hv[u] = e < kHalo * kHalo * 2 ? load8(a, b, ty0 + hy - 1, tx0 + hx - 1, cc * 16 + h * 8)
-                                    : make_uint4(0, 0, 0, 0);
-    }
+    for (int h = 0; h < CH; h++)
+#pragma unroll
+      for (int u = 0; u < NH; u++) {
+        const int e = (int)threadIdx.x + u * kThreads;
+        const int pix = e >> 1, hf = e & 1;
+        const int hy = pix / kHalo, hx = pix - hy * kHalo;
+        hv[h][u] = e < kHalo * kHalo * 2
+                       ? load8(a, b, ty0 + hy - 1, tx0 + hx - 1, (cc0 + h) * 16 + hf * 8)
+                       : make_uint4(0, 0, 0, 0);
+      }
 #pragma unroll
     for (int u = 0; u < ND; u++) {
       const int e = (int)threadIdx.x + u * kThreads;
@@ -327,23 +336,6 @@ __global__ __launch_bounds__(kThreads) void conv3x3_wgrad(ConvArgs a, int ntiles
   if (t < ntiles) fetch(t);
   for (; t < ntiles; t += gridDim.x) {
     __syncthreads();  // the previous tile's MFMAs are done with xs and ds
-#pragma unroll
-    for (int u = 0; u < NH; u++) {
-      const int e = (int)threadIdx.x + u * kThreads;
-      if (e >= kHalo * kHalo * 2) break;
-      const int pix = e >> 1, h = e & 1;
-      const int hy = pix / kHalo, hx = pix - hy * kHalo;
-      const uint32_t w4[4] = {hv[u].x, hv[u].y, hv[u].z, hv[u].w};
-#pragma unroll
-      for (int j = 0; j < 8; j++) {
-        const bf16 val = (bf16)((j & 1) ? (w4[j >> 1] >> 16) : (w4[j >> 1] & 0xFFFFu));
-#pragma unroll
-        for (int sh = 0; sh < 3; sh++) {
-          const int c = hx - sh;
-          if (c >= 0 && c < kTile) xs[sh][h * 8 + j][hy * kTile + c] = val;
-        }
-      }
-    }
 #pragma unroll
     for (int u = 0; u < ND; u++) {
       const int e = (int)threadIdx.x + u * kThreads;
@@ -361,19 +353,40 @@ __global__ __launch_bounds__(kThreads) void conv3x3_wgrad(ConvArgs a, int ntiles
         bsum[j] += bf2f(dm);
       }
     }
-    __syncthreads();
-    if (t + (int)gridDim.x < ntiles) fetch(t + gridDim.x);
 #pragma unroll
-    for (int k = 0; k < NP; k++) {
-      const int p = wv + 4 * k;
-      if (p < 9 * MT) {
-        const int tap = p / MT, mt = p - tap * MT;
-        const int ky = tap / 3, kx = tap - 3 * ky;
+    for (int h = 0; h < CH; h++) {
+      if (h > 0) __syncthreads();  // the previous chunk's MFMAs are done with xs
+#pragma unroll
+      for (int u = 0; u < NH; u++) {
+        const int e = (int)threadIdx.x + u * kThreads;
+        if (e >= kHalo * kHalo * 2) break;
+        const int pix = e >> 1, hf = e & 1;
+        const int hy = pix / kHalo, hx = pix - hy * kHalo;
+        const uint32_t w4[4] = {hv[h][u].x, hv[h][u].y, hv[h][u].z, hv[h][u].w};
+#pragma unroll
+        for (int j = 0; j < 8; j++) {
+          const bf16 val = (bf16)((j & 1) ? (w4[j >> 1] >> 16) : (w4[j >> 1] & 0xFFFFu));
+#pragma unroll
+          for (int sh = 0; sh < 3; sh++) {
+            const int c = hx - sh;
+            if (c >= 0 && c < kTile) xs[sh][hf * 8 + j][hy * kTile + c] = val;
+          }
+        }
+      }
+      __syncthreads();
+      if (h == CH - 1 && t + (int)gridDim.x < ntiles) fetch(t + gridDim.x);
+#pragma unroll
+      for (int k = 0; k < NP; k++) {
+        const int p = wv + 4 * k;
+        if (p < 9 * MT) {
+          const int tap = p / MT, mt = p - tap * MT;
+          const int ky = tap / 3, kx = tap - 3 * ky;
 #pragma unroll 4
-        for (int r = 0; r < kTile; r++) {
-          const s4 af = *(const s4*)&ds[mt * 16 + l16][r * kTile + 4 * q];
-          const s4 bfr = *(const s4*)&xs[kx][l16][(r + ky) * kTile + 4 * q];
-          acc[k] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(af, bfr, acc[k], 0, 0, 0);
+          for (int r = 0; r < kTile; r++) {
+            const s4 af = *(const s4*)&ds[mt * 16 + l16][r * kTile + 4 * q];
+            const s4 bfr = *(const s4*)&xs[kx][l16][(r + ky) * kTile + 4 * q];
+            acc[h][k] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(af, bfr, acc[h][k], 0, 0, 0);
+          }
         }
       }
     }
@@ -381,17 +394,21 @@ __global__ __launch_bounds__(kThreads) void conv3x3_wgrad(ConvArgs a, int ntiles
   // Partial sums of this workgroup (every element written once: no atomics, which would
   // pile up on the same few thousand addresses from every workgroup); conv3x3_wgrad_reduce
   // adds them up.  Layout [chunk][workgroup][tap][16 ci][COUT].
-  float* wp = a.wpart + ((size_t)cc * gridDim.x + blockIdx.x) * (9 * 16 * COUT);
 #pragma unroll
-  for (int k = 0; k < NP; k++) {
-    const int p = wv + 4 * k;
-    if (p < 9 * MT) {
-      const int tap = p / MT, mt = p - tap * MT;
+  for (int h = 0; h < CH; h++) {
+    float* wp = a.wpart + ((size_t)(cc0 + h) * gridDim.x + blockIdx.x) * (9 * 16 * COUT);
 #pragma unroll
-      for (int j = 0; j < 4; j++) wp[(tap * 16 + l16) * COUT + mt * 16 + 4 * q + j] = acc[k][j];
+    for (int k = 0; k < NP; k++) {
+      const int p = wv + 4 * k;
+      if (p < 9 * MT) {
+        const int tap = p / MT, mt = p - tap * MT;
+#pragma unroll
+        for (int j = 0; j < 4; j++)
+          wp[(tap * 16 + l16) * COUT + mt * 16 + 4 * q + j] = acc[h][k][j];
+      }
     }
   }
-  if (cc == 0) {
+  if (cc0 == 0) {
     // threads with equal threadIdx % (COUT / 8) hold the same 8 channels: sum them over
     // the wave by shuffles, then over the 4 waves in LDS (xs is free by now)
     constexpr int G = COUT / 8;
@@ -581,14 +598,26 @@ __global__ __launch_bounds__(kThreads) void head_bwd(const bf16* __restrict__ d1
   if (threadIdx.x == 0) part[blockIdx.x * 18 + 17] = s * inv;
 }
 
-// dhw[c] += sum of the partials, loss += sum of the loss partials.  One block of 64.
+// dhw[c] += sum of the partials, loss += sum of the loss partials.  Workgroup c (18 of
+// them, one wave each) sums column c: lane l adds partials l, l + 64, ... (eight loads in
+// flight), then the wave's lanes are added by a butterfly.
 __global__ __launch_bounds__(64) void head_reduce(const float* __restrict__ part, int nblk,
                                                   float* __restrict__ dhw,
                                                   float* __restrict__ loss) {
-  const int c = threadIdx.x;
-  if (c >= 18) return;
+  const int c = blockIdx.x;
   float s = 0.f;
-  for (int b = 0; b < nblk; b++) s += part[b * 18 + c];
+  constexpr int U = 8;
+  int b = threadIdx.x;
+  for (; b + (U - 1) * 64 < nblk; b += U * 64) {
+    float v[U];
+#pragma unroll
+    for (int u = 0; u < U; u++) v[u] = part[(b + u * 64) * 18 + c];
+#pragma unroll
+    for (int u = 0; u < U; u++) s += v[u];
+  }
+  for (; b < nblk; b += 64) s += part[b * 18 + c];
+  for (int o = 32; o > 0; o >>= 1) s += __shfl_xor(s, o, 64);
+  if (threadIdx.x != 0) return;
   if (c < 17)
     dhw[c] += s;
   else
@@ -646,9 +675,9 @@ __global__ __launch_bounds__(kThreads) void grad_up(const bf16* __restrict__ gu,
 // ---- parameters -------------------------------------------------------------------------
 // fp32 W[9][Cin][Cout] -> packed forward weights and packed data-gradient weights (the
 // transposed, tap-flipped kernel: a conv with Cin' = Cout, Cout' = Cin).
-__global__ __launch_bounds__(kThreads) void pack_layer(const float* __restrict__ W, int Cin,
-                                                       int Cout, bf16* __restrict__ pf,
-                                                       bf16* __restrict__ pt) {
+__device__ __forceinline__ void pack_layer_body(const float* __restrict__ W, int Cin,
+                                                int Cout, bf16* __restrict__ pf,
+                                                bf16* __restrict__ pt) {
   const int Cp = (Cin + 15) / 16 * 16;
   const int nf = 9 * Cp * Cout;
   for (int e = blockIdx.x * blockDim.x + threadIdx.x; e < nf; e += gridDim.x * blockDim.x) {
@@ -669,6 +698,19 @@ __global__ __launch_bounds__(kThreads) void pack_layer(const float* __restrict__
     const int co = ch * 16 + k;
     pt[e] = f2bf(W[((size_t)(8 - tp) * Cin + ci) * Cout + co]);
   }
+}
+
+// Every layer in one launch: blockIdx.y = layer.
+struct PackTable {
+  int32_t w_off[12], cin[12], cout[12];
+  int64_t pf_off[12], pt_off[12];
+};
+__global__ __launch_bounds__(kThreads) void pack_layers(const float* __restrict__ w,
+                                                        bf16* __restrict__ pf,
+                                                        bf16* __restrict__ pt, PackTable t) {
+  const int l = blockIdx.y;
+  pack_layer_body(w + t.w_off[l], t.cin[l], t.cout[l], pf + t.pf_off[l],
+                  pt ? pt + t.pt_off[l] : nullptr);
 }
 
 __global__ __launch_bounds__(kThreads) void adam_step(float* __restrict__ w,
@@ -1006,11 +1048,15 @@ void launch_conv(const ConvArgs& a, int B, hipStream_t s) {
   OB_REQUIRE(a.H % kTile == 0 && a.W % kTile == 0, "denoiser: image sides must be multiples of 64");
   const int tiles = (a.H / kTile) * (a.W / kTile);
   OB_REQUIRE(a.Cout % 16 == 0, "denoiser: output channels must be a multiple of 16");
-  const int nt = (a.Cout % 64 == 0) ? 4 : (a.Cout % 32 == 0) ? 2 : 1;
+  // 48 and 96 output channels (the data gradients of the concatenating layers) take three
+  // 16-channel tiles per workgroup, so the input tile is staged once or twice, not 3 times
+  const int nt = (a.Cout % 64 == 0) ? 4 : (a.Cout % 48 == 0) ? 3 : (a.Cout % 32 == 0) ? 2 : 1;
   const dim3 grid(tiles, a.Cout / (16 * nt), B);
   ProfScope ps("dn_conv");
   if (nt == 4)
     hipLaunchKernelGGL(conv3x3_fwd<4>, grid, dim3(kThreads), 0, s, a);
+  else if (nt == 3)
+    hipLaunchKernelGGL(conv3x3_fwd<3>, grid, dim3(kThreads), 0, s, a);
   else if (nt == 2)
     hipLaunchKernelGGL(conv3x3_fwd<2>, grid, dim3(kThreads), 0, s, a);
   else
@@ -1023,19 +1069,33 @@ int wgrad_blocks(int ntiles, int chunks) {
   return std::max(1, std::min(ntiles, (kWgradBlocks + chunks - 1) / chunks));
 }
 
+// Chunks per workgroup of the weight-gradient pass: as many as the accumulators allow
+// (CH x NP f4 per lane), so dY and Y are read once per CH chunks.
+int wgrad_chunks_per_block(int mt, int chunks) {
+  if (mt == 1) return chunks % 3 == 0 ? 3 : 1;
+  if (mt == 2) return chunks % 3 == 0 ? 3 : chunks % 2 == 0 ? 2 : 1;
+  return 1;
+}
+
 void launch_wgrad(const ConvArgs& a, int B, hipStream_t s) {
   const int ntiles = (a.H / kTile) * (a.W / kTile) * B;
   const int chunks = (a.Cin + 15) / 16;
-  const int blocks = wgrad_blocks(ntiles, chunks);
+  const int mt = a.Cout / 16;
+  const int ch = wgrad_chunks_per_block(mt, chunks);
+  const int blocks = wgrad_blocks(ntiles, chunks / ch);
   OB_REQUIRE((size_t)chunks * blocks * 9 * 16 * a.Cout <= kWpartFloats,
              "denoiser: weight-gradient partials exceed their buffer");
-  const dim3 grid(blocks, chunks);
+  const dim3 grid(blocks, chunks / ch);
   {
     ProfScope ps("dn_wgrad");
-    switch (a.Cout / 16) {
-      case 1: hipLaunchKernelGGL(conv3x3_wgrad<1>, grid, dim3(kThreads), 0, s, a, ntiles); break;
-      case 2: hipLaunchKernelGGL(conv3x3_wgrad<2>, grid, dim3(kThreads), 0, s, a, ntiles); break;
-      case 4: hipLaunchKernelGGL(conv3x3_wgrad<4>, grid, dim3(kThreads), 0, s, a, ntiles); break;
+    const int sel = mt * 4 + ch;
+    switch (sel) {
+      case 1 * 4 + 1: hipLaunchKernelGGL((conv3x3_wgrad<1, 1>), grid, dim3(kThreads), 0, s, a, ntiles); break;
+      case 1 * 4 + 3: hipLaunchKernelGGL((conv3x3_wgrad<1, 3>), grid, dim3(kThreads), 0, s, a, ntiles); break;
+      case 2 * 4 + 1: hipLaunchKernelGGL((conv3x3_wgrad<2, 1>), grid, dim3(kThreads), 0, s, a, ntiles); break;
+      case 2 * 4 + 2: hipLaunchKernelGGL((conv3x3_wgrad<2, 2>), grid, dim3(kThreads), 0, s, a, ntiles); break;
+      case 2 * 4 + 3: hipLaunchKernelGGL((conv3x3_wgrad<2, 3>), grid, dim3(kThreads), 0, s, a, ntiles); break;
+      case 4 * 4 + 1: hipLaunchKernelGGL((conv3x3_wgrad<4, 1>), grid, dim3(kThreads), 0, s, a, ntiles); break;
       default: OB_REQUIRE(false, "denoiser: unsupported output channel count");
     }
   }
@@ -1132,7 +1192,7 @@ void backward(ob_denoiser* d, const float* x, const float* target, int B, hipStr
     const int hb = (int)std::min<int64_t>(kHeadBlocks, (npix + kThreads - 1) / kThreads);
     hipLaunchKernelGGL(head_bwd, dim3(hb), dim3(kThreads), 0, s, d->act[11], d->out, target,
                        d->w + d->head_off, d->gact[11], d->hpart, npix, P);
-    hipLaunchKernelGGL(head_reduce, dim3(1), dim3(64), 0, s, d->hpart, hb, d->g + d->head_off,
+    hipLaunchKernelGGL(head_reduce, dim3(18), dim3(64), 0, s, d->hpart, hb, d->g + d->head_off,
                        d->loss);
   }
   // gradient w.r.t. each layer's output arrives in gact[l] (before its ReLU mask)
@@ -1214,11 +1274,16 @@ void backward(ob_denoiser* d, const float* x, const float* target, int B, hipStr
 
 void pack_all(ob_denoiser* d, hipStream_t s) {
   ProfScope ps("dn_pack");
+  PackTable t;
   for (int l = 0; l < 12; l++) {
     const Layer& L = d->L[l];
-    hipLaunchKernelGGL(pack_layer, dim3(64), dim3(kThreads), 0, s, d->w + L.w_off, L.Cin, L.Cout,
-                       d->pf + L.pf_off, d->pt + L.pt_off);
+    t.w_off[l] = (int32_t)L.w_off;
+    t.cin[l] = L.Cin;
+    t.cout[l] = L.Cout;
+    t.pf_off[l] = (int64_t)L.pf_off;
+    t.pt_off[l] = (int64_t)L.pt_off;
   }
+  hipLaunchKernelGGL(pack_layers, dim3(64, 12), dim3(kThreads), 0, s, d->w, d->pf, d->pt, t);
 }
 
 void check_batch(ob_denoiser* d, int B) {
