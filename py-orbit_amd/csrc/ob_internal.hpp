@@ -184,7 +184,8 @@ struct Tile2D {
   uint32_t* cursor;   // [ntiles] scatter positions taken, cleared before each scatter
   int4* items;        // [max_items] (tile, begin, end, 0)
   uint32_t* nitems;   // [1]
-  double *sx, *sy, *sm;
+  double2* sxy;       // [cap] (x, y) of the in-grid particles, sorted by tile
+  double* sm;         // [cap] their macrosizes (per-particle macrosize only)
   double* zpart;      // [zrows][nz] per-block z line densities (single-bunch path)
   double* mpart;      // [max_items][5] rho moments per deposit item
   uint32_t* icounts;  // [hist_rows][ntiles] tile counts per batch work item (batch path)

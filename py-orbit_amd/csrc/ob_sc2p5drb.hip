@@ -395,8 +395,7 @@ __device__ __forceinline__ void sc2d_scatter_body(const SC2DParams& P, const Dev
         const int k = tile2d_key(tg, P.nx, P.ny, g, x[u], y[u]);
         if (k < 0) continue;
         const uint32_t d = atomicAdd(&lc[k], 1u);
-        tg.sx[d] = x[u];
-        tg.sy[d] = y[u];
+        tg.sxy[d] = make_double2(x[u], y[u]);  // one 16-byte store per particle
         if (v.msize) tg.sm[d] = m[u];
       }
     }
@@ -429,8 +428,7 @@ __device__ __forceinline__ void sc2d_tdeposit_body(const SC2DParams& P, const Gr
   const double xMin = g.xMin, yMin = g.yMin, dx = g.dx, dy = g.dy;
   const int nx = P.nx, ny = P.ny;
   double mom[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
-  const double* __restrict__ sx = tg.sx;
-  const double* __restrict__ sy = tg.sy;
+  const double2* __restrict__ sxy = tg.sxy;
   const double* __restrict__ sm = tg.sm;
   const int stride = blockDim.x;
   for (int i0 = it.y + threadIdx.x; i0 < it.z; i0 += kUnroll * stride) {
@@ -438,8 +436,9 @@ __device__ __forceinline__ void sc2d_tdeposit_body(const SC2DParams& P, const Gr
 #pragma unroll
     for (int u = 0; u < kUnroll; u++) {
       const int i = i0 + u * stride;
-      xs[u] = i < it.z ? sx[i] : 0.0;
-      ys[u] = i < it.z ? sy[i] : 0.0;
+      const double2 v = sxy[i < it.z ? i : it.y];
+      xs[u] = v.x;
+      ys[u] = v.y;
       ms[u] = per_particle_m ? (i < it.z ? sm[i] : 0.0) : P.macrosize;
     }
 #pragma unroll
@@ -1267,12 +1266,12 @@ Tile2D sc2d_tile_reserve(ob_sc2p5drb* c, int64_t n, bool per_particle_m, int ite
   if (n > c->tile_ncap || (per_particle_m && !c->tile_msize) || item != g.item ||
       want_items > g.max_items) {
     OB_HIP(hipStreamSynchronize(stream()));  // buffers may be in use by queued work
-    for (void* p : {(void*)g.sx, (void*)g.sy, (void*)g.sm, (void*)g.items, (void*)g.mpart})
+    for (void* p : {(void*)g.sxy, (void*)g.sm, (void*)g.items, (void*)g.mpart})
       if (p) OB_HIP(hipFree(p));
-    g.sx = g.sy = g.sm = g.mpart = nullptr;
+    g.sxy = nullptr;
+    g.sm = g.mpart = nullptr;
     int64_t cap = std::max<int64_t>(n, std::max<int64_t>(c->tile_ncap, 1024));
-    OB_HIP(hipMalloc((void**)&g.sx, sizeof(double) * cap));
-    OB_HIP(hipMalloc((void**)&g.sy, sizeof(double) * cap));
+    OB_HIP(hipMalloc((void**)&g.sxy, sizeof(double2) * cap));
     c->tile_msize = per_particle_m || c->tile_msize;
     if (c->tile_msize) OB_HIP(hipMalloc((void**)&g.sm, sizeof(double) * cap));
     c->tile_ncap = cap;
@@ -1298,7 +1297,7 @@ Tile2D sc2d_tile_reserve(ob_sc2p5drb* c, int64_t n, bool per_particle_m, int ite
 
 bool sc2d_tile_same(ob_sc2p5drb* c, const Tile2D& t, int64_t n, bool per_particle_m) {
   const Tile2D& g = c->tg;
-  return g.sx == t.sx && g.items == t.items && g.item == t.item && n <= c->tile_ncap &&
+  return g.sxy == t.sxy && g.items == t.items && g.item == t.item && n <= c->tile_ncap &&
          g.icounts == t.icounts && g.hist_rows == t.hist_rows &&
          (!per_particle_m || g.sm != nullptr);
 }
@@ -1461,7 +1460,7 @@ void sc2d_destroy(ob_sc2p5drb* c) {
   for (void* p : {(void*)c->rho, (void*)c->phi, (void*)c->zg, (void*)c->zd, (void*)c->spec,
                   (void*)c->ghat_t, (void*)c->tw, (void*)c->partials, (void*)c->st,
                   (void*)c->gate, (void*)c->gbuf, (void*)c->tg.counts, (void*)c->tg.items,
-                  (void*)c->tg.sx, (void*)c->tg.sy, (void*)c->tg.sm, (void*)c->tg.zpart,
+                  (void*)c->tg.sxy, (void*)c->tg.sm, (void*)c->tg.zpart,
                   (void*)c->tg.mpart, (void*)c->tg.icounts})
     if (p) (void)hipFree(p);
   delete c;
