@@ -737,85 +737,114 @@ __global__ __launch_bounds__(kThreads) void fill_f32(float* __restrict__ p, int6
 }
 
 // ---- use inside a space-charge step -----------------------------------------------------
-// x = rho * n / total (mean 1), total = sum rho.  One block.
-__global__ __launch_bounds__(1024) void rho_to_image(const double* __restrict__ rho, int n,
-                                                     const int32_t* skip, float* __restrict__ x,
-                                                     double* __restrict__ total) {
-  if (*skip) return;
-  __shared__ double sh[16];
-  double s = 0.0;
-  for (int i = threadIdx.x; i < n; i += blockDim.x) s += rho[i];
-  for (int o = 32; o > 0; o >>= 1) s += __shfl_xor(s, o, 64);
-  if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = s;
-  __syncthreads();
-  double t = 0.0;
-  for (int w = 0; w < (int)(blockDim.x >> 6); w++) t += sh[w];
-  const double scale = t > 0.0 ? (double)n / t : 0.0;
-  for (int i = threadIdx.x; i < n; i += blockDim.x) x[i] = (float)(rho[i] * scale);
-  if (threadIdx.x == 0) *total = t;
-}
+// Charge-normalised images for the network and back.  Each grid is split into kRhoParts
+// equal slices; one workgroup of kRhoThreads sums a slice (pass 1), and in pass 2 every
+// workgroup of the grid adds the kRhoParts partial sums in slice order (so all of them see
+// the same total), scales its slice and, for slice 0, stores the total.
+constexpr int kRhoParts = 16;
+constexpr int kRhoThreads = 256;
 
-// rho = out * total / sum(out): the denoised grid with the deposited total charge.
-__global__ __launch_bounds__(1024) void image_to_rho(const float* __restrict__ out, int n,
-                                                     const int32_t* skip,
-                                                     const double* __restrict__ total,
-                                                     double* __restrict__ rho) {
-  if (*skip) return;
-  __shared__ double sh[16];
-  double s = 0.0;
-  for (int i = threadIdx.x; i < n; i += blockDim.x) s += (double)out[i];
-  for (int o = 32; o > 0; o >>= 1) s += __shfl_xor(s, o, 64);
-  if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = s;
-  __syncthreads();
-  double t = 0.0;
-  for (int w = 0; w < (int)(blockDim.x >> 6); w++) t += sh[w];
-  if (t == 0.0) return;  // keep the deposited grid
-  const double scale = *total / t;
-  for (int i = threadIdx.x; i < n; i += blockDim.x) rho[i] = (double)out[i] * scale;
-}
-
-// The same for the rho grids of up to 64 space-charge calculators at once: workgroup j
-// handles grid j (image j of x, total[j]).
+// Up to 64 grids per launch: grid j is rho[j] with skip flag skip[j] and image j.
 struct RhoJobs {
   double* rho[64];
   const int32_t* skip[64];
 };
-__global__ __launch_bounds__(1024) void rho_to_images(RhoJobs jobs, int n, float* __restrict__ x,
-                                                      double* __restrict__ total) {
-  const int j = blockIdx.x;
-  if (*jobs.skip[j]) return;
-  const double* __restrict__ rho = jobs.rho[j];
-  float* __restrict__ xj = x + (size_t)j * n;
-  __shared__ double sh[16];
+
+// Sum of v[a..b) by one workgroup, eight loads in flight per thread, in a fixed order.
+template <typename T>
+__device__ __forceinline__ double slice_sum(const T* __restrict__ v, int a, int b) {
+  __shared__ double sh[kRhoThreads / 64];
+  constexpr int U = 8;
   double s = 0.0;
-  for (int i = threadIdx.x; i < n; i += blockDim.x) s += rho[i];
+  int i = a + threadIdx.x;
+  for (; i + (U - 1) * kRhoThreads < b; i += U * kRhoThreads) {
+    double t[U];
+#pragma unroll
+    for (int u = 0; u < U; u++) t[u] = (double)v[i + u * kRhoThreads];
+#pragma unroll
+    for (int u = 0; u < U; u++) s += t[u];
+  }
+  for (; i < b; i += kRhoThreads) s += (double)v[i];
   for (int o = 32; o > 0; o >>= 1) s += __shfl_xor(s, o, 64);
   if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = s;
   __syncthreads();
   double t = 0.0;
-  for (int w = 0; w < (int)(blockDim.x >> 6); w++) t += sh[w];
-  const double scale = t > 0.0 ? (double)n / t : 0.0;
-  for (int i = threadIdx.x; i < n; i += blockDim.x) xj[i] = (float)(rho[i] * scale);
-  if (threadIdx.x == 0) total[j] = t;
+#pragma unroll
+  for (int w = 0; w < kRhoThreads / 64; w++) t += sh[w];
+  return t;
 }
-__global__ __launch_bounds__(1024) void images_to_rho(const float* __restrict__ out, int n,
-                                                      RhoJobs jobs,
-                                                      const double* __restrict__ total) {
-  const int j = blockIdx.x;
-  if (*jobs.skip[j]) return;
-  const float* __restrict__ oj = out + (size_t)j * n;
-  double* __restrict__ rho = jobs.rho[j];
-  __shared__ double sh[16];
-  double s = 0.0;
-  for (int i = threadIdx.x; i < n; i += blockDim.x) s += (double)oj[i];
-  for (int o = 32; o > 0; o >>= 1) s += __shfl_xor(s, o, 64);
-  if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = s;
-  __syncthreads();
+
+// dst[a..b) = src[a..b) * scale, eight elements in flight per thread.
+template <typename S, typename D>
+__device__ __forceinline__ void slice_scale(const S* __restrict__ src, int a, int b,
+                                            double scale, D* __restrict__ dst) {
+  constexpr int U = 8;
+  int i = a + threadIdx.x;
+  for (; i + (U - 1) * kRhoThreads < b; i += U * kRhoThreads) {
+    double t[U];
+#pragma unroll
+    for (int u = 0; u < U; u++) t[u] = (double)src[i + u * kRhoThreads];
+#pragma unroll
+    for (int u = 0; u < U; u++) dst[i + u * kRhoThreads] = (D)(t[u] * scale);
+  }
+  for (; i < b; i += kRhoThreads) dst[i] = (D)((double)src[i] * scale);
+}
+
+__device__ __forceinline__ double parts_total(const double* __restrict__ part) {
   double t = 0.0;
-  for (int w = 0; w < (int)(blockDim.x >> 6); w++) t += sh[w];
-  if (t == 0.0) return;  // keep the deposited grid
-  const double scale = total[j] / t;
-  for (int i = threadIdx.x; i < n; i += blockDim.x) rho[i] = (double)oj[i] * scale;
+#pragma unroll
+  for (int k = 0; k < kRhoParts; k++) t += part[k];
+  return t;
+}
+
+__device__ __forceinline__ int slice_begin(int n, int k) {
+  return (int)(((int64_t)n * k) / kRhoParts);
+}
+
+// Pass 1 over rho: part[j * kRhoParts + k] = sum of slice k of grid j.  Grid (kRhoParts, count).
+__global__ __launch_bounds__(kRhoThreads) void rho_parts(RhoJobs jobs, int n,
+                                                         double* __restrict__ part) {
+  const int j = blockIdx.y, k = blockIdx.x;
+  if (*jobs.skip[j]) return;
+  const double t = slice_sum(jobs.rho[j], slice_begin(n, k), slice_begin(n, k + 1));
+  if (threadIdx.x == 0) part[j * kRhoParts + k] = t;
+}
+
+// Pass 2: image j = rho_j * n / total_j (mean 1); total[j] = total_j.
+__global__ __launch_bounds__(kRhoThreads) void rho_images(RhoJobs jobs, int n,
+                                                          const double* __restrict__ part,
+                                                          float* __restrict__ x,
+                                                          double* __restrict__ total) {
+  const int j = blockIdx.y, k = blockIdx.x;
+  if (*jobs.skip[j]) return;
+  const double t = parts_total(part + j * kRhoParts);
+  slice_scale(jobs.rho[j], slice_begin(n, k), slice_begin(n, k + 1),
+              t > 0.0 ? (double)n / t : 0.0, x + (size_t)j * n);
+  if (k == 0 && threadIdx.x == 0) total[j] = t;
+}
+
+// Pass 1 over the network's output images.
+__global__ __launch_bounds__(kRhoThreads) void image_parts(const float* __restrict__ out, int n,
+                                                           RhoJobs jobs,
+                                                           double* __restrict__ part) {
+  const int j = blockIdx.y, k = blockIdx.x;
+  if (*jobs.skip[j]) return;
+  const double t = slice_sum(out + (size_t)j * n, slice_begin(n, k), slice_begin(n, k + 1));
+  if (threadIdx.x == 0) part[j * kRhoParts + k] = t;
+}
+
+// Pass 2: rho_j = out_j * total_j / sum(out_j), the denoised grid with the deposited charge;
+// a zero-sum output keeps the deposited grid.
+__global__ __launch_bounds__(kRhoThreads) void images_rho(const float* __restrict__ out, int n,
+                                                          RhoJobs jobs,
+                                                          const double* __restrict__ part,
+                                                          const double* __restrict__ total) {
+  const int j = blockIdx.y, k = blockIdx.x;
+  if (*jobs.skip[j]) return;
+  const double t = parts_total(part + j * kRhoParts);
+  if (t == 0.0) return;
+  slice_scale(out + (size_t)j * n, slice_begin(n, k), slice_begin(n, k + 1), total[j] / t,
+              jobs.rho[j]);
 }
 
 // dst image j <- src image idx[j] (j < count), images of n floats.
@@ -1036,6 +1065,7 @@ struct ob_denoiser {
   float lam[9] = {};
   float* xin = nullptr;   // normalized rho images (space-charge use), max_batch of them
   double* total = nullptr;  // its total charge
+  double* rpart = nullptr;  // per-slice charge sums, kRhoParts per image
 };
 
 namespace ob {
@@ -1286,6 +1316,20 @@ void pack_all(ob_denoiser* d, hipStream_t s) {
   hipLaunchKernelGGL(pack_layers, dim3(64, 12), dim3(kThreads), 0, s, d->w, d->pf, d->pt, t);
 }
 
+// rho grids -> normalised images -> network -> rho grids with their deposited charge.
+void denoise_grids(ob_denoiser* d, const RhoJobs& rj, int count, hipStream_t s) {
+  const int n = d->H * d->W;
+  const dim3 grid(kRhoParts, count);
+  hipLaunchKernelGGL(rho_parts, grid, dim3(kRhoThreads), 0, s, rj, n, d->rpart);
+  hipLaunchKernelGGL(rho_images, grid, dim3(kRhoThreads), 0, s, rj, n, d->rpart, d->xin,
+                     d->total);
+  forward(d, d->xin, count, s);
+  hipLaunchKernelGGL(image_parts, grid, dim3(kRhoThreads), 0, s, d->out, n, rj, d->rpart);
+  hipLaunchKernelGGL(images_rho, grid, dim3(kRhoThreads), 0, s, d->out, n, rj, d->rpart,
+                     d->total);
+  OB_HIP(hipGetLastError());
+}
+
 void check_batch(ob_denoiser* d, int B) {
   OB_REQUIRE(B >= 1 && B <= d->max_batch, "denoiser: batch size out of range");
 }
@@ -1312,24 +1356,19 @@ void dn_denoise_rhos(const std::vector<DenoiseJob>& jobs) {
       count++;
       a++;
     }
-    const int n = d->H * d->W;
     ProfScope ps("dn_sc_denoise_batch");
-    hipLaunchKernelGGL(dn::rho_to_images, dim3(count), dim3(1024), 0, s, rj, n, d->xin, d->total);
-    dn::forward(d, d->xin, count, s);
-    hipLaunchKernelGGL(dn::images_to_rho, dim3(count), dim3(1024), 0, s, d->out, n, rj, d->total);
-    OB_HIP(hipGetLastError());
+    dn::denoise_grids(d, rj, count, s);
   }
 }
 
 void dn_denoise_rho(ob_denoiser* d, double* rho, int nx, int ny, const int32_t* skip) {
   dn_check_shape(d, nx, ny);
   hipStream_t s = stream();
-  const int n = nx * ny;
   ProfScope ps("dn_sc_denoise");
-  hipLaunchKernelGGL(dn::rho_to_image, dim3(1), dim3(1024), 0, s, rho, n, skip, d->xin, d->total);
-  dn::forward(d, d->xin, 1, s);
-  hipLaunchKernelGGL(dn::image_to_rho, dim3(1), dim3(1024), 0, s, d->out, n, skip, d->total, rho);
-  OB_HIP(hipGetLastError());
+  dn::RhoJobs rj;
+  rj.rho[0] = rho;
+  rj.skip[0] = skip;
+  dn::denoise_grids(d, rj, 1, s);
 }
 }  // namespace ob
 
@@ -1402,6 +1441,7 @@ OB_API int ob_dn_create(int H, int W, int max_batch, ob_denoiser** out) {
     OB_HIP(hipMalloc((void**)&d->bpart, sizeof(float) * ob::dn::kWgradBlocks * 64));
     OB_HIP(hipMalloc((void**)&d->hpart, sizeof(float) * ob::dn::kHeadBlocks * 18));
     OB_HIP(hipMalloc((void**)&d->total, sizeof(double) * max_batch));
+    OB_HIP(hipMalloc((void**)&d->rpart, sizeof(double) * ob::dn::kRhoParts * max_batch));
   } catch (...) {
     delete d;  // device buffers of a failed create are reclaimed at process exit
     throw;
@@ -1415,7 +1455,7 @@ OB_API int ob_dn_destroy(ob_denoiser* d) {
   (void)hipStreamSynchronize(ob::stream());
   for (void* p : {(void*)d->w, (void*)d->g, (void*)d->m, (void*)d->v, (void*)d->pf, (void*)d->pt,
                   (void*)d->gcat4, (void*)d->gcat5, (void*)d->gtmp0, (void*)d->gtmp1,
-                  (void*)d->out, (void*)d->loss, (void*)d->xin, (void*)d->total,
+                  (void*)d->out, (void*)d->loss, (void*)d->xin, (void*)d->total, (void*)d->rpart,
                   (void*)d->pyr, (void*)d->wpart, (void*)d->bpart, (void*)d->hpart})
     if (p) (void)hipFree(p);
   for (int l = 0; l < 12; l++) {
