@@ -11,7 +11,7 @@
 //   out = x + head(d1)               1x1 conv, FP32
 //
 // 191,841 parameters.  Weights and activations are BF16 (activations NHWC), products
-// accumulate in FP32 on the matrix cores (v_mfma_f32_16x16x16_bf16); master weights, the
+// accumulate in FP32 on the matrix cores (v_mfma_f32_16x16x32_bf16); master weights, the
 // head and the optimizer (Adam) are FP32.  Every pass is hand-written here: forward, the
 // data and weight gradients, the gradient plumbing of pool / upsample / concat, the loss,
 // Adam, and the generator of training pairs (seeded particles of four distribution
@@ -32,8 +32,16 @@ namespace ob {
 namespace dn {
 
 using bf16 = uint16_t;
-typedef short s4 __attribute__((ext_vector_type(4)));
+typedef __bf16 bf8 __attribute__((ext_vector_type(8)));
 typedef float f4 __attribute__((ext_vector_type(4)));
+
+// v_mfma_f32_16x16x32_bf16: D(16x16) += A(16x32) B(32x16).  Lane l holds row (A) or
+// column (B) l % 16 and eight consecutive k of group l / 16; which k a slot stands for is
+// the caller's choice, as long as A and B use the same one.
+__device__ __forceinline__ f4 mfma32(uint4 a, uint4 b, f4 c) {
+  return __builtin_amdgcn_mfma_f32_16x16x32_bf16(__builtin_bit_cast(bf8, a),
+                                                 __builtin_bit_cast(bf8, b), c, 0, 0, 0);
+}
 
 constexpr int kTile = 16;
 constexpr int kHalo = kTile + 2;
@@ -148,6 +156,14 @@ __device__ __forceinline__ uint4 load8(const ConvArgs& a, int b, int y, int x, i
 // ---- forward / data-gradient convolution -------------------------------------------
 // grid: (tiles of the image, Cout / (16 NT), images).  Packed weights: element
 // ((ci / 16 * 9 + tap) * Cout + co) * 16 + ci % 16, tap = 3 * ky + kx.
+// One MFMA takes two taps of a 16-channel chunk (k = 16 * (tap & 1) + ci % 16): taps
+// (0, 1), (2, 3), (4, 5), (6, 7) and (8, zeros), 5 MFMAs per output tile and chunk.
+// In LDS, halo pixels and weight rows are 32-byte rows of 16 channels whose two 16-byte
+// halves are swapped in rows with bit 3 set: an operand read (16 consecutive rows, one half
+// each) then covers all 64 banks once.
+__device__ __forceinline__ int swz(int row, int h) {
+  return row * 16 + 8 * (h ^ ((row >> 3) & 1));
+}
 template <int NT>
 __global__ __launch_bounds__(kThreads) void conv3x3_fwd(ConvArgs a) {
   constexpr int NCO = NT * 16;
@@ -208,43 +224,48 @@ __global__ __launch_bounds__(kThreads) void conv3x3_fwd(ConvArgs a) {
 #pragma unroll
       for (int u = 0; u < NH; u++) {
         const int e = (int)threadIdx.x + u * kThreads;
-        if (e < kHalo * kHalo * 2) *(uint4*)&sx[e * 8] = hv[u];
+        if (e < kHalo * kHalo * 2) *(uint4*)&sx[swz(e >> 1, e & 1)] = hv[u];
       }
 #pragma unroll
       for (int u = 0; u < NW; u++) {
         const int e = (int)threadIdx.x + u * kThreads;
-        if (e < 9 * NCO * 2) *(uint4*)&sw[e * 8] = wvv[u];
+        if (e < 9 * NCO * 2) *(uint4*)&sw[swz(e >> 1, e & 1)] = wvv[u];
       }
     } else {
       for (int e = threadIdx.x; e < kHalo * kHalo * 2; e += kThreads) {
         const int pix = e >> 1, h = e & 1;
         const int hy = pix / kHalo, hx = pix - hy * kHalo;
-        *(uint4*)&sx[e * 8] = load8(a, b, ty0 + hy - 1, tx0 + hx - 1, cc * 16 + h * 8);
+        *(uint4*)&sx[swz(pix, h)] = load8(a, b, ty0 + hy - 1, tx0 + hx - 1, cc * 16 + h * 8);
       }
       const bf16* wsrc = a.w + (size_t)cc * 9 * a.Cout * 16;
       for (int e = threadIdx.x; e < 9 * NCO * 2; e += kThreads) {
         const int tap = e / (NCO * 2), r = e - tap * (NCO * 2);
         const int n = r >> 1, h = r & 1;
-        *(uint4*)&sw[e * 8] = ld16(wsrc + ((size_t)tap * a.Cout + co0 + n) * 16 + h * 8);
+        *(uint4*)&sw[swz(e >> 1, h)] = ld16(wsrc + ((size_t)tap * a.Cout + co0 + n) * 16 + h * 8);
       }
     }
     __syncthreads();
     if (cc + 1 < nchunks) fetch(cc + 1);
-#pragma unroll 1
-    for (int tap = 0; tap < 9; tap++) {
+    // lane group q: tap 2 tp + q / 2, channels 8 (q % 2) .. 8 (q % 2) + 7
+    const int h = q & 1;
+    auto step = [&](int tap, bool live) {
       const int ky = tap / 3, kx = tap - 3 * ky;
-      s4 bfr[NT];
+      uint4 bfr[NT];
 #pragma unroll
-      for (int n = 0; n < NT; n++) bfr[n] = *(const s4*)&sw[(tap * NCO + n * 16 + l16) * 16 + 4 * q];
+      for (int n = 0; n < NT; n++)
+        bfr[n] = live ? *(const uint4*)&sw[swz(tap * NCO + n * 16 + l16, h)] : make_uint4(0, 0, 0, 0);
 #pragma unroll
       for (int m = 0; m < 4; m++) {
         const int r = wv * 4 + m;
-        const s4 afr = *(const s4*)&sx[((r + ky) * kHalo + l16 + kx) * 16 + 4 * q];
+        const uint4 afr = live ? *(const uint4*)&sx[swz((r + ky) * kHalo + l16 + kx, h)]
+                               : make_uint4(0, 0, 0, 0);
 #pragma unroll
-        for (int n = 0; n < NT; n++)
-          acc[m][n] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(afr, bfr[n], acc[m][n], 0, 0, 0);
+        for (int n = 0; n < NT; n++) acc[m][n] = mfma32(afr, bfr[n], acc[m][n]);
       }
-    }
+    };
+#pragma unroll 1
+    for (int tp = 0; tp < 4; tp++) step(2 * tp + (q >> 1), true);
+    step(8, q < 2);
   }
   __syncthreads();
   bf16* so = smem;  // [pixel][OP]
@@ -274,7 +295,8 @@ __global__ __launch_bounds__(kThreads) void conv3x3_fwd(ConvArgs a) {
 // dW[tap][ci][co] += sum_p X[p + off(tap)][ci] * dY'[p][co], dY' = dY where Y > 0.
 // grid: (blocks over all tiles of the batch, groups of CH Cin chunks of 16).  Pixels are the
 // reduction dimension: the halo tile is stored channel-major in three copies shifted by kx,
-// so every MFMA operand is an aligned 4-pixel run.  Wave w accumulates the (tap, co-tile)
+// so every MFMA operand is an aligned 8-pixel run (an MFMA takes two tile rows, 32
+// pixels).  Wave w accumulates the (tap, co-tile)
 // pairs w, w + 4, ... of each of its CH chunks; dY' is staged once per tile for all of them
 // (a workgroup per chunk would read dY and Y once per chunk).  Chunk 0 also sums the bias
 // gradient.
@@ -288,9 +310,9 @@ __global__ __launch_bounds__(kThreads) void conv3x3_wgrad(ConvArgs a, int ntiles
   constexpr int ND = (kTile * kTile * COUT / 8) / kThreads;
   static_assert(ND * kThreads == kTile * kTile * COUT / 8, "dY tile split");
   // Channel planes padded to 148 (xs) and 132 (ds) dwords, both 4 mod 8: the 16 channels of
-  // an MFMA operand read start on 16 distinct multiples of 4 banks, so the 64 lanes' 8-byte
-  // reads fall two to a bank, and the two 8-channel halves of a staging write are 32 banks
-  // apart.
+  // an MFMA operand read start on 16 distinct multiples of 4 banks, so each 16 lanes'
+  // 16-byte reads cover the 64 banks once, and the two 8-channel halves of a staging write
+  // are 32 banks apart.
   constexpr int SX = kHalo * kTile + 8, SD = kTile * kTile + 8;
   __shared__ __align__(16) bf16 xs[3][16][SX];
   __shared__ __align__(16) bf16 ds[COUT][SD];
@@ -381,11 +403,12 @@ __global__ __launch_bounds__(kThreads) void conv3x3_wgrad(ConvArgs a, int ntiles
         if (p < 9 * MT) {
           const int tap = p / MT, mt = p - tap * MT;
           const int ky = tap / 3, kx = tap - 3 * ky;
-#pragma unroll 4
-          for (int r = 0; r < kTile; r++) {
-            const s4 af = *(const s4*)&ds[mt * 16 + l16][r * kTile + 4 * q];
-            const s4 bfr = *(const s4*)&xs[kx][l16][(r + ky) * kTile + 4 * q];
-            acc[h][k] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(af, bfr, acc[h][k], 0, 0, 0);
+          // k = 32 pixels: tile rows r and r + 1, lane group q the 8-pixel run 8 q .. 8 q + 7
+#pragma unroll 2
+          for (int r = 0; r < kTile; r += 2) {
+            const uint4 af = *(const uint4*)&ds[mt * 16 + l16][r * kTile + 8 * q];
+            const uint4 bfr = *(const uint4*)&xs[kx][l16][(r + ky) * kTile + 8 * q];
+            acc[h][k] = mfma32(af, bfr, acc[h][k]);
           }
         }
       }
@@ -1100,10 +1123,11 @@ int wgrad_blocks(int ntiles, int chunks) {
 }
 
 // Chunks per workgroup of the weight-gradient pass: as many as the accumulators allow
-// (CH x NP f4 per lane), so dY and Y are read once per CH chunks.
+// (CH x NP f4 per lane), so dY and Y are read once per CH chunks.  At MT = 2, CH = 3 needs
+// 247 registers and measured slower than CH = 2 (241 against 148 us for 6 chunks).
 int wgrad_chunks_per_block(int mt, int chunks) {
   if (mt == 1) return chunks % 3 == 0 ? 3 : 1;
-  if (mt == 2) return chunks % 3 == 0 ? 3 : chunks % 2 == 0 ? 2 : 1;
+  if (mt == 2) return chunks % 2 == 0 ? 2 : 1;
   return 1;
 }
 
@@ -1124,7 +1148,6 @@ void launch_wgrad(const ConvArgs& a, int B, hipStream_t s) {
       case 1 * 4 + 3: hipLaunchKernelGGL((conv3x3_wgrad<1, 3>), grid, dim3(kThreads), 0, s, a, ntiles); break;
       case 2 * 4 + 1: hipLaunchKernelGGL((conv3x3_wgrad<2, 1>), grid, dim3(kThreads), 0, s, a, ntiles); break;
       case 2 * 4 + 2: hipLaunchKernelGGL((conv3x3_wgrad<2, 2>), grid, dim3(kThreads), 0, s, a, ntiles); break;
-      case 2 * 4 + 3: hipLaunchKernelGGL((conv3x3_wgrad<2, 3>), grid, dim3(kThreads), 0, s, a, ntiles); break;
       case 4 * 4 + 1: hipLaunchKernelGGL((conv3x3_wgrad<4, 1>), grid, dim3(kThreads), 0, s, a, ntiles); break;
       default: OB_REQUIRE(false, "denoiser: unsupported output channel count");
     }
