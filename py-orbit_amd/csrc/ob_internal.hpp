@@ -138,7 +138,8 @@ struct SC2DState {
 };
 
 // Boundary2D inside test (BaseBoundary2D.cc:139-147, ShapedBoundary2D.cc:284-304).
-__host__ __device__ inline bool boundary_inside(int type, const double* b, double x, double y) {
+template <typename P>
+__host__ __device__ inline bool boundary_inside(int type, P b, double x, double y) {
   switch (type) {
     case 1: return x > b[0] && x < b[1] && y > b[2] && y < b[3];
     case 2: return x * x + y * y < b[4] * b[4];

@@ -404,10 +404,8 @@ __device__ __forceinline__ double wave_max(double v) {
   return v;
 }
 
-__device__ __forceinline__ void load6(const double* __restrict__ a, const double* __restrict__ b,
-                                      const double* __restrict__ c, const double* __restrict__ d,
-                                      const double* __restrict__ e, const double* __restrict__ f,
-                                      int64_t i, double q[6]) {
+template <typename P>
+__device__ __forceinline__ void load6(P a, P b, P c, P d, P e, P f, int64_t i, double q[6]) {
   q[0] = a[i];
   q[1] = b[i];
   q[2] = c[i];
@@ -416,27 +414,42 @@ __device__ __forceinline__ void load6(const double* __restrict__ a, const double
   q[5] = f[i];
 }
 
-template <bool EXTREMA, int LEVEL>
+// KICK0: the program's first op is a SpaceChargeCalc2p5Drb kick (the usual case: the
+// deferred kick of the previous SC node); it runs outside the op interpreter, with the
+// calculator's values read once, before any store (scalar loads).  Without it the kernel
+// keeps the interpreter's smaller register footprint.
+template <bool EXTREMA, int LEVEL, bool KICK0>
 __global__ __launch_bounds__(kPushThreads) void push_kernel(DeviceBunchView v, OpProgram prog,
                                                             double* __restrict__ partials) {
   double lo[3] = {DBL_MAX, DBL_MAX, DBL_MAX};
   double hi[3] = {-DBL_MAX, -DBL_MAX, -DBL_MAX};
   const int64_t stride = (int64_t)gridDim.x * blockDim.x;
-  double* __restrict__ cx = v.c[0];
-  double* __restrict__ cxp = v.c[1];
-  double* __restrict__ cy = v.c[2];
-  double* __restrict__ cyp = v.c[3];
-  double* __restrict__ cz = v.c[4];
-  double* __restrict__ cdE = v.c[5];
-  // The next particle's coordinates are loaded before this one is computed, so their
-  // latency overlaps the map arithmetic (few waves fit: the map switch needs many VGPRs).
+  const gptr<double> cx = to_global(v.c[0]);
+  const gptr<double> cxp = to_global(v.c[1]);
+  const gptr<double> cy = to_global(v.c[2]);
+  const gptr<double> cyp = to_global(v.c[3]);
+  const gptr<double> cz = to_global(v.c[4]);
+  const gptr<double> cdE = to_global(v.c[5]);
+  const Op& k0 = prog.ops[0];
+  SC2DKickParams kp;
+  if constexpr (KICK0) kp = sc2d_kick_params(op_ptr<const SC2DState*>(k0.p[0]));
+  // The next particle's coordinates are loaded before this one is computed (after its
+  // kick's loads), so their latency overlaps the kick and map arithmetic (few waves fit:
+  // the map switch needs many VGPRs).
   int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   double q[6];
   if (i < v.n) load6(cx, cxp, cy, cyp, cz, cdE, i, q);
   for (; i < v.n; i += stride) {
     double x = q[0], xp = q[1], y = q[2], yp = q[3], z = q[4], dE = q[5];
+    SC2DKickLoads L;
+    if constexpr (KICK0)
+      sc2p5drb_kick_loads(kp, to_global(op_ptr<const double*>(k0.p[1])), k0.i0, k0.i1,
+                          to_global(op_ptr<const double*>(k0.p[2])),
+                          to_global(op_ptr<const double*>(k0.p[3])), k0.i2, x, y, z, L);
     if (i + stride < v.n) load6(cx, cxp, cy, cyp, cz, cdE, i + stride, q);
-    for (int k = 0; k < prog.nops; k++) apply_op<LEVEL>(prog.ops[k], x, xp, y, yp, z, dE);
+    if constexpr (KICK0) sc2p5drb_kick_apply(kp, L, k0.i2, k0.p[4], k0.p[5], x, xp, y, yp, dE);
+    for (int k = KICK0 ? 1 : 0; k < prog.nops; k++)
+      apply_op<LEVEL>(prog.ops[k], x, xp, y, yp, z, dE);
     cx[i] = x;
     cxp[i] = xp;
     cy[i] = y;
@@ -484,40 +497,37 @@ constexpr int kBatchPhiLds = 4096;
 // bunch's record and the batch's common factors), then the map program; extrema are
 // reduced per work item.  Dynamic LDS: kBatchPhiLds doubles when the kick is active.
 template <bool EXTREMA, int LEVEL>
-__global__ __launch_bounds__(kPushThreads) void push_batch_kernel(
-    const BatchBunch* __restrict__ bb, const BatchItem* __restrict__ items, OpProgram prog,
-    BatchKick kick, double* __restrict__ partials) {
+__device__ __forceinline__ void push_batch_body(const BatchBunch* __restrict__ bb,
+                                                const BatchItem* __restrict__ items,
+                                                const OpProgram& prog, const BatchKick& kick,
+                                                double* __restrict__ partials) {
   const BatchItem it = items[blockIdx.x];
   const BatchBunch& B = bb[it.bunch];
-  Op kop;
-  kop.kind = OP_SC2P5DRB_KICK;
-  kop.i0 = B.nx;
-  kop.i1 = B.ny;
-  kop.i2 = B.nz;
-  kop.p[0] = __longlong_as_double((long long)(uintptr_t)B.st);
-  kop.p[1] = __longlong_as_double((long long)(uintptr_t)B.phi);
-  kop.p[2] = __longlong_as_double((long long)(uintptr_t)B.zg);
-  kop.p[3] = __longlong_as_double((long long)(uintptr_t)B.zd);
-  kop.p[4] = kick.p4;
-  kop.p[5] = kick.p5;
+  // the kick's values for this step, before any store of the kernel (scalar loads)
+  SC2DKickParams kp;
+  if (kick.active) kp = sc2d_kick_params(B.st);
+  const int nx = B.nx, ny = B.ny, nz = B.nz;
+  const gptr<const double> gphi = to_global((const double*)B.phi);
+  const gptr<const double> zg = to_global((const double*)B.zg);
+  const gptr<const double> zd = to_global((const double*)B.zd);
   double lo[3] = {DBL_MAX, DBL_MAX, DBL_MAX};
   double hi[3] = {-DBL_MAX, -DBL_MAX, -DBL_MAX};
-  double* __restrict__ cx = B.v.c[0];
-  double* __restrict__ cxp = B.v.c[1];
-  double* __restrict__ cy = B.v.c[2];
-  double* __restrict__ cyp = B.v.c[3];
-  double* __restrict__ cz = B.v.c[4];
-  double* __restrict__ cdE = B.v.c[5];
+  const gptr<double> cx = to_global(B.v.c[0]);
+  const gptr<double> cxp = to_global(B.v.c[1]);
+  const gptr<double> cy = to_global(B.v.c[2]);
+  const gptr<double> cyp = to_global(B.v.c[3]);
+  const gptr<double> cz = to_global(B.v.c[4]);
+  const gptr<double> cdE = to_global(B.v.c[5]);
   extern __shared__ double sphi[];
-  const bool lds_phi = kick.active && B.nx * B.ny <= kBatchPhiLds;
+  const bool lds_phi = kick.active && nx * ny <= kBatchPhiLds;
   if (lds_phi) {  // one load per thread and node slot, all in flight together
     constexpr int KP = kBatchPhiLds / kPushThreads;
-    const int ng = B.nx * B.ny;
+    const int ng = nx * ny;
     double v[KP];
 #pragma unroll
     for (int u = 0; u < KP; u++) {
       const int k = (int)threadIdx.x + u * kPushThreads;
-      v[u] = B.phi[k < ng ? k : 0];
+      v[u] = gphi[k < ng ? k : 0];
     }
 #pragma unroll
     for (int u = 0; u < KP; u++) {
@@ -531,11 +541,14 @@ __global__ __launch_bounds__(kPushThreads) void push_batch_kernel(
   if (i < it.end) load6(cx, cxp, cy, cyp, cz, cdE, i, q);
   for (; i < it.end; i += kPushThreads) {
     double x = q[0], xp = q[1], y = q[2], yp = q[3], z = q[4], dE = q[5];
-    if (i + kPushThreads < it.end) load6(cx, cxp, cy, cyp, cz, cdE, i + kPushThreads, q);
+    // this particle's kick loads, then the next particle's coordinates, then the arithmetic
+    SC2DKickLoads L;
     if (lds_phi)
-      sc2p5drb_kick(kop, sphi, x, xp, y, yp, z, dE);
+      sc2p5drb_kick_loads(kp, sphi, nx, ny, zg, zd, nz, x, y, z, L);
     else if (kick.active)
-      sc2p5drb_kick(kop, x, xp, y, yp, z, dE);
+      sc2p5drb_kick_loads(kp, gphi, nx, ny, zg, zd, nz, x, y, z, L);
+    if (i + kPushThreads < it.end) load6(cx, cxp, cy, cyp, cz, cdE, i + kPushThreads, q);
+    if (kick.active) sc2p5drb_kick_apply(kp, L, nz, kick.p4, kick.p5, x, xp, y, yp, dE);
     for (int k = 0; k < prog.nops; k++) apply_op<LEVEL>(prog.ops[k], x, xp, y, yp, z, dE);
     cx[i] = x;
     cxp[i] = xp;
@@ -573,6 +586,23 @@ __global__ __launch_bounds__(kPushThreads) void push_batch_kernel(
   }
 }
 
+
+// Levels 0 and 1 fit three waves per SIMD (168 VGPRs) without spilling, at most 12 bytes
+// at level 1; the bound measured 7% faster than the compiler's own 169-register, two-wave
+// allocation.  Level 2 would spill 220 bytes at that bound and keeps two waves.
+template <bool EXTREMA, int LEVEL>
+__global__ __launch_bounds__(kPushThreads) __attribute__((amdgpu_waves_per_eu(3))) void
+push_batch_kernel(const BatchBunch* __restrict__ bb, const BatchItem* __restrict__ items,
+                  OpProgram prog, BatchKick kick, double* __restrict__ partials) {
+  push_batch_body<EXTREMA, LEVEL>(bb, items, prog, kick, partials);
+}
+template <bool EXTREMA>
+__global__ __launch_bounds__(kPushThreads) void push_batch_kernel_l2(
+    const BatchBunch* __restrict__ bb, const BatchItem* __restrict__ items, OpProgram prog,
+    BatchKick kick, double* __restrict__ partials) {
+  push_batch_body<EXTREMA, 2>(bb, items, prog, kick, partials);
+}
+
 }  // namespace
 
 void launch_push_batch(const BatchBunch* bb, const BatchItem* items, int nitems,
@@ -590,7 +620,7 @@ void launch_push_batch(const BatchBunch* bb, const BatchItem* items, int nitems,
     OB_REQUIRE(partials != nullptr, "batch push: extrema requested without a buffer");
     ProfScope ps("push_batch_extrema");
     if (level == 2)
-      go(push_batch_kernel<true, 2>, partials);
+      go(push_batch_kernel_l2<true>, partials);
     else if (level == 1)
       go(push_batch_kernel<true, 1>, partials);
     else
@@ -599,7 +629,7 @@ void launch_push_batch(const BatchBunch* bb, const BatchItem* items, int nitems,
     if (prog.nops == 0 && !kick.active) return;
     ProfScope ps("push_batch");
     if (level == 2)
-      go(push_batch_kernel<false, 2>, nullptr);
+      go(push_batch_kernel_l2<false>, nullptr);
     else if (level == 1)
       go(push_batch_kernel<false, 1>, nullptr);
     else
@@ -619,6 +649,8 @@ int launch_push(const DeviceBunchView& v, const OpProgram& prog, double* ext_par
                 hipStream_t s) {
   const int blocks = push_grid_blocks(v.n);
   const int level = program_level(prog);
+  // levels 0 and 1 take a leading 2.5D kick out of the interpreter
+  const bool kick0 = level < 2 && prog.nops > 0 && prog.ops[0].kind == OP_SC2P5DRB_KICK;
   auto go = [&](auto kernel, double* part) {
     hipLaunchKernelGGL(kernel, dim3(blocks), dim3(kPushThreads), 0, s, v, prog, part);
   };
@@ -626,20 +658,22 @@ int launch_push(const DeviceBunchView& v, const OpProgram& prog, double* ext_par
     OB_REQUIRE(ext_partials != nullptr, "push: extrema requested without a buffer");
     ProfScope ps("push_extrema");
     if (level == 2)
-      go(push_kernel<true, 2>, ext_partials);
+      go(push_kernel<true, 2, false>, ext_partials);
     else if (level == 1)
-      go(push_kernel<true, 1>, ext_partials);
+      kick0 ? go(push_kernel<true, 1, true>, ext_partials)
+            : go(push_kernel<true, 1, false>, ext_partials);
     else
-      go(push_kernel<true, 0>, ext_partials);
+      kick0 ? go(push_kernel<true, 0, true>, ext_partials)
+            : go(push_kernel<true, 0, false>, ext_partials);
   } else {
     if (prog.nops == 0 || v.n == 0) return 0;
     ProfScope ps("push");
     if (level == 2)
-      go(push_kernel<false, 2>, nullptr);
+      go(push_kernel<false, 2, false>, nullptr);
     else if (level == 1)
-      go(push_kernel<false, 1>, nullptr);
+      kick0 ? go(push_kernel<false, 1, true>, nullptr) : go(push_kernel<false, 1, false>, nullptr);
     else
-      go(push_kernel<false, 0>, nullptr);
+      kick0 ? go(push_kernel<false, 0, true>, nullptr) : go(push_kernel<false, 0, false>, nullptr);
   }
   OB_HIP(hipGetLastError());
   return blocks;

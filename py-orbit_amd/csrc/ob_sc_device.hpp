@@ -17,6 +17,18 @@ __device__ __forceinline__ T op_ptr(double d) {
   return reinterpret_cast<T>(__double_as_longlong(d));
 }
 
+// Every buffer of the library is global memory, but a pointer read from a struct in device
+// memory is generic to the compiler: its accesses become flat instructions, which count on
+// the vector-memory and the LDS/scalar counters alike, so each wait for one of them drains
+// both.  Accesses through a gptr are global instructions; a uniform load through one that
+// no store of the kernel can precede becomes a scalar load.
+template <typename T>
+using gptr = __attribute__((address_space(1))) T*;
+template <typename T>
+__device__ __forceinline__ gptr<T> to_global(T* p) {
+  return (gptr<T>)p;
+}
+
 // Grid2D::getIndAndFracX/Y (Grid2D.cc:337-350); Grid3D.cc:212-224 is identical.
 __device__ __forceinline__ void node_ind_frac(double v, double vmin, double d, int n, int& ind,
                                               double& frac) {
@@ -56,8 +68,9 @@ __device__ __forceinline__ void grid1d_ind_wz(double z, double zMin, double dz, 
 }
 
 // Grid1D::getValue (Grid1D.cc:191-210): zero outside [zMin, zMax].
-__device__ __forceinline__ double grid1d_value(const double* __restrict__ arr, double z,
-                                               double zMin, double zMax, double dz, int nz) {
+template <typename P>
+__device__ __forceinline__ double grid1d_value(P arr, double z, double zMin, double zMax,
+                                               double dz, int nz) {
   if (z < zMin || z > zMax) return 0.;
   int i0, ip;
   double w0, wp;
@@ -66,14 +79,31 @@ __device__ __forceinline__ double grid1d_value(const double* __restrict__ arr, d
   return arr[0];
 }
 
-// Grid2D::calcGradient(x, y) (Grid2D.cc:212-256) on a row-major [nx][ny] array.
-__device__ __forceinline__ void grid2d_gradient(const double* __restrict__ a, int nx, int ny,
-                                                double xMin, double dx, double yMin, double dy,
-                                                double x, double y, double& ex, double& ey) {
-  int iX, iY;
+// Grid2D::calcGradient(x, y) (Grid2D.cc:212-256) on a row-major [nx][ny] array, in two
+// stages: the 3 x 3 values around the particle's node, then the weighted sums.
+struct Grad2DLoads {
   double xF, yF;
-  node_ind_frac(x, xMin, dx, nx, iX, xF);
-  node_ind_frac(y, yMin, dy, ny, iY, yF);
+  double a00, a01, a02, a10, a11, a12, a20, a21, a22;
+};
+template <typename P>
+__device__ __forceinline__ void grid2d_gradient_loads(P a, int nx, int ny, double xMin, double dx,
+                                                      double yMin, double dy, double x, double y,
+                                                      Grad2DLoads& g) {
+  int iX, iY;
+  node_ind_frac(x, xMin, dx, nx, iX, g.xF);
+  node_ind_frac(y, yMin, dy, ny, iY, g.yF);
+  const auto r0 = a + (int64_t)(iX - 1) * ny + iY;
+  const auto r1 = a + (int64_t)iX * ny + iY;
+  const auto r2 = a + (int64_t)(iX + 1) * ny + iY;
+  g.a00 = r0[-1]; g.a01 = r0[0]; g.a02 = r0[1];
+  g.a10 = r1[-1]; g.a11 = r1[0]; g.a12 = r1[1];
+  g.a20 = r2[-1]; g.a21 = r2[0]; g.a22 = r2[1];
+}
+__device__ __forceinline__ void grid2d_gradient_sums(const Grad2DLoads& g, double dx, double dy,
+                                                     double& ex, double& ey) {
+  const double xF = g.xF, yF = g.yF;
+  const double a00 = g.a00, a01 = g.a01, a02 = g.a02, a10 = g.a10, a11 = g.a11, a12 = g.a12;
+  const double a20 = g.a20, a21 = g.a21, a22 = g.a22;
   const double xF2 = xF * xF, yF2 = yF * yF;
   const double Wxm = 0.5 * (0.25 - xF + xF2), Wx0 = 0.75 - xF2, Wxp = 0.5 * (0.25 + xF + xF2);
   const double Wym = 0.5 * (0.25 - yF + yF2), Wy0 = 0.75 - yF2, Wyp = 0.5 * (0.25 + yF + yF2);
@@ -81,12 +111,6 @@ __device__ __forceinline__ void grid2d_gradient(const double* __restrict__ a, in
                dWxp = (-1.0) * (-(0.5 + xF));
   const double dWym = (-1.0) * (0.5 - yF), dWy0 = (-1.0) * (+2.) * yF,
                dWyp = (-1.0) * (-(0.5 + yF));
-  const double* r0 = a + (int64_t)(iX - 1) * ny + iY;
-  const double* r1 = a + (int64_t)iX * ny + iY;
-  const double* r2 = a + (int64_t)(iX + 1) * ny + iY;
-  const double a00 = r0[-1], a01 = r0[0], a02 = r0[1];
-  const double a10 = r1[-1], a11 = r1[0], a12 = r1[1];
-  const double a20 = r2[-1], a21 = r2[0], a22 = r2[1];
   double e = dWxm * Wym * a00 + dWxm * Wy0 * a01 + dWxm * Wyp * a02 + dWx0 * Wym * a10 +
              dWx0 * Wy0 * a11 + dWx0 * Wyp * a12 + dWxp * Wym * a20 + dWxp * Wy0 * a21 +
              dWxp * Wyp * a22;
@@ -95,6 +119,14 @@ __device__ __forceinline__ void grid2d_gradient(const double* __restrict__ a, in
       Wx0 * dWy0 * a11 + Wx0 * dWyp * a12 + Wxp * dWym * a20 + Wxp * dWy0 * a21 +
       Wxp * dWyp * a22;
   ey = e / dy;
+}
+template <typename P>
+__device__ __forceinline__ void grid2d_gradient(P a, int nx, int ny, double xMin, double dx,
+                                                double yMin, double dy, double x, double y,
+                                                double& ex, double& ey) {
+  Grad2DLoads g;
+  grid2d_gradient_loads(a, nx, ny, xMin, dx, yMin, dy, x, y, g);
+  grid2d_gradient_sums(g, dx, dy, ex, ey);
 }
 
 // Grid3D::getGridIndAndFrac z part (Grid3D.cc:236-263); wrap = longWrapping.
@@ -224,38 +256,108 @@ __device__ __forceinline__ void grid3d_gradient(const double* __restrict__ a, in
 }
 
 // SpaceChargeCalc2p5Drb kick loop body (SpaceChargeCalc2p5Drb.cc:146-167), and, with
-// st->long_kick 0, the SpaceChargeCalc2p5D one (SpaceChargeCalc2p5D.cc:105-119).
+// long_kick 0, the SpaceChargeCalc2p5D one (SpaceChargeCalc2p5D.cc:105-119), in three
+// parts: the calculator's values for the step (read once per workgroup by the push
+// kernels), the particle's loads (3 x 3 potential values, line density and its derivative
+// at z), and the arithmetic.  A push issues the next particle's coordinate loads between
+// the second and the third, so that the wait for one particle's loads is not a wait for the
+// next one's.
 // op: p[0]=SC2DState*, p[1]=phi, p[2]=z line density, p[3]=its derivative,
 //     p[4]=2*log(pipe_radius), p[5]=-L*r0*m/gamma^2; i0=nx, i1=ny, i2=nz.
-// phi: the potential op.p[1] points to, or a copy of it (in LDS).
-__device__ __forceinline__ void sc2p5drb_kick(const Op& op, const double* __restrict__ phi,
-                                              double x, double& xp, double y, double& yp,
-                                              double z, double& dE) {
-  const SC2DState* st = op_ptr<const SC2DState*>(op.p[0]);
-  if (st->skip) return;
-  if (st->bnd_type && !boundary_inside(st->bnd_type, st->bnd, x, y)) return;  // 2p5D :109
+struct SC2DKickParams {
+  double xMin, dx, yMin, dy, zMin, zMax, dz, factor, x_c, y_c, a2, lsf_in;
+  int32_t skip, long_kick, bnd_type;
+  gptr<const double> bnd;  // the boundary's values, read per particle only when it has one
+};
+__device__ __forceinline__ SC2DKickParams sc2d_kick_params(const SC2DState* state) {
+  const gptr<const SC2DState> st = to_global(state);
+  SC2DKickParams k;
+  k.xMin = st->xMin;
+  k.dx = st->dx;
+  k.yMin = st->yMin;
+  k.dy = st->dy;
+  k.zMin = st->zMin;
+  k.zMax = st->zMax;
+  k.dz = st->dz;
+  k.factor = st->factor;
+  k.x_c = st->x_c;
+  k.y_c = st->y_c;
+  k.a2 = st->a2;
+  k.lsf_in = st->lsf_in;
+  k.skip = st->skip;
+  k.long_kick = st->long_kick;
+  k.bnd_type = st->bnd_type;
+  k.bnd = st->bnd;
+  return k;
+}
+
+struct SC2DKickLoads {
+  Grad2DLoads g;
+  double lam0, lamp, ez0, ezp, w0, wp;
+  bool on;   // kicked: not skipped, inside the boundary (SpaceChargeCalc2p5D.cc:109)
+  bool zin;  // z within [zMin, zMax] (Grid1D::getValue, Grid1D.cc:191-210)
+};
+
+// phi: row-major [nx][ny] potential (global memory or an LDS copy); zg, zd: line density
+// and its derivative (nz values).
+template <typename P, typename Z>
+__device__ __forceinline__ void sc2p5drb_kick_loads(const SC2DKickParams& k, P phi, int nx,
+                                                    int ny, Z zg, Z zd, int nz, double x,
+                                                    double y, double z, SC2DKickLoads& L) {
+  L.on = !k.skip && (!k.bnd_type || boundary_inside(k.bnd_type, k.bnd, x, y));
+  L.zin = !(z < k.zMin || z > k.zMax);
+  if (!L.on) return;
+  grid2d_gradient_loads(phi, nx, ny, k.xMin, k.dx, k.yMin, k.dy, x, y, L.g);
+  if (!L.zin) return;
+  int i0, ip;
+  grid1d_ind_wz(z, k.zMin, k.dz, nz, i0, ip, L.w0, L.wp);
+  L.lam0 = zg[i0];
+  L.lamp = zg[ip];
+  if (k.long_kick) {
+    L.ez0 = zd[i0];
+    L.ezp = zd[ip];
+  }
+}
+
+__device__ __forceinline__ double sc2d_line_value(const SC2DKickLoads& L, double v0, double vp,
+                                                  int nz) {
+  if (!L.zin) return 0.;
+  if (nz > 1) return L.w0 * v0 + L.wp * vp;
+  return v0;
+}
+
+__device__ __forceinline__ void sc2p5drb_kick_apply(const SC2DKickParams& k,
+                                                    const SC2DKickLoads& L, int nz,
+                                                    double pipe_log2, double long_coef,
+                                                    double x, double& xp, double y,
+                                                    double& yp, double& dE) {
+  if (!L.on) return;
   double ex, ey;
-  grid2d_gradient(phi, op.i0, op.i1, st->xMin, st->dx, st->yMin, st->dy, x, y, ex, ey);
-  const double Lfactor =
-      -grid1d_value(op_ptr<const double*>(op.p[2]), z, st->zMin, st->zMax, st->dz, op.i2) *
-      st->factor;
+  grid2d_gradient_sums(L.g, k.dx, k.dy, ex, ey);
+  const double Lfactor = -sc2d_line_value(L, L.lam0, L.lamp, nz) * k.factor;
   xp += ex * Lfactor;
   yp += ey * Lfactor;
-  if (!st->long_kick) return;  // SpaceChargeCalc2p5D.cc:105-119: transverse only
-  const double xc = st->x_c, yc = st->y_c;
+  if (!k.long_kick) return;  // SpaceChargeCalc2p5D.cc:105-119: transverse only
+  const double xc = k.x_c, yc = k.y_c;
   const double r2 = (x - xc) * (x - xc) + (y - yc) * (y - yc);
-  const double ez =
-      grid1d_value(op_ptr<const double*>(op.p[3]), z, st->zMin, st->zMax, st->dz, op.i2);
+  const double ez = sc2d_line_value(L, L.ez0, L.ezp, nz);
   double coeff;
-  if (r2 <= st->a2)
-    coeff = st->lsf_in - r2 / st->a2;
+  if (r2 <= k.a2)
+    coeff = k.lsf_in - r2 / k.a2;
   else
-    coeff = op.p[4] - log(r2);
-  dE += ez * op.p[5] * coeff;
+    coeff = pipe_log2 - log(r2);
+  dE += ez * long_coef * coeff;
 }
+
+// The whole kick of one particle (the op-program interpreter's form).
 __device__ __forceinline__ void sc2p5drb_kick(const Op& op, double x, double& xp, double y,
                                               double& yp, double z, double& dE) {
-  sc2p5drb_kick(op, op_ptr<const double*>(op.p[1]), x, xp, y, yp, z, dE);
+  const SC2DKickParams k = sc2d_kick_params(op_ptr<const SC2DState*>(op.p[0]));
+  SC2DKickLoads L;
+  sc2p5drb_kick_loads(k, to_global(op_ptr<const double*>(op.p[1])), op.i0, op.i1,
+                      to_global(op_ptr<const double*>(op.p[2])),
+                      to_global(op_ptr<const double*>(op.p[3])), op.i2, x, y, z, L);
+  sc2p5drb_kick_apply(k, L, op.i2, op.p[4], op.p[5], x, xp, y, yp, dE);
 }
 
 // SpaceChargeCalc3D kick loop body (SpaceChargeCalc3D.cc:132-147).
