@@ -201,7 +201,8 @@ __device__ __forceinline__ void sc2d_write_solver(const SC2DParams& P, SC2DState
   for (int k = 0; k < 6; k++) st->bnd[k] = P.bnd[k];
 }
 
-__device__ __forceinline__ Grid2Step sc2d_load_grid(const SC2DState* st) {
+__device__ __forceinline__ Grid2Step sc2d_load_grid(const SC2DState* state) {
+  const gptr<const SC2DState> st = to_global(state);
   Grid2Step g;
   g.xMin = st->xMin; g.xMax = st->xMax; g.dx = st->dx;
   g.yMin = st->yMin; g.yMax = st->yMax; g.dy = st->dy;
@@ -268,10 +269,10 @@ __device__ __forceinline__ void sc2d_hist_body(const SC2DParams& P, const Device
   for (int t = threadIdx.x; t < tg.ntiles; t += blockDim.x) lh[t] = 0u;
   __syncthreads();
   if (!g.skip) {
-    const double* __restrict__ px = v.c[0];
-    const double* __restrict__ py = v.c[2];
-    const double* __restrict__ pz = v.c[4];
-    const double* __restrict__ pm = v.msize;
+    const gptr<const double> px = to_global((const double*)v.c[0]);
+    const gptr<const double> py = to_global((const double*)v.c[2]);
+    const gptr<const double> pz = to_global((const double*)v.c[4]);
+    const gptr<const double> pm = to_global((const double*)v.msize);
     // kUnroll particles per thread per pass: their loads are all in flight together
     for (int64_t i0 = first; i0 < end; i0 += kUnroll * stride) {
       double x[kUnroll], y[kUnroll], z[kUnroll], m[kUnroll];
@@ -354,8 +355,11 @@ __device__ __forceinline__ void sc2d_scatter_body(const SC2DParams& P, const Dev
   for (int t = threadIdx.x; t < tg.ntiles; t += blockDim.x) lc[t] = crow ? crow[t] : 0u;
   __syncthreads();
   const bool run = !g.skip;
-  const double* __restrict__ px = v.c[0];
-  const double* __restrict__ py = v.c[2];
+  const gptr<const double> px = to_global((const double*)v.c[0]);
+  const gptr<const double> py = to_global((const double*)v.c[2]);
+  const gptr<const double> pm = to_global((const double*)v.msize);
+  const gptr<dpair> sxy = to_global((dpair*)tg.sxy);
+  const gptr<double> sm = to_global(tg.sm);
   const int64_t stride = blockDim.x;
   if (run && !crow)
     for (int64_t i0 = lo + threadIdx.x; i0 < hi; i0 += kUnroll * stride) {
@@ -387,7 +391,7 @@ __device__ __forceinline__ void sc2d_scatter_body(const SC2DParams& P, const Dev
         const int64_t i = i0 + u * stride;
         x[u] = i < hi ? px[i] : 0.0;
         y[u] = i < hi ? py[i] : 0.0;
-        m[u] = (v.msize && i < hi) ? v.msize[i] : 0.0;
+        m[u] = (pm && i < hi) ? pm[i] : 0.0;
       }
 #pragma unroll
       for (int u = 0; u < kUnroll; u++) {
@@ -395,8 +399,8 @@ __device__ __forceinline__ void sc2d_scatter_body(const SC2DParams& P, const Dev
         const int k = tile2d_key(tg, P.nx, P.ny, g, x[u], y[u]);
         if (k < 0) continue;
         const uint32_t d = atomicAdd(&lc[k], 1u);
-        tg.sxy[d] = make_double2(x[u], y[u]);  // one 16-byte store per particle
-        if (v.msize) tg.sm[d] = m[u];
+        sxy[d] = dpair{x[u], y[u]};  // one 16-byte store per particle
+        if (pm) sm[d] = m[u];
       }
     }
 }
@@ -419,8 +423,9 @@ __device__ __forceinline__ void moment_add(double s[5], double val, int gx, int 
 // block's share of the rho moments goes to mrow[5] (if given).
 __device__ __forceinline__ void sc2d_tdeposit_body(const SC2DParams& P, const Grid2Step& g,
                                                    const Tile2D& tg, int4 it, bool per_particle_m,
-                                                   double* __restrict__ rho, double* acc,
+                                                   double* rho_ptr, double* acc,
                                                    double* mrow) {
+  const gptr<double> rho = to_global(rho_ptr);
   const int W = tg.T + 2, cells = W * W;
   const int x0 = (it.x / tg.ty) * tg.T - 1, y0 = (it.x % tg.ty) * tg.T - 1;
   for (int c = threadIdx.x; c < cells; c += blockDim.x) acc[c] = 0.0;
@@ -428,15 +433,15 @@ __device__ __forceinline__ void sc2d_tdeposit_body(const SC2DParams& P, const Gr
   const double xMin = g.xMin, yMin = g.yMin, dx = g.dx, dy = g.dy;
   const int nx = P.nx, ny = P.ny;
   double mom[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
-  const double2* __restrict__ sxy = tg.sxy;
-  const double* __restrict__ sm = tg.sm;
+  const gptr<const dpair> sxy = to_global((const dpair*)tg.sxy);
+  const gptr<const double> sm = to_global((const double*)tg.sm);
   const int stride = blockDim.x;
   for (int i0 = it.y + threadIdx.x; i0 < it.z; i0 += kUnroll * stride) {
     double xs[kUnroll], ys[kUnroll], ms[kUnroll];
 #pragma unroll
     for (int u = 0; u < kUnroll; u++) {
       const int i = i0 + u * stride;
-      const double2 v = sxy[i < it.z ? i : it.y];
+      const dpair v = sxy[i < it.z ? i : it.y];
       xs[u] = v.x;
       ys[u] = v.y;
       ms[u] = per_particle_m ? (i < it.z ? sm[i] : 0.0) : P.macrosize;
@@ -460,11 +465,11 @@ __device__ __forceinline__ void sc2d_tdeposit_body(const SC2DParams& P, const Gr
 #pragma unroll
           for (int c = 0; c < 3; c++) atomicAdd(base + a * W + c, wx[a] * wy[c] * m);
       } else {  // not in this tile's block (cannot happen for a consistent sort): global
-        double* base = rho + (int64_t)(iX - 1) * ny + (iY - 1);
+        const gptr<double> base = rho + (int64_t)(iX - 1) * ny + (iY - 1);
         for (int a = 0; a < 3; a++)
           for (int c = 0; c < 3; c++) {
             const double val = wx[a] * wy[c] * m;
-            atomicAdd(base + a * ny + c, val);
+            global_add(base + a * ny + c, val);
             moment_add(mom, val, iX - 1 + a, iY - 1 + c, xMin, dx);
           }
       }
@@ -476,7 +481,7 @@ __device__ __forceinline__ void sc2d_tdeposit_body(const SC2DParams& P, const Gr
     if (val == 0.0) continue;
     const int gx = x0 + c / W, gy = y0 + c % W;
     if (gx < 0 || gx >= nx || gy < 0 || gy >= ny) continue;
-    atomicAdd(&rho[(int64_t)gx * ny + gy], val);
+    global_add(rho + (int64_t)gx * ny + gy, val);
     moment_add(mom, val, gx, gy, xMin, dx);
   }
   if (mrow) {

@@ -28,6 +28,14 @@ template <typename T>
 __device__ __forceinline__ gptr<T> to_global(T* p) {
   return (gptr<T>)p;
 }
+// A 16-byte pair for loads and stores through a gptr (HIP's double2 class cannot be
+// assigned in another address space).
+typedef double dpair __attribute__((ext_vector_type(2)));
+// atomicAdd (relaxed, device scope) through a gptr
+template <typename T>
+__device__ __forceinline__ void global_add(gptr<T> p, T v) {
+  __hip_atomic_fetch_add(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
 
 // Grid2D::getIndAndFracX/Y (Grid2D.cc:337-350); Grid3D.cc:212-224 is identical.
 __device__ __forceinline__ void node_ind_frac(double v, double vmin, double d, int n, int& ind,
