@@ -389,9 +389,11 @@ __device__ __forceinline__ void kick_particle(double* __restrict__ cur, int64_t 
 
 // Blocks 0..nitems-1: work items; block nitems: the particles outside the grid (last in the
 // sorted order, from offsets[ntiles]), kicked from the whole grid like the reference.
-// At least 4 waves per SIMD: caps the kick at 128 VGPRs (no spills) instead of 155.
+// At least 3 waves per SIMD: the kick with the next particle's prefetch takes 157 VGPRs
+// without spills.  Bounded to 4 waves (128 VGPRs) the prefetch spills and the kick ran
+// 133 us against 113; without the prefetch, 119.
 template <bool BND>
-__global__ __launch_bounds__(kThreads, 4) void tile_kick(double* __restrict__ cur, int64_t cap,
+__global__ __launch_bounds__(kThreads, 3) void tile_kick(double* __restrict__ cur, int64_t cap,
                                                       int64_t n, TileGeom g, Kick3D k,
                                                       const int4* __restrict__ items,
                                                       const uint32_t* __restrict__ nitems,
@@ -440,9 +442,16 @@ __global__ __launch_bounds__(kThreads, 4) void tile_kick(double* __restrict__ cu
     sphi[c] = in ? k.phi[gx * gsx + (int64_t)gy * g.nz + gz] : 0.0;
   }
   __syncthreads();
-  if (i0 < it.z) kick_particle_all<BND>(cur, cap, i0, px, py, pz, pxp, pyp, pdE, g, k, sphi, &box);
-  for (int i = i0 + kThreads; i < it.z; i += kThreads)
-    kick_particle<BND>(cur, cap, i, g, k, sphi, &box);
+  // The next particle's coordinates are loaded before this one is kicked.
+  double q[6] = {px, py, pz, pxp, pyp, pdE};
+  for (int i = i0; i < it.z; i += kThreads) {
+    const int in = i + kThreads < it.z ? i + kThreads : i;
+    const double nq[6] = {cur[in], cur[2 * cap + in], cur[4 * cap + in],
+                          cur[cap + in], cur[3 * cap + in], cur[5 * cap + in]};
+    kick_particle_all<BND>(cur, cap, i, q[0], q[1], q[2], q[3], q[4], q[5], g, k, sphi, &box);
+#pragma unroll
+    for (int c = 0; c < 6; c++) q[c] = nq[c];
+  }
 }
 
 }  // namespace
