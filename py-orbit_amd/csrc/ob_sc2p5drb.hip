@@ -258,48 +258,77 @@ __device__ __forceinline__ int tile2d_key(const Tile2D& tg, int nx, int ny, cons
 // Particles first, first + stride, ... < end: tile counts (atomics into tg.counts) and the
 // z line density, added atomically into zg or written as this block's row zrow[nz].
 // LDS: zl[nz] doubles, lh[ntiles] counters.
+// kUnroll particles of a hist pass (x, y, z, macro size), loaded together.
+struct HistPass {
+  double x[kUnroll], y[kUnroll], z[kUnroll], m[kUnroll];
+};
+__device__ __forceinline__ void hist_load(const DeviceBunchView& v, double macrosize, int64_t i0,
+                                          int64_t end, int64_t stride, HistPass& q) {
+  const gptr<const double> px = to_global((const double*)v.c[0]);
+  const gptr<const double> py = to_global((const double*)v.c[2]);
+  const gptr<const double> pz = to_global((const double*)v.c[4]);
+  const gptr<const double> pm = to_global((const double*)v.msize);
+#pragma unroll
+  for (int u = 0; u < kUnroll; u++) {
+    const int64_t i = i0 + u * stride;
+    const bool ok = i < end;
+    q.x[u] = ok ? px[i] : 0.0;
+    q.y[u] = ok ? py[i] : 0.0;
+    q.z[u] = ok ? pz[i] : 0.0;
+    q.m[u] = pm ? (ok ? pm[i] : 0.0) : macrosize;
+  }
+}
+
+// PIPE: each pass's loads are issued before the previous pass is binned (the single-bunch
+// kernel, whose workgroups make one or two passes; the batch's, with many, keep their
+// registers for occupancy instead); pre: the first pass's particles, already loaded (by
+// hist_load at `first`), or null.
+template <bool PIPE>
 __device__ __forceinline__ void sc2d_hist_body(const SC2DParams& P, const DeviceBunchView& v,
                                                const Grid2Step& g, const Tile2D& tg,
                                                double* __restrict__ zg, double* __restrict__ zrow,
                                                int64_t first, int64_t end, int64_t stride,
                                                double* zl, uint32_t* lh,
-                                               uint32_t* __restrict__ crow = nullptr) {
+                                               uint32_t* __restrict__ crow = nullptr,
+                                               const HistPass* pre = nullptr) {
   const int nz = P.nz;
   for (int i = threadIdx.x; i < nz; i += blockDim.x) zl[i] = 0.0;
   for (int t = threadIdx.x; t < tg.ntiles; t += blockDim.x) lh[t] = 0u;
   __syncthreads();
   if (!g.skip) {
-    const gptr<const double> px = to_global((const double*)v.c[0]);
-    const gptr<const double> py = to_global((const double*)v.c[2]);
-    const gptr<const double> pz = to_global((const double*)v.c[4]);
-    const gptr<const double> pm = to_global((const double*)v.msize);
-    // kUnroll particles per thread per pass: their loads are all in flight together
+    // kUnroll particles per thread per pass, their loads all in flight together; the next
+    // pass's loads are issued before this one is binned
+    HistPass q;
+    if (PIPE) {
+      if (pre)
+        q = *pre;
+      else if (first < end)
+        hist_load(v, P.macrosize, first, end, stride, q);
+    }
     for (int64_t i0 = first; i0 < end; i0 += kUnroll * stride) {
-      double x[kUnroll], y[kUnroll], z[kUnroll], m[kUnroll];
-#pragma unroll
-      for (int u = 0; u < kUnroll; u++) {
-        const int64_t i = i0 + u * stride;
-        const bool ok = i < end;
-        x[u] = ok ? px[i] : 0.0;
-        y[u] = ok ? py[i] : 0.0;
-        z[u] = ok ? pz[i] : 0.0;
-        m[u] = pm ? (ok ? pm[i] : 0.0) : P.macrosize;
+      HistPass c;
+      if (PIPE) {
+        c = q;
+        if (i0 + kUnroll * stride < end)
+          hist_load(v, P.macrosize, i0 + kUnroll * stride, end, stride, q);
+      } else {
+        hist_load(v, P.macrosize, i0, end, stride, c);
       }
 #pragma unroll
       for (int u = 0; u < kUnroll; u++) {
         if (i0 + u * stride >= end) break;
-        const int k = tile2d_key(tg, P.nx, P.ny, g, x[u], y[u]);
+        const int k = tile2d_key(tg, P.nx, P.ny, g, c.x[u], c.y[u]);
         if (k >= 0) atomicAdd(&lh[k], 1u);
         // Grid1D::binValue (Grid1D.cc:422-440)
-        if (!(z[u] < g.zMin || z[u] > g.zMax)) {
+        if (!(c.z[u] < g.zMin || c.z[u] > g.zMax)) {
           int i0z, ip;
           double w0, wp;
-          grid1d_ind_wz(z[u], g.zMin, g.dz, nz, i0z, ip, w0, wp);
+          grid1d_ind_wz(c.z[u], g.zMin, g.dz, nz, i0z, ip, w0, wp);
           if (nz > 1) {
-            atomicAdd(&zl[i0z], w0 * m[u]);
-            atomicAdd(&zl[ip], wp * m[u]);
+            atomicAdd(&zl[i0z], w0 * c.m[u]);
+            atomicAdd(&zl[ip], wp * c.m[u]);
           } else {
-            atomicAdd(&zl[0], m[u]);
+            atomicAdd(&zl[0], c.m[u]);
           }
         }
       }
@@ -346,14 +375,13 @@ __device__ __forceinline__ void tile2d_scan_body(const Tile2D& tg, uint32_t* lof
 // Particles [lo, hi) of one bunch.  LDS: loff[ntiles] (tile offsets), lc[ntiles].
 // crow: the range's tile counts, kept by the bin pass over the same range (batch path);
 // without it the range is counted here first.
+// PIPE: as for sc2d_hist_body.
+template <bool PIPE>
 __device__ __forceinline__ void sc2d_scatter_body(const SC2DParams& P, const DeviceBunchView& v,
                                                   const Grid2Step& g, const Tile2D& tg,
                                                   int64_t lo, int64_t hi, uint32_t* loff,
                                                   uint32_t* lc, bool writer,
                                                   const uint32_t* __restrict__ crow = nullptr) {
-  tile2d_scan_body(tg, loff, writer);
-  for (int t = threadIdx.x; t < tg.ntiles; t += blockDim.x) lc[t] = crow ? crow[t] : 0u;
-  __syncthreads();
   const bool run = !g.skip;
   const gptr<const double> px = to_global((const double*)v.c[0]);
   const gptr<const double> py = to_global((const double*)v.c[2]);
@@ -361,14 +389,30 @@ __device__ __forceinline__ void sc2d_scatter_body(const SC2DParams& P, const Dev
   const gptr<dpair> sxy = to_global((dpair*)tg.sxy);
   const gptr<double> sm = to_global(tg.sm);
   const int64_t stride = blockDim.x;
+  // kUnroll particles per thread per pass, loaded together; the first pass's loads go out
+  // before the tile scan, each next pass's before this one is processed
+  auto load = [&](int64_t i0, double (&x)[kUnroll], double (&y)[kUnroll], double (&m)[kUnroll]) {
+#pragma unroll
+    for (int u = 0; u < kUnroll; u++) {
+      const int64_t i = i0 + u * stride;
+      x[u] = i < hi ? px[i] : 0.0;
+      y[u] = i < hi ? py[i] : 0.0;
+      m[u] = (pm && i < hi) ? pm[i] : 0.0;
+    }
+  };
+  double nx_[kUnroll], ny_[kUnroll], nm_[kUnroll];
+  if (PIPE) load(lo + threadIdx.x, nx_, ny_, nm_);
+  tile2d_scan_body(tg, loff, writer);
+  for (int t = threadIdx.x; t < tg.ntiles; t += blockDim.x) lc[t] = crow ? crow[t] : 0u;
+  __syncthreads();
   if (run && !crow)
     for (int64_t i0 = lo + threadIdx.x; i0 < hi; i0 += kUnroll * stride) {
-      double x[kUnroll], y[kUnroll];
+      double x[kUnroll], y[kUnroll], m[kUnroll];
+      if (PIPE && i0 == lo + threadIdx.x) {
 #pragma unroll
-      for (int u = 0; u < kUnroll; u++) {
-        const int64_t i = i0 + u * stride;
-        x[u] = i < hi ? px[i] : 0.0;
-        y[u] = i < hi ? py[i] : 0.0;
+        for (int u = 0; u < kUnroll; u++) { x[u] = nx_[u]; y[u] = ny_[u]; }
+      } else {
+        load(i0, x, y, m);
       }
 #pragma unroll
       for (int u = 0; u < kUnroll; u++) {
@@ -386,12 +430,16 @@ __device__ __forceinline__ void sc2d_scatter_body(const SC2DParams& P, const Dev
   if (run)
     for (int64_t i0 = lo + threadIdx.x; i0 < hi; i0 += kUnroll * stride) {
       double x[kUnroll], y[kUnroll], m[kUnroll];
+      if (PIPE) {
 #pragma unroll
-      for (int u = 0; u < kUnroll; u++) {
-        const int64_t i = i0 + u * stride;
-        x[u] = i < hi ? px[i] : 0.0;
-        y[u] = i < hi ? py[i] : 0.0;
-        m[u] = (pm && i < hi) ? pm[i] : 0.0;
+        for (int u = 0; u < kUnroll; u++) {
+          x[u] = nx_[u];
+          y[u] = ny_[u];
+          m[u] = nm_[u];
+        }
+        if (i0 + kUnroll * stride < hi) load(i0 + kUnroll * stride, nx_, ny_, nm_);
+      } else {
+        load(i0, x, y, m);
       }
 #pragma unroll
       for (int u = 0; u < kUnroll; u++) {
@@ -421,6 +469,8 @@ __device__ __forceinline__ void moment_add(double s[5], double val, int gx, int 
 // Grid2D::binValue (Grid2D.cc:142-166) of the particles of work item `it` (tile it.x,
 // scratch range [it.y, it.z)) into an LDS block of the tile plus halo, then into rho; the
 // block's share of the rho moments goes to mrow[5] (if given).
+// PIPE: as for sc2d_hist_body.
+template <bool PIPE>
 __device__ __forceinline__ void sc2d_tdeposit_body(const SC2DParams& P, const Grid2Step& g,
                                                    const Tile2D& tg, int4 it, bool per_particle_m,
                                                    double* rho_ptr, double* acc,
@@ -428,16 +478,12 @@ __device__ __forceinline__ void sc2d_tdeposit_body(const SC2DParams& P, const Gr
   const gptr<double> rho = to_global(rho_ptr);
   const int W = tg.T + 2, cells = W * W;
   const int x0 = (it.x / tg.ty) * tg.T - 1, y0 = (it.x % tg.ty) * tg.T - 1;
-  for (int c = threadIdx.x; c < cells; c += blockDim.x) acc[c] = 0.0;
-  __syncthreads();
-  const double xMin = g.xMin, yMin = g.yMin, dx = g.dx, dy = g.dy;
-  const int nx = P.nx, ny = P.ny;
-  double mom[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
   const gptr<const dpair> sxy = to_global((const dpair*)tg.sxy);
   const gptr<const double> sm = to_global((const double*)tg.sm);
   const int stride = blockDim.x;
-  for (int i0 = it.y + threadIdx.x; i0 < it.z; i0 += kUnroll * stride) {
-    double xs[kUnroll], ys[kUnroll], ms[kUnroll];
+  // kUnroll particles per thread per pass, loaded together; the first pass's loads go out
+  // before the LDS block is cleared, each next pass's before this one is deposited
+  auto load = [&](int i0, double (&xs)[kUnroll], double (&ys)[kUnroll], double (&ms)[kUnroll]) {
 #pragma unroll
     for (int u = 0; u < kUnroll; u++) {
       const int i = i0 + u * stride;
@@ -445,6 +491,27 @@ __device__ __forceinline__ void sc2d_tdeposit_body(const SC2DParams& P, const Gr
       xs[u] = v.x;
       ys[u] = v.y;
       ms[u] = per_particle_m ? (i < it.z ? sm[i] : 0.0) : P.macrosize;
+    }
+  };
+  double nxs[kUnroll], nys[kUnroll], nms[kUnroll];
+  if (PIPE && it.y < it.z) load(it.y + (int)threadIdx.x, nxs, nys, nms);
+  for (int c = threadIdx.x; c < cells; c += blockDim.x) acc[c] = 0.0;
+  __syncthreads();
+  const double xMin = g.xMin, yMin = g.yMin, dx = g.dx, dy = g.dy;
+  const int nx = P.nx, ny = P.ny;
+  double mom[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
+  for (int i0 = it.y + threadIdx.x; i0 < it.z; i0 += kUnroll * stride) {
+    double xs[kUnroll], ys[kUnroll], ms[kUnroll];
+    if (PIPE) {
+#pragma unroll
+      for (int u = 0; u < kUnroll; u++) {
+        xs[u] = nxs[u];
+        ys[u] = nys[u];
+        ms[u] = nms[u];
+      }
+      if (i0 + kUnroll * stride < it.z) load(i0 + kUnroll * stride, nxs, nys, nms);
+    } else {
+      load(i0, xs, ys, ms);
     }
 #pragma unroll
     for (int u = 0; u < kUnroll; u++) {
@@ -508,6 +575,11 @@ __global__ __launch_bounds__(kDepositThreads) void sc2d_bin(SC2DParams P, Device
                                                             Tile2D tg, double* __restrict__ rho,
                                                             int* status) {
   extern __shared__ double dyn[];
+  // the first pass's particle loads go out before the grid is computed from the extrema
+  const int64_t first = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  HistPass pre;
+  hist_load(v, P.macrosize, first, v.n, stride, pre);
   Grid2Step g;
   sc2d_grid_body(P, partials, 0, P.nblk, st, g);
   if (blockIdx.x == 0) {
@@ -521,9 +593,8 @@ __global__ __launch_bounds__(kDepositThreads) void sc2d_bin(SC2DParams P, Device
   const int ng = P.nx * P.ny;
   for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < ng; i += gridDim.x * blockDim.x)
     rho[i] = 0.0;
-  sc2d_hist_body(P, v, g, tg, nullptr, tg.zpart + (size_t)blockIdx.x * P.nz,
-                 (int64_t)blockIdx.x * blockDim.x + threadIdx.x, v.n,
-                 (int64_t)gridDim.x * blockDim.x, dyn, (uint32_t*)(dyn + P.nz));
+  sc2d_hist_body<true>(P, v, g, tg, nullptr, tg.zpart + (size_t)blockIdx.x * P.nz, first, v.n, stride,
+                 dyn, (uint32_t*)(dyn + P.nz), nullptr, &pre);
 }
 
 __global__ __launch_bounds__(kDepositThreads) void sc2d_scatter(SC2DParams P, DeviceBunchView v,
@@ -531,7 +602,7 @@ __global__ __launch_bounds__(kDepositThreads) void sc2d_scatter(SC2DParams P, De
                                                                 int64_t per_block) {
   extern __shared__ uint32_t lds[];
   const int64_t lo = (int64_t)blockIdx.x * per_block;
-  sc2d_scatter_body(P, v, sc2d_load_grid(st), tg, lo, min(v.n, lo + per_block), lds,
+  sc2d_scatter_body<true>(P, v, sc2d_load_grid(st), tg, lo, min(v.n, lo + per_block), lds,
                     lds + tg.ntiles, blockIdx.x == 0);
 }
 
@@ -545,7 +616,7 @@ __global__ __launch_bounds__(kDepositThreads) void sc2d_tdeposit(SC2DParams P,
     for (int t = threadIdx.x; t < tg.ntiles; t += blockDim.x) tg.counts[t] = 0u;
   if (blockIdx.x >= *tg.nitems) return;
   extern __shared__ double acc[];
-  sc2d_tdeposit_body(P, sc2d_load_grid(st), tg, tg.items[blockIdx.x], per_particle_m != 0, rho,
+  sc2d_tdeposit_body<true>(P, sc2d_load_grid(st), tg, tg.items[blockIdx.x], per_particle_m != 0, rho,
                      acc, moments ? tg.mpart + (size_t)blockIdx.x * 5 : nullptr);
 }
 
@@ -576,12 +647,15 @@ __device__ __forceinline__ void sc2d_post_body(const SC2DParams& P, SC2DState* s
                                                double* __restrict__ zd,
                                                const double* __restrict__ zpart, int zrows,
                                                const double* __restrict__ mpart,
-                                               const uint32_t* nitems) {
+                                               const uint32_t* nitems, int part = 3) {
   __shared__ double sh[kSetupThreads / 64];
   // the z line density, read below many times per bin, is kept in LDS
   __shared__ double zl[kMaxZBinsLDS];
   const int nz = P.nz;
-  if (zpart) {
+  // part 1: z line density and its derivative; part 2: rho moments and kick factors (two
+  // workgroups may run them side by side: neither reads what the other writes); 3: both
+  if (!(part & 1)) {
+  } else if (zpart) {
     // g = blockDim / nz threads per bin (when nz < blockDim) sum the rows in turn, then
     // their partial sums meet in LDS
     __shared__ double zs[kSetupThreads];
@@ -616,12 +690,13 @@ __device__ __forceinline__ void sc2d_post_body(const SC2DParams& P, SC2DState* s
     __syncthreads();
   }
   if (st->skip) {
-    for (int i = threadIdx.x; i < nz; i += blockDim.x) zd[i] = 0.0;
+    if (part & 1)
+      for (int i = threadIdx.x; i < nz; i += blockDim.x) zd[i] = 0.0;
     return;
   }
   const double z_step = st->dz;
   const double zMin = st->zMin;
-  if (P.plain) {
+  if (P.plain) {  // (part 3 only: the total comes from the line density)
     // SpaceChargeCalc2p5D.cc:229-233, 100: the total macrosize is the sum of the z line
     // density; no derivative and no rho moments are used
     double t = 0.0;
@@ -635,7 +710,8 @@ __device__ __forceinline__ void sc2d_post_body(const SC2DParams& P, SC2DState* s
     return;
   }
   // calculateLongDerivative (:257-310)
-  if (nz == 1) {
+  if (!(part & 1)) {
+  } else if (nz == 1) {
     if (threadIdx.x == 0) zd[0] = 0.;
   } else if (nz == 2) {
     if (threadIdx.x == 0) {
@@ -687,6 +763,7 @@ __device__ __forceinline__ void sc2d_post_body(const SC2DParams& P, SC2DState* s
       zd[iz] = y / z_step;
     }
   }
+  if (!(part & 2)) return;
   // rho moments (:229-254); quirk Q1: the y position uses getGridX(iy) (:238)
   const int nx = P.nx, ny = P.ny;
   const double gx0 = st->xMin, gdx = st->dx;
@@ -785,20 +862,24 @@ __global__ __launch_bounds__(kFFTThreads) void fft_rows_r2c(const double* __rest
 }
 
 // The single-bunch step's row transforms of rho and its post pass in one launch:
-// workgroups 0 .. rows/2 - 1 transform row pairs, the last one runs the post pass.  Both
-// only read rho; the post pass writes the state's kick factors, z line density and
-// derivative, which the column pass after this launch (scale_coeff) and the next map
-// kernel read.  One launch fewer per step.
+// workgroups 0 .. rows/2 - 1 transform row pairs, the last two run the post pass's two
+// independent halves side by side (z line density and derivative; rho moments and kick
+// factors).  All only read rho; the post pass writes the state's kick factors, z line
+// density and derivative, which the column pass after this launch (scale_coeff) and the
+// next map kernel read.  One launch fewer per step.
 template <int N>
 __global__ __launch_bounds__(kFFTThreads) void fft_rows_r2c_post(
     const double* __restrict__ rho, int nrows_in, int ncols_in, double2* __restrict__ out,
     const double2* __restrict__ tw, int nmax, SC2DParams P, SC2DState* st,
     double* __restrict__ zg, double* __restrict__ zd, Tile2D tg, int zrows, int moments) {
-  if ((int)blockIdx.x < (nrows_in + 1) / 2) {
+  const int rb = (nrows_in + 1) / 2;
+  if ((int)blockIdx.x < rb) {
     rows_r2c_body<N>(rho, nrows_in, ncols_in, ncols_in, out, tw, nmax, st, blockIdx.x);
     return;
   }
-  sc2d_post_body(P, st, rho, zg, zd, tg.zpart, zrows, moments ? tg.mpart : nullptr, tg.nitems);
+  // two more workgroups: the line density part and the moments part of the post pass
+  sc2d_post_body(P, st, rho, zg, zd, tg.zpart, zrows, moments ? tg.mpart : nullptr, tg.nitems,
+                 (int)blockIdx.x == rb ? 1 : 2);
 }
 
 // (b) column transforms along x (N = M0).  GREEN: full forward transform of all N rows,
@@ -964,7 +1045,7 @@ __global__ __launch_bounds__(kDepositThreads) void sc2d_hist_batch(const BatchBu
   const BatchItem it = items[blockIdx.x];
   const BatchBunch& B = bb[it.bunch];
   const SC2DParams P = batch_params(B, 0.0, 1.0);
-  sc2d_hist_body(P, B.v, sc2d_load_grid(B.st), B.tg, B.zg, nullptr, it.begin + threadIdx.x,
+  sc2d_hist_body<false>(P, B.v, sc2d_load_grid(B.st), B.tg, B.zg, nullptr, it.begin + threadIdx.x,
                  it.end, blockDim.x, dyn, (uint32_t*)(dyn + P.nz),
                  B.tg.icounts + (size_t)(blockIdx.x - B.item_begin) * B.tg.ntiles);
 }
@@ -975,7 +1056,7 @@ __global__ __launch_bounds__(kDepositThreads) void sc2d_scatter_batch(const Batc
   const BatchItem it = items[blockIdx.x];
   const BatchBunch& B = bb[it.bunch];
   const SC2DParams P = batch_params(B, 0.0, 1.0);
-  sc2d_scatter_body(P, B.v, sc2d_load_grid(B.st), B.tg, it.begin, it.end, lds,
+  sc2d_scatter_body<false>(P, B.v, sc2d_load_grid(B.st), B.tg, it.begin, it.end, lds,
                     lds + B.tg.ntiles, (int)blockIdx.x == B.item_begin,
                     B.tg.icounts + (size_t)(blockIdx.x - B.item_begin) * B.tg.ntiles);
 }
@@ -1000,7 +1081,7 @@ __global__ __launch_bounds__(kDepositThreads) void sc2d_tdeposit_batch(const Bat
     for (int t = threadIdx.x; t < B.tg.ntiles; t += blockDim.x) B.tg.counts[t] = 0u;
   if (j >= (int)*B.tg.nitems) return;
   const SC2DParams P = batch_params(B, 0.0, 1.0);
-  sc2d_tdeposit_body(P, sc2d_load_grid(B.st), B.tg, B.tg.items[j], B.v.msize != nullptr, B.rho,
+  sc2d_tdeposit_body<false>(P, sc2d_load_grid(B.st), B.tg, B.tg.items[j], B.v.msize != nullptr, B.rho,
                      acc, B.moments_from_rho ? nullptr : B.tg.mpart + (size_t)j * 5);
 }
 
@@ -1181,7 +1262,8 @@ void launch_fft_pipeline(ob_sc2p5drb* c, SC2DState* st, const PostArgs* post = n
           ready = true;
         }
         ob::ProfScope ps_("fft_rows_r2c_post");
-        hipLaunchKernelGGL(fft_rows_r2c_post<N>, dim3((c->nx + 1) / 2 + 1), dim3(kFFTThreads),
+        OB_REQUIRE(!post->P.plain, "2.5D post pass: the split form is for SpaceChargeCalc2p5Drb");
+        hipLaunchKernelGGL(fft_rows_r2c_post<N>, dim3((c->nx + 1) / 2 + 2), dim3(kFFTThreads),
                            fft_lds<N>(), stream(), c->rho, c->nx, c->ny, c->spec, c->tw, c->nmax,
                            post->P, st, c->zg, c->zd, post->tg, post->zrows, post->moments);
       }
