@@ -20,6 +20,8 @@
 #include <algorithm>
 #include <cmath>
 #include <cstring>
+#include <map>
+#include <utility>
 #include <vector>
 
 #include "ob_internal.hpp"
@@ -201,12 +203,43 @@ float time_ms(hipEvent_t a, hipEvent_t b) {
   return ms;
 }
 
+// Tuning results of this process by device and batch shape (bunch count, then every
+// bunch's particle count, grid and macro-size attribute): a batch of the same shape takes
+// them instead of timing the candidates again.
+std::map<std::vector<int64_t>, std::pair<int, int>>& tuning_cache() {
+  static std::map<std::vector<int64_t>, std::pair<int, int>> m;
+  return m;
+}
+std::vector<int64_t> tuning_key(const ob_batch* B) {
+  int dev = 0;
+  OB_HIP(hipGetDevice(&dev));
+  std::vector<int64_t> k{dev, (int64_t)B->h_bb.size(), B->calcs.empty() ? 0 : 1};
+  for (const BatchBunch& r : B->h_bb) {
+    k.push_back(r.v.n);
+    k.push_back(r.nx);
+    k.push_back(r.ny);
+    k.push_back(r.nz);
+    k.push_back(r.v.msize != nullptr ? 1 : 0);
+  }
+  return k;
+}
+
 // Time the candidates on the batch's data: work-item size with an identity push (reads
 // and writes every particle unchanged), then the tile-deposit item size with repeated
 // setup + deposit (setup clears the grids, so the repeats are idempotent).  The small
 // candidates serve batches of few, small bunches (a single 100k bunch: 512 particles per
 // item gives it 196 push workgroups instead of 49).
 void autotune(ob_batch* B) {
+  build_records(B, false);
+  const std::vector<int64_t> key = tuning_key(B);
+  const auto hit = tuning_cache().find(key);
+  if (hit != tuning_cache().end()) {
+    B->chunk = hit->second.first;
+    B->item = hit->second.second;
+    build_records(B, false);
+    B->tuned = true;
+    return;
+  }
   hipStream_t s = stream();
   hipEvent_t e0, e1;
   OB_HIP(hipEventCreate(&e0));
@@ -258,6 +291,7 @@ void autotune(ob_batch* B) {
   OB_HIP(hipEventDestroy(e0));
   OB_HIP(hipEventDestroy(e1));
   B->tuned = true;
+  tuning_cache()[key] = {B->chunk, B->item};
 }
 
 }  // namespace

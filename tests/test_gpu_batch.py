@@ -94,6 +94,27 @@ def test_batch_results_do_not_depend_on_tuning():
             assert rel_l2(outs[0][k][j], outs[1][k][j]) < 1e-11
 
 
+def test_batch_tuning_cached_by_shape():
+    """The autotuner times its candidates once per batch shape (bunch sizes, grids) and
+    process: a second batch of the same shape takes the first one's choice, and tracks
+    like single bunches with it."""
+    from batch import BunchBatch
+    sizes, grids = [5000, 11000], [(16, 16, 8), (32, 32, 16)]
+    ref, _ = _single(1, sizes, grids)
+    tunings = []
+    for _ in range(2):
+        bunches, calcs = _setup(sizes, grids)
+        bb = BunchBatch(bunches, calcs)
+        _lattice(bb.calculator(), 1).trackBunch(bb.bunch())
+        bb.synchronize()
+        tunings.append(bb.tuning())
+        for k, b in enumerate(bunches):
+            out = b.getCoordinates()
+            for j in range(6):
+                assert rel_l2(out[j], ref[k][j]) < 1e-9, (k, j)
+    assert tunings[0]["tuned"] and tunings[1] == tunings[0]
+
+
 def test_batch_maps_are_bit_exact_and_reads_apply_pending_work():
     """Maps only (no SC): the batch gives the bits of per-bunch tracking; reading a bunch
     applies the batch's queued maps first."""
