@@ -1039,8 +1039,9 @@ OB_API int ob_sc3d_get_freq(ob_sc3d* c, double* f) {
 
 OB_API int ob_sc3d_set_deposit_mode(ob_sc3d* c, int tiled, int lds_budget_bytes) {
   OB_TRY OB_REQUIRE(c != nullptr, "null handle");
-  OB_REQUIRE(lds_budget_bytes == 0 || (lds_budget_bytes >= 1024 && lds_budget_bytes <= 150 * 1024),
-             "LDS budget must be 0 (keep) or 1 KiB..150 KiB");
+  // (the deposit stages its work item's particles beside the tile block: 40 KiB more)
+  OB_REQUIRE(lds_budget_bytes == 0 || (lds_budget_bytes >= 1024 && lds_budget_bytes <= 116 * 1024),
+             "LDS budget must be 0 (keep) or 1 KiB..116 KiB");
   c->tiled = tiled != 0;
   if (lds_budget_bytes) c->lds_budget = lds_budget_bytes;
   OB_CATCH

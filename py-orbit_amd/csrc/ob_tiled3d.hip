@@ -256,6 +256,22 @@ __device__ __forceinline__ void deposit_particle(double* __restrict__ base, int6
   }
 }
 
+// LDS bank scheduling of the deposit: the 27 LDS atomics of a particle are at fixed word
+// offsets from its first node's word, so lanes whose first words differ modulo kBankWords
+// (8-byte words) hit distinct banks in all 27.  Half a work item at a time, the particles
+// are ranked within their residue class, staged in LDS grouped by class, and deposited in
+// rounds where lane l of each 32-lane group takes the next particle of class l (an idle
+// lane when the class is used up).  In particle order, 64 lanes fall on the banks about as
+// balls in bins.  Staging half an item keeps 4 workgroups per CU.
+constexpr int kBankWords = 32;
+constexpr int kDepU = kChunk / kThreads;  // particles per thread of a work item
+constexpr int kDepStage = kChunk / 2;     // particles staged at a time
+static_assert(kDepU % 2 == 0, "deposit halves");
+
+// staging after the accumulation block: xF, yF, zF, m (doubles), first-node word << 16 |
+// z index (int)
+__host__ __device__ inline int tile_dep_acc_doubles(int cells) { return (cells + 1) & ~1; }
+
 __global__ __launch_bounds__(kThreads) void tile_deposit(const double* __restrict__ cur,
                                                          int64_t cap,
                                                          const double* __restrict__ msize,
@@ -265,45 +281,98 @@ __global__ __launch_bounds__(kThreads) void tile_deposit(const double* __restric
                                                          double* __restrict__ rho) {
   if (blockIdx.x >= *nitems) return;
   extern __shared__ double acc[];
+  __shared__ uint32_t rcount[kBankWords], roff[kBankWords], rmax;
   const int4 it = items[blockIdx.x];
   const TileBox box(g, it.x);
   const int nz = g.nz, cells = box.cells();
-  for (int c = threadIdx.x; c < cells; c += blockDim.x) acc[c] = 0.0;
-  __syncthreads();
+  double* sxf = acc + tile_dep_acc_doubles(cells);
+  double* syf = sxf + kDepStage;
+  double* szf = syf + kDepStage;
+  double* smm = szf + kDepStage;
+  int* sbz = (int*)(smm + kDepStage);
   const int64_t gsx = (int64_t)g.ny * nz;
-  // the next particle's coordinates are requested before this one is deposited
-  int i = it.y + threadIdx.x;
-  double qx = 0.0, qy = 0.0, qz = 0.0, qm = macrosize;
-  if (i < it.z) {
-    qx = cur[i];
-    qy = cur[2 * cap + i];
-    qz = cur[4 * cap + i];
-    if (msize) qm = msize[i];
+  const int np = it.z - it.y;  // at most kChunk
+  // all of a thread's particle loads go out before the block is cleared
+  double px[kDepU], py[kDepU], pz[kDepU], pm[kDepU];
+#pragma unroll
+  for (int u = 0; u < kDepU; u++) {
+    const int j = (int)threadIdx.x + u * kThreads;
+    const int64_t i = it.y + (j < np ? j : 0);
+    px[u] = cur[i];
+    py[u] = cur[2 * cap + i];
+    pz[u] = cur[4 * cap + i];
+    pm[u] = msize ? msize[i] : macrosize;
   }
-  for (; i < it.z; i += blockDim.x) {
-    const double x = qx, y = qy, z = qz, m = qm;
-    const int inext = i + (int)blockDim.x;
-    if (inext < it.z) {
-      qx = cur[inext];
-      qy = cur[2 * cap + inext];
-      qz = cur[4 * cap + inext];
-      if (msize) qm = msize[inext];
+  for (int c = threadIdx.x; c < cells; c += blockDim.x) acc[c] = 0.0;
+#pragma unroll
+  for (int h = 0; h < 2; h++) {
+    if (threadIdx.x < kBankWords) rcount[threadIdx.x] = 0u;
+    __syncthreads();  // (h = 0: the block is clear)
+    // rank each particle within its residue class (particles outside the block: global)
+    constexpr int H = kDepU / 2;
+    int bz[H];
+    double xF[H], yF[H], zF[H];
+    uint32_t res[H], rank[H];
+#pragma unroll
+    for (int v = 0; v < H; v++) {
+      const int u = h * H + v;
+      const int j = (int)threadIdx.x + u * kThreads;
+      res[v] = kBankWords;  // none
+      if (j >= np) continue;
+      int iX, iY, iZ;
+      node_ind_frac(px[u], g.xMin, g.dx, g.nx, iX, xF[v]);
+      node_ind_frac(py[u], g.yMin, g.dy, g.ny, iY, yF[v]);
+      grid3d_z_ind_frac(pz[u], g.zMin, g.dz, nz, iZ, zF[v]);
+      const int zlo = nz >= 3 ? iZ - 1 : 0, zhi = nz >= 3 ? iZ + 1 : nz - 1;
+      if (!box.holds(iX, iY, zlo, zhi)) {  // not in this tile's block (cannot happen for a
+        // consistent sort): global
+        deposit_particle(rho + (iX - 1) * gsx + (int64_t)(iY - 1) * nz, gsx, nz, nz, 0, pm[u],
+                         xF[v], yF[v], iZ, zF[v]);
+        continue;
+      }
+      const int b0 = box.idx(iX - 1, iY - 1, box.z0);  // < 2^15 (tile blocks <= 116 KiB)
+      bz[v] = (b0 << 16) | iZ;
+      const int w0 = b0 + (nz >= 3 ? iZ - 1 - box.z0 : -box.z0);
+      res[v] = (uint32_t)w0 & (kBankWords - 1);
+      rank[v] = atomicAdd(&rcount[res[v]], 1u);
     }
-    int iX, iY, iZ;
-    double xF, yF, zF;
-    node_ind_frac(x, g.xMin, g.dx, g.nx, iX, xF);
-    node_ind_frac(y, g.yMin, g.dy, g.ny, iY, yF);
-    grid3d_z_ind_frac(z, g.zMin, g.dz, nz, iZ, zF);
-    const int zlo = nz >= 3 ? iZ - 1 : 0, zhi = nz >= 3 ? iZ + 1 : nz - 1;
-    if (box.holds(iX, iY, zlo, zhi)) {
-      deposit_particle(acc + box.idx(iX - 1, iY - 1, box.z0), box.W * box.Wz, box.Wz, nz,
-                             box.z0, m, xF, yF, iZ, zF);
-    } else {  // not in this tile's block (cannot happen for a consistent sort): global
-      deposit_particle(rho + (iX - 1) * gsx + (int64_t)(iY - 1) * nz, gsx, nz, nz, 0, m,
-                              xF, yF, iZ, zF);
+    __syncthreads();
+    if (threadIdx.x < 64) {  // one wave: exclusive scan and maximum of the class counts
+      const int l = threadIdx.x;
+      const uint32_t c = l < kBankWords ? rcount[l] : 0u;
+      uint32_t sc = c, mx = c;
+      for (int o = 1; o < 64; o <<= 1) {
+        const uint32_t t = __shfl_up(sc, o, 64);
+        if (l >= o) sc += t;
+        mx = max(mx, __shfl_xor(mx, o, 64));
+      }
+      if (l < kBankWords) roff[l] = sc - c;
+      if (l == 0) rmax = mx;
     }
+    __syncthreads();
+#pragma unroll
+    for (int v = 0; v < H; v++) {
+      if (res[v] >= (uint32_t)kBankWords) continue;
+      const uint32_t q = roff[res[v]] + rank[v];
+      sxf[q] = xF[v];
+      syf[q] = yF[v];
+      szf[q] = zF[v];
+      smm[q] = pm[h * H + v];
+      sbz[q] = bz[v];
+    }
+    __syncthreads();
+    const int l = (int)threadIdx.x & (kBankWords - 1);
+    const int grp = (int)threadIdx.x / kBankWords, ngrp = kThreads / kBankWords;
+    const uint32_t cl = rcount[l], ol = roff[l], rm = rmax;
+    for (uint32_t k = grp; k < rm; k += ngrp) {
+      if (k >= cl) continue;
+      const uint32_t q = ol + k;
+      const int b = sbz[q];
+      deposit_particle(acc + (b >> 16), box.W * box.Wz, box.Wz, nz, box.z0, smm[q], sxf[q],
+                       syf[q], b & 0xFFFF, szf[q]);
+    }
+    __syncthreads();  // every wave is done with the counters and the staging of this half
   }
-  __syncthreads();
   for (int c = threadIdx.x; c < cells; c += blockDim.x) {
     const double v = acc[c];
     if (v == 0.0) continue;
@@ -483,6 +552,12 @@ TileGeom make_tile_geom(int nx, int ny, int nz, double xMin, double xMax, double
   return g;
 }
 
+size_t tile_deposit_lds_bytes(const TileGeom& g) {
+  const int cells = (g.T + 2) * (g.T + 2) * (g.Tz + 2 * g.zh);
+  return sizeof(double) * tile_dep_acc_doubles(cells) +
+         (size_t)kDepStage * (4 * sizeof(double) + sizeof(int));
+}
+
 size_t tile_lds_bytes(const TileGeom& g) {
   return sizeof(double) * (g.T + 2) * (g.T + 2) * (g.Tz + 2 * g.zh);
 }
@@ -525,8 +600,8 @@ void sort_and_deposit_3d(ob_bunch* b, const TileGeom& g, double macrosize, doubl
                          TileWork& w) {
   static bool attr_done = false;
   if (!attr_done) {
-    OB_HIP(hipFuncSetAttribute((const void*)tile_deposit,
-                               hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024));
+    OB_HIP(hipFuncSetAttribute((const void*)tile_deposit,  // (plus its static class counters)
+                               hipFuncAttributeMaxDynamicSharedMemorySize, 159 * 1024));
     OB_HIP(hipFuncSetAttribute((const void*)tile_hist,
                                hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024));
     OB_HIP(hipFuncSetAttribute((const void*)tile_scatter,
@@ -541,6 +616,8 @@ void sort_and_deposit_3d(ob_bunch* b, const TileGeom& g, double macrosize, doubl
   }
   const int64_t n = bunch_size(b);
   OB_REQUIRE((int64_t)(g.ntiles + 1) * 4 <= 150 * 1024, "tile sort: too many tiles for LDS");
+  OB_REQUIRE(tile_deposit_lds_bytes(g) <= 159 * 1024, "tile deposit: tile block too large for LDS");
+  OB_REQUIRE(g.nz < 65536, "tile deposit: more than 65535 z nodes");
   OB_REQUIRE(n < (int64_t)1 << 31, "tile sort: more than 2^31 particles");
   w.reserve(n, g.ntiles);
   hipStream_t s = stream();
@@ -576,7 +653,8 @@ void sort_and_deposit_3d(ob_bunch* b, const TileGeom& g, double macrosize, doubl
   const int64_t max_items = n / kChunk + g.ntiles + 1;
   {
     ProfScope ps("tile_deposit");
-    hipLaunchKernelGGL(tile_deposit, dim3((unsigned)max_items), dim3(kThreads), tile_lds_bytes(g),
+    hipLaunchKernelGGL(tile_deposit, dim3((unsigned)max_items), dim3(kThreads),
+                       tile_deposit_lds_bytes(g),
                        s, v.c[0], (int64_t)(v.c[1] - v.c[0]), v.msize, macrosize, g, w.items,
                        w.nitems, rho);
   }
