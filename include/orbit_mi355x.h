@@ -295,7 +295,7 @@ int ob_sc3d_green_rebuilds(ob_sc3d* c, int* n);
 /* Deposit strategy (no reference counterpart; results agree to FP64 summation order):
  * tiled != 0 sorts the device bunch by xy tile and accumulates each tile in LDS (default),
  * tiled == 0 scatters in particle order with global atomics.  lds_budget_bytes bounds the
- * per-tile LDS block (0 keeps the current value). */
+ * per-tile LDS block, 1 KiB..116 KiB (0 keeps the current value; default 16 KiB). */
 int ob_sc3d_set_deposit_mode(ob_sc3d* c, int tiled, int lds_budget_bytes);
 
 /* ---- batches of bunches (no reference counterpart) ---------------------------------
@@ -317,7 +317,9 @@ int ob_batch_flush(ob_batch* batch);
 /* Wait; fails if a bunch had zero extent in its last SC step (the reference aborts). */
 int ob_batch_synchronize(ob_batch* batch);
 /* Autotuning: particles per push/binning work item (chunk) and per tile-deposit work item
- * (deposit_item).  Tuned by timing on the first SC step unless set here. */
+ * (deposit_item).  Tuned by timing on the first SC step unless set here; the choice is kept
+ * for the process per device and batch shape (bunch sizes, grids), and a batch of a shape
+ * already tuned takes it without timing. */
 int ob_batch_set_tuning(ob_batch* batch, int chunk, int deposit_item);
 int ob_batch_get_tuning(ob_batch* batch, int* chunk, int* deposit_item, int* tuned);
 /* hipGraph: the batch calls between begin and end are recorded (not run) and replayed by
