@@ -297,6 +297,25 @@ def test_sc3d_tiled_matches_particle_order_deposit(budget):
             assert np.array_equal(o1[k], ob.coords[k])
 
 
+def test_sc3d_deposit_budget_range():
+    """The tile deposit stages half a work item beside its LDS block: budgets above 116 KiB
+    are refused, 116 KiB is accepted and deposits like the default."""
+    import spacecharge
+    from _obnative import OrbitError
+    from workloads import waterbag_coords
+    c = spacecharge.SpaceChargeCalc3D(32, 32, 32)
+    with pytest.raises(OrbitError, match="LDS budget"):
+        c.setDepositMode(True, 117 * 1024)
+    c.setDepositMode(True, 116 * 1024)
+    coords = waterbag_coords(20000, seed=23)
+    b1, _ = _bunches(coords)
+    b2, _ = _bunches(coords)
+    c.trackBunch(b1, 0.5)
+    c2 = spacecharge.SpaceChargeCalc3D(32, 32, 32)
+    c2.trackBunch(b2, 0.5)
+    assert rel_l2(c.getRhoGrid().array(), c2.getRhoGrid().array()) < 1e-13
+
+
 def test_poisson_solver_fft2d_find_potential():
     """PoissonSolverFFT2D.findPotential on a user grid (solver on its own box, rho on a
     box of the same step ratio) against the oracle's PoissonSolverFFT2D."""
