@@ -136,6 +136,48 @@ __device__ __forceinline__ void twiddle_powers(double* vr, double* vi,
   }
 }
 
+// Exchange layouts of lds_fft.  Exchange e holds the data between stage e-1 and stage e
+// (exchange 0 is the input, exchange S the output).  Stage e-1 writes groups of
+// Ns = span(e-1) consecutive points every Ns*R points, j-fastest; for Ns < 32 the groups
+// written by one half-wave (32 lanes of doubles cover the 64 four-byte banks once) would
+// share banks.  For N >= 128 such an exchange inserts Ns doubles of padding after every
+// Q = max(Ns*R, 32) points.  The 32/Ns groups of a half-wave then start, modulo 32
+// doubles, at distinct multiples of Ns (group g = a*(Q/(Ns*R)) + b, b < Q/(Ns*R), starts
+// at a*Q + Ns*(b*R + a)), so its 32 lanes use 32 distinct bank pairs.  Stage e reads 32
+// consecutive points starting at a multiple of 32, which stay inside one padded block.
+// The input and output keep element i of line l at l*ls + i; a line needs fft_line(N)
+// doubles.
+__host__ __device__ constexpr int fft_pad_p(int n, int e) {
+  return (n < 128 || e == 0 || e >= fft_stages(n) || fft_span(n, e - 1) >= 32)
+             ? 0
+             : fft_span(n, e - 1);
+}
+__host__ __device__ constexpr int fft_pad_q(int n, int e) {
+  return fft_span(n, e) > 32 ? fft_span(n, e) : 32;
+}
+__host__ __device__ constexpr int fft_line_from(int n, int e) {
+  return e > fft_stages(n)
+             ? n
+             : (n + fft_pad_p(n, e) * (n / fft_pad_q(n, e)) > fft_line_from(n, e + 1)
+                    ? n + fft_pad_p(n, e) * (n / fft_pad_q(n, e))
+                    : fft_line_from(n, e + 1));
+}
+// Doubles per line (re or im) that lds_fft touches.
+__host__ __device__ constexpr int fft_line(int n) { return fft_line_from(n, 0); }
+// A line stride for several lines per block: at least fft_line(n), 1 modulo 32 for the
+// padded lengths (so lines that share a half-wave start on different banks), n + 1 below.
+__host__ __device__ constexpr int fft_line_stride(int n) {
+  return fft_line(n) == n ? n + 1 : fft_line(n) + ((33 - fft_line(n) % 32) % 32);
+}
+template <int N, int E>
+__device__ __forceinline__ int fft_xidx(int i) {
+  constexpr int P = fft_pad_p(N, E), Q = fft_pad_q(N, E);
+  if constexpr (P == 0)
+    return i;
+  else
+    return i + P * (int)((unsigned)i / Q);
+}
+
 // One Stockham stage S of an N-point transform over `lines` (<= LMAX) lines, NT threads.
 template <int N, int S, int LMAX, int NT, int SIGN>
 __device__ __forceinline__ void fft_stage(double* __restrict__ re, double* __restrict__ im,
@@ -154,12 +196,13 @@ __device__ __forceinline__ void fft_stage(double* __restrict__ re, double* __res
     const int b = (int)threadIdx.x + q * NT;
     if (b < total) {
       const int line = b / NB, j = b - line * NB;
-      const double* lr = re + line * ls + j;
-      const double* li = im + line * ls + j;
+      const double* lr = re + line * ls;
+      const double* li = im + line * ls;
 #pragma unroll
       for (int m = 0; m < R; m++) {
-        vr[q][m] = lr[m * NB];
-        vi[q][m] = li[m * NB];
+        const int a = fft_xidx<N, S>(j + m * NB);
+        vr[q][m] = lr[a];
+        vi[q][m] = li[a];
       }
       if constexpr (Ns > 1) {  // v[m] *= exp(SIGN 2 pi i m k / (Ns R))
         const int k = j & (Ns - 1);
@@ -175,12 +218,13 @@ __device__ __forceinline__ void fft_stage(double* __restrict__ re, double* __res
     if (b < total) {
       const int line = b / NB, j = b - line * NB;
       const int k = j & (Ns - 1);
-      double* dr = re + line * ls + (j - k) * R + k;
-      double* di = im + line * ls + (j - k) * R + k;
+      double* dr = re + line * ls;
+      double* di = im + line * ls;
 #pragma unroll
       for (int m = 0; m < R; m++) {
-        dr[m * Ns] = vr[q][m];
-        di[m * Ns] = vi[q][m];
+        const int a = fft_xidx<N, S + 1>((j - k) * R + k + m * Ns);
+        dr[a] = vr[q][m];
+        di[a] = vi[q][m];
       }
     }
   }
@@ -198,6 +242,7 @@ __device__ __forceinline__ void fft_stages_from(double* re, double* im, int line
 
 // N-point FFT of `lines` (<= LMAX) lines in LDS with NT threads; every thread of the block
 // must call it, and the data must be visible to all threads on entry (barrier before).
+// Line l occupies re/im[l*ls .. l*ls + fft_line(N)), so ls >= fft_line(N) when lines > 1.
 // Returns after a barrier.
 template <int N, int LMAX, int NT, int SIGN>
 __device__ __forceinline__ void lds_fft(double* __restrict__ re, double* __restrict__ im,

@@ -835,7 +835,8 @@ __device__ __forceinline__ void rows_r2c_body(const double* __restrict__ in, int
   if (st && st->skip) return;
   extern __shared__ double sm[];
   double* re = sm;
-  double* im = sm + N;
+  constexpr int LINE = fft_line(N);  // evaluated at compile time
+  double* im = sm + LINE;
   const int r0 = 2 * blk, r1 = r0 + 1;
   for (int j = threadIdx.x; j < N; j += kFFTThreads) {
     re[j] = (j < ncols_in) ? in[(int64_t)r0 * ld_in + j] : 0.0;
@@ -893,7 +894,8 @@ __device__ __forceinline__ void cols_body(double2* __restrict__ spec, int nrows_
   if (st && st->skip) return;
   extern __shared__ double sm[];
   double* re = sm;
-  double* im = sm + N;
+  constexpr int LINE = fft_line(N);  // evaluated at compile time
+  double* im = sm + LINE;
   // The column and (for a solve) its Green values are requested together before the
   // transform; a masked lane loads element 0 and drops it.
   constexpr int CI = (N + kFFTThreads - 1) / kFFTThreads;
@@ -956,7 +958,8 @@ __device__ __forceinline__ void rows_c2r_body(const double2* __restrict__ spec, 
   if (st && st->skip) return;
   extern __shared__ double sm[];
   double* re = sm;
-  double* im = sm + N;
+  constexpr int LINE = fft_line(N);  // evaluated at compile time
+  double* im = sm + LINE;
   constexpr int H = N / 2 + 1;
   const int r0 = 2 * blk, r1 = r0 + 1;
   const bool has1 = r1 < nrows;
@@ -1197,7 +1200,7 @@ struct ob_sc2p5d : ob_sc2p5drb {};
 namespace ob {
 namespace {
 template <int N>
-size_t fft_lds() { return sizeof(double) * 2 * N; }
+size_t fft_lds() { return sizeof(double) * 2 * fft_line(N); }
 
 // Dynamic LDS above 64 KiB has to be allowed per kernel; done once per instantiation.
 template <typename K>
@@ -1438,7 +1441,7 @@ void launch_sc2d_batch(const BatchBunch* bb, const BatchItem* items, const Batch
     hipLaunchKernelGGL(sc2d_post_batch, dim3(sh.nb), dim3(kSetupThreads), 0, s, bb, length,
                        pipe_radius);
   }
-  const size_t fl = sizeof(double) * 2 * sh.max_M;
+  const size_t fl = sizeof(double) * 2 * fft_line(sh.max_M);  // fft_line grows with N
   {
     ProfScope ps("fft_rows_r2c_batch");
     hipLaunchKernelGGL(rows_r2c_batch, dim3(sh.row_blocks), dim3(kFFTThreads), fl, s, bb, sh.nb);

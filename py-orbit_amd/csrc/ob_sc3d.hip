@@ -186,7 +186,7 @@ __global__ __launch_bounds__(kThreads) void fft3_z_r2c(const double* __restrict_
                                                        int64_t nlines, int NY, int nz_in, int M1,
                                                        int H2p, double2* __restrict__ spec,
                                                        const double2* __restrict__ tw, int nmax) {
-  constexpr int P = zpass_pairs(N), ls = N + 1, H2 = N / 2 + 1;
+  constexpr int P = zpass_pairs(N), ls = fft_line_stride(N), H2 = N / 2 + 1;
   extern __shared__ double sm[];
   double* re = sm;
   double* im = sm + P * ls;
@@ -242,7 +242,7 @@ __global__ __launch_bounds__(kThreads) void fft3_z_c2r(const double2* __restrict
                                                        int nz_out, double denom,
                                                        double* __restrict__ phi,
                                                        const double2* __restrict__ tw, int nmax) {
-  constexpr int P = zpass_pairs(N), ls = N + 1, H2 = N / 2 + 1;
+  constexpr int P = zpass_pairs(N), ls = fft_line_stride(N), H2 = N / 2 + 1;
   extern __shared__ double sm[];
   double* re = sm;
   double* im = sm + P * ls;
@@ -497,7 +497,7 @@ template <int N>  // N = M2
 __global__ __launch_bounds__(kThreads) void green3d_z(int qx, int qy, int H2p,
                                                       double* __restrict__ ghat, GreenGeom gg,
                                                       const double2* __restrict__ tw, int nmax) {
-  constexpr int P = zpass_pairs(N), ls = N + 1, H2 = N / 2 + 1;
+  constexpr int P = zpass_pairs(N), ls = fft_line_stride(N), H2 = N / 2 + 1;
   extern __shared__ double sm[];
   double* re = sm;
   double* im = sm + P * ls;
@@ -528,7 +528,7 @@ __global__ __launch_bounds__(kThreads) void green3d_axis(double* __restrict__ gh
                                                          int64_t outer_stride, int H2,
                                                          const double2* __restrict__ tw,
                                                          int nmax) {
-  constexpr int L = strided_lines(N), C = L / 2, ls = N + 1, Q = N / 2 + 1;
+  constexpr int L = strided_lines(N), C = L / 2, ls = fft_line_stride(N), Q = N / 2 + 1;
   extern __shared__ double sm[];
   double* re = sm;
   double* im = sm + C * ls;
@@ -603,7 +603,7 @@ void allow_lds(K* kernel, size_t bytes) {
 }
 
 template <int N>
-size_t zpass_lds() { return sizeof(double) * 2 * zpass_pairs(N) * (N + 1); }
+size_t zpass_lds() { return sizeof(double) * 2 * zpass_pairs(N) * fft_line_stride(N); }
 template <int N>
 size_t strided_lds() { return sizeof(double) * 2 * strided_lines(N) * strided_line_doubles<N>(); }
 
@@ -1158,7 +1158,7 @@ void slice_set_solver(ob_slice2d* c, double xMin, double xMax, double yMin, doub
       constexpr int N = decltype(NC)::value;
       if constexpr (N >= 8 && N <= 2048) {
         static bool ready = false;
-        const size_t lds = sizeof(double) * 2 * (strided_lines(N) / 2) * (N + 1);
+        const size_t lds = sizeof(double) * 2 * (strided_lines(N) / 2) * fft_line_stride(N);
         if (!ready) {
           allow_lds(green3d_axis<N>, lds);
           ready = true;
