@@ -133,7 +133,10 @@ def test_sc2p5drb_zero_extent_raises():
         b.getCoordinates()
 
 
-@pytest.mark.parametrize("grid", [(16, 16, 16), (32, 32, 16), (17, 24, 9)])
+# (24, 20, 64) and (16, 16, 256): z transforms of 128 and 512 points, whose LDS exchanges
+# are padded against bank conflicts
+@pytest.mark.parametrize("grid", [(16, 16, 16), (32, 32, 16), (17, 24, 9), (24, 20, 64),
+                                  (16, 16, 256)])
 def test_sc3d_step(grid):
     import spacecharge
     from workloads import waterbag_coords
@@ -316,22 +319,23 @@ def test_sc3d_deposit_budget_range():
     assert rel_l2(c.getRhoGrid().array(), c2.getRhoGrid().array()) < 1e-13
 
 
-def test_poisson_solver_fft2d_find_potential():
+# padded transform lengths (rows x columns): 128 x 64, 1024 x 512, 2048 x 2048, 4096 x 128
+@pytest.mark.parametrize("nx,ny", [(40, 24), (300, 200), (600, 520), (1100, 40)])
+def test_poisson_solver_fft2d_find_potential(nx, ny):
     """PoissonSolverFFT2D.findPotential on a user grid (solver on its own box, rho on a
     box of the same step ratio) against the oracle's PoissonSolverFFT2D."""
     import spacecharge
     r = np.random.default_rng(8)
-    nx, ny = 40, 24
     solver_lims = (-2.0, 2.0, -1.0, 1.0)
-    # the rho grid's steps keep dx/dy of the solver: (xMax-xMin)/(yMax-yMin) = 2
-    rho_lims = (0.3, 0.3 + 0.05 * 2, -0.7, -0.7 + 0.05)
+    # the rho grid's steps keep dx/dy of the solver:
+    # (xMax-xMin)/(yMax-yMin) = 4 (ny-1) / (2 (nx-1))
+    hx = 0.05
+    hy = hx * (2.0 / (ny - 1)) / (4.0 / (nx - 1))
+    rho_lims = (0.3, 0.3 + hx * (nx - 1), -0.7, -0.7 + hy * (ny - 1))
     ps = spacecharge.PoissonSolverFFT2D(nx, ny, *solver_lims)
-    rho = spacecharge.Grid2D(nx, ny, *rho_lims)
-    phi = spacecharge.Grid2D(nx, ny, *rho_lims)
     arr = r.normal(size=(nx, ny))
-    for ix in range(nx):
-        for iy in range(ny):
-            rho.setValue(arr[ix, iy], ix, iy)
+    rho = spacecharge.Grid2D(arr, *rho_lims)
+    phi = spacecharge.Grid2D(nx, ny, *rho_lims)
     ps.findPotential(rho, phi)
     assert rel_l2(phi.array(), O.poisson2d(arr, solver_lims, rho_lims)) < 1e-12
     bad = spacecharge.Grid2D(nx, ny, 0.0, 1.0, 0.0, 1.0)  # step ratio differs
