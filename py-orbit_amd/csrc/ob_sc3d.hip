@@ -16,6 +16,7 @@
 // The Green spectrum of the even function 1/r is real and even in kx and ky, so only the
 // quadrant kx <= M0/2, ky <= M1/2 of its real part is stored (8x less than the complex
 // spectrum the reference keeps; the discarded imaginary part is rounding noise).
+#include <chrono>
 #include <cmath>
 #include <cstring>
 
@@ -80,8 +81,13 @@ __device__ __forceinline__ double green3d(int i, int j, int k, const GreenGeom& 
 }
 
 // ---- extrema reduction (one block) -------------------------------------------
+// Writes the six extrema to out (pinned host memory) and then seq to *seq_out with a
+// system-scope release, all from one thread, so a host that reads *seq_out == seq (with
+// acquire) reads the extrema of this launch.
 __global__ __launch_bounds__(1024) void reduce_extrema(const double* __restrict__ partials,
-                                                       int nblk, double* __restrict__ out) {
+                                                       int nblk, double* __restrict__ out,
+                                                       uint64_t* __restrict__ seq_out,
+                                                       uint64_t seq) {
   __shared__ double red[16][6];
   double r[6] = {DBL_MAX, -DBL_MAX, DBL_MAX, -DBL_MAX, DBL_MAX, -DBL_MAX};
   // four rows per thread and round, their loads in flight together
@@ -107,13 +113,60 @@ __global__ __launch_bounds__(1024) void reduce_extrema(const double* __restrict_
   if ((threadIdx.x & 63) == 0)
     for (int k = 0; k < 6; k++) red[threadIdx.x >> 6][k] = r[k];
   __syncthreads();
-  if (threadIdx.x < 6) {
-    const int k = threadIdx.x;
-    double acc = red[0][k];
-    for (int w = 1; w < (int)(blockDim.x >> 6); w++)
-      acc = (k & 1) ? fmax(acc, red[w][k]) : fmin(acc, red[w][k]);
-    out[k] = acc;
+  if (threadIdx.x == 0) {
+    double acc[6];
+    for (int k = 0; k < 6; k++) {
+      acc[k] = red[0][k];
+      for (int w = 1; w < (int)(blockDim.x >> 6); w++)
+        acc[k] = (k & 1) ? fmax(acc[k], red[w][k]) : fmin(acc[k], red[w][k]);
+    }
+    for (int k = 0; k < 6; k++) out[k] = acc[k];
+    __hip_atomic_store(seq_out, seq, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_SYSTEM);
   }
+}
+
+// The bunch extrema on the host for the grid sizing of a step (bunchAnalysis runs on the
+// host, as its Green-function rebuild decision launches kernels).  reduce_extrema writes
+// them straight into a pinned coherent buffer, so there is no copy kernel, and the host
+// watches the sequence number instead of waiting for the stream to drain: it spins for at
+// most 20 ms, then synchronises the stream.
+struct ExtremaLink {
+  double* host = nullptr;  // [0..5] xMin, xMax, yMin, yMax, zMin, zMax; [6] sequence number
+  double* dev = nullptr;   // device address of host
+  uint64_t seq = 0;
+};
+
+void extrema_link_create(ExtremaLink& e) {
+  OB_HIP(hipHostMalloc((void**)&e.host, sizeof(double) * 8, hipHostMallocCoherent));
+  std::memset(e.host, 0, sizeof(double) * 8);
+  OB_HIP(hipHostGetDevicePointer((void**)&e.dev, e.host, 0));
+  e.seq = 0;
+}
+
+void extrema_link_destroy(ExtremaLink& e) {
+  if (e.host) (void)hipHostFree(e.host);
+  e.host = e.dev = nullptr;
+}
+
+const double* extrema_to_host(ExtremaLink& e, const double* partials, int nblk, hipStream_t s) {
+  const uint64_t want = ++e.seq;
+  uint64_t* seq_host = reinterpret_cast<uint64_t*>(e.host + 6);
+  {
+    ProfScope ps_("reduce_extrema");
+    hipLaunchKernelGGL(reduce_extrema, dim3(1), dim3(1024), 0, s, partials, nblk, e.dev,
+                       reinterpret_cast<uint64_t*>(e.dev + 6), want);
+  }
+  OB_HIP(hipGetLastError());
+  const auto t0 = std::chrono::steady_clock::now();
+  while (__atomic_load_n(seq_host, __ATOMIC_ACQUIRE) != want) {
+    if (std::chrono::steady_clock::now() - t0 > std::chrono::milliseconds(20)) {
+      OB_HIP(hipStreamSynchronize(s));
+      OB_REQUIRE(__atomic_load_n(seq_host, __ATOMIC_ACQUIRE) == want,
+                 "bunch extrema: the reduction did not reach the host");
+      break;
+    }
+  }
+  return e.host;
 }
 
 __global__ void fill_zero(double* __restrict__ a, int64_t n) {
@@ -577,8 +630,8 @@ struct ob_sc3d {
   double sMin[3], sMax[3], sd[3];  // solver grid limits and steps (PoissonSolver3D.cc)
   double g[9];                     // rho/phi grid: xMin,xMax,yMin,yMax,zMin,zMax (rest), dx,dy,dz
   int rebuilds = 0;
-  double *rho = nullptr, *phi = nullptr, *ghat = nullptr, *partials = nullptr, *ext_dev = nullptr;
-  double* ext_host = nullptr;
+  double *rho = nullptr, *phi = nullptr, *ghat = nullptr, *partials = nullptr;
+  ob::ExtremaLink ext;             // bunch extrema to the host
   double2 *spec = nullptr, *tw = nullptr;
   int nbunches = 0;                // external bunches (PoissonSolverFFT3D nBunches_, even)
   double frequency = 402.5e+6;     // bunch frequency (SpaceChargeCalc3D.cc:58)
@@ -793,8 +846,7 @@ OB_API int ob_sc3d_create(int nx, int ny, int nz, ob_sc3d** out) {
   OB_HIP(hipMalloc((void**)&c->spec, sizeof(double2) * (size_t)nx * c->M1 * c->H2p));
   OB_HIP(hipMalloc((void**)&c->ghat, sizeof(double) * (size_t)c->qx * c->qy * c->H2g));
   OB_HIP(hipMalloc((void**)&c->partials, sizeof(double) * 6 * 4096));
-  OB_HIP(hipMalloc((void**)&c->ext_dev, sizeof(double) * 6));
-  OB_HIP(hipHostMalloc((void**)&c->ext_host, sizeof(double) * 6, hipHostMallocDefault));
+  ob::extrema_link_create(c->ext);
   OB_HIP(hipMemset(c->rho, 0, sizeof(double) * ng));
   OB_HIP(hipMemset(c->phi, 0, sizeof(double) * ng));
   OB_HIP(hipMemset(c->spec, 0, sizeof(double2) * (size_t)nx * c->M1 * c->H2p));
@@ -830,8 +882,8 @@ OB_API int ob_sc3d_destroy(ob_sc3d* c) {
   if (c->ev_main) (void)hipEventDestroy(c->ev_main);
   if (c->ev_green) (void)hipEventDestroy(c->ev_green);
   (void)hipFree(c->rho); (void)hipFree(c->phi); (void)hipFree(c->spec); (void)hipFree(c->ghat);
-  (void)hipFree(c->partials); (void)hipFree(c->ext_dev); (void)hipFree(c->tw);
-  (void)hipHostFree(c->ext_host);
+  (void)hipFree(c->partials); (void)hipFree(c->tw);
+  ob::extrema_link_destroy(c->ext);
   c->work.release();
   delete c;
   OB_REQUIRE(err.empty(), err);
@@ -845,17 +897,11 @@ OB_API int ob_sc3d_track(ob_sc3d* c, ob_bunch* b, double length) {
   if (n < 2) return 0;  // :102-103
   hipStream_t s = ob::stream();
   const int nblk = ob::bunch_flush(b, c->partials);
-  {
-    ob::ProfScope ps_("reduce_extrema");
-    hipLaunchKernelGGL(ob::reduce_extrema, dim3(1), dim3(1024), 0, s, c->partials, nblk, c->ext_dev);
-  }
-  OB_HIP(hipGetLastError());
-  OB_HIP(hipMemcpyAsync(c->ext_host, c->ext_dev, sizeof(double) * 6, hipMemcpyDeviceToHost, s));
-  OB_HIP(hipStreamSynchronize(s));
+  const double* ext = ob::extrema_to_host(c->ext, c->partials, nblk, s);
   const ob::SyncInfo sy = ob::bunch_sync(b);
   const double gamma = sy.gamma, beta = sy.beta;
-  double xMin = c->ext_host[0], xMax = c->ext_host[1], yMin = c->ext_host[2];
-  double yMax = c->ext_host[3], zMin = c->ext_host[4], zMax = c->ext_host[5];
+  double xMin = ext[0], xMax = ext[1], yMin = ext[2];
+  double yMax = ext[3], zMin = ext[4], zMax = ext[5];
   double center, width;
   const bool wrapped = c->nbunches > 0;
   double lambda = 0.0, lambda_cm = 0.0;
@@ -1064,7 +1110,7 @@ struct ob_slice2d {
   bool green_valid = false;
   double g[9];           // last grid: xMin, xMax, yMin, yMax, zMin, zMax, dx, dy, dz
   double *rho = nullptr, *phi = nullptr, *ghat = nullptr, *partials = nullptr;
-  double *ext_dev = nullptr, *ext_host = nullptr;
+  ob::ExtremaLink ext;   // bunch extrema to the host
   double2 *spec = nullptr, *tw = nullptr;
   ob::SC2DState* geo = nullptr;    // grid geometry for the boundary kernels (device)
   ob::SC2DState* geo_host = nullptr;  // pinned staging of it
@@ -1183,16 +1229,10 @@ void slice_track(ob_slice2d* c, ob_bunch* b, double length, ob_boundary2d* bnd) 
   if (n < 2) return;  // :69-70
   hipStream_t s = stream();
   const int nblk = bunch_flush(b, c->partials);
-  {
-    ProfScope ps_("reduce_extrema");
-    hipLaunchKernelGGL(reduce_extrema, dim3(1), dim3(1024), 0, s, c->partials, nblk, c->ext_dev);
-  }
-  OB_HIP(hipGetLastError());
-  OB_HIP(hipMemcpyAsync(c->ext_host, c->ext_dev, sizeof(double) * 6, hipMemcpyDeviceToHost, s));
-  OB_HIP(hipStreamSynchronize(s));
+  const double* ext = extrema_to_host(c->ext, c->partials, nblk, s);
   const SyncInfo sy = bunch_sync(b);
-  double xMin = c->ext_host[0], xMax = c->ext_host[1], yMin = c->ext_host[2];
-  double yMax = c->ext_host[3], zMin = c->ext_host[4], zMax = c->ext_host[5];
+  double xMin = ext[0], xMax = ext[1], yMin = ext[2];
+  double yMax = ext[3], zMin = ext[4], zMax = ext[5];
   double bnd6[6] = {0, 0, 0, 0, 0, 0};
   int bnd_type = 0;
   // bunchAnalysis (:116-218), scalar arithmetic as in the reference
@@ -1343,8 +1383,7 @@ OB_API int ob_slice2d_create(int nx, int ny, int nz, double xy_ratio, ob_slice2d
   OB_HIP(hipMalloc((void**)&c->spec, sizeof(double2) * (size_t)nx * c->M1 * c->H2p));
   OB_HIP(hipMalloc((void**)&c->ghat, sizeof(double) * (size_t)c->qx * c->qy));
   OB_HIP(hipMalloc((void**)&c->partials, sizeof(double) * 6 * 4096));
-  OB_HIP(hipMalloc((void**)&c->ext_dev, sizeof(double) * 6));
-  OB_HIP(hipHostMalloc((void**)&c->ext_host, sizeof(double) * 6, hipHostMallocDefault));
+  ob::extrema_link_create(c->ext);
   OB_HIP(hipMalloc((void**)&c->geo, sizeof(ob::SC2DState)));
   OB_HIP(hipHostMalloc((void**)&c->geo_host, sizeof(ob::SC2DState), hipHostMallocDefault));
   std::memset(c->geo_host, 0, sizeof(ob::SC2DState));
@@ -1368,9 +1407,9 @@ OB_API int ob_slice2d_destroy(ob_slice2d* c) {
   }
   (void)hipStreamSynchronize(ob::stream());
   for (void* p : {(void*)c->rho, (void*)c->phi, (void*)c->spec, (void*)c->ghat,
-                  (void*)c->partials, (void*)c->ext_dev, (void*)c->tw, (void*)c->geo})
+                  (void*)c->partials, (void*)c->tw, (void*)c->geo})
     if (p) (void)hipFree(p);
-  (void)hipHostFree(c->ext_host);
+  ob::extrema_link_destroy(c->ext);
   (void)hipHostFree(c->geo_host);
   c->work.release();
   delete c;
