@@ -641,7 +641,8 @@ struct ob_sc3d {
   int lds_budget = 16 * 1024;     // LDS bytes for one deposit / kick tile
   ob::TileWork work;
   hipEvent_t ev_main = nullptr, ev_green = nullptr;  // Green rebuild on the second stream
-  bool green_pending = false;
+  bool green_pending = false;      // ev_green recorded, not yet waited on by the main stream
+  bool green_launch = false;       // box changed, Green kernels not yet enqueued
 };
 
 namespace ob {
@@ -693,12 +694,17 @@ void launch_strided(ob_sc3d* c, int n, int n_in, int n_out, int64_t line_stride,
   });
 }
 
-// PoissonSolverFFT3D::setGridXYZ + _defineGreenF (:136-247)
-void set_solver_grid(ob_sc3d* c, double xMin, double xMax, double yMin, double yMax, double zMin,
-                     double zMax, hipStream_t s) {
+// PoissonSolverFFT3D::setGridXYZ (:136-158): the solver box and steps, one more rebuild
+void set_solver_box(ob_sc3d* c, double xMin, double xMax, double yMin, double yMax, double zMin,
+                    double zMax) {
   c->sMin[0] = xMin; c->sMax[0] = xMax; c->sd[0] = (xMax - xMin) / (c->nx - 1);
   c->sMin[1] = yMin; c->sMax[1] = yMax; c->sd[1] = (yMax - yMin) / (c->ny - 1);
   c->sMin[2] = zMin; c->sMax[2] = zMax; c->sd[2] = (zMax - zMin) / c->nz;
+  c->rebuilds++;
+}
+
+// _defineGreenF (:160-247) on the current box, enqueued on s
+void build_green(ob_sc3d* c, hipStream_t s) {
   GreenGeom gg{c->M0, c->M1, c->M2, c->sd[0], c->sd[1], c->sd[2], c->nbunches, c->lambda_cm};
   dispatch_pow2(c->M2, [&](auto NC) {
     constexpr int N = decltype(NC)::value;
@@ -743,22 +749,40 @@ void set_solver_grid(ob_sc3d* c, double xMin, double xMax, double yMin, double y
   const int64_t H2q = c->H2g / 2;
   axis(c->M1, H2q, (int64_t)c->qy * H2q, c->qx);  // y: lines (i, kz pair) along j
   axis(c->M0, (int64_t)c->qy * H2q, H2q, c->qy);  // x: lines (j, kz pair) along i
-  c->rebuilds++;
+}
+
+// PoissonSolverFFT3D::setGridXYZ + _defineGreenF (:136-247), enqueued on s
+void set_solver_grid(ob_sc3d* c, double xMin, double xMax, double yMin, double yMax, double zMin,
+                     double zMax, hipStream_t s) {
+  c->green_launch = false;  // supersedes a deferred rebuild of an earlier box
+  set_solver_box(c, xMin, xMax, yMin, yMax, zMin, zMax);
+  build_green(c, s);
 }
 
 // Rebuild the Green function on the second stream: it is first read by the x pass of the
 // next solve, so it overlaps the deposit and the first passes; the previous solves, which
-// read the old one, are enqueued before it starts.
+// read the old one, are enqueued before it starts (ev_main, recorded here).  The box is
+// set at once (the step's grid depends on it); the kernels are enqueued by
+// launch_pending_green after the step's deposit, so the host time spent enqueuing them
+// overlaps the tile sort instead of delaying it.
 void set_solver_grid_async(ob_sc3d* c, double xMin, double xMax, double yMin, double yMax,
                            double zMin, double zMax) {
   OB_HIP(hipEventRecord(c->ev_main, stream()));
+  set_solver_box(c, xMin, xMax, yMin, yMax, zMin, zMax);
+  c->green_launch = true;
+}
+
+void launch_pending_green(ob_sc3d* c) {
+  if (!c->green_launch) return;
+  c->green_launch = false;
   OB_HIP(hipStreamWaitEvent(aux_stream(), c->ev_main, 0));
-  set_solver_grid(c, xMin, xMax, yMin, yMax, zMin, zMax, aux_stream());
+  build_green(c, aux_stream());
   OB_HIP(hipEventRecord(c->ev_green, aux_stream()));
   c->green_pending = true;
 }
 
 void solve(ob_sc3d* c, double denom) {
+  launch_pending_green(c);
   hipStream_t s = stream();
   const int64_t nl = (int64_t)c->nx * c->ny;
   const int64_t H2p = c->H2p;
