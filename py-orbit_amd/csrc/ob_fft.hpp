@@ -1,12 +1,14 @@
 // ob_fft.hpp -- FP64 complex FFTs staged in LDS, used by the Hockney Poisson solvers.
 //
 // A workgroup transforms `lines` lines of power-of-two length N held in LDS as split
-// re/im arrays (line l starts at l*ls; ls = N+1 spreads lines over the banks).  The
-// transform is a Stockham autosort FFT: stages of radix 8 (4 or 2 for the remaining
-// factor of N), natural order in and out, so forward -> multiply -> inverse needs no
-// permutation.  A stage reads each radix-R butterfly into registers, applies the twiddles
-// and an R-point DFT, and after a barrier writes it back in place.  Conventions follow
-// FFTW: forward exponent -1, unnormalised.
+// re/im arrays (line l starts at l*ls; fft_line_stride(N), 1 modulo 32 doubles, spreads
+// lines over the banks).  The transform is a Stockham autosort FFT: stages of radix 8 (4
+// or 2 for the remaining factor of N), natural order in and out, so forward -> multiply
+// -> inverse needs no permutation.  A stage reads each radix-R butterfly into registers,
+// applies the twiddles and an R-point DFT, and after a barrier writes it back in place;
+// for N >= 128 the exchanges between stages are padded against bank conflicts (a line
+// then spans fft_line(N) > N doubles).  Conventions follow FFTW: forward exponent -1,
+// unnormalised.
 //
 // Twiddles come from a table tw[q] = exp(-2 pi i q / nmax), q < nmax, computed on the
 // host in long double from the exact rational angle and rounded once; a butterfly loads
