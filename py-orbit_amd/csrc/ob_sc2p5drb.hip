@@ -3,10 +3,13 @@
 // Per trackBunch (SpaceChargeCalc2p5Drb.cc:110-168), all on the library stream with no
 // host round trip:
 //   1. fused map kernel flushes the bunch's queued maps and emits per-block extrema;
-//   2. setup   (1 block): reduce extrema, size the grids (bunchAnalysis :170-213), clear;
-//   3. deposit          : 9-point TSC charge (Grid2D.cc:120-166) with FP64 atomics, and the
-//                         periodic 2-point z line density (Grid1D.cc:241-275, 568-626)
-//                         accumulated per block in LDS;
+//   2. bin     (every block reduces the extrema and sizes the grids, bunchAnalysis
+//               :170-213): clears rho, counts particles per tile and writes per-block rows
+//               of the periodic 2-point z line density (Grid1D.cc:241-275, 568-626);
+//   3. deposit : 9-point TSC charge (Grid2D.cc:120-166).  Grids up to 128x128 nodes: each
+//               workgroup deposits its particles into a private copy of the grid in LDS
+//               and adds it to rho with FP64 atomics.  Larger grids: counting-sort scatter
+//               of x, y by tile, then one LDS tile block per work item;
 //   4. post    (1 block): dlambda/dz by 3-point LSQ (:257-310), rho moments (:229-254),
 //                         kick factors;
 //   5. Poisson (3 kernels, PoissonSolverFFT2D.cc:174-256): row r2c of two rows per complex
@@ -44,6 +47,14 @@ constexpr int kDepositThreads = 256;
 constexpr int kMaxZBinsLDS = 4096;
 constexpr int kUnroll = 4;  // particles per thread whose loads are issued together
 constexpr int kFFTThreads = 256;
+// Private-grid single-bunch deposit: grids up to kPrivNodes nodes (128 KiB of LDS),
+// bunches up to kPrivMaxParticles, kPrivPerBlock particles per workgroup of kPrivThreads
+// (at 100k particles on 128x128: 12.0 us against 18.6 us for the tile sort's scatter and
+// deposit; 2048 particles per workgroup 12.6 us, 4096 15.7 us, 512 11.7 us).
+constexpr int kPrivNodes = 128 * 128;
+constexpr int64_t kPrivMaxParticles = 1 << 18;
+constexpr int64_t kPrivPerBlock = 1024;
+constexpr int kPrivThreads = 1024;
 
 struct SC2DParams {
   int nx, ny, nz, nblk;       // grid sizes, number of extrema partial blocks
@@ -618,6 +629,90 @@ __global__ __launch_bounds__(kDepositThreads) void sc2d_tdeposit(SC2DParams P,
   extern __shared__ double acc[];
   sc2d_tdeposit_body<true>(P, sc2d_load_grid(st), tg, tg.items[blockIdx.x], per_particle_m != 0, rho,
                      acc, moments ? tg.mpart + (size_t)blockIdx.x * 5 : nullptr);
+}
+
+// Single bunch, grids that fit in LDS (nx*ny <= kPrivNodes): deposit without the tile
+// sort.  Workgroup b takes particles [b*per, min(n, (b+1)*per)) in particle order and
+// deposits them (Grid2D::binValue, Grid2D.cc:142-166, the same weights as the tiled
+// deposit) into its own copy of the whole grid in LDS, then adds the non-zero nodes to
+// rho with global atomics and its share of the rho moments to mpart row b.  Block 0 sets
+// the row count the post pass reads and clears the bin pass's tile counts (unused here).
+__global__ __launch_bounds__(kPrivThreads) void sc2d_pdeposit(SC2DParams P, DeviceBunchView v,
+                                                                 const SC2DState* st, Tile2D tg,
+                                                                 int64_t per,
+                                                                 double* __restrict__ rho_ptr,
+                                                                 int moments) {
+  extern __shared__ double grid[];
+  const Grid2Step g = sc2d_load_grid(st);
+  if (blockIdx.x == 0) {
+    for (int t = threadIdx.x; t < tg.ntiles; t += blockDim.x) tg.counts[t] = 0u;
+    if (threadIdx.x == 0) *tg.nitems = g.skip ? 0u : gridDim.x;
+  }
+  if (g.skip) return;
+  const gptr<const double> px = to_global((const double*)v.c[0]);
+  const gptr<const double> py = to_global((const double*)v.c[2]);
+  const gptr<const double> pm = to_global((const double*)v.msize);
+  const gptr<double> rho = to_global(rho_ptr);
+  const int nx = P.nx, ny = P.ny, nodes = nx * ny;
+  const int64_t lo = (int64_t)blockIdx.x * per, hi = min(v.n, lo + per);
+  const int64_t stride = blockDim.x;
+  // kUnroll particles per thread per pass, loaded together; the first pass's loads go out
+  // before the LDS grid is cleared
+  double x[kUnroll], y[kUnroll], m[kUnroll];
+  auto load = [&](int64_t i0) {
+#pragma unroll
+    for (int u = 0; u < kUnroll; u++) {
+      const int64_t i = i0 + u * stride;
+      x[u] = i < hi ? px[i] : 0.0;
+      y[u] = i < hi ? py[i] : 0.0;
+      m[u] = pm ? (i < hi ? pm[i] : 0.0) : P.macrosize;
+    }
+  };
+  load(lo + threadIdx.x);
+  for (int c = threadIdx.x; c < nodes; c += blockDim.x) grid[c] = 0.0;
+  __syncthreads();
+  const double xMin = g.xMin, yMin = g.yMin, dx = g.dx, dy = g.dy;
+  for (int64_t i0 = lo + threadIdx.x; i0 < hi; i0 += kUnroll * stride) {
+    if (i0 != lo + (int64_t)threadIdx.x) load(i0);
+#pragma unroll
+    for (int u = 0; u < kUnroll; u++) {
+      if (i0 + u * stride >= hi) break;
+      if (x[u] < g.xMin || x[u] > g.xMax || y[u] < g.yMin || y[u] > g.yMax) continue;
+      int iX, iY;
+      double xF, yF;
+      node_ind_frac(x[u], xMin, dx, nx, iX, xF);
+      node_ind_frac(y[u], yMin, dy, ny, iY, yF);
+      const double xF2 = xF * xF, yF2 = yF * yF;
+      const double wx[3] = {0.5 * (0.25 - xF + xF2), 0.75 - xF2, 0.5 * (0.25 + xF + xF2)};
+      const double wy[3] = {0.5 * (0.25 - yF + yF2), 0.75 - yF2, 0.5 * (0.25 + yF + yF2)};
+      double* base = grid + (iX - 1) * ny + (iY - 1);  // 1 <= iX <= nx-2, 1 <= iY <= ny-2
+#pragma unroll
+      for (int a = 0; a < 3; a++)
+#pragma unroll
+        for (int c = 0; c < 3; c++) atomicAdd(base + a * ny + c, wx[a] * wy[c] * m[u]);
+    }
+  }
+  __syncthreads();
+  double mom[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
+  for (int c = threadIdx.x; c < nodes; c += blockDim.x) {
+    const double val = grid[c];
+    if (val == 0.0) continue;
+    global_add(rho + c, val);
+    moment_add(mom, val, c / ny, c % ny, xMin, dx);
+  }
+  if (moments) {
+    __shared__ double red[kPrivThreads / 64][5];
+    for (int k = 0; k < 5; k++)
+      for (int o = 32; o > 0; o >>= 1) mom[k] += __shfl_xor(mom[k], o, 64);
+    if ((threadIdx.x & 63) == 0)
+      for (int k = 0; k < 5; k++) red[threadIdx.x >> 6][k] = mom[k];
+    __syncthreads();
+    if (threadIdx.x < 5) {
+      double sum = 0.0;
+      for (int w = 0; w < (int)(blockDim.x >> 6); w++) sum += red[w][threadIdx.x];
+      tg.mpart[(size_t)blockIdx.x * 5 + threadIdx.x] = sum;
+    }
+  }
 }
 
 // ---- 4: post (one block) ---------------------------------------------------
@@ -1607,20 +1702,39 @@ void sc2d_track(ob_sc2p5drb* c, ob_bunch* b, double length, double pipe_radius,
     launch_cols<true>(c, c->M0, c->gate, 0.0);
     c->par ^= 1;
   }
-  {
-    ProfScope ps_("sc2d_scatter");
-    const int64_t per = std::max<int64_t>(2048, (n + 1023) / 1024);
-    hipLaunchKernelGGL(sc2d_scatter, dim3((unsigned)((n + per - 1) / per)),
-                       dim3(kDepositThreads), 2 * sizeof(uint32_t) * tg.ntiles, s, P, v,
-                       c->st, tg, per);
-  }
   // rb: the rho moments come from the deposit unless the denoiser changes rho after it
   const int moments = (c->denoiser || c->plain) ? 0 : 1;
-  {
-    ProfScope ps_("sc2d_tdeposit");
-    hipLaunchKernelGGL(sc2d_tdeposit, dim3((unsigned)(n / tg.item + tg.ntiles + 1)),
-                       dim3(kDepositThreads), sizeof(double) * (tg.T + 2) * (tg.T + 2), s, P,
-                       c->st, tg, v.msize != nullptr ? 1 : 0, c->rho, moments);
+  if ((int64_t)c->nx * c->ny <= kPrivNodes && n <= kPrivMaxParticles) {
+    // the whole grid fits in LDS: private-grid deposit in particle order, no tile sort
+    static bool attr_done = false;
+    if (!attr_done) {
+      OB_HIP(hipFuncSetAttribute((const void*)sc2d_pdeposit,
+                                 hipFuncAttributeMaxDynamicSharedMemorySize,
+                                 (int)(sizeof(double) * kPrivNodes)));
+      attr_done = true;
+    }
+    const int64_t blocks = std::max<int64_t>(
+        1, std::min<int64_t>({(n + kPrivPerBlock - 1) / kPrivPerBlock,
+                              (int64_t)device_cu_count(), (int64_t)tg.max_items}));
+    const int64_t per = (n + blocks - 1) / blocks;
+    ProfScope ps_("sc2d_pdeposit");
+    hipLaunchKernelGGL(sc2d_pdeposit, dim3((unsigned)blocks), dim3(kPrivThreads),
+                       sizeof(double) * c->nx * c->ny, s, P, v, c->st, tg, per, c->rho,
+                       moments);
+  } else {
+    {
+      ProfScope ps_("sc2d_scatter");
+      const int64_t per = std::max<int64_t>(2048, (n + 1023) / 1024);
+      hipLaunchKernelGGL(sc2d_scatter, dim3((unsigned)((n + per - 1) / per)),
+                         dim3(kDepositThreads), 2 * sizeof(uint32_t) * tg.ntiles, s, P, v,
+                         c->st, tg, per);
+    }
+    {
+      ProfScope ps_("sc2d_tdeposit");
+      hipLaunchKernelGGL(sc2d_tdeposit, dim3((unsigned)(n / tg.item + tg.ntiles + 1)),
+                         dim3(kDepositThreads), sizeof(double) * (tg.T + 2) * (tg.T + 2), s,
+                         P, c->st, tg, v.msize != nullptr ? 1 : 0, c->rho, moments);
+    }
   }
   if (c->denoiser) dn_denoise_rho(c->denoiser, c->rho, c->nx, c->ny, &c->st->skip);
   if (c->plain || bnd) {  // their solve reads the post pass's results: it runs first
