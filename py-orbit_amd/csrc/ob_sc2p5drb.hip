@@ -293,8 +293,9 @@ __device__ __forceinline__ void hist_load(const DeviceBunchView& v, double macro
 // PIPE: each pass's loads are issued before the previous pass is binned (the single-bunch
 // kernel, whose workgroups make one or two passes; the batch's, with many, keep their
 // registers for occupancy instead); pre: the first pass's particles, already loaded (by
-// hist_load at `first`), or null.
-template <bool PIPE>
+// hist_load at `first`), or null.  COUNT: count particles per tile (not needed by the
+// private-grid deposit).
+template <bool PIPE, bool COUNT = true>
 __device__ __forceinline__ void sc2d_hist_body(const SC2DParams& P, const DeviceBunchView& v,
                                                const Grid2Step& g, const Tile2D& tg,
                                                double* __restrict__ zg, double* __restrict__ zrow,
@@ -328,8 +329,10 @@ __device__ __forceinline__ void sc2d_hist_body(const SC2DParams& P, const Device
 #pragma unroll
       for (int u = 0; u < kUnroll; u++) {
         if (i0 + u * stride >= end) break;
-        const int k = tile2d_key(tg, P.nx, P.ny, g, c.x[u], c.y[u]);
-        if (k >= 0) atomicAdd(&lh[k], 1u);
+        if (COUNT) {
+          const int k = tile2d_key(tg, P.nx, P.ny, g, c.x[u], c.y[u]);
+          if (k >= 0) atomicAdd(&lh[k], 1u);
+        }
         // Grid1D::binValue (Grid1D.cc:422-440)
         if (!(c.z[u] < g.zMin || c.z[u] > g.zMax)) {
           int i0z, ip;
@@ -580,6 +583,7 @@ __device__ __forceinline__ void sc2d_tdeposit_body(const SC2DParams& P, const Gr
 // Single bunch: the grid from the extrema partials in every block (block 0 writes the
 // state and clears the scatter cursors); blocks clear rho in turn (it is next written by
 // the deposit, two launches later) and write their z line density rows.
+template <bool COUNT>
 __global__ __launch_bounds__(kDepositThreads) void sc2d_bin(SC2DParams P, DeviceBunchView v,
                                                             const double* __restrict__ partials,
                                                             SC2DState* st, SC2DState* gate,
@@ -604,7 +608,7 @@ __global__ __launch_bounds__(kDepositThreads) void sc2d_bin(SC2DParams P, Device
   const int ng = P.nx * P.ny;
   for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < ng; i += gridDim.x * blockDim.x)
     rho[i] = 0.0;
-  sc2d_hist_body<true>(P, v, g, tg, nullptr, tg.zpart + (size_t)blockIdx.x * P.nz, first, v.n, stride,
+  sc2d_hist_body<true, COUNT>(P, v, g, tg, nullptr, tg.zpart + (size_t)blockIdx.x * P.nz, first, v.n, stride,
                  dyn, (uint32_t*)(dyn + P.nz), nullptr, &pre);
 }
 
@@ -1683,11 +1687,18 @@ void sc2d_track(ob_sc2p5drb* c, ob_bunch* b, double length, double pipe_radius,
       1, std::min<int64_t>({(n + 2047) / 2048, (int64_t)4 * device_cu_count(),
                             (int64_t)std::max(1, (1 << 18) / c->nz)}));
   const Tile2D tg = sc2d_tile_reserve(c, n, v.msize != nullptr, c->tile_item, hb);
+  // the whole grid fits in LDS: private-grid deposit in particle order, no tile sort
+  const bool private_grid = (int64_t)c->nx * c->ny <= kPrivNodes && n <= kPrivMaxParticles;
   {
     ProfScope ps_("sc2d_bin");
-    hipLaunchKernelGGL(sc2d_bin, dim3(hb), dim3(kDepositThreads),
-                       sizeof(double) * c->nz + sizeof(uint32_t) * tg.ntiles, s, P, v,
-                       c->partials, c->st, c->gate, tg, c->rho, c->status.dev);
+    if (private_grid)
+      hipLaunchKernelGGL(sc2d_bin<false>, dim3(hb), dim3(kDepositThreads),
+                         sizeof(double) * c->nz + sizeof(uint32_t) * tg.ntiles, s, P, v,
+                         c->partials, c->st, c->gate, tg, c->rho, c->status.dev);
+    else
+      hipLaunchKernelGGL(sc2d_bin<true>, dim3(hb), dim3(kDepositThreads),
+                         sizeof(double) * c->nz + sizeof(uint32_t) * tg.ntiles, s, P, v,
+                         c->partials, c->st, c->gate, tg, c->rho, c->status.dev);
   }
   if (c->plain) {
     // PoissonSolverFFT2D::setGridXY (PoissonSolverFFT2D.cc:103-111) when the grid pass
@@ -1704,8 +1715,7 @@ void sc2d_track(ob_sc2p5drb* c, ob_bunch* b, double length, double pipe_radius,
   }
   // rb: the rho moments come from the deposit unless the denoiser changes rho after it
   const int moments = (c->denoiser || c->plain) ? 0 : 1;
-  if ((int64_t)c->nx * c->ny <= kPrivNodes && n <= kPrivMaxParticles) {
-    // the whole grid fits in LDS: private-grid deposit in particle order, no tile sort
+  if (private_grid) {
     static bool attr_done = false;
     if (!attr_done) {
       OB_HIP(hipFuncSetAttribute((const void*)sc2d_pdeposit,
