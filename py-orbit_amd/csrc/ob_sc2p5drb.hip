@@ -640,7 +640,7 @@ __global__ __launch_bounds__(kDepositThreads) void sc2d_tdeposit(SC2DParams P,
 // deposits them (Grid2D::binValue, Grid2D.cc:142-166, the same weights as the tiled
 // deposit) into its own copy of the whole grid in LDS, then adds the non-zero nodes to
 // rho with global atomics and its share of the rho moments to mpart row b.  Block 0 sets
-// the row count the post pass reads and clears the bin pass's tile counts (unused here).
+// the row count the post pass reads and keeps the tile counts zero for the tiled path.
 __global__ __launch_bounds__(kPrivThreads) void sc2d_pdeposit(SC2DParams P, DeviceBunchView v,
                                                                  const SC2DState* st, Tile2D tg,
                                                                  int64_t per,
