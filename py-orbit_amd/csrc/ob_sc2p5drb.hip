@@ -698,11 +698,21 @@ __global__ __launch_bounds__(kPrivThreads) void sc2d_pdeposit(SC2DParams P, Devi
   }
   __syncthreads();
   double mom[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
+  // node c = gx*ny + gy, stepped by blockDim.x without a division per node
+  const int step_x = (int)blockDim.x / ny, step_y = (int)blockDim.x - step_x * ny;
+  int gx = (int)threadIdx.x / ny, gy = (int)threadIdx.x - gx * ny;
   for (int c = threadIdx.x; c < nodes; c += blockDim.x) {
     const double val = grid[c];
-    if (val == 0.0) continue;
-    global_add(rho + c, val);
-    moment_add(mom, val, c / ny, c % ny, xMin, dx);
+    if (val != 0.0) {
+      global_add(rho + c, val);
+      moment_add(mom, val, gx, gy, xMin, dx);
+    }
+    gx += step_x;
+    gy += step_y;
+    if (gy >= ny) {
+      gy -= ny;
+      gx++;
+    }
   }
   if (moments) {
     __shared__ double red[kPrivThreads / 64][5];
