@@ -43,6 +43,31 @@ __device__ __forceinline__ uint32_t tile_key(const TileGeom& g, double x, double
   return (uint32_t)(((iX / g.T) * g.ty + (iY / g.T)) * g.tz + iZ / g.Tz);
 }
 
+// Cell c = (lx*W + ly)*Wz + lz of a tile block, advanced by kThreads cells per step with
+// carries instead of a division per cell (the divisions are done once per thread).
+struct CellWalk {
+  int lx, ly, lz, sx, sy, sz, W, Wz;
+  __device__ __forceinline__ CellWalk(int c, int W_, int Wz_) : W(W_), Wz(Wz_) {
+    lz = c % Wz;
+    const int r = c / Wz;
+    ly = r % W;
+    lx = r / W;
+    sz = kThreads % Wz;
+    const int sr = kThreads / Wz;
+    sy = sr % W;
+    sx = sr / W;
+  }
+  __device__ __forceinline__ void next() {
+    lz += sz;
+    int carry = lz >= Wz ? 1 : 0;
+    lz -= carry * Wz;
+    ly += sy + carry;  // < 2W
+    carry = ly >= W ? 1 : 0;
+    ly -= carry * W;
+    lx += sx + carry;
+  }
+};
+
 // Tile t's LDS block: nodes x0..x0+W-1, y0..y0+W-1, z0..z0+Wz-1, z fastest.
 struct TileBox {
   int x0, y0, z0, W, Wz;
@@ -373,11 +398,11 @@ __global__ __launch_bounds__(kThreads) void tile_deposit(const double* __restric
     }
     __syncthreads();  // every wave is done with the counters and the staging of this half
   }
-  for (int c = threadIdx.x; c < cells; c += blockDim.x) {
+  CellWalk w((int)threadIdx.x, box.W, box.Wz);
+  for (int c = threadIdx.x; c < cells; c += kThreads, w.next()) {
     const double v = acc[c];
     if (v == 0.0) continue;
-    const int lz = c % box.Wz, r = c / box.Wz, ly = r % box.W, lx = r / box.W;
-    const int gx = box.x0 + lx, gy = box.y0 + ly, gz = box.z0 + lz;
+    const int gx = box.x0 + w.lx, gy = box.y0 + w.ly, gz = box.z0 + w.lz;
     if (gx < 0 || gx >= g.nx || gy < 0 || gy >= g.ny || gz < 0 || gz >= nz) continue;
     atomicAdd(rho + gx * gsx + (int64_t)gy * nz + gz, v);
   }
@@ -489,11 +514,11 @@ __global__ __launch_bounds__(kThreads, 3) void tile_kick(double* __restrict__ cu
   constexpr int KB = 8;  // cells <= KB * kThreads for blocks of up to 16 KiB
   double bv[KB];
   uint32_t inmask = 0;
+  CellWalk w((int)threadIdx.x, box.W, box.Wz);
 #pragma unroll
-  for (int u = 0; u < KB; u++) {
+  for (int u = 0; u < KB; u++, w.next()) {
     const int c = (int)threadIdx.x + u * kThreads;
-    const int lz = c % box.Wz, r = c / box.Wz, ly = r % box.W, lx = r / box.W;
-    const int gx = box.x0 + lx, gy = box.y0 + ly, gz = box.z0 + lz;
+    const int gx = box.x0 + w.lx, gy = box.y0 + w.ly, gz = box.z0 + w.lz;
     const bool in = c < cells && gx >= 0 && gx < g.nx && gy >= 0 && gy < g.ny && gz >= 0 &&
                     gz < g.nz;
     bv[u] = k.phi[in ? gx * gsx + (int64_t)gy * g.nz + gz : 0];
@@ -504,9 +529,9 @@ __global__ __launch_bounds__(kThreads, 3) void tile_kick(double* __restrict__ cu
     const int c = (int)threadIdx.x + u * kThreads;
     if (c < cells) sphi[c] = ((inmask >> u) & 1u) ? bv[u] : 0.0;
   }
-  for (int c = (int)threadIdx.x + KB * kThreads; c < cells; c += kThreads) {  // larger blocks
-    const int lz = c % box.Wz, r = c / box.Wz, ly = r % box.W, lx = r / box.W;
-    const int gx = box.x0 + lx, gy = box.y0 + ly, gz = box.z0 + lz;
+  for (int c = (int)threadIdx.x + KB * kThreads; c < cells; c += kThreads, w.next()) {
+    // larger blocks: the walk continues from cell threadIdx.x + KB * kThreads
+    const int gx = box.x0 + w.lx, gy = box.y0 + w.ly, gz = box.z0 + w.lz;
     const bool in = gx >= 0 && gx < g.nx && gy >= 0 && gy < g.ny && gz >= 0 && gz < g.nz;
     sphi[c] = in ? k.phi[gx * gsx + (int64_t)gy * g.nz + gz] : 0.0;
   }
