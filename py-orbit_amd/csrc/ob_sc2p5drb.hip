@@ -1693,12 +1693,15 @@ void sc2d_track(ob_sc2p5drb* c, ob_bunch* b, double length, double pipe_radius,
   for (int k = 0; k < 6; k++) P.bnd[k] = 0.0;
   P.bnd_type = bnd ? boundary_describe(bnd, P.bnd) : 0;
   // bin (grid + tile counts + z line density), scatter (scan + counting sort), deposit
-  const int hb = (int)std::max<int64_t>(
-      1, std::min<int64_t>({(n + 2047) / 2048, (int64_t)4 * device_cu_count(),
-                            (int64_t)std::max(1, (1 << 18) / c->nz)}));
-  const Tile2D tg = sc2d_tile_reserve(c, n, v.msize != nullptr, c->tile_item, hb);
   // the whole grid fits in LDS: private-grid deposit in particle order, no tile sort
   const bool private_grid = (int64_t)c->nx * c->ny <= kPrivNodes && n <= kPrivMaxParticles;
+  // bin workgroups of 2048 particles; 1024 with the private-grid deposit, whose bin pass
+  // counts no tiles (100k on 128x128: 0.8% faster per step than 2048, 512 4% slower)
+  const int64_t bin_per = private_grid ? 1024 : 2048;
+  const int hb = (int)std::max<int64_t>(
+      1, std::min<int64_t>({(n + bin_per - 1) / bin_per, (int64_t)4 * device_cu_count(),
+                            (int64_t)std::max(1, (1 << 18) / c->nz)}));
+  const Tile2D tg = sc2d_tile_reserve(c, n, v.msize != nullptr, c->tile_item, hb);
   {
     ProfScope ps_("sc2d_bin");
     if (private_grid)
