@@ -277,7 +277,10 @@ int ob_boundary2d_add_potential(ob_boundary2d* b, double* phi, int nx, int ny,
 /* ---- SpaceChargeCalc3D (src/spacecharge/SpaceChargeCalc3D.cc) ---- */
 int ob_sc3d_create(int nx, int ny, int nz, ob_sc3d** out);             /* :32-58 */
 int ob_sc3d_destroy(ob_sc3d* c);
-int ob_sc3d_track(ob_sc3d* c, ob_bunch* b, double length);             /* :100-148 */
+/* trackBunch(bunch, length) (:100-148).  Blocks until the bunch's queued maps and its
+ * extrema are done: the grid sizing and the Green-function rebuild decision
+ * (bunchAnalysis, :150-249) run on the host; the rest is queued asynchronously. */
+int ob_sc3d_track(ob_sc3d* c, ob_bunch* b, double length);
 /* which = 0 rho, 1 phi ([nz][nx][ny]); lims = xMin,xMax,yMin,yMax,zMin,zMax. */
 int ob_sc3d_get_grid(ob_sc3d* c, int which, double* out, double* lims);
 int ob_sc3d_set_ratio_limit(ob_sc3d* c, double r);                     /* :334-337 */
