@@ -231,6 +231,22 @@ __global__ __launch_bounds__(kThreads) void deposit3d(DeviceBunchView v, Grid3 g
   }
 }
 
+// Line q = i*NY + j of a z pass (0 <= q < 2^31, checked on the host): (i, j) of q with one
+// 32-bit division, and of q + 1 from it.
+__device__ __forceinline__ void line_ij(int64_t q, int NY, int& i, int& j) {
+  const uint32_t u = (uint32_t)q;
+  i = (int)(u / (uint32_t)NY);
+  j = (int)(u - (uint32_t)i * (uint32_t)NY);
+}
+__device__ __forceinline__ void line_next(int NY, int i, int j, int& i1, int& j1) {
+  j1 = j + 1;
+  i1 = i;
+  if (j1 == NY) {
+    j1 = 0;
+    i1++;
+  }
+}
+
 // ---- z pass r2c: two real z-lines per complex FFT -------------------------------
 // Real line q = i*NY + j (q < nlines) holds nz_in values, zero-padded to N = M2.  The
 // half spectrum kz <= N/2 of line q goes to spec[(i*M1 + j)*H2p + kz].
@@ -279,11 +295,13 @@ __global__ __launch_bounds__(kThreads) void fft3_z_r2c(const double* __restrict_
     const int nk = (N - k) & (N - 1);
     const double zr = re[lp * ls + k], zi = im[lp * ls + k];
     const double nr = re[lp * ls + nk], ni = im[lp * ls + nk];
-    const int64_t i0 = q0 / NY, j0 = q0 - i0 * NY;
-    spec[(i0 * M1 + j0) * H2p + k] = make_double2((zr + nr) * 0.5, (zi - ni) * 0.5);
+    int i0, j0;
+    line_ij(q0, NY, i0, j0);
+    spec[((int64_t)i0 * M1 + j0) * H2p + k] = make_double2((zr + nr) * 0.5, (zi - ni) * 0.5);
     if (q1 < nlines) {
-      const int64_t i1 = q1 / NY, j1 = q1 - i1 * NY;
-      spec[(i1 * M1 + j1) * H2p + k] = make_double2((zi + ni) * 0.5, -(zr - nr) * 0.5);
+      int i1, j1;
+      line_next(NY, i0, j0, i1, j1);
+      spec[((int64_t)i1 * M1 + j1) * H2p + k] = make_double2((zi + ni) * 0.5, -(zr - nr) * 0.5);
     }
   }
 }
@@ -310,9 +328,11 @@ __global__ __launch_bounds__(kThreads) void fft3_z_c2r(const double2* __restrict
     const int lp = idx / H2, k = idx - lp * H2;
     const int64_t q0 = 2 * (pair0 + lp), q1 = q0 + 1;
     const bool ok0 = idx < P * H2 && q0 < nlines, ok1 = idx < P * H2 && q1 < nlines;
-    const int64_t i0 = q0 / NY, i1 = q1 / NY;
-    av[u] = spec[ok0 ? (i0 * M1 + (q0 - i0 * NY)) * H2p + k : 0];
-    bv[u] = spec[ok1 ? (i1 * M1 + (q1 - i1 * NY)) * H2p + k : 0];
+    int i0, j0, i1, j1;
+    line_ij(q0, NY, i0, j0);
+    line_next(NY, i0, j0, i1, j1);
+    av[u] = spec[ok0 ? ((int64_t)i0 * M1 + j0) * H2p + k : 0];
+    bv[u] = spec[ok1 ? ((int64_t)i1 * M1 + j1) * H2p + k : 0];
     ma |= (ok0 ? 1u : 0u) << u;
     mb |= (ok1 ? 1u : 0u) << u;
   }
@@ -559,8 +579,11 @@ __global__ __launch_bounds__(kThreads) void green3d_z(int qx, int qy, int H2p,
   for (int idx = threadIdx.x; idx < P * N; idx += kThreads) {
     const int lp = idx / N, k = idx - lp * N;
     const int64_t q0 = 2 * (pair0 + lp), q1 = q0 + 1;
-    re[lp * ls + k] = q0 < nlines ? green3d((int)(q0 / qy), (int)(q0 % qy), k, gg) : 0.0;
-    im[lp * ls + k] = q1 < nlines ? green3d((int)(q1 / qy), (int)(q1 % qy), k, gg) : 0.0;
+    int i0, j0, i1, j1;
+    line_ij(q0, qy, i0, j0);
+    line_next(qy, i0, j0, i1, j1);
+    re[lp * ls + k] = q0 < nlines ? green3d(i0, j0, k, gg) : 0.0;
+    im[lp * ls + k] = q1 < nlines ? green3d(i1, j1, k, gg) : 0.0;
   }
   __syncthreads();
   lds_fft<N, P, kThreads, -1>(re, im, P, ls, tw, nmax);
@@ -782,6 +805,7 @@ void launch_pending_green(ob_sc3d* c) {
 }
 
 void solve(ob_sc3d* c, double denom) {
+  OB_REQUIRE((int64_t)c->nx * c->ny < ((int64_t)1 << 31), "SpaceChargeCalc3D: grid too large");
   launch_pending_green(c);
   hipStream_t s = stream();
   const int64_t nl = (int64_t)c->nx * c->ny;
